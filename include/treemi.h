@@ -1,0 +1,148 @@
+/*
+ * treemi.h -- C-ABI of the MI355X Barnes-Hut engine (libtreemi.so).
+ *
+ * The engine holds one whole plasma domain (electrons + singly charged ions)
+ * resident on one GPU behind an opaque handle, and replaces the reference's
+ * single-domain force / time-step path:
+ *
+ *   tb_build          <- BuildLocalTree     tree.f90:39-100 (+ Insert_Tree_e/_i
+ *                                            tree.f90:103-447, PSeudoE/I 450-659)
+ *   tb_force          <- LocalCoulombSum    tree.f90:662-717 (+ Search_Tree_* 721-1133)
+ *   tb_vv1            <- VVerletNotemp1     vverlet.f90:9-44
+ *   tb_vv2            <- VVerletNotemp2     vverlet.f90:49-112
+ *   tb_set_particles  <- PTEgathering2 / PTIgathering2, main.f90:430-502
+ *                        (positions -> device; the MPI gather collapses)
+ *   tb_get_particles  <- host view of qel/qion (TYPE PT) for TEMPORAL / Restart
+ *   tb_step           <- the time loop body main.f90:126-133 (VV1, Force, VV2)
+ *
+ * Units are the reference's (data.f90:29-40): um, eV, amu, t0 = 0.1018 ns.
+ * Every call returns TB_OK (0) or a TB_ERR_* code; tb_last_error() gives the
+ * message.  The reference STOPs on errors; its Fortran binding does the same
+ * on a nonzero status.
+ *
+ * Threading: one host thread per handle; calls on one handle are not
+ * re-entrant.  All device work of a handle runs on the handle's own HIP
+ * stream; calls that return host data synchronise that stream.
+ */
+#ifndef TREEMI_H
+#define TREEMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TB_OK 0
+#define TB_ERR_ARG 1        /* bad argument (null pointer, n < 1, theta <= 0, ...) */
+#define TB_ERR_HIP 2        /* HIP runtime error */
+#define TB_ERR_DEPTH 3      /* two particles share all 21 key levels (coincident) */
+#define TB_ERR_NOMEM 4      /* device allocation failed or over the handle's cap */
+#define TB_ERR_STATE 5      /* call order (e.g. tb_force before tb_build) */
+#define TB_ERR_NONFINITE 6  /* NaN / Inf position */
+#define TB_ERR_NODEV 7      /* no usable GPU */
+
+/* TYPE PT, data.f90:47-50: xx(3), xv(3), ff(3) REAL*8, id INTEGER (+pad) */
+typedef struct tb_pt {
+    double xx[3];
+    double xv[3];
+    double ff[3];
+    int32_t id;
+    int32_t pad_;
+} tb_pt;
+
+/* walk variants */
+#define TB_WALK_REF64 0   /* per-target FP64 walk, reference MAC / moments / order */
+#define TB_WALK_MIXED 1   /* wavefront-group walk, FP32 P2P/M2P, compensated sums */
+/* opening criterion for TB_WALK_MIXED */
+#define TB_MAC_FIXED 0    /* fixed theta */
+#define TB_MAC_LEARNED 1  /* per-interaction theta from the trained MLP (tb_mlp_*) */
+
+/* TYPE PM, data.f90:90-93, plus engine options */
+typedef struct tb_config {
+    double xm[2];      /* masses, amu: [0] electron, [1] ion          (PM%xm)      */
+    double rs;         /* Kelbg length, um                           (PM%rs)      */
+    double theta;      /* opening angle of TB_WALK_REF64             (PM%sd_ratio) */
+    double theta_mixed;/* opening angle of TB_WALK_MIXED (0: = theta)              */
+    double theta_lo;   /* TB_MAC_LEARNED: smallest theta the MLP may pick          */
+    double err_target; /* TB_MAC_LEARNED: accepted predicted per-interaction error */
+    int32_t walk;      /* TB_WALK_*                                               */
+    int32_t mac;       /* TB_MAC_*                                                */
+    int32_t group;     /* TB_WALK_MIXED targets per group: 0 = autotuned, else 32..256 */
+    int32_t device;    /* HIP device ordinal                                       */
+    int64_t mem_cap;   /* device bytes the handle may hold (0 = unlimited)         */
+} tb_config;
+
+typedef struct tb_handle tb_handle;
+
+int tb_create(tb_handle **h, const tb_config *cfg);
+int tb_destroy(tb_handle *h);
+int tb_set_config(tb_handle *h, const tb_config *cfg);
+
+/* positions, velocities (forces ignored) in the reference's index order */
+int tb_set_particles(tb_handle *h, int32_t ne, const tb_pt *e, int32_t ni, const tb_pt *i);
+/* x, v, f back in the order given to tb_set_particles */
+int tb_get_particles(tb_handle *h, tb_pt *e, tb_pt *i);
+
+int tb_build(tb_handle *h);
+int tb_force(tb_handle *h, double *epot_e, double *epot_i);
+int tb_vv1(tb_handle *h, double dt);
+/* sums (if sample != 0): [0] mv2el [1] mv2ion [2] Te_rad [3] Ti_rad (vverlet.f90:81-106) */
+int tb_vv2(tb_handle *h, double dt, int32_t sample, double *sums);
+/*
+ * nsteps x (VV1, build, force, VV2); the kick of VV2 and the next VV1 run as
+ * one fused kernel.  out (optional, 6 doubles) after the last step:
+ * Epot_e, Epot_i, mv2el, mv2ion, Te_rad, Ti_rad.
+ */
+int tb_step(tb_handle *h, double dt, int32_t nsteps, double *out);
+
+/*
+ * Node table of one species' tree (0 electron, 1 ion) in preorder with
+ * children in the reference's order 1..8; next[] = preorder index after the
+ * subtree; pt[] = particle index (reference order) of a leaf, -1 for a twig.
+ * Vectors are [cap][3] for xx, lx, D, Q, QQ.  *nnodes receives the count
+ * (TB_ERR_ARG if cap is too small).
+ */
+int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, int32_t *depth, int32_t *leaf,
+                   int32_t *pt, int32_t *count, int32_t *next, double *xx, double *lx, double *qeff, double *dx,
+                   double *D, double *Q, double *QQ);
+
+/*
+ * FP64 O(N^2) direct sum on sampled targets with the reference's pair rules
+ * (like species Coulomb with r^2 > 0.01 um^2, unlike species simplified
+ * Kelbg).  tspecies[t] 0/1, tidx[t] index in reference order; f out [nt][3].
+ */
+int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
+
+typedef struct tb_stats {
+    int64_t nodes[2];        /* node count per species tree                     */
+    int32_t max_depth[2];
+    int32_t pad_;
+    int64_t visits[4];       /* MAC tests   ee, ie, ei, ii (last counted force) */
+    int64_t m2p[4];          /* accepted cells                                  */
+    int64_t p2p[4];          /* leaf interactions                               */
+    double ms_build, ms_force, ms_vv;  /* last call, device time               */
+} tb_stats;
+/* counting pass: re-runs the last force with interaction counters */
+int tb_count_interactions(tb_handle *h, tb_stats *s);
+int tb_get_stats(tb_handle *h, tb_stats *s);
+
+const char *tb_last_error(void);
+
+/* ------------------------------------------------------------ host helpers */
+
+/* Kelbg length: thermal de Broglie lambda hbar/sqrt(2 mu k_B T_e), um */
+double tb_kelbg_length(double te_kelvin, double xm_e, double xm_i);
+
+/*
+ * Seeded synthetic plasma: spherical Gaussian positions (sigma um) per species
+ * from Philox4x32-10 + Box-Muller, Maxwell-Boltzmann velocities rescaled to
+ * exactly T (Vinit, main.f90:709-767: 3N-6 degrees of freedom).  ids 1..n.
+ */
+int tb_gen_plasma(uint64_t seed, int32_t ne, int32_t ni, double sigma, double te, double ti, double xm_e,
+                  double xm_i, tb_pt *e, tb_pt *i);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

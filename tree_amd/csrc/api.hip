@@ -1,0 +1,681 @@
+// api.hip -- the C-ABI of libtreemi.so (declared in include/treemi.h):
+// handle lifetime, device memory, and the order of launches that stands in
+// for the reference's Force (main.f90:506-541), BuildLocalTree
+// (tree.f90:39-100), LocalCoulombSum (tree.f90:662-717) and
+// VVerletNotemp1/2 (vverlet.f90:9-112).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tb_internal.h"
+#include "tb_mixed.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(TB_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct tb_handle {
+    tb_config cfg{};
+    hipStream_t s = nullptr;
+    SpeciesDev sp[2];
+    MixedDev mx;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    void *scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    double *bbox_tmp = nullptr;
+    int32_t *d_misc = nullptr;     // [0..1] err per species, [2..3] totals, [4..] hist 2 x 2 x (L+2)
+    int32_t *h_misc = nullptr;     // pinned mirror
+    double *epart = nullptr;       // per-block partial sums
+    size_t epart_n = 0;
+    double *d_sums = nullptr;      // 16 doubles
+    double *h_sums = nullptr;      // pinned
+    int64_t *d_cnt = nullptr;      // 12 counters
+    size_t bytes = 0;              // device bytes held
+    bool built = false, have_force = false;
+    double epot[2] = {0, 0};
+    tb_stats st{};
+    hipEvent_t ev[6] = {};
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+constexpr int MISC_HIST = 4;
+constexpr int HW = TB_LEVELS + 2;  // histogram width
+
+int dalloc(tb_handle *h, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (h->cfg.mem_cap > 0 && (int64_t)(h->bytes + bytes) > h->cfg.mem_cap)
+        return fail(TB_ERR_NOMEM, "device allocation of %zu bytes exceeds the handle cap %lld", bytes,
+                    (long long)h->cfg.mem_cap);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(TB_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    h->bytes += bytes;
+    return TB_OK;
+}
+
+void dfree(tb_handle *h, void *p, size_t bytes)
+{
+    if (!p) return;
+    hipFree(p);
+    h->bytes -= std::min(h->bytes, bytes == 0 ? (size_t)16 : bytes);
+}
+
+template <typename T>
+int dalloc_t(tb_handle *h, T **p, size_t count)
+{
+    return dalloc(h, (void **)p, count * sizeof(T));
+}
+
+void free_species(tb_handle *h, SpeciesDev &S)
+{
+    size_t n = (size_t)S.n;
+    for (int b = 0; b < 2; b++) {
+        for (int k = 0; k < 3; k++) {
+            dfree(h, S.x[b][k], n * 8); dfree(h, S.v[b][k], n * 8); dfree(h, S.f[b][k], n * 8);
+        }
+        dfree(h, S.orig[b], n * 4);
+        dfree(h, S.key[b], n * 8);
+        dfree(h, S.val[b], n * 4);
+    }
+    dfree(h, S.cnt, n * 4);
+    dfree(h, S.offs, n * 4);
+    dfree(h, S.root, 64 * 8);
+    size_t c = (size_t)S.cap;
+    dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
+    dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
+    for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
+    S = SpeciesDev();
+}
+
+int alloc_species(tb_handle *h, SpeciesDev &S, int n)
+{
+    S.n = n;
+    int rc;
+    for (int b = 0; b < 2; b++) {
+        for (int k = 0; k < 3; k++) {
+            if ((rc = dalloc_t(h, &S.x[b][k], n))) return rc;
+            if ((rc = dalloc_t(h, &S.v[b][k], n))) return rc;
+            if ((rc = dalloc_t(h, &S.f[b][k], n))) return rc;
+        }
+        if ((rc = dalloc_t(h, &S.orig[b], n))) return rc;
+        if ((rc = dalloc_t(h, &S.key[b], n))) return rc;
+        if ((rc = dalloc_t(h, &S.val[b], n))) return rc;
+    }
+    if ((rc = dalloc_t(h, &S.cnt, n))) return rc;
+    if ((rc = dalloc_t(h, &S.offs, n))) return rc;
+    if ((rc = dalloc_t(h, &S.root, 64))) return rc;
+    return TB_OK;
+}
+
+int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
+{
+    if (nn <= S.cap) return TB_OK;
+    size_t c = (size_t)S.cap;
+    dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
+    dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
+    for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
+    int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
+    S.cap = 0;
+    int rc;
+    if ((rc = dalloc_t(h, &S.nmac, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.nnext, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.nmom, (size_t)cap * 10))) return rc;
+    if ((rc = dalloc_t(h, &S.nm2, (size_t)cap * 6))) return rc;
+    if ((rc = dalloc_t(h, &S.ndx, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.nstart, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.ncount, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.ndepth, cap))) return rc;
+    for (int b = 0; b < 2; b++) {
+        if ((rc = dalloc_t(h, &S.dkey[b], cap))) return rc;
+        if ((rc = dalloc_t(h, &S.dval[b], cap))) return rc;
+    }
+    S.cap = cap;
+    return TB_OK;
+}
+
+int ensure_tmp(tb_handle *h, int n)
+{
+    int rc;
+    size_t sb = tb_sort_temp_bytes(n);
+    if (sb > h->sort_tmp_bytes) {
+        dfree(h, h->sort_tmp, h->sort_tmp_bytes);
+        h->sort_tmp = nullptr; h->sort_tmp_bytes = 0;
+        if ((rc = dalloc(h, &h->sort_tmp, sb))) return rc;
+        h->sort_tmp_bytes = sb;
+    }
+    size_t cb = tb_scan_temp_bytes(n);
+    if (cb > h->scan_tmp_bytes) {
+        dfree(h, h->scan_tmp, h->scan_tmp_bytes);
+        h->scan_tmp = nullptr; h->scan_tmp_bytes = 0;
+        if ((rc = dalloc(h, &h->scan_tmp, cb))) return rc;
+        h->scan_tmp_bytes = cb;
+    }
+    size_t pe = (size_t)std::max(tb_walk64_blocks(n), tb_vv_blocks(n)) * 4 + 4096;
+    if (pe > h->epart_n) {
+        dfree(h, h->epart, h->epart_n * 8);
+        h->epart = nullptr; h->epart_n = 0;
+        if ((rc = dalloc_t(h, &h->epart, pe))) return rc;
+        h->epart_n = pe;
+    }
+    return TB_OK;
+}
+
+int check_cfg(const tb_config *c)
+{
+    if (!c) return fail(TB_ERR_ARG, "null config");
+    if (!(c->theta > 0.0)) return fail(TB_ERR_ARG, "theta must be > 0 (got %g)", c->theta);
+    if (c->theta_mixed < 0.0) return fail(TB_ERR_ARG, "theta_mixed must be >= 0");
+    if (!(c->rs > 0.0)) return fail(TB_ERR_ARG, "rs must be > 0 (got %g)", c->rs);
+    if (!(c->xm[0] > 0.0) || !(c->xm[1] > 0.0)) return fail(TB_ERR_ARG, "masses must be > 0");
+    if (c->walk != TB_WALK_REF64 && c->walk != TB_WALK_MIXED) return fail(TB_ERR_ARG, "unknown walk %d", c->walk);
+    if (c->mac != TB_MAC_FIXED && c->mac != TB_MAC_LEARNED) return fail(TB_ERR_ARG, "unknown mac %d", c->mac);
+    if (c->group != 0 && (c->group < 32 || c->group > 256))
+        return fail(TB_ERR_ARG, "group must be 0 or in [32, 256] (got %d)", c->group);
+    return TB_OK;
+}
+
+// BuildLocalTree for both species: two phases around one small host read-back
+int build_all(tb_handle *h)
+{
+    hipStream_t s = h->s;
+    int rc;
+    if ((rc = ensure_tmp(h, std::max(h->sp[0].n, h->sp[1].n)))) return rc;
+    HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * (MISC_HIST + 2 * 2 * HW), s));
+    int which[2];
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        const int c = S.cur;
+        tb_bbox(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, h->bbox_tmp);
+        tb_keys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
+        tb_radix_sort(s, S.n, S.key, S.val, 0, 64, h->sort_tmp, &which[sp], false);
+        tb_permute(s, S, S.val[which[sp]], h->d_misc + sp);
+        tb_lcp(s, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
+        tb_scan_excl(s, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, h->scan_tmp);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, sizeof(int32_t) * (MISC_HIST + 2 * 2 * HW), hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int sp = 0; sp < 2; sp++) {
+        int e = h->h_misc[sp];
+        if (e & 4) return fail(TB_ERR_HIP, "species %d: internal sort check failed", sp);
+        if (e & 2) return fail(TB_ERR_NONFINITE, "non-finite position in species %d", sp);
+        if (e & 1)
+            return fail(TB_ERR_DEPTH, "species %d: two particles share all %d key levels (coincident)", sp,
+                        TB_LEVELS);
+    }
+    // all (re)allocation happens here, while the stream is idle
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        S.nn = 1 + h->h_misc[2 + sp];
+        if ((rc = ensure_nodes(h, S, S.nn))) return rc;
+    }
+    if ((rc = ensure_tmp(h, std::max(h->sp[0].nn, h->sp[1].nn)))) return rc;
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        const int32_t *hist = h->h_misc + MISC_HIST + sp * 2 * HW;
+        int all[HW + 1];
+        for (int d = 0; d < HW; d++) {
+            S.twig_hist[d] = hist[d] + (d == 0 ? 1 : 0);
+            all[d] = hist[d] + hist[HW + d] + (d == 0 ? 1 : 0);
+        }
+        S.lvl_off[0] = 0;
+        S.maxdepth = 0;
+        for (int d = 0; d < HW; d++) {
+            S.lvl_off[d + 1] = S.lvl_off[d] + all[d];
+            if (all[d]) S.maxdepth = d;
+        }
+        const uint64_t *key = S.key[which[sp]];
+        const int sign = sp == 0 ? -1 : 1;
+        tb_emit(s, S, key, sign);
+        int w2;
+        tb_radix_sort(s, S.nn, S.dkey, S.dval, 0, 8, h->sort_tmp, &w2, true);
+        S.bfs = S.dval[w2];
+        tb_upward(s, S, key, sign, h->cfg.theta, h->d_misc + sp);
+        h->st.nodes[sp] = S.nn;
+        h->st.max_depth[sp] = S.maxdepth;
+    }
+    HIPCHK(hipGetLastError());
+    // integrity flags raised during the second phase (bad permutation / table)
+    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int sp = 0; sp < 2; sp++)
+        if (h->h_misc[sp] & 12) return fail(TB_ERR_HIP, "species %d: internal tree integrity check failed (%d)", sp,
+                                            h->h_misc[sp]);
+    if (h->cfg.walk == TB_WALK_MIXED) {
+        if ((rc = tb_mixed_prep(h->s, h->mx, h->sp, h->cfg, &h->bytes))) return fail(rc, "%s", tb_mixed_error());
+    }
+    h->built = true;
+    return TB_OK;
+}
+
+int force_all(tb_handle *h, bool counting)
+{
+    hipStream_t s = h->s;
+    if (h->cfg.walk == TB_WALK_MIXED && !counting) {
+        int rc = tb_mixed_force(s, h->mx, h->sp, h->cfg, h->epart, h->d_sums);
+        if (rc) return fail(rc, "%s", tb_mixed_error());
+    } else {
+        if (counting) HIPCHK(hipMemsetAsync(h->d_cnt, 0, 12 * sizeof(int64_t), s));
+        const int nb0 = tb_walk64_blocks(h->sp[0].n);
+        for (int tsp = 0; tsp < 2; tsp++) {
+            double *part = h->epart + (tsp == 0 ? 0 : 2 * nb0);
+            tb_walk64(s, tsp, h->sp[tsp], h->sp[0], h->sp[1], h->cfg.rs, part,
+                      counting ? h->d_cnt + 6 * tsp : nullptr);
+            tb_reduce_partials(s, tb_walk64_blocks(h->sp[tsp].n), 2, part, h->d_sums + 2 * tsp);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    h->have_force = true;
+    return TB_OK;
+}
+
+// d_sums[0..3] = ee, ei, ie, ii (electron-target launch, then ion-target launch)
+void epot_from_sums(const double *v, double *ee)
+{
+    // sys%Epot_e = Epot(ee) + Epot(ie) + Epot(ei); sys%Epot_i = Epot(ii) (tree.f90:688-712)
+    ee[0] = v[0] + v[2] + v[1];
+    ee[1] = v[3];
+}
+
+int vv2_all(tb_handle *h, double dt, int sample, bool fuse_next)
+{
+    hipStream_t s = h->s;
+    const int nb0 = tb_vv_blocks(h->sp[0].n);
+    for (int sp = 0; sp < 2; sp++) {
+        double *part = h->epart + (sp == 0 ? 0 : 2 * nb0);
+        if (fuse_next) tb_kdk(s, h->sp[sp], h->cfg.xm[sp], dt, sample, part);
+        else tb_vv2(s, h->sp[sp], h->cfg.xm[sp], dt, sample, part);
+        if (sample) tb_reduce_partials(s, tb_vv_blocks(h->sp[sp].n), 2, part, h->d_sums + 4 + 2 * sp);
+    }
+    HIPCHK(hipGetLastError());
+    return TB_OK;
+}
+
+}  // namespace
+
+// ============================================================== C-ABI
+
+extern "C" {
+
+const char *tb_last_error(void) { return g_err.c_str(); }
+
+int tb_create(tb_handle **out, const tb_config *cfg)
+{
+    if (!out) return fail(TB_ERR_ARG, "null handle pointer");
+    *out = nullptr;
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(TB_ERR_NODEV, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(TB_ERR_ARG, "device %d out of range", cfg->device);
+    HIPCHK(hipSetDevice(cfg->device));
+    tb_handle *h = new tb_handle();
+    h->cfg = *cfg;
+    if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(TB_ERR_HIP, "stream creation failed");
+    }
+    bool ok = dalloc_t(h, &h->bbox_tmp, 1024 * 6) == TB_OK && dalloc_t(h, &h->d_misc, 256) == TB_OK &&
+              dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, 12) == TB_OK;
+    ok = ok && hipHostMalloc((void **)&h->h_misc, 256 * sizeof(int32_t), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&h->h_sums, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    for (int k = 0; k < 6 && ok; k++) ok = hipEventCreate(&h->ev[k]) == hipSuccess;
+    if (!ok) {
+        std::string msg = g_err;
+        tb_destroy(h);
+        return fail(TB_ERR_NOMEM, "handle allocation failed %s", msg.c_str());
+    }
+    *out = h;
+    return TB_OK;
+}
+
+int tb_destroy(tb_handle *h)
+{
+    if (!h) return TB_OK;
+    hipSetDevice(h->cfg.device);
+    if (h->s) hipStreamSynchronize(h->s);
+    for (int sp = 0; sp < 2; sp++) free_species(h, h->sp[sp]);
+    tb_mixed_free(h->mx, &h->bytes);
+    dfree(h, h->sort_tmp, h->sort_tmp_bytes);
+    dfree(h, h->scan_tmp, h->scan_tmp_bytes);
+    dfree(h, h->bbox_tmp, 0);
+    dfree(h, h->d_misc, 0);
+    dfree(h, h->epart, 0);
+    dfree(h, h->d_sums, 0);
+    dfree(h, h->d_cnt, 0);
+    if (h->h_misc) hipHostFree(h->h_misc);
+    if (h->h_sums) hipHostFree(h->h_sums);
+    for (int k = 0; k < 6; k++)
+        if (h->ev[k]) hipEventDestroy(h->ev[k]);
+    if (h->s) hipStreamDestroy(h->s);
+    delete h;
+    return TB_OK;
+}
+
+int tb_set_config(tb_handle *h, const tb_config *cfg)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (cfg->device != h->cfg.device) return fail(TB_ERR_ARG, "device cannot change on a live handle");
+    h->cfg = *cfg;
+    h->built = false;  // rcrit and walk records depend on theta / walk
+    return TB_OK;
+}
+
+int tb_set_particles(tb_handle *h, int32_t ne, const tb_pt *e, int32_t ni, const tb_pt *i)
+{
+    if (!h || !e || !i) return fail(TB_ERR_ARG, "null argument");
+    if (ne < 1 || ni < 1) return fail(TB_ERR_ARG, "need at least one particle per species (ne=%d ni=%d)", ne, ni);
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipStreamSynchronize(h->s));
+    const int nn[2] = {ne, ni};
+    const tb_pt *src[2] = {e, i};
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        if (S.n != nn[sp]) {
+            free_species(h, S);
+            int rc = alloc_species(h, S, nn[sp]);
+            if (rc) return rc;
+        }
+        S.cur = 0;
+        const int n = nn[sp];
+        std::vector<double> col((size_t)n);
+        for (int k = 0; k < 9; k++) {
+            for (int j = 0; j < n; j++) {
+                const tb_pt &p = src[sp][j];
+                col[j] = k < 3 ? p.xx[k] : (k < 6 ? p.xv[k - 3] : p.ff[k - 6]);
+            }
+            double *dst = k < 3 ? S.x[0][k] : (k < 6 ? S.v[0][k - 3] : S.f[0][k - 6]);
+            HIPCHK(hipMemcpy(dst, col.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        }
+        std::vector<int32_t> id((size_t)n);
+        for (int j = 0; j < n; j++) id[j] = j;
+        HIPCHK(hipMemcpy(S.orig[0], id.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    }
+    h->built = false;
+    h->have_force = false;
+    return TB_OK;
+}
+
+int tb_get_particles(tb_handle *h, tb_pt *e, tb_pt *i)
+{
+    if (!h || !e || !i) return fail(TB_ERR_ARG, "null argument");
+    if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "no particles set");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipStreamSynchronize(h->s));
+    tb_pt *dst[2] = {e, i};
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        const int n = S.n, c = S.cur;
+        std::vector<int32_t> orig((size_t)n);
+        HIPCHK(hipMemcpy(orig.data(), S.orig[c], (size_t)n * 4, hipMemcpyDeviceToHost));
+        std::vector<double> col((size_t)n);
+        for (int k = 0; k < 9; k++) {
+            const double *src = k < 3 ? S.x[c][k] : (k < 6 ? S.v[c][k - 3] : S.f[c][k - 6]);
+            HIPCHK(hipMemcpy(col.data(), src, (size_t)n * 8, hipMemcpyDeviceToHost));
+            for (int j = 0; j < n; j++) {
+                tb_pt &p = dst[sp][orig[j]];
+                if (k < 3) p.xx[k] = col[j];
+                else if (k < 6) p.xv[k - 3] = col[j];
+                else p.ff[k - 6] = col[j];
+            }
+        }
+    }
+    return TB_OK;
+}
+
+int tb_build(tb_handle *h)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "tb_build before tb_set_particles");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipEventRecord(h->ev[0], h->s));
+    int rc = build_all(h);
+    if (rc) {
+        h->built = false;
+        return rc;
+    }
+    HIPCHK(hipEventRecord(h->ev[1], h->s));
+    return TB_OK;
+}
+
+int tb_force(tb_handle *h, double *epot_e, double *epot_i)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (!h->built) return fail(TB_ERR_STATE, "tb_force before tb_build");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipEventRecord(h->ev[2], h->s));
+    int rc = force_all(h, false);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(h->ev[3], h->s));
+    HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 4 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    epot_from_sums(h->h_sums, h->epot);
+    if (epot_e) *epot_e = h->epot[0];
+    if (epot_i) *epot_i = h->epot[1];
+    return TB_OK;
+}
+
+int tb_vv1(tb_handle *h, double dt)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "no particles set");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    for (int sp = 0; sp < 2; sp++) tb_vv1(h->s, h->sp[sp], h->cfg.xm[sp], dt);
+    HIPCHK(hipGetLastError());
+    h->built = false;  // positions moved
+    return TB_OK;
+}
+
+int tb_vv2(tb_handle *h, double dt, int32_t sample, double *sums)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "no particles set");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    int rc = vv2_all(h, dt, sample ? 1 : 0, false);
+    if (rc) return rc;
+    if (sample) {
+        HIPCHK(hipMemcpyAsync(h->h_sums + 4, h->d_sums + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        if (sums) {
+            sums[0] = h->h_sums[4];  // mv2el
+            sums[1] = h->h_sums[6];  // mv2ion
+            sums[2] = h->h_sums[5];  // Te_rad
+            sums[3] = h->h_sums[7];  // Ti_rad
+        }
+    }
+    return TB_OK;
+}
+
+int tb_step(tb_handle *h, double dt, int32_t nsteps, double *out)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (nsteps < 1) return fail(TB_ERR_ARG, "nsteps must be >= 1");
+    if (!h->have_force) return fail(TB_ERR_STATE, "tb_step needs forces: call tb_build + tb_force first");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    for (int sp = 0; sp < 2; sp++) tb_vv1(h->s, h->sp[sp], h->cfg.xm[sp], dt);
+    int rc;
+    for (int k = 0; k < nsteps; k++) {
+        HIPCHK(hipEventRecord(h->ev[0], h->s));
+        if ((rc = build_all(h))) return rc;
+        HIPCHK(hipEventRecord(h->ev[1], h->s));
+        if ((rc = force_all(h, false))) return rc;
+        HIPCHK(hipEventRecord(h->ev[3], h->s));
+        const bool last = k == nsteps - 1;
+        if ((rc = vv2_all(h, dt, last && out ? 1 : 0, !last))) return rc;
+        HIPCHK(hipEventRecord(h->ev[5], h->s));
+    }
+    HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 8 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    epot_from_sums(h->h_sums, h->epot);
+    h->built = false;
+    if (out) {
+        out[0] = h->epot[0];
+        out[1] = h->epot[1];
+        out[2] = h->h_sums[4];
+        out[3] = h->h_sums[6];
+        out[4] = h->h_sums[5];
+        out[5] = h->h_sums[7];
+    }
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, h->ev[0], h->ev[1]) == hipSuccess) h->st.ms_build = ms;
+    if (hipEventElapsedTime(&ms, h->ev[1], h->ev[3]) == hipSuccess) h->st.ms_force = ms;
+    if (hipEventElapsedTime(&ms, h->ev[3], h->ev[5]) == hipSuccess) h->st.ms_vv = ms;
+    return TB_OK;
+}
+
+int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, int32_t *depth, int32_t *leaf,
+                   int32_t *pt, int32_t *count, int32_t *next, double *xx, double *lx, double *qeff, double *dx,
+                   double *D, double *Q, double *QQ)
+{
+    if (!h || !nnodes) return fail(TB_ERR_ARG, "null argument");
+    if (species != 0 && species != 1) return fail(TB_ERR_ARG, "species must be 0 or 1");
+    if (!h->built) return fail(TB_ERR_STATE, "tb_export_tree before tb_build");
+    SpeciesDev &S = h->sp[species];
+    *nnodes = S.nn;
+    if (cap < S.nn) return fail(TB_ERR_ARG, "cap %d < %d nodes", cap, S.nn);
+    if (!depth || !leaf || !pt || !count || !next || !xx || !lx || !qeff || !dx || !D || !Q || !QQ)
+        return fail(TB_ERR_ARG, "null output array");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    const size_t n = (size_t)S.nn;
+    void *tmp = nullptr;
+    int rc = dalloc(h, &tmp, n * (5 * 4 + 12 * 8) + 256);
+    if (rc) return rc;
+    char *b = (char *)tmp;
+    int32_t *d_depth = (int32_t *)b, *d_leaf = d_depth + n, *d_pt = d_leaf + n, *d_cnt = d_pt + n,
+            *d_next = d_cnt + n;
+    double *dd = (double *)(((uintptr_t)(d_next + n) + 15) & ~(uintptr_t)15);
+    double *d_xx = dd, *d_q = d_xx + 3 * n, *d_dx = d_q + n, *d_D = d_dx + n, *d_Q = d_D + 3 * n, *d_QQ = d_Q + 3 * n;
+    tb_export_gather(h->s, S, d_depth, d_leaf, d_pt, d_cnt, d_next, d_xx, d_q, d_dx, d_D, d_Q, d_QQ);
+    double root[6];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    if (e == hipSuccess) e = hipMemcpy(root, S.root, sizeof(root), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(depth, d_depth, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(leaf, d_leaf, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(pt, d_pt, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(count, d_cnt, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(next, d_next, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(xx, d_xx, n * 24, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(qeff, d_q, n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(dx, d_dx, n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(D, d_D, n * 24, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(Q, d_Q, n * 24, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(QQ, d_QQ, n * 24, hipMemcpyDeviceToHost);
+    dfree(h, tmp, n * (5 * 4 + 12 * 8) + 256);
+    if (e != hipSuccess) return fail(TB_ERR_HIP, "tb_export_tree: %s", hipGetErrorString(e));
+    // cell size: the root length halved once per level (exact), tree.f90:126
+    for (size_t j = 0; j < n; j++) {
+        double l[3] = {root[3], root[4], root[5]};
+        for (int d = 0; d < depth[j]; d++)
+            for (int k = 0; k < 3; k++) l[k] = l[k] * 0.5;
+        for (int k = 0; k < 3; k++) lx[3 * j + k] = l[k];
+    }
+    return TB_OK;
+}
+
+int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f)
+{
+    if (!h || !tspecies || !tidx || !f) return fail(TB_ERR_ARG, "null argument");
+    if (nt < 1) return fail(TB_ERR_ARG, "nt must be >= 1");
+    if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "no particles set");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipStreamSynchronize(h->s));
+    // reference index -> current (key-ordered) index
+    std::vector<int32_t> sidx((size_t)nt);
+    std::vector<int32_t> inv[2];
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        std::vector<int32_t> orig((size_t)S.n);
+        HIPCHK(hipMemcpy(orig.data(), S.orig[S.cur], (size_t)S.n * 4, hipMemcpyDeviceToHost));
+        inv[sp].assign((size_t)S.n, 0);
+        for (int j = 0; j < S.n; j++) inv[sp][orig[j]] = j;
+    }
+    for (int t = 0; t < nt; t++) {
+        int sp = tspecies[t];
+        if (sp != 0 && sp != 1) return fail(TB_ERR_ARG, "tspecies[%d] = %d", t, sp);
+        if (tidx[t] < 0 || tidx[t] >= h->sp[sp].n) return fail(TB_ERR_ARG, "tidx[%d] = %d out of range", t, tidx[t]);
+        sidx[t] = inv[sp][tidx[t]];
+    }
+    void *tmp = nullptr;
+    int rc = dalloc(h, &tmp, (size_t)nt * (8 + 24) + 64);
+    if (rc) return rc;
+    int32_t *d_sp = (int32_t *)tmp, *d_ix = d_sp + nt;
+    double *d_f = (double *)(((uintptr_t)(d_ix + nt) + 15) & ~(uintptr_t)15);
+    hipError_t e = hipMemcpy(d_sp, tspecies, (size_t)nt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ix, sidx.data(), (size_t)nt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        tb_direct(h->s, nt, d_sp, d_ix, h->sp[0], h->sp[1], h->cfg.rs, d_f);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    if (e == hipSuccess) e = hipMemcpy(f, d_f, (size_t)nt * 24, hipMemcpyDeviceToHost);
+    dfree(h, tmp, (size_t)nt * (8 + 24) + 64);
+    if (e != hipSuccess) return fail(TB_ERR_HIP, "tb_direct_sample: %s", hipGetErrorString(e));
+    return TB_OK;
+}
+
+int tb_count_interactions(tb_handle *h, tb_stats *st)
+{
+    if (!h || !st) return fail(TB_ERR_ARG, "null argument");
+    if (!h->built) return fail(TB_ERR_STATE, "tb_count_interactions before tb_build");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    int64_t c[12];
+    if (h->cfg.walk == TB_WALK_MIXED) {
+        int rc = tb_mixed_count(h->s, h->mx, h->sp, h->cfg, h->epart, h->d_sums, h->d_cnt);
+        if (rc) return fail(rc, "%s", tb_mixed_error());
+    } else {
+        int rc = force_all(h, true);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c, h->d_cnt, sizeof(c), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    // launch order: electron targets (e-tree = ee, i-tree = ei), ion targets (ie, ii)
+    const int map[4] = {0, 6, 3, 9};  // ee, ie, ei, ii -> offset in c
+    for (int p = 0; p < 4; p++) {
+        h->st.visits[p] = c[map[p]];
+        h->st.m2p[p] = c[map[p] + 1];
+        h->st.p2p[p] = c[map[p] + 2];
+    }
+    *st = h->st;
+    return TB_OK;
+}
+
+int tb_get_stats(tb_handle *h, tb_stats *st)
+{
+    if (!h || !st) return fail(TB_ERR_ARG, "null argument");
+    *st = h->st;
+    return TB_OK;
+}
+
+}  // extern "C"

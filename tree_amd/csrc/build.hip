@@ -1,0 +1,653 @@
+// build.hip -- tree construction on the GPU (replaces BuildLocalTree,
+// tree.f90:39-100, Insert_Tree_e/_i 103-447 and PSeudoE/I 450-659).
+//
+// Pipeline per species (all on the handle's stream):
+//   bbox      -> root centre (max+min)/2 and per-axis length max-min   (tree.f90:52-63)
+//   keys      -> 21 octal digits from the reference's FP64 centre recurrence
+//                c' = c + (+-0.25)*L, L' = L*0.5 with '<' going low   (tree.f90:114-173)
+//   sort      -> LSD radix sort, 8-bit digits, LDS-staged wave-ranked scatter
+//   lcp       -> leaf depth = 1 + max common-prefix with a neighbour; one
+//                particle per leaf, single-child chains kept (insertion semantics)
+//   emit      -> preorder node table (start particle, depth) = reference DFS
+//                order; skip pointers replace the 8 child pointers
+//   upward    -> level by level, child-point moments exactly as PSeudoE/I
+//                (centroid, dx, D, Q, QQ over the children's centroids),
+//                plus the exact second moment for the mixed walk.
+// Contraction is off in this file so that every FP64 operation rounds as the
+// reference's does; node tables are bit-identical to the reference's.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "tb_internal.h"
+
+namespace {
+
+constexpr int BT = 256;  // block threads
+
+__device__ inline int lcp_digits(uint64_t a, uint64_t b)
+{
+    uint64_t x = a ^ b;
+    if (x == 0) return TB_LEVELS;
+    return (__clzll((long long)x) - 1) / 3;
+}
+
+// ------------------------------------------------------------------ bbox
+
+__global__ __launch_bounds__(BT) void k_bbox_part(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                  const double *__restrict__ z, double *__restrict__ part)
+{
+    double mn0 = DBL_MAX, mn1 = DBL_MAX, mn2 = DBL_MAX, mx0 = -DBL_MAX, mx1 = -DBL_MAX, mx2 = -DBL_MAX;
+    for (int i = blockIdx.x * BT + threadIdx.x; i < n; i += gridDim.x * BT) {
+        double a = x[i], b = y[i], c = z[i];
+        mn0 = fmin(mn0, a); mx0 = fmax(mx0, a);
+        mn1 = fmin(mn1, b); mx1 = fmax(mx1, b);
+        mn2 = fmin(mn2, c); mx2 = fmax(mx2, c);
+    }
+    __shared__ double s[6][BT];
+    s[0][threadIdx.x] = mn0; s[1][threadIdx.x] = mn1; s[2][threadIdx.x] = mn2;
+    s[3][threadIdx.x] = mx0; s[4][threadIdx.x] = mx1; s[5][threadIdx.x] = mx2;
+    __syncthreads();
+    for (int w = BT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+            for (int k = 3; k < 6; k++) s[k][threadIdx.x] = fmax(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void k_bbox_final(int nb, const double *__restrict__ part, double *__restrict__ root)
+{
+    if (threadIdx.x != 0) return;
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fmin(mn[k], part[b * 6 + k]);
+            mx[k] = fmax(mx[k], part[b * 6 + 3 + k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        root[k] = (mx[k] + mn[k]) / 2.0;   // Roote%xx, tree.f90:58-60
+        root[3 + k] = mx[k] - mn[k];       // Roote%lx, tree.f90:61-63
+    }
+}
+
+// ------------------------------------------------------------------ keys
+
+__global__ __launch_bounds__(BT) void k_keys(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ z, const double *__restrict__ root,
+                                             uint64_t *__restrict__ key, uint32_t *__restrict__ val,
+                                             int32_t *__restrict__ err)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    double px = x[i], py = y[i], pz = z[i];
+    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(err, 2);
+    double cx = root[0], cy = root[1], cz = root[2];
+    double lx = root[3], ly = root[4], lz = root[5];
+    uint64_t k = 0;
+#pragma unroll
+    for (int l = 0; l < TB_LEVELS; l++) {
+        // strict '<' goes to the low half, ties and NaN to the high half
+        uint32_t bx = (px < cx) ? 0u : 1u;
+        uint32_t by = (py < cy) ? 0u : 1u;
+        uint32_t bz = (pz < cz) ? 0u : 1u;
+        k = (k << 3) | (uint64_t)(bx | (by << 1) | (bz << 2));
+        cx = cx + (bx ? 0.25 : -0.25) * lx;
+        cy = cy + (by ? 0.25 : -0.25) * ly;
+        cz = cz + (bz ? 0.25 : -0.25) * lz;
+        lx = lx * 0.5;
+        ly = ly * 0.5;
+        lz = lz * 0.5;
+    }
+    key[i] = k;
+    val[i] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------ radix sort
+
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = BT * RS_ITEMS;  // 4096 keys per block
+constexpr int RS_WAVES = BT / 64;
+constexpr int RS_WSEG = 64 * RS_ITEMS;  // keys per wave
+
+// lanes holding the same 8-bit digit
+__device__ inline uint64_t match8(uint32_t d)
+{
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        bool bit = (d >> b) & 1u;
+        uint64_t bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(BT) void k_rs_up(int n, const uint64_t *__restrict__ key, int shift,
+                                              uint32_t *__restrict__ hist, int nb)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long base = (long)blockIdx.x * RS_TILE + wave * RS_WSEG;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        long i = base + r * 64 + lane;
+        uint64_t k = i < n ? key[i] : ~0ull;  // pad: digit 255, sorts after all real keys
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint64_t m = match8(d);
+        if ((int)__builtin_ctzll(m) == lane) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of 256 values held one per thread; returns the block total
+__device__ inline uint32_t scan256_excl(uint32_t v, uint32_t *lds, uint32_t *total)
+{
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t t = threadIdx.x >= off ? lds[threadIdx.x - off] : 0u;
+        __syncthreads();
+        lds[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t incl = lds[threadIdx.x];
+    if (total) *total = lds[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(BT) void k_rs_scan(int nb, uint32_t *__restrict__ hist, uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t lds[256];
+    uint32_t *row = hist + (long)blockIdx.x * nb;
+    int per = (nb + BT - 1) / BT;
+    int beg = threadIdx.x * per, end = min(beg + per, nb);
+    uint32_t sum = 0;
+    for (int j = beg; j < end; j++) sum += row[j];
+    uint32_t tot;
+    uint32_t run = scan256_excl(sum, lds, &tot);
+    for (int j = beg; j < end; j++) {
+        uint32_t v = row[j];
+        row[j] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(BT) void k_rs_down(int n, const uint64_t *__restrict__ kin,
+                                                const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+                                                uint32_t *__restrict__ vout, int shift,
+                                                const uint32_t *__restrict__ hist, const uint32_t *__restrict__ totals,
+                                                int nb, int iota)
+{
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ uint32_t sval[RS_TILE];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    __shared__ uint32_t bstart[256];
+    __shared__ uint32_t gstart[256];
+    __shared__ uint32_t lds[256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int w = 0; w < RS_WAVES; w++) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+
+    const long tile = (long)blockIdx.x * RS_TILE;
+    const long base = tile + wave * RS_WSEG;
+    uint64_t k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    uint32_t pos[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        long i = base + r * 64 + lane;
+        k[r] = i < n ? kin[i] : ~0ull;
+        v[r] = i < n ? (iota ? (uint32_t)i : vin[i]) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+        uint64_t m = match8(d);
+        uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        uint32_t rank = (uint32_t)__popcll(m & lt);
+        uint32_t prior = wcnt[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if ((int)__builtin_ctzll(m) == lane) wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        pos[r] = prior + rank;
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+        for (int w = 0; w < RS_WAVES; w++) {
+            uint32_t c = wcnt[w][d];
+            wcnt[w][d] = run;
+            run += c;
+        }
+        uint32_t bs = scan256_excl(run, lds, nullptr);
+        bstart[d] = bs;
+        uint32_t gb = scan256_excl(totals[d], lds, nullptr);
+        gstart[d] = gb + hist[(long)d * nb + blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+        uint32_t lp = bstart[d] + wcnt[wave][d] + pos[r];
+        skey[lp] = k[r];
+        sval[lp] = v[r];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < RS_TILE; j += BT) {
+        uint64_t kk = skey[j];
+        uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        long g = (long)gstart[d] + (j - (long)bstart[d]);
+        if (g < n) {
+            kout[g] = kk;
+            vout[g] = sval[j];
+        }
+    }
+}
+
+// -------------------------------------------------------------- permute
+
+struct PermCols {
+    const double *src[9];
+    double *dst[9];
+};
+
+__global__ __launch_bounds__(BT) void k_permute(int n, const uint32_t *__restrict__ perm, PermCols cols,
+                                                const int32_t *__restrict__ osrc, int32_t *__restrict__ odst,
+                                                int32_t *__restrict__ err)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    uint32_t j = perm[i];
+    if (j >= (uint32_t)n) {  // never for a valid sort; reported instead of read out of bounds
+        atomicOr(err, 4);
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 9; a++) cols.dst[a][i] = cols.src[a][j];
+    odst[i] = osrc[j];
+}
+
+// ------------------------------------------------------------------ scan
+
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = BT * SC_ITEMS;
+
+__global__ __launch_bounds__(BT) void k_scan_part(int n, const int32_t *__restrict__ in, int32_t *__restrict__ bsum)
+{
+    __shared__ uint32_t lds[256];
+    long base = (long)blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+    int32_t s = 0;
+    for (int r = 0; r < SC_ITEMS; r++) {
+        long i = base + r;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    scan256_excl((uint32_t)s, lds, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (int32_t)tot;
+}
+
+__global__ __launch_bounds__(BT) void k_scan_bsum(int nb, int32_t *__restrict__ bsum, int32_t *__restrict__ total)
+{
+    __shared__ uint32_t lds[256];
+    int per = (nb + BT - 1) / BT;
+    int beg = threadIdx.x * per, end = min(beg + per, nb);
+    uint32_t s = 0;
+    for (int j = beg; j < end; j++) s += (uint32_t)bsum[j];
+    uint32_t tot;
+    uint32_t run = scan256_excl(s, lds, &tot);
+    for (int j = beg; j < end; j++) {
+        uint32_t v = (uint32_t)bsum[j];
+        bsum[j] = (int32_t)run;
+        run += v;
+    }
+    if (threadIdx.x == 0) *total = (int32_t)tot;
+}
+
+__global__ __launch_bounds__(BT) void k_scan_final(int n, const int32_t *__restrict__ in,
+                                                   const int32_t *__restrict__ bsum, int32_t *__restrict__ out)
+{
+    __shared__ uint32_t lds[256];
+    long base = (long)blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+    int32_t v[SC_ITEMS];
+    int32_t s = 0;
+    for (int r = 0; r < SC_ITEMS; r++) {
+        long i = base + r;
+        v[r] = i < n ? in[i] : 0;
+        s += v[r];
+    }
+    uint32_t run = scan256_excl((uint32_t)s, lds, nullptr) + (uint32_t)bsum[blockIdx.x];
+    for (int r = 0; r < SC_ITEMS; r++) {
+        long i = base + r;
+        if (i < n) out[i] = (int32_t)run;
+        run += (uint32_t)v[r];
+    }
+}
+
+// ------------------------------------------------------------------- lcp
+
+__global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ key, int32_t *__restrict__ cnt,
+                                            int32_t *__restrict__ hist, int32_t *__restrict__ err)
+{
+    __shared__ int32_t lh[2][TB_LEVELS + 2];
+    if (threadIdx.x < 2 * (TB_LEVELS + 2)) (&lh[0][0])[threadIdx.x] = 0;
+    __syncthreads();
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i < n) {
+        uint64_t k = key[i];
+        int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
+        int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
+        int mx = max(lm, lp);
+        if (mx >= TB_LEVELS) {
+            atomicOr(err, 1);
+            mx = TB_LEVELS - 1;
+            lm = min(lm, mx);
+        }
+        int leafd = 1 + mx;
+        cnt[i] = leafd - lm;
+        for (int d = lm + 1; d < leafd; d++) atomicAdd(&lh[0][d], 1);
+        atomicAdd(&lh[1][leafd], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * (TB_LEVELS + 2)) {
+        int32_t v = (&lh[0][0])[threadIdx.x];
+        if (v) atomicAdd(&hist[threadIdx.x], v);
+    }
+}
+
+// ------------------------------------------------------------------ emit
+
+__global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__ key, const int32_t *__restrict__ offs,
+                                             int total, const double *__restrict__ px, const double *__restrict__ py,
+                                             const double *__restrict__ pz, double sign, double4 *__restrict__ nmac,
+                                             uint32_t *__restrict__ nnext, double *__restrict__ nmom,
+                                             double *__restrict__ nm2, double *__restrict__ ndx,
+                                             int32_t *__restrict__ nstart, int32_t *__restrict__ ncount,
+                                             int32_t *__restrict__ ndepth, uint64_t *__restrict__ dkey)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0) {
+        ndepth[0] = 0; nstart[0] = 0; ncount[0] = n; nnext[0] = (uint32_t)total; dkey[0] = 0;
+    }
+    uint64_t k = key[i];
+    int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
+    int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
+    int mx = min(max(lm, lp), TB_LEVELS - 1);
+    lm = min(lm, mx);
+    int leafd = 1 + mx;
+    int base = offs[i] + 1;
+    for (int d = lm + 1; d <= leafd; d++) {
+        int id = base + (d - lm - 1);
+        ndepth[id] = d;
+        nstart[id] = i;
+        dkey[id] = (uint64_t)d;
+        int end;
+        if (d == leafd) {
+            end = i + 1;
+        } else {
+            // first particle past the cell: gallop then bisect on the key prefix
+            int sh = 3 * (TB_LEVELS - d);
+            uint64_t p = k >> sh;
+            int lo = i, hi = i + 1, step = 1;
+            while (hi < n && (key[hi] >> sh) == p) {
+                lo = hi;
+                step <<= 1;
+                hi = i + step;
+            }
+            if (hi > n) hi = n;
+            while (hi - lo > 1) {
+                int mid = lo + ((hi - lo) >> 1);
+                if ((key[mid] >> sh) == p) lo = mid;
+                else hi = mid;
+            }
+            end = hi;
+        }
+        ncount[id] = end - i;
+        uint32_t nxt = end >= n ? (uint32_t)total : (uint32_t)(offs[end] + 1);
+        if (d == leafd) {
+            nnext[id] = nxt | TB_LEAF;
+            nmac[id] = make_double4(px[i], py[i], pz[i], -1.0);   // leaf xx = particle, tree.f90:461
+            double *m = nmom + (long)id * 10;
+            m[0] = sign;
+            for (int c = 1; c < 10; c++) m[c] = 0.0;
+            double *q = nm2 + (long)id * 6;
+            for (int c = 0; c < 6; c++) q[c] = 0.0;
+            ndx[id] = 0.0;
+        } else {
+            nnext[id] = nxt;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- upward
+
+__global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restrict__ seg,
+                                               const uint64_t *__restrict__ key, const double *__restrict__ root,
+                                               double sign, double theta, double4 *__restrict__ nmac,
+                                               const uint32_t *__restrict__ nnext, double *__restrict__ nmom,
+                                               double *__restrict__ nm2, double *__restrict__ ndx,
+                                               const int32_t *__restrict__ nstart, const int32_t *__restrict__ ndepth,
+                                               const int32_t *__restrict__ ncount, int32_t *__restrict__ err)
+{
+    int t = blockIdx.x * BT + threadIdx.x;
+    if (t >= cnt) return;
+    uint32_t p = seg[t];
+    uint32_t endp = nnext[p];
+    if (endp & TB_LEAF) return;
+    int d = ndepth[p];
+    // geometric centre and size of the cell, replaying Insert_Tree's placement
+    double ox = root[0], oy = root[1], oz = root[2];
+    double lx = root[3], ly = root[4], lz = root[5];
+    uint64_t k = key[nstart[p]];
+    for (int l = 0; l < d; l++) {
+        uint32_t dig = (uint32_t)(k >> (3 * (TB_LEVELS - 1 - l))) & 7u;
+        ox = ox + ((dig & 1u) ? 0.25 : -0.25) * lx;
+        oy = oy + ((dig & 2u) ? 0.25 : -0.25) * ly;
+        oz = oz + ((dig & 4u) ? 0.25 : -0.25) * lz;
+        lx = lx * 0.5;
+        ly = ly * 0.5;
+        lz = lz * 0.5;
+    }
+    // PSeudoE: centroid over the children, children in order 1..8
+    // children are p+1 and then successive skip pointers; a pointer that does
+    // not advance would be a malformed table: flag it and stop
+    for (uint32_t c = p + 1; c < endp;) {
+        uint32_t nc = nnext[c] & ~TB_LEAF;
+        if (nc <= c) {
+            atomicOr(err, 8);
+            return;
+        }
+        c = nc;
+    }
+    double qsum = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+    for (uint32_t c = p + 1; c < endp; c = nnext[c] & ~TB_LEAF) {
+        double4 m = nmac[c];
+        double q = nmom[(long)c * 10];
+        qsum = qsum + q;
+        X = X + q * m.x;
+        Y = Y + q * m.y;
+        Z = Z + q * m.z;
+    }
+    X = X / qsum;
+    Y = Y / qsum;
+    Z = Z / qsum;
+    double ax = X - ox, ay = Y - oy, az = Z - oz;
+    double dx = sqrt(ax * ax + ay * ay + az * az);
+    double D0 = 0.0, D1 = 0.0, D2 = 0.0, Q0 = 0.0, Q1 = 0.0, Q2 = 0.0, R0 = 0.0, R1 = 0.0, R2 = 0.0;
+    double M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t c = p + 1; c < endp; c = nnext[c] & ~TB_LEAF) {
+        double4 m = nmac[c];
+        double q = nmom[(long)c * 10];
+        double d0 = m.x - X, d1 = m.y - Y, d2 = m.z - Z;
+        D0 = D0 + q * d0;
+        D1 = D1 + q * d1;
+        D2 = D2 + q * d2;
+        Q0 = Q0 + q * (d0 * d0) / 2.0;
+        Q1 = Q1 + q * (d1 * d1) / 2.0;
+        Q2 = Q2 + q * (d2 * d2) / 2.0;
+        R0 = R0 + q * d0 * d1;
+        R1 = R1 + q * d1 * d2;
+        R2 = R2 + q * d2 * d0;
+        const double *cm = nm2 + (long)c * 6;
+        M[0] = M[0] + (cm[0] + q * (d0 * d0));
+        M[1] = M[1] + (cm[1] + q * (d1 * d1));
+        M[2] = M[2] + (cm[2] + q * (d2 * d2));
+        M[3] = M[3] + (cm[3] + q * (d0 * d1));
+        M[4] = M[4] + (cm[4] + q * (d1 * d2));
+        M[5] = M[5] + (cm[5] + q * (d2 * d0));
+    }
+    double s = lx;
+    if (ly > s) s = ly;
+    if (lz > s) s = lz;
+    double rcrit = s / theta + dx;   // MAC left side, tree.f90:742
+    nmac[p] = make_double4(X, Y, Z, rcrit);
+    double *mo = nmom + (long)p * 10;
+    mo[0] = sign * (double)ncount[p];
+    mo[1] = D0; mo[2] = D1; mo[3] = D2;
+    mo[4] = Q0; mo[5] = Q1; mo[6] = Q2;
+    mo[7] = R0; mo[8] = R1; mo[9] = R2;
+    double *m2 = nm2 + (long)p * 6;
+    for (int c = 0; c < 6; c++) m2[c] = M[c];
+    ndx[p] = dx;
+}
+
+// ------------------------------------------------------------ export
+
+__global__ __launch_bounds__(BT) void k_export(int nn, const double4 *__restrict__ nmac,
+                                               const uint32_t *__restrict__ nnext, const double *__restrict__ nmom,
+                                               const double *__restrict__ ndx, const int32_t *__restrict__ nstart,
+                                               const int32_t *__restrict__ ncount, const int32_t *__restrict__ ndepth,
+                                               const int32_t *__restrict__ orig, int32_t *depth, int32_t *leaf,
+                                               int32_t *pt, int32_t *count, int32_t *next, double *xx, double *qeff,
+                                               double *dx, double *D, double *Q, double *QQ)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= nn) return;
+    uint32_t nx = nnext[i];
+    bool lf = (nx & TB_LEAF) != 0;
+    depth[i] = ndepth[i];
+    leaf[i] = lf ? 1 : 0;
+    pt[i] = lf ? orig[nstart[i]] : -1;
+    count[i] = ncount[i];
+    next[i] = (int32_t)(nx & ~TB_LEAF);
+    double4 m = nmac[i];
+    xx[3 * i] = m.x; xx[3 * i + 1] = m.y; xx[3 * i + 2] = m.z;
+    const double *mo = nmom + (long)i * 10;
+    qeff[i] = mo[0];
+    for (int k = 0; k < 3; k++) {
+        D[3 * i + k] = mo[1 + k];
+        Q[3 * i + k] = mo[4 + k];
+        QQ[3 * i + k] = mo[7 + k];
+    }
+    dx[i] = ndx[i];
+}
+
+inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
+
+}  // namespace
+
+// ====================================================================== host
+
+size_t tb_sort_temp_bytes(int n)
+{
+    int nb = nblk(n, RS_TILE);
+    return (size_t)256 * nb * sizeof(uint32_t) + 256 * sizeof(uint32_t) + 256;
+}
+
+size_t tb_scan_temp_bytes(int n) { return (size_t)(nblk(n, SC_TILE) + 64) * sizeof(int32_t); }
+
+void tb_bbox(hipStream_t s, int n, const double *x, const double *y, const double *z, double *root, double *tmp)
+{
+    int nb = std::min(nblk(n, BT), 1024);
+    hipLaunchKernelGGL(k_bbox_part, dim3(nb), dim3(BT), 0, s, n, x, y, z, tmp);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, nb, tmp, root);
+}
+
+void tb_keys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
+             uint64_t *key, uint32_t *val, int32_t *err)
+{
+    hipLaunchKernelGGL(k_keys, dim3(nblk(n, BT)), dim3(BT), 0, s, n, x, y, z, root, key, val, err);
+}
+
+void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int lo_bit, int hi_bit, void *tmp,
+                   int *which, bool init_iota)
+{
+    int nb = nblk(n, RS_TILE);
+    uint32_t *hist = (uint32_t *)tmp;
+    uint32_t *totals = hist + (size_t)256 * nb;
+    int src = 0;
+    bool iota = init_iota;
+    for (int sh = lo_bit; sh < hi_bit; sh += 8) {
+        hipLaunchKernelGGL(k_rs_up, dim3(nb), dim3(BT), 0, s, n, key[src], sh, hist, nb);
+        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s, nb, hist, totals);
+        hipLaunchKernelGGL(k_rs_down, dim3(nb), dim3(BT), 0, s, n, key[src], val[src], key[1 - src], val[1 - src],
+                           sh, hist, totals, nb, iota ? 1 : 0);
+        iota = false;
+        src = 1 - src;
+    }
+    *which = src;
+}
+
+void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err)
+{
+    const int c = S.cur, o = 1 - S.cur;
+    PermCols cols;
+    for (int k = 0; k < 3; k++) {
+        cols.src[k] = S.x[c][k]; cols.src[3 + k] = S.v[c][k]; cols.src[6 + k] = S.f[c][k];
+        cols.dst[k] = S.x[o][k]; cols.dst[3 + k] = S.v[o][k]; cols.dst[6 + k] = S.f[o][k];
+    }
+    hipLaunchKernelGGL(k_permute, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, perm, cols, (const int32_t *)S.orig[c],
+                       S.orig[o], err);
+    S.cur = o;
+}
+
+void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist, int32_t *err)
+{
+    hipLaunchKernelGGL(k_lcp, dim3(nblk(n, BT)), dim3(BT), 0, s, n, key, cnt, hist, err);
+}
+
+void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t *total, void *tmp)
+{
+    int nb = nblk(n, SC_TILE);
+    int32_t *bsum = (int32_t *)tmp;
+    hipLaunchKernelGGL(k_scan_part, dim3(nb), dim3(BT), 0, s, n, in, bsum);
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(BT), 0, s, nb, bsum, total);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(BT), 0, s, n, in, bsum, out);
+}
+
+void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign)
+{
+    const int c = S.cur;
+    hipLaunchKernelGGL(k_emit, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, key, S.offs, S.nn, S.x[c][0], S.x[c][1],
+                       S.x[c][2], (double)sign, S.nmac, S.nnext, S.nmom, S.nm2, S.ndx, S.nstart, S.ncount, S.ndepth,
+                       S.dkey[0]);
+}
+
+void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err)
+{
+    for (int d = S.maxdepth; d >= 0; d--) {
+        int beg = S.lvl_off[d], cnt = S.lvl_off[d + 1] - beg;
+        if (cnt <= 0 || S.twig_hist[d] == 0) continue;
+        hipLaunchKernelGGL(k_upward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, cnt, S.bfs + beg, key, S.root,
+                           (double)sign, theta, S.nmac, S.nnext, S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, S.ncount, err);
+    }
+}
+
+void tb_export_gather(hipStream_t s, const SpeciesDev &S, int32_t *depth, int32_t *leaf, int32_t *pt,
+                      int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,
+                      double *QQ)
+{
+    hipLaunchKernelGGL(k_export, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.nmac, S.nnext, S.nmom, S.ndx,
+                       S.nstart, S.ncount, S.ndepth, S.orig[S.cur], depth, leaf, pt, count, next, xx, qeff, dx, D, Q,
+                       QQ);
+}

@@ -1,0 +1,84 @@
+// tb_internal.h -- device data layout and launcher declarations shared by the
+// HIP translation units of libtreemi.so.  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/treemi.h"
+
+// Key depth: 21 octal digits in bits 62..0 of a 64-bit key.
+#define TB_LEVELS 21
+// Leaf marker in the preorder skip-pointer array.
+#define TB_LEAF 0x80000000u
+// Coulomb constant eV*um (data.f90:40).
+#define TB_EPS 0.0014399835661145824
+
+// Per-species device buffers.  Particle state is SoA FP64 kept in the key
+// order of the last build (double-buffered for the permutation); orig[] maps
+// back to the order the caller handed in.
+struct SpeciesDev {
+    int n = 0;
+    int cur = 0;
+    double *x[2][3] = {}, *v[2][3] = {}, *f[2][3] = {};
+    int32_t *orig[2] = {};
+    uint64_t *key[2] = {};
+    uint32_t *val[2] = {};
+    int32_t *cnt = nullptr, *offs = nullptr;   // nodes started per particle, exclusive scan
+    double *root = nullptr;                    // c0[3], L0[3]
+    // node table in preorder (reference DFS order)
+    int nn = 0, cap = 0, maxdepth = 0;
+    double4 *nmac = nullptr;    // centroid xyz, rcrit = max(lx)/theta + dx (leaf: -1)
+    uint32_t *nnext = nullptr;  // preorder index after the subtree | TB_LEAF
+    double *nmom = nullptr;     // [nn][10] qeff, D(3), Q(3), QQ(3)  (child-point, tree.f90:537-551)
+    double *nm2 = nullptr;      // [nn][6] exact second moment about the centroid xx,yy,zz,xy,yz,zx
+    double *ndx = nullptr;      // centroid offset from the geometric centre
+    int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
+    uint64_t *dkey[2] = {};     // depth-sort keys / node ids -> breadth-first order
+    uint32_t *dval[2] = {};
+    uint32_t *bfs = nullptr;    // preorder ids grouped by depth, preorder within a depth
+    int lvl_off[TB_LEVELS + 3] = {};
+    int twig_hist[TB_LEVELS + 2] = {};
+    // mixed walk: breadth-first FP32 records (filled by the mixed-walk prep)
+    float4 *wmac = nullptr;     // centroid hi xyz, rcrit (fp32, rounded up)
+    float4 *wlo = nullptr;      // centroid lo xyz, child info bits
+    float4 *wmom = nullptr;     // [nn][2] q, M2xx, M2yy, M2zz | M2xy, M2yz, M2zx, 0
+    uint32_t *inv = nullptr;    // preorder -> breadth-first index
+};
+
+// scratch shared by a handle's launches
+struct Scratch {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// ----- build.hip
+size_t tb_sort_temp_bytes(int n);
+void tb_bbox(hipStream_t s, int n, const double *x, const double *y, const double *z, double *root, double *tmp);
+void tb_keys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
+             uint64_t *key, uint32_t *val, int32_t *err);
+// sorts key[0]/val[0] by bits [lo_bit, hi_bit); result ends in key[*which]/val[*which]
+void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int lo_bit, int hi_bit, void *tmp,
+                   int *which, bool init_iota);
+void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err);
+void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist /*2*(TB_LEVELS+2)*/,
+            int32_t *err);
+void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t *total, void *tmp);
+size_t tb_scan_temp_bytes(int n);
+void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign);
+void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err);
+void tb_export_gather(hipStream_t s, const SpeciesDev &S, int32_t *depth, int32_t *leaf, int32_t *pt,
+                      int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,
+                      double *QQ);
+
+// ----- walk64.hip
+void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,
+               double *epart, int64_t *counters);
+void tb_reduce_partials(hipStream_t s, int nparts, int width, const double *part, double *out);
+int tb_walk64_blocks(int n);
+void tb_direct(hipStream_t s, int nt, const int32_t *tsp, const int32_t *tidx_sorted, const SpeciesDev &E,
+               const SpeciesDev &I, double rs, double *fout);
+void tb_vv1(hipStream_t s, SpeciesDev &S, double xm, double dt);
+void tb_vv2(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, double *part);
+void tb_kdk(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, double *part);
+int tb_vv_blocks(int n);
+void tb_inverse_perm(hipStream_t s, int n, const int32_t *orig, int32_t *inv);

@@ -1,0 +1,370 @@
+// walk64.hip -- the plain FP64 force walk (replaces LocalCoulombSum,
+// tree.f90:662-717 and Search_Tree_ee/ie/ei/ii, tree.f90:721-1133), the FP64
+// direct-sum checker kernel, and the velocity-Verlet kernels (VVerletNotemp1/2,
+// vverlet.f90:9-112).
+//
+// The walk gives one lane per target.  The recursion of Search_Tree_* becomes
+// a stackless preorder traversal with skip pointers: a node is either opened
+// (continue at node+1, its first child) or consumed (accepted twig or leaf:
+// continue at next[node]); this visits exactly the reference's DFS sequence,
+// so each target's force is summed in the reference's order.  Targets run in
+// key order, so neighbouring lanes take nearly the same path.  Each target
+// walks the electron tree then the ion tree, the order in which the
+// reference adds into ff (ee then ei for electrons, ie then ii for ions).
+// Contraction is off: FP64 results match the CPU restatement bit for bit
+// except where exp() rounds differently (Kelbg leaves).
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "tb_internal.h"
+
+namespace {
+
+constexpr int BT = 256;
+
+struct TreeView {
+    const double4 *__restrict__ mac;
+    const uint32_t *__restrict__ next;
+    const double *__restrict__ mom;
+    uint32_t nn;
+};
+
+struct Cnt {
+    unsigned long long visits, m2p, p2p;
+};
+
+// one tree, one target: Search_Tree_* with the recursion unrolled into skip pointers
+template <bool COUNT>
+__device__ inline void walk_tree(const TreeView T, double px, double py, double pz, bool like, double ffm,
+                                 double ffp, double rs, double &f0, double &f1, double &f2, double &e, Cnt &c)
+{
+    uint32_t i = 0;
+    while (i < T.nn) {
+        const double4 m = T.mac[i];
+        const uint32_t nx = T.next[i];
+        const double x = px - m.x, y = py - m.y, z = pz - m.z;
+        const double r2 = x * x + y * y + z * z;
+        if (nx & TB_LEAF) {
+            if (like) {
+                // generic Coulomb with the 0.1 um cut, tree.f90:806-815
+                if (r2 > 0.01) {
+                    if (COUNT) c.p2p++;
+                    double r = sqrt(r2);
+                    e = e + 0.5 * ffp / r;
+                    f0 = f0 + ffp * x / r / r2;
+                    f1 = f1 + ffp * y / r / r2;
+                    f2 = f2 + ffp * z / r / r2;
+                }
+            } else {
+                // simplified Kelbg, tree.f90:905-912
+                if (COUNT) c.p2p++;
+                double r = sqrt(r2);
+                double ex = exp(-r / rs);
+                e = e + 0.5 * ffp * (1.0 - ex) / r;
+                double g = (1.0 / r - ex * (1.0 / r + 1.0 / rs));
+                f0 = f0 + ffp * x * g / r2;
+                f1 = f1 + ffp * y * g / r2;
+                f2 = f2 + ffp * z * g / r2;
+            }
+            const uint32_t ni = nx & ~TB_LEAF;
+            if (ni <= i) break;  // skip pointers only advance; guards a malformed table
+            i = ni;
+            continue;
+        }
+        if (COUNT) c.visits++;
+        const double r = sqrt(r2);
+        if (m.w < r) {
+            // monopole + dipole + quadrupole M2P, tree.f90:744-778
+            if (COUNT) c.m2p++;
+            const double2 *mp = (const double2 *)(T.mom + (size_t)i * 10);
+            const double2 a0 = mp[0], a1 = mp[1], a2 = mp[2], a3 = mp[3], a4 = mp[4];
+            const double qeff = a0.x, PD0 = a0.y, PD1 = a1.x, PD2 = a1.y, PQ0 = a2.x, PQ1 = a2.y, PQ2 = a3.x,
+                         PR0 = a3.y, PR1 = a4.x, PR2 = a4.y;
+            const double R3 = r2 * r, R5 = R3 * r2, R7 = R5 * r2;
+            const double x2 = x * x, y2 = y * y, z2 = z * z;
+            const double D0 = -(1.0 / R3) + 3.0 * x2 / R5;
+            const double D1 = -(1.0 / R3) + 3.0 * y2 / R5;
+            const double D2 = -(1.0 / R3) + 3.0 * z2 / R5;
+            const double D3 = 3.0 * x * y / R5;
+            const double D4 = 3.0 * y * z / R5;
+            const double D5 = 3.0 * z * x / R5;
+            const double Q0 = 15.0 * (x2 * x) / R7 - 9.0 * x / R5;
+            const double Q1 = 15.0 * (y2 * y) / R7 - 9.0 * y / R5;
+            const double Q2 = 15.0 * (z2 * z) / R7 - 9.0 * z / R5;
+            const double Q3 = 15.0 * x2 * y / R7 - 3.0 * y / R5;
+            const double Q4 = 15.0 * x2 * z / R7 - 3.0 * z / R5;
+            const double Q5 = 15.0 * y2 * x / R7 - 3.0 * x / R5;
+            const double Q6 = 15.0 * y2 * z / R7 - 3.0 * z / R5;
+            const double Q7 = 15.0 * z2 * x / R7 - 3.0 * x / R5;
+            const double Q8 = 15.0 * z2 * y / R7 - 3.0 * y / R5;
+            const double Q9 = 15.0 * x * y * z / R7;
+            e = e + 0.5 * ffm * (qeff / r + (PD0 * x + PD1 * y + PD2 * z) / R3 + PQ0 * D0 + PQ1 * D1 + PQ2 * D2 +
+                                 PR0 * D3 + PR1 * D4 + PR2 * D5);
+            f0 = f0 + ffm * (qeff * x / R3 + D0 * PD0 + D3 * PD1 + D5 * PD2 + Q0 * PQ0 + Q5 * PQ1 + Q7 * PQ2 +
+                             Q3 * PR0 + Q4 * PR2 + Q9 * PR1);
+            f1 = f1 + ffm * (qeff * y / R3 + D3 * PD0 + D1 * PD1 + D4 * PD2 + Q3 * PQ0 + Q1 * PQ1 + Q8 * PQ2 +
+                             Q5 * PR0 + Q9 * PR2 + Q6 * PR1);
+            f2 = f2 + ffm * (qeff * z / R3 + D5 * PD0 + D4 * PD1 + D2 * PD2 + Q4 * PQ0 + Q6 * PQ1 + Q2 * PQ2 +
+                             Q9 * PR0 + Q7 * PR2 + Q8 * PR1);
+            if (nx <= i) break;
+            i = nx;
+        } else {
+            i = i + 1;
+        }
+    }
+}
+
+__device__ inline double block_sum(double v, double *lds)
+{
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = BT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(BT) void k_walk64(int n, int tsp, const double *__restrict__ px,
+                                               const double *__restrict__ py, const double *__restrict__ pz,
+                                               double *__restrict__ fx, double *__restrict__ fy,
+                                               double *__restrict__ fz, TreeView te, TreeView ti, double rs,
+                                               double *__restrict__ epart, unsigned long long *__restrict__ counters)
+{
+    __shared__ double lds[BT];
+    const int i = blockIdx.x * BT + threadIdx.x;
+    double eA = 0.0, eB = 0.0;
+    Cnt cA = {0, 0, 0}, cB = {0, 0, 0};
+    if (i < n) {
+        const double x = px[i], y = py[i], z = pz[i];
+        const double tq = tsp ? 1.0 : -1.0;
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;  // LocalCoulombSum zeroes ff, tree.f90:675-680
+        // electron tree: ee (like) for electron targets, ie (Kelbg) for ions
+        walk_tree<COUNT>(te, x, y, z, tsp == 0, tq * TB_EPS, tsp == 0 ? TB_EPS : -TB_EPS, rs, f0, f1, f2, eA, cA);
+        // ion tree: ei (Kelbg) for electron targets, ii (like) for ions
+        walk_tree<COUNT>(ti, x, y, z, tsp == 1, tq * TB_EPS, tsp == 1 ? TB_EPS : -TB_EPS, rs, f0, f1, f2, eB, cB);
+        fx[i] = f0;
+        fy[i] = f1;
+        fz[i] = f2;
+    }
+    double sA = block_sum(eA, lds);
+    double sB = block_sum(eB, lds);
+    if (threadIdx.x == 0) {
+        epart[blockIdx.x * 2] = sA;
+        epart[blockIdx.x * 2 + 1] = sB;
+    }
+    if (COUNT) {
+        unsigned long long v[6] = {cA.visits, cA.m2p, cA.p2p, cB.visits, cB.m2p, cB.p2p};
+        for (int k = 0; k < 6; k++) {
+            double s = block_sum((double)v[k], lds);
+            if (threadIdx.x == 0) atomicAdd(&counters[k], (unsigned long long)s);
+        }
+    }
+}
+
+// deterministic column sums of per-block partials
+__global__ __launch_bounds__(BT) void k_reduce_cols(int nparts, int width, const double *__restrict__ part,
+                                                    double *__restrict__ out)
+{
+    __shared__ double lds[BT];
+    for (int c = 0; c < width; c++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += BT) s += part[b * width + c];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) out[c] = s;
+    }
+}
+
+// ----------------------------------------------------------- direct sum
+
+__global__ __launch_bounds__(BT) void k_direct(const int32_t *__restrict__ tsp, const int32_t *__restrict__ tidx,
+                                               const double *__restrict__ ex, const double *__restrict__ ey,
+                                               const double *__restrict__ ez, int ne, const double *__restrict__ ix,
+                                               const double *__restrict__ iy, const double *__restrict__ iz, int ni,
+                                               double rs, double *__restrict__ fout)
+{
+    __shared__ double lds[BT];
+    const int t = blockIdx.x;
+    const int ts = tsp[t], j = tidx[t];
+    const double px = ts == 0 ? ex[j] : ix[j];
+    const double py = ts == 0 ? ey[j] : iy[j];
+    const double pz = ts == 0 ? ez[j] : iz[j];
+    const double tq = ts == 0 ? -1.0 : 1.0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int s = 0; s < 2; s++) {
+        const double *sx = s == 0 ? ex : ix, *sy = s == 0 ? ey : iy, *sz = s == 0 ? ez : iz;
+        const int ns = s == 0 ? ne : ni;
+        const double ff = tq * (s == 0 ? -1.0 : 1.0) * TB_EPS;
+        const bool like = s == ts;
+        for (int k = threadIdx.x; k < ns; k += BT) {
+            double x = px - sx[k], y = py - sy[k], z = pz - sz[k];
+            double r2 = x * x + y * y + z * z;
+            if (like) {
+                if (!(r2 > 0.01)) continue;
+                double r = sqrt(r2);
+                f0 += ff * x / r / r2;
+                f1 += ff * y / r / r2;
+                f2 += ff * z / r / r2;
+            } else {
+                double r = sqrt(r2);
+                double e = exp(-r / rs);
+                double g = (1.0 / r - e * (1.0 / r + 1.0 / rs));
+                f0 += ff * x * g / r2;
+                f1 += ff * y * g / r2;
+                f2 += ff * z * g / r2;
+            }
+        }
+    }
+    f0 = block_sum(f0, lds);
+    f1 = block_sum(f1, lds);
+    f2 = block_sum(f2, lds);
+    if (threadIdx.x == 0) {
+        fout[3 * t] = f0;
+        fout[3 * t + 1] = f1;
+        fout[3 * t + 2] = f2;
+    }
+}
+
+// ------------------------------------------------------- velocity Verlet
+
+struct StateCols {
+    double *x[3], *v[3];
+    const double *f[3];
+};
+
+// VVerletNotemp1, vverlet.f90:26-40
+__global__ __launch_bounds__(BT) void k_vv1(int n, StateCols S, double xm, double dt)
+{
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double v = S.v[k][i] + 0.5 * dt * S.f[k][i] / xm;
+        S.v[k][i] = v;
+        S.x[k][i] = S.x[k][i] + dt * v;
+    }
+}
+
+__device__ inline void sample_sums(double xm, const double *x, const double *v, double &mv2, double &rad)
+{
+    // vverlet.f90:83-86
+    mv2 = xm * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double xv = x[0] * v[0] + x[1] * v[1] + x[2] * v[2];
+    rad = xm * (xv * xv) / (x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+}
+
+// VVerletNotemp2 (kick; sampled sums), vverlet.f90:67-106; with FUSE the next
+// step's VVerletNotemp1 follows in the same pass (same force, kick + drift)
+template <bool FUSE>
+__global__ __launch_bounds__(BT) void k_vv2(int n, StateCols S, double xm, double dt, int sample,
+                                            double *__restrict__ part)
+{
+    __shared__ double lds[BT];
+    const int i = blockIdx.x * BT + threadIdx.x;
+    double mv2 = 0.0, rad = 0.0;
+    if (i < n) {
+        double x[3], v[3], f[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            x[k] = S.x[k][i];
+            f[k] = S.f[k][i];
+            v[k] = S.v[k][i] + 0.5 * dt * f[k] / xm;
+        }
+        if (sample) sample_sums(xm, x, v, mv2, rad);
+        if (FUSE) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                v[k] = v[k] + 0.5 * dt * f[k] / xm;
+                x[k] = x[k] + dt * v[k];
+                S.x[k][i] = x[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) S.v[k][i] = v[k];
+    }
+    if (sample) {
+        double a = block_sum(mv2, lds);
+        double b = block_sum(rad, lds);
+        if (threadIdx.x == 0) {
+            part[blockIdx.x * 2] = a;
+            part[blockIdx.x * 2 + 1] = b;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BT) void k_inverse(int n, const int32_t *__restrict__ orig, int32_t *__restrict__ inv)
+{
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i < n) inv[orig[i]] = i;
+}
+
+inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
+
+inline TreeView view(const SpeciesDev &S) { return TreeView{S.nmac, S.nnext, S.nmom, (uint32_t)S.nn}; }
+
+inline StateCols cols(SpeciesDev &S)
+{
+    StateCols c;
+    for (int k = 0; k < 3; k++) {
+        c.x[k] = S.x[S.cur][k];
+        c.v[k] = S.v[S.cur][k];
+        c.f[k] = S.f[S.cur][k];
+    }
+    return c;
+}
+
+}  // namespace
+
+int tb_walk64_blocks(int n) { return nblk(n, BT); }
+int tb_vv_blocks(int n) { return nblk(n, BT); }
+
+void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,
+               double *epart, int64_t *counters)
+{
+    const int c = T.cur;
+    if (counters)
+        hipLaunchKernelGGL(k_walk64<true>, dim3(nblk(T.n, BT)), dim3(BT), 0, s, T.n, tsp, T.x[c][0], T.x[c][1],
+                           T.x[c][2], T.f[c][0], T.f[c][1], T.f[c][2], view(E), view(I), rs, epart,
+                           (unsigned long long *)counters);
+    else
+        hipLaunchKernelGGL(k_walk64<false>, dim3(nblk(T.n, BT)), dim3(BT), 0, s, T.n, tsp, T.x[c][0], T.x[c][1],
+                           T.x[c][2], T.f[c][0], T.f[c][1], T.f[c][2], view(E), view(I), rs, epart,
+                           (unsigned long long *)nullptr);
+}
+
+void tb_reduce_partials(hipStream_t s, int nparts, int width, const double *part, double *out)
+{
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(BT), 0, s, nparts, width, part, out);
+}
+
+void tb_direct(hipStream_t s, int nt, const int32_t *tsp, const int32_t *tidx_sorted, const SpeciesDev &E,
+               const SpeciesDev &I, double rs, double *fout)
+{
+    const int ce = E.cur, ci = I.cur;
+    hipLaunchKernelGGL(k_direct, dim3(nt), dim3(BT), 0, s, tsp, tidx_sorted, E.x[ce][0], E.x[ce][1], E.x[ce][2],
+                       E.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], I.n, rs, fout);
+}
+
+void tb_vv1(hipStream_t s, SpeciesDev &S, double xm, double dt)
+{
+    hipLaunchKernelGGL(k_vv1, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, cols(S), xm, dt);
+}
+
+void tb_vv2(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, double *part)
+{
+    hipLaunchKernelGGL(k_vv2<false>, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, cols(S), xm, dt, sample, part);
+}
+
+void tb_kdk(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, double *part)
+{
+    hipLaunchKernelGGL(k_vv2<true>, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, cols(S), xm, dt, sample, part);
+}
+
+void tb_inverse_perm(hipStream_t s, int n, const int32_t *orig, int32_t *inv)
+{
+    hipLaunchKernelGGL(k_inverse, dim3(nblk(n, BT)), dim3(BT), 0, s, n, orig, inv);
+}
