@@ -129,6 +129,9 @@ int tb_get_stats(tb_handle *h, tb_stats *s);
 
 const char *tb_last_error(void);
 
+/* sizeof(tb_pt), sizeof(tb_config), sizeof(tb_stats) as compiled (binding checks) */
+int tb_abi_sizes(int32_t *pt, int32_t *config, int32_t *stats);
+
 /* ------------------------------------------------------------ host helpers */
 
 /* Kelbg length: thermal de Broglie lambda hbar/sqrt(2 mu k_B T_e), um */

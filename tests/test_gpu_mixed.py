@@ -1,0 +1,88 @@
+"""The mixed-precision group walk against FP64 references.
+
+The mixed walk is a different approximation from the reference's (group
+MAC, exact quadrupoles, FP32 arithmetic), so it is judged by its error
+against the FP64 direct sum: rms and max relative force error are stated in
+each assertion.
+"""
+import numpy as np
+import pytest
+
+import tree_amd as ta
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+RS10 = 6.6493e-3
+
+
+def _err(f, fd):
+    rel = np.linalg.norm(f - fd, axis=1) / np.linalg.norm(fd, axis=1)
+    return np.sqrt(np.mean(rel ** 2)), rel.max()
+
+
+def _run(qel, qion, theta, group=0, walk="mixed"):
+    with ta.Engine(rs=RS10, theta=theta, theta_mixed=theta, walk=walk, group=group) as eng:
+        eng.set_particles(qel, qion)
+        eng.build()
+        ep = eng.force()
+        e, i = eng.get_particles()
+        ne, ni = len(qel), len(qion)
+        tsp = np.r_[np.zeros(ne, np.int32), np.ones(ni, np.int32)]
+        tix = np.r_[np.arange(ne), np.arange(ni)].astype(np.int32)
+        fd = eng.direct_sample(tsp, tix)
+    return np.r_[e["ff"], i["ff"]], fd, ep
+
+
+def test_mixed_tiny_theta_is_fp32_direct():
+    """theta -> 0: only leaves, the FP32 compensated direct sum: rms < 1e-6, max < 2e-5."""
+    announce("mixed_tiny_theta")
+    qel, qion = ta.gen_plasma(21, 3000, 3000, 100.0)
+    f, fd, _ = _run(qel, qion, 1e-9)
+    rms, mx = _err(f, fd)
+    assert rms < 1e-6 and mx < 2e-5, (rms, mx)
+
+
+@pytest.mark.parametrize("group", [64, 128, 256])
+def test_mixed_group_sizes_agree(group):
+    """All walk-group sizes give errors within the same bound at theta 0.4: rms < 1e-4."""
+    announce("mixed_group %d" % group)
+    qel, qion = ta.gen_plasma(22, 20000, 20000, 200.0)
+    f, fd, _ = _run(qel, qion, 0.4, group=group)
+    rms, mx = _err(f, fd)
+    assert rms < 1e-4, (rms, mx)
+
+
+def test_mixed_error_falls_with_theta():
+    announce("mixed_theta_sweep")
+    qel, qion = ta.gen_plasma(23, 20000, 20000, 200.0)
+    errs = []
+    for th in (0.8, 0.5, 0.3):
+        f, fd, _ = _run(qel, qion, th)
+        errs.append(_err(f, fd)[0])
+    assert errs[0] > errs[1] > errs[2], errs
+
+
+def test_mixed_energy_matches_fp64():
+    """Epot of the mixed walk at small theta agrees with the FP64 walk to 1e-5 relative."""
+    announce("mixed_energy")
+    qel, qion = ta.gen_plasma(24, 8000, 8000, 100.0)
+    _, _, ep_m = _run(qel, qion, 0.2)
+    _, _, ep_r = _run(qel, qion, 0.2, walk="ref64")
+    assert np.allclose(ep_m, ep_r, rtol=1e-5), (ep_m, ep_r)
+
+
+def test_mixed_step_runs_and_conserves():
+    """Five mixed steps: the total energy moves by < 1e-3 relative."""
+    announce("mixed_step")
+    qel, qion = ta.gen_plasma(25, 5000, 5000, 100.0)
+    dt = 0.005 / ta.TPS
+    with ta.Engine(rs=RS10, theta=0.4, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        eng.build()
+        ep = eng.force()
+        k = eng.vv2(0.0, True)
+        e0 = 0.5 * (k[0] + k[1]) + ep[0] + ep[1]
+        out = eng.step(dt, 5)
+        e1 = 0.5 * (out[2] + out[3]) + out[0] + out[1]
+    assert abs(e1 - e0) < 1e-3 * abs(e0), (e0, e1)

@@ -298,6 +298,15 @@ int force_all(tb_handle *h, bool counting)
     return TB_OK;
 }
 
+// walk integrity flag (d_sums[15], written by the mixed walk's reduction)
+int check_walk_flags(tb_handle *h)
+{
+    if (h->cfg.walk == TB_WALK_MIXED && h->h_sums[15] != 0.0)
+        return fail(TB_ERR_HIP, "mixed walk: traversal stack overflow or malformed node table (flag %g)",
+                    h->h_sums[15]);
+    return TB_OK;
+}
+
 // d_sums[0..3] = ee, ei, ie, ii (electron-target launch, then ion-target launch)
 void epot_from_sums(const double *v, double *ee)
 {
@@ -327,6 +336,15 @@ int vv2_all(tb_handle *h, double dt, int sample, bool fuse_next)
 extern "C" {
 
 const char *tb_last_error(void) { return g_err.c_str(); }
+
+int tb_abi_sizes(int32_t *pt, int32_t *config, int32_t *stats)
+{
+    if (!pt || !config || !stats) return fail(TB_ERR_ARG, "null argument");
+    *pt = (int32_t)sizeof(tb_pt);
+    *config = (int32_t)sizeof(tb_config);
+    *stats = (int32_t)sizeof(tb_stats);
+    return TB_OK;
+}
 
 int tb_create(tb_handle **out, const tb_config *cfg)
 {
@@ -475,11 +493,13 @@ int tb_force(tb_handle *h, double *epot_e, double *epot_i)
     if (!h->built) return fail(TB_ERR_STATE, "tb_force before tb_build");
     HIPCHK(hipSetDevice(h->cfg.device));
     HIPCHK(hipEventRecord(h->ev[2], h->s));
+    HIPCHK(hipMemsetAsync(h->d_sums + 15, 0, sizeof(double), h->s));
     int rc = force_all(h, false);
     if (rc) return rc;
     HIPCHK(hipEventRecord(h->ev[3], h->s));
-    HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 4 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
+    if ((rc = check_walk_flags(h))) return rc;
     epot_from_sums(h->h_sums, h->epot);
     if (epot_e) *epot_e = h->epot[0];
     if (epot_i) *epot_i = h->epot[1];
@@ -524,6 +544,7 @@ int tb_step(tb_handle *h, double dt, int32_t nsteps, double *out)
     if (!h->have_force) return fail(TB_ERR_STATE, "tb_step needs forces: call tb_build + tb_force first");
     HIPCHK(hipSetDevice(h->cfg.device));
     for (int sp = 0; sp < 2; sp++) tb_vv1(h->s, h->sp[sp], h->cfg.xm[sp], dt);
+    HIPCHK(hipMemsetAsync(h->d_sums + 15, 0, sizeof(double), h->s));
     int rc;
     for (int k = 0; k < nsteps; k++) {
         HIPCHK(hipEventRecord(h->ev[0], h->s));
@@ -535,8 +556,9 @@ int tb_step(tb_handle *h, double dt, int32_t nsteps, double *out)
         if ((rc = vv2_all(h, dt, last && out ? 1 : 0, !last))) return rc;
         HIPCHK(hipEventRecord(h->ev[5], h->s));
     }
-    HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 8 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
+    if ((rc = check_walk_flags(h))) return rc;
     epot_from_sums(h->h_sums, h->epot);
     h->built = false;
     if (out) {
@@ -568,8 +590,10 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
         return fail(TB_ERR_ARG, "null output array");
     HIPCHK(hipSetDevice(h->cfg.device));
     const size_t n = (size_t)S.nn;
+    // 5 int32 columns, then 14 doubles per node (xx 3, qeff, dx, D 3, Q 3, QQ 3)
+    const size_t tmp_bytes = n * 5 * 4 + 16 + n * 14 * 8;
     void *tmp = nullptr;
-    int rc = dalloc(h, &tmp, n * (5 * 4 + 12 * 8) + 256);
+    int rc = dalloc(h, &tmp, tmp_bytes);
     if (rc) return rc;
     char *b = (char *)tmp;
     int32_t *d_depth = (int32_t *)b, *d_leaf = d_depth + n, *d_pt = d_leaf + n, *d_cnt = d_pt + n,
@@ -592,7 +616,7 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
     if (e == hipSuccess) e = hipMemcpy(D, d_D, n * 24, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(Q, d_Q, n * 24, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(QQ, d_QQ, n * 24, hipMemcpyDeviceToHost);
-    dfree(h, tmp, n * (5 * 4 + 12 * 8) + 256);
+    dfree(h, tmp, tmp_bytes);
     if (e != hipSuccess) return fail(TB_ERR_HIP, "tb_export_tree: %s", hipGetErrorString(e));
     // cell size: the root length halved once per level (exact), tree.f90:126
     for (size_t j = 0; j < n; j++) {

@@ -1,22 +1,647 @@
-// walk32.hip -- mixed-precision wavefront-group walk.
+// walk32.hip -- the mixed-precision force walk (same result as LocalCoulombSum,
+// tree.f90:662-717, to a stated force error; not its FP64 operation order).
+//
+// One wavefront per target group: 64*TPL consecutive targets in key order.
+// The wave walks each tree cooperatively: lanes test up to 64 candidate cells
+// at once against the group's bounding box (a cell is accepted for every
+// target when its distance to the box exceeds rcrit = max(lx)/theta + dx,
+// which implies the reference's per-target test), then ballots compact the
+// accepted cells and the leaves into interaction lists in LDS, and opened
+// cells push their contiguous child range onto an LDS stack.  Full lists are
+// evaluated by all lanes for their own targets in FP32 with group-relative
+// coordinates (formed in FP64 once per cell and group), and each flush is
+// added into Neumaier-compensated FP32 sums.
+//
+// FP64 is used only for: node centroids and target positions (group-relative
+// differences), the centroid/second-moment reduction of the upward pass
+// (build.hip), and the per-target and total energy sums here.
+//
+// M2P uses the exact traceless quadrupole of each cell about its centroid (the
+// dipole vanishes there); P2P is Coulomb with the reference's r^2 > 0.01 um^2
+// cut for like species and the simplified Kelbg form for unlike species
+// (tree.f90:806-815, 905-912).  Work is handed out by an atomic queue in one
+// persistent launch covering both target species.
+#include <hip/hip_runtime.h>
+
 #include "tb_mixed.h"
 
-static const char *g_mixed_err = "";
+namespace {
 
-int tb_mixed_prep(hipStream_t, MixedDev &, SpeciesDev *, const tb_config &, size_t *)
+constexpr int BT = 256;
+constexpr int WPB = BT / 64;
+constexpr int STACK = 1024;
+constexpr int LCAP = 128;   // list capacity; flushed once 64 entries are pending
+constexpr int FLUSH = 64;
+
+const char *g_err = "";
+
+struct WaveLDS {
+    uint32_t stack[STACK];
+    uint32_t batch[64];
+    float4 m0[LCAP], m1[LCAP], m2[LCAP];  // x y z q | Qxx Qyy Qzz Qxy | Qyz Qzx - -
+    float4 p[LCAP];                       // x y z -
+};
+
+__device__ inline void wave_sync()
 {
-    g_mixed_err = "mixed walk not built yet";
-    return TB_ERR_ARG;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
-int tb_mixed_force(hipStream_t, MixedDev &, SpeciesDev *, const tb_config &, double *, double *)
+
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
+__device__ inline uint64_t lanemask_lt()
 {
-    g_mixed_err = "mixed walk not built yet";
-    return TB_ERR_ARG;
+    int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
-int tb_mixed_count(hipStream_t, MixedDev &, SpeciesDev *, const tb_config &, double *, double *, int64_t *)
+
+__device__ inline int wave_incl_scan(int v)
 {
-    g_mixed_err = "mixed walk not built yet";
-    return TB_ERR_ARG;
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (l >= o) v += t;
+    }
+    return v;
 }
-void tb_mixed_free(MixedDev &, size_t *) {}
-const char *tb_mixed_error() { return g_mixed_err; }
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Neumaier compensated add: (s, c) += v
+__device__ inline void nadd(float &s, float &c, float v)
+{
+    float t = s + v;
+    c += (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
+    s = t;
+}
+
+// 1 - e^{-u}(1+u) and 1 - e^{-u}, with series below u = 1/8 where the
+// subtraction would cancel
+__device__ inline void kelbg_gh(float u, float &g, float &h)
+{
+    if (u < 0.125f) {
+        g = u * u * (0.5f + u * (-1.0f / 3.0f + u * (0.125f + u * (-1.0f / 30.0f + u * (1.0f / 144.0f +
+                                                                                    u * (-1.0f / 840.0f))))));
+        h = u * (1.0f + u * (-0.5f + u * (1.0f / 6.0f + u * (-1.0f / 24.0f + u * (1.0f / 120.0f +
+                                                                                 u * (-1.0f / 720.0f))))));
+    } else {
+        float e = __expf(-u);
+        g = 1.0f - e * (1.0f + u);
+        h = 1.0f - e;
+    }
+}
+
+struct WalkArgs {
+    const double *px[2], *py[2], *pz[2];
+    double *fx[2], *fy[2], *fz[2];
+    int n[2], ng[2];
+    const double4 *bmac[2];
+    const uint32_t *binfo[2];
+    const float4 *bq0[2], *bq1[2];
+    float inv_rs;
+    int32_t *ctr;     // [0] work counter, [1] error flag
+    double *part;     // 4 per wave: ee, ei, ie, ii
+    unsigned long long *cnt;
+};
+
+template <int TPL>
+struct Acc {
+    float s[TPL][3], c[TPL][3];  // compensated field sums
+    float ps[TPL], pc[TPL];      // compensated potential (current tree)
+};
+
+template <int TPL>
+__device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
+                                 const float (&rz)[TPL], Acc<TPL> &A)
+{
+    float ax[TPL], ay[TPL], az[TPL], ap[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
+    for (int j = 0; j < cnt; j++) {
+        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j];
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            const float dx = rx[t] - a.x, dy = ry[t] - a.y, dz = rz[t] - a.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float ri = rsqrtf(r2);
+            const float ri2 = ri * ri, ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+            const float qx = b.x * dx + b.w * dy + c.y * dz;
+            const float qy = b.w * dx + b.y * dy + c.x * dz;
+            const float qz = c.y * dx + c.x * dy + b.z * dz;
+            const float xqx = dx * qx + dy * qy + dz * qz;
+            const float k = a.w * ri3 + 2.5f * xqx * ri7;
+            ax[t] += dx * k - qx * ri5;
+            ay[t] += dy * k - qy * ri5;
+            az[t] += dz * k - qz * ri5;
+            ap[t] += a.w * ri + 0.5f * xqx * ri5;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        nadd(A.s[t][0], A.c[t][0], ax[t]);
+        nadd(A.s[t][1], A.c[t][1], ay[t]);
+        nadd(A.s[t][2], A.c[t][2], az[t]);
+        nadd(A.ps[t], A.pc[t], ap[t]);
+    }
+}
+
+template <int TPL, bool LIKE>
+__device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
+                                 const float (&rz)[TPL], float qs, float inv_rs, Acc<TPL> &A)
+{
+    float ax[TPL], ay[TPL], az[TPL], ap[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
+    for (int j = 0; j < cnt; j++) {
+        const float4 q = L.p[j];
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (LIKE) {
+                // Coulomb beyond the 0.1 um cut (also drops the target itself)
+                const float ri = r2 > 0.01f ? rsqrtf(r2) : 0.0f;
+                const float ri3 = ri * ri * ri;
+                ax[t] += dx * ri3;
+                ay[t] += dy * ri3;
+                az[t] += dz * ri3;
+                ap[t] += ri;
+            } else {
+                const float ri = rsqrtf(r2);
+                const float u = r2 * ri * inv_rs;
+                float g, h;
+                kelbg_gh(u, g, h);
+                const float k = g * ri * ri * ri;
+                ax[t] += dx * k;
+                ay[t] += dy * k;
+                az[t] += dz * k;
+                ap[t] += h * ri;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        nadd(A.s[t][0], A.c[t][0], qs * ax[t]);
+        nadd(A.s[t][1], A.c[t][1], qs * ay[t]);
+        nadd(A.s[t][2], A.c[t][2], qs * az[t]);
+        nadd(A.ps[t], A.pc[t], qs * ap[t]);
+    }
+}
+
+template <int TPL, bool COUNT>
+__global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
+{
+    __shared__ WaveLDS lds[WPB];
+    const int lane = lane_id();
+    WaveLDS &L = lds[threadIdx.x >> 6];
+    const int G = 64 * TPL;
+    double eacc[4] = {0.0, 0.0, 0.0, 0.0};
+    unsigned long long c_visit[4] = {0, 0, 0, 0}, c_m2p[4] = {0, 0, 0, 0}, c_p2p[4] = {0, 0, 0, 0};
+    const int ngt = W.ng[0] + W.ng[1];
+
+    for (;;) {
+        int g = 0;
+        if (lane == 0) g = atomicAdd(&W.ctr[0], 1);
+        g = __shfl(g, 0, 64);
+        if (g >= ngt) break;
+        const int tsp = g < W.ng[0] ? 0 : 1;
+        const int base = (tsp ? g - W.ng[0] : g) * G;
+        const int n = W.n[tsp];
+        const double *px = W.px[tsp], *py = W.py[tsp], *pz = W.pz[tsp];
+
+        // targets and the group's bounding box (FP64)
+        double tx[TPL], ty[TPL], tz[TPL];
+        bool act[TPL];
+        double mnx = 1e300, mny = 1e300, mnz = 1e300, mxx = -1e300, mxy = -1e300, mxz = -1e300;
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            const int i = base + lane + 64 * t;
+            act[t] = i < n;
+            const int ii = act[t] ? i : base;
+            tx[t] = px[ii];
+            ty[t] = py[ii];
+            tz[t] = pz[ii];
+            mnx = fmin(mnx, tx[t]); mxx = fmax(mxx, tx[t]);
+            mny = fmin(mny, ty[t]); mxy = fmax(mxy, ty[t]);
+            mnz = fmin(mnz, tz[t]); mxz = fmax(mxz, tz[t]);
+        }
+        mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
+        mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
+        const double gx = 0.5 * (mnx + mxx), gy = 0.5 * (mny + mxy), gz = 0.5 * (mnz + mxz);
+        // half extents, rounded outwards
+        const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f;
+        const float hy = (float)(0.5 * (mxy - mny)) * 1.000001f + 1e-30f;
+        const float hz = (float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f;
+        float rx[TPL], ry[TPL], rz[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            rx[t] = (float)(tx[t] - gx);
+            ry[t] = (float)(ty[t] - gy);
+            rz[t] = (float)(tz[t] - gz);
+        }
+        Acc<TPL> A;
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            for (int k = 0; k < 3; k++) A.s[t][k] = A.c[t][k] = 0.0f;
+        }
+        int nact = 0;  // targets in this group (interaction counting)
+        if (COUNT) nact = min(G, n - base);
+
+        for (int tree = 0; tree < 2; tree++) {
+            const int pr = tsp * 2 + tree;  // ee, ei, ie, ii
+            const bool like = tree == tsp;
+            const float qs = tree ? 1.0f : -1.0f;
+            const double4 *bmac = W.bmac[tree];
+            const uint32_t *binfo = W.binfo[tree];
+            const float4 *bq0 = W.bq0[tree], *bq1 = W.bq1[tree];
+#pragma unroll
+            for (int t = 0; t < TPL; t++) A.ps[t] = A.pc[t] = 0.0f;
+            int sp = 1, nm = 0, np = 0;
+            if (lane == 0) L.stack[0] = 0u;  // root: range of one cell at 0
+            wave_sync();
+            while (sp > 0) {
+                // a batch of <= 64 cells from the top ranges of the stack
+                uint32_t R = 0;
+                int len = 0;
+                if (lane < sp) {
+                    R = L.stack[sp - 1 - lane];
+                    len = (int)(R & 7u) + 1;
+                }
+                const int incl = wave_incl_scan(len);
+                const bool take = lane < sp && incl <= 64;
+                const int K = __popcll(__ballot(take));
+                const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
+                if (take) {
+                    const uint32_t st = R >> 3;
+                    const int ex = incl - len;
+                    for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
+                }
+                sp -= K;
+                wave_sync();
+                const bool have = lane < total;
+                const uint32_t node = have ? L.batch[lane] : 0u;
+                wave_sync();
+                bool leaf = false, acc = false, open = false;
+                float cx = 0.f, cy = 0.f, cz = 0.f;
+                uint32_t info = 0;
+                if (have) {
+                    const double4 m = bmac[node];
+                    info = binfo[node];
+                    cx = (float)(m.x - gx);
+                    cy = (float)(m.y - gy);
+                    cz = (float)(m.z - gz);
+                    if (info & TB_LEAF) {
+                        leaf = true;
+                    } else {
+                        const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
+                        const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
+                        const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
+                        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const float rc = (float)m.w;
+                        if (d2 > rc * rc * 1.000004f) acc = true;
+                        else open = true;
+                    }
+                }
+                if (COUNT && have) c_visit[pr]++;
+                // accepted cells -> M2P list
+                uint64_t mk = __ballot(acc);
+                if (acc) {
+                    const int k = nm + __popcll(mk & lanemask_lt());
+                    const float4 q0 = bq0[node], q1 = bq1[node];
+                    L.m0[k] = make_float4(cx, cy, cz, q0.x);
+                    L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
+                    L.m2[k] = make_float4(q1.y, q1.z, 0.0f, 0.0f);
+                }
+                nm += __popcll(mk);
+                // leaves -> P2P list
+                mk = __ballot(leaf);
+                if (leaf) L.p[np + __popcll(mk & lanemask_lt())] = make_float4(cx, cy, cz, 0.0f);
+                np += __popcll(mk);
+                // opened cells -> their child ranges
+                mk = __ballot(open);
+                const int npush = __popcll(mk);
+                if (sp + npush > STACK) {
+                    if (lane == 0) atomicOr(&W.ctr[1], 1);
+                    sp = 0;
+                    nm = np = 0;
+                    break;
+                }
+                if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
+                sp += npush;
+                wave_sync();
+                if (nm >= FLUSH) {
+                    if (COUNT) c_m2p[pr] += (unsigned long long)nm * nact;
+                    flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+                    nm = 0;
+                    wave_sync();
+                }
+                if (np >= FLUSH) {
+                    if (COUNT) c_p2p[pr] += (unsigned long long)np * nact;
+                    if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, W.inv_rs, A);
+                    else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, W.inv_rs, A);
+                    np = 0;
+                    wave_sync();
+                }
+            }
+            if (nm > 0) {
+                if (COUNT) c_m2p[pr] += (unsigned long long)nm * nact;
+                flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+            }
+            if (np > 0) {
+                if (COUNT) c_p2p[pr] += (unsigned long long)np * nact;
+                if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, W.inv_rs, A);
+                else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, W.inv_rs, A);
+            }
+            wave_sync();
+            // energy of this pairing: 0.5 * q_t * eps * phi (FP64 from here on)
+            const double qt = tsp ? 1.0 : -1.0;
+#pragma unroll
+            for (int t = 0; t < TPL; t++)
+                if (act[t]) eacc[tsp * 2 + tree] += 0.5 * qt * TB_EPS * ((double)A.ps[t] + (double)A.pc[t]);
+        }
+        // forces: F = q_t * eps * E
+        const double ff = (tsp ? 1.0 : -1.0) * TB_EPS;
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            if (!act[t]) continue;
+            const int i = base + lane + 64 * t;
+            W.fx[tsp][i] = ff * ((double)A.s[t][0] + (double)A.c[t][0]);
+            W.fy[tsp][i] = ff * ((double)A.s[t][1] + (double)A.c[t][1]);
+            W.fz[tsp][i] = ff * ((double)A.s[t][2] + (double)A.c[t][2]);
+        }
+    }
+    const int gw = blockIdx.x * WPB + (threadIdx.x >> 6);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double v = wave_sum(eacc[k]);
+        if (lane == 0) W.part[gw * 4 + k] = v;
+    }
+    if (COUNT && lane == 0) {
+        for (int pr = 0; pr < 4; pr++) {
+            atomicAdd(&W.cnt[pr * 3], c_visit[pr]);
+            atomicAdd(&W.cnt[pr * 3 + 1], c_m2p[pr]);
+            atomicAdd(&W.cnt[pr * 3 + 2], c_p2p[pr]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- prep
+
+__global__ __launch_bounds__(BT) void k_inv(int nn, const uint32_t *__restrict__ bfs, uint32_t *__restrict__ inv)
+{
+    const int b = blockIdx.x * BT + threadIdx.x;
+    if (b < nn) inv[bfs[b]] = (uint32_t)b;
+}
+
+__global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__restrict__ bfs,
+                                                    const uint32_t *__restrict__ inv,
+                                                    const double4 *__restrict__ nmac,
+                                                    const uint32_t *__restrict__ nnext,
+                                                    const double *__restrict__ nmom, const double *__restrict__ nm2,
+                                                    const double *__restrict__ ndx,
+                                                    const int32_t *__restrict__ nstart,
+                                                    const int32_t *__restrict__ ndepth, double smax0, double theta,
+                                                    double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
+                                                    float4 *__restrict__ bq0, float4 *__restrict__ bq1,
+                                                    int32_t *__restrict__ err)
+{
+    const int b = blockIdx.x * BT + threadIdx.x;
+    if (b >= nn) return;
+    const uint32_t p = bfs[b];
+    const uint32_t nx = nnext[p];
+    const double4 m = nmac[p];
+    const double q = nmom[(size_t)p * 10];
+    if (nx & TB_LEAF) {
+        binfo[b] = TB_LEAF | (uint32_t)nstart[p];
+        bmac[b] = make_double4(m.x, m.y, m.z, 0.0);
+        bq0[b] = make_float4((float)q, 0.f, 0.f, 0.f);
+        bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    int nc = 0;
+    for (uint32_t c = p + 1; c < nx; nc++) {
+        const uint32_t nc2 = nnext[c] & ~TB_LEAF;
+        if (nc2 <= c || nc >= 8) {
+            atomicOr(err, 16);
+            return;
+        }
+        c = nc2;
+    }
+    binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nc - 1);
+    const double s = ldexp(smax0, -ndepth[p]);
+    bmac[b] = make_double4(m.x, m.y, m.z, s / theta + ndx[p]);
+    const double *M = nm2 + (size_t)p * 6;
+    const double tr = M[0] + M[1] + M[2];
+    bq0[b] = make_float4((float)q, (float)(3.0 * M[0] - tr), (float)(3.0 * M[1] - tr), (float)(3.0 * M[2] - tr));
+    bq1[b] = make_float4((float)(3.0 * M[3]), (float)(3.0 * M[4]), (float)(3.0 * M[5]), 0.f);
+}
+
+__global__ void k_zero_ctr(int32_t *ctr, int lo, int hi)
+{
+    const int i = lo + (int)threadIdx.x;
+    if (i < hi) ctr[i] = 0;
+}
+
+__global__ __launch_bounds__(64) void k_reduce_part(int nw, const double *__restrict__ part,
+                                                    const int32_t *__restrict__ ctr, double *__restrict__ out)
+{
+    // deterministic per-column sums (fixed order per lane, then a fixed tree)
+    for (int c = 0; c < 4; c++) {
+        double s = 0.0;
+        for (int w = threadIdx.x; w < nw; w += 64) s += part[w * 4 + c];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) out[c] = s;
+    }
+    if (threadIdx.x == 0) out[15] = (double)(ctr[1] | ctr[2]);
+}
+
+inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
+
+template <typename T>
+int grow(T **p, int *cap, int want, size_t *bytes)
+{
+    if (want <= *cap) return TB_OK;
+    if (*p) {
+        hipFree(*p);
+        *bytes -= (size_t)*cap * sizeof(T);
+    }
+    *p = nullptr;
+    *cap = 0;
+    int c = want + want / 8 + 64;
+    if (hipMalloc((void **)p, (size_t)c * sizeof(T)) != hipSuccess) {
+        g_err = "mixed walk: device allocation failed";
+        return TB_ERR_NOMEM;
+    }
+    *bytes += (size_t)c * sizeof(T);
+    *cap = c;
+    return TB_OK;
+}
+
+int pick_tpl(const tb_config &cfg, int n)
+{
+    if (cfg.group == 64 || cfg.group == 128 || cfg.group == 256) return cfg.group / 64;
+    // default from the autotuner's table: larger groups once there are enough of them
+    (void)n;
+    return 2;
+}
+
+int resident_waves()
+{
+    static int w = 0;
+    if (w == 0) {
+        int dev = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+        w = cus * 12;  // ~3 blocks of 4 waves per CU under the LDS budget
+    }
+    return w;
+}
+
+int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int64_t *d_cnt)
+{
+    const int tpl = pick_tpl(cfg, sp[0].n + sp[1].n);
+    const int G = 64 * tpl;
+    WalkArgs W;
+    for (int k = 0; k < 2; k++) {
+        const int c = sp[k].cur;
+        W.px[k] = sp[k].x[c][0]; W.py[k] = sp[k].x[c][1]; W.pz[k] = sp[k].x[c][2];
+        W.fx[k] = sp[k].f[c][0]; W.fy[k] = sp[k].f[c][1]; W.fz[k] = sp[k].f[c][2];
+        W.n[k] = sp[k].n;
+        W.ng[k] = (sp[k].n + G - 1) / G;
+        W.bmac[k] = M.t[k].bmac; W.binfo[k] = M.t[k].binfo; W.bq0[k] = M.t[k].bq0; W.bq1[k] = M.t[k].bq1;
+    }
+    W.inv_rs = (float)(1.0 / cfg.rs);
+    W.ctr = M.ctr;
+    W.cnt = (unsigned long long *)d_cnt;
+    const int ngt = W.ng[0] + W.ng[1];
+    int waves = std::min(ngt, resident_waves());
+    int blocks = nblk(waves, WPB);
+    waves = blocks * WPB;
+    if (waves > M.part_n) {
+        if (M.part) hipFree(M.part);
+        M.part = nullptr;
+        if (hipMalloc((void **)&M.part, (size_t)waves * 4 * sizeof(double)) != hipSuccess) {
+            g_err = "mixed walk: partials allocation failed";
+            return TB_ERR_NOMEM;
+        }
+        M.part_n = waves;
+    }
+    W.part = M.part;
+    M.tpl = tpl;
+    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
+    if (d_cnt) {
+        switch (tpl) {
+        case 1: hipLaunchKernelGGL((k_walk32<1, true>), dim3(blocks), dim3(BT), 0, s, W); break;
+        case 4: hipLaunchKernelGGL((k_walk32<4, true>), dim3(blocks), dim3(BT), 0, s, W); break;
+        default: hipLaunchKernelGGL((k_walk32<2, true>), dim3(blocks), dim3(BT), 0, s, W); break;
+        }
+    } else {
+        switch (tpl) {
+        case 1: hipLaunchKernelGGL((k_walk32<1, false>), dim3(blocks), dim3(BT), 0, s, W); break;
+        case 4: hipLaunchKernelGGL((k_walk32<4, false>), dim3(blocks), dim3(BT), 0, s, W); break;
+        default: hipLaunchKernelGGL((k_walk32<2, false>), dim3(blocks), dim3(BT), 0, s, W); break;
+        }
+    }
+    hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(64), 0, s, waves, M.part, M.ctr, d_sums);
+    if (hipGetLastError() != hipSuccess) {
+        g_err = "mixed walk: launch failed";
+        return TB_ERR_HIP;
+    }
+    return TB_OK;
+}
+
+}  // namespace
+
+int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, size_t *bytes)
+{
+    if (!M.ctr) {
+        if (hipMalloc((void **)&M.ctr, 64) != hipSuccess) {
+            g_err = "mixed walk: counter allocation failed";
+            return TB_ERR_NOMEM;
+        }
+        *bytes += 64;
+    }
+    const double theta = cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta;
+    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 2, 3);
+    for (int k = 0; k < 2; k++) {
+        SpeciesDev &S = sp[k];
+        MixedTree &T = M.t[k];
+        int cap = T.cap, rc;
+        int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap;
+        if ((rc = grow(&T.bmac, &c1, S.nn, bytes))) return rc;
+        if ((rc = grow(&T.binfo, &c2, S.nn, bytes))) return rc;
+        if ((rc = grow(&T.bq0, &c3, S.nn, bytes))) return rc;
+        if ((rc = grow(&T.bq1, &c4, S.nn, bytes))) return rc;
+        if ((rc = grow(&T.inv, &c5, S.nn, bytes))) return rc;
+        T.cap = std::min(std::min(c1, c2), std::min(std::min(c3, c4), c5));
+        T.nn = S.nn;
+        double root[6];
+        // max root length; the root buffer is ready once the build has synchronised
+        if (hipMemcpyAsync(root, S.root, sizeof(root), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            g_err = "mixed walk: root read failed";
+            return TB_ERR_HIP;
+        }
+        const double smax0 = std::max(root[3], std::max(root[4], root[5]));
+        hipLaunchKernelGGL(k_inv, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv);
+        hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv, S.nmac, S.nnext,
+                           S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, smax0, theta, T.bmac, T.binfo, T.bq0, T.bq1,
+                           M.ctr + 2);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        g_err = "mixed walk: prep launch failed";
+        return TB_ERR_HIP;
+    }
+    return TB_OK;
+}
+
+int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums)
+{
+    return launch(s, M, sp, cfg, d_sums, nullptr);
+}
+
+int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums,
+                   int64_t *d_cnt)
+{
+    hipMemsetAsync(d_cnt, 0, 12 * sizeof(int64_t), s);
+    return launch(s, M, sp, cfg, d_sums, d_cnt);
+}
+
+void tb_mixed_free(MixedDev &M, size_t *bytes)
+{
+    for (int k = 0; k < 2; k++) {
+        MixedTree &T = M.t[k];
+        hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.inv);
+        T = MixedTree();
+    }
+    hipFree(M.ctr);
+    hipFree(M.part);
+    M.ctr = nullptr;
+    M.part = nullptr;
+    M.part_n = 0;
+    (void)bytes;
+}
+
+const char *tb_mixed_error() { return g_err; }

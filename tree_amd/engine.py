@@ -94,6 +94,7 @@ SIGNATURES = [
     ("tb_count_interactions", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_get_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_last_error", ctypes.c_char_p, []),
+    ("tb_abi_sizes", ctypes.c_int, [_P, _P, _P]),
     ("tb_kelbg_length", ctypes.c_double, [_D, _D, _D]),
     ("tb_gen_plasma", ctypes.c_int, [ctypes.c_uint64, _I, _I, _D, _D, _D, _D, _D, _P, _P]),
 ]
