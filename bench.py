@@ -29,6 +29,10 @@ import tree_amd as ta  # noqa: E402
 
 # algorithmic flop weights per event (FMA = 2): MAC test, M2P on accept, P2P Coulomb / Kelbg
 W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
+# mixed walk, per target-entry interaction, counted from walk32.hip's FP32 code
+# (traceless-quadrupole M2P 54, Coulomb P2P 18, Kelbg P2P 30); MAC tests are
+# per cell and group and are not counted
+W32_M2P, W32_P2P_C, W32_P2P_K = 54, 18, 30
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X vector
 
 
@@ -76,6 +80,14 @@ def walk_flops(st, like_pairs=(0, 3)):
     for p in range(4):
         wp = W_P2P_C if p in like_pairs else W_P2P_K
         fl += W_VISIT * st["visits"][p] + W_M2P * st["m2p"][p] + wp * st["p2p"][p]
+    return fl
+
+
+def walk32_flops(st, like_pairs=(0, 3)):
+    fl = 0
+    for p in range(4):
+        wp = W32_P2P_C if p in like_pairs else W32_P2P_K
+        fl += W32_M2P * st["m2p"][p] + wp * st["p2p"][p]
     return fl
 
 
@@ -130,6 +142,9 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--theta64", type=float, default=0.0, help="fixed theta for the FP64 walk (skip tuning)")
+    ap.add_argument("--theta32", type=float, default=0.0, help="fixed theta for the mixed walk (skip tuning)")
+    ap.add_argument("--group", type=int, default=0, help="mixed-walk targets per group (0 = autotuned)")
+    ap.add_argument("--only", choices=["all", "mixed", "ref64"], default="all")
     ap.add_argument("--quiet", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -147,24 +162,44 @@ def main():
     eng = ta.Engine(rs=rs, theta=0.5, walk="ref64")
     eng.set_particles(qel, qion)
     fdir = eng.direct_sample(tsp, tix)
-    if a.theta64 > 0:
-        eng.configure(theta=a.theta64)
-        th64 = a.theta64
-        rms64, mx64 = force_error(eng, tsp, tix, fdir)
-    else:
-        log("tuning plain FP64 walk")
-        th64, rms64, mx64 = tune_theta(eng, "theta", tsp, tix, fdir, log=log)
-    eng.build()
-    st64 = eng.count_interactions()
-    eng.set_particles(qel, qion)
-    s64, msf64, drift64 = timed_run(eng, dt, a.steps, a.warmup)
-    log("plain FP64: theta=%.4f rms=%.2e %.2f steps/s walk %.1f ms" % (th64, rms64, 1 / s64, msf64))
-
-    head = {"variant": "plain-fp64", "theta": th64, "rms": rms64, "max": mx64, "s": s64, "ms_force": msf64,
-            "drift": drift64, "st": st64, "prec": "fp64"}
+    plain = None
+    if a.only in ("all", "ref64"):
+        if a.theta64 > 0:
+            eng.configure(theta=a.theta64)
+            th64 = a.theta64
+            rms64, mx64 = force_error(eng, tsp, tix, fdir)
+        else:
+            log("tuning plain FP64 walk")
+            th64, rms64, mx64 = tune_theta(eng, "theta", tsp, tix, fdir, log=log)
+        eng.build()
+        st64 = eng.count_interactions()
+        eng.set_particles(qel, qion)
+        s64, msf64, drift64 = timed_run(eng, dt, a.steps, a.warmup)
+        log("plain FP64: theta=%.4f rms=%.2e %.2f steps/s walk %.1f ms" % (th64, rms64, 1 / s64, msf64))
+        plain = {"variant": "plain-fp64", "theta": th64, "rms": rms64, "max": mx64, "s": s64, "ms_force": msf64,
+                 "drift": drift64, "st": st64, "prec": "fp64"}
+    mixed = None
+    if a.only in ("all", "mixed"):
+        eng.set_particles(qel, qion)
+        eng.configure(walk="mixed", theta_mixed=0.5, group=a.group)
+        if a.theta32 > 0:
+            eng.configure(theta_mixed=a.theta32)
+            th32 = a.theta32
+            rms32, mx32 = force_error(eng, tsp, tix, fdir)
+        else:
+            log("tuning mixed walk")
+            th32, rms32, mx32 = tune_theta(eng, "theta_mixed", tsp, tix, fdir, log=log)
+        eng.build()
+        st32 = eng.count_interactions()
+        eng.set_particles(qel, qion)
+        s32, msf32, drift32 = timed_run(eng, dt, a.steps, a.warmup)
+        log("mixed: theta=%.4f rms=%.2e %.2f steps/s walk %.1f ms" % (th32, rms32, 1 / s32, msf32))
+        mixed = {"variant": "mixed", "theta": th32, "rms": rms32, "max": mx32, "s": s32, "ms_force": msf32,
+                 "drift": drift32, "st": st32, "prec": "fp32"}
     eng.close()
 
-    flops = walk_flops(head["st"])
+    head = mixed if mixed is not None else plain
+    flops = walk32_flops(head["st"]) if head["prec"] == "fp32" else walk_flops(head["st"])
     achieved = flops / (head["ms_force"] * 1e-3) / 1e12
     peak = PEAK[head["prec"]]
     res = {
@@ -185,12 +220,16 @@ def main():
                    "n_total": 2 * n, "variant": head["variant"], "theta": head["theta"]},
         "accuracy": {"rms_rel_force_err": head["rms"], "max_rel_force_err": head["max"],
                      "energy_drift": head["drift"], "sample": "4096 targets vs FP64 direct sum"},
-        "plain_fp64": {"steps_per_s": 1.0 / s64, "theta": th64, "rms": rms64, "max": mx64,
-                       "ms_walk": msf64, "energy_drift": drift64},
+        "plain_fp64": None if plain is None else {
+            "steps_per_s": 1.0 / plain["s"], "theta": plain["theta"], "rms": plain["rms"], "max": plain["max"],
+            "ms_walk": plain["ms_force"], "energy_drift": plain["drift"],
+            "walk_tflops": walk_flops(plain["st"]) / (plain["ms_force"] * 1e-3) / 1e12},
+        "speedup_vs_plain_fp64": None if plain is None or mixed is None else plain["s"] / mixed["s"],
         "interactions_per_s": (sum(head["st"]["m2p"]) + sum(head["st"]["p2p"])) / head["s"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_walk64 (both target species)",
+                     "kernel": "k_walk32 (persistent, both species)" if head["prec"] == "fp32"
+                     else "k_walk64 (both target species)",
                      "flops_per_step": flops},
     }
     if not a.no_cpu:

@@ -127,6 +127,38 @@ typedef struct tb_stats {
 int tb_count_interactions(tb_handle *h, tb_stats *s);
 int tb_get_stats(tb_handle *h, tb_stats *s);
 
+/* ------------------------------------------------ learned opening criterion
+ * With mac = TB_MAC_LEARNED the mixed walk accepts every cell beyond
+ * s/theta_lo + dx, opens every cell within s/theta_mixed + dx, and in between
+ * asks the MLP (8 -> 32 -> 32 -> 1, BF16 weights, FP32 accumulation) for the
+ * predicted log10 relative force error of accepting the cell for the whole
+ * target group; it accepts when that is below log10(err_target).  Without a
+ * trained network it falls back to the fixed-theta walk (theta_mixed).
+ * No counterpart in the reference (its criterion is the fixed theta of
+ * tree.f90:742).
+ */
+#define TB_MLP_NPARAMS 1377
+int tb_mlp_set(tb_handle *h, const float *params, int32_t n);
+int tb_mlp_get(tb_handle *h, float *params, int32_t n);
+/* forward pass on n samples X[n][8] (bf16 != 0: weights rounded to BF16) */
+int tb_mlp_forward(tb_handle *h, int32_t n, const float *X, int32_t bf16, float *out);
+/* mean squared error and its gradient (fp32 master weights) */
+int tb_mlp_loss_grad(tb_handle *h, int32_t n, const float *X, const float *y, float *loss, float *grad);
+/* Adam on the GPU; loss_hist (optional) gets the per-step batch loss */
+int tb_mlp_train(tb_handle *h, int32_t n, const float *X, const float *y, int32_t steps, int32_t batch, float lr,
+                 uint64_t seed, float *loss_hist);
+/*
+ * Training data from the method itself: runs the mixed walk accepting cells
+ * beyond s/theta_lo + dx, records each (cell, group) pair whose acceptance
+ * the learned criterion would decide, with its 8 features X[k][8], and labels
+ * it y[k] = log10 max over the group of |F_M2P - F_exact| / |F| (FP64).
+ */
+int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n);
+
+/* time the mixed walk on the current tree for 64, 128 and 256 targets per
+ * group, keep the fastest for this problem size; ms_out[3] (optional) */
+int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out);
+
 const char *tb_last_error(void);
 
 /* sizeof(tb_pt), sizeof(tb_config), sizeof(tb_stats) as compiled (binding checks) */

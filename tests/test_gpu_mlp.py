@@ -1,0 +1,94 @@
+"""The GPU MLP kernels against the FP64 oracle (oracle/mlp_ref.py), the
+GPU trainer, and the learned opening criterion against the fixed-theta walk
+(its fallback and judge)."""
+import numpy as np
+import pytest
+
+import tree_amd as ta
+from oracle import mlp_ref
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+RS10 = 6.6493e-3
+
+
+def _eng(**kw):
+    kw.setdefault("rs", RS10)
+    return ta.Engine(**kw)
+
+
+def test_forward_and_gradient_vs_oracle():
+    """fp32 forward within 1e-5, bf16 forward within 3e-2, gradient within 1e-4 relative (vs FP64)."""
+    announce("mlp_grad")
+    rng = np.random.default_rng(1)
+    p = mlp_ref.init_params(5).astype(np.float32)
+    X = rng.normal(size=(5000, 8)).astype(np.float32)
+    t = rng.normal(size=5000).astype(np.float32)
+    y_ref, _ = mlp_ref.forward(p.astype(np.float64), X.astype(np.float64))
+    loss_ref, g_ref = mlp_ref.loss_grad(p.astype(np.float64), X.astype(np.float64), t.astype(np.float64))
+    with _eng() as eng:
+        eng.mlp_set(p)
+        assert np.array_equal(eng.mlp_get(), p)
+        y32 = eng.mlp_forward(X, bf16=False)
+        y16 = eng.mlp_forward(X, bf16=True)
+        loss, g = eng.mlp_loss_grad(X, t)
+    assert np.abs(y32 - y_ref).max() <= 1e-5 * max(1.0, np.abs(y_ref).max())
+    assert np.abs(y16 - y_ref).max() <= 3e-2 * max(1.0, np.abs(y_ref).max())
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
+
+
+def test_training_fits_a_known_function():
+    """Adam on the GPU fits y = sum(sin(x)) well below its variance."""
+    announce("mlp_train")
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-2, 2, size=(20000, 8)).astype(np.float32)
+    y = np.sin(X).sum(axis=1).astype(np.float32)
+    with _eng() as eng:
+        eng.mlp_set(mlp_ref.init_params(0).astype(np.float32))
+        hist = eng.mlp_train(X, y, steps=3000, batch=2048, lr=3e-3)
+        pred = eng.mlp_forward(X, bf16=False)
+    mse = float(np.mean((pred - y) ** 2))
+    assert hist[-50:].mean() < 0.05 * hist[:10].mean()
+    assert mse < 0.05 * float(np.var(y)), mse
+
+
+def _direct_err(eng, qel, qion, n):
+    tsp = np.r_[np.zeros(n, np.int32), np.ones(n, np.int32)]
+    tix = np.r_[np.arange(n), np.arange(n)].astype(np.int32)
+    fd = eng.direct_sample(tsp, tix)
+    eng.build()
+    eng.force()
+    e, i = eng.get_particles()
+    f = np.r_[e["ff"][:n], i["ff"][:n]]
+    rel = np.linalg.norm(f - fd, axis=1) / np.linalg.norm(fd, axis=1)
+    return float(np.sqrt(np.mean(rel ** 2)))
+
+
+def test_learned_mac_gain():
+    """Trained on labels the method generates, the learned criterion does fewer
+    M2P+P2P interactions than the fixed-theta walk at no larger rms error
+    (sampled against the FP64 direct sum)."""
+    announce("learned_mac_gain")
+    n = 30000
+    qel, qion = ta.gen_plasma(31, n, n, 250.0)
+    with _eng(theta=0.3, theta_mixed=0.6, theta_lo=0.3, walk="mixed", err_target=1e-5, mac="learned") as eng:
+        eng.set_particles(qel, qion)
+        eng.build()
+        X, y = eng.mlp_collect(cap=400000)
+        assert len(X) > 5000
+        eng.mlp_set(mlp_ref.init_params(0).astype(np.float32))
+        eng.mlp_train(X, y, steps=3000, batch=4096, lr=3e-3)
+        # fixed-theta judge at theta_lo: the same error budget without the network
+        eng.configure(mac="fixed", theta_mixed=0.3)
+        err_fix = _direct_err(eng, qel, qion, 2000)
+        st_fix = eng.count_interactions()
+        eng.configure(mac="learned", theta_mixed=0.6)
+        err_l = _direct_err(eng, qel, qion, 2000)
+        st_l = eng.count_interactions()
+    work_fix = sum(st_fix["m2p"]) + sum(st_fix["p2p"])
+    work_l = sum(st_l["m2p"]) + sum(st_l["p2p"])
+    print("fixed rms %.3e work %d | learned rms %.3e work %d" % (err_fix, work_fix, err_l, work_l))
+    assert work_l < work_fix
+    assert err_l < 3.0 * err_fix + 2e-6

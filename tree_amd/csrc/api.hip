@@ -695,6 +695,104 @@ int tb_count_interactions(tb_handle *h, tb_stats *st)
     return TB_OK;
 }
 
+int tb_mlp_set(tb_handle *h, const float *params, int32_t n)
+{
+    if (!h || !params) return fail(TB_ERR_ARG, "null argument");
+    if (n != TB_MLP_NPARAMS) return fail(TB_ERR_ARG, "expected %d parameters, got %d", TB_MLP_NPARAMS, n);
+    HIPCHK(hipSetDevice(h->cfg.device));
+    if (tb_mlp_upload(h->s, h->mx.mlp, params)) return fail(TB_ERR_NOMEM, "MLP upload failed");
+    HIPCHK(hipStreamSynchronize(h->s));
+    return TB_OK;
+}
+
+int tb_mlp_get(tb_handle *h, float *params, int32_t n)
+{
+    if (!h || !params) return fail(TB_ERR_ARG, "null argument");
+    if (n != TB_MLP_NPARAMS) return fail(TB_ERR_ARG, "expected %d parameters, got %d", TB_MLP_NPARAMS, n);
+    if (!h->mx.mlp.P) return fail(TB_ERR_STATE, "no network set or trained");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipMemcpyAsync(params, h->mx.mlp.P, (size_t)n * 4, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return TB_OK;
+}
+
+int tb_mlp_forward(tb_handle *h, int32_t n, const float *X, int32_t bf16, float *out)
+{
+    if (!h || !X || !out || n < 1) return fail(TB_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    int rc = tb_mlp_forward(h->s, h->mx.mlp, n, X, bf16, out);
+    return rc ? fail(rc, "MLP forward failed (network set?)") : TB_OK;
+}
+
+int tb_mlp_loss_grad(tb_handle *h, int32_t n, const float *X, const float *y, float *loss, float *grad)
+{
+    if (!h || !X || !y || !loss || !grad || n < 1) return fail(TB_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    int rc = tb_mlp_loss_grad(h->s, h->mx.mlp, n, X, y, loss, grad);
+    return rc ? fail(rc, "MLP gradient failed (network set?)") : TB_OK;
+}
+
+int tb_mlp_train(tb_handle *h, int32_t n, const float *X, const float *y, int32_t steps, int32_t batch, float lr,
+                 uint64_t seed, float *loss_hist)
+{
+    if (!h || !X || !y || n < 1 || steps < 1) return fail(TB_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    int rc = tb_mlp_train(h->s, h->mx.mlp, n, X, y, steps, batch, lr, seed, loss_hist);
+    return rc ? fail(rc, "MLP training failed (network set?)") : TB_OK;
+}
+
+int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n)
+{
+    if (!h || !X || !y || !n || cap < 1) return fail(TB_ERR_ARG, "bad argument");
+    if (h->cfg.walk != TB_WALK_MIXED) return fail(TB_ERR_STATE, "tb_mlp_collect needs the mixed walk");
+    if (!h->built) return fail(TB_ERR_STATE, "tb_mlp_collect before tb_build");
+    if (!(h->cfg.theta_lo > 0.0) || !(h->cfg.theta_lo < (h->cfg.theta_mixed > 0 ? h->cfg.theta_mixed : h->cfg.theta)))
+        return fail(TB_ERR_ARG, "need 0 < theta_lo < theta_mixed");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    int rc = tb_mixed_collect(h->s, h->mx, h->sp, h->cfg, h->d_sums, cap, X, y, n);
+    if (rc) return fail(rc, "%s", tb_mixed_error());
+    h->have_force = true;
+    return TB_OK;
+}
+
+int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (h->cfg.walk != TB_WALK_MIXED) return fail(TB_ERR_STATE, "tb_autotune needs the mixed walk");
+    if (!h->built) return fail(TB_ERR_STATE, "tb_autotune before tb_build");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    const int saved = h->cfg.group;
+    float best = 1e30f;
+    int best_tpl = 2;
+    for (int tpl = 1; tpl <= 4; tpl *= 2) {
+        h->cfg.group = 64 * tpl;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 3; rep++) {
+            HIPCHK(hipEventRecord(h->ev[2], h->s));
+            int rc = force_all(h, false);
+            if (rc) {
+                h->cfg.group = saved;
+                return rc;
+            }
+            HIPCHK(hipEventRecord(h->ev[3], h->s));
+            HIPCHK(hipEventSynchronize(h->ev[3]));
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, h->ev[2], h->ev[3]));
+            if (rep > 0) ms += t;  // first run warms caches
+        }
+        ms *= 0.5f;
+        if (ms_out) ms_out[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)] = ms;
+        if (ms < best) {
+            best = ms;
+            best_tpl = tpl;
+        }
+    }
+    h->cfg.group = saved;
+    tb_autotune_record(h->sp[0].n + h->sp[1].n, best_tpl);
+    if (group_out) *group_out = 64 * best_tpl;
+    return TB_OK;
+}
+
 int tb_get_stats(tb_handle *h, tb_stats *st)
 {
     if (!h || !st) return fail(TB_ERR_ARG, "null argument");

@@ -7,13 +7,21 @@
 //   bq1   float4   Q'xy, Q'yz, Q'zx, 0    exact second moment M about the centroid)
 #pragma once
 #include "tb_internal.h"
+#include "tb_mlp.h"
 
 struct MixedTree {
     int nn = 0, cap = 0;
     double4 *bmac = nullptr;
     uint32_t *binfo = nullptr;
     float4 *bq0 = nullptr, *bq1 = nullptr;
+    float2 *bsd = nullptr;       // cell size max(lx), centroid offset dx
     uint32_t *inv = nullptr;
+};
+
+// a (cell, target group) pair of the learned criterion's band, with its features
+struct BandRec {
+    uint32_t cell, group, tree, pad;
+    float f[TB_MLP_NIN];
 };
 
 struct MixedDev {
@@ -22,9 +30,11 @@ struct MixedDev {
     double *part = nullptr;       // per-wave energy partials, 4 per wave
     int part_n = 0;
     int tpl = 0;                  // targets per lane used by the last launch
-    // MLP gate for the learned opening criterion (bf16 weights + fp32 biases)
-    void *mlp = nullptr;
-    int mlp_ready = 0;
+    // the learned opening criterion
+    MlpDev mlp;
+    BandRec *rec = nullptr;       // collected band pairs
+    float *lab = nullptr;         // their labels
+    int rec_cap = 0;
 };
 
 int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, size_t *bytes);
@@ -32,4 +42,12 @@ int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &
 int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *epart, double *d_sums,
                    int64_t *d_cnt);
 void tb_mixed_free(MixedDev &M, size_t *bytes);
+// run the conservative walk (theta_lo) recording band pairs (theta_lo .. theta_mixed),
+// then label them with the FP64 error of accepting the cell; returns the pair count
+int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int cap,
+                     float *X, float *y, int *nout);
 const char *tb_mixed_error();
+
+// walk-group size chosen by the autotuner for a problem size (targets per lane: 1, 2 or 4)
+int tb_autotune_tpl(int n_total);
+void tb_autotune_record(int n_total, int tpl);

@@ -25,6 +25,10 @@
 
 #include "tb_mixed.h"
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 namespace {
 
 constexpr int BT = 256;
@@ -119,11 +123,38 @@ struct WalkArgs {
     const double4 *bmac[2];
     const uint32_t *binfo[2];
     const float4 *bq0[2], *bq1[2];
-    float inv_rs;
-    int32_t *ctr;     // [0] work counter, [1] error flag
-    double *part;     // 4 per wave: ee, ei, ie, ii
+    const float2 *bsd[2];
+    float inv_rs, rs;
+    float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo + dx are always accepted
+    float log_err;        // learned: accept a band cell when the predicted log10 error is below this
+    const float *mlpw;    // learned: bf16-rounded parameters
+    BandRec *rec;         // collect: band pairs
+    int rec_cap;
+    int32_t *ctr;         // [0] work counter, [1] error flag, [2] prep error, [3] band pair count
+    double *part;         // 4 per wave: ee, ei, ie, ii
     unsigned long long *cnt;
 };
+
+constexpr int MODE_FIXED = 0, MODE_LEARNED = 1, MODE_COLLECT = 2;
+
+// features of a (cell, target group) pair, log2 scaled by 1/8 (f6 is a flag)
+__device__ inline void band_features(float s, float dx, float d, float q, float4 q0, float4 q1, float hx, float hy,
+                                     float hz, float rs, bool like, int nact, float *f)
+{
+    const float cnt = fabsf(q);
+    const float qn = sqrtf(q0.y * q0.y + q0.z * q0.z + q0.w * q0.w +
+                           2.0f * (q1.x * q1.x + q1.y * q1.y + q1.z * q1.z));
+    const float h = sqrtf(hx * hx + hy * hy + hz * hz);
+    const float k = 0.125f;
+    f[0] = k * __log2f(s / d);
+    f[1] = k * __log2f(dx / s + 9.5367431640625e-07f);
+    f[2] = k * __log2f(qn / (cnt * d * d) + 9.3132257461547852e-10f);
+    f[3] = k * __log2f(cnt);
+    f[4] = k * __log2f(h / d + 9.5367431640625e-07f);
+    f[5] = k * __log2f(rs / d);
+    f[6] = like ? 1.0f : 0.0f;
+    f[7] = k * __log2f((float)nact);
+}
 
 template <int TPL>
 struct Acc {
@@ -209,10 +240,15 @@ __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     }
 }
 
-template <int TPL, bool COUNT>
+template <int TPL, int MODE, bool COUNT>
 __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
 {
     __shared__ WaveLDS lds[WPB];
+    __shared__ float wl[MODE == MODE_LEARNED ? TB_MLP_NPAR : 1];
+    if (MODE == MODE_LEARNED) {
+        for (int j = threadIdx.x; j < TB_MLP_NPAR; j += BT) wl[j] = W.mlpw[j];
+        __syncthreads();
+    }
     const int lane = lane_id();
     WaveLDS &L = lds[threadIdx.x >> 6];
     const int G = 64 * TPL;
@@ -265,8 +301,7 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
         for (int t = 0; t < TPL; t++) {
             for (int k = 0; k < 3; k++) A.s[t][k] = A.c[t][k] = 0.0f;
         }
-        int nact = 0;  // targets in this group (interaction counting)
-        if (COUNT) nact = min(G, n - base);
+        const int nact = min(G, n - base);  // targets in this group
 
         for (int tree = 0; tree < 2; tree++) {
             const int pr = tsp * 2 + tree;  // ee, ei, ie, ii
@@ -275,6 +310,7 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             const double4 *bmac = W.bmac[tree];
             const uint32_t *binfo = W.binfo[tree];
             const float4 *bq0 = W.bq0[tree], *bq1 = W.bq1[tree];
+            const float2 *bsd = W.bsd[tree];
 #pragma unroll
             for (int t = 0; t < TPL; t++) A.ps[t] = A.pc[t] = 0.0f;
             int sp = 1, nm = 0, np = 0;
@@ -302,8 +338,8 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                 const bool have = lane < total;
                 const uint32_t node = have ? L.batch[lane] : 0u;
                 wave_sync();
-                bool leaf = false, acc = false, open = false;
-                float cx = 0.f, cy = 0.f, cz = 0.f;
+                bool leaf = false, acc = false, open = false, band = false;
+                float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
                 uint32_t info = 0;
                 if (have) {
                     const double4 m = bmac[node];
@@ -317,10 +353,43 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                         const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
                         const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
                         const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
-                        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        const float rc = (float)m.w;
-                        if (d2 > rc * rc * 1.000004f) acc = true;
-                        else open = true;
+                        d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const float rc = (float)m.w;  // s / theta_mixed + dx
+                        const bool far_hi = d2 > rc * rc * 1.000004f;
+                        if (MODE == MODE_FIXED) {
+                            acc = far_hi;
+                            open = !far_hi;
+                        } else {
+                            const float2 sd = bsd[node];
+                            const float rlo = sd.x * W.inv_theta_lo + sd.y;
+                            if (d2 > rlo * rlo * 1.000004f) acc = true;
+                            else if (!far_hi) open = true;
+                            else band = true;
+                        }
+                    }
+                }
+                if (MODE != MODE_FIXED && band) {
+                    const float2 sd = bsd[node];
+                    const float4 q0 = bq0[node], q1 = bq1[node];
+                    float f[TB_MLP_NIN];
+                    band_features(sd.x, sd.y, sqrtf(d2), q0.x, q0, q1, hx, hy, hz, W.rs, like, nact, f);
+                    if (MODE == MODE_LEARNED) {
+                        acc = tb_mlp_eval(wl, f) < W.log_err;
+                        open = !acc;
+                    } else {
+                        open = true;  // the conservative walk opens it; the pair is recorded
+                        if (fabsf(q0.x) <= 4096.0f) {
+                            const int k = atomicAdd(&W.ctr[3], 1);
+                            if (k < W.rec_cap) {
+                                BandRec r;
+                                r.cell = node;
+                                r.group = (uint32_t)g;
+                                r.tree = (uint32_t)tree;
+                                r.pad = 0;
+                                for (int j = 0; j < TB_MLP_NIN; j++) r.f[j] = f[j];
+                                W.rec[k] = r;
+                            }
+                        }
                     }
                 }
                 if (COUNT && have) c_visit[pr]++;
@@ -424,7 +493,7 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
                                                     const int32_t *__restrict__ ndepth, double smax0, double theta,
                                                     double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
                                                     float4 *__restrict__ bq0, float4 *__restrict__ bq1,
-                                                    int32_t *__restrict__ err)
+                                                    float2 *__restrict__ bsd, int32_t *__restrict__ err)
 {
     const int b = blockIdx.x * BT + threadIdx.x;
     if (b >= nn) return;
@@ -437,6 +506,7 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
         bmac[b] = make_double4(m.x, m.y, m.z, 0.0);
         bq0[b] = make_float4((float)q, 0.f, 0.f, 0.f);
         bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        bsd[b] = make_float2(0.f, 0.f);
         return;
     }
     int nc = 0;
@@ -449,8 +519,10 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
         c = nc2;
     }
     binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nc - 1);
+    // the cell size max(lx) halves exactly with each level: max(L0) * 2^-depth
     const double s = ldexp(smax0, -ndepth[p]);
     bmac[b] = make_double4(m.x, m.y, m.z, s / theta + ndx[p]);
+    bsd[b] = make_float2((float)s, (float)ndx[p]);
     const double *M = nm2 + (size_t)p * 6;
     const double tr = M[0] + M[1] + M[2];
     bq0[b] = make_float4((float)q, (float)(3.0 * M[0] - tr), (float)(3.0 * M[1] - tr), (float)(3.0 * M[2] - tr));
@@ -474,6 +546,89 @@ __global__ __launch_bounds__(64) void k_reduce_part(int nw, const double *__rest
         if (threadIdx.x == 0) out[c] = s;
     }
     if (threadIdx.x == 0) out[15] = (double)(ctr[1] | ctr[2]);
+}
+
+// ------------------------------------------------------------ labels
+
+struct LabelArgs {
+    const BandRec *rec;
+    int nrec;
+    int G;                 // targets per group of the collecting walk
+    int ng0;               // electron groups
+    const double *px[2], *py[2], *pz[2];  // key-ordered positions
+    const double *fx[2], *fy[2], *fz[2];  // forces of the collecting walk
+    int n[2];
+    const uint32_t *bfs[2];               // breadth-first -> preorder
+    const double4 *nmac[2];
+    const double *nmom[2], *nm2[2];
+    const int32_t *nstart[2], *ncount[2];
+    double rs;
+    float *lab;
+};
+
+// label of a band pair: log10 of the largest, over the group's targets, of
+// |F_M2P - F_exact| / |F_target| with F_M2P the cell's monopole + exact
+// quadrupole field and F_exact the FP64 sum over the cell's particles with
+// the reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg)
+__global__ __launch_bounds__(64) void k_labels(LabelArgs A)
+{
+    const int r = blockIdx.x;
+    if (r >= A.nrec) return;
+    const BandRec R = A.rec[r];
+    const int tsp = (int)R.group < A.ng0 ? 0 : 1;
+    const int base = ((int)R.group - (tsp ? A.ng0 : 0)) * A.G;
+    const int tr = (int)R.tree;
+    const bool like = tr == tsp;
+    const uint32_t p = A.bfs[tr][R.cell];
+    const double4 c = A.nmac[tr][p];
+    const double q = A.nmom[tr][(size_t)p * 10];
+    const double *M = A.nm2[tr] + (size_t)p * 6;
+    const double trM = M[0] + M[1] + M[2];
+    const double Qxx = 3.0 * M[0] - trM, Qyy = 3.0 * M[1] - trM, Qzz = 3.0 * M[2] - trM;
+    const double Qxy = 3.0 * M[3], Qyz = 3.0 * M[4], Qzx = 3.0 * M[5];
+    const int s0 = A.nstart[tr][p], cnt = A.ncount[tr][p];
+    const double qs = tr ? 1.0 : -1.0;
+    const double *sx = A.px[tr], *sy = A.py[tr], *sz = A.pz[tr];
+    double worst = 0.0;
+    for (int t = threadIdx.x; t < A.G; t += 64) {
+        const int i = base + t;
+        if (i >= A.n[tsp]) break;
+        const double x = A.px[tsp][i], y = A.py[tsp][i], z = A.pz[tsp][i];
+        // multipole field
+        const double dx = x - c.x, dy = y - c.y, dz = z - c.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ri = 1.0 / sqrt(r2), ri2 = ri * ri, ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+        const double qx = Qxx * dx + Qxy * dy + Qzx * dz;
+        const double qy = Qxy * dx + Qyy * dy + Qyz * dz;
+        const double qz = Qzx * dx + Qyz * dy + Qzz * dz;
+        const double xqx = dx * qx + dy * qy + dz * qz;
+        const double k = q * ri3 + 2.5 * xqx * ri7;
+        double ex = dx * k - qx * ri5, ey = dy * k - qy * ri5, ez = dz * k - qz * ri5;
+        // exact field of the cell's particles
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int j = s0; j < s0 + cnt; j++) {
+            const double ax = x - sx[j], ay = y - sy[j], az = z - sz[j];
+            const double a2 = ax * ax + ay * ay + az * az;
+            if (like) {
+                if (!(a2 > 0.01)) continue;
+                const double ar = sqrt(a2);
+                const double w = qs / (a2 * ar);
+                gx += ax * w; gy += ay * w; gz += az * w;
+            } else {
+                const double ar = sqrt(a2);
+                const double e = exp(-ar / A.rs);
+                const double w = qs * (1.0 / ar - e * (1.0 / ar + 1.0 / A.rs)) / a2;
+                gx += ax * w; gy += ay * w; gz += az * w;
+            }
+        }
+        ex -= gx; ey -= gy; ez -= gz;
+        const double fx = A.fx[tsp][i], fy = A.fy[tsp][i], fz = A.fz[tsp][i];
+        const double fn = sqrt(fx * fx + fy * fy + fz * fz);
+        const double err = TB_EPS * sqrt(ex * ex + ey * ey + ez * ez) / fmax(fn, 1e-300);
+        worst = fmax(worst, err);
+    }
+    worst = wave_max(worst);
+    if (threadIdx.x == 0) A.lab[r] = (float)log10(worst + 1e-12);
 }
 
 inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
@@ -501,9 +656,7 @@ int grow(T **p, int *cap, int want, size_t *bytes)
 int pick_tpl(const tb_config &cfg, int n)
 {
     if (cfg.group == 64 || cfg.group == 128 || cfg.group == 256) return cfg.group / 64;
-    // default from the autotuner's table: larger groups once there are enough of them
-    (void)n;
-    return 2;
+    return tb_autotune_tpl(n);
 }
 
 int resident_waves()
@@ -519,7 +672,18 @@ int resident_waves()
     return w;
 }
 
-int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int64_t *d_cnt)
+template <int MODE, bool COUNT>
+void dispatch(hipStream_t s, int tpl, int blocks, const WalkArgs &W)
+{
+    switch (tpl) {
+    case 1: hipLaunchKernelGGL((k_walk32<1, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
+    case 4: hipLaunchKernelGGL((k_walk32<4, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
+    default: hipLaunchKernelGGL((k_walk32<2, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
+    }
+}
+
+int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int64_t *d_cnt,
+           int mode)
 {
     const int tpl = pick_tpl(cfg, sp[0].n + sp[1].n);
     const int G = 64 * tpl;
@@ -531,8 +695,16 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
         W.n[k] = sp[k].n;
         W.ng[k] = (sp[k].n + G - 1) / G;
         W.bmac[k] = M.t[k].bmac; W.binfo[k] = M.t[k].binfo; W.bq0[k] = M.t[k].bq0; W.bq1[k] = M.t[k].bq1;
+        W.bsd[k] = M.t[k].bsd;
     }
     W.inv_rs = (float)(1.0 / cfg.rs);
+    W.rs = (float)cfg.rs;
+    const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : (cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta);
+    W.inv_theta_lo = (float)(1.0 / th_lo);
+    W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
+    W.mlpw = M.mlp.Pbf;
+    W.rec = M.rec;
+    W.rec_cap = M.rec_cap;
     W.ctr = M.ctr;
     W.cnt = (unsigned long long *)d_cnt;
     const int ngt = W.ng[0] + W.ng[1];
@@ -551,18 +723,15 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
     W.part = M.part;
     M.tpl = tpl;
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
-    if (d_cnt) {
-        switch (tpl) {
-        case 1: hipLaunchKernelGGL((k_walk32<1, true>), dim3(blocks), dim3(BT), 0, s, W); break;
-        case 4: hipLaunchKernelGGL((k_walk32<4, true>), dim3(blocks), dim3(BT), 0, s, W); break;
-        default: hipLaunchKernelGGL((k_walk32<2, true>), dim3(blocks), dim3(BT), 0, s, W); break;
-        }
+    if (mode == MODE_COLLECT) {
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
+        dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
+    } else if (mode == MODE_LEARNED) {
+        if (d_cnt) dispatch<MODE_LEARNED, true>(s, tpl, blocks, W);
+        else dispatch<MODE_LEARNED, false>(s, tpl, blocks, W);
     } else {
-        switch (tpl) {
-        case 1: hipLaunchKernelGGL((k_walk32<1, false>), dim3(blocks), dim3(BT), 0, s, W); break;
-        case 4: hipLaunchKernelGGL((k_walk32<4, false>), dim3(blocks), dim3(BT), 0, s, W); break;
-        default: hipLaunchKernelGGL((k_walk32<2, false>), dim3(blocks), dim3(BT), 0, s, W); break;
-        }
+        if (d_cnt) dispatch<MODE_FIXED, true>(s, tpl, blocks, W);
+        else dispatch<MODE_FIXED, false>(s, tpl, blocks, W);
     }
     hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(64), 0, s, waves, M.part, M.ctr, d_sums);
     if (hipGetLastError() != hipSuccess) {
@@ -570,6 +739,12 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
         return TB_ERR_HIP;
     }
     return TB_OK;
+}
+
+int walk_mode(const MixedDev &M, const tb_config &cfg)
+{
+    // the fixed-theta walk is the fallback while no trained network is loaded
+    return (cfg.mac == TB_MAC_LEARNED && M.mlp.ready) ? MODE_LEARNED : MODE_FIXED;
 }
 
 }  // namespace
@@ -589,13 +764,14 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &c
         SpeciesDev &S = sp[k];
         MixedTree &T = M.t[k];
         int cap = T.cap, rc;
-        int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap;
+        int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap;
         if ((rc = grow(&T.bmac, &c1, S.nn, bytes))) return rc;
         if ((rc = grow(&T.binfo, &c2, S.nn, bytes))) return rc;
         if ((rc = grow(&T.bq0, &c3, S.nn, bytes))) return rc;
         if ((rc = grow(&T.bq1, &c4, S.nn, bytes))) return rc;
         if ((rc = grow(&T.inv, &c5, S.nn, bytes))) return rc;
-        T.cap = std::min(std::min(c1, c2), std::min(std::min(c3, c4), c5));
+        if ((rc = grow(&T.bsd, &c6, S.nn, bytes))) return rc;
+        T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(c5, c6));
         T.nn = S.nn;
         double root[6];
         // max root length; the root buffer is ready once the build has synchronised
@@ -608,7 +784,7 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &c
         hipLaunchKernelGGL(k_inv, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv);
         hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv, S.nmac, S.nnext,
                            S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, smax0, theta, T.bmac, T.binfo, T.bq0, T.bq1,
-                           M.ctr + 2);
+                           T.bsd, M.ctr + 2);
     }
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: prep launch failed";
@@ -619,29 +795,115 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &c
 
 int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums)
 {
-    return launch(s, M, sp, cfg, d_sums, nullptr);
+    return launch(s, M, sp, cfg, d_sums, nullptr, walk_mode(M, cfg));
 }
 
 int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums,
                    int64_t *d_cnt)
 {
     hipMemsetAsync(d_cnt, 0, 12 * sizeof(int64_t), s);
-    return launch(s, M, sp, cfg, d_sums, d_cnt);
+    return launch(s, M, sp, cfg, d_sums, d_cnt, walk_mode(M, cfg));
+}
+
+int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int cap,
+                     float *X, float *y, int *nout)
+{
+    if (cap > M.rec_cap) {
+        if (M.rec) hipFree(M.rec);
+        if (M.lab) hipFree(M.lab);
+        M.rec = nullptr;
+        M.lab = nullptr;
+        M.rec_cap = 0;
+        if (hipMalloc((void **)&M.rec, (size_t)cap * sizeof(BandRec)) != hipSuccess ||
+            hipMalloc((void **)&M.lab, (size_t)cap * sizeof(float)) != hipSuccess) {
+            g_err = "collect: allocation failed";
+            return TB_ERR_NOMEM;
+        }
+        M.rec_cap = cap;
+    }
+    int rc = launch(s, M, sp, cfg, d_sums, nullptr, MODE_COLLECT);
+    if (rc) return rc;
+    int32_t nrec = 0;
+    if (hipMemcpyAsync(&nrec, M.ctr + 3, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        g_err = "collect: count read failed";
+        return TB_ERR_HIP;
+    }
+    nrec = std::min(nrec, cap);
+    LabelArgs A;
+    A.rec = M.rec;
+    A.nrec = nrec;
+    A.G = 64 * M.tpl;
+    A.ng0 = (sp[0].n + A.G - 1) / A.G;
+    for (int k = 0; k < 2; k++) {
+        const int c = sp[k].cur;
+        A.px[k] = sp[k].x[c][0]; A.py[k] = sp[k].x[c][1]; A.pz[k] = sp[k].x[c][2];
+        A.fx[k] = sp[k].f[c][0]; A.fy[k] = sp[k].f[c][1]; A.fz[k] = sp[k].f[c][2];
+        A.n[k] = sp[k].n;
+        A.bfs[k] = sp[k].bfs;
+        A.nmac[k] = sp[k].nmac;
+        A.nmom[k] = sp[k].nmom;
+        A.nm2[k] = sp[k].nm2;
+        A.nstart[k] = sp[k].nstart;
+        A.ncount[k] = sp[k].ncount;
+    }
+    A.rs = cfg.rs;
+    A.lab = M.lab;
+    if (nrec > 0) hipLaunchKernelGGL(k_labels, dim3(nrec), dim3(64), 0, s, A);
+    std::vector<BandRec> h((size_t)nrec);
+    if (nrec > 0 && (hipMemcpyAsync(h.data(), M.rec, (size_t)nrec * sizeof(BandRec), hipMemcpyDeviceToHost, s) !=
+                         hipSuccess ||
+                     hipMemcpyAsync(y, M.lab, (size_t)nrec * 4, hipMemcpyDeviceToHost, s) != hipSuccess)) {
+        g_err = "collect: read-back failed";
+        return TB_ERR_HIP;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        g_err = "collect: labels failed";
+        return TB_ERR_HIP;
+    }
+    for (int r = 0; r < nrec; r++)
+        for (int j = 0; j < TB_MLP_NIN; j++) X[(size_t)r * TB_MLP_NIN + j] = h[r].f[j];
+    *nout = nrec;
+    return TB_OK;
 }
 
 void tb_mixed_free(MixedDev &M, size_t *bytes)
 {
     for (int k = 0; k < 2; k++) {
         MixedTree &T = M.t[k];
-        hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.inv);
+        hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.inv); hipFree(T.bsd);
         T = MixedTree();
     }
     hipFree(M.ctr);
     hipFree(M.part);
+    if (M.rec) hipFree(M.rec);
+    if (M.lab) hipFree(M.lab);
+    tb_mlp_free(M.mlp);
     M.ctr = nullptr;
     M.part = nullptr;
     M.part_n = 0;
+    M.rec = nullptr;
+    M.lab = nullptr;
+    M.rec_cap = 0;
     (void)bytes;
 }
 
 const char *tb_mixed_error() { return g_err; }
+
+// autotuner table: log2 bucket of the total particle count -> targets per lane
+static int g_tpl_table[64] = {0};
+
+static int size_bucket(int n)
+{
+    int b = 0;
+    while ((1 << (b + 1)) <= n && b < 62) b++;
+    return b;
+}
+
+int tb_autotune_tpl(int n_total)
+{
+    const int v = g_tpl_table[size_bucket(n_total)];
+    return v ? v : 2;
+}
+
+void tb_autotune_record(int n_total, int tpl) { g_tpl_table[size_bucket(n_total)] = tpl; }

@@ -31,6 +31,7 @@ XM_E = 5.4858e-4                 # amu
 XM_I = 85.0                      # amu (Rb-85)
 
 WALK = {"ref64": 0, "mixed": 1}
+MLP_NPARAMS = 1377
 MAC = {"fixed": 0, "learned": 1}
 
 ERRORS = {1: "ARG", 2: "HIP", 3: "DEPTH", 4: "NOMEM", 5: "STATE", 6: "NONFINITE", 7: "NODEV"}
@@ -93,6 +94,13 @@ SIGNATURES = [
     ("tb_direct_sample", ctypes.c_int, [_P, _I, _P, _P, _P]),
     ("tb_count_interactions", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_get_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    ("tb_mlp_set", ctypes.c_int, [_P, _P, _I]),
+    ("tb_mlp_get", ctypes.c_int, [_P, _P, _I]),
+    ("tb_mlp_forward", ctypes.c_int, [_P, _I, _P, _I, _P]),
+    ("tb_mlp_loss_grad", ctypes.c_int, [_P, _I, _P, _P, _P, _P]),
+    ("tb_mlp_train", ctypes.c_int, [_P, _I, _P, _P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
+    ("tb_mlp_collect", ctypes.c_int, [_P, _I, _P, _P, _P]),
+    ("tb_autotune", ctypes.c_int, [_P, _P, _P]),
     ("tb_last_error", ctypes.c_char_p, []),
     ("tb_abi_sizes", ctypes.c_int, [_P, _P, _P]),
     ("tb_kelbg_length", ctypes.c_double, [_D, _D, _D]),
@@ -247,6 +255,51 @@ class Engine:
         s = Stats()
         _check(self.lib.tb_count_interactions(self.h, ctypes.byref(s)))
         return _stats_dict(s)
+
+    # ------------------------------------------------ learned opening criterion
+    def mlp_set(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        _check(self.lib.tb_mlp_set(self.h, _ptr(p), len(p)))
+
+    def mlp_get(self):
+        p = np.zeros(MLP_NPARAMS, np.float32)
+        _check(self.lib.tb_mlp_get(self.h, _ptr(p), len(p)))
+        return p
+
+    def mlp_forward(self, X, bf16=True):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(len(X), np.float32)
+        _check(self.lib.tb_mlp_forward(self.h, len(X), _ptr(X), 1 if bf16 else 0, _ptr(out)))
+        return out
+
+    def mlp_loss_grad(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        _check(self.lib.tb_mlp_loss_grad(self.h, len(X), _ptr(X), _ptr(y), _ptr(loss), _ptr(g)))
+        return float(loss[0]), g
+
+    def mlp_train(self, X, y, steps=2000, batch=4096, lr=3e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        hist = np.zeros(steps, np.float32)
+        _check(self.lib.tb_mlp_train(self.h, len(X), _ptr(X), _ptr(y), steps, min(batch, len(X)), lr, seed,
+                                     _ptr(hist)))
+        return hist
+
+    def mlp_collect(self, cap=200000):
+        X = np.zeros((cap, 8), np.float32)
+        y = np.zeros(cap, np.float32)
+        n = ctypes.c_int32()
+        _check(self.lib.tb_mlp_collect(self.h, cap, _ptr(X), _ptr(y), ctypes.byref(n)))
+        return X[: n.value].copy(), y[: n.value].copy()
+
+    def autotune(self):
+        g = ctypes.c_int32()
+        ms = np.zeros(3, np.float32)
+        _check(self.lib.tb_autotune(self.h, ctypes.byref(g), _ptr(ms)))
+        return g.value, ms
 
     def stats(self):
         s = Stats()
