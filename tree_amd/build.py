@@ -50,7 +50,42 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_fortran(verbose)
     return OUT
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FSRC = ["datastr.f90", "treemi_capi.f90", "treemi_force.f90", "main.f90"]
+FBIN = os.path.join(HERE, "bin", "octree_gpu")
+
+
+def build_fortran(verbose=False):
+    """The Fortran driver (tree_amd/bin/octree_gpu), linked to libtreemi.so."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("amdflang not found: Fortran driver skipped")
+        return None
+    fdir = os.path.join(BUILD, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    os.makedirs(os.path.dirname(FBIN), exist_ok=True)
+    objs = []
+    for f in FSRC:
+        src = os.path.join(HERE, "fortran", f)
+        obj = os.path.join(fdir, f.replace(".f90", ".o"))
+        r = subprocess.run([FLANG, "-O2", "-c", src, "-o", obj, "-module-dir", fdir], capture_output=True,
+                           text=True, cwd=fdir)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran compile failed: %s\n%s" % (f, r.stderr))
+        objs.append(obj)
+    tmp = FBIN + ".tmp"
+    r = subprocess.run([FLANG, "-o", tmp] + objs + ["-L" + HERE, "-ltreemi", "-Wl,-rpath,$ORIGIN/.."],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran link failed:\n%s" % r.stderr)
+    os.replace(tmp, FBIN)
+    if verbose:
+        print("built", FBIN)
+    return FBIN
 
 
 if __name__ == "__main__":
