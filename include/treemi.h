@@ -159,6 +159,24 @@ int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n);
  * group, keep the fastest for this problem size; ms_out[3] (optional) */
 int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out);
 
+/* ------------------------------------------------------- batched ensemble
+ * Many independent realisations (each its own electron and ion trees) on one
+ * GPU: per step the builds of all realisations share one host read-back and
+ * the force is one persistent walk launch over all their target groups, with
+ * the autotuned group size.  Mixed walk only.  No counterpart in the
+ * reference, which runs one realisation per program run.
+ */
+typedef struct tb_ensemble tb_ensemble;
+int tb_ensemble_create(tb_ensemble **e, int32_t n, const tb_config *cfgs);
+int tb_ensemble_destroy(tb_ensemble *e);
+int tb_ensemble_set_particles(tb_ensemble *e, int32_t r, int32_t ne, const tb_pt *el, int32_t ni, const tb_pt *io);
+int tb_ensemble_get_particles(tb_ensemble *e, int32_t r, tb_pt *el, tb_pt *io);
+/* build + force for all; epot (optional) [n][2] = Epot_e, Epot_i */
+int tb_ensemble_force(tb_ensemble *e, double *epot);
+/* nsteps for all; out (optional) [n][6] as tb_step after the last step */
+int tb_ensemble_step(tb_ensemble *e, double dt, int32_t nsteps, double *out);
+int tb_ensemble_autotune(tb_ensemble *e, int32_t *group_out, float *ms_out);
+
 const char *tb_last_error(void);
 
 /* sizeof(tb_pt), sizeof(tb_config), sizeof(tb_stats) as compiled (binding checks) */

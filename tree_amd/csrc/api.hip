@@ -202,14 +202,15 @@ int check_cfg(const tb_config *c)
     return TB_OK;
 }
 
-// BuildLocalTree for both species: two phases around one small host read-back
-int build_all(tb_handle *h)
+// BuildLocalTree for both species, in two phases around one small host
+// read-back (node totals, depth histograms, error flags).  An ensemble runs
+// phase A for every realisation, synchronises once, then runs phase B.
+int build_phase_a(tb_handle *h, int *which)
 {
     hipStream_t s = h->s;
     int rc;
     if ((rc = ensure_tmp(h, std::max(h->sp[0].n, h->sp[1].n)))) return rc;
     HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * (MISC_HIST + 2 * 2 * HW), s));
-    int which[2];
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
         const int c = S.cur;
@@ -223,7 +224,14 @@ int build_all(tb_handle *h)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, sizeof(int32_t) * (MISC_HIST + 2 * 2 * HW), hipMemcpyDeviceToHost,
                           s));
-    HIPCHK(hipStreamSynchronize(s));
+    return TB_OK;
+}
+
+// after the stream has been synchronised
+int build_phase_b(tb_handle *h, const int *which)
+{
+    hipStream_t s = h->s;
+    int rc;
     for (int sp = 0; sp < 2; sp++) {
         int e = h->h_misc[sp];
         if (e & 4) return fail(TB_ERR_HIP, "species %d: internal sort check failed", sp);
@@ -264,17 +272,33 @@ int build_all(tb_handle *h)
         h->st.max_depth[sp] = S.maxdepth;
     }
     HIPCHK(hipGetLastError());
-    // integrity flags raised during the second phase (bad permutation / table)
-    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (int sp = 0; sp < 2; sp++)
-        if (h->h_misc[sp] & 12) return fail(TB_ERR_HIP, "species %d: internal tree integrity check failed (%d)", sp,
-                                            h->h_misc[sp]);
     if (h->cfg.walk == TB_WALK_MIXED) {
         if ((rc = tb_mixed_prep(h->s, h->mx, h->sp, h->cfg, &h->bytes))) return fail(rc, "%s", tb_mixed_error());
     }
     h->built = true;
     return TB_OK;
+}
+
+// integrity flags raised during phase B (bad permutation / malformed table)
+int build_check(tb_handle *h)
+{
+    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int sp = 0; sp < 2; sp++)
+        if (h->h_misc[sp] & 12) {
+            h->built = false;
+            return fail(TB_ERR_HIP, "species %d: internal tree integrity check failed (%d)", sp, h->h_misc[sp]);
+        }
+    return TB_OK;
+}
+
+int build_all(tb_handle *h)
+{
+    int which[2], rc;
+    if ((rc = build_phase_a(h, which))) return rc;
+    HIPCHK(hipStreamSynchronize(h->s));
+    if ((rc = build_phase_b(h, which))) return rc;
+    return build_check(h);
 }
 
 int force_all(tb_handle *h, bool counting)
@@ -789,6 +813,236 @@ int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out)
     }
     h->cfg.group = saved;
     tb_autotune_record(h->sp[0].n + h->sp[1].n, best_tpl);
+    if (group_out) *group_out = 64 * best_tpl;
+    return TB_OK;
+}
+
+// =================================================== batched ensemble
+
+}  // extern "C"
+
+struct tb_ensemble {
+    std::vector<tb_handle *> h;
+    hipStream_t s = nullptr;
+    tb_config cfg{};
+    MixedDev mx;                 // shared launch state: descriptors, counters, energy sums
+    double *d_esum = nullptr;    // 4 per realisation
+    double *h_buf = nullptr;     // pinned: 4 energies + 4 kinetic sums per realisation
+    bool have_force = false;
+    size_t bytes = 0;
+};
+
+namespace {
+
+int ens_build_all(tb_ensemble *E)
+{
+    const int n = (int)E->h.size();
+    std::vector<int> which(2 * (size_t)n);
+    int rc;
+    for (int r = 0; r < n; r++)
+        if ((rc = build_phase_a(E->h[r], &which[2 * r]))) return rc;
+    HIPCHK(hipStreamSynchronize(E->s));
+    for (int r = 0; r < n; r++)
+        if ((rc = build_phase_b(E->h[r], &which[2 * r]))) return rc;
+    return TB_OK;
+}
+
+int ens_force(tb_ensemble *E)
+{
+    const int n = (int)E->h.size();
+    std::vector<SpeciesDev *> sp(2 * (size_t)n);
+    std::vector<const MixedTree *> tr(2 * (size_t)n);
+    for (int r = 0; r < n; r++)
+        for (int k = 0; k < 2; k++) {
+            sp[2 * r + k] = &E->h[r]->sp[k];
+            tr[2 * r + k] = &E->h[r]->mx.t[k];
+        }
+    int rc = tb_mixed_force_multi(E->s, E->mx, sp.data(), tr.data(), n, E->cfg, E->d_esum);
+    if (rc) return fail(rc, "%s", tb_mixed_error());
+    for (int r = 0; r < n; r++) E->h[r]->have_force = true;
+    return TB_OK;
+}
+
+// energies (and kinetic sums when sampled) of every realisation back to the host
+int ens_finish(tb_ensemble *E, double *out, bool sampled)
+{
+    const int n = (int)E->h.size();
+    HIPCHK(hipMemcpyAsync(E->h_buf, E->d_esum, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, E->s));
+    if (sampled)
+        for (int r = 0; r < n; r++)
+            HIPCHK(hipMemcpyAsync(E->h_buf + 4 * n + 4 * r, E->h[r]->d_sums + 4, 4 * sizeof(double),
+                                  hipMemcpyDeviceToHost, E->s));
+    HIPCHK(hipStreamSynchronize(E->s));
+    int32_t fl[2];
+    HIPCHK(hipMemcpy(fl, E->mx.ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (fl[1]) return fail(TB_ERR_HIP, "ensemble walk: traversal stack overflow");
+    for (int r = 0; r < n; r++) {
+        HIPCHK(hipMemcpy(fl, E->h[r]->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if ((fl[0] | fl[1]) & 12) return fail(TB_ERR_HIP, "realisation %d: tree integrity check failed", r);
+        const double *v = E->h_buf + 4 * r;
+        tb_handle *h = E->h[r];
+        epot_from_sums(v, h->epot);
+        if (out) {
+            double *o = out + 6 * r;
+            o[0] = h->epot[0];
+            o[1] = h->epot[1];
+            if (sampled) {
+                const double *k = E->h_buf + 4 * n + 4 * r;
+                o[2] = k[0]; o[3] = k[2]; o[4] = k[1]; o[5] = k[3];
+            } else {
+                o[2] = o[3] = o[4] = o[5] = 0.0;
+            }
+        }
+    }
+    return TB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tb_ensemble_create(tb_ensemble **out, int32_t n, const tb_config *cfgs)
+{
+    if (!out || !cfgs || n < 1) return fail(TB_ERR_ARG, "bad argument");
+    *out = nullptr;
+    for (int r = 0; r < n; r++) {
+        int rc = check_cfg(&cfgs[r]);
+        if (rc) return rc;
+        if (cfgs[r].walk != TB_WALK_MIXED) return fail(TB_ERR_ARG, "ensemble realisations use the mixed walk");
+        if (cfgs[r].device != cfgs[0].device) return fail(TB_ERR_ARG, "ensemble realisations share one device");
+    }
+    tb_ensemble *E = new tb_ensemble();
+    E->cfg = cfgs[0];
+    E->cfg.mac = TB_MAC_FIXED;
+    for (int r = 0; r < n; r++) {
+        tb_handle *h = nullptr;
+        int rc = tb_create(&h, &cfgs[r]);
+        if (rc) {
+            std::string msg = g_err;
+            tb_ensemble_destroy(E);
+            return fail(rc, "%s", msg.c_str());
+        }
+        E->h.push_back(h);
+    }
+    // one stream for the whole ensemble
+    E->s = E->h[0]->s;
+    for (int r = 1; r < n; r++) {
+        hipStreamDestroy(E->h[r]->s);
+        E->h[r]->s = E->s;
+    }
+    bool ok = hipMalloc((void **)&E->mx.ctr, 64) == hipSuccess &&
+              hipMalloc((void **)&E->d_esum, (size_t)4 * n * sizeof(double)) == hipSuccess &&
+              hipHostMalloc((void **)&E->h_buf, (size_t)8 * n * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        tb_ensemble_destroy(E);
+        return fail(TB_ERR_NOMEM, "ensemble allocation failed");
+    }
+    hipMemset(E->mx.ctr, 0, 64);
+    *out = E;
+    return TB_OK;
+}
+
+int tb_ensemble_destroy(tb_ensemble *E)
+{
+    if (!E) return TB_OK;
+    if (E->s) hipStreamSynchronize(E->s);
+    // every member shares member 0's stream: only that one is destroyed with its handle
+    for (size_t r = 1; r < E->h.size(); r++) E->h[r]->s = nullptr;
+    for (auto *h : E->h) tb_destroy(h);
+    size_t b = 0;
+    tb_mixed_free(E->mx, &b);
+    if (E->d_esum) hipFree(E->d_esum);
+    if (E->h_buf) hipHostFree(E->h_buf);
+    delete E;
+    return TB_OK;
+}
+
+int tb_ensemble_set_particles(tb_ensemble *E, int32_t r, int32_t ne, const tb_pt *e, int32_t ni, const tb_pt *i)
+{
+    if (!E || r < 0 || r >= (int)E->h.size()) return fail(TB_ERR_ARG, "bad realisation index");
+    E->have_force = false;
+    return tb_set_particles(E->h[r], ne, e, ni, i);
+}
+
+int tb_ensemble_get_particles(tb_ensemble *E, int32_t r, tb_pt *e, tb_pt *i)
+{
+    if (!E || r < 0 || r >= (int)E->h.size()) return fail(TB_ERR_ARG, "bad realisation index");
+    return tb_get_particles(E->h[r], e, i);
+}
+
+int tb_ensemble_force(tb_ensemble *E, double *out)
+{
+    if (!E) return fail(TB_ERR_ARG, "null ensemble");
+    for (auto *h : E->h)
+        if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "a realisation has no particles");
+    HIPCHK(hipSetDevice(E->cfg.device));
+    int rc;
+    if ((rc = ens_build_all(E))) return rc;
+    if ((rc = ens_force(E))) return rc;
+    E->have_force = true;
+    std::vector<double> o(6 * E->h.size());
+    if ((rc = ens_finish(E, o.data(), false))) return rc;
+    if (out)
+        for (size_t r = 0; r < E->h.size(); r++) {
+            out[2 * r] = o[6 * r];
+            out[2 * r + 1] = o[6 * r + 1];
+        }
+    return TB_OK;
+}
+
+int tb_ensemble_step(tb_ensemble *E, double dt, int32_t nsteps, double *out)
+{
+    if (!E || nsteps < 1) return fail(TB_ERR_ARG, "bad argument");
+    if (!E->have_force) return fail(TB_ERR_STATE, "tb_ensemble_step needs forces: call tb_ensemble_force first");
+    HIPCHK(hipSetDevice(E->cfg.device));
+    int rc;
+    for (auto *h : E->h)
+        for (int sp = 0; sp < 2; sp++) tb_vv1(E->s, h->sp[sp], h->cfg.xm[sp], dt);
+    for (int k = 0; k < nsteps; k++) {
+        if ((rc = ens_build_all(E))) return rc;
+        if ((rc = ens_force(E))) return rc;
+        const bool last = k == nsteps - 1;
+        for (auto *h : E->h)
+            if ((rc = vv2_all(h, dt, last && out ? 1 : 0, !last))) return rc;
+    }
+    for (auto *h : E->h) h->built = false;
+    return ens_finish(E, out, out != nullptr);
+}
+
+int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
+{
+    if (!E || !E->have_force) return fail(TB_ERR_STATE, "tb_ensemble_autotune needs tb_ensemble_force first");
+    HIPCHK(hipSetDevice(E->cfg.device));
+    const int saved = E->cfg.group;
+    float best = 1e30f;
+    int best_tpl = 2, ntot = 0;
+    for (auto *h : E->h) ntot += h->sp[0].n + h->sp[1].n;
+    hipEvent_t a = E->h[0]->ev[4], b = E->h[0]->ev[5];
+    for (int tpl = 1; tpl <= 4; tpl *= 2) {
+        E->cfg.group = 64 * tpl;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 3; rep++) {
+            HIPCHK(hipEventRecord(a, E->s));
+            int rc = ens_force(E);
+            if (rc) {
+                E->cfg.group = saved;
+                return rc;
+            }
+            HIPCHK(hipEventRecord(b, E->s));
+            HIPCHK(hipEventSynchronize(b));
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, a, b));
+            if (rep > 0) ms += t;
+        }
+        ms *= 0.5f;
+        if (ms_out) ms_out[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)] = ms;
+        if (ms < best) {
+            best = ms;
+            best_tpl = tpl;
+        }
+    }
+    E->cfg.group = saved;
+    tb_autotune_record(ntot / (int)E->h.size(), best_tpl);
     if (group_out) *group_out = 64 * best_tpl;
     return TB_OK;
 }

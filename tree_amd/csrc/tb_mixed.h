@@ -24,12 +24,30 @@ struct BandRec {
     float f[TB_MLP_NIN];
 };
 
+// one realisation (electron + ion trees and particle state) of a walk launch
+struct Domain {
+    const double *px[2], *py[2], *pz[2];
+    double *fx[2], *fy[2], *fz[2];
+    int n[2], ng[2];
+    const double4 *bmac[2];
+    const uint32_t *binfo[2];
+    const float4 *bq0[2], *bq1[2];
+    const float2 *bsd[2];
+    float inv_rs, rs;
+    int gbase;        // first global group index of this domain
+    int pad_;
+};
+
 struct MixedDev {
     MixedTree t[2];
     int32_t *ctr = nullptr;       // work-queue counter (+ error flag)
     double *part = nullptr;       // per-wave energy partials, 4 per wave
     int part_n = 0;
     int tpl = 0;                  // targets per lane used by the last launch
+    Domain *ddom = nullptr;       // device copy of the launch's domain descriptors
+    int ddom_cap = 0;
+    double *esum = nullptr;       // 4 energy sums (ee, ei, ie, ii) per domain
+    int esum_cap = 0;
     // the learned opening criterion
     MlpDev mlp;
     BandRec *rec = nullptr;       // collected band pairs
@@ -47,6 +65,11 @@ void tb_mixed_free(MixedDev &M, size_t *bytes);
 int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int cap,
                      float *X, float *y, int *nout);
 const char *tb_mixed_error();
+
+// one persistent launch over several domains (the batched ensemble): fills
+// esum[4 * d] per domain; rec/collect are not used
+int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees, int ndom,
+                         const tb_config &cfg, double *esum_out_dev);
 
 // walk-group size chosen by the autotuner for a problem size (targets per lane: 1, 2 or 4)
 int tb_autotune_tpl(int n_total);

@@ -117,21 +117,15 @@ __device__ inline void kelbg_gh(float u, float &g, float &h)
 }
 
 struct WalkArgs {
-    const double *px[2], *py[2], *pz[2];
-    double *fx[2], *fy[2], *fz[2];
-    int n[2], ng[2];
-    const double4 *bmac[2];
-    const uint32_t *binfo[2];
-    const float4 *bq0[2], *bq1[2];
-    const float2 *bsd[2];
-    float inv_rs, rs;
+    const Domain *dom;    // ndom descriptors, ordered by gbase
+    int ndom, ngt;        // domains, groups in all domains
     float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo + dx are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
     const float *mlpw;    // learned: bf16-rounded parameters
     BandRec *rec;         // collect: band pairs
     int rec_cap;
     int32_t *ctr;         // [0] work counter, [1] error flag, [2] prep error, [3] band pair count
-    double *part;         // 4 per wave: ee, ei, ie, ii
+    double *esum;         // 4 per domain: ee, ei, ie, ii
     unsigned long long *cnt;
 };
 
@@ -252,19 +246,28 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
     const int lane = lane_id();
     WaveLDS &L = lds[threadIdx.x >> 6];
     const int G = 64 * TPL;
-    double eacc[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned long long c_visit[4] = {0, 0, 0, 0}, c_m2p[4] = {0, 0, 0, 0}, c_p2p[4] = {0, 0, 0, 0};
-    const int ngt = W.ng[0] + W.ng[1];
 
     for (;;) {
         int g = 0;
         if (lane == 0) g = atomicAdd(&W.ctr[0], 1);
         g = __shfl(g, 0, 64);
-        if (g >= ngt) break;
-        const int tsp = g < W.ng[0] ? 0 : 1;
-        const int base = (tsp ? g - W.ng[0] : g) * G;
-        const int n = W.n[tsp];
-        const double *px = W.px[tsp], *py = W.py[tsp], *pz = W.pz[tsp];
+        if (g >= W.ngt) break;
+        // the domain holding global group g (descriptors ordered by gbase)
+        int lo = 0, hi = W.ndom - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (W.dom[mid].gbase <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const int di = lo;
+        const Domain &D = W.dom[di];
+        const int gl = g - D.gbase;
+        const int tsp = gl < D.ng[0] ? 0 : 1;
+        const int base = (tsp ? gl - D.ng[0] : gl) * G;
+        const int n = D.n[tsp];
+        const double *px = D.px[tsp], *py = D.py[tsp], *pz = D.pz[tsp];
+        double eg[2] = {0.0, 0.0};
 
         // targets and the group's bounding box (FP64)
         double tx[TPL], ty[TPL], tz[TPL];
@@ -307,10 +310,10 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             const int pr = tsp * 2 + tree;  // ee, ei, ie, ii
             const bool like = tree == tsp;
             const float qs = tree ? 1.0f : -1.0f;
-            const double4 *bmac = W.bmac[tree];
-            const uint32_t *binfo = W.binfo[tree];
-            const float4 *bq0 = W.bq0[tree], *bq1 = W.bq1[tree];
-            const float2 *bsd = W.bsd[tree];
+            const double4 *bmac = D.bmac[tree];
+            const uint32_t *binfo = D.binfo[tree];
+            const float4 *bq0 = D.bq0[tree], *bq1 = D.bq1[tree];
+            const float2 *bsd = D.bsd[tree];
 #pragma unroll
             for (int t = 0; t < TPL; t++) A.ps[t] = A.pc[t] = 0.0f;
             int sp = 1, nm = 0, np = 0;
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                     const float2 sd = bsd[node];
                     const float4 q0 = bq0[node], q1 = bq1[node];
                     float f[TB_MLP_NIN];
-                    band_features(sd.x, sd.y, sqrtf(d2), q0.x, q0, q1, hx, hy, hz, W.rs, like, nact, f);
+                    band_features(sd.x, sd.y, sqrtf(d2), q0.x, q0, q1, hx, hy, hz, D.rs, like, nact, f);
                     if (MODE == MODE_LEARNED) {
                         acc = tb_mlp_eval(wl, f) < W.log_err;
                         open = !acc;
@@ -427,8 +430,8 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                 }
                 if (np >= FLUSH) {
                     if (COUNT) c_p2p[pr] += (unsigned long long)np * nact;
-                    if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, W.inv_rs, A);
-                    else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, W.inv_rs, A);
+                    if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, D.inv_rs, A);
+                    else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, D.inv_rs, A);
                     np = 0;
                     wave_sync();
                 }
@@ -439,15 +442,15 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             }
             if (np > 0) {
                 if (COUNT) c_p2p[pr] += (unsigned long long)np * nact;
-                if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, W.inv_rs, A);
-                else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, W.inv_rs, A);
+                if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, D.inv_rs, A);
+                else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, D.inv_rs, A);
             }
             wave_sync();
             // energy of this pairing: 0.5 * q_t * eps * phi (FP64 from here on)
             const double qt = tsp ? 1.0 : -1.0;
 #pragma unroll
             for (int t = 0; t < TPL; t++)
-                if (act[t]) eacc[tsp * 2 + tree] += 0.5 * qt * TB_EPS * ((double)A.ps[t] + (double)A.pc[t]);
+                if (act[t]) eg[tree] += 0.5 * qt * TB_EPS * ((double)A.ps[t] + (double)A.pc[t]);
         }
         // forces: F = q_t * eps * E
         const double ff = (tsp ? 1.0 : -1.0) * TB_EPS;
@@ -455,16 +458,16 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
         for (int t = 0; t < TPL; t++) {
             if (!act[t]) continue;
             const int i = base + lane + 64 * t;
-            W.fx[tsp][i] = ff * ((double)A.s[t][0] + (double)A.c[t][0]);
-            W.fy[tsp][i] = ff * ((double)A.s[t][1] + (double)A.c[t][1]);
-            W.fz[tsp][i] = ff * ((double)A.s[t][2] + (double)A.c[t][2]);
+            D.fx[tsp][i] = ff * ((double)A.s[t][0] + (double)A.c[t][0]);
+            D.fy[tsp][i] = ff * ((double)A.s[t][1] + (double)A.c[t][1]);
+            D.fz[tsp][i] = ff * ((double)A.s[t][2] + (double)A.c[t][2]);
         }
-    }
-    const int gw = blockIdx.x * WPB + (threadIdx.x >> 6);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const double v = wave_sum(eacc[k]);
-        if (lane == 0) W.part[gw * 4 + k] = v;
+        // group energies into the domain's pairing sums (FP64)
+        const double e0 = wave_sum(eg[0]), e1 = wave_sum(eg[1]);
+        if (lane == 0) {
+            atomicAdd(&W.esum[di * 4 + tsp * 2], e0);
+            atomicAdd(&W.esum[di * 4 + tsp * 2 + 1], e1);
+        }
     }
     if (COUNT && lane == 0) {
         for (int pr = 0; pr < 4; pr++) {
@@ -490,7 +493,8 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
                                                     const double *__restrict__ nmom, const double *__restrict__ nm2,
                                                     const double *__restrict__ ndx,
                                                     const int32_t *__restrict__ nstart,
-                                                    const int32_t *__restrict__ ndepth, double smax0, double theta,
+                                                    const int32_t *__restrict__ ndepth,
+                                                    const double *__restrict__ root, double theta,
                                                     double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
                                                     float4 *__restrict__ bq0, float4 *__restrict__ bq1,
                                                     float2 *__restrict__ bsd, int32_t *__restrict__ err)
@@ -520,6 +524,7 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
     }
     binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nc - 1);
     // the cell size max(lx) halves exactly with each level: max(L0) * 2^-depth
+    const double smax0 = fmax(root[3], fmax(root[4], root[5]));
     const double s = ldexp(smax0, -ndepth[p]);
     bmac[b] = make_double4(m.x, m.y, m.z, s / theta + ndx[p]);
     bsd[b] = make_float2((float)s, (float)ndx[p]);
@@ -535,16 +540,15 @@ __global__ void k_zero_ctr(int32_t *ctr, int lo, int hi)
     if (i < hi) ctr[i] = 0;
 }
 
-__global__ __launch_bounds__(64) void k_reduce_part(int nw, const double *__restrict__ part,
-                                                    const int32_t *__restrict__ ctr, double *__restrict__ out)
+__global__ void k_zero_d(double *p, int n)
 {
-    // deterministic per-column sums (fixed order per lane, then a fixed tree)
-    for (int c = 0; c < 4; c++) {
-        double s = 0.0;
-        for (int w = threadIdx.x; w < nw; w += 64) s += part[w * 4 + c];
-        s = wave_sum(s);
-        if (threadIdx.x == 0) out[c] = s;
-    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+// a single domain's pairing sums and the walk's error flag into the handle's sums
+__global__ void k_finish(const double *__restrict__ esum, const int32_t *__restrict__ ctr, double *__restrict__ out)
+{
+    if (threadIdx.x < 4) out[threadIdx.x] = esum[threadIdx.x];
     if (threadIdx.x == 0) out[15] = (double)(ctr[1] | ctr[2]);
 }
 
@@ -682,23 +686,59 @@ void dispatch(hipStream_t s, int tpl, int blocks, const WalkArgs &W)
     }
 }
 
-int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int64_t *d_cnt,
-           int mode)
+Domain make_domain(SpeciesDev *const sp[2], const MixedTree *const t[2], const tb_config &cfg, int G, int gbase)
 {
-    const int tpl = pick_tpl(cfg, sp[0].n + sp[1].n);
-    const int G = 64 * tpl;
-    WalkArgs W;
+    Domain d;
     for (int k = 0; k < 2; k++) {
-        const int c = sp[k].cur;
-        W.px[k] = sp[k].x[c][0]; W.py[k] = sp[k].x[c][1]; W.pz[k] = sp[k].x[c][2];
-        W.fx[k] = sp[k].f[c][0]; W.fy[k] = sp[k].f[c][1]; W.fz[k] = sp[k].f[c][2];
-        W.n[k] = sp[k].n;
-        W.ng[k] = (sp[k].n + G - 1) / G;
-        W.bmac[k] = M.t[k].bmac; W.binfo[k] = M.t[k].binfo; W.bq0[k] = M.t[k].bq0; W.bq1[k] = M.t[k].bq1;
-        W.bsd[k] = M.t[k].bsd;
+        const int c = sp[k]->cur;
+        d.px[k] = sp[k]->x[c][0]; d.py[k] = sp[k]->x[c][1]; d.pz[k] = sp[k]->x[c][2];
+        d.fx[k] = sp[k]->f[c][0]; d.fy[k] = sp[k]->f[c][1]; d.fz[k] = sp[k]->f[c][2];
+        d.n[k] = sp[k]->n;
+        d.ng[k] = (sp[k]->n + G - 1) / G;
+        d.bmac[k] = t[k]->bmac; d.binfo[k] = t[k]->binfo; d.bq0[k] = t[k]->bq0; d.bq1[k] = t[k]->bq1;
+        d.bsd[k] = t[k]->bsd;
     }
-    W.inv_rs = (float)(1.0 / cfg.rs);
-    W.rs = (float)cfg.rs;
+    d.inv_rs = (float)(1.0 / cfg.rs);
+    d.rs = (float)cfg.rs;
+    d.gbase = gbase;
+    d.pad_ = 0;
+    return d;
+}
+
+// Uploads the domain descriptors (synchronously: they change only after a
+// build, which leaves the stream idle) and launches one persistent walk.
+int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, int tpl, const tb_config &cfg,
+                   int64_t *d_cnt, int mode)
+{
+    const int nd = (int)doms.size();
+    if (nd > M.ddom_cap) {
+        if (M.ddom) hipFree(M.ddom);
+        M.ddom = nullptr;
+        M.ddom_cap = 0;
+        if (hipMalloc((void **)&M.ddom, (size_t)nd * sizeof(Domain)) != hipSuccess) {
+            g_err = "mixed walk: descriptor allocation failed";
+            return TB_ERR_NOMEM;
+        }
+        M.ddom_cap = nd;
+    }
+    if (4 * nd > M.esum_cap) {
+        if (M.esum) hipFree(M.esum);
+        M.esum = nullptr;
+        M.esum_cap = 0;
+        if (hipMalloc((void **)&M.esum, (size_t)4 * nd * sizeof(double)) != hipSuccess) {
+            g_err = "mixed walk: energy sums allocation failed";
+            return TB_ERR_NOMEM;
+        }
+        M.esum_cap = 4 * nd;
+    }
+    if (hipMemcpy(M.ddom, doms.data(), (size_t)nd * sizeof(Domain), hipMemcpyHostToDevice) != hipSuccess) {
+        g_err = "mixed walk: descriptor upload failed";
+        return TB_ERR_HIP;
+    }
+    WalkArgs W;
+    W.dom = M.ddom;
+    W.ndom = nd;
+    W.ngt = doms.back().gbase + doms.back().ng[0] + doms.back().ng[1];
     const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : (cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta);
     W.inv_theta_lo = (float)(1.0 / th_lo);
     W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
@@ -706,23 +746,13 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
     W.rec = M.rec;
     W.rec_cap = M.rec_cap;
     W.ctr = M.ctr;
+    W.esum = M.esum;
     W.cnt = (unsigned long long *)d_cnt;
-    const int ngt = W.ng[0] + W.ng[1];
-    int waves = std::min(ngt, resident_waves());
-    int blocks = nblk(waves, WPB);
-    waves = blocks * WPB;
-    if (waves > M.part_n) {
-        if (M.part) hipFree(M.part);
-        M.part = nullptr;
-        if (hipMalloc((void **)&M.part, (size_t)waves * 4 * sizeof(double)) != hipSuccess) {
-            g_err = "mixed walk: partials allocation failed";
-            return TB_ERR_NOMEM;
-        }
-        M.part_n = waves;
-    }
-    W.part = M.part;
+    const int waves = std::min(W.ngt, resident_waves());
+    const int blocks = nblk(waves, WPB);
     M.tpl = tpl;
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, M.esum, 4 * nd);
     if (mode == MODE_COLLECT) {
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
         dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
@@ -733,12 +763,24 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
         if (d_cnt) dispatch<MODE_FIXED, true>(s, tpl, blocks, W);
         else dispatch<MODE_FIXED, false>(s, tpl, blocks, W);
     }
-    hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(64), 0, s, waves, M.part, M.ctr, d_sums);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: launch failed";
         return TB_ERR_HIP;
     }
     return TB_OK;
+}
+
+int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int64_t *d_cnt,
+           int mode)
+{
+    const int tpl = pick_tpl(cfg, sp[0].n + sp[1].n);
+    SpeciesDev *const spp[2] = {&sp[0], &sp[1]};
+    const MixedTree *const tp[2] = {&M.t[0], &M.t[1]};
+    std::vector<Domain> doms(1, make_domain(spp, tp, cfg, 64 * tpl, 0));
+    int rc = launch_domains(s, M, doms, tpl, cfg, d_cnt, mode);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, M.esum, M.ctr, d_sums);
+    return hipGetLastError() == hipSuccess ? TB_OK : TB_ERR_HIP;
 }
 
 int walk_mode(const MixedDev &M, const tb_config &cfg)
@@ -773,17 +815,9 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &c
         if ((rc = grow(&T.bsd, &c6, S.nn, bytes))) return rc;
         T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(c5, c6));
         T.nn = S.nn;
-        double root[6];
-        // max root length; the root buffer is ready once the build has synchronised
-        if (hipMemcpyAsync(root, S.root, sizeof(root), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            g_err = "mixed walk: root read failed";
-            return TB_ERR_HIP;
-        }
-        const double smax0 = std::max(root[3], std::max(root[4], root[5]));
         hipLaunchKernelGGL(k_inv, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv);
         hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv, S.nmac, S.nnext,
-                           S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, smax0, theta, T.bmac, T.binfo, T.bq0, T.bq1,
+                           S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, S.root, theta, T.bmac, T.binfo, T.bq0, T.bq1,
                            T.bsd, M.ctr + 2);
     }
     if (hipGetLastError() != hipSuccess) {
@@ -867,6 +901,32 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config
     return TB_OK;
 }
 
+int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees, int ndom,
+                         const tb_config &cfg, double *esum_out_dev)
+{
+    int ntot = 0;
+    for (int d = 0; d < ndom; d++) ntot += sp[2 * d]->n + sp[2 * d + 1]->n;
+    const int tpl = pick_tpl(cfg, ntot / std::max(ndom, 1));
+    std::vector<Domain> doms;
+    doms.reserve(ndom);
+    int gb = 0;
+    for (int d = 0; d < ndom; d++) {
+        SpeciesDev *const spp[2] = {sp[2 * d], sp[2 * d + 1]};
+        const MixedTree *const tp[2] = {trees[2 * d], trees[2 * d + 1]};
+        doms.push_back(make_domain(spp, tp, cfg, 64 * tpl, gb));
+        gb += doms.back().ng[0] + doms.back().ng[1];
+    }
+    int rc = launch_domains(s, M, doms, tpl, cfg, nullptr, MODE_FIXED);
+    if (rc) return rc;
+    if (esum_out_dev &&
+        hipMemcpyAsync(esum_out_dev, M.esum, (size_t)4 * ndom * sizeof(double), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess) {
+        g_err = "mixed walk: energy copy failed";
+        return TB_ERR_HIP;
+    }
+    return TB_OK;
+}
+
 void tb_mixed_free(MixedDev &M, size_t *bytes)
 {
     for (int k = 0; k < 2; k++) {
@@ -876,6 +936,11 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
     }
     hipFree(M.ctr);
     hipFree(M.part);
+    if (M.ddom) hipFree(M.ddom);
+    if (M.esum) hipFree(M.esum);
+    M.ddom = nullptr;
+    M.esum = nullptr;
+    M.ddom_cap = M.esum_cap = 0;
     if (M.rec) hipFree(M.rec);
     if (M.lab) hipFree(M.lab);
     tb_mlp_free(M.mlp);

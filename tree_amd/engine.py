@@ -101,6 +101,13 @@ SIGNATURES = [
     ("tb_mlp_train", ctypes.c_int, [_P, _I, _P, _P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
     ("tb_mlp_collect", ctypes.c_int, [_P, _I, _P, _P, _P]),
     ("tb_autotune", ctypes.c_int, [_P, _P, _P]),
+    ("tb_ensemble_create", ctypes.c_int, [ctypes.POINTER(_P), _I, _P]),
+    ("tb_ensemble_destroy", ctypes.c_int, [_P]),
+    ("tb_ensemble_set_particles", ctypes.c_int, [_P, _I, _I, _P, _I, _P]),
+    ("tb_ensemble_get_particles", ctypes.c_int, [_P, _I, _P, _P]),
+    ("tb_ensemble_force", ctypes.c_int, [_P, _P]),
+    ("tb_ensemble_step", ctypes.c_int, [_P, _D, _I, _P]),
+    ("tb_ensemble_autotune", ctypes.c_int, [_P, _P, _P]),
     ("tb_last_error", ctypes.c_char_p, []),
     ("tb_abi_sizes", ctypes.c_int, [_P, _P, _P]),
     ("tb_kelbg_length", ctypes.c_double, [_D, _D, _D]),
@@ -305,6 +312,74 @@ class Engine:
         s = Stats()
         _check(self.lib.tb_get_stats(self.h, ctypes.byref(s)))
         return _stats_dict(s)
+
+
+def make_config(rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed", theta_lo=0.0,
+                err_target=0.0, group=0, device=0, mem_cap=0):
+    c = Config()
+    c.xm[0], c.xm[1] = xm
+    c.rs, c.theta, c.theta_mixed, c.theta_lo, c.err_target = rs, theta, theta_mixed, theta_lo, err_target
+    c.walk, c.mac, c.group, c.device, c.mem_cap = WALK[walk], MAC[mac], group, device, mem_cap
+    return c
+
+
+class Ensemble:
+    """Independent realisations stepped together: one build read-back and one
+    persistent mixed-walk launch per step (tb_ensemble_*)."""
+
+    def __init__(self, configs):
+        self.lib = load_library()
+        arr = (Config * len(configs))(*configs)
+        h = ctypes.c_void_p()
+        _check(self.lib.tb_ensemble_create(ctypes.byref(h), len(configs), ctypes.cast(arr, ctypes.c_void_p)))
+        self.h = h
+        self.n = len(configs)
+        self.sizes = [(0, 0)] * self.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tb_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_particles(self, r, qel, qion):
+        qel = np.ascontiguousarray(qel, dtype=PT_DTYPE)
+        qion = np.ascontiguousarray(qion, dtype=PT_DTYPE)
+        _check(self.lib.tb_ensemble_set_particles(self.h, r, len(qel), _ptr(qel), len(qion), _ptr(qion)))
+        self.sizes[r] = (len(qel), len(qion))
+
+    def get_particles(self, r):
+        qel = np.zeros(self.sizes[r][0], PT_DTYPE)
+        qion = np.zeros(self.sizes[r][1], PT_DTYPE)
+        _check(self.lib.tb_ensemble_get_particles(self.h, r, _ptr(qel), _ptr(qion)))
+        return qel, qion
+
+    def force(self):
+        ep = np.zeros((self.n, 2))
+        _check(self.lib.tb_ensemble_force(self.h, _ptr(ep)))
+        return ep
+
+    def step(self, dt, nsteps=1):
+        out = np.zeros((self.n, 6))
+        _check(self.lib.tb_ensemble_step(self.h, dt, nsteps, _ptr(out)))
+        return out
+
+    def autotune(self):
+        g = ctypes.c_int32()
+        ms = np.zeros(3, np.float32)
+        _check(self.lib.tb_ensemble_autotune(self.h, ctypes.byref(g), _ptr(ms)))
+        return g.value, ms
 
 
 def _stats_dict(s):
