@@ -30,9 +30,9 @@ import tree_amd as ta  # noqa: E402
 # algorithmic flop weights per event (FMA = 2): MAC test, M2P on accept, P2P Coulomb / Kelbg
 W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # mixed walk, per target-entry interaction, counted from walk32.hip's FP32 code
-# (traceless-quadrupole M2P 54, Coulomb P2P 18, Kelbg P2P 30); MAC tests are
-# per cell and group and are not counted
-W32_M2P, W32_P2P_C, W32_P2P_K = 54, 18, 30
+# (monopole + dipole + traceless-quadrupole M2P 72, Coulomb P2P 18, Kelbg P2P
+# 30); MAC tests are per cell and group and are not counted
+W32_M2P, W32_P2P_C, W32_P2P_K = 72, 18, 30
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X vector
 
 

@@ -43,6 +43,7 @@ struct tb_handle {
     tb_config cfg{};
     hipStream_t s = nullptr;
     SpeciesDev sp[2];
+    JointDev jt;                   // mixed walk: one tree over both species
     MixedDev mx;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -58,6 +59,7 @@ struct tb_handle {
     int64_t *d_cnt = nullptr;      // 12 counters
     size_t bytes = 0;              // device bytes held
     bool built = false, have_force = false;
+    int tree_walk = -1;            // which walk the current trees were built for
     double epot[2] = {0, 0};
     tb_stats st{};
     hipEvent_t ev[6] = {};
@@ -161,6 +163,67 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     return TB_OK;
 }
 
+void free_joint(tb_handle *h, JointDev &J)
+{
+    size_t n = (size_t)J.n, c = (size_t)J.cap;
+    for (int b = 0; b < 2; b++) {
+        dfree(h, J.key[b], n * 8);
+        dfree(h, J.val[b], n * 4);
+        dfree(h, J.dkey[b], c * 8);
+        dfree(h, J.dval[b], c * 4);
+    }
+    dfree(h, J.pos, n * 32);
+    dfree(h, J.cnt, n * 4);
+    dfree(h, J.offs, n * 4);
+    dfree(h, J.root, 64 * 8);
+    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 80);
+    dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
+    J = JointDev();
+}
+
+int ensure_joint(tb_handle *h, int n)
+{
+    JointDev &J = h->jt;
+    if (J.n == n && J.key[0]) return TB_OK;
+    free_joint(h, J);
+    J.n = n;
+    int rc;
+    for (int b = 0; b < 2; b++) {
+        if ((rc = dalloc_t(h, &J.key[b], n))) return rc;
+        if ((rc = dalloc_t(h, &J.val[b], n))) return rc;
+    }
+    if ((rc = dalloc_t(h, &J.pos, n))) return rc;
+    if ((rc = dalloc_t(h, &J.cnt, n))) return rc;
+    if ((rc = dalloc_t(h, &J.offs, n))) return rc;
+    if ((rc = dalloc_t(h, &J.root, 64))) return rc;
+    return TB_OK;
+}
+
+int ensure_joint_nodes(tb_handle *h, int nn)
+{
+    JointDev &J = h->jt;
+    if (nn <= J.cap) return TB_OK;
+    size_t c = (size_t)J.cap;
+    for (int b = 0; b < 2; b++) { dfree(h, J.dkey[b], c * 8); dfree(h, J.dval[b], c * 4); }
+    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 80);
+    dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
+    int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
+    J.cap = 0;
+    int rc;
+    if ((rc = dalloc_t(h, &J.nc, cap))) return rc;
+    if ((rc = dalloc_t(h, &J.nnext, cap))) return rc;
+    if ((rc = dalloc_t(h, &J.nq, (size_t)cap * 10))) return rc;
+    if ((rc = dalloc_t(h, &J.nstart, cap))) return rc;
+    if ((rc = dalloc_t(h, &J.ncount, cap))) return rc;
+    if ((rc = dalloc_t(h, &J.ndepth, cap))) return rc;
+    for (int b = 0; b < 2; b++) {
+        if ((rc = dalloc_t(h, &J.dkey[b], cap))) return rc;
+        if ((rc = dalloc_t(h, &J.dval[b], cap))) return rc;
+    }
+    J.cap = cap;
+    return TB_OK;
+}
+
 int ensure_tmp(tb_handle *h, int n)
 {
     int rc;
@@ -205,12 +268,17 @@ int check_cfg(const tb_config *c)
 // BuildLocalTree for both species, in two phases around one small host
 // read-back (node totals, depth histograms, error flags).  An ensemble runs
 // phase A for every realisation, synchronises once, then runs phase B.
+constexpr int MISC_J = MISC_HIST + 2 * 2 * HW;  // joint tree: err, total, 2 x HW histograms
+
 int build_phase_a(tb_handle *h, int *which)
 {
     hipStream_t s = h->s;
     int rc;
-    if ((rc = ensure_tmp(h, std::max(h->sp[0].n, h->sp[1].n)))) return rc;
-    HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * (MISC_HIST + 2 * 2 * HW), s));
+    const bool mixed = h->cfg.walk == TB_WALK_MIXED;
+    const int nj = h->sp[0].n + h->sp[1].n;
+    if ((rc = ensure_tmp(h, mixed ? nj : std::max(h->sp[0].n, h->sp[1].n)))) return rc;
+    if (mixed && (rc = ensure_joint(h, nj))) return rc;
+    HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * (MISC_J + 2 + 2 * HW), s));
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
         const int c = S.cur;
@@ -218,12 +286,30 @@ int build_phase_a(tb_handle *h, int *which)
         tb_keys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
         tb_radix_sort(s, S.n, S.key, S.val, 0, 64, h->sort_tmp, &which[sp], false);
         tb_permute(s, S, S.val[which[sp]], h->d_misc + sp);
-        tb_lcp(s, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
-        tb_scan_excl(s, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, h->scan_tmp);
+        if (!mixed) {
+            tb_lcp(s, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
+            tb_scan_excl(s, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, h->scan_tmp);
+        }
+    }
+    if (mixed) {
+        // joint tree over both species, on the freshly permuted positions
+        JointDev &J = h->jt;
+        SpeciesDev &E = h->sp[0], &I = h->sp[1];
+        const int ce = E.cur, ci = I.cur;
+        tb_jbbox(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root,
+                 h->bbox_tmp);
+        tb_jkeys(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], J.root, J.key[0], J.val[0], 0u, h->d_misc + MISC_J);
+        tb_jkeys(s, I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root, J.key[0] + E.n, J.val[0] + E.n, 0x80000000u,
+                 h->d_misc + MISC_J);
+        tb_radix_sort(s, nj, J.key, J.val, 0, 64, h->sort_tmp, &which[2], false);
+        const double *px[2] = {E.x[ce][0], I.x[ci][0]}, *py[2] = {E.x[ce][1], I.x[ci][1]},
+                     *pz[2] = {E.x[ce][2], I.x[ci][2]};
+        tb_jgather(s, nj, J.val[which[2]], px, py, pz, J.pos);
+        tb_lcp(s, nj, J.key[which[2]], J.cnt, h->d_misc + MISC_J + 2, h->d_misc + MISC_J);
+        tb_scan_excl(s, nj, J.cnt, J.offs, h->d_misc + MISC_J + 1, h->scan_tmp);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, sizeof(int32_t) * (MISC_HIST + 2 * 2 * HW), hipMemcpyDeviceToHost,
-                          s));
+    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, sizeof(int32_t) * (MISC_J + 2 + 2 * HW), hipMemcpyDeviceToHost, s));
     return TB_OK;
 }
 
@@ -232,13 +318,48 @@ int build_phase_b(tb_handle *h, const int *which)
 {
     hipStream_t s = h->s;
     int rc;
-    for (int sp = 0; sp < 2; sp++) {
-        int e = h->h_misc[sp];
+    const bool mixed = h->cfg.walk == TB_WALK_MIXED;
+    for (int sp = 0; sp < 3; sp++) {
+        int e = sp < 2 ? h->h_misc[sp] : (mixed ? h->h_misc[MISC_J] : 0);
         if (e & 4) return fail(TB_ERR_HIP, "species %d: internal sort check failed", sp);
-        if (e & 2) return fail(TB_ERR_NONFINITE, "non-finite position in species %d", sp);
+        if (e & 2) return fail(TB_ERR_NONFINITE, "non-finite position in species %d", sp % 2);
         if (e & 1)
-            return fail(TB_ERR_DEPTH, "species %d: two particles share all %d key levels (coincident)", sp,
-                        TB_LEVELS);
+            return fail(TB_ERR_DEPTH, "%s: two particles share all %d key levels (coincident)",
+                        sp == 0 ? "electrons" : (sp == 1 ? "ions" : "joint tree"), TB_LEVELS);
+    }
+    if (mixed) {
+        JointDev &J = h->jt;
+        const int32_t *hist = h->h_misc + MISC_J + 2;
+        J.nn = 1 + h->h_misc[MISC_J + 1];
+        if ((rc = ensure_joint_nodes(h, J.nn))) return rc;
+        if ((rc = ensure_tmp(h, std::max(J.nn, J.n)))) return rc;
+        int all[HW + 1];
+        for (int d = 0; d < HW; d++) {
+            J.twig_hist[d] = hist[d] + (d == 0 ? 1 : 0);
+            all[d] = hist[d] + hist[HW + d] + (d == 0 ? 1 : 0);
+        }
+        J.lvl_off[0] = 0;
+        J.maxdepth = 0;
+        for (int d = 0; d < HW; d++) {
+            J.lvl_off[d + 1] = J.lvl_off[d] + all[d];
+            if (all[d]) J.maxdepth = d;
+        }
+        const uint64_t *key = J.key[which[2]];
+        tb_jemit(s, J, key);
+        int w2;
+        tb_radix_sort(s, J.nn, J.dkey, J.dval, 0, 8, h->sort_tmp, &w2, true);
+        J.bfs = J.dval[w2];
+        tb_jupward(s, J, key, h->d_misc);
+        h->st.nodes[0] = J.nn;
+        h->st.nodes[1] = 0;
+        h->st.max_depth[0] = J.maxdepth;
+        h->st.max_depth[1] = 0;
+        HIPCHK(hipGetLastError());
+        if ((rc = tb_mixed_prep(h->s, h->mx, h->sp, &h->jt, h->cfg, &h->bytes)))
+            return fail(rc, "%s", tb_mixed_error());
+        h->built = true;
+        h->tree_walk = TB_WALK_MIXED;
+        return TB_OK;
     }
     // all (re)allocation happens here, while the stream is idle
     for (int sp = 0; sp < 2; sp++) {
@@ -272,10 +393,8 @@ int build_phase_b(tb_handle *h, const int *which)
         h->st.max_depth[sp] = S.maxdepth;
     }
     HIPCHK(hipGetLastError());
-    if (h->cfg.walk == TB_WALK_MIXED) {
-        if ((rc = tb_mixed_prep(h->s, h->mx, h->sp, h->cfg, &h->bytes))) return fail(rc, "%s", tb_mixed_error());
-    }
     h->built = true;
+    h->tree_walk = TB_WALK_REF64;
     return TB_OK;
 }
 
@@ -294,7 +413,7 @@ int build_check(tb_handle *h)
 
 int build_all(tb_handle *h)
 {
-    int which[2], rc;
+    int which[3], rc;
     if ((rc = build_phase_a(h, which))) return rc;
     HIPCHK(hipStreamSynchronize(h->s));
     if ((rc = build_phase_b(h, which))) return rc;
@@ -386,7 +505,7 @@ int tb_create(tb_handle **out, const tb_config *cfg)
         delete h;
         return fail(TB_ERR_HIP, "stream creation failed");
     }
-    bool ok = dalloc_t(h, &h->bbox_tmp, 1024 * 6) == TB_OK && dalloc_t(h, &h->d_misc, 256) == TB_OK &&
+    bool ok = dalloc_t(h, &h->bbox_tmp, 2048 * 6) == TB_OK && dalloc_t(h, &h->d_misc, 256) == TB_OK &&
               dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, 12) == TB_OK;
     ok = ok && hipHostMalloc((void **)&h->h_misc, 256 * sizeof(int32_t), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&h->h_sums, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
@@ -406,6 +525,7 @@ int tb_destroy(tb_handle *h)
     hipSetDevice(h->cfg.device);
     if (h->s) hipStreamSynchronize(h->s);
     for (int sp = 0; sp < 2; sp++) free_species(h, h->sp[sp]);
+    free_joint(h, h->jt);
     tb_mixed_free(h->mx, &h->bytes);
     dfree(h, h->sort_tmp, h->sort_tmp_bytes);
     dfree(h, h->scan_tmp, h->scan_tmp_bytes);
@@ -607,6 +727,7 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
     if (!h || !nnodes) return fail(TB_ERR_ARG, "null argument");
     if (species != 0 && species != 1) return fail(TB_ERR_ARG, "species must be 0 or 1");
     if (!h->built) return fail(TB_ERR_STATE, "tb_export_tree before tb_build");
+    if (h->tree_walk != TB_WALK_REF64) return fail(TB_ERR_STATE, "tb_export_tree needs trees built for the FP64 walk");
     SpeciesDev &S = h->sp[species];
     *nnodes = S.nn;
     if (cap < S.nn) return fail(TB_ERR_ARG, "cap %d < %d nodes", cap, S.nn);
@@ -773,7 +894,7 @@ int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n)
     if (!(h->cfg.theta_lo > 0.0) || !(h->cfg.theta_lo < (h->cfg.theta_mixed > 0 ? h->cfg.theta_mixed : h->cfg.theta)))
         return fail(TB_ERR_ARG, "need 0 < theta_lo < theta_mixed");
     HIPCHK(hipSetDevice(h->cfg.device));
-    int rc = tb_mixed_collect(h->s, h->mx, h->sp, h->cfg, h->d_sums, cap, X, y, n);
+    int rc = tb_mixed_collect(h->s, h->mx, h->sp, &h->jt, h->cfg, h->d_sums, cap, X, y, n);
     if (rc) return fail(rc, "%s", tb_mixed_error());
     h->have_force = true;
     return TB_OK;
@@ -837,13 +958,13 @@ namespace {
 int ens_build_all(tb_ensemble *E)
 {
     const int n = (int)E->h.size();
-    std::vector<int> which(2 * (size_t)n);
+    std::vector<int> which(3 * (size_t)n);
     int rc;
     for (int r = 0; r < n; r++)
-        if ((rc = build_phase_a(E->h[r], &which[2 * r]))) return rc;
+        if ((rc = build_phase_a(E->h[r], &which[3 * r]))) return rc;
     HIPCHK(hipStreamSynchronize(E->s));
     for (int r = 0; r < n; r++)
-        if ((rc = build_phase_b(E->h[r], &which[2 * r]))) return rc;
+        if ((rc = build_phase_b(E->h[r], &which[3 * r]))) return rc;
     return TB_OK;
 }
 
@@ -851,12 +972,11 @@ int ens_force(tb_ensemble *E)
 {
     const int n = (int)E->h.size();
     std::vector<SpeciesDev *> sp(2 * (size_t)n);
-    std::vector<const MixedTree *> tr(2 * (size_t)n);
-    for (int r = 0; r < n; r++)
-        for (int k = 0; k < 2; k++) {
-            sp[2 * r + k] = &E->h[r]->sp[k];
-            tr[2 * r + k] = &E->h[r]->mx.t[k];
-        }
+    std::vector<const MixedTree *> tr((size_t)n);
+    for (int r = 0; r < n; r++) {
+        for (int k = 0; k < 2; k++) sp[2 * r + k] = &E->h[r]->sp[k];
+        tr[r] = &E->h[r]->mx.t[0];
+    }
     int rc = tb_mixed_force_multi(E->s, E->mx, sp.data(), tr.data(), n, E->cfg, E->d_esum);
     if (rc) return fail(rc, "%s", tb_mixed_error());
     for (int r = 0; r < n; r++) E->h[r]->have_force = true;

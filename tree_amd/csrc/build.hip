@@ -552,6 +552,194 @@ __global__ __launch_bounds__(BT) void k_export(int nn, const double4 *__restrict
     dx[i] = ndx[i];
 }
 
+// ------------------------------------------------ joint (both species) tree
+//
+// The mixed walk uses one tree over electrons and ions together: its cells
+// carry the net charge, dipole and quadrupole of both species about the cell's
+// geometric centre.  In a quasi-neutral plasma the two species' far fields
+// nearly cancel, so expanding the net charge distribution keeps the far-field
+// error proportional to the (small) net field instead of to each species'
+// (large) field.  Keys use the same FP64 centre recurrence over the joint box.
+
+__global__ __launch_bounds__(BT) void k_jkeys(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                              const double *__restrict__ z, const double *__restrict__ root,
+                                              uint64_t *__restrict__ key, uint32_t *__restrict__ val,
+                                              uint32_t tag, int32_t *__restrict__ err)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    double px = x[i], py = y[i], pz = z[i];
+    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(err, 2);
+    double cx = root[0], cy = root[1], cz = root[2];
+    double lx = root[3], ly = root[4], lz = root[5];
+    uint64_t k = 0;
+#pragma unroll
+    for (int l = 0; l < TB_LEVELS; l++) {
+        uint32_t bx = (px < cx) ? 0u : 1u;
+        uint32_t by = (py < cy) ? 0u : 1u;
+        uint32_t bz = (pz < cz) ? 0u : 1u;
+        k = (k << 3) | (uint64_t)(bx | (by << 1) | (bz << 2));
+        cx = cx + (bx ? 0.25 : -0.25) * lx;
+        cy = cy + (by ? 0.25 : -0.25) * ly;
+        cz = cz + (bz ? 0.25 : -0.25) * lz;
+        lx = lx * 0.5;
+        ly = ly * 0.5;
+        lz = lz * 0.5;
+    }
+    key[i] = k;
+    val[i] = tag | (uint32_t)i;
+}
+
+// joint box from the two species' partial min/max (k_bbox_part output)
+__global__ __launch_bounds__(64) void k_jbox(int nb, const double *__restrict__ part, double *__restrict__ root)
+{
+    if (threadIdx.x != 0) return;
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fmin(mn[k], part[b * 6 + k]);
+            mx[k] = fmax(mx[k], part[b * 6 + 3 + k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        root[k] = (mx[k] + mn[k]) / 2.0;
+        root[3 + k] = mx[k] - mn[k];
+    }
+}
+
+struct JSrc {
+    const double *x[2], *y[2], *z[2];
+};
+
+// sorted joint particles: position and charge (electrons -1, ions +1)
+__global__ __launch_bounds__(BT) void k_jgather(int n, const uint32_t *__restrict__ val, JSrc src,
+                                                double4 *__restrict__ pos)
+{
+    int j = blockIdx.x * BT + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t v = val[j];
+    const int sp = (int)(v >> 31);
+    const uint32_t i = v & 0x7fffffffu;
+    pos[j] = make_double4(src.x[sp][i], src.y[sp][i], src.z[sp][i], sp ? 1.0 : -1.0);
+}
+
+__global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict__ key, const int32_t *__restrict__ offs,
+                                              int total, const double4 *__restrict__ pos,
+                                              double4 *__restrict__ nc, uint32_t *__restrict__ nnext,
+                                              double *__restrict__ nq, int32_t *__restrict__ nstart,
+                                              int32_t *__restrict__ ncount, int32_t *__restrict__ ndepth,
+                                              uint64_t *__restrict__ dkey)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0) {
+        ndepth[0] = 0; nstart[0] = 0; ncount[0] = n; nnext[0] = (uint32_t)total; dkey[0] = 0;
+    }
+    uint64_t k = key[i];
+    int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
+    int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
+    int mx = min(max(lm, lp), TB_LEVELS - 1);
+    lm = min(lm, mx);
+    int leafd = 1 + mx;
+    int base = offs[i] + 1;
+    for (int d = lm + 1; d <= leafd; d++) {
+        int id = base + (d - lm - 1);
+        ndepth[id] = d;
+        nstart[id] = i;
+        dkey[id] = (uint64_t)d;
+        int end;
+        if (d == leafd) {
+            end = i + 1;
+        } else {
+            int sh = 3 * (TB_LEVELS - d);
+            uint64_t p = k >> sh;
+            int lo = i, hi = i + 1, step = 1;
+            while (hi < n && (key[hi] >> sh) == p) {
+                lo = hi;
+                step <<= 1;
+                hi = i + step;
+            }
+            if (hi > n) hi = n;
+            while (hi - lo > 1) {
+                int mid = lo + ((hi - lo) >> 1);
+                if ((key[mid] >> sh) == p) lo = mid;
+                else hi = mid;
+            }
+            end = hi;
+        }
+        ncount[id] = end - i;
+        uint32_t nxt = end >= n ? (uint32_t)total : (uint32_t)(offs[end] + 1);
+        if (d == leafd) {
+            nnext[id] = nxt | TB_LEAF;
+            const double4 p = pos[i];
+            nc[id] = make_double4(p.x, p.y, p.z, 0.0);
+            double *q = nq + (size_t)id * 10;
+            q[0] = p.w;
+            for (int c = 1; c < 10; c++) q[c] = 0.0;
+        } else {
+            nnext[id] = nxt;
+        }
+    }
+}
+
+// net charge q, dipole D and second moment M of a cell about its geometric
+// centre, shifted up from the children's own expansion centres
+__global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restrict__ seg,
+                                                const uint64_t *__restrict__ key, const double *__restrict__ root,
+                                                double4 *__restrict__ nc, const uint32_t *__restrict__ nnext,
+                                                double *__restrict__ nq, const int32_t *__restrict__ nstart,
+                                                const int32_t *__restrict__ ndepth, int32_t *__restrict__ err)
+{
+    int t = blockIdx.x * BT + threadIdx.x;
+    if (t >= cnt) return;
+    uint32_t p = seg[t];
+    uint32_t endp = nnext[p];
+    if (endp & TB_LEAF) return;
+    int d = ndepth[p];
+    double ox = root[0], oy = root[1], oz = root[2];
+    double lx = root[3], ly = root[4], lz = root[5];
+    uint64_t k = key[nstart[p]];
+    for (int l = 0; l < d; l++) {
+        uint32_t dig = (uint32_t)(k >> (3 * (TB_LEVELS - 1 - l))) & 7u;
+        ox = ox + ((dig & 1u) ? 0.25 : -0.25) * lx;
+        oy = oy + ((dig & 2u) ? 0.25 : -0.25) * ly;
+        oz = oz + ((dig & 4u) ? 0.25 : -0.25) * lz;
+        lx = lx * 0.5;
+        ly = ly * 0.5;
+        lz = lz * 0.5;
+    }
+    double q = 0.0, D[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t c = p + 1; c < endp;) {
+        const uint32_t nxc = nnext[c] & ~TB_LEAF;
+        if (nxc <= c) {
+            atomicOr(err, 8);
+            return;
+        }
+        const double4 cc = nc[c];
+        const double *cq = nq + (size_t)c * 10;
+        const double a = cc.x - ox, b = cc.y - oy, e = cc.z - oz;
+        const double qc = cq[0], dx = cq[1], dy = cq[2], dz = cq[3];
+        q += qc;
+        D[0] += dx + qc * a;
+        D[1] += dy + qc * b;
+        D[2] += dz + qc * e;
+        M[0] += cq[4] + 2.0 * dx * a + qc * a * a;
+        M[1] += cq[5] + 2.0 * dy * b + qc * b * b;
+        M[2] += cq[6] + 2.0 * dz * e + qc * e * e;
+        M[3] += cq[7] + dx * b + dy * a + qc * a * b;
+        M[4] += cq[8] + dy * e + dz * b + qc * b * e;
+        M[5] += cq[9] + dz * a + dx * e + qc * e * a;
+        c = nxc;
+    }
+    double s = lx;
+    if (ly > s) s = ly;
+    if (lz > s) s = lz;
+    nc[p] = make_double4(ox, oy, oz, s);
+    double *o = nq + (size_t)p * 10;
+    o[0] = q;
+    o[1] = D[0]; o[2] = D[1]; o[3] = D[2];
+    for (int c = 0; c < 6; c++) o[4 + c] = M[c];
+}
+
 inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
 
 }  // namespace
@@ -650,4 +838,47 @@ void tb_export_gather(hipStream_t s, const SpeciesDev &S, int32_t *depth, int32_
     hipLaunchKernelGGL(k_export, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.nmac, S.nnext, S.nmom, S.ndx,
                        S.nstart, S.ncount, S.ndepth, S.orig[S.cur], depth, leaf, pt, count, next, xx, qeff, dx, D, Q,
                        QQ);
+}
+
+void tb_jbbox(hipStream_t s, int ne, const double *ex, const double *ey, const double *ez, int ni, const double *ix,
+              const double *iy, const double *iz, double *root, double *tmp)
+{
+    int nbe = std::min(nblk(ne, BT), 1024), nbi = std::min(nblk(ni, BT), 1024);
+    hipLaunchKernelGGL(k_bbox_part, dim3(nbe), dim3(BT), 0, s, ne, ex, ey, ez, tmp);
+    hipLaunchKernelGGL(k_bbox_part, dim3(nbi), dim3(BT), 0, s, ni, ix, iy, iz, tmp + 6 * nbe);
+    hipLaunchKernelGGL(k_jbox, dim3(1), dim3(64), 0, s, nbe + nbi, tmp, root);
+}
+
+void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
+              uint64_t *key, uint32_t *val, uint32_t tag, int32_t *err)
+{
+    hipLaunchKernelGGL(k_jkeys, dim3(nblk(n, BT)), dim3(BT), 0, s, n, x, y, z, root, key, val, tag, err);
+}
+
+void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x[2], const double *const y[2],
+                const double *const z[2], double4 *pos)
+{
+    JSrc src;
+    for (int k = 0; k < 2; k++) {
+        src.x[k] = x[k];
+        src.y[k] = y[k];
+        src.z[k] = z[k];
+    }
+    hipLaunchKernelGGL(k_jgather, dim3(nblk(n, BT)), dim3(BT), 0, s, n, val, src, pos);
+}
+
+void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)
+{
+    hipLaunchKernelGGL(k_jemit, dim3(nblk(J.n, BT)), dim3(BT), 0, s, J.n, key, J.offs, J.nn, J.pos, J.nc, J.nnext,
+                       J.nq, J.nstart, J.ncount, J.ndepth, J.dkey[0]);
+}
+
+void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err)
+{
+    for (int d = J.maxdepth; d >= 0; d--) {
+        int beg = J.lvl_off[d], cnt = J.lvl_off[d + 1] - beg;
+        if (cnt <= 0 || J.twig_hist[d] == 0) continue;
+        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, cnt, J.bfs + beg, key, J.root, J.nc,
+                           J.nnext, J.nq, J.nstart, J.ndepth, err);
+    }
 }

@@ -45,6 +45,27 @@ struct SpeciesDev {
     uint32_t *inv = nullptr;    // preorder -> breadth-first index
 };
 
+// The joint tree over both species (mixed walk): nodes in preorder, expansion
+// about each cell's geometric centre (a leaf: its particle).
+struct JointDev {
+    int n = 0;                  // ne + ni
+    uint64_t *key[2] = {};
+    uint32_t *val[2] = {};      // species << 31 | index in the species' key order
+    double4 *pos = nullptr;     // sorted: x, y, z, charge
+    int32_t *cnt = nullptr, *offs = nullptr;
+    double *root = nullptr;     // joint box c0[3], L0[3]
+    int nn = 0, cap = 0, maxdepth = 0;
+    double4 *nc = nullptr;      // expansion centre, cell size max(lx) (leaf: particle, 0)
+    uint32_t *nnext = nullptr;
+    double *nq = nullptr;       // [nn][10] q, D(3), M(6) (xx yy zz xy yz zx) about the centre
+    int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
+    uint64_t *dkey[2] = {};
+    uint32_t *dval[2] = {};
+    uint32_t *bfs = nullptr;
+    int lvl_off[TB_LEVELS + 3] = {};
+    int twig_hist[TB_LEVELS + 2] = {};
+};
+
 // scratch shared by a handle's launches
 struct Scratch {
     void *p = nullptr;
@@ -69,6 +90,15 @@ void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, doub
 void tb_export_gather(hipStream_t s, const SpeciesDev &S, int32_t *depth, int32_t *leaf, int32_t *pt,
                       int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,
                       double *QQ);
+
+void tb_jbbox(hipStream_t s, int ne, const double *ex, const double *ey, const double *ez, int ni, const double *ix,
+              const double *iy, const double *iz, double *root, double *tmp);
+void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
+              uint64_t *key, uint32_t *val, uint32_t tag, int32_t *err);
+void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x[2], const double *const y[2],
+                const double *const z[2], double4 *pos);
+void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key);
+void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err);
 
 // ----- walk64.hip
 void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,

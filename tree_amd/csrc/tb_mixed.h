@@ -1,10 +1,12 @@
-// tb_mixed.h -- the mixed-precision wavefront-group walk (walk32.hip).
+// tb_mixed.h -- the mixed-precision wavefront-group walk (walk32.hip) over
+// the joint tree of both species (JointDev, build.hip).
 //
 // Tree records in breadth-first order (children of a cell contiguous):
-//   bmac  double4  centroid (FP64), rcrit = max(lx)/theta_mixed + dx
-//   binfo uint32   leaf: TB_LEAF | particle index; twig: first child << 3 | (nchild - 1)
-//   bq0   float4   q, Q'xx, Q'yy, Q'zz   (traceless quadrupole 3M - tr(M) I of the
-//   bq1   float4   Q'xy, Q'yz, Q'zx, 0    exact second moment M about the centroid)
+//   bmac  double4  expansion centre (FP64; a leaf: its particle), rcrit = max(lx)/theta_mixed
+//   binfo uint32   leaf: TB_LEAF | joint particle index; twig: first child << 3 | (nchild - 1)
+//   bq0   float4   q, Dx, Dy, Dz          net charge and dipole about the centre
+//   bq1   float4   Q'xx, Q'yy, Q'xy, Q'yz traceless quadrupole 3M - tr(M) I
+//   bq2   float4   Q'zx, s, count, 0      (Q'zz = -Q'xx - Q'yy)
 #pragma once
 #include "tb_internal.h"
 #include "tb_mlp.h"
@@ -13,8 +15,7 @@ struct MixedTree {
     int nn = 0, cap = 0;
     double4 *bmac = nullptr;
     uint32_t *binfo = nullptr;
-    float4 *bq0 = nullptr, *bq1 = nullptr;
-    float2 *bsd = nullptr;       // cell size max(lx), centroid offset dx
+    float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr;
     uint32_t *inv = nullptr;
 };
 
@@ -24,25 +25,22 @@ struct BandRec {
     float f[TB_MLP_NIN];
 };
 
-// one realisation (electron + ion trees and particle state) of a walk launch
+// one realisation (joint tree and particle state) of a walk launch
 struct Domain {
     const double *px[2], *py[2], *pz[2];
     double *fx[2], *fy[2], *fz[2];
     int n[2], ng[2];
-    const double4 *bmac[2];
-    const uint32_t *binfo[2];
-    const float4 *bq0[2], *bq1[2];
-    const float2 *bsd[2];
+    const double4 *bmac;
+    const uint32_t *binfo;
+    const float4 *bq0, *bq1, *bq2;
     float inv_rs, rs;
     int gbase;        // first global group index of this domain
     int pad_;
 };
 
 struct MixedDev {
-    MixedTree t[2];
+    MixedTree t[1];
     int32_t *ctr = nullptr;       // work-queue counter (+ error flag)
-    double *part = nullptr;       // per-wave energy partials, 4 per wave
-    int part_n = 0;
     int tpl = 0;                  // targets per lane used by the last launch
     Domain *ddom = nullptr;       // device copy of the launch's domain descriptors
     int ddom_cap = 0;
@@ -55,14 +53,15 @@ struct MixedDev {
     int rec_cap = 0;
 };
 
-int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, size_t *bytes);
+int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, const tb_config &cfg, size_t *bytes);
 int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *epart, double *d_sums);
 int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *epart, double *d_sums,
                    int64_t *d_cnt);
 void tb_mixed_free(MixedDev &M, size_t *bytes);
 // run the conservative walk (theta_lo) recording band pairs (theta_lo .. theta_mixed),
 // then label them with the FP64 error of accepting the cell; returns the pair count
-int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int cap,
+int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev *jt, const tb_config &cfg,
+                     double *d_sums, int cap,
                      float *X, float *y, int *nout);
 const char *tb_mixed_error();
 

@@ -1,26 +1,30 @@
 // walk32.hip -- the mixed-precision force walk (same result as LocalCoulombSum,
 // tree.f90:662-717, to a stated force error; not its FP64 operation order).
 //
-// One wavefront per target group: 64*TPL consecutive targets in key order.
-// The wave walks each tree cooperatively: lanes test up to 64 candidate cells
-// at once against the group's bounding box (a cell is accepted for every
-// target when its distance to the box exceeds rcrit = max(lx)/theta + dx,
-// which implies the reference's per-target test), then ballots compact the
-// accepted cells and the leaves into interaction lists in LDS, and opened
-// cells push their contiguous child range onto an LDS stack.  Full lists are
-// evaluated by all lanes for their own targets in FP32 with group-relative
-// coordinates (formed in FP64 once per cell and group), and each flush is
-// added into Neumaier-compensated FP32 sums.
+// The walk runs over one joint tree of both species (build.hip): its cells
+// carry the net charge, dipole and traceless quadrupole about their geometric
+// centre, so in a quasi-neutral plasma the far field is expanded from the
+// small net charge distribution rather than as the difference of two large
+// per-species fields.
 //
-// FP64 is used only for: node centroids and target positions (group-relative
-// differences), the centroid/second-moment reduction of the upward pass
-// (build.hip), and the per-target and total energy sums here.
+// One wavefront per target group: 64*TPL consecutive targets of one species
+// in key order.  Lanes test up to 64 candidate cells at once against the
+// group's bounding box (a cell is accepted for every target when its distance
+// to the box exceeds max(lx)/theta), ballots compact accepted cells and
+// leaves into interaction lists in LDS, and opened cells push their
+// contiguous child range onto an LDS stack.  Full lists are evaluated by all
+// lanes for their own targets in FP32 with group-relative coordinates (formed
+// in FP64 once per cell and group); each flush is added into
+// Neumaier-compensated FP32 sums.
 //
-// M2P uses the exact traceless quadrupole of each cell about its centroid (the
-// dipole vanishes there); P2P is Coulomb with the reference's r^2 > 0.01 um^2
-// cut for like species and the simplified Kelbg form for unlike species
-// (tree.f90:806-815, 905-912).  Work is handed out by an atomic queue in one
-// persistent launch covering both target species.
+// FP64 is kept only for: cell moments (build.hip), positions and the
+// group-relative differences, and the energy sums.
+//
+// P2P follows the reference's pair rules (tree.f90:806-815, 905-912): Coulomb
+// beyond 0.1 um for like charges, simplified Kelbg for unlike ones; the rule
+// is uniform across a wave because a group holds one target species.  Work is
+// handed out by an atomic queue in one persistent launch over every target
+// group of every domain.
 #include <hip/hip_runtime.h>
 
 #include "tb_mixed.h"
@@ -119,43 +123,44 @@ __device__ inline void kelbg_gh(float u, float &g, float &h)
 struct WalkArgs {
     const Domain *dom;    // ndom descriptors, ordered by gbase
     int ndom, ngt;        // domains, groups in all domains
-    float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo + dx are always accepted
+    float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
     const float *mlpw;    // learned: bf16-rounded parameters
     BandRec *rec;         // collect: band pairs
     int rec_cap;
     int32_t *ctr;         // [0] work counter, [1] error flag, [2] prep error, [3] band pair count
-    double *esum;         // 4 per domain: ee, ei, ie, ii
+    double *gen;          // 2 per global group: energy of electron targets / of ion targets
     unsigned long long *cnt;
 };
 
 constexpr int MODE_FIXED = 0, MODE_LEARNED = 1, MODE_COLLECT = 2;
 
-// features of a (cell, target group) pair, log2 scaled by 1/8 (f6 is a flag)
-__device__ inline void band_features(float s, float dx, float d, float q, float4 q0, float4 q1, float hx, float hy,
-                                     float hz, float rs, bool like, int nact, float *f)
+// features of a (cell, target group) pair, log2 scaled by 1/8
+__device__ inline void band_features(float s, float cnt, float d, float4 q0, float4 q1, float4 q2, float hx,
+                                     float hy, float hz, float rs, bool like_net, int nact, float *f)
 {
-    const float cnt = fabsf(q);
-    const float qn = sqrtf(q0.y * q0.y + q0.z * q0.z + q0.w * q0.w +
-                           2.0f * (q1.x * q1.x + q1.y * q1.y + q1.z * q1.z));
+    const float dn = sqrtf(q0.y * q0.y + q0.z * q0.z + q0.w * q0.w);
+    const float qzz = -q1.x - q1.y;
+    const float qn = sqrtf(q1.x * q1.x + q1.y * q1.y + qzz * qzz + 2.0f * (q1.z * q1.z + q1.w * q1.w + q2.x * q2.x));
     const float h = sqrtf(hx * hx + hy * hy + hz * hz);
     const float k = 0.125f;
     f[0] = k * __log2f(s / d);
-    f[1] = k * __log2f(dx / s + 9.5367431640625e-07f);
-    f[2] = k * __log2f(qn / (cnt * d * d) + 9.3132257461547852e-10f);
-    f[3] = k * __log2f(cnt);
-    f[4] = k * __log2f(h / d + 9.5367431640625e-07f);
-    f[5] = k * __log2f(rs / d);
-    f[6] = like ? 1.0f : 0.0f;
-    f[7] = k * __log2f((float)nact);
+    f[1] = k * __log2f(fabsf(q0.x) / cnt + 9.5367431640625e-07f);
+    f[2] = k * __log2f(dn / (cnt * s) + 9.5367431640625e-07f);
+    f[3] = k * __log2f(qn / (cnt * s * s) + 9.5367431640625e-07f);
+    f[4] = k * __log2f(cnt);
+    f[5] = k * __log2f(h / d + 9.5367431640625e-07f);
+    f[6] = k * __log2f(rs / d);
+    f[7] = (like_net ? 1.0f : -1.0f) * k * __log2f((float)nact);
 }
 
 template <int TPL>
 struct Acc {
     float s[TPL][3], c[TPL][3];  // compensated field sums
-    float ps[TPL], pc[TPL];      // compensated potential (current tree)
+    float ps[TPL], pc[TPL];      // compensated potential
 };
 
+// monopole + dipole + traceless quadrupole about the cell centre
 template <int TPL>
 __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
                                  const float (&rz)[TPL], Acc<TPL> &A)
@@ -165,21 +170,23 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
     for (int j = 0; j < cnt; j++) {
         const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j];
+        const float qxx = b.w, qyy = c.x, qzz = -b.w - c.x, qxy = c.y, qyz = c.z, qzx = c.w;
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - a.x, dy = ry[t] - a.y, dz = rz[t] - a.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float ri = rsqrtf(r2);
             const float ri2 = ri * ri, ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
-            const float qx = b.x * dx + b.w * dy + c.y * dz;
-            const float qy = b.w * dx + b.y * dy + c.x * dz;
-            const float qz = c.y * dx + c.x * dy + b.z * dz;
+            const float dp = b.x * dx + b.y * dy + b.z * dz;
+            const float qx = qxx * dx + qxy * dy + qzx * dz;
+            const float qy = qxy * dx + qyy * dy + qyz * dz;
+            const float qz = qzx * dx + qyz * dy + qzz * dz;
             const float xqx = dx * qx + dy * qy + dz * qz;
-            const float k = a.w * ri3 + 2.5f * xqx * ri7;
-            ax[t] += dx * k - qx * ri5;
-            ay[t] += dy * k - qy * ri5;
-            az[t] += dz * k - qz * ri5;
-            ap[t] += a.w * ri + 0.5f * xqx * ri5;
+            const float k = a.w * ri3 + 3.0f * dp * ri5 + 2.5f * xqx * ri7;
+            ax[t] += dx * k - b.x * ri3 - qx * ri5;
+            ay[t] += dy * k - b.y * ri3 - qy * ri5;
+            az[t] += dz * k - b.z * ri3 - qz * ri5;
+            ap[t] += a.w * ri + dp * ri3 + 0.5f * xqx * ri5;
         }
     }
 #pragma unroll
@@ -191,46 +198,52 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     }
 }
 
-template <int TPL, bool LIKE>
+// leaves: p.w is the source charge; like charges (same species as the
+// group's targets) use Coulomb beyond the 0.1 um cut, unlike ones Kelbg
+template <int TPL>
 __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
-                                 const float (&rz)[TPL], float qs, float inv_rs, Acc<TPL> &A)
+                                 const float (&rz)[TPL], float qt, float inv_rs, Acc<TPL> &A)
 {
     float ax[TPL], ay[TPL], az[TPL], ap[TPL];
 #pragma unroll
     for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
     for (int j = 0; j < cnt; j++) {
         const float4 q = L.p[j];
+        if (q.w == qt) {
 #pragma unroll
-        for (int t = 0; t < TPL; t++) {
-            const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (LIKE) {
-                // Coulomb beyond the 0.1 um cut (also drops the target itself)
-                const float ri = r2 > 0.01f ? rsqrtf(r2) : 0.0f;
-                const float ri3 = ri * ri * ri;
-                ax[t] += dx * ri3;
-                ay[t] += dy * ri3;
-                az[t] += dz * ri3;
-                ap[t] += ri;
-            } else {
+            for (int t = 0; t < TPL; t++) {
+                const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float ri = r2 > 0.01f ? rsqrtf(r2) : 0.0f;  // also drops the target itself
+                const float w = q.w * ri * ri * ri;
+                ax[t] += dx * w;
+                ay[t] += dy * w;
+                az[t] += dz * w;
+                ap[t] += q.w * ri;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < TPL; t++) {
+                const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
                 const float ri = rsqrtf(r2);
                 const float u = r2 * ri * inv_rs;
                 float g, h;
                 kelbg_gh(u, g, h);
-                const float k = g * ri * ri * ri;
-                ax[t] += dx * k;
-                ay[t] += dy * k;
-                az[t] += dz * k;
-                ap[t] += h * ri;
+                const float w = q.w * g * ri * ri * ri;
+                ax[t] += dx * w;
+                ay[t] += dy * w;
+                az[t] += dz * w;
+                ap[t] += q.w * h * ri;
             }
         }
     }
 #pragma unroll
     for (int t = 0; t < TPL; t++) {
-        nadd(A.s[t][0], A.c[t][0], qs * ax[t]);
-        nadd(A.s[t][1], A.c[t][1], qs * ay[t]);
-        nadd(A.s[t][2], A.c[t][2], qs * az[t]);
-        nadd(A.ps[t], A.pc[t], qs * ap[t]);
+        nadd(A.s[t][0], A.c[t][0], ax[t]);
+        nadd(A.s[t][1], A.c[t][1], ay[t]);
+        nadd(A.s[t][2], A.c[t][2], az[t]);
+        nadd(A.ps[t], A.pc[t], ap[t]);
     }
 }
 
@@ -246,7 +259,8 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
     const int lane = lane_id();
     WaveLDS &L = lds[threadIdx.x >> 6];
     const int G = 64 * TPL;
-    unsigned long long c_visit[4] = {0, 0, 0, 0}, c_m2p[4] = {0, 0, 0, 0}, c_p2p[4] = {0, 0, 0, 0};
+    // per target species: cells tested, M2P and like / unlike P2P interactions
+    unsigned long long c_visit[2] = {0, 0}, c_m2p[2] = {0, 0}, c_like[2] = {0, 0}, c_unlike[2] = {0, 0};
 
     for (;;) {
         int g = 0;
@@ -260,14 +274,13 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             if (W.dom[mid].gbase <= g) lo = mid;
             else hi = mid - 1;
         }
-        const int di = lo;
-        const Domain &D = W.dom[di];
+        const Domain &D = W.dom[lo];
         const int gl = g - D.gbase;
         const int tsp = gl < D.ng[0] ? 0 : 1;
+        const float qt = tsp ? 1.0f : -1.0f;
         const int base = (tsp ? gl - D.ng[0] : gl) * G;
         const int n = D.n[tsp];
         const double *px = D.px[tsp], *py = D.py[tsp], *pz = D.pz[tsp];
-        double eg[2] = {0.0, 0.0};
 
         // targets and the group's bounding box (FP64)
         double tx[TPL], ty[TPL], tz[TPL];
@@ -303,157 +316,143 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             for (int k = 0; k < 3; k++) A.s[t][k] = A.c[t][k] = 0.0f;
+            A.ps[t] = A.pc[t] = 0.0f;
         }
         const int nact = min(G, n - base);  // targets in this group
 
-        for (int tree = 0; tree < 2; tree++) {
-            const int pr = tsp * 2 + tree;  // ee, ei, ie, ii
-            const bool like = tree == tsp;
-            const float qs = tree ? 1.0f : -1.0f;
-            const double4 *bmac = D.bmac[tree];
-            const uint32_t *binfo = D.binfo[tree];
-            const float4 *bq0 = D.bq0[tree], *bq1 = D.bq1[tree];
-            const float2 *bsd = D.bsd[tree];
-#pragma unroll
-            for (int t = 0; t < TPL; t++) A.ps[t] = A.pc[t] = 0.0f;
-            int sp = 1, nm = 0, np = 0;
-            if (lane == 0) L.stack[0] = 0u;  // root: range of one cell at 0
+        int sp = 1, nm = 0, np = 0;
+        if (lane == 0) L.stack[0] = 0u;  // root: range of one cell at 0
+        wave_sync();
+        while (sp > 0) {
+            // a batch of <= 64 cells from the top ranges of the stack
+            uint32_t R = 0;
+            int len = 0;
+            if (lane < sp) {
+                R = L.stack[sp - 1 - lane];
+                len = (int)(R & 7u) + 1;
+            }
+            const int incl = wave_incl_scan(len);
+            const bool take = lane < sp && incl <= 64;
+            const int K = __popcll(__ballot(take));
+            const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
+            if (take) {
+                const uint32_t st = R >> 3;
+                const int ex = incl - len;
+                for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
+            }
+            sp -= K;
             wave_sync();
-            while (sp > 0) {
-                // a batch of <= 64 cells from the top ranges of the stack
-                uint32_t R = 0;
-                int len = 0;
-                if (lane < sp) {
-                    R = L.stack[sp - 1 - lane];
-                    len = (int)(R & 7u) + 1;
-                }
-                const int incl = wave_incl_scan(len);
-                const bool take = lane < sp && incl <= 64;
-                const int K = __popcll(__ballot(take));
-                const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
-                if (take) {
-                    const uint32_t st = R >> 3;
-                    const int ex = incl - len;
-                    for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
-                }
-                sp -= K;
-                wave_sync();
-                const bool have = lane < total;
-                const uint32_t node = have ? L.batch[lane] : 0u;
-                wave_sync();
-                bool leaf = false, acc = false, open = false, band = false;
-                float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
-                uint32_t info = 0;
-                if (have) {
-                    const double4 m = bmac[node];
-                    info = binfo[node];
-                    cx = (float)(m.x - gx);
-                    cy = (float)(m.y - gy);
-                    cz = (float)(m.z - gz);
-                    if (info & TB_LEAF) {
-                        leaf = true;
+            const bool have = lane < total;
+            const uint32_t node = have ? L.batch[lane] : 0u;
+            wave_sync();
+            bool leaf = false, acc = false, open = false, band = false;
+            float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
+            uint32_t info = 0;
+            if (have) {
+                const double4 m = D.bmac[node];
+                info = D.binfo[node];
+                cx = (float)(m.x - gx);
+                cy = (float)(m.y - gy);
+                cz = (float)(m.z - gz);
+                if (info & TB_LEAF) {
+                    leaf = true;
+                } else {
+                    const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
+                    const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
+                    const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
+                    d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    const float rc = (float)m.w;  // s / theta_mixed
+                    const bool far_hi = d2 > rc * rc * 1.000004f;
+                    if (MODE == MODE_FIXED) {
+                        acc = far_hi;
+                        open = !far_hi;
                     } else {
-                        const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
-                        const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
-                        const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
-                        d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        const float rc = (float)m.w;  // s / theta_mixed + dx
-                        const bool far_hi = d2 > rc * rc * 1.000004f;
-                        if (MODE == MODE_FIXED) {
-                            acc = far_hi;
-                            open = !far_hi;
-                        } else {
-                            const float2 sd = bsd[node];
-                            const float rlo = sd.x * W.inv_theta_lo + sd.y;
-                            if (d2 > rlo * rlo * 1.000004f) acc = true;
-                            else if (!far_hi) open = true;
-                            else band = true;
-                        }
+                        const float rlo = D.bq2[node].y * W.inv_theta_lo;
+                        if (d2 > rlo * rlo * 1.000004f) acc = true;
+                        else if (!far_hi) open = true;
+                        else band = true;
                     }
-                }
-                if (MODE != MODE_FIXED && band) {
-                    const float2 sd = bsd[node];
-                    const float4 q0 = bq0[node], q1 = bq1[node];
-                    float f[TB_MLP_NIN];
-                    band_features(sd.x, sd.y, sqrtf(d2), q0.x, q0, q1, hx, hy, hz, D.rs, like, nact, f);
-                    if (MODE == MODE_LEARNED) {
-                        acc = tb_mlp_eval(wl, f) < W.log_err;
-                        open = !acc;
-                    } else {
-                        open = true;  // the conservative walk opens it; the pair is recorded
-                        if (fabsf(q0.x) <= 4096.0f) {
-                            const int k = atomicAdd(&W.ctr[3], 1);
-                            if (k < W.rec_cap) {
-                                BandRec r;
-                                r.cell = node;
-                                r.group = (uint32_t)g;
-                                r.tree = (uint32_t)tree;
-                                r.pad = 0;
-                                for (int j = 0; j < TB_MLP_NIN; j++) r.f[j] = f[j];
-                                W.rec[k] = r;
-                            }
-                        }
-                    }
-                }
-                if (COUNT && have) c_visit[pr]++;
-                // accepted cells -> M2P list
-                uint64_t mk = __ballot(acc);
-                if (acc) {
-                    const int k = nm + __popcll(mk & lanemask_lt());
-                    const float4 q0 = bq0[node], q1 = bq1[node];
-                    L.m0[k] = make_float4(cx, cy, cz, q0.x);
-                    L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
-                    L.m2[k] = make_float4(q1.y, q1.z, 0.0f, 0.0f);
-                }
-                nm += __popcll(mk);
-                // leaves -> P2P list
-                mk = __ballot(leaf);
-                if (leaf) L.p[np + __popcll(mk & lanemask_lt())] = make_float4(cx, cy, cz, 0.0f);
-                np += __popcll(mk);
-                // opened cells -> their child ranges
-                mk = __ballot(open);
-                const int npush = __popcll(mk);
-                if (sp + npush > STACK) {
-                    if (lane == 0) atomicOr(&W.ctr[1], 1);
-                    sp = 0;
-                    nm = np = 0;
-                    break;
-                }
-                if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
-                sp += npush;
-                wave_sync();
-                if (nm >= FLUSH) {
-                    if (COUNT) c_m2p[pr] += (unsigned long long)nm * nact;
-                    flush_m2p<TPL>(L, nm, rx, ry, rz, A);
-                    nm = 0;
-                    wave_sync();
-                }
-                if (np >= FLUSH) {
-                    if (COUNT) c_p2p[pr] += (unsigned long long)np * nact;
-                    if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, D.inv_rs, A);
-                    else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, D.inv_rs, A);
-                    np = 0;
-                    wave_sync();
                 }
             }
-            if (nm > 0) {
-                if (COUNT) c_m2p[pr] += (unsigned long long)nm * nact;
+            if (MODE != MODE_FIXED && band) {
+                const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
+                float f[TB_MLP_NIN];
+                band_features(q2.y, q2.z, sqrtf(d2), q0, q1, q2, hx, hy, hz, D.rs, q0.x * qt >= 0.0f, nact, f);
+                if (MODE == MODE_LEARNED) {
+                    acc = tb_mlp_eval(wl, f) < W.log_err;
+                    open = !acc;
+                } else {
+                    open = true;  // the conservative walk opens it; the pair is recorded
+                    if (q2.z <= 4096.0f) {
+                        const int k = atomicAdd(&W.ctr[3], 1);
+                        if (k < W.rec_cap) {
+                            BandRec r;
+                            r.cell = node;
+                            r.group = (uint32_t)g;
+                            r.tree = 0;
+                            r.pad = 0;
+                            for (int j = 0; j < TB_MLP_NIN; j++) r.f[j] = f[j];
+                            W.rec[k] = r;
+                        }
+                    }
+                }
+            }
+            if (COUNT && have) c_visit[tsp]++;
+            // accepted cells -> M2P list
+            uint64_t mk = __ballot(acc);
+            if (acc) {
+                const int k = nm + __popcll(mk & lanemask_lt());
+                const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
+                L.m0[k] = make_float4(cx, cy, cz, q0.x);
+                L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
+                L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
+            }
+            nm += __popcll(mk);
+            // leaves -> P2P list with their charge
+            mk = __ballot(leaf);
+            const float qleaf = leaf ? D.bq0[node].x : 0.0f;
+            if (leaf) L.p[np + __popcll(mk & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
+            np += __popcll(mk);
+            if (COUNT) {
+                const int nl = __popcll(__ballot(leaf && qleaf == qt));
+                c_like[tsp] += (unsigned long long)nl * nact;
+                c_unlike[tsp] += (unsigned long long)(__popcll(mk) - nl) * nact;
+            }
+            // opened cells -> their child ranges
+            mk = __ballot(open);
+            const int npush = __popcll(mk);
+            if (sp + npush > STACK) {
+                if (lane == 0) atomicOr(&W.ctr[1], 1);
+                sp = 0;
+                nm = np = 0;
+                break;
+            }
+            if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
+            sp += npush;
+            wave_sync();
+            if (nm >= FLUSH) {
+                if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
                 flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+                nm = 0;
+                wave_sync();
             }
-            if (np > 0) {
-                if (COUNT) c_p2p[pr] += (unsigned long long)np * nact;
-                if (like) flush_p2p<TPL, true>(L, np, rx, ry, rz, qs, D.inv_rs, A);
-                else flush_p2p<TPL, false>(L, np, rx, ry, rz, qs, D.inv_rs, A);
+            if (np >= FLUSH) {
+                flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
+                np = 0;
+                wave_sync();
             }
-            wave_sync();
-            // energy of this pairing: 0.5 * q_t * eps * phi (FP64 from here on)
-            const double qt = tsp ? 1.0 : -1.0;
-#pragma unroll
-            for (int t = 0; t < TPL; t++)
-                if (act[t]) eg[tree] += 0.5 * qt * TB_EPS * ((double)A.ps[t] + (double)A.pc[t]);
         }
-        // forces: F = q_t * eps * E
-        const double ff = (tsp ? 1.0 : -1.0) * TB_EPS;
+        if (nm > 0) {
+            if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
+            flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+        }
+        if (np > 0) {
+            flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
+        }
+        wave_sync();
+        // forces F = q_t eps E; energy 0.5 q_t eps phi (FP64 from here on)
+        const double ff = (double)qt * TB_EPS;
+        double eg = 0.0;
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             if (!act[t]) continue;
@@ -461,20 +460,35 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             D.fx[tsp][i] = ff * ((double)A.s[t][0] + (double)A.c[t][0]);
             D.fy[tsp][i] = ff * ((double)A.s[t][1] + (double)A.c[t][1]);
             D.fz[tsp][i] = ff * ((double)A.s[t][2] + (double)A.c[t][2]);
+            eg += 0.5 * ff * ((double)A.ps[t] + (double)A.pc[t]);
         }
-        // group energies into the domain's pairing sums (FP64)
-        const double e0 = wave_sum(eg[0]), e1 = wave_sum(eg[1]);
+        eg = wave_sum(eg);
         if (lane == 0) {
-            atomicAdd(&W.esum[di * 4 + tsp * 2], e0);
-            atomicAdd(&W.esum[di * 4 + tsp * 2 + 1], e1);
+            W.gen[2 * g + tsp] = eg;
+            W.gen[2 * g + 1 - tsp] = 0.0;
         }
     }
+    // counter slots (visits, m2p, p2p) x (ee, ei, ie, ii): like P2P and all
+    // cell counts go to ee / ii by target species, unlike P2P to ei / ie
     if (COUNT && lane == 0) {
-        for (int pr = 0; pr < 4; pr++) {
-            atomicAdd(&W.cnt[pr * 3], c_visit[pr]);
-            atomicAdd(&W.cnt[pr * 3 + 1], c_m2p[pr]);
-            atomicAdd(&W.cnt[pr * 3 + 2], c_p2p[pr]);
-        }
+        atomicAdd(&W.cnt[0], c_visit[0]);
+        atomicAdd(&W.cnt[1], c_m2p[0]);
+        atomicAdd(&W.cnt[2], c_like[0]);
+        atomicAdd(&W.cnt[5], c_unlike[0]);
+        atomicAdd(&W.cnt[8], c_unlike[1]);
+        atomicAdd(&W.cnt[9], c_visit[1]);
+        atomicAdd(&W.cnt[10], c_m2p[1]);
+        atomicAdd(&W.cnt[11], c_like[1]);
+    }
+}
+
+template <int MODE, bool COUNT>
+void dispatch(hipStream_t s, int tpl, int blocks, const WalkArgs &W)
+{
+    switch (tpl) {
+    case 1: hipLaunchKernelGGL((k_walk32<1, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
+    case 4: hipLaunchKernelGGL((k_walk32<4, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
+    default: hipLaunchKernelGGL((k_walk32<2, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
     }
 }
 
@@ -488,50 +502,44 @@ __global__ __launch_bounds__(BT) void k_inv(int nn, const uint32_t *__restrict__
 
 __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__restrict__ bfs,
                                                     const uint32_t *__restrict__ inv,
-                                                    const double4 *__restrict__ nmac,
+                                                    const double4 *__restrict__ nc,
                                                     const uint32_t *__restrict__ nnext,
-                                                    const double *__restrict__ nmom, const double *__restrict__ nm2,
-                                                    const double *__restrict__ ndx,
+                                                    const double *__restrict__ nq,
                                                     const int32_t *__restrict__ nstart,
-                                                    const int32_t *__restrict__ ndepth,
-                                                    const double *__restrict__ root, double theta,
+                                                    const int32_t *__restrict__ ncount, double theta,
                                                     double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
                                                     float4 *__restrict__ bq0, float4 *__restrict__ bq1,
-                                                    float2 *__restrict__ bsd, int32_t *__restrict__ err)
+                                                    float4 *__restrict__ bq2, int32_t *__restrict__ err)
 {
     const int b = blockIdx.x * BT + threadIdx.x;
     if (b >= nn) return;
     const uint32_t p = bfs[b];
     const uint32_t nx = nnext[p];
-    const double4 m = nmac[p];
-    const double q = nmom[(size_t)p * 10];
+    const double4 c = nc[p];
+    const double *Q = nq + (size_t)p * 10;
     if (nx & TB_LEAF) {
         binfo[b] = TB_LEAF | (uint32_t)nstart[p];
-        bmac[b] = make_double4(m.x, m.y, m.z, 0.0);
-        bq0[b] = make_float4((float)q, 0.f, 0.f, 0.f);
+        bmac[b] = make_double4(c.x, c.y, c.z, 0.0);
+        bq0[b] = make_float4((float)Q[0], 0.f, 0.f, 0.f);
         bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
-        bsd[b] = make_float2(0.f, 0.f);
+        bq2[b] = make_float4(0.f, 0.f, 1.f, 0.f);
         return;
     }
-    int nc = 0;
-    for (uint32_t c = p + 1; c < nx; nc++) {
-        const uint32_t nc2 = nnext[c] & ~TB_LEAF;
-        if (nc2 <= c || nc >= 8) {
+    int nch = 0;
+    for (uint32_t k = p + 1; k < nx; nch++) {
+        const uint32_t k2 = nnext[k] & ~TB_LEAF;
+        if (k2 <= k || nch >= 8) {
             atomicOr(err, 16);
             return;
         }
-        c = nc2;
+        k = k2;
     }
-    binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nc - 1);
-    // the cell size max(lx) halves exactly with each level: max(L0) * 2^-depth
-    const double smax0 = fmax(root[3], fmax(root[4], root[5]));
-    const double s = ldexp(smax0, -ndepth[p]);
-    bmac[b] = make_double4(m.x, m.y, m.z, s / theta + ndx[p]);
-    bsd[b] = make_float2((float)s, (float)ndx[p]);
-    const double *M = nm2 + (size_t)p * 6;
-    const double tr = M[0] + M[1] + M[2];
-    bq0[b] = make_float4((float)q, (float)(3.0 * M[0] - tr), (float)(3.0 * M[1] - tr), (float)(3.0 * M[2] - tr));
-    bq1[b] = make_float4((float)(3.0 * M[3]), (float)(3.0 * M[4]), (float)(3.0 * M[5]), 0.f);
+    binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nch - 1);
+    bmac[b] = make_double4(c.x, c.y, c.z, c.w / theta);
+    const double tr = Q[4] + Q[5] + Q[6];
+    bq0[b] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
+    bq1[b] = make_float4((float)(3.0 * Q[4] - tr), (float)(3.0 * Q[5] - tr), (float)(3.0 * Q[7]), (float)(3.0 * Q[8]));
+    bq2[b] = make_float4((float)(3.0 * Q[9]), (float)c.w, (float)ncount[p], 0.f);
 }
 
 __global__ void k_zero_ctr(int32_t *ctr, int lo, int hi)
@@ -540,9 +548,32 @@ __global__ void k_zero_ctr(int32_t *ctr, int lo, int hi)
     if (i < hi) ctr[i] = 0;
 }
 
-__global__ void k_zero_d(double *p, int n)
+// per domain, deterministic: esum[4d + 2*tsp + tree] = sum over the domain's groups
+__global__ __launch_bounds__(256) void k_domain_energy(const Domain *__restrict__ dom,
+                                                       const double *__restrict__ gen, double *__restrict__ esum)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+    __shared__ double red[2][256];
+    const Domain &D = dom[blockIdx.x];
+    double a[2] = {0.0, 0.0};
+    const int ng = D.ng[0] + D.ng[1];
+    for (int k = threadIdx.x; k < ng; k += 256) {
+        a[0] += gen[2 * (D.gbase + k)];
+        a[1] += gen[2 * (D.gbase + k) + 1];
+    }
+    for (int c = 0; c < 2; c++) red[c][threadIdx.x] = a[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c < 2; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    // split by target species: electron targets -> the ee slot, ion targets -> the ii slot
+    if (threadIdx.x == 0) {
+        esum[4 * blockIdx.x] = red[0][0];
+        esum[4 * blockIdx.x + 1] = 0.0;
+        esum[4 * blockIdx.x + 2] = 0.0;
+        esum[4 * blockIdx.x + 3] = red[1][0];
+    }
 }
 
 // a single domain's pairing sums and the walk's error flag into the handle's sums
@@ -559,77 +590,73 @@ struct LabelArgs {
     int nrec;
     int G;                 // targets per group of the collecting walk
     int ng0;               // electron groups
-    const double *px[2], *py[2], *pz[2];  // key-ordered positions
+    const double *px[2], *py[2], *pz[2];  // key-ordered target positions
     const double *fx[2], *fy[2], *fz[2];  // forces of the collecting walk
     int n[2];
-    const uint32_t *bfs[2];               // breadth-first -> preorder
-    const double4 *nmac[2];
-    const double *nmom[2], *nm2[2];
-    const int32_t *nstart[2], *ncount[2];
+    const uint32_t *bfs;                  // breadth-first -> preorder (joint tree)
+    const double4 *nc;
+    const double *nq;
+    const int32_t *nstart, *ncount;
+    const double4 *pos;                   // joint particles: x, y, z, charge
     double rs;
     float *lab;
 };
 
 // label of a band pair: log10 of the largest, over the group's targets, of
-// |F_M2P - F_exact| / |F_target| with F_M2P the cell's monopole + exact
-// quadrupole field and F_exact the FP64 sum over the cell's particles with
-// the reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg)
+// |F_M2P - F_exact| / |F_target| with F_M2P the cell's monopole + dipole +
+// quadrupole field and F_exact the FP64 sum over the cell's particles with the
+// reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg)
 __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
 {
     const int r = blockIdx.x;
     if (r >= A.nrec) return;
     const BandRec R = A.rec[r];
     const int tsp = (int)R.group < A.ng0 ? 0 : 1;
+    const double qt = tsp ? 1.0 : -1.0;
     const int base = ((int)R.group - (tsp ? A.ng0 : 0)) * A.G;
-    const int tr = (int)R.tree;
-    const bool like = tr == tsp;
-    const uint32_t p = A.bfs[tr][R.cell];
-    const double4 c = A.nmac[tr][p];
-    const double q = A.nmom[tr][(size_t)p * 10];
-    const double *M = A.nm2[tr] + (size_t)p * 6;
-    const double trM = M[0] + M[1] + M[2];
-    const double Qxx = 3.0 * M[0] - trM, Qyy = 3.0 * M[1] - trM, Qzz = 3.0 * M[2] - trM;
-    const double Qxy = 3.0 * M[3], Qyz = 3.0 * M[4], Qzx = 3.0 * M[5];
-    const int s0 = A.nstart[tr][p], cnt = A.ncount[tr][p];
-    const double qs = tr ? 1.0 : -1.0;
-    const double *sx = A.px[tr], *sy = A.py[tr], *sz = A.pz[tr];
+    const uint32_t p = A.bfs[R.cell];
+    const double4 c = A.nc[p];
+    const double *Q = A.nq + (size_t)p * 10;
+    const double q = Q[0], Dx = Q[1], Dy = Q[2], Dz = Q[3];
+    const double tr = Q[4] + Q[5] + Q[6];
+    const double Qxx = 3.0 * Q[4] - tr, Qyy = 3.0 * Q[5] - tr, Qzz = 3.0 * Q[6] - tr;
+    const double Qxy = 3.0 * Q[7], Qyz = 3.0 * Q[8], Qzx = 3.0 * Q[9];
+    const int s0 = A.nstart[p], cnt = A.ncount[p];
     double worst = 0.0;
     for (int t = threadIdx.x; t < A.G; t += 64) {
         const int i = base + t;
         if (i >= A.n[tsp]) break;
         const double x = A.px[tsp][i], y = A.py[tsp][i], z = A.pz[tsp][i];
-        // multipole field
         const double dx = x - c.x, dy = y - c.y, dz = z - c.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const double ri = 1.0 / sqrt(r2), ri2 = ri * ri, ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+        const double dp = Dx * dx + Dy * dy + Dz * dz;
         const double qx = Qxx * dx + Qxy * dy + Qzx * dz;
         const double qy = Qxy * dx + Qyy * dy + Qyz * dz;
         const double qz = Qzx * dx + Qyz * dy + Qzz * dz;
         const double xqx = dx * qx + dy * qy + dz * qz;
-        const double k = q * ri3 + 2.5 * xqx * ri7;
-        double ex = dx * k - qx * ri5, ey = dy * k - qy * ri5, ez = dz * k - qz * ri5;
-        // exact field of the cell's particles
+        const double k = q * ri3 + 3.0 * dp * ri5 + 2.5 * xqx * ri7;
+        double ex = dx * k - Dx * ri3 - qx * ri5, ey = dy * k - Dy * ri3 - qy * ri5, ez = dz * k - Dz * ri3 - qz * ri5;
         double gx = 0.0, gy = 0.0, gz = 0.0;
         for (int j = s0; j < s0 + cnt; j++) {
-            const double ax = x - sx[j], ay = y - sy[j], az = z - sz[j];
+            const double4 sj = A.pos[j];
+            const double ax = x - sj.x, ay = y - sj.y, az = z - sj.z;
             const double a2 = ax * ax + ay * ay + az * az;
-            if (like) {
+            const double ar = sqrt(a2);
+            double w;
+            if (sj.w == qt) {
                 if (!(a2 > 0.01)) continue;
-                const double ar = sqrt(a2);
-                const double w = qs / (a2 * ar);
-                gx += ax * w; gy += ay * w; gz += az * w;
+                w = sj.w / (a2 * ar);
             } else {
-                const double ar = sqrt(a2);
                 const double e = exp(-ar / A.rs);
-                const double w = qs * (1.0 / ar - e * (1.0 / ar + 1.0 / A.rs)) / a2;
-                gx += ax * w; gy += ay * w; gz += az * w;
+                w = sj.w * (1.0 / ar - e * (1.0 / ar + 1.0 / A.rs)) / a2;
             }
+            gx += ax * w; gy += ay * w; gz += az * w;
         }
         ex -= gx; ey -= gy; ez -= gz;
         const double fx = A.fx[tsp][i], fy = A.fy[tsp][i], fz = A.fz[tsp][i];
         const double fn = sqrt(fx * fx + fy * fy + fz * fz);
-        const double err = TB_EPS * sqrt(ex * ex + ey * ey + ez * ez) / fmax(fn, 1e-300);
-        worst = fmax(worst, err);
+        worst = fmax(worst, TB_EPS * sqrt(ex * ex + ey * ey + ez * ez) / fmax(fn, 1e-300));
     }
     worst = wave_max(worst);
     if (threadIdx.x == 0) A.lab[r] = (float)log10(worst + 1e-12);
@@ -676,17 +703,7 @@ int resident_waves()
     return w;
 }
 
-template <int MODE, bool COUNT>
-void dispatch(hipStream_t s, int tpl, int blocks, const WalkArgs &W)
-{
-    switch (tpl) {
-    case 1: hipLaunchKernelGGL((k_walk32<1, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
-    case 4: hipLaunchKernelGGL((k_walk32<4, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
-    default: hipLaunchKernelGGL((k_walk32<2, MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W); break;
-    }
-}
-
-Domain make_domain(SpeciesDev *const sp[2], const MixedTree *const t[2], const tb_config &cfg, int G, int gbase)
+Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, const tb_config &cfg, int G, int gbase)
 {
     Domain d;
     for (int k = 0; k < 2; k++) {
@@ -695,9 +712,8 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *const t[2], const t
         d.fx[k] = sp[k]->f[c][0]; d.fy[k] = sp[k]->f[c][1]; d.fz[k] = sp[k]->f[c][2];
         d.n[k] = sp[k]->n;
         d.ng[k] = (sp[k]->n + G - 1) / G;
-        d.bmac[k] = t[k]->bmac; d.binfo[k] = t[k]->binfo; d.bq0[k] = t[k]->bq0; d.bq1[k] = t[k]->bq1;
-        d.bsd[k] = t[k]->bsd;
     }
+    d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2;
     d.inv_rs = (float)(1.0 / cfg.rs);
     d.rs = (float)cfg.rs;
     d.gbase = gbase;
@@ -721,15 +737,17 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         }
         M.ddom_cap = nd;
     }
-    if (4 * nd > M.esum_cap) {
+    const int ngt = doms.back().gbase + doms.back().ng[0] + doms.back().ng[1];
+    if (4 * nd + 2 * ngt > M.esum_cap) {
         if (M.esum) hipFree(M.esum);
         M.esum = nullptr;
         M.esum_cap = 0;
-        if (hipMalloc((void **)&M.esum, (size_t)4 * nd * sizeof(double)) != hipSuccess) {
+        const int cap = 4 * nd + 2 * ngt + 1024;
+        if (hipMalloc((void **)&M.esum, (size_t)cap * sizeof(double)) != hipSuccess) {
             g_err = "mixed walk: energy sums allocation failed";
             return TB_ERR_NOMEM;
         }
-        M.esum_cap = 4 * nd;
+        M.esum_cap = cap;
     }
     if (hipMemcpy(M.ddom, doms.data(), (size_t)nd * sizeof(Domain), hipMemcpyHostToDevice) != hipSuccess) {
         g_err = "mixed walk: descriptor upload failed";
@@ -738,7 +756,7 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     WalkArgs W;
     W.dom = M.ddom;
     W.ndom = nd;
-    W.ngt = doms.back().gbase + doms.back().ng[0] + doms.back().ng[1];
+    W.ngt = ngt;
     const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : (cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta);
     W.inv_theta_lo = (float)(1.0 / th_lo);
     W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
@@ -746,13 +764,12 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     W.rec = M.rec;
     W.rec_cap = M.rec_cap;
     W.ctr = M.ctr;
-    W.esum = M.esum;
+    W.gen = M.esum + 4 * nd;
     W.cnt = (unsigned long long *)d_cnt;
-    const int waves = std::min(W.ngt, resident_waves());
+    const int waves = std::min(ngt, resident_waves());
     const int blocks = nblk(waves, WPB);
     M.tpl = tpl;
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
-    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, M.esum, 4 * nd);
     if (mode == MODE_COLLECT) {
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
         dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
@@ -763,6 +780,7 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         if (d_cnt) dispatch<MODE_FIXED, true>(s, tpl, blocks, W);
         else dispatch<MODE_FIXED, false>(s, tpl, blocks, W);
     }
+    hipLaunchKernelGGL(k_domain_energy, dim3(nd), dim3(256), 0, s, M.ddom, W.gen, M.esum);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: launch failed";
         return TB_ERR_HIP;
@@ -775,8 +793,7 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
 {
     const int tpl = pick_tpl(cfg, sp[0].n + sp[1].n);
     SpeciesDev *const spp[2] = {&sp[0], &sp[1]};
-    const MixedTree *const tp[2] = {&M.t[0], &M.t[1]};
-    std::vector<Domain> doms(1, make_domain(spp, tp, cfg, 64 * tpl, 0));
+    std::vector<Domain> doms(1, make_domain(spp, &M.t[0], cfg, 64 * tpl, 0));
     int rc = launch_domains(s, M, doms, tpl, cfg, d_cnt, mode);
     if (rc) return rc;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, M.esum, M.ctr, d_sums);
@@ -791,8 +808,9 @@ int walk_mode(const MixedDev &M, const tb_config &cfg)
 
 }  // namespace
 
-int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, size_t *bytes)
+int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, const tb_config &cfg, size_t *bytes)
 {
+    (void)sp;
     if (!M.ctr) {
         if (hipMalloc((void **)&M.ctr, 64) != hipSuccess) {
             g_err = "mixed walk: counter allocation failed";
@@ -802,24 +820,21 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &c
     }
     const double theta = cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta;
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 2, 3);
-    for (int k = 0; k < 2; k++) {
-        SpeciesDev &S = sp[k];
-        MixedTree &T = M.t[k];
-        int cap = T.cap, rc;
-        int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap;
-        if ((rc = grow(&T.bmac, &c1, S.nn, bytes))) return rc;
-        if ((rc = grow(&T.binfo, &c2, S.nn, bytes))) return rc;
-        if ((rc = grow(&T.bq0, &c3, S.nn, bytes))) return rc;
-        if ((rc = grow(&T.bq1, &c4, S.nn, bytes))) return rc;
-        if ((rc = grow(&T.inv, &c5, S.nn, bytes))) return rc;
-        if ((rc = grow(&T.bsd, &c6, S.nn, bytes))) return rc;
-        T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(c5, c6));
-        T.nn = S.nn;
-        hipLaunchKernelGGL(k_inv, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv);
-        hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, T.inv, S.nmac, S.nnext,
-                           S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, S.root, theta, T.bmac, T.binfo, T.bq0, T.bq1,
-                           T.bsd, M.ctr + 2);
-    }
+    const JointDev &J = *jt;
+    MixedTree &T = M.t[0];
+    int cap = T.cap, rc;
+    int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap;
+    if ((rc = grow(&T.bmac, &c1, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.binfo, &c2, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bq0, &c3, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bq1, &c4, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bq2, &c5, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.inv, &c6, J.nn, bytes))) return rc;
+    T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(c5, c6));
+    T.nn = J.nn;
+    hipLaunchKernelGGL(k_inv, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv);
+    hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv, J.nc, J.nnext, J.nq,
+                       J.nstart, J.ncount, theta, T.bmac, T.binfo, T.bq0, T.bq1, T.bq2, M.ctr + 2);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: prep launch failed";
         return TB_ERR_HIP;
@@ -839,8 +854,8 @@ int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &
     return launch(s, M, sp, cfg, d_sums, d_cnt, walk_mode(M, cfg));
 }
 
-int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int cap,
-                     float *X, float *y, int *nout)
+int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev *jt, const tb_config &cfg,
+                     double *d_sums, int cap, float *X, float *y, int *nout)
 {
     if (cap > M.rec_cap) {
         if (M.rec) hipFree(M.rec);
@@ -874,13 +889,13 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config
         A.px[k] = sp[k].x[c][0]; A.py[k] = sp[k].x[c][1]; A.pz[k] = sp[k].x[c][2];
         A.fx[k] = sp[k].f[c][0]; A.fy[k] = sp[k].f[c][1]; A.fz[k] = sp[k].f[c][2];
         A.n[k] = sp[k].n;
-        A.bfs[k] = sp[k].bfs;
-        A.nmac[k] = sp[k].nmac;
-        A.nmom[k] = sp[k].nmom;
-        A.nm2[k] = sp[k].nm2;
-        A.nstart[k] = sp[k].nstart;
-        A.ncount[k] = sp[k].ncount;
     }
+    A.bfs = jt->bfs;
+    A.nc = jt->nc;
+    A.nq = jt->nq;
+    A.nstart = jt->nstart;
+    A.ncount = jt->ncount;
+    A.pos = jt->pos;
     A.rs = cfg.rs;
     A.lab = M.lab;
     if (nrec > 0) hipLaunchKernelGGL(k_labels, dim3(nrec), dim3(64), 0, s, A);
@@ -912,8 +927,7 @@ int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, cons
     int gb = 0;
     for (int d = 0; d < ndom; d++) {
         SpeciesDev *const spp[2] = {sp[2 * d], sp[2 * d + 1]};
-        const MixedTree *const tp[2] = {trees[2 * d], trees[2 * d + 1]};
-        doms.push_back(make_domain(spp, tp, cfg, 64 * tpl, gb));
+        doms.push_back(make_domain(spp, trees[d], cfg, 64 * tpl, gb));
         gb += doms.back().ng[0] + doms.back().ng[1];
     }
     int rc = launch_domains(s, M, doms, tpl, cfg, nullptr, MODE_FIXED);
@@ -929,24 +943,19 @@ int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, cons
 
 void tb_mixed_free(MixedDev &M, size_t *bytes)
 {
-    for (int k = 0; k < 2; k++) {
-        MixedTree &T = M.t[k];
-        hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.inv); hipFree(T.bsd);
-        T = MixedTree();
-    }
+    MixedTree &T = M.t[0];
+    hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.bq2); hipFree(T.inv);
+    T = MixedTree();
     hipFree(M.ctr);
-    hipFree(M.part);
     if (M.ddom) hipFree(M.ddom);
     if (M.esum) hipFree(M.esum);
-    M.ddom = nullptr;
-    M.esum = nullptr;
-    M.ddom_cap = M.esum_cap = 0;
     if (M.rec) hipFree(M.rec);
     if (M.lab) hipFree(M.lab);
     tb_mlp_free(M.mlp);
     M.ctr = nullptr;
-    M.part = nullptr;
-    M.part_n = 0;
+    M.ddom = nullptr;
+    M.esum = nullptr;
+    M.ddom_cap = M.esum_cap = 0;
     M.rec = nullptr;
     M.lab = nullptr;
     M.rec_cap = 0;
