@@ -114,6 +114,15 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
  */
 int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
 
+/*
+ * The FP64 walk (TB_WALK_REF64 trees) for a list of targets only (reference
+ * order): f [nt][3].  Same forces as tb_force gives those targets.
+ */
+int tb_force_targets(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
+/* device time of the FP64 walk over every stride-th target of each species
+ * (sampled timing; per-target walks are independent) */
+int tb_time_walk(tb_handle *h, int32_t stride, float *ms);
+
 typedef struct tb_stats {
     int64_t nodes[2];        /* node count per species tree                     */
     int32_t max_depth[2];

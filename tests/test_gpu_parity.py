@@ -193,3 +193,22 @@ def test_errors():
         e = ta.Engine(rs=RS10, theta=0.5, mem_cap=1 << 16)
         e.set_particles(*_plasma(1, 100000, 100000, 400.0))
     assert ex.value.code == 4
+
+
+def test_force_targets_and_sampled_timing():
+    """The target-list walk gives exactly tb_force's forces for those targets."""
+    announce("force_targets")
+    qel, qion = _plasma(17, 5000, 4000, 100.0)
+    rng = np.random.default_rng(3)
+    tsp = rng.integers(0, 2, 300).astype(np.int32)
+    tix = np.where(tsp == 0, rng.integers(0, 5000, 300), rng.integers(0, 4000, 300)).astype(np.int32)
+    with _engine(theta=0.4) as eng:
+        eng.set_particles(qel, qion)
+        eng.build()
+        eng.force()
+        e, i = eng.get_particles()
+        f = eng.force_targets(tsp, tix)
+        ms = eng.time_walk(8)
+    fr = np.where(tsp[:, None] == 0, e["ff"][np.where(tsp == 0, tix, 0)], i["ff"][np.where(tsp == 1, tix, 0)])
+    assert np.array_equal(f, fr)
+    assert ms > 0

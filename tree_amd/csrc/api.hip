@@ -814,6 +814,73 @@ int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const in
     return TB_OK;
 }
 
+// reference index -> key-order index for sampled targets
+static int map_targets(tb_handle *h, int nt, const int32_t *tspecies, const int32_t *tidx, std::vector<int32_t> &sidx)
+{
+    sidx.assign((size_t)nt, 0);
+    std::vector<int32_t> inv[2];
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        std::vector<int32_t> orig((size_t)S.n);
+        HIPCHK(hipMemcpy(orig.data(), S.orig[S.cur], (size_t)S.n * 4, hipMemcpyDeviceToHost));
+        inv[sp].assign((size_t)S.n, 0);
+        for (int j = 0; j < S.n; j++) inv[sp][orig[j]] = j;
+    }
+    for (int t = 0; t < nt; t++) {
+        int sp = tspecies[t];
+        if (sp != 0 && sp != 1) return fail(TB_ERR_ARG, "tspecies[%d] = %d", t, sp);
+        if (tidx[t] < 0 || tidx[t] >= h->sp[sp].n) return fail(TB_ERR_ARG, "tidx[%d] = %d out of range", t, tidx[t]);
+        sidx[t] = inv[sp][tidx[t]];
+    }
+    return TB_OK;
+}
+
+int tb_force_targets(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f)
+{
+    if (!h || !tspecies || !tidx || !f || nt < 1) return fail(TB_ERR_ARG, "bad argument");
+    if (!h->built || h->tree_walk != TB_WALK_REF64)
+        return fail(TB_ERR_STATE, "tb_force_targets needs trees built for the FP64 walk");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipStreamSynchronize(h->s));
+    std::vector<int32_t> sidx;
+    int rc = map_targets(h, nt, tspecies, tidx, sidx);
+    if (rc) return rc;
+    void *tmp = nullptr;
+    const size_t bytes = (size_t)nt * (8 + 24) + 64;
+    if ((rc = dalloc(h, &tmp, bytes))) return rc;
+    int32_t *d_sp = (int32_t *)tmp, *d_ix = d_sp + nt;
+    double *d_f = (double *)(((uintptr_t)(d_ix + nt) + 15) & ~(uintptr_t)15);
+    hipError_t e = hipMemcpy(d_sp, tspecies, (size_t)nt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ix, sidx.data(), (size_t)nt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        tb_walk64_list(h->s, nt, d_sp, d_ix, 1, h->sp[0], h->sp[1], h->cfg.rs, d_f);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    if (e == hipSuccess) e = hipMemcpy(f, d_f, (size_t)nt * 24, hipMemcpyDeviceToHost);
+    dfree(h, tmp, bytes);
+    return e == hipSuccess ? TB_OK : fail(TB_ERR_HIP, "tb_force_targets: %s", hipGetErrorString(e));
+}
+
+int tb_time_walk(tb_handle *h, int32_t stride, float *ms)
+{
+    if (!h || !ms || stride < 1) return fail(TB_ERR_ARG, "bad argument");
+    if (!h->built || h->tree_walk != TB_WALK_REF64)
+        return fail(TB_ERR_STATE, "tb_time_walk needs trees built for the FP64 walk");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    const int nt = (h->sp[0].n + stride - 1) / stride + (h->sp[1].n + stride - 1) / stride;
+    int32_t *dummy = nullptr;
+    int rc = dalloc_t(h, &dummy, 16);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(h->ev[2], h->s));
+    tb_walk64_list(h->s, nt, nullptr, dummy, stride, h->sp[0], h->sp[1], h->cfg.rs, nullptr);
+    HIPCHK(hipEventRecord(h->ev[3], h->s));
+    HIPCHK(hipEventSynchronize(h->ev[3]));
+    HIPCHK(hipEventElapsedTime(ms, h->ev[2], h->ev[3]));
+    dfree(h, dummy, 64);
+    return TB_OK;
+}
+
 int tb_count_interactions(tb_handle *h, tb_stats *st)
 {
     if (!h || !st) return fail(TB_ERR_ARG, "null argument");

@@ -104,6 +104,9 @@ void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err);
 void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,
                double *epart, int64_t *counters);
 void tb_reduce_partials(hipStream_t s, int nparts, int width, const double *part, double *out);
+// FP64 walk for a target list (tsp/idx, key order) or, with tsp == nullptr, every stride-th target
+void tb_walk64_list(hipStream_t s, int nt, const int32_t *tsp, const int32_t *idx, int stride, const SpeciesDev &E,
+                    const SpeciesDev &I, double rs, double *fout);
 int tb_walk64_blocks(int n);
 void tb_direct(hipStream_t s, int nt, const int32_t *tsp, const int32_t *tidx_sorted, const SpeciesDev &E,
                const SpeciesDev &I, double rs, double *fout);

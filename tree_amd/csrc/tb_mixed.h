@@ -35,7 +35,7 @@ struct Domain {
     const float4 *bq0, *bq1, *bq2;
     float inv_rs, rs;
     int gbase;        // first global group index of this domain
-    int pad_;
+    int nn;           // cells in the tree
 };
 
 struct MixedDev {

@@ -263,8 +263,12 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
     unsigned long long c_visit[2] = {0, 0}, c_m2p[2] = {0, 0}, c_like[2] = {0, 0}, c_unlike[2] = {0, 0};
 
     for (;;) {
-        int g = 0;
-        if (lane == 0) g = atomicAdd(&W.ctr[0], 1);
+        // No branch on the lane index inside this loop: the compiler may
+        // thread such a (loop-invariant) branch across the back edge and split
+        // the wave, leaving lanes 1..63 to walk without lane 0.  So every lane
+        // issues the fetch (lane 0 adds 1, the rest 0) and the stores below
+        // are made by every lane with the same value.
+        int g = atomicAdd(&W.ctr[0], lane == 0 ? 1 : 0);
         g = __shfl(g, 0, 64);
         if (g >= W.ngt) break;
         // the domain holding global group g (descriptors ordered by gbase)
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
         const int nact = min(G, n - base);  // targets in this group
 
         int sp = 1, nm = 0, np = 0;
-        if (lane == 0) L.stack[0] = 0u;  // root: range of one cell at 0
+        L.stack[0] = 0u;  // root: range of one cell at 0
         wave_sync();
         while (sp > 0) {
             // a batch of <= 64 cells from the top ranges of the stack
@@ -418,11 +422,21 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                 c_like[tsp] += (unsigned long long)nl * nact;
                 c_unlike[tsp] += (unsigned long long)(__popcll(mk) - nl) * nact;
             }
-            // opened cells -> their child ranges
+            // opened cells -> their child ranges.  Children follow their
+            // parent in breadth-first order, so a range not strictly after the
+            // cell (or past the table) is a malformed tree: flagged, never
+            // pushed, which bounds every path through the traversal.
+            if (open) {
+                const uint32_t st = info >> 3, len = (info & 7u) + 1u;
+                if (st <= node || st + len > (uint32_t)D.nn) {
+                    atomicOr(&W.ctr[1], 2);
+                    open = false;
+                }
+            }
             mk = __ballot(open);
             const int npush = __popcll(mk);
             if (sp + npush > STACK) {
-                if (lane == 0) atomicOr(&W.ctr[1], 1);
+                atomicOr(&W.ctr[1], 1);
                 sp = 0;
                 nm = np = 0;
                 break;
@@ -462,11 +476,10 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             D.fz[tsp][i] = ff * ((double)A.s[t][2] + (double)A.c[t][2]);
             eg += 0.5 * ff * ((double)A.ps[t] + (double)A.pc[t]);
         }
-        eg = wave_sum(eg);
-        if (lane == 0) {
-            W.gen[2 * g + tsp] = eg;
-            W.gen[2 * g + 1 - tsp] = 0.0;
-        }
+        // lane 0's sum (butterfly sums may differ in the last bit between lanes)
+        eg = __shfl(wave_sum(eg), 0, 64);
+        W.gen[2 * g + tsp] = eg;
+        W.gen[2 * g + 1 - tsp] = 0.0;
     }
     // counter slots (visits, m2p, p2p) x (ee, ei, ie, ii): like P2P and all
     // cell counts go to ee / ii by target species, unlike P2P to ei / ie
@@ -717,7 +730,7 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, const tb_config 
     d.inv_rs = (float)(1.0 / cfg.rs);
     d.rs = (float)cfg.rs;
     d.gbase = gbase;
-    d.pad_ = 0;
+    d.nn = t->nn;
     return d;
 }
 

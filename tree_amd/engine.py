@@ -92,6 +92,8 @@ SIGNATURES = [
     ("tb_step", ctypes.c_int, [_P, _D, _I, _P]),
     ("tb_export_tree", ctypes.c_int, [_P, _I, _I, _P] + [_P] * 12),
     ("tb_direct_sample", ctypes.c_int, [_P, _I, _P, _P, _P]),
+    ("tb_force_targets", ctypes.c_int, [_P, _I, _P, _P, _P]),
+    ("tb_time_walk", ctypes.c_int, [_P, _I, _P]),
     ("tb_count_interactions", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_get_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_mlp_set", ctypes.c_int, [_P, _P, _I]),
@@ -257,6 +259,18 @@ class Engine:
         f = np.zeros((len(tidx), 3))
         _check(self.lib.tb_direct_sample(self.h, len(tidx), _ptr(tspecies), _ptr(tidx), _ptr(f)))
         return f
+
+    def force_targets(self, tspecies, tidx):
+        tspecies = np.ascontiguousarray(tspecies, np.int32)
+        tidx = np.ascontiguousarray(tidx, np.int32)
+        f = np.zeros((len(tidx), 3))
+        _check(self.lib.tb_force_targets(self.h, len(tidx), _ptr(tspecies), _ptr(tidx), _ptr(f)))
+        return f
+
+    def time_walk(self, stride):
+        ms = ctypes.c_float()
+        _check(self.lib.tb_time_walk(self.h, stride, ctypes.byref(ms)))
+        return ms.value
 
     def count_interactions(self):
         s = Stats()
