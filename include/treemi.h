@@ -68,7 +68,7 @@ typedef struct tb_config {
     double err_target; /* TB_MAC_LEARNED: accepted predicted per-interaction error */
     int32_t walk;      /* TB_WALK_*                                               */
     int32_t mac;       /* TB_MAC_*                                                */
-    int32_t group;     /* TB_WALK_MIXED targets per group: 0 = autotuned, else 32..256 */
+    int32_t group;     /* TB_WALK_MIXED targets per group: 0 = autotuned, else 64, 128 or 256 */
     int32_t device;    /* HIP device ordinal                                       */
     int64_t mem_cap;   /* device bytes the handle may hold (0 = unlimited)         */
 } tb_config;

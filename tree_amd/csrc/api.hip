@@ -110,6 +110,10 @@ void free_species(tb_handle *h, SpeciesDev &S)
     dfree(h, S.cnt, n * 4);
     dfree(h, S.offs, n * 4);
     dfree(h, S.root, 64 * 8);
+    dfree(h, S.gpre, n);
+    dfree(h, S.gsuf, n);
+    dfree(h, S.gseg, n * 4);
+    dfree(h, S.grp, n * 8);
     size_t c = (size_t)S.cap;
     dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
@@ -134,6 +138,10 @@ int alloc_species(tb_handle *h, SpeciesDev &S, int n)
     if ((rc = dalloc_t(h, &S.cnt, n))) return rc;
     if ((rc = dalloc_t(h, &S.offs, n))) return rc;
     if ((rc = dalloc_t(h, &S.root, 64))) return rc;
+    if ((rc = dalloc_t(h, &S.gpre, n))) return rc;
+    if ((rc = dalloc_t(h, &S.gsuf, n))) return rc;
+    if ((rc = dalloc_t(h, &S.gseg, n))) return rc;
+    if ((rc = dalloc_t(h, &S.grp, n))) return rc;
     return TB_OK;
 }
 
@@ -260,8 +268,8 @@ int check_cfg(const tb_config *c)
     if (!(c->xm[0] > 0.0) || !(c->xm[1] > 0.0)) return fail(TB_ERR_ARG, "masses must be > 0");
     if (c->walk != TB_WALK_REF64 && c->walk != TB_WALK_MIXED) return fail(TB_ERR_ARG, "unknown walk %d", c->walk);
     if (c->mac != TB_MAC_FIXED && c->mac != TB_MAC_LEARNED) return fail(TB_ERR_ARG, "unknown mac %d", c->mac);
-    if (c->group != 0 && (c->group < 32 || c->group > 256))
-        return fail(TB_ERR_ARG, "group must be 0 or in [32, 256] (got %d)", c->group);
+    if (c->group != 0 && c->group != 64 && c->group != 128 && c->group != 256)
+        return fail(TB_ERR_ARG, "group must be 0, 64, 128 or 256 (got %d)", c->group);
     return TB_OK;
 }
 
@@ -269,8 +277,12 @@ int check_cfg(const tb_config *c)
 // read-back (node totals, depth histograms, error flags).  An ensemble runs
 // phase A for every realisation, synchronises once, then runs phase B.
 constexpr int MISC_J = MISC_HIST + 2 * 2 * HW;  // joint tree: err, total, 2 x HW histograms
+constexpr int MISC_G = MISC_J + 2 + 2 * HW;      // per species: segments, target groups
+constexpr int MISC_N = MISC_G + 4;               // ints read back after phase A
 
-int build_phase_a(tb_handle *h, int *which)
+// Phase A.  tpl = walk-group size / 64 for the mixed walk's target groups
+// (0: chosen from the configuration and the autotuner's table).
+int build_phase_a(tb_handle *h, int *which, int tpl = 0)
 {
     hipStream_t s = h->s;
     int rc;
@@ -278,7 +290,7 @@ int build_phase_a(tb_handle *h, int *which)
     const int nj = h->sp[0].n + h->sp[1].n;
     if ((rc = ensure_tmp(h, mixed ? nj : std::max(h->sp[0].n, h->sp[1].n)))) return rc;
     if (mixed && (rc = ensure_joint(h, nj))) return rc;
-    HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * (MISC_J + 2 + 2 * HW), s));
+    HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * MISC_N, s));
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
         const int c = S.cur;
@@ -286,6 +298,9 @@ int build_phase_a(tb_handle *h, int *which)
         tb_keys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
         tb_radix_sort(s, S.n, S.key, S.val, 0, 64, h->sort_tmp, &which[sp], false);
         tb_permute(s, S, S.val[which[sp]], h->d_misc + sp);
+        S.skey = S.key[which[sp]];
+        if (mixed) tb_groups(s, S, 64 * (tpl ? tpl : tb_mixed_tpl(h->cfg, nj)), h->d_misc + MISC_G + 2 * sp,
+                             h->scan_tmp);
         if (!mixed) {
             tb_lcp(s, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
             tb_scan_excl(s, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, h->scan_tmp);
@@ -309,7 +324,7 @@ int build_phase_a(tb_handle *h, int *which)
         tb_scan_excl(s, nj, J.cnt, J.offs, h->d_misc + MISC_J + 1, h->scan_tmp);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, sizeof(int32_t) * (MISC_J + 2 + 2 * HW), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h->h_misc, h->d_misc, sizeof(int32_t) * MISC_N, hipMemcpyDeviceToHost, s));
     return TB_OK;
 }
 
@@ -328,6 +343,7 @@ int build_phase_b(tb_handle *h, const int *which)
                         sp == 0 ? "electrons" : (sp == 1 ? "ions" : "joint tree"), TB_LEVELS);
     }
     if (mixed) {
+        for (int sp = 0; sp < 2; sp++) h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];
         JointDev &J = h->jt;
         const int32_t *hist = h->h_misc + MISC_J + 2;
         J.nn = 1 + h->h_misc[MISC_J + 1];
@@ -395,6 +411,19 @@ int build_phase_b(tb_handle *h, const int *which)
     HIPCHK(hipGetLastError());
     h->built = true;
     h->tree_walk = TB_WALK_REF64;
+    return TB_OK;
+}
+
+// Cuts both species' target groups again, for walk-group size 64 * tpl, from
+// the sorted keys the last build kept; synchronises to read the counts.
+int regroup(tb_handle *h, int tpl)
+{
+    for (int sp = 0; sp < 2; sp++)
+        tb_groups(h->s, h->sp[sp], 64 * tpl, h->d_misc + MISC_G + 2 * sp, h->scan_tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->h_misc + MISC_G, h->d_misc + MISC_G, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int sp = 0; sp < 2; sp++) h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];
     return TB_OK;
 }
 
@@ -976,12 +1005,17 @@ int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out)
     const int saved = h->cfg.group;
     float best = 1e30f;
     int best_tpl = 2;
+    int rc;
     for (int tpl = 1; tpl <= 4; tpl *= 2) {
         h->cfg.group = 64 * tpl;
+        if ((rc = regroup(h, tpl))) {
+            h->cfg.group = saved;
+            return rc;
+        }
         float ms = 0.0f;
         for (int rep = 0; rep < 3; rep++) {
             HIPCHK(hipEventRecord(h->ev[2], h->s));
-            int rc = force_all(h, false);
+            rc = force_all(h, false);
             if (rc) {
                 h->cfg.group = saved;
                 return rc;
@@ -1001,6 +1035,7 @@ int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out)
     }
     h->cfg.group = saved;
     tb_autotune_record(h->sp[0].n + h->sp[1].n, best_tpl);
+    if ((rc = regroup(h, tb_mixed_tpl(h->cfg, h->sp[0].n + h->sp[1].n)))) return rc;
     if (group_out) *group_out = 64 * best_tpl;
     return TB_OK;
 }
@@ -1022,13 +1057,30 @@ struct tb_ensemble {
 
 namespace {
 
+int ens_tpl(const tb_ensemble *E)
+{
+    int ntot = 0;
+    for (auto *h : E->h) ntot += h->sp[0].n + h->sp[1].n;
+    return tb_mixed_tpl(E->cfg, ntot / (int)E->h.size());
+}
+
+int ens_regroup(tb_ensemble *E, int tpl)
+{
+    int rc;
+    for (auto *h : E->h)
+        if ((rc = regroup(h, tpl))) return rc;
+    return TB_OK;
+}
+
 int ens_build_all(tb_ensemble *E)
 {
     const int n = (int)E->h.size();
     std::vector<int> which(3 * (size_t)n);
     int rc;
+    // one walk launch serves every realisation, so all are grouped for one size
+    const int tpl = ens_tpl(E);
     for (int r = 0; r < n; r++)
-        if ((rc = build_phase_a(E->h[r], &which[3 * r]))) return rc;
+        if ((rc = build_phase_a(E->h[r], &which[3 * r], tpl))) return rc;
     HIPCHK(hipStreamSynchronize(E->s));
     for (int r = 0; r < n; r++)
         if ((rc = build_phase_b(E->h[r], &which[3 * r]))) return rc;
@@ -1040,11 +1092,13 @@ int ens_force(tb_ensemble *E)
     const int n = (int)E->h.size();
     std::vector<SpeciesDev *> sp(2 * (size_t)n);
     std::vector<const MixedTree *> tr((size_t)n);
+    std::vector<double> rs((size_t)n);
     for (int r = 0; r < n; r++) {
         for (int k = 0; k < 2; k++) sp[2 * r + k] = &E->h[r]->sp[k];
         tr[r] = &E->h[r]->mx.t[0];
+        rs[r] = E->h[r]->cfg.rs;  // each realisation's own Kelbg length
     }
-    int rc = tb_mixed_force_multi(E->s, E->mx, sp.data(), tr.data(), n, E->cfg, E->d_esum);
+    int rc = tb_mixed_force_multi(E->s, E->mx, sp.data(), tr.data(), rs.data(), n, E->cfg, E->d_esum);
     if (rc) return fail(rc, "%s", tb_mixed_error());
     for (int r = 0; r < n; r++) E->h[r]->have_force = true;
     return TB_OK;
@@ -1205,12 +1259,17 @@ int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
     int best_tpl = 2, ntot = 0;
     for (auto *h : E->h) ntot += h->sp[0].n + h->sp[1].n;
     hipEvent_t a = E->h[0]->ev[4], b = E->h[0]->ev[5];
+    int rc;
     for (int tpl = 1; tpl <= 4; tpl *= 2) {
         E->cfg.group = 64 * tpl;
+        if ((rc = ens_regroup(E, tpl))) {
+            E->cfg.group = saved;
+            return rc;
+        }
         float ms = 0.0f;
         for (int rep = 0; rep < 3; rep++) {
             HIPCHK(hipEventRecord(a, E->s));
-            int rc = ens_force(E);
+            rc = ens_force(E);
             if (rc) {
                 E->cfg.group = saved;
                 return rc;
@@ -1230,6 +1289,7 @@ int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
     }
     E->cfg.group = saved;
     tb_autotune_record(ntot / (int)E->h.size(), best_tpl);
+    if ((rc = ens_regroup(E, ens_tpl(E)))) return rc;
     if (group_out) *group_out = 64 * best_tpl;
     return TB_OK;
 }

@@ -740,6 +740,115 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
     for (int c = 0; c < 6; c++) o[4 + c] = M[c];
 }
 
+// ------------------------------------------------ mixed-walk target groups
+//
+// The mixed walk treats 64*TPL targets of one species as one group and
+// accepts a cell only when it is far from the group's bounding box.  Groups
+// of consecutive key-ordered particles that straddle a coarse cell boundary
+// get boxes spanning a large part of the domain and walk almost every leaf.
+// So the key-ordered sequence is first cut into segments, each inside one
+// cell of at most W = 8G particles: a cut goes between j-1 and j when the
+// deepest cell holding both has more than W particles.  Each segment (at most
+// W long) is then split into ceil(m/G) near-equal groups.
+//
+// The cell at level d holding j has more than W particles exactly when some
+// window key[s..s+W] with s in [j-W, j] shares d leading digits, so with
+// w[s] = lcp(key[s], key[s+W]) the cut test is lcp(key[j-1], key[j]) <=
+// max(w[j-W..j]).  That window maximum is max(suffix max of w over j-W's
+// block of W, prefix max of w over j's block), both from k_gwin.
+
+__global__ __launch_bounds__(BT) void k_gwin(int n, int W, const uint64_t *__restrict__ key,
+                                             int8_t *__restrict__ pre, int8_t *__restrict__ suf)
+{
+    __shared__ int tp[BT], ts[BT];
+    const int E = W / BT;  // 2, 4 or 8 entries per thread
+    const long base = (long)blockIdx.x * W + (long)threadIdx.x * E;
+    int w[8];
+    int run = -1;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        w[e] = -1;
+        const long q = base + e;
+        if (e < E && q + W < n) w[e] = lcp_digits(key[q], key[q + W]);
+        run = max(run, w[e]);
+    }
+    tp[threadIdx.x] = run;
+    ts[threadIdx.x] = run;
+    __syncthreads();
+    // inclusive max-scans of the per-thread maxima, forwards and backwards
+    for (int o = 1; o < BT; o <<= 1) {
+        const int a = threadIdx.x >= o ? tp[threadIdx.x - o] : -1;
+        const int c = threadIdx.x + o < BT ? ts[threadIdx.x + o] : -1;
+        __syncthreads();
+        tp[threadIdx.x] = max(tp[threadIdx.x], a);
+        ts[threadIdx.x] = max(ts[threadIdx.x], c);
+        __syncthreads();
+    }
+    int m = threadIdx.x > 0 ? tp[threadIdx.x - 1] : -1;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        m = max(m, w[e]);
+        if (e < E && base + e < n) pre[base + e] = (int8_t)m;
+    }
+    m = threadIdx.x + 1 < BT ? ts[threadIdx.x + 1] : -1;
+#pragma unroll
+    for (int e = 7; e >= 0; e--) {
+        m = max(m, w[e]);
+        if (e < E && base + e < n) suf[base + e] = (int8_t)m;
+    }
+}
+
+// flag[j] = 1 where a segment starts
+__global__ __launch_bounds__(BT) void k_gflag(int n, int W, const uint64_t *__restrict__ key,
+                                              const int8_t *__restrict__ pre, const int8_t *__restrict__ suf,
+                                              int32_t *__restrict__ flag)
+{
+    const int j = blockIdx.x * BT + threadIdx.x;
+    if (j >= n) return;
+    int f = 1;
+    if (j > 0) {
+        int mx = pre[j];
+        if (j >= W) mx = max(mx, (int)suf[j - W]);
+        f = lcp_digits(key[j - 1], key[j]) <= mx ? 1 : 0;
+    }
+    flag[j] = f;
+}
+
+__global__ __launch_bounds__(BT) void k_gseg(int n, const int32_t *__restrict__ flag,
+                                             const int32_t *__restrict__ idx, int32_t *__restrict__ seg)
+{
+    const int j = blockIdx.x * BT + threadIdx.x;
+    if (j < n && flag[j]) seg[idx[j]] = j;
+}
+
+// groups per segment (zero past the last segment, so the scan can run over n)
+__global__ __launch_bounds__(BT) void k_gcount(int n, int G, const int32_t *__restrict__ nseg,
+                                               const int32_t *__restrict__ seg, int32_t *__restrict__ cnt)
+{
+    const int k = blockIdx.x * BT + threadIdx.x;
+    if (k >= n) return;
+    const int S = *nseg;
+    if (k >= S) {
+        cnt[k] = 0;
+        return;
+    }
+    const int m = (k + 1 < S ? seg[k + 1] : n) - seg[k];
+    cnt[k] = (m + G - 1) / G;
+}
+
+__global__ __launch_bounds__(BT) void k_gemit(int n, const int32_t *__restrict__ nseg,
+                                              const int32_t *__restrict__ seg, const int32_t *__restrict__ cnt,
+                                              const int32_t *__restrict__ off, int2 *__restrict__ grp)
+{
+    const int k = blockIdx.x * BT + threadIdx.x;
+    if (k >= n || k >= *nseg) return;
+    const int st = seg[k], m = (k + 1 < *nseg ? seg[k + 1] : n) - st, p = cnt[k], o = off[k];
+    for (int q = 0; q < p; q++) {
+        const int a = st + (int)((long)q * m / p), b = st + (int)((long)(q + 1) * m / p);
+        grp[o + q] = make_int2(a, b - a);
+    }
+}
+
 inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
 
 }  // namespace
@@ -865,6 +974,19 @@ void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x
         src.z[k] = z[k];
     }
     hipLaunchKernelGGL(k_jgather, dim3(nblk(n, BT)), dim3(BT), 0, s, n, val, src, pos);
+}
+
+void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp)
+{
+    const int n = S.n, W = 8 * G;
+    hipLaunchKernelGGL(k_gwin, dim3(nblk(n, W)), dim3(BT), 0, s, n, W, S.skey, S.gpre, S.gsuf);
+    hipLaunchKernelGGL(k_gflag, dim3(nblk(n, BT)), dim3(BT), 0, s, n, W, S.skey, S.gpre, S.gsuf, S.cnt);
+    tb_scan_excl(s, n, S.cnt, S.offs, totals, scan_tmp);
+    hipLaunchKernelGGL(k_gseg, dim3(nblk(n, BT)), dim3(BT), 0, s, n, S.cnt, S.offs, S.gseg);
+    hipLaunchKernelGGL(k_gcount, dim3(nblk(n, BT)), dim3(BT), 0, s, n, G, totals, S.gseg, S.cnt);
+    tb_scan_excl(s, n, S.cnt, S.offs, totals + 1, scan_tmp);
+    hipLaunchKernelGGL(k_gemit, dim3(nblk(n, BT)), dim3(BT), 0, s, n, totals, S.gseg, S.cnt, S.offs, S.grp);
+    S.gG = G;
 }
 
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)

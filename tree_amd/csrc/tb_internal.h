@@ -38,11 +38,13 @@ struct SpeciesDev {
     uint32_t *bfs = nullptr;    // preorder ids grouped by depth, preorder within a depth
     int lvl_off[TB_LEVELS + 3] = {};
     int twig_hist[TB_LEVELS + 2] = {};
-    // mixed walk: breadth-first FP32 records (filled by the mixed-walk prep)
-    float4 *wmac = nullptr;     // centroid hi xyz, rcrit (fp32, rounded up)
-    float4 *wlo = nullptr;      // centroid lo xyz, child info bits
-    float4 *wmom = nullptr;     // [nn][2] q, M2xx, M2yy, M2zz | M2xy, M2yz, M2zx, 0
-    uint32_t *inv = nullptr;    // preorder -> breadth-first index
+    // mixed walk: target groups of at most gG consecutive key-ordered
+    // particles, each inside one cell of at most 8 gG particles (tb_groups)
+    const uint64_t *skey = nullptr;  // this build's sorted keys
+    int8_t *gpre = nullptr, *gsuf = nullptr;  // windowed lcp maxima
+    int32_t *gseg = nullptr;    // segment starts
+    int2 *grp = nullptr;        // group start, length
+    int ng = 0, gG = 0;         // groups, and the size bound they were cut for
 };
 
 // The joint tree over both species (mixed walk): nodes in preorder, expansion
@@ -99,6 +101,9 @@ void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x
                 const double *const z[2], double4 *pos);
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key);
 void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err);
+// target groups of at most G particles over S.skey; totals[0] = segments,
+// totals[1] = groups (device); uses S.cnt and S.offs as scratch
+void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp);
 
 // ----- walk64.hip
 void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,

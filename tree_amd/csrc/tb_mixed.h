@@ -29,6 +29,7 @@ struct BandRec {
 struct Domain {
     const double *px[2], *py[2], *pz[2];
     double *fx[2], *fy[2], *fz[2];
+    const int2 *grp[2];   // target groups (start, length) per species
     int n[2], ng[2];
     const double4 *bmac;
     const uint32_t *binfo;
@@ -66,10 +67,14 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev 
 const char *tb_mixed_error();
 
 // one persistent launch over several domains (the batched ensemble): fills
-// esum[4 * d] per domain; rec/collect are not used
-int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees, int ndom,
+// esum[4 * d] per domain; rs[d] is domain d's Kelbg length; rec/collect are
+// not used
+int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees,
+                         const double *rs, int ndom,
                          const tb_config &cfg, double *esum_out_dev);
 
 // walk-group size chosen by the autotuner for a problem size (targets per lane: 1, 2 or 4)
 int tb_autotune_tpl(int n_total);
+// targets per lane for a configuration: its group size, else the autotuner's choice
+int tb_mixed_tpl(const tb_config &cfg, int n_total);
 void tb_autotune_record(int n_total, int tpl);

@@ -282,8 +282,9 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
         const int gl = g - D.gbase;
         const int tsp = gl < D.ng[0] ? 0 : 1;
         const float qt = tsp ? 1.0f : -1.0f;
-        const int base = (tsp ? gl - D.ng[0] : gl) * G;
-        const int n = D.n[tsp];
+        const int2 gr = D.grp[tsp][tsp ? gl - D.ng[0] : gl];
+        const int base = gr.x;
+        const int nact = gr.y;  // targets in this group (at most G)
         const double *px = D.px[tsp], *py = D.py[tsp], *pz = D.pz[tsp];
 
         // targets and the group's bounding box (FP64)
@@ -292,9 +293,8 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
         double mnx = 1e300, mny = 1e300, mnz = 1e300, mxx = -1e300, mxy = -1e300, mxz = -1e300;
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
-            const int i = base + lane + 64 * t;
-            act[t] = i < n;
-            const int ii = act[t] ? i : base;
+            act[t] = lane + 64 * t < nact;
+            const int ii = act[t] ? base + lane + 64 * t : base;
             tx[t] = px[ii];
             ty[t] = py[ii];
             tz[t] = pz[ii];
@@ -322,7 +322,6 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             for (int k = 0; k < 3; k++) A.s[t][k] = A.c[t][k] = 0.0f;
             A.ps[t] = A.pc[t] = 0.0f;
         }
-        const int nact = min(G, n - base);  // targets in this group
 
         int sp = 1, nm = 0, np = 0;
         L.stack[0] = 0u;  // root: range of one cell at 0
@@ -401,7 +400,7 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                     }
                 }
             }
-            if (COUNT && have) c_visit[tsp]++;
+            if (COUNT) c_visit[tsp] += (unsigned long long)__popcll(__ballot(have));
             // accepted cells -> M2P list
             uint64_t mk = __ballot(acc);
             if (acc) {
@@ -601,7 +600,7 @@ __global__ void k_finish(const double *__restrict__ esum, const int32_t *__restr
 struct LabelArgs {
     const BandRec *rec;
     int nrec;
-    int G;                 // targets per group of the collecting walk
+    const int2 *grp[2];    // target groups (start, length) per species
     int ng0;               // electron groups
     const double *px[2], *py[2], *pz[2];  // key-ordered target positions
     const double *fx[2], *fy[2], *fz[2];  // forces of the collecting walk
@@ -626,7 +625,7 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const BandRec R = A.rec[r];
     const int tsp = (int)R.group < A.ng0 ? 0 : 1;
     const double qt = tsp ? 1.0 : -1.0;
-    const int base = ((int)R.group - (tsp ? A.ng0 : 0)) * A.G;
+    const int2 gr = A.grp[tsp][(int)R.group - (tsp ? A.ng0 : 0)];
     const uint32_t p = A.bfs[R.cell];
     const double4 c = A.nc[p];
     const double *Q = A.nq + (size_t)p * 10;
@@ -636,9 +635,8 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const double Qxy = 3.0 * Q[7], Qyz = 3.0 * Q[8], Qzx = 3.0 * Q[9];
     const int s0 = A.nstart[p], cnt = A.ncount[p];
     double worst = 0.0;
-    for (int t = threadIdx.x; t < A.G; t += 64) {
-        const int i = base + t;
-        if (i >= A.n[tsp]) break;
+    for (int t = threadIdx.x; t < gr.y; t += 64) {
+        const int i = gr.x + t;
         const double x = A.px[tsp][i], y = A.py[tsp][i], z = A.pz[tsp][i];
         const double dx = x - c.x, dy = y - c.y, dz = z - c.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
@@ -697,10 +695,16 @@ int grow(T **p, int *cap, int want, size_t *bytes)
     return TB_OK;
 }
 
-int pick_tpl(const tb_config &cfg, int n)
+// targets per lane the species' groups were cut for (0 with g_err set if
+// they were not cut, or not for one size)
+int group_tpl(const SpeciesDev &e, const SpeciesDev &i)
 {
-    if (cfg.group == 64 || cfg.group == 128 || cfg.group == 256) return cfg.group / 64;
-    return tb_autotune_tpl(n);
+    const int G = e.gG;
+    if (G != i.gG || (G != 64 && G != 128 && G != 256) || e.ng < 1 || i.ng < 1) {
+        g_err = "mixed walk: target groups were not built for this walk";
+        return 0;
+    }
+    return G / 64;
 }
 
 int resident_waves()
@@ -716,7 +720,7 @@ int resident_waves()
     return w;
 }
 
-Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, const tb_config &cfg, int G, int gbase)
+Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int gbase)
 {
     Domain d;
     for (int k = 0; k < 2; k++) {
@@ -724,11 +728,12 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, const tb_config 
         d.px[k] = sp[k]->x[c][0]; d.py[k] = sp[k]->x[c][1]; d.pz[k] = sp[k]->x[c][2];
         d.fx[k] = sp[k]->f[c][0]; d.fy[k] = sp[k]->f[c][1]; d.fz[k] = sp[k]->f[c][2];
         d.n[k] = sp[k]->n;
-        d.ng[k] = (sp[k]->n + G - 1) / G;
+        d.ng[k] = sp[k]->ng;
+        d.grp[k] = sp[k]->grp;
     }
     d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2;
-    d.inv_rs = (float)(1.0 / cfg.rs);
-    d.rs = (float)cfg.rs;
+    d.inv_rs = (float)(1.0 / rs);
+    d.rs = (float)rs;
     d.gbase = gbase;
     d.nn = t->nn;
     return d;
@@ -804,9 +809,10 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
 int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *d_sums, int64_t *d_cnt,
            int mode)
 {
-    const int tpl = pick_tpl(cfg, sp[0].n + sp[1].n);
+    const int tpl = group_tpl(sp[0], sp[1]);
+    if (!tpl) return TB_ERR_STATE;
     SpeciesDev *const spp[2] = {&sp[0], &sp[1]};
-    std::vector<Domain> doms(1, make_domain(spp, &M.t[0], cfg, 64 * tpl, 0));
+    std::vector<Domain> doms(1, make_domain(spp, &M.t[0], cfg.rs, 0));
     int rc = launch_domains(s, M, doms, tpl, cfg, d_cnt, mode);
     if (rc) return rc;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, M.esum, M.ctr, d_sums);
@@ -895,13 +901,13 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev 
     LabelArgs A;
     A.rec = M.rec;
     A.nrec = nrec;
-    A.G = 64 * M.tpl;
-    A.ng0 = (sp[0].n + A.G - 1) / A.G;
+    A.ng0 = sp[0].ng;
     for (int k = 0; k < 2; k++) {
         const int c = sp[k].cur;
         A.px[k] = sp[k].x[c][0]; A.py[k] = sp[k].x[c][1]; A.pz[k] = sp[k].x[c][2];
         A.fx[k] = sp[k].f[c][0]; A.fy[k] = sp[k].f[c][1]; A.fz[k] = sp[k].f[c][2];
         A.n[k] = sp[k].n;
+        A.grp[k] = sp[k].grp;
     }
     A.bfs = jt->bfs;
     A.nc = jt->nc;
@@ -929,18 +935,24 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev 
     return TB_OK;
 }
 
-int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees, int ndom,
+int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees,
+                         const double *rs, int ndom,
                          const tb_config &cfg, double *esum_out_dev)
 {
-    int ntot = 0;
-    for (int d = 0; d < ndom; d++) ntot += sp[2 * d]->n + sp[2 * d + 1]->n;
-    const int tpl = pick_tpl(cfg, ntot / std::max(ndom, 1));
+    // one kernel instantiation for every domain: all were grouped for one size
+    const int tpl = group_tpl(*sp[0], *sp[1]);
+    for (int d = 1; d < ndom && tpl; d++)
+        if (group_tpl(*sp[2 * d], *sp[2 * d + 1]) != tpl) {
+            g_err = "mixed walk: realisations grouped for different walk-group sizes";
+            return TB_ERR_STATE;
+        }
+    if (!tpl) return TB_ERR_STATE;
     std::vector<Domain> doms;
     doms.reserve(ndom);
     int gb = 0;
     for (int d = 0; d < ndom; d++) {
         SpeciesDev *const spp[2] = {sp[2 * d], sp[2 * d + 1]};
-        doms.push_back(make_domain(spp, trees[d], cfg, 64 * tpl, gb));
+        doms.push_back(make_domain(spp, trees[d], rs[d], gb));
         gb += doms.back().ng[0] + doms.back().ng[1];
     }
     int rc = launch_domains(s, M, doms, tpl, cfg, nullptr, MODE_FIXED);
@@ -985,6 +997,12 @@ static int size_bucket(int n)
     int b = 0;
     while ((1 << (b + 1)) <= n && b < 62) b++;
     return b;
+}
+
+int tb_mixed_tpl(const tb_config &cfg, int n_total)
+{
+    if (cfg.group == 64 || cfg.group == 128 || cfg.group == 256) return cfg.group / 64;
+    return tb_autotune_tpl(n_total);
 }
 
 int tb_autotune_tpl(int n_total)
