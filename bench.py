@@ -11,9 +11,10 @@ against an FP64 direct sum on 4096 sampled targets (2048 e- + 2048 ions, seed
   * plain FP64: the reference's algorithm (per-target MAC, child-point
     moments, two species trees).  Its theta is bisected on the sampled targets
     only, and its step time is measured with the walk timed on every S-th
-    target and scaled to all targets (per-target walks are independent), plus
-    the measured build and Verlet times.  At 1e-4 its theta is small and a
-    full step takes minutes, which is why it is sampled.
+    target (at least 2^17 targets per species, enough to fill the GPU) and
+    scaled to all targets (per-target walks are independent), plus the
+    measured build and Verlet times.  At 1e-4 its theta is small and a full
+    step takes minutes, which is why it is sampled.
   * mixed: the wavefront-group walk on the joint tree, theta bisected on full
     force evaluations; K full steps timed.
   * mixed+learned (headline): the MLP opening criterion, trained on the GPU
@@ -45,7 +46,10 @@ W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # (monopole + dipole + traceless-quadrupole M2P 72, Coulomb P2P 18, Kelbg P2P
 # 30); MAC tests are per cell and group and are not counted
 W32_M2P, W32_P2P_C, W32_P2P_K = 72, 18, 30
-PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X vector
+# TFLOP/s, MI355X published vector peaks.  The FP32 figure counts packed FP32
+# (two lanes per instruction); this build keeps packed FP32 off, so one
+# instruction per lane per cycle (78.6) is its own ceiling, stated beside it.
+PEAK = {"fp64": 78.6, "fp32": 157.3, "fp32_nonpacked": 78.6}
 TARGET = 1e-4
 
 
@@ -123,18 +127,19 @@ def timed_steps(eng, dt, steps, warmup):
     e0 = 0.5 * (k0[0] + k0[1]) + ep[0] + ep[1]
     if warmup:
         eng.step(dt, warmup)
-    ms_force, ms_build = [], []
+    ms_force, ms_build, ms_walk = [], [], []
     t0 = time.perf_counter()
     out = None
     for _ in range(steps):
-        out = eng.step(dt, 1)
+        out = eng.step(dt, 1)  # returns after the stream has synchronised
         st = eng.stats()
         ms_force.append(st["ms_force"])
         ms_build.append(st["ms_build"])
+        ms_walk.append(st["ms_walk"])
     t1 = time.perf_counter()
     e1 = 0.5 * (out[2] + out[3]) + out[0] + out[1]
     return {"s": (t1 - t0) / steps, "ms_force": float(np.mean(ms_force)), "ms_build": float(np.mean(ms_build)),
-            "drift": (e1 - e0) / abs(e0)}
+            "ms_walk": float(np.mean(ms_walk)), "drift": (e1 - e0) / abs(e0)}
 
 
 def cpu_baseline(qel, qion, theta, rs, threads):
@@ -143,16 +148,22 @@ def cpu_baseline(qel, qion, theta, rs, threads):
         from oracle import pyoracle
     except Exception as ex:  # pragma: no cover
         return {"value": None, "error": str(ex)}
-    stride = 1024
     n = len(qel) + len(qion)
-    if pyoracle.ref_available():
-        buf = pyoracle.run_ref(3, qel["xx"], qel["xv"], qion["xx"], qion["xv"], theta, rs, (ta.XM_E, ta.XM_I),
-                               0.0, stride=stride, threads=threads)
-        tb, tw, nw = struct.unpack_from("ddi", buf, 0)
-        kind = "reference"
-    else:
-        tb, tw, nw = pyoracle.time_sample(qel["xx"], qion["xx"], theta, rs, stride, threads)
-        kind = "port"
+    kind = "reference" if pyoracle.ref_available() else "port"
+
+    def run(stride):
+        if kind == "reference":
+            buf = pyoracle.run_ref(3, qel["xx"], qel["xv"], qion["xx"], qion["xv"], theta, rs, (ta.XM_E, ta.XM_I),
+                                   0.0, stride=stride, threads=threads)
+            return struct.unpack_from("ddi", buf, 0)
+        return pyoracle.time_sample(qel["xx"], qion["xx"], theta, rs, stride, threads)
+
+    # ~128 targets first; if their walk is short, widen the sample to ~15 s of walking
+    stride = max(1, len(qel) // 64)
+    tb, tw, nw = run(stride)
+    if tw < 5.0 and stride > 1:
+        stride = max(1, int(stride * max(tw, 1e-3) / 15.0))
+        tb, tw, nw = run(stride)
     t_step = tb + tw * n / nw
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": kind,
             "sample": "one serial tree build of both species (%.1f s) + walk of every %d-th target (%d targets, "
@@ -170,16 +181,15 @@ def run_plain(qel, qion, rs, tsp, tix, fdir, a):
     else:
         log("tuning the plain FP64 walk (sampled targets)")
         th, rms, mx = bisect(lambda v: eng.configure(theta=v), lambda: err_sampled64(eng, tsp, tix, fdir),
-                             0.01, 0.6, 9)
+                             0.01, 0.3, 7)
     eng.build()
     ms_build = eng.stats()["ms_build"]
-    # sampled walk timing: shrink the stride until the sample takes >= 0.3 s
+    # sampled walk timing: every stride-th target of each species, at least
+    # 2^17 per species so that the sample fills the GPU (the scaled time is a
+    # throughput, not one wave's latency)
     n = len(qel) + len(qion)
-    stride = 1 << 14
+    stride = max(1, min(len(qel), len(qion)) // (1 << 17))
     ms = eng.time_walk(stride)
-    while ms < 300.0 and stride > 1:
-        stride = max(1, stride // 8)
-        ms = eng.time_walk(stride)
     nsamp = -(-len(qel) // stride) + -(-len(qion) // stride)
     ms_walk = ms * n / nsamp
     t0 = time.perf_counter()
@@ -194,7 +204,8 @@ def run_plain(qel, qion, rs, tsp, tix, fdir, a):
             "ms_build": ms_build, "stride": stride, "prec": "fp64"}
 
 
-def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a, learned_params=None):
+def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
+    """The fixed-theta mixed walk, theta bisected to the error target."""
     eng = ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=0.5, group=a.group)
     eng.set_particles(qel, qion)
     if a.theta32 > 0:
@@ -204,45 +215,29 @@ def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a, learned_params=None):
     else:
         log("tuning the mixed walk")
         th, rms, mx = bisect(lambda v: eng.configure(theta_mixed=v), lambda: err_full(eng, tsp, tix, fdir),
-                             0.15, 1.0, 7)
+                             0.05, 1.0, 8)
     eng.build()
     if a.group == 0:
         g, ms = eng.autotune()
         log("  autotuned group size %d (ms per walk for 64/128/256: %s)" % (g, np.round(ms, 2)))
     eng.build()
+    eng.force()
     st = eng.count_interactions()
     eng.set_particles(qel, qion)
     r = timed_steps(eng, dt, a.steps, a.warmup)
-    fixed = {"variant": "mixed", "theta": th, "rms": rms, "max": mx, "st": st, "prec": "fp32", **r}
-    log("mixed: theta=%.4f rms=%.2e %.3f steps/s walk %.2f ms" % (th, rms, 1 / r["s"], r["ms_force"]))
-    learned = None
-    if learned_params is not None:
-        eng.set_particles(qel, qion)
-        eng.mlp_set(learned_params)
-        th_lo, th_hi = 0.7 * th, min(1.0, 1.6 * th)
-        eng.configure(mac="learned", theta_lo=th_lo, theta_mixed=th_hi, err_target=1e-5)
-        log("tuning the learned criterion's error threshold (band theta %.3f .. %.3f)" % (th_lo, th_hi))
-        et, rms_l, mx_l = bisect(lambda v: eng.configure(err_target=v), lambda: err_full(eng, tsp, tix, fdir),
-                                 1e-9, 1e-2, 8)
-        eng.build()
-        st_l = eng.count_interactions()
-        eng.set_particles(qel, qion)
-        r = timed_steps(eng, dt, a.steps, a.warmup)
-        learned = {"variant": "mixed+learned", "theta": th_hi, "theta_lo": th_lo, "err_target": et, "rms": rms_l,
-                   "max": mx_l, "st": st_l, "prec": "fp32", **r}
-        log("mixed+learned: err_target=%.2e rms=%.2e %.3f steps/s walk %.2f ms"
-            % (et, rms_l, 1 / r["s"], r["ms_force"]))
     eng.close()
-    return fixed, learned
+    log("mixed: theta=%.4f rms=%.2e %.3f steps/s walk %.2f ms" % (th, rms, 1 / r["s"], r["ms_walk"]))
+    return {"variant": "mixed", "theta": th, "rms": rms, "max": mx, "st": st, "prec": "fp32", **r}
 
 
-def train_mlp(a):
-    """Labels from the method itself on a 2^20-particle plasma, then GPU training."""
+def train_mlp(a, th_lo, th_hi):
+    """Labels from the method itself on a 2^20-particle plasma (the band the
+    learned walk will decide), then GPU training."""
     from oracle import mlp_ref  # the seeded initial parameter vector only
     n = 1 << 19
     qel, qion = ta.gen_plasma(a.seed + 1, n, n, 400.0, te=a.te, ti=1e-3)
     rs = ta.kelbg_length(a.te)
-    with ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=0.8, theta_lo=0.3) as eng:
+    with ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=th_hi, theta_lo=th_lo) as eng:
         eng.set_particles(qel, qion)
         eng.build()
         X, y = eng.mlp_collect(cap=1 << 20)
@@ -250,7 +245,36 @@ def train_mlp(a):
         hist = eng.mlp_train(X, y, steps=4000, batch=8192, lr=2e-3, seed=1)
         p = eng.mlp_get()
     log("  MLP: %d labelled band pairs, loss %.3f -> %.3f" % (len(X), hist[:20].mean(), hist[-50:].mean()))
-    return p, {"samples": int(len(X)), "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean())}
+    return p, {"samples": int(len(X)), "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
+               "band_theta": [th_lo, th_hi], "train_n_total": 2 * n}
+
+
+def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
+    """The learned criterion in the band theta_lo .. theta_hi, its error threshold bisected to the target."""
+    eng = ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=th_hi, theta_lo=th_lo, mac="learned",
+                    err_target=1e-9, group=a.group)
+    eng.set_particles(qel, qion)
+    eng.mlp_set(params)
+    eng.build()
+    eng.force()  # the fixed-theta fallback: gives the first previous forces
+    if a.err_target > 0:
+        et = a.err_target
+        eng.configure(err_target=et)
+        rms_l, mx_l = err_full(eng, tsp, tix, fdir)
+    else:
+        log("tuning the learned criterion's error threshold (band theta %.3f .. %.3f)" % (th_lo, th_hi))
+        et, rms_l, mx_l = bisect(lambda v: eng.configure(err_target=v), lambda: err_full(eng, tsp, tix, fdir),
+                                 1e-9, 1e-1, 9)
+    eng.build()
+    eng.force()
+    st_l = eng.count_interactions()
+    eng.set_particles(qel, qion)
+    r = timed_steps(eng, dt, a.steps, a.warmup)
+    eng.close()
+    log("mixed+learned: err_target=%.2e rms=%.2e %.3f steps/s walk %.2f ms"
+        % (et, rms_l, 1 / r["s"], r["ms_walk"]))
+    return {"variant": "mixed+learned", "theta": th_hi, "theta_lo": th_lo, "err_target": et, "rms": rms_l,
+            "max": mx_l, "st": st_l, "prec": "fp32", **r}
 
 
 def main():
@@ -268,6 +292,8 @@ def main():
     ap.add_argument("--theta64", type=float, default=0.0, help="fixed theta for the FP64 walk (skip tuning)")
     ap.add_argument("--theta32", type=float, default=0.0, help="fixed theta for the mixed walk (skip tuning)")
     ap.add_argument("--group", type=int, default=0, help="mixed-walk targets per group (0 = autotuned)")
+    ap.add_argument("--err-target", type=float, default=0.0,
+                    help="fixed error threshold for the learned criterion (skip tuning)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the engine holds one whole domain per GPU and this build has no "
@@ -284,22 +310,26 @@ def main():
         fdir = eng.direct_sample(tsp, tix)
 
     plain = None if a.no_plain else run_plain(qel, qion, rs, tsp, tix, fdir, a)
-    params, mlp_info = None, None
+    fixed = run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a)
+    learned, mlp_info = None, None
     if not a.no_learned:
+        th_lo, th_hi = 0.7 * fixed["theta"], min(1.0, 1.6 * fixed["theta"])
         log("collecting labels and training the opening-criterion MLP")
-        params, mlp_info = train_mlp(a)
-    fixed, learned = run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a, params)
+        params, mlp_info = train_mlp(a, th_lo, th_hi)
+        learned = run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi)
     head = learned if learned is not None else fixed
 
     flops = walk_flops32(head["st"])
-    achieved = flops / (head["ms_force"] * 1e-3) / 1e12
+    achieved = flops / (head["ms_walk"] * 1e-3) / 1e12
     peak = PEAK["fp32"]
 
     def summary(v):
         if v is None:
             return None
         out = {"steps_per_s": 1.0 / v["s"], "theta": v["theta"], "rms": v["rms"], "max": v["max"],
-               "ms_walk": v["ms_force"], "ms_build": v["ms_build"]}
+               "ms_force": v["ms_force"], "ms_build": v["ms_build"]}
+        if "ms_walk" in v:
+            out["ms_walk_kernel"] = v["ms_walk"]
         for k in ("drift", "stride", "err_target", "theta_lo"):
             if k in v:
                 out[k] = v[k]
@@ -330,7 +360,9 @@ def main():
         "mlp": mlp_info,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_walk32 (persistent, both target species)", "flops_per_step": flops},
+                     "frac_of_nonpacked_fp32": achieved / PEAK["fp32_nonpacked"],
+                     "kernel": "k_walk32 (one persistent launch, both target species)",
+                     "ms_per_launch": head["ms_walk"], "flops_per_launch": flops},
     }
     if not a.no_cpu:
         log("timing the reference CPU path on a bounded sample")

@@ -131,6 +131,8 @@ typedef struct tb_stats {
     int64_t m2p[4];          /* accepted cells                                  */
     int64_t p2p[4];          /* leaf interactions                               */
     double ms_build, ms_force, ms_vv;  /* last call, device time               */
+    double ms_walk;          /* last force: the walk kernel alone (mixed walk;  */
+                             /* the FP64 walk reports its force phase)          */
 } tb_stats;
 /* counting pass: re-runs the last force with interaction counters */
 int tb_count_interactions(tb_handle *h, tb_stats *s);
@@ -161,6 +163,8 @@ int tb_mlp_train(tb_handle *h, int32_t n, const float *X, const float *y, int32_
  * beyond s/theta_lo + dx, records each (cell, group) pair whose acceptance
  * the learned criterion would decide, with its 8 features X[k][8], and labels
  * it y[k] = log10 max over the group of |F_M2P - F_exact| / |F| (FP64).
+ * One fixed-theta force evaluation runs first: one feature is the group's
+ * smallest net field from the previous force.
  */
 int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n);
 

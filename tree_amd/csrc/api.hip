@@ -427,6 +427,13 @@ int regroup(tb_handle *h, int tpl)
     return TB_OK;
 }
 
+// walk-kernel time of the last force (after the stream has synchronised)
+void note_walk_ms(tb_handle *h)
+{
+    const float w = h->cfg.walk == TB_WALK_MIXED ? tb_mixed_walk_ms(h->mx) : -1.0f;
+    h->st.ms_walk = w >= 0.0f ? (double)w : h->st.ms_force;
+}
+
 // integrity flags raised during phase B (bad permutation / malformed table)
 int build_check(tb_handle *h)
 {
@@ -673,6 +680,9 @@ int tb_force(tb_handle *h, double *epot_e, double *epot_i)
     HIPCHK(hipMemcpyAsync(h->h_sums, h->d_sums, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
     if ((rc = check_walk_flags(h))) return rc;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, h->ev[2], h->ev[3]) == hipSuccess) h->st.ms_force = ms;
+    note_walk_ms(h);
     epot_from_sums(h->h_sums, h->epot);
     if (epot_e) *epot_e = h->epot[0];
     if (epot_i) *epot_i = h->epot[1];
@@ -745,6 +755,7 @@ int tb_step(tb_handle *h, double dt, int32_t nsteps, double *out)
     float ms = 0;
     if (hipEventElapsedTime(&ms, h->ev[0], h->ev[1]) == hipSuccess) h->st.ms_build = ms;
     if (hipEventElapsedTime(&ms, h->ev[1], h->ev[3]) == hipSuccess) h->st.ms_force = ms;
+    note_walk_ms(h);
     if (hipEventElapsedTime(&ms, h->ev[3], h->ev[5]) == hipSuccess) h->st.ms_vv = ms;
     return TB_OK;
 }
@@ -990,7 +1001,14 @@ int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n)
     if (!(h->cfg.theta_lo > 0.0) || !(h->cfg.theta_lo < (h->cfg.theta_mixed > 0 ? h->cfg.theta_mixed : h->cfg.theta)))
         return fail(TB_ERR_ARG, "need 0 < theta_lo < theta_mixed");
     HIPCHK(hipSetDevice(h->cfg.device));
-    int rc = tb_mixed_collect(h->s, h->mx, h->sp, &h->jt, h->cfg, h->d_sums, cap, X, y, n);
+    // a fixed-theta force first: its forces are the features' previous forces
+    tb_config c = h->cfg;
+    c.mac = TB_MAC_FIXED;
+    std::swap(c, h->cfg);
+    int rc = force_all(h, false);
+    std::swap(c, h->cfg);
+    if (rc) return rc;
+    rc = tb_mixed_collect(h->s, h->mx, h->sp, &h->jt, h->cfg, h->d_sums, cap, X, y, n);
     if (rc) return fail(rc, "%s", tb_mixed_error());
     h->have_force = true;
     return TB_OK;

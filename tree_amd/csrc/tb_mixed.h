@@ -43,6 +43,7 @@ struct MixedDev {
     MixedTree t[1];
     int32_t *ctr = nullptr;       // work-queue counter (+ error flag)
     int tpl = 0;                  // targets per lane used by the last launch
+    hipEvent_t ev[2] = {};        // around the last walk dispatch
     Domain *ddom = nullptr;       // device copy of the launch's domain descriptors
     int ddom_cap = 0;
     double *esum = nullptr;       // 4 energy sums (ee, ei, ie, ii) per domain
@@ -65,6 +66,8 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev 
                      double *d_sums, int cap,
                      float *X, float *y, int *nout);
 const char *tb_mixed_error();
+// duration of the last walk dispatch (after the stream has synchronised); -1 if none
+float tb_mixed_walk_ms(MixedDev &M);
 
 // one persistent launch over several domains (the batched ensemble): fills
 // esum[4 * d] per domain; rs[d] is domain d's Kelbg length; rec/collect are

@@ -135,9 +135,12 @@ struct WalkArgs {
 
 constexpr int MODE_FIXED = 0, MODE_LEARNED = 1, MODE_COLLECT = 2;
 
-// features of a (cell, target group) pair, log2 scaled by 1/8
+// features of a (cell, target group) pair, log2 scaled by 1/8.  emin is the
+// smallest net field |E| = |F|/eps over the group's targets from the previous
+// force evaluation: the label is an error relative to the target's net force,
+// which in a quasi-neutral plasma is far below the cell's gross field cnt/d^2.
 __device__ inline void band_features(float s, float cnt, float d, float4 q0, float4 q1, float4 q2, float hx,
-                                     float hy, float hz, float rs, bool like_net, int nact, float *f)
+                                     float hy, float hz, float emin, bool like_net, int nact, float *f)
 {
     const float dn = sqrtf(q0.y * q0.y + q0.z * q0.z + q0.w * q0.w);
     const float qzz = -q1.x - q1.y;
@@ -150,7 +153,7 @@ __device__ inline void band_features(float s, float cnt, float d, float4 q0, flo
     f[3] = k * __log2f(qn / (cnt * s * s) + 9.5367431640625e-07f);
     f[4] = k * __log2f(cnt);
     f[5] = k * __log2f(h / d + 9.5367431640625e-07f);
-    f[6] = k * __log2f(rs / d);
+    f[6] = k * __log2f(emin * d * d / cnt + 9.5367431640625e-07f);
     f[7] = (like_net ? 1.0f : -1.0f) * k * __log2f((float)nact);
 }
 
@@ -304,6 +307,19 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
         }
         mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
         mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
+        // learned / collect: the group's weakest net field from the previous force
+        float emin = 0.0f;
+        if (MODE != MODE_FIXED) {
+            double e2 = 1e300;
+#pragma unroll
+            for (int t = 0; t < TPL; t++) {
+                if (!act[t]) continue;
+                const int i = base + lane + 64 * t;
+                const double fx = D.fx[tsp][i], fy = D.fy[tsp][i], fz = D.fz[tsp][i];
+                e2 = fmin(e2, fx * fx + fy * fy + fz * fz);
+            }
+            emin = (float)(sqrt(wave_min(e2)) / TB_EPS);
+        }
         const double gx = 0.5 * (mnx + mxx), gy = 0.5 * (mny + mxy), gz = 0.5 * (mnz + mxz);
         // half extents, rounded outwards
         const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f;
@@ -380,9 +396,10 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             if (MODE != MODE_FIXED && band) {
                 const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
                 float f[TB_MLP_NIN];
-                band_features(q2.y, q2.z, sqrtf(d2), q0, q1, q2, hx, hy, hz, D.rs, q0.x * qt >= 0.0f, nact, f);
+                band_features(q2.y, q2.z, sqrtf(d2), q0, q1, q2, hx, hy, hz, emin, q0.x * qt >= 0.0f, nact, f);
                 if (MODE == MODE_LEARNED) {
-                    acc = tb_mlp_eval(wl, f) < W.log_err;
+                    // without a previous force (emin = 0) band cells are opened
+                    acc = emin > 0.0f && tb_mlp_eval(wl, f) < W.log_err;
                     open = !acc;
                 } else {
                     open = true;  // the conservative walk opens it; the pair is recorded
@@ -787,7 +804,12 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     const int waves = std::min(ngt, resident_waves());
     const int blocks = nblk(waves, WPB);
     M.tpl = tpl;
+    if (!M.ev[0] && (hipEventCreate(&M.ev[0]) != hipSuccess || hipEventCreate(&M.ev[1]) != hipSuccess)) {
+        g_err = "mixed walk: event creation failed";
+        return TB_ERR_HIP;
+    }
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
+    hipEventRecord(M.ev[0], s);
     if (mode == MODE_COLLECT) {
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
         dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
@@ -798,6 +820,7 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         if (d_cnt) dispatch<MODE_FIXED, true>(s, tpl, blocks, W);
         else dispatch<MODE_FIXED, false>(s, tpl, blocks, W);
     }
+    hipEventRecord(M.ev[1], s);
     hipLaunchKernelGGL(k_domain_energy, dim3(nd), dim3(256), 0, s, M.ddom, W.gen, M.esum);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: launch failed";
@@ -977,6 +1000,9 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
     if (M.rec) hipFree(M.rec);
     if (M.lab) hipFree(M.lab);
     tb_mlp_free(M.mlp);
+    for (int k = 0; k < 2; k++)
+        if (M.ev[k]) hipEventDestroy(M.ev[k]);
+    M.ev[0] = M.ev[1] = nullptr;
     M.ctr = nullptr;
     M.ddom = nullptr;
     M.esum = nullptr;
@@ -988,6 +1014,13 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
 }
 
 const char *tb_mixed_error() { return g_err; }
+
+float tb_mixed_walk_ms(MixedDev &M)
+{
+    float ms = -1.0f;
+    if (!M.ev[0] || hipEventElapsedTime(&ms, M.ev[0], M.ev[1]) != hipSuccess) return -1.0f;
+    return ms;
+}
 
 // autotuner table: log2 bucket of the total particle count -> targets per lane
 static int g_tpl_table[64] = {0};

@@ -70,6 +70,7 @@ class Stats(ctypes.Structure):
         ("ms_build", ctypes.c_double),
         ("ms_force", ctypes.c_double),
         ("ms_vv", ctypes.c_double),
+        ("ms_walk", ctypes.c_double),
     ]
 
 
@@ -400,7 +401,7 @@ def _stats_dict(s):
     return {
         "nodes": list(s.nodes), "max_depth": list(s.max_depth),
         "visits": list(s.visits), "m2p": list(s.m2p), "p2p": list(s.p2p),
-        "ms_build": s.ms_build, "ms_force": s.ms_force, "ms_vv": s.ms_vv,
+        "ms_build": s.ms_build, "ms_force": s.ms_force, "ms_vv": s.ms_vv, "ms_walk": s.ms_walk,
     }
 
 
