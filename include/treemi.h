@@ -162,7 +162,7 @@ int tb_mlp_train(tb_handle *h, int32_t n, const float *X, const float *y, int32_
  * Training data from the method itself: runs the mixed walk accepting cells
  * beyond s/theta_lo + dx, records each (cell, group) pair whose acceptance
  * the learned criterion would decide, with its 8 features X[k][8], and labels
- * it y[k] = log10 max over the group of |F_M2P - F_exact| / |F| (FP64).
+ * it y[k] = log10 rms over the group of |F_M2P - F_exact| / |F| (FP64).
  * One fixed-theta force evaluation runs first: one feature is the group's
  * smallest net field from the previous force.
  */

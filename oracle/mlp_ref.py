@@ -61,6 +61,22 @@ def loss_grad(p, X, t):
     return loss, g
 
 
+def bf16(a):
+    """Round to the nearest bfloat16, ties to even, returned as float64 (finite inputs)."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, X):
+    """The walk's inference arithmetic (matrix cores): bf16 parameters, inputs
+    and first hidden layer; exact products; sums in higher precision."""
+    W1, b1, W2, b2, w3, b3 = unpack(bf16(p))
+    h1 = bf16(np.maximum(bf16(X) @ W1 + b1, 0.0))
+    h2 = np.maximum(h1 @ W2 + b2, 0.0)
+    return h2 @ w3 + b3
+
+
 def init_params(seed=0):
     """He-uniform weights, zero biases (the GPU trainer starts from the same vector)."""
     rng = np.random.default_rng(seed)

@@ -19,12 +19,16 @@ def _eng(**kw):
 
 
 def test_forward_and_gradient_vs_oracle():
-    """fp32 forward within 1e-5, bf16 forward within 3e-2, gradient within 1e-4 relative (vs FP64)."""
+    """fp32 forward within 1e-5 of FP64; the bf16 forward (the walk's matrix-core
+    path) within 2e-3 of the oracle's bf16 arithmetic (bf16 rounding of the first
+    hidden layer may flip on ties that FP32 and FP64 sums round differently) and
+    within 3e-2 of FP64; gradient within 1e-4 relative.  5003 samples: the last
+    wavefront is partial."""
     announce("mlp_grad")
     rng = np.random.default_rng(1)
     p = mlp_ref.init_params(5).astype(np.float32)
-    X = rng.normal(size=(5000, 8)).astype(np.float32)
-    t = rng.normal(size=5000).astype(np.float32)
+    X = rng.normal(size=(5003, 8)).astype(np.float32)
+    t = rng.normal(size=5003).astype(np.float32)
     y_ref, _ = mlp_ref.forward(p.astype(np.float64), X.astype(np.float64))
     loss_ref, g_ref = mlp_ref.loss_grad(p.astype(np.float64), X.astype(np.float64), t.astype(np.float64))
     with _eng() as eng:
@@ -34,7 +38,11 @@ def test_forward_and_gradient_vs_oracle():
         y16 = eng.mlp_forward(X, bf16=True)
         loss, g = eng.mlp_loss_grad(X, t)
     assert np.abs(y32 - y_ref).max() <= 1e-5 * max(1.0, np.abs(y_ref).max())
-    assert np.abs(y16 - y_ref).max() <= 3e-2 * max(1.0, np.abs(y_ref).max())
+    y_bf = mlp_ref.forward_bf16(p.astype(np.float64), X.astype(np.float64))
+    scale = max(1.0, np.abs(y_ref).max())
+    assert np.median(np.abs(y16 - y_bf)) <= 1e-5 * scale
+    assert np.abs(y16 - y_bf).max() <= 2e-3 * scale
+    assert np.abs(y16 - y_ref).max() <= 3e-2 * scale
     assert abs(loss - loss_ref) <= 1e-5 * loss_ref
     assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
 

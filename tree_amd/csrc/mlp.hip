@@ -135,6 +135,22 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(int B, const float *__restrict_
     for (int j = threadIdx.x; j < NPAR; j += 256) sP[j] = bf16 ? tb_bf16_round(P[j]) : P[j];
     __syncthreads();
     const int s = blockIdx.x * 256 + threadIdx.x;
+    if (bf16) {
+        // the walk's inference path: 64 samples per wavefront on the matrix cores
+        const int l = threadIdx.x & 63, h = l >> 5;
+        const int w0 = s - l;
+        float pr[2];
+#pragma unroll
+        for (int tile = 0; tile < 2; tile++) {
+            const int si = w0 + 32 * tile + (l & 31);
+            float xq[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) xq[q] = si < B ? X[(size_t)si * NIN + 4 * h + q] : 0.0f;
+            pr[tile] = tb_mlp_eval32_mfma(sP, xq);
+        }
+        if (s < B) out[s] = l < 32 ? pr[0] : pr[1];
+        return;
+    }
     if (s >= B) return;
     float x[NIN];
     for (int k = 0; k < NIN; k++) x[k] = X[(size_t)s * NIN + k];

@@ -54,6 +54,63 @@ __device__ inline float tb_mlp_eval(const float *P, const float *x)
     return y;
 }
 
+// ---- batched inference on the matrix cores (the walk's band decisions)
+//
+// 32 samples per call, one wavefront, BF16 operands (weights already
+// bf16-rounded; features and the first hidden layer rounded to bf16) with FP32
+// accumulation, as transposed products so that no data moves between layers:
+//   H1^T (32 hidden x 32 samples) = W1^T (32 x 8) . X^T (8 x 32)      1 MFMA
+//   H2^T (32 x 32)                = W2^T (32 x 32) . H1^T (32 x 32)   4 MFMAs
+// with v_mfma_f32_32x32x8_bf16 layouts: A[i][k] in lane i + 32 (k / 4),
+// element k % 4; B[k][j] in lane j + 32 (k / 4), element k % 4; D[i][j] in
+// lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8).  Layer 1's
+// registers 4c .. 4c+3 are then exactly layer 2's B operand for K-chunk c.
+typedef short tb_v4s __attribute__((ext_vector_type(4)));
+typedef float tb_v16f __attribute__((ext_vector_type(16)));
+
+__device__ inline short tb_bf16_bits(float f) { return (short)(__float_as_uint(tb_bf16_round(f)) >> 16); }
+
+// lane l supplies features 4 (l / 32) .. 4 (l / 32) + 3 of sample l % 32 in
+// xq and receives the output for sample l % 32 (lanes l and l ^ 32 both).
+// P: the bf16-rounded parameter vector.  All 64 lanes must call it together.
+__device__ inline float tb_mlp_eval32_mfma(const float *P, const float (&xq)[4])
+{
+    constexpr int NH = TB_MLP_NH;
+    constexpr int O_B1 = TB_MLP_NIN * NH, O_W2 = O_B1 + NH, O_B2 = O_W2 + NH * NH, O_W3 = O_B2 + NH,
+                  O_B3 = O_W3 + NH;
+    const int l = threadIdx.x & 63, i = l & 31, h = l >> 5;
+    tb_v4s a, b;
+    tb_v16f c, d;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        a[q] = tb_bf16_bits(P[(4 * h + q) * NH + i]);  // W1[k][hidden i]
+        b[q] = tb_bf16_bits(xq[q]);                     // X[sample i][k]
+    }
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int r = 8 * (v >> 2) + 4 * h + (v & 3);
+        c[v] = P[O_B1 + r];
+        d[v] = P[O_B2 + r];
+    }
+    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 16; v++) c[v] = fmaxf(c[v], 0.0f);
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            a[q] = tb_bf16_bits(P[O_W2 + (8 * cc + 4 * h + q) * NH + i]);  // W2[in][out i]
+            b[q] = tb_bf16_bits(c[4 * cc + q]);                             // H1^T[in][sample i]
+        }
+        d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, d, 0, 0, 0);
+    }
+    float y = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 16; v++) y += fmaxf(d[v], 0.0f) * P[O_W3 + 8 * (v >> 2) + 4 * h + (v & 3)];
+    y += __shfl_xor(y, 32, 64);
+    return y + P[O_B3];
+}
+
 int tb_mlp_alloc(MlpDev &M);
 void tb_mlp_free(MlpDev &M);
 int tb_mlp_upload(hipStream_t s, MlpDev &M, const float *params);

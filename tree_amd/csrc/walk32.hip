@@ -46,6 +46,7 @@ const char *g_err = "";
 struct WaveLDS {
     uint32_t stack[STACK];
     uint32_t batch[64];
+    uint32_t bandq[LCAP];                 // learned: band cells waiting for a decision
     float4 m0[LCAP], m1[LCAP], m2[LCAP];  // x y z q | Qxx Qyy Qzz Qxy | Qyz Qzx - -
     float4 p[LCAP];                       // x y z -
 };
@@ -105,10 +106,14 @@ __device__ inline void nadd(float &s, float &c, float v)
 }
 
 // 1 - e^{-u}(1+u) and 1 - e^{-u}, with series below u = 1/8 where the
-// subtraction would cancel
+// subtraction would cancel; both round to 1 in FP32 from u = 22 on
+// (e^{-22} (1 + 22) < 2^-25, half an ulp below 1), which skips the
+// exponential for all but close unlike pairs
 __device__ inline void kelbg_gh(float u, float &g, float &h)
 {
-    if (u < 0.125f) {
+    if (u >= 22.0f) {
+        g = h = 1.0f;
+    } else if (u < 0.125f) {
         g = u * u * (0.5f + u * (-1.0f / 3.0f + u * (0.125f + u * (-1.0f / 30.0f + u * (1.0f / 144.0f +
                                                                                     u * (-1.0f / 840.0f))))));
         h = u * (1.0f + u * (-0.5f + u * (1.0f / 6.0f + u * (-1.0f / 24.0f + u * (1.0f / 120.0f +
@@ -178,18 +183,23 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - a.x, dy = ry[t] - a.y, dz = rz[t] - a.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const float ri = rsqrtf(r2);
-            const float ri2 = ri * ri, ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+            // raw v_rsq_f32: an accepted cell is at least s/theta away, so r2
+            // is a normal number and the denormal-safe rsqrtf is not needed
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const float u = ri * ri, ri3 = ri * u, ri5 = ri3 * u;
             const float dp = b.x * dx + b.y * dy + b.z * dz;
             const float qx = qxx * dx + qxy * dy + qzx * dz;
             const float qy = qxy * dx + qyy * dy + qyz * dz;
             const float qz = qzx * dx + qyz * dy + qzz * dz;
             const float xqx = dx * qx + dy * qy + dz * qz;
-            const float k = a.w * ri3 + 3.0f * dp * ri5 + 2.5f * xqx * ri7;
-            ax[t] += dx * k - b.x * ri3 - qx * ri5;
-            ay[t] += dy * k - b.y * ri3 - qy * ri5;
-            az[t] += dz * k - b.z * ri3 - qz * ri5;
-            ap[t] += a.w * ri + dp * ri3 + 0.5f * xqx * ri5;
+            // E = dx ri3 (q + u (3 dp + 2.5 xqx u)) - D ri3 - Q.dx ri5
+            // phi = ri (q + u (dp + 0.5 xqx u))
+            const float a5 = 2.5f * u * xqx;
+            const float w = ri3 * fmaf(u, fmaf(3.0f, dp, a5), a.w);
+            ax[t] = fmaf(-qx, ri5, fmaf(-b.x, ri3, fmaf(dx, w, ax[t])));
+            ay[t] = fmaf(-qy, ri5, fmaf(-b.y, ri3, fmaf(dy, w, ay[t])));
+            az[t] = fmaf(-qz, ri5, fmaf(-b.z, ri3, fmaf(dz, w, az[t])));
+            ap[t] = fmaf(ri, fmaf(u, fmaf(0.2f, a5, dp), a.w), ap[t]);
         }
     }
 #pragma unroll
@@ -217,7 +227,8 @@ __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TP
             for (int t = 0; t < TPL; t++) {
                 const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                const float ri = r2 > 0.01f ? rsqrtf(r2) : 0.0f;  // also drops the target itself
+                // also drops the target itself; beyond the cut r2 is normal (raw v_rsq_f32)
+                const float ri = r2 > 0.01f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
                 const float w = q.w * ri * ri * ri;
                 ax[t] += dx * w;
                 ay[t] += dy * w;
@@ -250,15 +261,12 @@ __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     }
 }
 
+// At most 3 blocks of 4 waves fit a CU's LDS, i.e. 3 waves per SIMD: the
+// register budget is capped to match (512 / 3 = 170 VGPRs).
 template <int TPL, int MODE, bool COUNT>
-__global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
+__global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
 {
     __shared__ WaveLDS lds[WPB];
-    __shared__ float wl[MODE == MODE_LEARNED ? TB_MLP_NPAR : 1];
-    if (MODE == MODE_LEARNED) {
-        for (int j = threadIdx.x; j < TB_MLP_NPAR; j += BT) wl[j] = W.mlpw[j];
-        __syncthreads();
-    }
     const int lane = lane_id();
     WaveLDS &L = lds[threadIdx.x >> 6];
     const int G = 64 * TPL;
@@ -339,10 +347,10 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
             A.ps[t] = A.pc[t] = 0.0f;
         }
 
-        int sp = 1, nm = 0, np = 0;
+        int sp = 1, nm = 0, np = 0, nb = 0;  // stack, M2P, P2P, band-queue lengths
         L.stack[0] = 0u;  // root: range of one cell at 0
         wave_sync();
-        while (sp > 0) {
+        while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
             // a batch of <= 64 cells from the top ranges of the stack
             uint32_t R = 0;
             int len = 0;
@@ -393,15 +401,12 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                     }
                 }
             }
-            if (MODE != MODE_FIXED && band) {
+            // learned: band cells are queued and decided 64 at a time below
+            if (MODE == MODE_COLLECT && band) {
                 const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
                 float f[TB_MLP_NIN];
                 band_features(q2.y, q2.z, sqrtf(d2), q0, q1, q2, hx, hy, hz, emin, q0.x * qt >= 0.0f, nact, f);
-                if (MODE == MODE_LEARNED) {
-                    // without a previous force (emin = 0) band cells are opened
-                    acc = emin > 0.0f && tb_mlp_eval(wl, f) < W.log_err;
-                    open = !acc;
-                } else {
+                {
                     open = true;  // the conservative walk opens it; the pair is recorded
                     if (q2.z <= 4096.0f) {
                         const int k = atomicAdd(&W.ctr[3], 1);
@@ -470,6 +475,86 @@ __global__ __launch_bounds__(BT) void k_walk32(WalkArgs W)
                 flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
                 np = 0;
                 wave_sync();
+            }
+            if (MODE == MODE_LEARNED) {
+                const uint64_t bm = __ballot(band);
+                if (band) L.bandq[nb + __popcll(bm & lanemask_lt())] = node;
+                nb += __popcll(bm);
+                wave_sync();
+                if (nb >= 64 || (sp == 0 && nb > 0)) {
+                    // Decide up to 64 queued cells with one batched MLP pass on
+                    // the matrix cores (tb_mlp_eval32_mfma, two tiles of 32).
+                    // Entry e's features are recomputed from its records.
+                    const int ne = min(nb, 64);
+                    float pr[2];
+#pragma unroll 1
+                    for (int tile = 0; tile < 2; tile++) {
+                        const int e = 32 * tile + (lane & 31);
+                        float f[TB_MLP_NIN];
+#pragma unroll
+                        for (int j = 0; j < TB_MLP_NIN; j++) f[j] = 0.0f;
+                        if (e < ne) {
+                            const uint32_t nd = L.bandq[e];
+                            const double4 m = D.bmac[nd];
+                            const float ex = fmaxf(fabsf((float)(m.x - gx)) - hx, 0.0f);
+                            const float ey = fmaxf(fabsf((float)(m.y - gy)) - hy, 0.0f);
+                            const float ez = fmaxf(fabsf((float)(m.z - gz)) - hz, 0.0f);
+                            const float4 q0 = D.bq0[nd], q1 = D.bq1[nd], q2 = D.bq2[nd];
+                            band_features(q2.y, q2.z, sqrtf(ex * ex + ey * ey + ez * ez), q0, q1, q2, hx, hy, hz,
+                                          emin, q0.x * qt >= 0.0f, nact, f);
+                        }
+                        float xq[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) xq[q] = (lane >> 5) ? f[4 + q] : f[q];
+                        pr[tile] = tb_mlp_eval32_mfma(W.mlpw, xq);
+                    }
+                    // lane l decides entry l (tile l / 32 holds it); without a
+                    // previous force (emin = 0) band cells are opened
+                    const bool mine = lane < ne;
+                    const uint32_t nd = mine ? L.bandq[lane] : 0u;
+                    const bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < W.log_err;
+                    bool bopen = mine && !bacc;
+                    uint64_t bk = __ballot(bacc);
+                    if (bacc) {
+                        const int k = nm + __popcll(bk & lanemask_lt());
+                        const double4 m = D.bmac[nd];
+                        const float4 q0 = D.bq0[nd], q1 = D.bq1[nd], q2 = D.bq2[nd];
+                        L.m0[k] = make_float4((float)(m.x - gx), (float)(m.y - gy), (float)(m.z - gz), q0.x);
+                        L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
+                        L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
+                    }
+                    nm += __popcll(bk);
+                    const uint32_t binf = bopen ? D.binfo[nd] : 0u;
+                    if (bopen) {
+                        const uint32_t st = binf >> 3, len = (binf & 7u) + 1u;
+                        if (st <= nd || st + len > (uint32_t)D.nn) {
+                            atomicOr(&W.ctr[1], 2);
+                            bopen = false;
+                        }
+                    }
+                    bk = __ballot(bopen);
+                    const int bpush = __popcll(bk);
+                    if (sp + bpush > STACK) {
+                        atomicOr(&W.ctr[1], 1);
+                        sp = 0;
+                        nm = np = nb = 0;
+                        break;
+                    }
+                    if (bopen) L.stack[sp + __popcll(bk & lanemask_lt())] = binf;
+                    sp += bpush;
+                    // the rest of the queue (at most 63 entries) moves down
+                    const uint32_t rest = lane + 64 < nb ? L.bandq[lane + 64] : 0u;
+                    wave_sync();
+                    if (lane + 64 < nb) L.bandq[lane] = rest;
+                    nb -= ne;
+                    wave_sync();
+                    if (nm >= FLUSH) {
+                        if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
+                        flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+                        nm = 0;
+                        wave_sync();
+                    }
+                }
             }
         }
         if (nm > 0) {
@@ -631,10 +716,12 @@ struct LabelArgs {
     float *lab;
 };
 
-// label of a band pair: log10 of the largest, over the group's targets, of
+// label of a band pair: log10 of the rms, over the group's targets, of
 // |F_M2P - F_exact| / |F_target| with F_M2P the cell's monopole + dipole +
 // quadrupole field and F_exact the FP64 sum over the cell's particles with the
-// reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg)
+// reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg).  The
+// rms (not the largest) is the cell's share of the rms force error that the
+// walk is tuned to.
 __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
 {
     const int r = blockIdx.x;
@@ -651,7 +738,7 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const double Qxx = 3.0 * Q[4] - tr, Qyy = 3.0 * Q[5] - tr, Qzz = 3.0 * Q[6] - tr;
     const double Qxy = 3.0 * Q[7], Qyz = 3.0 * Q[8], Qzx = 3.0 * Q[9];
     const int s0 = A.nstart[p], cnt = A.ncount[p];
-    double worst = 0.0;
+    double sq = 0.0;  // sum over the group's targets of the squared relative error
     for (int t = threadIdx.x; t < gr.y; t += 64) {
         const int i = gr.x + t;
         const double x = A.px[tsp][i], y = A.py[tsp][i], z = A.pz[tsp][i];
@@ -684,10 +771,11 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
         ex -= gx; ey -= gy; ez -= gz;
         const double fx = A.fx[tsp][i], fy = A.fy[tsp][i], fz = A.fz[tsp][i];
         const double fn = sqrt(fx * fx + fy * fy + fz * fz);
-        worst = fmax(worst, TB_EPS * sqrt(ex * ex + ey * ey + ez * ez) / fmax(fn, 1e-300));
+        const double rel = TB_EPS * sqrt(ex * ex + ey * ey + ez * ez) / fmax(fn, 1e-300);
+        sq += rel * rel;
     }
-    worst = wave_max(worst);
-    if (threadIdx.x == 0) A.lab[r] = (float)log10(worst + 1e-12);
+    sq = wave_sum(sq);
+    if (threadIdx.x == 0) A.lab[r] = (float)log10(sqrt(sq / gr.y) + 1e-12);
 }
 
 inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
