@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--group", type=int, default=0, help="mixed-walk targets per group (0 = autotuned)")
     ap.add_argument("--err-target", type=float, default=0.0,
                     help="fixed error threshold for the learned criterion (skip tuning)")
+    ap.add_argument("--band", type=float, nargs=2, default=(0.7, 1.6), metavar=("LO", "HI"),
+                    help="learned band: theta_lo = LO * theta, theta_hi = HI * theta (theta: the tuned fixed walk's)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the engine holds one whole domain per GPU and this build has no "
@@ -313,7 +315,7 @@ def main():
     fixed = run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a)
     learned, mlp_info = None, None
     if not a.no_learned:
-        th_lo, th_hi = 0.7 * fixed["theta"], min(1.0, 1.6 * fixed["theta"])
+        th_lo, th_hi = a.band[0] * fixed["theta"], min(1.0, a.band[1] * fixed["theta"])
         log("collecting labels and training the opening-criterion MLP")
         params, mlp_info = train_mlp(a, th_lo, th_hi)
         learned = run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi)

@@ -78,13 +78,13 @@ def test_learned_mac_gain():
     """Trained on labels the method generates, the learned criterion reaches the
     fixed-theta walk's rms error (sampled against the FP64 direct sum) with fewer
     M2P+P2P interactions.  Fixed walk: theta 0.3.  Learned walk: cells beyond
-    s/0.2 always accepted, cells within s/0.45 always opened, the network
-    deciding in between; its err_target is raised from 1e-7 while its error
-    stays at or below the fixed walk's, and the work is compared there."""
+    s/0.15 always accepted, cells within s/0.5 always opened, the network
+    deciding in between; err_target is bisected (in log10) to the largest value
+    whose error is at or below the fixed walk's, and the work is compared there."""
     announce("learned_mac_gain")
     n = 30000
     qel, qion = ta.gen_plasma(31, n, n, 250.0)
-    with _eng(theta=0.3, theta_mixed=0.45, theta_lo=0.2, walk="mixed", err_target=1e-7, mac="learned") as eng:
+    with _eng(theta=0.3, theta_mixed=0.5, theta_lo=0.15, walk="mixed", err_target=1e-7, mac="learned") as eng:
         eng.set_particles(qel, qion)
         eng.build()
         X, y = eng.mlp_collect(cap=400000)
@@ -93,20 +93,21 @@ def test_learned_mac_gain():
         eng.mlp_train(X, y, steps=3000, batch=4096, lr=3e-3)
         eng.configure(mac="fixed", theta_mixed=0.3)
         err_fix = _direct_err(eng, qel, qion, 2000)
-        st_fix = eng.count_interactions()
-        work_fix = sum(st_fix["m2p"]) + sum(st_fix["p2p"])
-        eng.configure(mac="learned", theta_mixed=0.45)
-        best, rows = None, []
-        for et in (1e-7, 3e-7, 1e-6, 3e-6, 1e-5, 3e-5, 1e-4, 3e-4, 1e-3, 3e-3, 1e-2):
-            eng.configure(err_target=et)
+        work_fix = sum(eng.count_interactions()["m2p"]) + sum(eng.count_interactions()["p2p"])
+        eng.configure(mac="learned", theta_mixed=0.5)
+        lo, hi, best, rows = -10.0, -2.0, None, []
+        for _ in range(10):
+            mid = 0.5 * (lo + hi)
+            eng.configure(err_target=10.0 ** mid)
             err_l = _direct_err(eng, qel, qion, 2000)
-            st_l = eng.count_interactions()
-            work_l = sum(st_l["m2p"]) + sum(st_l["p2p"])
-            rows.append((et, err_l, work_l))
-            print("err_target %.0e: learned rms %.3e work %d | fixed rms %.3e work %d"
-                  % (et, err_l, work_l, err_fix, work_fix))
-            if err_l > err_fix:
-                break
-            best = (et, err_l, work_l)
+            st = eng.count_interactions()
+            work_l = sum(st["m2p"]) + sum(st["p2p"])
+            rows.append((mid, err_l, work_l))
+            print("err_target 1e%.2f: learned rms %.3e work %d | fixed rms %.3e work %d"
+                  % (mid, err_l, work_l, err_fix, work_fix))
+            if err_l <= err_fix:
+                best, lo = (mid, err_l, work_l), mid
+            else:
+                hi = mid
     assert best is not None, rows
     assert best[2] < work_fix, (best, work_fix, rows)

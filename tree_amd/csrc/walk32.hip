@@ -40,6 +40,7 @@ constexpr int WPB = BT / 64;
 constexpr int STACK = 1024;
 constexpr int LCAP = 128;   // list capacity; flushed once 64 entries are pending
 constexpr int FLUSH = 64;
+constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decided by the network
 
 const char *g_err = "";
 
@@ -394,10 +395,16 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         acc = far_hi;
                         open = !far_hi;
                     } else {
-                        const float rlo = D.bq2[node].y * W.inv_theta_lo;
+                        const float4 q2 = D.bq2[node];
+                        const float rlo = q2.y * W.inv_theta_lo;
                         if (d2 > rlo * rlo * 1.000004f) acc = true;
                         else if (!far_hi) open = true;
-                        else band = true;
+                        else if (MODE == MODE_COLLECT || q2.z <= (float)BAND_MAXCNT) band = true;
+                        // larger cells are never labelled (the label sums the
+                        // cell's particles): in the learned walk the band's
+                        // geometric-mean radius decides them
+                        else if (d2 > rlo * rc * 1.000004f) acc = true;
+                        else open = true;
                     }
                 }
             }
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 band_features(q2.y, q2.z, sqrtf(d2), q0, q1, q2, hx, hy, hz, emin, q0.x * qt >= 0.0f, nact, f);
                 {
                     open = true;  // the conservative walk opens it; the pair is recorded
-                    if (q2.z <= 4096.0f) {
+                    if (q2.z <= (float)BAND_MAXCNT) {
                         const int k = atomicAdd(&W.ctr[3], 1);
                         if (k < W.rec_cap) {
                             BandRec r;
