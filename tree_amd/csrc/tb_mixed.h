@@ -41,7 +41,7 @@ struct Domain {
 
 struct MixedDev {
     MixedTree t[1];
-    int32_t *ctr = nullptr;       // work-queue counter (+ error flag)
+    int32_t *ctr = nullptr;       // 16 ints: error flags, band pair count, work-queue chunk counters
     int tpl = 0;                  // targets per lane used by the last launch
     hipEvent_t ev[2] = {};        // around the last walk dispatch
     Domain *ddom = nullptr;       // device copy of the launch's domain descriptors

@@ -40,6 +40,8 @@ constexpr int WPB = BT / 64;
 constexpr int STACK = 1024;
 constexpr int LCAP = 128;   // list capacity; flushed once 64 entries are pending
 constexpr int FLUSH = 64;
+constexpr int NXCD = 8;       // XCDs of an MI355X (work chunks of the walk queue)
+constexpr int CTR_CHUNK = 8;  // first chunk counter in the counter block (16 ints)
 constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decided by the network
 
 const char *g_err = "";
@@ -134,7 +136,7 @@ struct WalkArgs {
     const float *mlpw;    // learned: bf16-rounded parameters
     BandRec *rec;         // collect: band pairs
     int rec_cap;
-    int32_t *ctr;         // [0] work counter, [1] error flag, [2] prep error, [3] band pair count
+    int32_t *ctr;         // [1] error flag, [2] prep error, [3] band pair count, [8..15] chunk counters
     double *gen;          // 2 per global group: energy of electron targets / of ion targets
     unsigned long long *cnt;
 };
@@ -274,15 +276,30 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
     // per target species: cells tested, M2P and like / unlike P2P interactions
     unsigned long long c_visit[2] = {0, 0}, c_m2p[2] = {0, 0}, c_like[2] = {0, 0}, c_unlike[2] = {0, 0};
 
+    // Work queue, XCD-aware: workgroups go to the 8 XCDs round-robin, so
+    // block b's waves first take groups from chunk b % 8 of the group range
+    // (contiguous in key order, hence in space: that XCD's L2 keeps the tree
+    // region its groups share), then from the other chunks in turn.
+    int chunk = blockIdx.x & (NXCD - 1), tried = 0;
     for (;;) {
         // No branch on the lane index inside this loop: the compiler may
         // thread such a (loop-invariant) branch across the back edge and split
         // the wave, leaving lanes 1..63 to walk without lane 0.  So every lane
         // issues the fetch (lane 0 adds 1, the rest 0) and the stores below
         // are made by every lane with the same value.
-        int g = atomicAdd(&W.ctr[0], lane == 0 ? 1 : 0);
-        g = __shfl(g, 0, 64);
-        if (g >= W.ngt) break;
+        int g = -1;
+        while (tried < NXCD) {
+            const int c0 = (int)((long)W.ngt * chunk / NXCD), c1 = (int)((long)W.ngt * (chunk + 1) / NXCD);
+            int v = atomicAdd(&W.ctr[CTR_CHUNK + chunk], lane == 0 ? 1 : 0);
+            v = __shfl(v, 0, 64);
+            if (c0 + v < c1) {
+                g = c0 + v;
+                break;
+            }
+            chunk = (chunk + 1) & (NXCD - 1);
+            tried++;
+        }
+        if (g < 0) break;
         // the domain holding global group g (descriptors ordered by gbase)
         int lo = 0, hi = W.ndom - 1;
         while (lo < hi) {
@@ -904,6 +921,7 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         return TB_ERR_HIP;
     }
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
+    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_CHUNK, CTR_CHUNK + NXCD);
     hipEventRecord(M.ev[0], s);
     if (mode == MODE_COLLECT) {
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
