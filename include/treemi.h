@@ -190,6 +190,20 @@ int tb_ensemble_force(tb_ensemble *e, double *epot);
 int tb_ensemble_step(tb_ensemble *e, double dt, int32_t nsteps, double *out);
 int tb_ensemble_autotune(tb_ensemble *e, int32_t *group_out, float *ms_out);
 
+/* ------------------------------------------------ radial distribution
+ * Leaf-pair RDF histograms, as the reference accumulates them in every
+ * LocalCoulombSum (tree.f90:916-919, 1021-1024, 1125-1129; NS%RDF_ei /
+ * NS%RDF_ii, data.f90:139) and RDF prints and resets them (sample.f90:5-50):
+ * every unlike leaf pair of the ie and ei walks adds 1 to rdf_ei[INT(r/dx_ei)],
+ * every like ion pair beyond the 0.1 um cut adds 1 to rdf_ii[INT(r/dx_ii)],
+ * for bins 0 .. nbin-1 (the reference's 1 .. Ndist+1, Ndist = 10000).  The
+ * FP64 walk accumulates them (TB_WALK_REF64 only); the counting pass adds
+ * nothing.  nbin = 0 disables and frees them.
+ */
+int tb_rdf_enable(tb_handle *h, double dx_ei, double dx_ii, int32_t nbin);
+/* copy out (and with reset != 0 clear) the histograms accumulated since enable / the last reset */
+int tb_rdf_get(tb_handle *h, int32_t nbin, int64_t *rdf_ei, int64_t *rdf_ii, int32_t reset);
+
 const char *tb_last_error(void);
 
 /* sizeof(tb_pt), sizeof(tb_config), sizeof(tb_stats) as compiled (binding checks) */

@@ -27,6 +27,7 @@ def lib():
         L = ctypes.CDLL(LIB)
         P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         L.orc_local_coulomb_sum.argtypes = [I, P, I, P, D, D, P, P, P, P, I]
+        L.orc_local_coulomb_sum_rdf.argtypes = [I, P, I, P, D, D, P, P, P, P, I, D, D, I, P, P]
         L.orc_export_tree.argtypes = [I, P, I, I] + [P] * 12
         L.orc_direct.argtypes = [I, P, I, P, D, I, P, P, P, P, I]
         L.orc_direct.restype = None
@@ -45,6 +46,9 @@ def _p(a):
 
 def _aos(x):
     return np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+
+
+NDIST = 10000  # data.f90:38: the histograms hold NDIST + 1 bins
 
 
 def local_coulomb_sum(xe, xi, theta, rs, nthreads=0):
@@ -146,11 +150,32 @@ def parse_ref_mode1(buf, ne, ni):
         off += 4
         trees.append(np.frombuffer(buf, dtype=_NODE, count=n, offset=off).copy())
         off += n * _NODE.itemsize
-    f = np.frombuffer(buf, dtype="<f8", offset=off)
+    f = np.frombuffer(buf, dtype="<f8", offset=off, count=3 * ne + 3 * ni + 2)
     fe = f[: 3 * ne].reshape(ne, 3).copy()
     fi = f[3 * ne: 3 * ne + 3 * ni].reshape(ni, 3).copy()
     ep = f[3 * ne + 3 * ni: 3 * ne + 3 * ni + 2].copy()
     return trees, fe, fi, ep
+
+
+def parse_ref_rdf(buf, ne, ni):
+    """The RDF histograms (RDF_ei, RDF_ii: NDIST + 1 int32 each) at the end of a mode-1 run."""
+    h = np.frombuffer(buf[len(buf) - 8 * (NDIST + 1):], dtype="<i4")
+    return h[: NDIST + 1].astype(np.int64), h[NDIST + 1:].astype(np.int64)
+
+
+def local_coulomb_sum_rdf(xe, xi, theta, rs, dx_ei, dx_ii, nbin=None, nthreads=0):
+    """local_coulomb_sum plus the leaf-pair RDF histograms (rdf_ei, rdf_ii)."""
+    nbin = NDIST + 1 if nbin is None else nbin
+    xe, xi = _aos(xe), _aos(xi)
+    fe, fi = np.zeros_like(xe), np.zeros_like(xi)
+    ep = np.zeros(2)
+    cnt = np.zeros(12, np.int64)
+    hei, hii = np.zeros(nbin, np.int64), np.zeros(nbin, np.int64)
+    rc = lib().orc_local_coulomb_sum_rdf(len(xe), _p(xe), len(xi), _p(xi), theta, rs, _p(fe), _p(fi), _p(ep),
+                                         _p(cnt), nthreads, dx_ei, dx_ii, nbin, _p(hei), _p(hii))
+    if rc:
+        raise RuntimeError("oracle: coincident particles")
+    return fe, fi, ep, cnt, hei, hii
 
 
 def parse_ref_mode2(buf, ne, ni, nsteps):

@@ -14,7 +14,10 @@
 !   mode 1: for each species (e then i): int32 nnodes, then per node in
 !           preorder (children 1..8): int32 depth, ileaf, pt, next;
 !           real64 xx(3), lx(3), qeff, dx, D(3), Q(3), QQ(3).
-!           Then real64 fe(3,ne), fi(3,ni), Epot_e, Epot_i.
+!           Then real64 fe(3,ne), fi(3,ni), Epot_e, Epot_i; then the
+!           reference's RDF histograms int32 RDF_ei(Ndist+1), RDF_ii(Ndist+1)
+!           (bin width 0.01 um; run single-threaded: Search_Tree_ie adds into
+!           RDF_ei outside the OpenMP reduction, tree.f90:918).
 !   mode 2: nsteps of VV1 / build+LocalCoulombSum / VV2 (sampling every step);
 !           per step real64 Epot_e, Epot_i, mv2el, mv2ion, Te_rad, Ti_rad;
 !           then final xe, ve, fe, xi, vi, fi.
@@ -86,6 +89,7 @@ PROGRAM REF_HARNESS
         WRITE(12) qion(i)%ff(:)
      END DO
      WRITE(12) sys%Epot_e, sys%Epot_i
+     WRITE(12) NS%RDF_ei(:), NS%RDF_ii(:)
   ELSE IF (mode == 2) THEN
      DO istep = 1, nsteps
         ttime%Nloop = istep

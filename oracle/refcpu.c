@@ -212,7 +212,21 @@ typedef struct wctx {
     double f[3];
     double epot;
     int64_t nvisit, nm2p, np2p;
+    int64_t *rdf;     /* leaf-pair histogram (NULL: none, as for ee) */
+    double rdx;       /* its bin width */
+    int nbin;         /* Ndist + 1 bins */
 } wctx;
+
+/* RDF bin of a leaf pair, tree.f90:916-919 (INT truncates; r >= 0) */
+static void rdf_count(wctx *W, double r)
+{
+    if (!W->rdf) return;
+    int id = (int)(r / W->rdx) + 1;
+    if (id <= W->nbin && id > 0) {
+#pragma omp atomic
+        W->rdf[id - 1]++;
+    }
+}
 
 /* Search_Tree_{ee,ie,ei,ii}, tree.f90:721-1133 */
 static void search(wctx *W, const onode *P)
@@ -281,6 +295,7 @@ static void search(wctx *W, const onode *P)
                 double ff = W->ff_p2p;
                 W->epot = W->epot + 0.5 * ff / r;
                 for (int k = 0; k < 3; k++) W->f[k] = W->f[k] + ff * xr[k] / r / r2;
+                rdf_count(W, r);  /* ii only: ee walks carry no histogram (tree.f90:1125-1129) */
             }
         } else {
             W->np2p++;
@@ -290,6 +305,7 @@ static void search(wctx *W, const onode *P)
             W->epot = W->epot + 0.5 * ff * (1.0 - e) / r;
             for (int k = 0; k < 3; k++)
                 W->f[k] = W->f[k] + ff * xr[k] * (1.0 / r - e * (1.0 / r + 1.0 / W->rs)) / r2;
+            rdf_count(W, r);  /* ie and ei, tree.f90:916-919, 1021-1024 */
         }
     }
 }
@@ -303,8 +319,26 @@ static void search(wctx *W, const onode *P)
  * counts (optional, 12 int64): {visits, m2p, p2p} for ee, ie, ei, ii.
  * Returns 0, or 1 if a tree could not be built (coincident particles).
  */
+int orc_local_coulomb_sum_rdf(int ne, const double *xe, int ni, const double *xi, double theta, double rs,
+                              double *fe, double *fi, double *epot, int64_t *counts, int nthreads,
+                              double dx_ei, double dx_ii, int nbin, int64_t *rdf_ei, int64_t *rdf_ii);
+
 int orc_local_coulomb_sum(int ne, const double *xe, int ni, const double *xi, double theta, double rs,
                           double *fe, double *fi, double *epot, int64_t *counts, int nthreads)
+{
+    return orc_local_coulomb_sum_rdf(ne, xe, ni, xi, theta, rs, fe, fi, epot, counts, nthreads, 0.0, 0.0, 0,
+                                     NULL, NULL);
+}
+
+/*
+ * The same, also adding every leaf pair into the RDF histograms as the
+ * reference does (tree.f90:916-919, 1021-1024, 1125-1129): unlike pairs of
+ * both walks (ie, ei) into rdf_ei with bin width dx_ei, like ion pairs beyond
+ * the cut into rdf_ii with dx_ii; nbin = Ndist + 1 bins each (NULL: skipped).
+ */
+int orc_local_coulomb_sum_rdf(int ne, const double *xe, int ni, const double *xi, double theta, double rs,
+                              double *fe, double *fi, double *epot, int64_t *counts, int nthreads,
+                              double dx_ei, double dx_ii, int nbin, int64_t *rdf_ei, int64_t *rdf_ii)
 {
     otree *Te = build(ne, xe, -1);
     otree *Ti = build(ni, xi, +1);
@@ -330,6 +364,7 @@ int orc_local_coulomb_sum(int ne, const double *xe, int ni, const double *xi, do
             E_ee = W.epot; c0 += W.nvisit; c1 += W.nm2p; c2 += W.np2p;
             W.epot = E_ei; W.nvisit = W.nm2p = W.np2p = 0;
             W.like = 0; W.ff_m2p = -ORC_EPS; W.ff_p2p = -ORC_EPS;
+            W.rdf = rdf_ei; W.rdx = dx_ei; W.nbin = nbin;
             search(&W, Ti->root);
             E_ei = W.epot; c6 += W.nvisit; c7 += W.nm2p; c8 += W.np2p;
             for (int k = 0; k < 3; k++) fe[3 * i + k] = W.f[k];
@@ -340,11 +375,13 @@ int orc_local_coulomb_sum(int ne, const double *xe, int ni, const double *xi, do
             wctx W = {0};
             W.t = xi + 3 * i; W.theta = theta; W.rs = rs;
             W.like = 0; W.ff_m2p = ORC_EPS; W.ff_p2p = -ORC_EPS;
+            W.rdf = rdf_ei; W.rdx = dx_ei; W.nbin = nbin;
             W.epot = E_ie;
             search(&W, Te->root);
             E_ie = W.epot; c3 += W.nvisit; c4 += W.nm2p; c5 += W.np2p;
             W.epot = E_ii; W.nvisit = W.nm2p = W.np2p = 0;
             W.like = 1; W.ff_m2p = ORC_EPS; W.ff_p2p = ORC_EPS;
+            W.rdf = rdf_ii; W.rdx = dx_ii;
             search(&W, Ti->root);
             E_ii = W.epot; c9 += W.nvisit; c10 += W.nm2p; c11 += W.np2p;
             for (int k = 0; k < 3; k++) fi[3 * i + k] = W.f[k];

@@ -21,6 +21,7 @@ from oracle import pyoracle  # noqa: E402
 
 RS10 = 6.6493e-3
 DT = 0.005 / ta.TPS
+RDF_DX = 0.01  # the harness's RDF bin width (dx_ei = dx_ii), um
 
 
 def tree_arrays(t):
@@ -37,9 +38,11 @@ def digest(a):
 
 def small_case(name, xe, xi, theta):
     z = np.zeros_like
-    buf = pyoracle.run_ref(1, xe, z(xe), xi, z(xi), theta, RS10, (ta.XM_E, ta.XM_I), DT)
+    buf = pyoracle.run_ref(1, xe, z(xe), xi, z(xi), theta, RS10, (ta.XM_E, ta.XM_I), DT, threads=1)
     trees, fe, fi, ep = pyoracle.parse_ref_mode1(buf, len(xe), len(xi))
-    out = dict(xe=xe, xi=xi, theta=theta, rs=RS10, fe=fe, fi=fi, epot=ep)
+    rdf_ei, rdf_ii = pyoracle.parse_ref_rdf(buf, len(xe), len(xi))
+    out = dict(xe=xe, xi=xi, theta=theta, rs=RS10, fe=fe, fi=fi, epot=ep, rdf_dx=RDF_DX, rdf_ei=rdf_ei,
+               rdf_ii=rdf_ii)
     for sp, t in enumerate(trees):
         out["tree%d_int" % sp], out["tree%d_dbl" % sp] = tree_arrays(t)
     np.savez_compressed(os.path.join(HERE, name), **out)
@@ -60,10 +63,11 @@ def main():
     qel, qion = ta.gen_plasma(1, 4096, 4096, 100.0)
     xe, xi = qel["xx"].copy(), qion["xx"].copy()
     z = np.zeros_like
-    buf = pyoracle.run_ref(1, xe, z(xe), xi, z(xi), 0.5, RS10, (ta.XM_E, ta.XM_I), DT)
+    buf = pyoracle.run_ref(1, xe, z(xe), xi, z(xi), 0.5, RS10, (ta.XM_E, ta.XM_I), DT, threads=1)
     trees, fe, fi, ep = pyoracle.parse_ref_mode1(buf, 4096, 4096)
+    rdf_ei, rdf_ii = pyoracle.parse_ref_rdf(buf, 4096, 4096)
     meta = dict(pos_digest=digest(np.r_[xe, xi]), fe=fe, fi=fi, epot=ep, theta=0.5, rs=RS10,
-                nnodes=np.array([len(trees[0]), len(trees[1])]))
+                nnodes=np.array([len(trees[0]), len(trees[1])]), rdf_dx=RDF_DX, rdf_ei=rdf_ei, rdf_ii=rdf_ii)
     for sp, t in enumerate(trees):
         ti, td = tree_arrays(t)
         meta["tree%d_int_digest" % sp] = digest(ti)

@@ -164,6 +164,41 @@ def test_step_fused_equals_separate():
                                                                   rtol=1e-14)
 
 
+def test_rdf_matches_reference_and_oracle():
+    """The FP64 walk's leaf-pair RDF histograms (tree.f90:916-919, 1021-1024,
+    1125-1129) equal the reference's own at C1 (4096 + 4096, theta 0.5, dx 0.01 um)
+    bin for bin; two forces add twice; the counting pass adds nothing; reset
+    clears.  With dx = 0.001 um most pairs land beyond the block-local bins
+    (global atomics) and the result still equals the oracle's."""
+    import os
+    announce("rdf C1")
+    gold = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_c1.npz"), allow_pickle=False))
+    qel, qion = ta.gen_plasma(1, 4096, 4096, 100.0)
+    dx = float(gold["rdf_dx"])
+    with _engine(theta=0.5) as eng:
+        eng.set_particles(qel, qion)
+        eng.rdf_enable(dx, dx)
+        eng.build()
+        eng.force()
+        ei, ii = eng.rdf_get()
+        assert np.array_equal(ei, gold["rdf_ei"]) and np.array_equal(ii, gold["rdf_ii"])
+        eng.force()
+        eng.count_interactions()
+        ei2, ii2 = eng.rdf_get(reset=True)
+        assert np.array_equal(ei2, 2 * gold["rdf_ei"]) and np.array_equal(ii2, 2 * gold["rdf_ii"])
+        ei3, ii3 = eng.rdf_get()
+        assert not ei3.any() and not ii3.any()
+        eng.rdf_enable(0.001, 0.0015)
+        eng.force()
+        ei, ii = eng.rdf_get()
+    *_, oei, oii = pyoracle.local_coulomb_sum_rdf(qel["xx"], qion["xx"], 0.5, RS10, 0.001, 0.0015, nthreads=4)
+    assert np.array_equal(ei, oei) and np.array_equal(ii, oii)
+    assert ei[2048:].sum() > ei[:2048].sum() > 0
+    with _engine(theta=0.5, walk="mixed", theta_mixed=0.5) as eng:
+        with pytest.raises(ta.TreeError):
+            eng.rdf_enable(dx, dx)
+
+
 def test_errors():
     announce("errors")
     qel, qion = _plasma(16, 100, 100, 100.0)

@@ -97,6 +97,58 @@ def test_trajectory_matches_reference():
     assert np.array_equal(np.c_[xe, ve, fe], g["e"]) and np.array_equal(np.c_[xi, vi, fi], g["i"])
 
 
+@pytest.mark.parametrize("name", ["ref_n96.npz", "ref_lattice.npz"])
+def test_rdf_matches_reference(name):
+    """Leaf-pair RDF histograms (tree.f90:916-919, 1021-1024, 1125-1129) equal the reference's, bin for bin."""
+    g = _load(name)
+    dx = float(g["rdf_dx"])
+    *_, hei, hii = pyoracle.local_coulomb_sum_rdf(g["xe"], g["xi"], float(g["theta"]), float(g["rs"]), dx, dx,
+                                                  nthreads=1)
+    assert np.array_equal(hei, g["rdf_ei"]) and np.array_equal(hii, g["rdf_ii"])
+
+
+def test_rdf_c1_matches_reference():
+    import tree_amd as ta
+    g = _load("ref_c1.npz")
+    qel, qion = ta.gen_plasma(1, 4096, 4096, 100.0)
+    dx = float(g["rdf_dx"])
+    *_, hei, hii = pyoracle.local_coulomb_sum_rdf(qel["xx"], qion["xx"], 0.5, RS10, dx, dx, nthreads=4)
+    assert np.array_equal(hei, g["rdf_ei"]) and np.array_equal(hii, g["rdf_ii"])
+
+
+def test_rdf_totals_are_the_leaf_counts():
+    """With bins covering every distance the histograms hold exactly the walk's
+    leaf interactions: ie + ei for RDF_ei, ii (beyond the cut) for RDF_ii."""
+    rng = np.random.default_rng(8)
+    xe, xi = rng.normal(size=(600, 3)) * 50.0, rng.normal(size=(550, 3)) * 50.0
+    *_, c, hei, hii = pyoracle.local_coulomb_sum_rdf(xe, xi, 0.5, RS10, 0.1, 0.1, nthreads=2)
+    assert hei.sum() == c[5] + c[8] and hii.sum() == c[11]
+    assert hei.sum() > 0 and hii.sum() > 0
+
+
+def test_rdf_theta_zero_is_the_pair_histogram():
+    """theta -> 0 makes every pair a leaf pair: the histograms become the histograms
+    of all ordered e-i pairs (seen from both sides) and i-i pairs beyond 0.1 um,
+    binned as INT(r / dx) + 1 up to Ndist + 1."""
+    rng = np.random.default_rng(9)
+    xe, xi = rng.normal(size=(70, 3)) * 3.0, rng.normal(size=(60, 3)) * 3.0
+    dx = 0.0013
+    *_, hei, hii = pyoracle.local_coulomb_sum_rdf(xe, xi, 1e-12, RS10, dx, dx, nthreads=1)
+    nb = pyoracle.NDIST + 1
+
+    def hist(r):
+        idx = np.floor(r / dx).astype(np.int64) + 1
+        idx = idx[(idx >= 1) & (idx <= nb)]
+        return np.bincount(idx - 1, minlength=nb)
+
+    rei = np.sqrt(((xe[:, None, :] - xi[None, :, :]) ** 2).sum(-1)).ravel()
+    dii = xi[:, None, :] - xi[None, :, :]
+    r2 = (dii ** 2).sum(-1)
+    rii = np.sqrt(r2[r2 > 0.01])
+    assert np.array_equal(hei, 2 * hist(rei)) and np.array_equal(hii, hist(rii))
+    assert (rei / dx < nb).sum() > 100 and (rei / dx >= nb).sum() > 100  # both sides of the last bin
+
+
 @pytest.mark.skipif(not pyoracle.ref_available(), reason="reference harness not built here")
 def test_live_reference_random():
     rng = np.random.default_rng(77)

@@ -63,6 +63,8 @@ struct tb_handle {
     double epot[2] = {0, 0};
     tb_stats st{};
     hipEvent_t ev[6] = {};
+    TbRdf rdf;                     // leaf-pair RDF histograms (FP64 walk), when enabled
+    bool rdf_on = false;
     std::vector<void *> allocs;
 };
 
@@ -467,8 +469,9 @@ int force_all(tb_handle *h, bool counting)
         const int nb0 = tb_walk64_blocks(h->sp[0].n);
         for (int tsp = 0; tsp < 2; tsp++) {
             double *part = h->epart + (tsp == 0 ? 0 : 2 * nb0);
+            // the counting pass re-runs a force already counted in the RDF: it adds nothing
             tb_walk64(s, tsp, h->sp[tsp], h->sp[0], h->sp[1], h->cfg.rs, part,
-                      counting ? h->d_cnt + 6 * tsp : nullptr);
+                      counting ? h->d_cnt + 6 * tsp : nullptr, h->rdf_on && !counting ? &h->rdf : nullptr);
             tb_reduce_partials(s, tb_walk64_blocks(h->sp[tsp].n), 2, part, h->d_sums + 2 * tsp);
         }
     }
@@ -570,6 +573,7 @@ int tb_destroy(tb_handle *h)
     dfree(h, h->epart, 0);
     dfree(h, h->d_sums, 0);
     dfree(h, h->d_cnt, 0);
+    dfree(h, h->rdf.hist, 0);
     if (h->h_misc) hipHostFree(h->h_misc);
     if (h->h_sums) hipHostFree(h->h_sums);
     for (int k = 0; k < 6; k++)
@@ -585,6 +589,8 @@ int tb_set_config(tb_handle *h, const tb_config *cfg)
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (cfg->device != h->cfg.device) return fail(TB_ERR_ARG, "device cannot change on a live handle");
+    if (h->rdf_on && cfg->walk != TB_WALK_REF64)
+        return fail(TB_ERR_STATE, "the RDF is accumulated by the FP64 walk: disable it before switching walks");
     h->cfg = *cfg;
     h->built = false;  // rcrit and walk records depend on theta / walk
     return TB_OK;
@@ -1309,6 +1315,51 @@ int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
     tb_autotune_record(ntot / (int)E->h.size(), best_tpl);
     if ((rc = ens_regroup(E, ens_tpl(E)))) return rc;
     if (group_out) *group_out = 64 * best_tpl;
+    return TB_OK;
+}
+
+// ------------------------------------------------------------------ RDF
+
+int tb_rdf_enable(tb_handle *h, double dx_ei, double dx_ii, int32_t nbin)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    if (nbin == 0) {
+        HIPCHK(hipSetDevice(h->cfg.device));
+        HIPCHK(hipStreamSynchronize(h->s));
+        dfree(h, h->rdf.hist, 0);
+        h->rdf = TbRdf();
+        h->rdf_on = false;
+        return TB_OK;
+    }
+    if (!(dx_ei > 0.0) || !(dx_ii > 0.0) || nbin < 1)
+        return fail(TB_ERR_ARG, "RDF needs dx_ei > 0, dx_ii > 0 and nbin >= 1");
+    if (h->cfg.walk != TB_WALK_REF64) return fail(TB_ERR_STATE, "the RDF is accumulated by the FP64 walk");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipStreamSynchronize(h->s));
+    dfree(h, h->rdf.hist, 0);
+    h->rdf = TbRdf();
+    h->rdf_on = false;
+    int rc = dalloc_t(h, &h->rdf.hist, (size_t)2 * nbin);
+    if (rc) return fail(rc, "RDF allocation failed");
+    HIPCHK(hipMemsetAsync(h->rdf.hist, 0, (size_t)2 * nbin * sizeof(int64_t), h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->rdf.dx[0] = dx_ei;
+    h->rdf.dx[1] = dx_ii;
+    h->rdf.nbin = nbin;
+    h->rdf_on = true;
+    return TB_OK;
+}
+
+int tb_rdf_get(tb_handle *h, int32_t nbin, int64_t *rdf_ei, int64_t *rdf_ii, int32_t reset)
+{
+    if (!h || !rdf_ei || !rdf_ii) return fail(TB_ERR_ARG, "null argument");
+    if (!h->rdf_on) return fail(TB_ERR_STATE, "tb_rdf_get before tb_rdf_enable");
+    if (nbin != h->rdf.nbin) return fail(TB_ERR_ARG, "nbin %d differs from the enabled %d", nbin, h->rdf.nbin);
+    HIPCHK(hipSetDevice(h->cfg.device));
+    HIPCHK(hipMemcpyAsync(rdf_ei, h->rdf.hist, (size_t)nbin * 8, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(rdf_ii, h->rdf.hist + nbin, (size_t)nbin * 8, hipMemcpyDeviceToHost, h->s));
+    if (reset) HIPCHK(hipMemsetAsync(h->rdf.hist, 0, (size_t)2 * nbin * sizeof(int64_t), h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
     return TB_OK;
 }
 

@@ -106,8 +106,14 @@ void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err);
 void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp);
 
 // ----- walk64.hip
+// RDF accumulation of the FP64 walk: hist = [2][nbin] int64 (e-i, i-i) on the device
+struct TbRdf {
+    int64_t *hist = nullptr;
+    double dx[2] = {0.0, 0.0};
+    int nbin = 0;
+};
 void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,
-               double *epart, int64_t *counters);
+               double *epart, int64_t *counters, const TbRdf *rdf = nullptr);
 void tb_reduce_partials(hipStream_t s, int nparts, int width, const double *part, double *out);
 // FP64 walk for a target list (tsp/idx, key order) or, with tsp == nullptr, every stride-th target
 void tb_walk64_list(hipStream_t s, int nt, const int32_t *tsp, const int32_t *idx, int stride, const SpeciesDev &E,

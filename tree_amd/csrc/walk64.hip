@@ -34,10 +34,34 @@ struct Cnt {
     unsigned long long visits, m2p, p2p;
 };
 
-// one tree, one target: Search_Tree_* with the recursion unrolled into skip pointers
-template <bool COUNT>
+// Leaf-pair RDF histograms (tree.f90:916-919, 1021-1024, 1125-1129): bins
+// INT(r / dx) + 1 = 1 .. nbin.  The first RDF_LDS bins of each histogram are
+// counted in the block's LDS (32-bit: a block's 256 targets cannot pass 2^32
+// pairs) and added to the global 64-bit histograms once per block; higher bins
+// go straight to global memory.
+constexpr int RDF_LDS = 2048;
+struct RdfHist {
+    unsigned long long *g;  // [2][nbin]: e-i, i-i
+    double dx[2];
+    int nbin;
+    uint32_t *l;            // the block's LDS copy of the first RDF_LDS bins, [2][RDF_LDS]
+};
+
+__device__ inline void rdf_add(const RdfHist &H, int h, double r)
+{
+    const int id = (int)(r / H.dx[h]) + 1;
+    if (id <= H.nbin && id > 0) {
+        if (id <= RDF_LDS) atomicAdd(&H.l[h * RDF_LDS + id - 1], 1u);
+        else atomicAdd(&H.g[(size_t)h * H.nbin + id - 1], 1ull);
+    }
+}
+
+// one tree, one target: Search_Tree_* with the recursion unrolled into skip
+// pointers.  rh: the RDF histogram its leaf pairs go to (0 e-i, 1 i-i, -1 none)
+template <bool COUNT, bool RDF = false>
 __device__ inline void walk_tree(const TreeView T, double px, double py, double pz, bool like, double ffm,
-                                 double ffp, double rs, double &f0, double &f1, double &f2, double &e, Cnt &c)
+                                 double ffp, double rs, double &f0, double &f1, double &f2, double &e, Cnt &c,
+                                 const RdfHist *H = nullptr, int rh = -1)
 {
     uint32_t i = 0;
     while (i < T.nn) {
@@ -55,6 +79,7 @@ __device__ inline void walk_tree(const TreeView T, double px, double py, double 
                     f0 = f0 + ffp * x / r / r2;
                     f1 = f1 + ffp * y / r / r2;
                     f2 = f2 + ffp * z / r / r2;
+                    if (RDF && rh >= 0) rdf_add(*H, rh, r);
                 }
             } else {
                 // simplified Kelbg, tree.f90:905-912
@@ -66,6 +91,7 @@ __device__ inline void walk_tree(const TreeView T, double px, double py, double 
                 f0 = f0 + ffp * x * g / r2;
                 f1 = f1 + ffp * y * g / r2;
                 f2 = f2 + ffp * z * g / r2;
+                if (RDF && rh >= 0) rdf_add(*H, rh, r);
             }
             const uint32_t ni = nx & ~TB_LEAF;
             if (ni <= i) break;  // skip pointers only advance; guards a malformed table
@@ -128,14 +154,21 @@ __device__ inline double block_sum(double v, double *lds)
     return r;
 }
 
-template <bool COUNT>
+template <bool COUNT, bool RDF>
 __global__ __launch_bounds__(BT) void k_walk64(int n, int tsp, const double *__restrict__ px,
                                                const double *__restrict__ py, const double *__restrict__ pz,
                                                double *__restrict__ fx, double *__restrict__ fy,
                                                double *__restrict__ fz, TreeView te, TreeView ti, double rs,
-                                               double *__restrict__ epart, unsigned long long *__restrict__ counters)
+                                               double *__restrict__ epart, unsigned long long *__restrict__ counters,
+                                               RdfHist H)
 {
     __shared__ double lds[BT];
+    __shared__ uint32_t hl[RDF ? 2 * RDF_LDS : 1];
+    if (RDF) {
+        for (int k = threadIdx.x; k < 2 * RDF_LDS; k += BT) hl[k] = 0u;
+        H.l = hl;
+        __syncthreads();
+    }
     const int i = blockIdx.x * BT + threadIdx.x;
     double eA = 0.0, eB = 0.0;
     Cnt cA = {0, 0, 0}, cB = {0, 0, 0};
@@ -143,13 +176,22 @@ __global__ __launch_bounds__(BT) void k_walk64(int n, int tsp, const double *__r
         const double x = px[i], y = py[i], z = pz[i];
         const double tq = tsp ? 1.0 : -1.0;
         double f0 = 0.0, f1 = 0.0, f2 = 0.0;  // LocalCoulombSum zeroes ff, tree.f90:675-680
-        // electron tree: ee (like) for electron targets, ie (Kelbg) for ions
-        walk_tree<COUNT>(te, x, y, z, tsp == 0, tq * TB_EPS, tsp == 0 ? TB_EPS : -TB_EPS, rs, f0, f1, f2, eA, cA);
-        // ion tree: ei (Kelbg) for electron targets, ii (like) for ions
-        walk_tree<COUNT>(ti, x, y, z, tsp == 1, tq * TB_EPS, tsp == 1 ? TB_EPS : -TB_EPS, rs, f0, f1, f2, eB, cB);
+        // electron tree: ee (like, no RDF) for electron targets, ie (Kelbg, e-i RDF) for ions
+        walk_tree<COUNT, RDF>(te, x, y, z, tsp == 0, tq * TB_EPS, tsp == 0 ? TB_EPS : -TB_EPS, rs, f0, f1, f2,
+                              eA, cA, &H, tsp == 0 ? -1 : 0);
+        // ion tree: ei (Kelbg, e-i RDF) for electron targets, ii (like, i-i RDF) for ions
+        walk_tree<COUNT, RDF>(ti, x, y, z, tsp == 1, tq * TB_EPS, tsp == 1 ? TB_EPS : -TB_EPS, rs, f0, f1, f2,
+                              eB, cB, &H, tsp == 0 ? 0 : 1);
         fx[i] = f0;
         fy[i] = f1;
         fz[i] = f2;
+    }
+    if (RDF) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < 2 * RDF_LDS; k += BT) {
+            const int h = k / RDF_LDS, b = k % RDF_LDS;
+            if (hl[k] && b < H.nbin) atomicAdd(&H.g[(size_t)h * H.nbin + b], (unsigned long long)hl[k]);
+        }
     }
     double sA = block_sum(eA, lds);
     double sB = block_sum(eB, lds);
@@ -361,17 +403,28 @@ int tb_walk64_blocks(int n) { return nblk(n, BT); }
 int tb_vv_blocks(int n) { return nblk(n, BT); }
 
 void tb_walk64(hipStream_t s, int tsp, const SpeciesDev &T, const SpeciesDev &E, const SpeciesDev &I, double rs,
-               double *epart, int64_t *counters)
+               double *epart, int64_t *counters, const TbRdf *rdf)
 {
     const int c = T.cur;
-    if (counters)
-        hipLaunchKernelGGL(k_walk64<true>, dim3(nblk(T.n, BT)), dim3(BT), 0, s, T.n, tsp, T.x[c][0], T.x[c][1],
-                           T.x[c][2], T.f[c][0], T.f[c][1], T.f[c][2], view(E), view(I), rs, epart,
-                           (unsigned long long *)counters);
-    else
-        hipLaunchKernelGGL(k_walk64<false>, dim3(nblk(T.n, BT)), dim3(BT), 0, s, T.n, tsp, T.x[c][0], T.x[c][1],
-                           T.x[c][2], T.f[c][0], T.f[c][1], T.f[c][2], view(E), view(I), rs, epart,
-                           (unsigned long long *)nullptr);
+    RdfHist H;
+    H.g = rdf ? (unsigned long long *)rdf->hist : nullptr;
+    H.dx[0] = rdf ? rdf->dx[0] : 1.0;
+    H.dx[1] = rdf ? rdf->dx[1] : 1.0;
+    H.nbin = rdf ? rdf->nbin : 0;
+    H.l = nullptr;
+    const dim3 g(nblk(T.n, BT)), b(BT);
+    unsigned long long *cnt = (unsigned long long *)counters;
+#define TB_W64(C, R)                                                                                                  \
+    hipLaunchKernelGGL((k_walk64<C, R>), g, b, 0, s, T.n, tsp, T.x[c][0], T.x[c][1], T.x[c][2], T.f[c][0],          \
+                       T.f[c][1], T.f[c][2], view(E), view(I), rs, epart, cnt, H)
+    if (counters) {
+        if (rdf) TB_W64(true, true);
+        else TB_W64(true, false);
+    } else {
+        if (rdf) TB_W64(false, true);
+        else TB_W64(false, false);
+    }
+#undef TB_W64
 }
 
 void tb_reduce_partials(hipStream_t s, int nparts, int width, const double *part, double *out)

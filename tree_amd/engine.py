@@ -30,6 +30,7 @@ TPS = 0.10180505305371923        # ns per code time unit, data.f90:39
 XM_E = 5.4858e-4                 # amu
 XM_I = 85.0                      # amu (Rb-85)
 
+NDIST = 10000  # histogram bins of the reference's RDF, less one (data.f90:38)
 WALK = {"ref64": 0, "mixed": 1}
 MLP_NPARAMS = 1377
 MAC = {"fixed": 0, "learned": 1}
@@ -97,6 +98,8 @@ SIGNATURES = [
     ("tb_time_walk", ctypes.c_int, [_P, _I, _P]),
     ("tb_count_interactions", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_get_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    ("tb_rdf_enable", ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _I]),
+    ("tb_rdf_get", ctypes.c_int, [_P, _I, _P, _P, _I]),
     ("tb_mlp_set", ctypes.c_int, [_P, _P, _I]),
     ("tb_mlp_get", ctypes.c_int, [_P, _P, _I]),
     ("tb_mlp_forward", ctypes.c_int, [_P, _I, _P, _I, _P]),
@@ -322,6 +325,18 @@ class Engine:
         ms = np.zeros(3, np.float32)
         _check(self.lib.tb_autotune(self.h, ctypes.byref(g), _ptr(ms)))
         return g.value, ms
+
+    def rdf_enable(self, dx_ei, dx_ii, nbin=NDIST + 1):
+        """Accumulate the leaf-pair RDF histograms in every force (FP64 walk); nbin = 0 disables."""
+        _check(self.lib.tb_rdf_enable(self.h, dx_ei, dx_ii, nbin))
+        self.rdf_nbin = nbin
+
+    def rdf_get(self, reset=False):
+        """(rdf_ei, rdf_ii) int64 histograms accumulated since enable or the last reset."""
+        ei = np.zeros(self.rdf_nbin, np.int64)
+        ii = np.zeros(self.rdf_nbin, np.int64)
+        _check(self.lib.tb_rdf_get(self.h, self.rdf_nbin, _ptr(ei), _ptr(ii), 1 if reset else 0))
+        return ei, ii
 
     def stats(self):
         s = Stats()

@@ -42,6 +42,8 @@ PROGRAM OCTREE_GPU
      CALL Vinit_REST(NS, qel, qion, param, ttime, sys, dt)
   END IF
 
+  ! the RDF accumulates from here (main.f90:118-120); the mixed walk keeps none
+  IF (walk == TB_WALK_REF64) CALL Engine_RDF_Init(NS)
   ttime%Ndump = NINT(ttime%tdump/dt)
   ttime%Nrest = NINT(ttime%trest/dt)
   ttime%Nsamp = NINT(ttime%tsamp/dt)
@@ -56,6 +58,7 @@ PROGRAM OCTREE_GPU
      CALL VVerletNotemp1(NS, qel, qion, param, dt, ttime)
      CALL Force(NS, qel, qion, param, ttime)
      CALL VVerletNotemp2(NS, qel, qion, param, sys, dt, ttime)
+     IF (walk == TB_WALK_REF64 .AND. MOD(ttime%Nloop, ttime%Ndump) == 0) CALL RDF(NS, ttime)
      sys%Epot = sys%Epot_e + sys%Epot_i
      IF (MOD(ttime%Nloop, ttime%Nsamp) == 0) CALL TEMPORAL(NS, sys, ttime)
      IF (MOD(ttime%Nloop, ttime%Nrest) == 0) THEN
@@ -213,6 +216,31 @@ CONTAINS
     CALL BuildLocalTree(NS, qel, qion, Roote, Rooti, ttime)
     CALL LocalCoulombSum(NS, Roote, Rooti, qel, qion, sys, param, ttime)
   END SUBROUTINE Force
+
+  ! sample.f90:5-50 on one domain: rdf###.dat, counts averaged over the Ndump
+  ! steps since the last print, then cleared
+  SUBROUTINE RDF(NS, ttime)
+    TYPE(NM), INTENT(IN) :: NS
+    TYPE(TM), INTENT(IN) :: ttime
+    INTEGER(C_INT64_T) :: Nei(Ndist + 1), Nii(Ndist + 1)
+    REAL(C_DOUBLE) :: Ntau
+    INTEGER :: i, Nfreq
+    CHARACTER(LEN=20) :: FILENAME
+    CALL Engine_RDF_Take(Nei, Nii)
+    Ntau = REAL(ttime%Ndump, C_DOUBLE)
+    Nfreq = INT(ttime%Nloop/ttime%Ndump)
+    WRITE(FILENAME, 50) Nfreq
+    OPEN(UNIT=30, FILE=FILENAME)
+    WRITE(30, 100)
+    DO i = 1, Ndist + 1
+       WRITE(30, 200) NS%dx_ei*REAL(i, C_DOUBLE), REAL(Nei(i), C_DOUBLE)/Ntau, &
+            NS%dx_ii*REAL(i, C_DOUBLE), REAL(Nii(i), C_DOUBLE)/Ntau
+    END DO
+    CLOSE(30)
+50  FORMAT("rdf", I3.3, ".dat")
+100 FORMAT("# R_ei,   RDF_ei       R_ii     RDF_ii     scaled RDF_ii")
+200 FORMAT(5(ES14.6, 1X))
+  END SUBROUTINE RDF
 
   ! print.f90:55-96 on one domain
   SUBROUTINE TEMPORAL(NS, sys, ttime)

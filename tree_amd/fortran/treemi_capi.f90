@@ -55,6 +55,18 @@ MODULE TREEMI_CAPI
        INTEGER(C_INT32_T), VALUE :: sample
        REAL(C_DOUBLE), INTENT(OUT) :: sums(4)
      END FUNCTION tb_vv2
+     INTEGER(C_INT) FUNCTION tb_rdf_enable(h, dx_ei, dx_ii, nbin) BIND(C, NAME='tb_rdf_enable')
+       IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+       TYPE(C_PTR), VALUE :: h
+       REAL(C_DOUBLE), VALUE :: dx_ei, dx_ii
+       INTEGER(C_INT32_T), VALUE :: nbin
+     END FUNCTION tb_rdf_enable
+     INTEGER(C_INT) FUNCTION tb_rdf_get(h, nbin, rdf_ei, rdf_ii, reset) BIND(C, NAME='tb_rdf_get')
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: h
+       INTEGER(C_INT32_T), VALUE :: nbin, reset
+       INTEGER(C_INT64_T), INTENT(OUT) :: rdf_ei(*), rdf_ii(*)
+     END FUNCTION tb_rdf_get
      TYPE(C_PTR) FUNCTION tb_last_error() BIND(C, NAME='tb_last_error')
        IMPORT :: C_PTR
      END FUNCTION tb_last_error

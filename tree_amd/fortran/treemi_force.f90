@@ -53,6 +53,19 @@ CONTAINS
     CALL tb_check(tb_get_particles(hnd, qel, qion), "Engine_Sync")
   END SUBROUTINE Engine_Sync
 
+  ! the RDF histograms (NS%RDF_ei / RDF_ii, data.f90:139), accumulated by the
+  ! FP64 walk in every force from here on (main.f90:118-120 zeroes them here)
+  SUBROUTINE Engine_RDF_Init(NS)
+    TYPE(NM), INTENT(IN) :: NS
+    CALL tb_check(tb_rdf_enable(hnd, NS%dx_ei, NS%dx_ii, INT(Ndist + 1, C_INT32_T)), "Engine_RDF_Init")
+  END SUBROUTINE Engine_RDF_Init
+
+  ! the histograms accumulated since the last call, which clears them
+  SUBROUTINE Engine_RDF_Take(rdf_ei, rdf_ii)
+    INTEGER(C_INT64_T), INTENT(OUT) :: rdf_ei(Ndist + 1), rdf_ii(Ndist + 1)
+    CALL tb_check(tb_rdf_get(hnd, INT(Ndist + 1, C_INT32_T), rdf_ei, rdf_ii, 1_C_INT32_T), "Engine_RDF_Take")
+  END SUBROUTINE Engine_RDF_Take
+
   SUBROUTINE Engine_Done()
     INTEGER(C_INT) :: rc
     rc = tb_destroy(hnd)

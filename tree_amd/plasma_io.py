@@ -42,6 +42,11 @@ def read_ener(path):
     return np.loadtxt(path, comments="#", ndmin=2)
 
 
+def read_rdf(path):
+    """rdf###.dat (sample.f90:34-38): columns R_ei, RDF_ei, R_ii, RDF_ii (counts per step)."""
+    return np.loadtxt(path, comments="#", ndmin=2)
+
+
 def read_res(path):
     """(qel, qion) from a res###.xyz restart frame."""
     rows = [ln.split() for ln in open(path).read().splitlines()[2:] if ln.strip()]
