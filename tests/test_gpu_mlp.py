@@ -77,29 +77,31 @@ def _direct_err(eng, qel, qion, n):
 def test_learned_mac_gain():
     """Trained on labels the method generates, the learned criterion reaches the
     fixed-theta walk's rms error (sampled against the FP64 direct sum) with fewer
-    M2P+P2P interactions.  Fixed walk: theta 0.3.  Learned walk: cells beyond
-    s/0.15 always accepted, cells within s/0.5 always opened, the network
-    deciding in between; err_target is bisected (in log10) to the largest value
-    whose error is at or below the fixed walk's, and the work is compared there."""
+    M2P+P2P interactions, in the regime of the 1e-4 headline: 2^17 + 2^17
+    particles, fixed walk at theta 0.15; learned walk with cells beyond s/0.105
+    always accepted, cells within s/0.24 always opened, the network deciding in
+    between.  err_target is bisected (in log10) to the largest value whose error
+    is at or below the fixed walk's, and the work is compared there."""
     announce("learned_mac_gain")
-    n = 30000
-    qel, qion = ta.gen_plasma(31, n, n, 250.0)
-    with _eng(theta=0.3, theta_mixed=0.5, theta_lo=0.15, walk="mixed", err_target=1e-7, mac="learned") as eng:
+    n = 1 << 17
+    qel, qion = ta.gen_plasma(31, n, n, 300.0)
+    with _eng(theta=0.15, theta_mixed=0.24, theta_lo=0.105, walk="mixed", err_target=1e-7, mac="learned") as eng:
         eng.set_particles(qel, qion)
         eng.build()
-        X, y = eng.mlp_collect(cap=400000)
+        X, y = eng.mlp_collect(cap=1 << 20)
         assert len(X) > 5000
         eng.mlp_set(mlp_ref.init_params(0).astype(np.float32))
-        eng.mlp_train(X, y, steps=3000, batch=4096, lr=3e-3)
-        eng.configure(mac="fixed", theta_mixed=0.3)
-        err_fix = _direct_err(eng, qel, qion, 2000)
-        work_fix = sum(eng.count_interactions()["m2p"]) + sum(eng.count_interactions()["p2p"])
-        eng.configure(mac="learned", theta_mixed=0.5)
-        lo, hi, best, rows = -10.0, -2.0, None, []
+        eng.mlp_train(X, y, steps=3000, batch=8192, lr=2e-3)
+        eng.configure(mac="fixed", theta_mixed=0.15)
+        err_fix = _direct_err(eng, qel, qion, 2048)
+        st = eng.count_interactions()
+        work_fix = sum(st["m2p"]) + sum(st["p2p"])
+        eng.configure(mac="learned", theta_mixed=0.24)
+        lo, hi, best, rows = -10.0, -3.0, None, []
         for _ in range(10):
             mid = 0.5 * (lo + hi)
             eng.configure(err_target=10.0 ** mid)
-            err_l = _direct_err(eng, qel, qion, 2000)
+            err_l = _direct_err(eng, qel, qion, 2048)
             st = eng.count_interactions()
             work_l = sum(st["m2p"]) + sum(st["p2p"])
             rows.append((mid, err_l, work_l))

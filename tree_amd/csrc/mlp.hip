@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float
 }
 
 __global__ __launch_bounds__(256) void k_mlp_fwd(int B, const float *__restrict__ X, const float *__restrict__ P,
-                                                 int bf16, float *__restrict__ out)
+                                                 const TbMlpPack *__restrict__ K, int bf16, float *__restrict__ out)
 {
     __shared__ float sP[NPAR];
     for (int j = threadIdx.x; j < NPAR; j += 256) sP[j] = bf16 ? tb_bf16_round(P[j]) : P[j];
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(int B, const float *__restrict_
             float xq[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) xq[q] = si < B ? X[(size_t)si * NIN + 4 * h + q] : 0.0f;
-            pr[tile] = tb_mlp_eval32_mfma(sP, xq);
+            pr[tile] = tb_mlp_eval32_mfma(K, xq);
         }
         if (s < B) out[s] = l < 32 ? pr[0] : pr[1];
         return;
@@ -166,7 +166,8 @@ inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
 int tb_mlp_alloc(MlpDev &M)
 {
     if (M.P) return TB_OK;
-    size_t bytes = (size_t)5 * (NPAR + 1) * sizeof(float) + (size_t)NPAR * sizeof(uint16_t) + 64;
+    size_t bytes = (size_t)5 * (NPAR + 1) * sizeof(float) + (size_t)NPAR * sizeof(uint16_t) + 64 + 256 +
+                   sizeof(TbMlpPack);
     if (hipMalloc(&M.base, bytes) != hipSuccess) return TB_ERR_NOMEM;
     float *f = (float *)M.base;
     M.P = f;
@@ -174,6 +175,8 @@ int tb_mlp_alloc(MlpDev &M)
     M.m = f + 2 * (NPAR + 1);
     M.v = f + 3 * (NPAR + 1);
     M.Pbf = f + 4 * (NPAR + 1);
+    const size_t po = ((size_t)5 * (NPAR + 1) * sizeof(float) + 255) & ~(size_t)255;
+    M.Ppk = (TbMlpPack *)((char *)M.base + po);
     hipMemset(M.base, 0, bytes);
     return TB_OK;
 }
@@ -201,11 +204,30 @@ __global__ void k_bf16_copy(const float *__restrict__ P, float *__restrict__ Q)
 {
     for (int j = threadIdx.x; j < NPAR; j += blockDim.x) Q[j] = tb_bf16_round(P[j]);
 }
+
+// TbMlpPack from the bf16-rounded parameters (one block of 64 threads)
+__global__ void k_mlp_pack(const float *__restrict__ Pb, TbMlpPack *__restrict__ K)
+{
+    constexpr int O_B1 = NIN * NH, O_W2 = O_B1 + NH, O_B2 = O_W2 + NH * NH, O_W3 = O_B2 + NH, O_B3 = O_W3 + NH;
+    const int l = threadIdx.x, i = l & 31, h = l >> 5;
+    for (int q = 0; q < 4; q++) {
+        K->a1[l][q] = tb_bf16_bits(Pb[(4 * h + q) * NH + i]);
+        for (int c = 0; c < 4; c++) K->a2[l][c][q] = tb_bf16_bits(Pb[O_W2 + (8 * c + 4 * h + q) * NH + i]);
+    }
+    if (l < 32) {
+        const int hh = l >> 4, v = l & 15, r = 8 * (v >> 2) + 4 * hh + (v & 3);
+        K->b1[hh][v] = Pb[O_B1 + r];
+        K->b2[hh][v] = Pb[O_B2 + r];
+        K->w3[hh][v] = Pb[O_W3 + r];
+    }
+    if (l == 0) K->b3 = Pb[O_B3];
+}
 }  // namespace
 
 void tb_mlp_refresh_bf16(hipStream_t s, MlpDev &M)
 {
     hipLaunchKernelGGL(k_bf16_copy, dim3(1), dim3(256), 0, s, M.P, M.Pbf);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, s, M.Pbf, M.Ppk);
 }
 
 static int ensure_data(MlpDev &M, int n)
@@ -257,7 +279,7 @@ int tb_mlp_forward(hipStream_t s, MlpDev &M, int n, const float *X, int bf16, fl
     if (rc) return rc;
     float *dX = (float *)M.data, *dY = dX + (size_t)n * NIN;
     if (hipMemcpyAsync(dX, X, (size_t)n * NIN * 4, hipMemcpyHostToDevice, s) != hipSuccess) return TB_ERR_HIP;
-    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk(n, 256)), dim3(256), 0, s, n, dX, M.P, bf16, dY);
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk(n, 256)), dim3(256), 0, s, n, dX, M.P, M.Ppk, bf16, dY);
     if (hipMemcpyAsync(out, dY, (size_t)n * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return TB_ERR_HIP;
     return hipStreamSynchronize(s) == hipSuccess ? TB_OK : TB_ERR_HIP;
 }

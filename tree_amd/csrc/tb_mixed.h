@@ -6,7 +6,9 @@
 //   binfo uint32   leaf: TB_LEAF | joint particle index; twig: first child << 3 | (nchild - 1)
 //   bq0   float4   q, Dx, Dy, Dz          net charge and dipole about the centre
 //   bq1   float4   Q'xx, Q'yy, Q'xy, Q'yz traceless quadrupole 3M - tr(M) I
-//   bq2   float4   Q'zx, s, count, 0      (Q'zz = -Q'xx - Q'yy)
+//   bq2   float4   Q'zx, s, count, log2(s)/8      (Q'zz = -Q'xx - Q'yy)
+//   bf    float4   the learned criterion's per-cell features (log2 / 8 of
+//                  |q|/count, |D|/(count s), |Q'|/(count s^2), count)
 #pragma once
 #include "tb_internal.h"
 #include "tb_mlp.h"
@@ -15,7 +17,7 @@ struct MixedTree {
     int nn = 0, cap = 0;
     double4 *bmac = nullptr;
     uint32_t *binfo = nullptr;
-    float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr;
+    float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr, *bf = nullptr;
     uint32_t *inv = nullptr;
 };
 
@@ -33,7 +35,7 @@ struct Domain {
     int n[2], ng[2];
     const double4 *bmac;
     const uint32_t *binfo;
-    const float4 *bq0, *bq1, *bq2;
+    const float4 *bq0, *bq1, *bq2, *bf;
     float inv_rs, rs;
     int gbase;        // first global group index of this domain
     int nn;           // cells in the tree

@@ -16,6 +16,7 @@ struct MlpDev {
     float *G = nullptr;     // gradient (+ loss at [NPAR])
     float *m = nullptr, *v = nullptr;  // Adam moments
     float *Pbf = nullptr;   // parameters rounded to bf16 (stored as fp32) for inference
+    struct TbMlpPack *Ppk = nullptr;  // the same, packed as matrix-core operands
     float *part = nullptr;  // per-block gradient partials
     void *data = nullptr;   // uploaded samples
     size_t data_bytes = 0;
@@ -70,45 +71,43 @@ typedef float tb_v16f __attribute__((ext_vector_type(16)));
 
 __device__ inline short tb_bf16_bits(float f) { return (short)(__float_as_uint(tb_bf16_round(f)) >> 16); }
 
+// The bf16 parameters laid out as the matrix-core operands need them, so each
+// lane loads its operands with a few vector loads and no conversion (built by
+// tb_mlp_refresh_bf16 whenever the parameters change).
+struct TbMlpPack {
+    tb_v4s a1[64];                  // lane l: W1[4 (l / 32) + q][l % 32]
+    tb_v4s a2[64][4];               // lane l, K-chunk c: W2[8 c + 4 (l / 32) + q][l % 32]
+    float b1[2][16], b2[2][16], w3[2][16];  // half h, register v: unit 8 (v / 4) + 4 h + v % 4
+    float b3;
+};
+
 // lane l supplies features 4 (l / 32) .. 4 (l / 32) + 3 of sample l % 32 in
 // xq and receives the output for sample l % 32 (lanes l and l ^ 32 both).
-// P: the bf16-rounded parameter vector.  All 64 lanes must call it together.
-__device__ inline float tb_mlp_eval32_mfma(const float *P, const float (&xq)[4])
+// All 64 lanes must call it together.
+__device__ inline float tb_mlp_eval32_mfma(const TbMlpPack *__restrict__ K, const float (&xq)[4])
 {
-    constexpr int NH = TB_MLP_NH;
-    constexpr int O_B1 = TB_MLP_NIN * NH, O_W2 = O_B1 + NH, O_B2 = O_W2 + NH * NH, O_W3 = O_B2 + NH,
-                  O_B3 = O_W3 + NH;
-    const int l = threadIdx.x & 63, i = l & 31, h = l >> 5;
-    tb_v4s a, b;
+    const int l = threadIdx.x & 63, h = l >> 5;
+    tb_v4s b;
     tb_v16f c, d;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        a[q] = tb_bf16_bits(P[(4 * h + q) * NH + i]);  // W1[k][hidden i]
-        b[q] = tb_bf16_bits(xq[q]);                     // X[sample i][k]
-    }
+    for (int q = 0; q < 4; q++) b[q] = tb_bf16_bits(xq[q]);  // X[sample][k]
 #pragma unroll
     for (int v = 0; v < 16; v++) {
-        const int r = 8 * (v >> 2) + 4 * h + (v & 3);
-        c[v] = P[O_B1 + r];
-        d[v] = P[O_B2 + r];
+        c[v] = K->b1[h][v];
+        d[v] = K->b2[h][v];
     }
-    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
-#pragma unroll
-    for (int v = 0; v < 16; v++) c[v] = fmaxf(c[v], 0.0f);
+    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(K->a1[l], b, c, 0, 0, 0);
 #pragma unroll
     for (int cc = 0; cc < 4; cc++) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            a[q] = tb_bf16_bits(P[O_W2 + (8 * cc + 4 * h + q) * NH + i]);  // W2[in][out i]
-            b[q] = tb_bf16_bits(c[4 * cc + q]);                             // H1^T[in][sample i]
-        }
-        d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, d, 0, 0, 0);
+        for (int q = 0; q < 4; q++) b[q] = tb_bf16_bits(fmaxf(c[4 * cc + q], 0.0f));  // H1^T[in][sample]
+        d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(K->a2[l][cc], b, d, 0, 0, 0);
     }
     float y = 0.0f;
 #pragma unroll
-    for (int v = 0; v < 16; v++) y += fmaxf(d[v], 0.0f) * P[O_W3 + 8 * (v >> 2) + 4 * h + (v & 3)];
+    for (int v = 0; v < 16; v++) y += fmaxf(d[v], 0.0f) * K->w3[h][v];
     y += __shfl_xor(y, 32, 64);
-    return y + P[O_B3];
+    return y + K->b3;
 }
 
 int tb_mlp_alloc(MlpDev &M);

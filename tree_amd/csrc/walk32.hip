@@ -133,7 +133,7 @@ struct WalkArgs {
     int ndom, ngt;        // domains, groups in all domains
     float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
-    const float *mlpw;    // learned: bf16-rounded parameters
+    const TbMlpPack *mlpk;  // learned: the bf16 parameters packed as matrix-core operands
     BandRec *rec;         // collect: band pairs
     int rec_cap;
     int32_t *ctr;         // [1] error flag, [2] prep error, [3] band pair count, [8..15] chunk counters
@@ -143,26 +143,29 @@ struct WalkArgs {
 
 constexpr int MODE_FIXED = 0, MODE_LEARNED = 1, MODE_COLLECT = 2;
 
-// features of a (cell, target group) pair, log2 scaled by 1/8.  emin is the
-// smallest net field |E| = |F|/eps over the group's targets from the previous
-// force evaluation: the label is an error relative to the target's net force,
-// which in a quasi-neutral plasma is far below the cell's gross field cnt/d^2.
-__device__ inline void band_features(float s, float cnt, float d, float4 q0, float4 q1, float4 q2, float hx,
-                                     float hy, float hz, float emin, bool like_net, int nact, float *f)
+// features of a (cell, target group) pair, log2 scaled by 1/8 (k):
+//   f0 s/d; f1 |q|/count; f2 |D|/(count s); f3 |Q'|/(count s^2); f4 count;
+//   f5 group half-diagonal h / d; f6 emin d^2 / count; f7 +-targets in the
+//   group (+ when the cell's net charge is like the targets').
+// f1..f4 depend on the cell only and come precomputed in FP64 (bf, prep);
+// k log2 s sits in bq2.w.  emin is the smallest net field |E| = |F| / eps
+// over the group's targets from the previous force: the label is an error
+// relative to the target's net force, which in a quasi-neutral plasma is far
+// below the cell's gross field count / d^2.  Per group: klemin = k log2 emin,
+// knact = k log2 nact.
+__device__ inline void band_features(float4 bf, float kls, float d, float h, float klemin, bool like_net,
+                                     float knact, float *f)
 {
-    const float dn = sqrtf(q0.y * q0.y + q0.z * q0.z + q0.w * q0.w);
-    const float qzz = -q1.x - q1.y;
-    const float qn = sqrtf(q1.x * q1.x + q1.y * q1.y + qzz * qzz + 2.0f * (q1.z * q1.z + q1.w * q1.w + q2.x * q2.x));
-    const float h = sqrtf(hx * hx + hy * hy + hz * hz);
     const float k = 0.125f;
-    f[0] = k * __log2f(s / d);
-    f[1] = k * __log2f(fabsf(q0.x) / cnt + 9.5367431640625e-07f);
-    f[2] = k * __log2f(dn / (cnt * s) + 9.5367431640625e-07f);
-    f[3] = k * __log2f(qn / (cnt * s * s) + 9.5367431640625e-07f);
-    f[4] = k * __log2f(cnt);
+    const float kld = k * __log2f(d);
+    f[0] = kls - kld;
+    f[1] = bf.x;
+    f[2] = bf.y;
+    f[3] = bf.z;
+    f[4] = bf.w;
     f[5] = k * __log2f(h / d + 9.5367431640625e-07f);
-    f[6] = k * __log2f(emin * d * d / cnt + 9.5367431640625e-07f);
-    f[7] = (like_net ? 1.0f : -1.0f) * k * __log2f((float)nact);
+    f[6] = klemin + 2.0f * kld - bf.w;
+    f[7] = like_net ? knact : -knact;
 }
 
 template <int TPL>
@@ -265,7 +268,10 @@ __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TP
 }
 
 // At most 3 blocks of 4 waves fit a CU's LDS, i.e. 3 waves per SIMD: the
-// register budget is capped to match (512 / 3 = 170 VGPRs).
+// register budget is capped to match (512 / 3 = 170 VGPRs).  The learned
+// walk spills a few registers around its batched band decision under that
+// cap; measured at N = 4M that is still faster (257 ms) than giving it 256
+// registers at 2 waves per SIMD (290 ms).
 template <int TPL, int MODE, bool COUNT>
 __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
 {
@@ -351,6 +357,10 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f;
         const float hy = (float)(0.5 * (mxy - mny)) * 1.000001f + 1e-30f;
         const float hz = (float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f;
+        // per-group terms of the learned criterion's features
+        const float hdiag = sqrtf(hx * hx + hy * hy + hz * hz);
+        const float klemin = 0.125f * __log2f(fmaxf(emin, 1e-30f));
+        const float knact = 0.125f * __log2f((float)nact);
         float rx[TPL], ry[TPL], rz[TPL];
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
@@ -429,7 +439,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             if (MODE == MODE_COLLECT && band) {
                 const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
                 float f[TB_MLP_NIN];
-                band_features(q2.y, q2.z, sqrtf(d2), q0, q1, q2, hx, hy, hz, emin, q0.x * qt >= 0.0f, nact, f);
+                band_features(D.bf[node], q2.w, sqrtf(d2), hdiag, klemin, q0.x * qt >= 0.0f, knact, f);
                 {
                     open = true;  // the conservative walk opens it; the pair is recorded
                     if (q2.z <= (float)BAND_MAXCNT) {
@@ -523,14 +533,13 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                             const float ex = fmaxf(fabsf((float)(m.x - gx)) - hx, 0.0f);
                             const float ey = fmaxf(fabsf((float)(m.y - gy)) - hy, 0.0f);
                             const float ez = fmaxf(fabsf((float)(m.z - gz)) - hz, 0.0f);
-                            const float4 q0 = D.bq0[nd], q1 = D.bq1[nd], q2 = D.bq2[nd];
-                            band_features(q2.y, q2.z, sqrtf(ex * ex + ey * ey + ez * ez), q0, q1, q2, hx, hy, hz,
-                                          emin, q0.x * qt >= 0.0f, nact, f);
+                            band_features(D.bf[nd], D.bq2[nd].w, sqrtf(ex * ex + ey * ey + ez * ez), hdiag, klemin,
+                                          D.bq0[nd].x * qt >= 0.0f, knact, f);
                         }
                         float xq[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) xq[q] = (lane >> 5) ? f[4 + q] : f[q];
-                        pr[tile] = tb_mlp_eval32_mfma(W.mlpw, xq);
+                        pr[tile] = tb_mlp_eval32_mfma(W.mlpk, xq);
                     }
                     // lane l decides entry l (tile l / 32 holds it); without a
                     // previous force (emin = 0) band cells are opened
@@ -647,7 +656,8 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
                                                     const int32_t *__restrict__ ncount, double theta,
                                                     double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
                                                     float4 *__restrict__ bq0, float4 *__restrict__ bq1,
-                                                    float4 *__restrict__ bq2, int32_t *__restrict__ err)
+                                                    float4 *__restrict__ bq2, float4 *__restrict__ bf,
+                                                    int32_t *__restrict__ err)
 {
     const int b = blockIdx.x * BT + threadIdx.x;
     if (b >= nn) return;
@@ -661,6 +671,7 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
         bq0[b] = make_float4((float)Q[0], 0.f, 0.f, 0.f);
         bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         bq2[b] = make_float4(0.f, 0.f, 1.f, 0.f);
+        bf[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     int nch = 0;
@@ -677,7 +688,15 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
     const double tr = Q[4] + Q[5] + Q[6];
     bq0[b] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
     bq1[b] = make_float4((float)(3.0 * Q[4] - tr), (float)(3.0 * Q[5] - tr), (float)(3.0 * Q[7]), (float)(3.0 * Q[8]));
-    bq2[b] = make_float4((float)(3.0 * Q[9]), (float)c.w, (float)ncount[p], 0.f);
+    // the learned criterion's per-cell features, FP64 then rounded (see band_features)
+    const double k = 0.125, cn = (double)ncount[p], sz = c.w, tiny = 9.5367431640625e-07;
+    const double qxx = 3.0 * Q[4] - tr, qyy = 3.0 * Q[5] - tr, qzz = 3.0 * Q[6] - tr;
+    const double qn = sqrt(qxx * qxx + qyy * qyy + qzz * qzz +
+                           2.0 * (9.0 * Q[7] * Q[7] + 9.0 * Q[8] * Q[8] + 9.0 * Q[9] * Q[9]));
+    const double dn = sqrt(Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]);
+    bq2[b] = make_float4((float)(3.0 * Q[9]), (float)sz, (float)cn, (float)(k * log2(sz)));
+    bf[b] = make_float4((float)(k * log2(fabs(Q[0]) / cn + tiny)), (float)(k * log2(dn / (cn * sz) + tiny)),
+                        (float)(k * log2(qn / (cn * sz * sz) + tiny)), (float)(k * log2(cn)));
 }
 
 __global__ void k_zero_ctr(int32_t *ctr, int lo, int hi)
@@ -860,7 +879,7 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int g
         d.ng[k] = sp[k]->ng;
         d.grp[k] = sp[k]->grp;
     }
-    d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2;
+    d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2; d.bf = t->bf;
     d.inv_rs = (float)(1.0 / rs);
     d.rs = (float)rs;
     d.gbase = gbase;
@@ -907,7 +926,7 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : (cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta);
     W.inv_theta_lo = (float)(1.0 / th_lo);
     W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
-    W.mlpw = M.mlp.Pbf;
+    W.mlpk = M.mlp.Ppk;
     W.rec = M.rec;
     W.rec_cap = M.rec_cap;
     W.ctr = M.ctr;
@@ -978,18 +997,19 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, cons
     const JointDev &J = *jt;
     MixedTree &T = M.t[0];
     int cap = T.cap, rc;
-    int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap;
+    int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap;
     if ((rc = grow(&T.bmac, &c1, J.nn, bytes))) return rc;
     if ((rc = grow(&T.binfo, &c2, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq0, &c3, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq1, &c4, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq2, &c5, J.nn, bytes))) return rc;
     if ((rc = grow(&T.inv, &c6, J.nn, bytes))) return rc;
-    T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(c5, c6));
+    if ((rc = grow(&T.bf, &c7, J.nn, bytes))) return rc;
+    T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(std::min(c5, c6), c7));
     T.nn = J.nn;
     hipLaunchKernelGGL(k_inv, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv);
     hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv, J.nc, J.nnext, J.nq,
-                       J.nstart, J.ncount, theta, T.bmac, T.binfo, T.bq0, T.bq1, T.bq2, M.ctr + 2);
+                       J.nstart, J.ncount, theta, T.bmac, T.binfo, T.bq0, T.bq1, T.bq2, T.bf, M.ctr + 2);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: prep launch failed";
         return TB_ERR_HIP;
@@ -1106,6 +1126,7 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
 {
     MixedTree &T = M.t[0];
     hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.bq2); hipFree(T.inv);
+    hipFree(T.bf);
     T = MixedTree();
     hipFree(M.ctr);
     if (M.ddom) hipFree(M.ddom);
