@@ -277,6 +277,163 @@ def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
             "max": mx_l, "st": st_l, "prec": "fp32", **r}
 
 
+# ------------------------------------------------------------------ other configurations
+#
+# `--config c2|c3|c4|c5` measures the other configurations of the workload
+# family (one JSON line each; the default run is the headline above).
+
+
+def _steps_per_s(eng, dt, steps):
+    eng.build()
+    eng.force()
+    t0 = time.perf_counter()
+    eng.step(dt, steps)
+    return steps / (time.perf_counter() - t0), eng.stats()
+
+
+def run_c2(a):
+    """2^19 + 2^19, sigma 400 um, theta 0.5: the plain FP64 walk and the mixed walk at the same theta."""
+    n, dt = 1 << 19, 0.005 / ta.TPS
+    rs = ta.kelbg_length(10.0)
+    qel, qion = ta.gen_plasma(2, n, n, 400.0, te=10.0, ti=1e-3)
+    tsp, tix = sample_targets(n, n)
+    out = {"config": "C2: 2^19 e- + 2^19 ions, sigma 400 um, T_e 10 K, theta 0.5", "steps": a.steps}
+    for walk in ("ref64", "mixed"):
+        with ta.Engine(rs=rs, theta=0.5, theta_mixed=0.5, walk=walk) as eng:
+            eng.set_particles(qel, qion)
+            if walk == "ref64":
+                fdir = eng.direct_sample(tsp, tix)
+            rms, mx = err_full(eng, tsp, tix, fdir)
+            st = eng.count_interactions()
+            eng.set_particles(qel, qion)
+            sps, stt = _steps_per_s(eng, dt, a.steps)
+            out[walk] = {"steps_per_s": sps, "rms": rms, "max": mx, "ms_build": stt["ms_build"],
+                         "ms_walk": stt["ms_walk"],
+                         "interactions_per_s": (sum(st["m2p"]) + sum(st["p2p"])) * sps}
+            log("C2 %s: %.2f steps/s rms %.2e" % (walk, sps, rms))
+    print(json.dumps(out))
+
+
+def run_c3(a):
+    """4M, sigma 600 um: the mixed walk over theta in {0.3, 0.5, 0.7} and T_e in {1, 10, 100} K."""
+    n, dt = 1 << 21, 0.005 / ta.TPS
+    rows = []
+    for te in (1.0, 10.0, 100.0):
+        rs = ta.kelbg_length(te)
+        qel, qion = ta.gen_plasma(3, n, n, 600.0, te=te, ti=1e-3)
+        tsp, tix = sample_targets(n, n)
+        with ta.Engine(rs=rs, theta=0.5) as eng:
+            eng.set_particles(qel, qion)
+            fdir = eng.direct_sample(tsp, tix)
+        for th in (0.3, 0.5, 0.7):
+            with ta.Engine(rs=rs, theta=0.5, theta_mixed=th, walk="mixed") as eng:
+                eng.set_particles(qel, qion)
+                rms, mx = err_full(eng, tsp, tix, fdir)
+                st = eng.count_interactions()
+                eng.set_particles(qel, qion)
+                sps, stt = _steps_per_s(eng, dt, a.steps)
+                fl = walk_flops32(st)
+                rows.append({"T_e": te, "theta": th, "steps_per_s": sps, "rms": rms, "max": mx,
+                             "ms_walk": stt["ms_walk"], "ms_build": stt["ms_build"],
+                             "walk_tflops": fl / (stt["ms_walk"] * 1e-3) / 1e12,
+                             "walk_frac_nonpacked_fp32": fl / (stt["ms_walk"] * 1e-3) / 1e12 / PEAK["fp32_nonpacked"],
+                             "build_Mparticles_per_s": 2 * n / (stt["ms_build"] * 1e-3) / 1e6})
+                log("C3 T_e %g theta %.1f: %.2f steps/s rms %.2e walk %.1f ms" % (te, th, sps, rms, stt["ms_walk"]))
+    print(json.dumps({"config": "C3: 2^21 e- + 2^21 ions, sigma 600 um, mixed walk", "steps": a.steps, "rows": rows}))
+
+
+def run_c4(a):
+    """64 realisations, N_total log-uniform in [16k, 256k], T_e log-uniform in [1, 100] K,
+    sigma from a total peak density of 1e15 m^-3: one persistent walk launch per step,
+    against the same realisations stepped one engine at a time."""
+    rng = np.random.default_rng(1000)
+    R, dt, steps = 64, 0.005 / ta.TPS, a.steps
+    ntot = np.exp(rng.uniform(np.log(16384), np.log(262144), R)).astype(int) // 2 * 2
+    tes = np.exp(rng.uniform(0.0, np.log(100.0), R))
+    cfgs, data = [], []
+    for r in range(R):
+        sigma = (ntot[r] / ((2 * np.pi) ** 1.5 * 1e15)) ** (1 / 3) * 1e6
+        cfgs.append(ta.make_config(rs=ta.kelbg_length(tes[r]), theta=0.5, theta_mixed=0.5, walk="mixed"))
+        data.append(ta.gen_plasma(1000 + r, int(ntot[r]) // 2, int(ntot[r]) // 2, sigma, te=float(tes[r]), ti=1e-3))
+    with ta.Ensemble(cfgs) as ens:
+        for r, (qe, qi) in enumerate(data):
+            ens.set_particles(r, qe, qi)
+        ens.force()
+        g, ms = ens.autotune()
+        ens.step(dt, 2)
+        t0 = time.perf_counter()
+        ens.step(dt, steps)
+        t_ens = time.perf_counter() - t0
+    t_seq = 0.0  # the same steps, one engine at a time (set-up untimed, as for the ensemble)
+    for r, (qe, qi) in enumerate(data):
+        with ta.Engine(rs=ta.kelbg_length(tes[r]), theta=0.5, theta_mixed=0.5, walk="mixed", group=g) as eng:
+            eng.set_particles(qe, qi)
+            eng.build()
+            eng.force()
+            eng.step(dt, 2)
+            t0 = time.perf_counter()
+            eng.step(dt, steps)
+            t_seq += time.perf_counter() - t0
+    out = {"config": "C4: 64 realisations, N_total log-uniform 16k..256k (sum %d), T_e 1..100 K, theta 0.5, "
+                     "mixed walk" % int(ntot.sum()), "steps": steps, "group": int(g),
+           "ensemble_steps_per_s": steps / t_ens, "particle_steps_per_s": steps * float(ntot.sum()) / t_ens,
+           "one_engine_at_a_time_s": t_seq, "ensemble_s": t_ens, "batching_speedup": t_seq / t_ens}
+    log("C4: %.2f ensemble steps/s, %.3g particle-steps/s, batching x%.2f" %
+        (out["ensemble_steps_per_s"], out["particle_steps_per_s"], out["batching_speedup"]))
+    print(json.dumps(out))
+
+
+def run_c5(a):
+    """2^23 + 2^23, sigma 1 mm: the learned criterion (trained at 2^20) against the fixed-theta walk at
+    theta 0.5, its fallback and judge; err_target bisected to the fixed walk's rms error."""
+    global TARGET
+    n, dt = 1 << 23, 0.005 / ta.TPS
+    rs = ta.kelbg_length(10.0)
+    qel, qion = ta.gen_plasma(5, n, n, 1000.0, te=10.0, ti=1e-3)
+    tsp, tix = sample_targets(n, n)
+    with ta.Engine(rs=rs, theta=0.5) as eng:
+        eng.set_particles(qel, qion)
+        fdir = eng.direct_sample(tsp, tix)
+    with ta.Engine(rs=rs, theta=0.5, theta_mixed=0.5, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        rms_f, mx_f = err_full(eng, tsp, tix, fdir)
+        st_f = eng.count_interactions()
+        eng.set_particles(qel, qion)
+        sps_f, stt_f = _steps_per_s(eng, dt, a.steps)
+    th_lo, th_hi = a.band[0] * 0.5, min(1.0, a.band[1] * 0.5)
+    params, info = train_mlp(a, th_lo, th_hi)
+    TARGET = rms_f
+    r = run_learned(qel, qion, rs, dt, tsp, tix, fdir, argparse.Namespace(**{**vars(a), "err_target": 0.0}),
+                    params, th_lo, th_hi)
+    out = {"config": "C5: 2^23 e- + 2^23 ions, sigma 1 mm, fixed theta 0.5 vs learned", "steps": a.steps,
+           "fixed": {"steps_per_s": sps_f, "rms": rms_f, "max": mx_f, "ms_walk": stt_f["ms_walk"],
+                     "interactions": sum(st_f["m2p"]) + sum(st_f["p2p"])},
+           "learned": {"steps_per_s": 1.0 / r["s"], "rms": r["rms"], "max": r["max"], "ms_walk": r["ms_walk"],
+                       "err_target": r["err_target"], "interactions": sum(r["st"]["m2p"]) + sum(r["st"]["p2p"])},
+           "mlp": info}
+    log("C5: fixed %.3f steps/s rms %.2e | learned %.3f steps/s rms %.2e" % (sps_f, rms_f, 1 / r["s"], r["rms"]))
+    print(json.dumps(out))
+
+
+def run_build(a):
+    """The tree build alone at N = 4M: the FP64 walk's two species trees, and the
+    mixed walk's species sort + target groups + joint tree + cell records."""
+    n = 1 << 21
+    qel, qion = ta.gen_plasma(3, n, n, 600.0)
+    out = {"config": "build: 2^21 e- + 2^21 ions, sigma 600 um", "builds": a.steps}
+    for walk in ("ref64", "mixed"):
+        with ta.Engine(rs=ta.kelbg_length(10.0), theta=0.5, theta_mixed=0.152, walk=walk) as eng:
+            eng.set_particles(qel, qion)
+            eng.build()
+            ms = []
+            for _ in range(a.steps):
+                eng.build()
+                ms.append(eng.stats()["ms_build"])
+            out[walk] = {"ms_build": float(np.mean(ms)), "Mparticles_per_s": 2 * n / (np.mean(ms) * 1e-3) / 1e6}
+            log("build %s: %.3f ms" % (walk, np.mean(ms)))
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,11 +453,16 @@ def main():
                     help="fixed error threshold for the learned criterion (skip tuning)")
     ap.add_argument("--band", type=float, nargs=2, default=(0.7, 1.6), metavar=("LO", "HI"),
                     help="learned band: theta_lo = LO * theta, theta_hi = HI * theta (theta: the tuned fixed walk's)")
+    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "build"), default=None,
+                    help="measure another configuration instead of the headline (one JSON line)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the engine holds one whole domain per GPU and this build has no "
               "multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.config:
+        {"c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "build": run_build}[a.config](a)
+        return
 
     n = a.n
     rs = ta.kelbg_length(a.te)
