@@ -257,6 +257,14 @@ def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
     eng.mlp_set(params)
     eng.build()
     eng.force()  # the fixed-theta fallback: gives the first previous forces
+    if a.group == 0:
+        # the learned walk's own group size (its band decisions change the
+        # register budget), chosen before the error threshold is tuned
+        eng.configure(err_target=a.err_target if a.err_target > 0 else 1e-6)
+        eng.build()
+        eng.force()
+        g, ms = eng.autotune()
+        log("  learned walk: autotuned group size %d (ms per walk for 64/128/256: %s)" % (g, np.round(ms, 2)))
     if a.err_target > 0:
         et = a.err_target
         eng.configure(err_target=et)

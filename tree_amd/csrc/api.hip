@@ -670,6 +670,10 @@ int tb_build(tb_handle *h)
         return rc;
     }
     HIPCHK(hipEventRecord(h->ev[1], h->s));
+    // the build ends synchronised (its integrity read-back), so this is its latency
+    HIPCHK(hipEventSynchronize(h->ev[1]));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, h->ev[0], h->ev[1]) == hipSuccess) h->st.ms_build = ms;
     return TB_OK;
 }
 

@@ -60,18 +60,34 @@ __global__ __launch_bounds__(BT) void k_bbox_part(int n, const double *__restric
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(64) void k_bbox_final(int nb, const double *__restrict__ part, double *__restrict__ root)
+// min and max are exact and independent of order, so the block's strided
+// reduction gives the same box as a serial pass over the partials
+__global__ __launch_bounds__(BT) void k_bbox_final(int nb, const double *__restrict__ part, double *__restrict__ root)
 {
-    if (threadIdx.x != 0) return;
-    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int b = 0; b < nb; b++)
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = threadIdx.x; b < nb; b += BT)
         for (int k = 0; k < 3; k++) {
-            mn[k] = fmin(mn[k], part[b * 6 + k]);
-            mx[k] = fmax(mx[k], part[b * 6 + 3 + k]);
+            v[k] = fmin(v[k], part[b * 6 + k]);
+            v[3 + k] = fmax(v[3 + k], part[b * 6 + 3 + k]);
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 3; k++) {
+            v[k] = fmin(v[k], __shfl_xor(v[k], o));
+            v[3 + k] = fmax(v[3 + k], __shfl_xor(v[3 + k], o));
+        }
+    __shared__ double red[BT / 64][6];
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) red[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < BT / 64; w++)
+        for (int k = 0; k < 3; k++) {
+            v[k] = fmin(v[k], red[w][k]);
+            v[3 + k] = fmax(v[3 + k], red[w][3 + k]);
         }
     for (int k = 0; k < 3; k++) {
-        root[k] = (mx[k] + mn[k]) / 2.0;   // Roote%xx, tree.f90:58-60
-        root[3 + k] = mx[k] - mn[k];       // Roote%lx, tree.f90:61-63
+        root[k] = (v[3 + k] + v[k]) / 2.0;   // Roote%xx, tree.f90:58-60
+        root[3 + k] = v[3 + k] - v[k];       // Roote%lx, tree.f90:61-63
     }
 }
 
@@ -342,9 +358,10 @@ __global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ 
     if (threadIdx.x < 2 * (TB_LEVELS + 2)) (&lh[0][0])[threadIdx.x] = 0;
     __syncthreads();
     int i = blockIdx.x * BT + threadIdx.x;
+    int lm = TB_LEVELS + 2, leafd = 0;   // no depth counted for i >= n
     if (i < n) {
         uint64_t k = key[i];
-        int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
+        lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
         int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
         int mx = max(lm, lp);
         if (mx >= TB_LEVELS) {
@@ -352,10 +369,18 @@ __global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ 
             mx = TB_LEVELS - 1;
             lm = min(lm, mx);
         }
-        int leafd = 1 + mx;
+        leafd = 1 + mx;
         cnt[i] = leafd - lm;
-        for (int d = lm + 1; d < leafd; d++) atomicAdd(&lh[0][d], 1);
-        atomicAdd(&lh[1][leafd], 1);
+    }
+    // particle i opens the twigs at depths lm+1 .. leafd-1 and a leaf at
+    // depth leafd: counted per wavefront by ballots, one LDS add per depth
+    for (int d = 1; d < TB_LEVELS + 2; d++) {
+        const uint64_t tw = __ballot(lm < d && d < leafd);
+        const uint64_t lf = __ballot(leafd == d);
+        if ((threadIdx.x & 63) == 0) {
+            if (tw) atomicAdd(&lh[0][d], (int32_t)__popcll(tw));
+            if (lf) atomicAdd(&lh[1][d], (int32_t)__popcll(lf));
+        }
     }
     __syncthreads();
     if (threadIdx.x < 2 * (TB_LEVELS + 2)) {
@@ -591,21 +616,6 @@ __global__ __launch_bounds__(BT) void k_jkeys(int n, const double *__restrict__ 
 }
 
 // joint box from the two species' partial min/max (k_bbox_part output)
-__global__ __launch_bounds__(64) void k_jbox(int nb, const double *__restrict__ part, double *__restrict__ root)
-{
-    if (threadIdx.x != 0) return;
-    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int b = 0; b < nb; b++)
-        for (int k = 0; k < 3; k++) {
-            mn[k] = fmin(mn[k], part[b * 6 + k]);
-            mx[k] = fmax(mx[k], part[b * 6 + 3 + k]);
-        }
-    for (int k = 0; k < 3; k++) {
-        root[k] = (mx[k] + mn[k]) / 2.0;
-        root[3 + k] = mx[k] - mn[k];
-    }
-}
-
 struct JSrc {
     const double *x[2], *y[2], *z[2];
 };
@@ -867,7 +877,7 @@ void tb_bbox(hipStream_t s, int n, const double *x, const double *y, const doubl
 {
     int nb = std::min(nblk(n, BT), 1024);
     hipLaunchKernelGGL(k_bbox_part, dim3(nb), dim3(BT), 0, s, n, x, y, z, tmp);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, nb, tmp, root);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BT), 0, s, nb, tmp, root);
 }
 
 void tb_keys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
@@ -955,7 +965,7 @@ void tb_jbbox(hipStream_t s, int ne, const double *ex, const double *ey, const d
     int nbe = std::min(nblk(ne, BT), 1024), nbi = std::min(nblk(ni, BT), 1024);
     hipLaunchKernelGGL(k_bbox_part, dim3(nbe), dim3(BT), 0, s, ne, ex, ey, ez, tmp);
     hipLaunchKernelGGL(k_bbox_part, dim3(nbi), dim3(BT), 0, s, ni, ix, iy, iz, tmp + 6 * nbe);
-    hipLaunchKernelGGL(k_jbox, dim3(1), dim3(64), 0, s, nbe + nbi, tmp, root);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BT), 0, s, nbe + nbi, tmp, root);
 }
 
 void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
