@@ -46,6 +46,9 @@ W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # (monopole + dipole + traceless-quadrupole M2P 72, Coulomb P2P 18, Kelbg P2P
 # 30); MAC tests are per cell and group and are not counted
 W32_M2P, W32_P2P_C, W32_P2P_K = 72, 18, 30
+# far cells: flops per (cell, target group) pair of the order-4 local
+# expansion (FMA = 2), counted from the compiled code of m2l_acc_p4h
+W32_M2L_PAIR = 1003
 # TFLOP/s, MI355X published vector peaks.  The FP32 figure counts packed FP32
 # (two lanes per instruction); this build keeps packed FP32 off, so one
 # instruction per lane per cycle (78.6) is its own ceiling, stated beside it.
@@ -117,7 +120,13 @@ def walk_flops32(st):
     for p in range(4):
         wp = W32_P2P_C if p in (0, 3) else W32_P2P_K
         fl += W32_M2P * st["m2p"][p] + wp * st["p2p"][p]
-    return fl
+    return fl + W32_M2L_PAIR * sum(st.get("far_pairs", [0, 0]))
+
+
+def interactions(st):
+    """(target, cell or particle) interactions of a force: M2P, far cells taken
+    through the group's local expansion (per target), and P2P."""
+    return sum(st["m2p"]) + sum(st.get("m2l", [0])) + sum(st["p2p"])
 
 
 def timed_steps(eng, dt, steps, warmup):
@@ -317,7 +326,7 @@ def run_c2(a):
             sps, stt = _steps_per_s(eng, dt, a.steps)
             out[walk] = {"steps_per_s": sps, "rms": rms, "max": mx, "ms_build": stt["ms_build"],
                          "ms_walk": stt["ms_walk"],
-                         "interactions_per_s": (sum(st["m2p"]) + sum(st["p2p"])) * sps}
+                         "interactions_per_s": interactions(st) * sps}
             log("C2 %s: %.2f steps/s rms %.2e" % (walk, sps, rms))
     print(json.dumps(out))
 
@@ -415,9 +424,9 @@ def run_c5(a):
                     params, th_lo, th_hi)
     out = {"config": "C5: 2^23 e- + 2^23 ions, sigma 1 mm, fixed theta 0.5 vs learned", "steps": a.steps,
            "fixed": {"steps_per_s": sps_f, "rms": rms_f, "max": mx_f, "ms_walk": stt_f["ms_walk"],
-                     "interactions": sum(st_f["m2p"]) + sum(st_f["p2p"])},
+                     "interactions": interactions(st_f)},
            "learned": {"steps_per_s": 1.0 / r["s"], "rms": r["rms"], "max": r["max"], "ms_walk": r["ms_walk"],
-                       "err_target": r["err_target"], "interactions": sum(r["st"]["m2p"]) + sum(r["st"]["p2p"])},
+                       "err_target": r["err_target"], "interactions": interactions(r["st"])},
            "mlp": info}
     log("C5: fixed %.3f steps/s rms %.2e | learned %.3f steps/s rms %.2e" % (sps_f, rms_f, 1 / r["s"], r["rms"]))
     print(json.dumps(out))
@@ -506,7 +515,7 @@ def main():
             if k in v:
                 out[k] = v[k]
         if "st" in v:
-            out["interactions_per_s"] = (sum(v["st"]["m2p"]) + sum(v["st"]["p2p"])) / v["s"]
+            out["interactions_per_s"] = interactions(v["st"]) / v["s"]
         return out
 
     res = {

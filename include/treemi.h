@@ -71,6 +71,11 @@ typedef struct tb_config {
     int32_t group;     /* TB_WALK_MIXED targets per group: 0 = autotuned, else 64, 128 or 256 */
     int32_t device;    /* HIP device ordinal                                       */
     int64_t mem_cap;   /* device bytes the handle may hold (0 = unlimited)         */
+    double far_ratio;  /* TB_WALK_MIXED: an accepted cell farther than far_ratio    */
+                       /* times the target group's radius from the group's centre */
+                       /* goes through the group's local expansion instead of     */
+                       /* being evaluated at every target (0: default 6, < 0:   */
+                       /* off, else >= 2)                                        */
 } tb_config;
 
 typedef struct tb_handle tb_handle;
@@ -115,6 +120,17 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
 int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
 
 /*
+ * Check of the mixed walk's far-field local expansion (no reference
+ * counterpart; tests only).  cells [ncell][12]: R = expansion centre - cell
+ * centre (3), q, D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx (traceless quadrupole).
+ * The cells' local coefficients of order `order` (3 or 4) are summed across a
+ * wavefront as in the walk and evaluated at the offsets y [ny][3]:
+ * out [ny][4] = potential, field x, y, z (FP32).
+ */
+int tb_local_eval(tb_handle *h, int32_t order, int32_t ncell, const float *cells, int32_t ny, const float *y,
+                  float *out);
+
+/*
  * The FP64 walk (TB_WALK_REF64 trees) for a list of targets only (reference
  * order): f [nt][3].  Same forces as tb_force gives those targets.
  */
@@ -133,6 +149,10 @@ typedef struct tb_stats {
     double ms_build, ms_force, ms_vv;  /* last call, device time               */
     double ms_walk;          /* last force: the walk kernel alone (mixed walk;  */
                              /* the FP64 walk reports its force phase)          */
+    int64_t m2l[4];          /* mixed walk: far cells taken through the target  */
+                             /* group's local expansion (counted per target)    */
+    int64_t far_pairs[2];    /* the same as (far cell, target group) pairs, by  */
+                             /* target species                                  */
 } tb_stats;
 /* counting pass: re-runs the last force with interaction counters */
 int tb_count_interactions(tb_handle *h, tb_stats *s);

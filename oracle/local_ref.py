@@ -1,0 +1,152 @@
+"""FP64 reference of the far-field local expansion used by the mixed walk.
+
+TEST INFRASTRUCTURE ONLY: the checker for the walk's far-cell path
+(tree_amd/csrc/local_exp.h).  The reference code has no local expansion;
+this is the project's own method, so the oracle is pinned by mathematics
+(tests/test_local_oracle.py): closed-form derivatives of 1/r, finite
+differences, harmonicity, the Taylor series of 1/|R + y|, direct sums over
+point charges, and the walk's own monopole + dipole + traceless-quadrupole
+cell field (m2p below, the expression of the walk's M2P loop) as the limit
+for targets close to the expansion centre.
+
+Notation.  For a multi-index a = (a1, a2, a3), |a| = a1 + a2 + a3 and
+a! = a1! a2! a3!.  T_a(R) = (1/a!) d^a (1/|R|) / dR^a, so that
+1/|R + y| = sum_a T_a(R) y^a.  For |a| = n >= 1 (all terms with a negative
+index dropped):
+
+    n |R|^2 T_a = -(2n - 1) sum_i R_i T_{a - e_i} - (n - 1) sum_i T_{a - 2 e_i}
+
+A cell with centre X and moments q (net charge), D (dipole) and traceless
+quadrupole Q' = 3 M - tr(M) I (M the second moment about X) produces at a
+point g + y, with R = g - X:
+
+    phi(y) = sum_b L_b y^b,
+    L_b = sum_c C(b + c, b) (-1)^|c| T_{b+c}(R) M_c,   |c| <= 2,
+
+with M_0 = q, M_{e_i} = D_i, M_c = Q'_c / 3 for |c| = 2 (the trace of M
+drops out because 1/r is harmonic).  The field is E = -grad phi.  Keeping
+|b| <= P gives the field through order P - 1 in y.
+"""
+from math import comb, factorial
+
+import numpy as np
+
+
+def multi_indices(n):
+    """All (a1, a2, a3) with a1 + a2 + a3 = n, a1 descending, then a2."""
+    return [(a, b, n - a - b) for a in range(n, -1, -1) for b in range(n - a, -1, -1)]
+
+
+def all_indices(nmax):
+    out = []
+    for n in range(nmax + 1):
+        out += multi_indices(n)
+    return out
+
+
+def taylor_coeffs(R, nmax):
+    """T_a(R) for |a| <= nmax by the recurrence above (dict keyed by a)."""
+    R = np.asarray(R, dtype=np.float64)
+    r2 = float(R @ R)
+    T = {(0, 0, 0): 1.0 / np.sqrt(r2)}
+    for n in range(1, nmax + 1):
+        for a in multi_indices(n):
+            s1 = 0.0
+            s2 = 0.0
+            for i in range(3):
+                if a[i] >= 1:
+                    b = list(a)
+                    b[i] -= 1
+                    s1 += R[i] * T[tuple(b)]
+                if a[i] >= 2:
+                    b = list(a)
+                    b[i] -= 2
+                    s2 += T[tuple(b)]
+            T[a] = -((2 * n - 1) * s1 + (n - 1) * s2) / (n * r2)
+    return T
+
+
+def _c(b, c):
+    return comb(b[0] + c[0], b[0]) * comb(b[1] + c[1], b[1]) * comb(b[2] + c[2], b[2])
+
+
+def cell_moments(q, D, Qp):
+    """M_c for |c| <= 2 from q, D[3] and the traceless Q' (3x3, symmetric)."""
+    Qp = np.asarray(Qp, dtype=np.float64)
+    M = {(0, 0, 0): float(q)}
+    for i in range(3):
+        e = [0, 0, 0]
+        e[i] = 1
+        M[tuple(e)] = float(D[i])
+    for c in multi_indices(2):
+        idx = [i for i in range(3) for _ in range(c[i])]
+        M[c] = Qp[idx[0], idx[1]] / 3.0
+    return M
+
+
+def m2l(X, q, D, Qp, g, P):
+    """Local coefficients L_b (|b| <= P) about g of a cell at X with moments q, D, Q'."""
+    R = np.asarray(g, dtype=np.float64) - np.asarray(X, dtype=np.float64)
+    T = taylor_coeffs(R, P + 2)
+    M = cell_moments(q, D, Qp)
+    L = {}
+    for b in all_indices(P):
+        s = 0.0
+        for c, mc in M.items():
+            a = (b[0] + c[0], b[1] + c[1], b[2] + c[2])
+            s += _c(b, c) * (-1.0) ** sum(c) * T[a] * mc
+        L[b] = s
+    return L
+
+
+def l2p(L, y):
+    """Potential and field at offset y from the expansion centre."""
+    y = np.asarray(y, dtype=np.float64)
+    phi = 0.0
+    E = np.zeros(3)
+    for b, lb in L.items():
+        phi += lb * y[0] ** b[0] * y[1] ** b[1] * y[2] ** b[2]
+        for i in range(3):
+            if b[i] == 0:
+                continue
+            e = list(b)
+            e[i] -= 1
+            E[i] -= lb * b[i] * y[0] ** e[0] * y[1] ** e[1] * y[2] ** e[2]
+    return phi, E
+
+
+def m2p(X, q, D, Qp, x):
+    """The walk's cell expansion evaluated directly at x (FP64):
+    phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5), E = -grad phi, r = x - X."""
+    r = np.asarray(x, dtype=np.float64) - np.asarray(X, dtype=np.float64)
+    Qp = np.asarray(Qp, dtype=np.float64)
+    r2 = float(r @ r)
+    ri = 1.0 / np.sqrt(r2)
+    ri3, ri5, ri7 = ri ** 3, ri ** 5, ri ** 7
+    dp = float(np.dot(D, r))
+    Qr = Qp @ r
+    rQr = float(r @ Qr)
+    phi = q * ri + dp * ri3 + 0.5 * rQr * ri5
+    E = r * (q * ri3 + 3.0 * dp * ri5 + 2.5 * rQr * ri7) - np.asarray(D) * ri3 - Qr * ri5
+    return phi, E
+
+
+def inv_r_derivative_fd(R, a, h=1e-3):
+    """(1/a!) d^a (1/|R|) by nested central differences (independent check)."""
+    R = np.asarray(R, dtype=np.float64)
+
+    def f(x):
+        return 1.0 / np.sqrt(x @ x)
+
+    def d(fun, i):
+        def g(x):
+            e = np.zeros(3)
+            e[i] = h
+            return (fun(x + e) - fun(x - e)) / (2 * h)
+        return g
+
+    fun = f
+    for i in range(3):
+        for _ in range(a[i]):
+            fun = d(fun, i)
+    return fun(R) / (factorial(a[0]) * factorial(a[1]) * factorial(a[2]))

@@ -1,0 +1,104 @@
+"""The mixed walk's far-field local expansion (tree_amd/csrc/local_exp.h) on
+the GPU against the FP64 oracle (oracle/local_ref.py), and the walk with its
+far-cell path against the walk without it."""
+import numpy as np
+import pytest
+
+import tree_amd as ta
+from oracle import local_ref as lr
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+RS10 = 6.6493e-3
+
+
+def _cells(rng, n, dmin, dmax):
+    u = rng.normal(size=(n, 3))
+    R = u / np.linalg.norm(u, axis=1)[:, None] * rng.uniform(dmin, dmax, size=n)[:, None]
+    q = rng.normal(size=n)
+    D = rng.normal(size=(n, 3)) * 0.3
+    A = rng.normal(size=(n, 3, 3)) * 0.1
+    Qp = A + A.transpose(0, 2, 1)
+    Qp -= np.trace(Qp, axis1=1, axis2=2)[:, None, None] / 3 * np.eye(3)
+    cells = np.c_[R, q, D, Qp[:, 0, 0], Qp[:, 1, 1], Qp[:, 0, 1], Qp[:, 1, 2], Qp[:, 2, 0]]
+    return cells, R, q, D, Qp
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_local_expansion_vs_oracle(order):
+    """FP32 coefficients summed over 150 cells (a partial last pass of the
+    wavefront) and evaluated at 70 offsets: within 2e-5 of the FP64 oracle,
+    relative to the field's scale."""
+    announce("local_eval order %d" % order)
+    rng = np.random.default_rng(order)
+    cells, R, q, D, Qp = _cells(rng, 150, 5.0, 20.0)
+    y = rng.uniform(-1.0, 1.0, size=(70, 3))
+    with ta.Engine(rs=RS10) as eng:
+        out = eng.local_eval(order, cells, y)
+    L = {}
+    for k in range(len(cells)):
+        Lk = lr.m2l(-R[k], q[k], D[k], Qp[k], np.zeros(3), order)
+        for b, v in Lk.items():
+            L[b] = L.get(b, 0.0) + v
+    ref = np.array([[*lr.l2p(L, yy)[:1], *lr.l2p(L, yy)[1]] for yy in y])
+    escale = np.abs(ref[:, 1:]).max()
+    pscale = np.abs(ref[:, 0]).max()
+    assert np.abs(out[:, 1:] - ref[:, 1:]).max() <= 2e-5 * escale
+    assert np.abs(out[:, 0] - ref[:, 0]).max() <= 2e-5 * pscale
+    # and against the cells' own fields (M2P, FP64): the local truncation, ~(|y|/R)^order
+    direct = np.zeros((len(y), 4))
+    for k in range(len(cells)):
+        for j, yy in enumerate(y):
+            ph, E = lr.m2p(-R[k], q[k], D[k], Qp[k], yy)
+            direct[j, 0] += ph
+            direct[j, 1:] += E
+    rho = np.sqrt(3.0) / 5.0
+    assert np.abs(out[:, 1:] - direct[:, 1:]).max() <= 30 * rho ** order * np.abs(direct[:, 1:]).max()
+
+
+def _forces(eng, ne, ni):
+    e, i = eng.get_particles()
+    return np.r_[e["ff"], i["ff"]]
+
+
+def test_walk_far_cells_through_local_expansion():
+    """The same accepted cells with and without the far path: the M2P cells
+    with it plus its far cells equal the M2P cells without it (the route
+    changes, the acceptance does not); forces differ by the local expansion's
+    truncation, below the walk's own error; both stay close to the FP64
+    direct sum; the total potential energy agrees to 1e-5 relative."""
+    announce("walk_far_path")
+    n = 1 << 16
+    qel, qion = ta.gen_plasma(31, n, n, 300.0)
+    rng = np.random.default_rng(4)
+    tsp = np.r_[np.zeros(512, np.int32), np.ones(512, np.int32)]
+    tix = np.r_[rng.choice(n, 512, replace=False), rng.choice(n, 512, replace=False)].astype(np.int32)
+    out = {}
+    for far in (-1.0, 8.0, 4.0):
+        with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.3, walk="mixed", far_ratio=far) as eng:
+            eng.set_particles(qel, qion)
+            if far < 0:
+                fd = eng.direct_sample(tsp, tix)
+            eng.build()
+            ep = eng.force()
+            f = _forces(eng, n, n)
+            st = eng.count_interactions()
+            out[far] = (f, ep, st)
+    f0, ep0, st0 = out[-1.0]
+    assert sum(st0["m2l"]) == 0
+    for far in (8.0, 4.0):
+        f1, ep1, st1 = out[far]
+        assert sum(st1["m2l"]) > 0
+        assert sum(st1["m2p"]) + sum(st1["m2l"]) == sum(st0["m2p"]), (far, st0, st1)
+        assert st1["p2p"] == st0["p2p"]
+        d = np.linalg.norm(f1 - f0, axis=1) / np.linalg.norm(f0, axis=1)
+        bound = 2e-3 if far == 8.0 else 1e-2
+        assert np.sqrt(np.mean(d ** 2)) < bound, (far, np.sqrt(np.mean(d ** 2)))
+        assert abs(sum(ep1) - sum(ep0)) <= 1e-5 * abs(ep0[0]) + 1e-5 * abs(ep0[1]), (ep0, ep1)
+    idx = np.where(tsp == 0, tix, n + tix)
+    for far in (-1.0, 8.0):
+        f = out[far][0][idx]
+        rel = np.linalg.norm(f - fd, axis=1) / np.linalg.norm(fd, axis=1)
+        out[far] = out[far] + (np.sqrt(np.mean(rel ** 2)),)
+    assert out[8.0][3] < 1.3 * out[-1.0][3], (out[-1.0][3], out[8.0][3])

@@ -1,0 +1,151 @@
+"""The far-field local expansion oracle (oracle/local_ref.py), pinned by
+closed forms, finite differences, harmonicity, the Taylor series of 1/|R + y|
+and direct sums over point charges."""
+import numpy as np
+import pytest
+
+from oracle import local_ref as lr
+
+R0 = np.array([0.7, -1.3, 0.9])
+
+
+def test_taylor_closed_forms():
+    T = lr.taylor_coeffs(R0, 3)
+    x, y, z = R0
+    r = np.sqrt(R0 @ R0)
+    assert T[(0, 0, 0)] == pytest.approx(1 / r, rel=1e-15)
+    assert T[(1, 0, 0)] == pytest.approx(-x / r ** 3, rel=1e-14)
+    assert T[(0, 0, 1)] == pytest.approx(-z / r ** 3, rel=1e-14)
+    assert T[(2, 0, 0)] == pytest.approx((3 * x * x - r * r) / (2 * r ** 5), rel=1e-13)
+    assert T[(0, 1, 1)] == pytest.approx(3 * y * z / r ** 5, rel=1e-13)
+    assert T[(3, 0, 0)] == pytest.approx((-15 * x ** 3 / r ** 7 + 9 * x / r ** 5) / 6, rel=1e-12)
+    assert T[(2, 1, 0)] == pytest.approx((-15 * x * x * y / r ** 7 + 3 * y / r ** 5) / 2, rel=1e-12)
+    assert T[(1, 1, 1)] == pytest.approx(-15 * x * y * z / r ** 7, rel=1e-12)
+
+
+@pytest.mark.parametrize("a", [(4, 0, 0), (1, 2, 1), (0, 3, 1), (2, 2, 0)])
+def test_taylor_vs_finite_differences(a):
+    T = lr.taylor_coeffs(R0, 4)
+    fd = lr.inv_r_derivative_fd(R0, a, h=2e-3)
+    assert T[a] == pytest.approx(fd, rel=2e-4, abs=1e-6)
+
+
+def test_taylor_harmonic():
+    # sum_i (a_i + 2)(a_i + 1) T_{a + 2 e_i} = 0: 1/r is harmonic
+    T = lr.taylor_coeffs(R0, 6)
+    for a in lr.all_indices(4):
+        s = 0.0
+        scale = 0.0
+        for i in range(3):
+            b = list(a)
+            b[i] += 2
+            v = (a[i] + 2) * (a[i] + 1) * T[tuple(b)]
+            s += v
+            scale += abs(v)
+        assert abs(s) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("N", [3, 4, 6])
+def test_taylor_series_converges(N):
+    rng = np.random.default_rng(N)
+    T = lr.taylor_coeffs(R0, N)
+    r = np.sqrt(R0 @ R0)
+    errs = []
+    for rho in (0.1, 0.05):
+        e = 0.0
+        for _ in range(20):
+            u = rng.normal(size=3)
+            yv = rho * r * u / np.linalg.norm(u)
+            s = sum(t * yv[0] ** a[0] * yv[1] ** a[1] * yv[2] ** a[2] for a, t in T.items())
+            e = max(e, abs(s * np.linalg.norm(R0 + yv) - 1.0))
+        errs.append(e)
+    assert errs[0] < 3 * 0.1 ** (N + 1)
+    # order of convergence: halving rho divides the error by about 2^(N+1)
+    assert errs[0] / errs[1] > 0.5 * 2 ** (N + 1)
+
+
+def _cell(rng, n=40, b=0.3):
+    s = rng.uniform(-b, b, size=(n, 3))
+    q = rng.choice([-1.0, 1.0], size=n)
+    X = np.array([0.2, 0.1, -0.4])
+    M = np.einsum("j,ji,jk->ik", q, s, s)
+    Qp = 3 * M - np.trace(M) * np.eye(3)
+    return X, X + s, q, q.sum(), q @ s, Qp
+
+
+@pytest.mark.parametrize("P", [3, 4])
+def test_local_expansion_converges_to_cell_field(P):
+    rng = np.random.default_rng(10 + P)
+    X, _, _, q, D, Qp = _cell(rng)
+    g = X + np.array([6.0, -3.0, 4.0])
+    L = lr.m2l(X, q, D, Qp, g, P)
+    d = np.linalg.norm(g - X)
+    errs = []
+    for rho in (0.08, 0.04):
+        ep = ee = 0.0
+        for _ in range(20):
+            u = rng.normal(size=3)
+            yv = rho * d * u / np.linalg.norm(u)
+            phi, E = lr.l2p(L, yv)
+            phi0, E0 = lr.m2p(X, q, D, Qp, g + yv)
+            ep = max(ep, abs(phi - phi0) / abs(phi0))
+            ee = max(ee, np.linalg.norm(E - E0) / np.linalg.norm(E0))
+        errs.append((ep, ee))
+    # field through order P-1 in y: error ~ rho^P; potential ~ rho^(P+1)
+    assert errs[0][1] < 60 * 0.08 ** P
+    assert errs[0][1] / errs[1][1] > 0.4 * 2 ** P
+    assert errs[0][0] / errs[1][0] > 0.4 * 2 ** (P + 1)
+
+
+def test_local_expansion_vs_point_charges():
+    # cell truncation (b/d)^3 and local truncation (|y|/d)^4 against a direct sum
+    rng = np.random.default_rng(3)
+    X, xs, qs, q, D, Qp = _cell(rng, n=60, b=0.25)
+    g = X + np.array([-8.0, 5.0, 3.0])
+    L = lr.m2l(X, q, D, Qp, g, 4)
+    d = np.linalg.norm(g - X)
+    for _ in range(10):
+        u = rng.normal(size=3)
+        yv = 0.1 * d * u / np.linalg.norm(u)
+        x = g + yv
+        dr = x - xs
+        r = np.linalg.norm(dr, axis=1)
+        E_dir = (qs[:, None] * dr / r[:, None] ** 3).sum(0)
+        phi_dir = (qs / r).sum()
+        phi, E = lr.l2p(L, yv)
+        scale = np.abs(qs).sum() / d ** 2
+        assert np.linalg.norm(E - E_dir) < 20 * scale * ((0.25 * np.sqrt(3) / d) ** 3 + 0.1 ** 4)
+        assert abs(phi - phi_dir) < 20 * scale * d * ((0.25 * np.sqrt(3) / d) ** 3 + 0.1 ** 5)
+
+
+def test_moment_trace_drops_out():
+    # adding c I to the second moment (not traceless) leaves L unchanged
+    rng = np.random.default_rng(5)
+    X, _, _, q, D, Qp = _cell(rng)
+    g = X + np.array([3.0, 4.0, -5.0])
+    L1 = lr.m2l(X, q, D, Qp, g, 4)
+    T = lr.taylor_coeffs(g - X, 6)
+    M = lr.cell_moments(q, D, Qp)
+    for b in lr.all_indices(4):
+        extra = 0.0
+        for c in lr.multi_indices(2):
+            if max(c) == 2:  # diagonal: add 1.7 to M_ii
+                a = (b[0] + c[0], b[1] + c[1], b[2] + c[2])
+                extra += lr._c(b, c) * T[a] * 1.7
+        assert abs(extra) <= 1e-12 * max(1.0, abs(L1[b]))
+
+
+def test_local_coefficients_harmonic():
+    # each homogeneous part of the local expansion is harmonic:
+    # sum_i (b_i + 2)(b_i + 1) L_{b + 2 e_i} = 0 for |b| <= P - 2.  The walk
+    # forms only the coefficients with b_z <= 1 and recovers the others from this.
+    rng = np.random.default_rng(8)
+    X, _, _, q, D, Qp = _cell(rng)
+    L = lr.m2l(X, q, D, Qp, X + np.array([4.0, -2.0, 3.0]), 4)
+    for b in lr.all_indices(2):
+        terms = []
+        for i in range(3):
+            c = list(b)
+            c[i] += 2
+            terms.append((b[i] + 2) * (b[i] + 1) * L[tuple(c)])
+        assert abs(sum(terms)) <= 1e-12 * sum(abs(t) for t in terms)

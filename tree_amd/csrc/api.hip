@@ -56,7 +56,7 @@ struct tb_handle {
     size_t epart_n = 0;
     double *d_sums = nullptr;      // 16 doubles
     double *h_sums = nullptr;      // pinned
-    int64_t *d_cnt = nullptr;      // 12 counters
+    int64_t *d_cnt = nullptr;      // 16 counters
     size_t bytes = 0;              // device bytes held
     bool built = false, have_force = false;
     int tree_walk = -1;            // which walk the current trees were built for
@@ -272,6 +272,8 @@ int check_cfg(const tb_config *c)
     if (c->mac != TB_MAC_FIXED && c->mac != TB_MAC_LEARNED) return fail(TB_ERR_ARG, "unknown mac %d", c->mac);
     if (c->group != 0 && c->group != 64 && c->group != 128 && c->group != 256)
         return fail(TB_ERR_ARG, "group must be 0, 64, 128 or 256 (got %d)", c->group);
+    if (!std::isfinite(c->far_ratio) || (c->far_ratio > 0.0 && c->far_ratio < 2.0))
+        return fail(TB_ERR_ARG, "far_ratio must be 0 (default), negative (off) or >= 2 (got %g)", c->far_ratio);
     return TB_OK;
 }
 
@@ -465,7 +467,7 @@ int force_all(tb_handle *h, bool counting)
         int rc = tb_mixed_force(s, h->mx, h->sp, h->cfg, h->epart, h->d_sums);
         if (rc) return fail(rc, "%s", tb_mixed_error());
     } else {
-        if (counting) HIPCHK(hipMemsetAsync(h->d_cnt, 0, 12 * sizeof(int64_t), s));
+        if (counting) HIPCHK(hipMemsetAsync(h->d_cnt, 0, 16 * sizeof(int64_t), s));
         const int nb0 = tb_walk64_blocks(h->sp[0].n);
         for (int tsp = 0; tsp < 2; tsp++) {
             double *part = h->epart + (tsp == 0 ? 0 : 2 * nb0);
@@ -545,7 +547,7 @@ int tb_create(tb_handle **out, const tb_config *cfg)
         return fail(TB_ERR_HIP, "stream creation failed");
     }
     bool ok = dalloc_t(h, &h->bbox_tmp, 2048 * 6) == TB_OK && dalloc_t(h, &h->d_misc, 256) == TB_OK &&
-              dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, 12) == TB_OK;
+              dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, 16) == TB_OK;
     ok = ok && hipHostMalloc((void **)&h->h_misc, 256 * sizeof(int32_t), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&h->h_sums, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     for (int k = 0; k < 6 && ok; k++) ok = hipEventCreate(&h->ev[k]) == hipSuccess;
@@ -864,6 +866,29 @@ int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const in
     return TB_OK;
 }
 
+int tb_local_eval(tb_handle *h, int32_t order, int32_t ncell, const float *cells, int32_t ny, const float *y,
+                  float *out)
+{
+    if (!h || !cells || !y || !out) return fail(TB_ERR_ARG, "null argument");
+    if (order != 3 && order != 4) return fail(TB_ERR_ARG, "order must be 3 or 4 (got %d)", order);
+    if (ncell < 1 || ny < 1 || ncell > (1 << 20) || ny > (1 << 20))
+        return fail(TB_ERR_ARG, "ncell and ny must be in 1 .. 2^20");
+    HIPCHK(hipSetDevice(h->cfg.device));
+    const size_t nb = (size_t)ncell * 48 + (size_t)ny * 12 + (size_t)ny * 16 + 64;
+    void *tmp = nullptr;
+    int rc = dalloc(h, &tmp, nb);
+    if (rc) return rc;
+    float *d_c = (float *)tmp, *d_y = d_c + (size_t)ncell * 12, *d_o = d_y + (size_t)ny * 3;
+    hipError_t e = hipMemcpy(d_c, cells, (size_t)ncell * 48, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_y, y, (size_t)ny * 12, hipMemcpyHostToDevice);
+    if (e == hipSuccess && tb_local_run(h->s, order, ncell, d_c, ny, d_y, d_o) != TB_OK) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    if (e == hipSuccess) e = hipMemcpy(out, d_o, (size_t)ny * 16, hipMemcpyDeviceToHost);
+    dfree(h, tmp, nb);
+    if (e != hipSuccess) return fail(TB_ERR_HIP, "tb_local_eval: %s", hipGetErrorString(e));
+    return TB_OK;
+}
+
 // reference index -> key-order index for sampled targets
 static int map_targets(tb_handle *h, int nt, const int32_t *tspecies, const int32_t *tidx, std::vector<int32_t> &sidx)
 {
@@ -936,7 +961,7 @@ int tb_count_interactions(tb_handle *h, tb_stats *st)
     if (!h || !st) return fail(TB_ERR_ARG, "null argument");
     if (!h->built) return fail(TB_ERR_STATE, "tb_count_interactions before tb_build");
     HIPCHK(hipSetDevice(h->cfg.device));
-    int64_t c[12];
+    int64_t c[16] = {};
     if (h->cfg.walk == TB_WALK_MIXED) {
         int rc = tb_mixed_count(h->s, h->mx, h->sp, h->cfg, h->epart, h->d_sums, h->d_cnt);
         if (rc) return fail(rc, "%s", tb_mixed_error());
@@ -953,6 +978,12 @@ int tb_count_interactions(tb_handle *h, tb_stats *st)
         h->st.m2p[p] = c[map[p] + 1];
         h->st.p2p[p] = c[map[p] + 2];
     }
+    // far cells of the mixed walk's local expansions, by target species (ee, ii)
+    h->st.m2l[0] = c[12];
+    h->st.m2l[1] = h->st.m2l[2] = 0;
+    h->st.m2l[3] = c[13];
+    h->st.far_pairs[0] = c[14];
+    h->st.far_pairs[1] = c[15];
     *st = h->st;
     return TB_OK;
 }

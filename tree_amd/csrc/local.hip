@@ -1,0 +1,68 @@
+// local.hip -- a direct check of the far-field local expansion
+// (local_exp.h) outside the walk: the local coefficients of a set of cells
+// about one centre, summed across a wavefront the way the walk sums them,
+// then evaluated at given offsets.  Order 3 keeps every coefficient; order 4
+// (the walk's) keeps the independent ones of the harmonic expansion.
+#include <hip/hip_runtime.h>
+
+#include "local_exp.h"
+#include "tb_internal.h"
+
+namespace {
+
+template <int N>
+__device__ inline void wave_sum_all(float (&L)[N])
+{
+#pragma unroll
+    for (int k = 0; k < N; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) L[k] += __shfl_xor(L[k], o, 64);
+}
+
+// cells: 12 floats each, R = centre - cell centre (3), q, D (3), Q'xx,
+// Q'yy, Q'xy, Q'yz, Q'zx.  out: phi, Ex, Ey, Ez per offset.
+template <int P>
+__global__ __launch_bounds__(64) void k_local_eval(int ncell, const float *__restrict__ cells, int ny,
+                                                   const float *__restrict__ y, float *__restrict__ out)
+{
+    constexpr int N = P == 3 ? TB_LOCAL_N3 : TB_LOCAL_N4H;
+    float L[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) L[k] = 0.0f;
+    for (int c = threadIdx.x; c < ncell; c += 64) {
+        const float *e = cells + 12 * c;
+        if constexpr (P == 3)
+            m2l_acc_p3(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11],
+                       reinterpret_cast<float(&)[TB_LOCAL_N3]>(L));
+        else
+            m2l_acc_p4h(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11],
+                        reinterpret_cast<float(&)[TB_LOCAL_N4H]>(L));
+    }
+    wave_sum_all(L);
+    for (int j = threadIdx.x; j < ny; j += 64) {
+        float phi, ex, ey, ez;
+        if constexpr (P == 3)
+            l2p_p3(reinterpret_cast<const float(&)[TB_LOCAL_N3]>(L), y[3 * j], y[3 * j + 1], y[3 * j + 2], phi, ex,
+                   ey, ez);
+        else
+            l2p_p4h(reinterpret_cast<const float(&)[TB_LOCAL_N4H]>(L), y[3 * j], y[3 * j + 1], y[3 * j + 2], phi,
+                    ex, ey, ez);
+        out[4 * j] = phi;
+        out[4 * j + 1] = ex;
+        out[4 * j + 2] = ey;
+        out[4 * j + 3] = ez;
+    }
+}
+
+}  // namespace
+
+int tb_local_run(hipStream_t s, int order, int ncell, const float *cells, int ny, const float *y, float *out)
+{
+    if (order == 3)
+        hipLaunchKernelGGL(k_local_eval<3>, dim3(1), dim3(64), 0, s, ncell, cells, ny, y, out);
+    else if (order == 4)
+        hipLaunchKernelGGL(k_local_eval<4>, dim3(1), dim3(64), 0, s, ncell, cells, ny, y, out);
+    else
+        return TB_ERR_ARG;
+    return hipGetLastError() == hipSuccess ? TB_OK : TB_ERR_HIP;
+}

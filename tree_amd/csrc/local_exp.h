@@ -1,0 +1,842 @@
+// local_exp.h -- written by gen_local.py; do not edit by hand.
+//
+// Far-field local expansion of the mixed walk: Taylor coefficients of
+// 1/|R + y| by their three-term recurrence, contracted with a cell's
+// net charge, dipole and traceless quadrupole into the local
+// coefficients L_b of the potential about a target group's centre
+// (|b| <= P, multi-indices in the order of all_indices), and the
+// evaluation of potential and field at a target.
+#pragma once
+#include <hip/hip_runtime.h>
+
+constexpr int TB_LOCAL_N3 = 20;   // order 3, all coefficients
+constexpr int TB_LOCAL_N4H = 25;  // order 4, independent coefficients only
+
+// adds one cell's local coefficients about the expansion centre to L[20]
+// (order P = 3; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)
+__device__ inline void m2l_acc_p3(float rx, float ry, float rz, float q, float dx, float dy, float dz,
+                                  float qxx, float qyy, float qxy, float qyz, float qzx, float (&L)[20])
+{
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float mdx = -dx, mdy = -dy, mdz = -dz;
+    const float mxx = qxx * (1.0f / 3.0f), myy = qyy * (1.0f / 3.0f), mzz = -(qxx + qyy) * (1.0f / 3.0f);
+    const float mxy = qxy * (1.0f / 3.0f), myz = qyz * (1.0f / 3.0f), mzx = qzx * (1.0f / 3.0f);
+    const float t000 = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = t000 * t000;
+    L[0] = fmaf(q, t000, L[0]);
+    const float c1_1 = 1.0f * ir2, c2_1 = 0.0f * ir2;
+    const float t100 = -c1_1 * (rx * t000);
+    L[1] = fmaf(q, t100, L[1]);
+    L[0] = fmaf(mdx, t100, L[0]);
+    const float t010 = -c1_1 * (ry * t000);
+    L[2] = fmaf(q, t010, L[2]);
+    L[0] = fmaf(mdy, t010, L[0]);
+    const float t001 = -c1_1 * (rz * t000);
+    L[3] = fmaf(q, t001, L[3]);
+    L[0] = fmaf(mdz, t001, L[0]);
+    const float c1_2 = 1.5f * ir2, c2_2 = 0.5f * ir2;
+    const float t200 = -fmaf(c1_2, rx * t100, c2_2 * (t000));
+    L[4] = fmaf(q, t200, L[4]);
+    L[1] = fmaf(2.0f * mdx, t200, L[1]);
+    L[0] = fmaf(mxx, t200, L[0]);
+    const float t110 = -c1_2 * (rx * t010 + ry * t100);
+    L[5] = fmaf(q, t110, L[5]);
+    L[2] = fmaf(mdx, t110, L[2]);
+    L[1] = fmaf(mdy, t110, L[1]);
+    L[0] = fmaf(mxy, t110, L[0]);
+    const float t101 = -c1_2 * (rx * t001 + rz * t100);
+    L[6] = fmaf(q, t101, L[6]);
+    L[3] = fmaf(mdx, t101, L[3]);
+    L[1] = fmaf(mdz, t101, L[1]);
+    L[0] = fmaf(mzx, t101, L[0]);
+    const float t020 = -fmaf(c1_2, ry * t010, c2_2 * (t000));
+    L[7] = fmaf(q, t020, L[7]);
+    L[2] = fmaf(2.0f * mdy, t020, L[2]);
+    L[0] = fmaf(myy, t020, L[0]);
+    const float t011 = -c1_2 * (ry * t001 + rz * t010);
+    L[8] = fmaf(q, t011, L[8]);
+    L[3] = fmaf(mdy, t011, L[3]);
+    L[2] = fmaf(mdz, t011, L[2]);
+    L[0] = fmaf(myz, t011, L[0]);
+    const float t002 = -fmaf(c1_2, rz * t001, c2_2 * (t000));
+    L[9] = fmaf(q, t002, L[9]);
+    L[3] = fmaf(2.0f * mdz, t002, L[3]);
+    L[0] = fmaf(mzz, t002, L[0]);
+    const float c1_3 = 1.6666666666666667f * ir2, c2_3 = 0.6666666666666666f * ir2;
+    const float t300 = -fmaf(c1_3, rx * t200, c2_3 * (t100));
+    L[10] = fmaf(q, t300, L[10]);
+    L[4] = fmaf(3.0f * mdx, t300, L[4]);
+    L[1] = fmaf(3.0f * mxx, t300, L[1]);
+    const float t210 = -fmaf(c1_3, rx * t110 + ry * t200, c2_3 * (t010));
+    L[11] = fmaf(q, t210, L[11]);
+    L[5] = fmaf(2.0f * mdx, t210, L[5]);
+    L[4] = fmaf(mdy, t210, L[4]);
+    L[2] = fmaf(mxx, t210, L[2]);
+    L[1] = fmaf(2.0f * mxy, t210, L[1]);
+    const float t201 = -fmaf(c1_3, rx * t101 + rz * t200, c2_3 * (t001));
+    L[12] = fmaf(q, t201, L[12]);
+    L[6] = fmaf(2.0f * mdx, t201, L[6]);
+    L[4] = fmaf(mdz, t201, L[4]);
+    L[3] = fmaf(mxx, t201, L[3]);
+    L[1] = fmaf(2.0f * mzx, t201, L[1]);
+    const float t120 = -fmaf(c1_3, rx * t020 + ry * t110, c2_3 * (t100));
+    L[13] = fmaf(q, t120, L[13]);
+    L[7] = fmaf(mdx, t120, L[7]);
+    L[5] = fmaf(2.0f * mdy, t120, L[5]);
+    L[1] = fmaf(myy, t120, L[1]);
+    L[2] = fmaf(2.0f * mxy, t120, L[2]);
+    const float t111 = -c1_3 * (rx * t011 + ry * t101 + rz * t110);
+    L[14] = fmaf(q, t111, L[14]);
+    L[8] = fmaf(mdx, t111, L[8]);
+    L[6] = fmaf(mdy, t111, L[6]);
+    L[5] = fmaf(mdz, t111, L[5]);
+    L[3] = fmaf(mxy, t111, L[3]);
+    L[1] = fmaf(myz, t111, L[1]);
+    L[2] = fmaf(mzx, t111, L[2]);
+    const float t102 = -fmaf(c1_3, rx * t002 + rz * t101, c2_3 * (t100));
+    L[15] = fmaf(q, t102, L[15]);
+    L[9] = fmaf(mdx, t102, L[9]);
+    L[6] = fmaf(2.0f * mdz, t102, L[6]);
+    L[1] = fmaf(mzz, t102, L[1]);
+    L[3] = fmaf(2.0f * mzx, t102, L[3]);
+    const float t030 = -fmaf(c1_3, ry * t020, c2_3 * (t010));
+    L[16] = fmaf(q, t030, L[16]);
+    L[7] = fmaf(3.0f * mdy, t030, L[7]);
+    L[2] = fmaf(3.0f * myy, t030, L[2]);
+    const float t021 = -fmaf(c1_3, ry * t011 + rz * t020, c2_3 * (t001));
+    L[17] = fmaf(q, t021, L[17]);
+    L[8] = fmaf(2.0f * mdy, t021, L[8]);
+    L[7] = fmaf(mdz, t021, L[7]);
+    L[3] = fmaf(myy, t021, L[3]);
+    L[2] = fmaf(2.0f * myz, t021, L[2]);
+    const float t012 = -fmaf(c1_3, ry * t002 + rz * t011, c2_3 * (t010));
+    L[18] = fmaf(q, t012, L[18]);
+    L[9] = fmaf(mdy, t012, L[9]);
+    L[8] = fmaf(2.0f * mdz, t012, L[8]);
+    L[2] = fmaf(mzz, t012, L[2]);
+    L[3] = fmaf(2.0f * myz, t012, L[3]);
+    const float t003 = -fmaf(c1_3, rz * t002, c2_3 * (t001));
+    L[19] = fmaf(q, t003, L[19]);
+    L[9] = fmaf(3.0f * mdz, t003, L[9]);
+    L[3] = fmaf(3.0f * mzz, t003, L[3]);
+    const float c1_4 = 1.75f * ir2, c2_4 = 0.75f * ir2;
+    const float t400 = -fmaf(c1_4, rx * t300, c2_4 * (t200));
+    L[10] = fmaf(4.0f * mdx, t400, L[10]);
+    L[4] = fmaf(6.0f * mxx, t400, L[4]);
+    const float t310 = -fmaf(c1_4, rx * t210 + ry * t300, c2_4 * (t110));
+    L[11] = fmaf(3.0f * mdx, t310, L[11]);
+    L[10] = fmaf(mdy, t310, L[10]);
+    L[5] = fmaf(3.0f * mxx, t310, L[5]);
+    L[4] = fmaf(3.0f * mxy, t310, L[4]);
+    const float t301 = -fmaf(c1_4, rx * t201 + rz * t300, c2_4 * (t101));
+    L[12] = fmaf(3.0f * mdx, t301, L[12]);
+    L[10] = fmaf(mdz, t301, L[10]);
+    L[6] = fmaf(3.0f * mxx, t301, L[6]);
+    L[4] = fmaf(3.0f * mzx, t301, L[4]);
+    const float t220 = -fmaf(c1_4, rx * t120 + ry * t210, c2_4 * (t020 + t200));
+    L[13] = fmaf(2.0f * mdx, t220, L[13]);
+    L[11] = fmaf(2.0f * mdy, t220, L[11]);
+    L[7] = fmaf(mxx, t220, L[7]);
+    L[4] = fmaf(myy, t220, L[4]);
+    L[5] = fmaf(4.0f * mxy, t220, L[5]);
+    const float t211 = -fmaf(c1_4, rx * t111 + ry * t201 + rz * t210, c2_4 * (t011));
+    L[14] = fmaf(2.0f * mdx, t211, L[14]);
+    L[12] = fmaf(mdy, t211, L[12]);
+    L[11] = fmaf(mdz, t211, L[11]);
+    L[8] = fmaf(mxx, t211, L[8]);
+    L[6] = fmaf(2.0f * mxy, t211, L[6]);
+    L[4] = fmaf(myz, t211, L[4]);
+    L[5] = fmaf(2.0f * mzx, t211, L[5]);
+    const float t202 = -fmaf(c1_4, rx * t102 + rz * t201, c2_4 * (t002 + t200));
+    L[15] = fmaf(2.0f * mdx, t202, L[15]);
+    L[12] = fmaf(2.0f * mdz, t202, L[12]);
+    L[9] = fmaf(mxx, t202, L[9]);
+    L[4] = fmaf(mzz, t202, L[4]);
+    L[6] = fmaf(4.0f * mzx, t202, L[6]);
+    const float t130 = -fmaf(c1_4, rx * t030 + ry * t120, c2_4 * (t110));
+    L[16] = fmaf(mdx, t130, L[16]);
+    L[13] = fmaf(3.0f * mdy, t130, L[13]);
+    L[5] = fmaf(3.0f * myy, t130, L[5]);
+    L[7] = fmaf(3.0f * mxy, t130, L[7]);
+    const float t121 = -fmaf(c1_4, rx * t021 + ry * t111 + rz * t120, c2_4 * (t101));
+    L[17] = fmaf(mdx, t121, L[17]);
+    L[14] = fmaf(2.0f * mdy, t121, L[14]);
+    L[13] = fmaf(mdz, t121, L[13]);
+    L[6] = fmaf(myy, t121, L[6]);
+    L[8] = fmaf(2.0f * mxy, t121, L[8]);
+    L[5] = fmaf(2.0f * myz, t121, L[5]);
+    L[7] = fmaf(mzx, t121, L[7]);
+    const float t112 = -fmaf(c1_4, rx * t012 + ry * t102 + rz * t111, c2_4 * (t110));
+    L[18] = fmaf(mdx, t112, L[18]);
+    L[15] = fmaf(mdy, t112, L[15]);
+    L[14] = fmaf(2.0f * mdz, t112, L[14]);
+    L[5] = fmaf(mzz, t112, L[5]);
+    L[9] = fmaf(mxy, t112, L[9]);
+    L[6] = fmaf(2.0f * myz, t112, L[6]);
+    L[8] = fmaf(2.0f * mzx, t112, L[8]);
+    const float t103 = -fmaf(c1_4, rx * t003 + rz * t102, c2_4 * (t101));
+    L[19] = fmaf(mdx, t103, L[19]);
+    L[15] = fmaf(3.0f * mdz, t103, L[15]);
+    L[6] = fmaf(3.0f * mzz, t103, L[6]);
+    L[9] = fmaf(3.0f * mzx, t103, L[9]);
+    const float t040 = -fmaf(c1_4, ry * t030, c2_4 * (t020));
+    L[16] = fmaf(4.0f * mdy, t040, L[16]);
+    L[7] = fmaf(6.0f * myy, t040, L[7]);
+    const float t031 = -fmaf(c1_4, ry * t021 + rz * t030, c2_4 * (t011));
+    L[17] = fmaf(3.0f * mdy, t031, L[17]);
+    L[16] = fmaf(mdz, t031, L[16]);
+    L[8] = fmaf(3.0f * myy, t031, L[8]);
+    L[7] = fmaf(3.0f * myz, t031, L[7]);
+    const float t022 = -fmaf(c1_4, ry * t012 + rz * t021, c2_4 * (t002 + t020));
+    L[18] = fmaf(2.0f * mdy, t022, L[18]);
+    L[17] = fmaf(2.0f * mdz, t022, L[17]);
+    L[9] = fmaf(myy, t022, L[9]);
+    L[7] = fmaf(mzz, t022, L[7]);
+    L[8] = fmaf(4.0f * myz, t022, L[8]);
+    const float t013 = -fmaf(c1_4, ry * t003 + rz * t012, c2_4 * (t011));
+    L[19] = fmaf(mdy, t013, L[19]);
+    L[18] = fmaf(3.0f * mdz, t013, L[18]);
+    L[8] = fmaf(3.0f * mzz, t013, L[8]);
+    L[9] = fmaf(3.0f * myz, t013, L[9]);
+    const float t004 = -fmaf(c1_4, rz * t003, c2_4 * (t002));
+    L[19] = fmaf(4.0f * mdz, t004, L[19]);
+    L[9] = fmaf(6.0f * mzz, t004, L[9]);
+    const float c1_5 = 1.8f * ir2, c2_5 = 0.8f * ir2;
+    const float t500 = -fmaf(c1_5, rx * t400, c2_5 * (t300));
+    L[10] = fmaf(10.0f * mxx, t500, L[10]);
+    const float t410 = -fmaf(c1_5, rx * t310 + ry * t400, c2_5 * (t210));
+    L[11] = fmaf(6.0f * mxx, t410, L[11]);
+    L[10] = fmaf(4.0f * mxy, t410, L[10]);
+    const float t401 = -fmaf(c1_5, rx * t301 + rz * t400, c2_5 * (t201));
+    L[12] = fmaf(6.0f * mxx, t401, L[12]);
+    L[10] = fmaf(4.0f * mzx, t401, L[10]);
+    const float t320 = -fmaf(c1_5, rx * t220 + ry * t310, c2_5 * (t120 + t300));
+    L[13] = fmaf(3.0f * mxx, t320, L[13]);
+    L[10] = fmaf(myy, t320, L[10]);
+    L[11] = fmaf(6.0f * mxy, t320, L[11]);
+    const float t311 = -fmaf(c1_5, rx * t211 + ry * t301 + rz * t310, c2_5 * (t111));
+    L[14] = fmaf(3.0f * mxx, t311, L[14]);
+    L[12] = fmaf(3.0f * mxy, t311, L[12]);
+    L[10] = fmaf(myz, t311, L[10]);
+    L[11] = fmaf(3.0f * mzx, t311, L[11]);
+    const float t302 = -fmaf(c1_5, rx * t202 + rz * t301, c2_5 * (t102 + t300));
+    L[15] = fmaf(3.0f * mxx, t302, L[15]);
+    L[10] = fmaf(mzz, t302, L[10]);
+    L[12] = fmaf(6.0f * mzx, t302, L[12]);
+    const float t230 = -fmaf(c1_5, rx * t130 + ry * t220, c2_5 * (t030 + t210));
+    L[16] = fmaf(mxx, t230, L[16]);
+    L[11] = fmaf(3.0f * myy, t230, L[11]);
+    L[13] = fmaf(6.0f * mxy, t230, L[13]);
+    const float t221 = -fmaf(c1_5, rx * t121 + ry * t211 + rz * t220, c2_5 * (t021 + t201));
+    L[17] = fmaf(mxx, t221, L[17]);
+    L[12] = fmaf(myy, t221, L[12]);
+    L[14] = fmaf(4.0f * mxy, t221, L[14]);
+    L[11] = fmaf(2.0f * myz, t221, L[11]);
+    L[13] = fmaf(2.0f * mzx, t221, L[13]);
+    const float t212 = -fmaf(c1_5, rx * t112 + ry * t202 + rz * t211, c2_5 * (t012 + t210));
+    L[18] = fmaf(mxx, t212, L[18]);
+    L[11] = fmaf(mzz, t212, L[11]);
+    L[15] = fmaf(2.0f * mxy, t212, L[15]);
+    L[12] = fmaf(2.0f * myz, t212, L[12]);
+    L[14] = fmaf(4.0f * mzx, t212, L[14]);
+    const float t203 = -fmaf(c1_5, rx * t103 + rz * t202, c2_5 * (t003 + t201));
+    L[19] = fmaf(mxx, t203, L[19]);
+    L[12] = fmaf(3.0f * mzz, t203, L[12]);
+    L[15] = fmaf(6.0f * mzx, t203, L[15]);
+    const float t140 = -fmaf(c1_5, rx * t040 + ry * t130, c2_5 * (t120));
+    L[13] = fmaf(6.0f * myy, t140, L[13]);
+    L[16] = fmaf(4.0f * mxy, t140, L[16]);
+    const float t131 = -fmaf(c1_5, rx * t031 + ry * t121 + rz * t130, c2_5 * (t111));
+    L[14] = fmaf(3.0f * myy, t131, L[14]);
+    L[17] = fmaf(3.0f * mxy, t131, L[17]);
+    L[13] = fmaf(3.0f * myz, t131, L[13]);
+    L[16] = fmaf(mzx, t131, L[16]);
+    const float t122 = -fmaf(c1_5, rx * t022 + ry * t112 + rz * t121, c2_5 * (t102 + t120));
+    L[15] = fmaf(myy, t122, L[15]);
+    L[13] = fmaf(mzz, t122, L[13]);
+    L[18] = fmaf(2.0f * mxy, t122, L[18]);
+    L[14] = fmaf(4.0f * myz, t122, L[14]);
+    L[17] = fmaf(2.0f * mzx, t122, L[17]);
+    const float t113 = -fmaf(c1_5, rx * t013 + ry * t103 + rz * t112, c2_5 * (t111));
+    L[14] = fmaf(3.0f * mzz, t113, L[14]);
+    L[19] = fmaf(mxy, t113, L[19]);
+    L[15] = fmaf(3.0f * myz, t113, L[15]);
+    L[18] = fmaf(3.0f * mzx, t113, L[18]);
+    const float t104 = -fmaf(c1_5, rx * t004 + rz * t103, c2_5 * (t102));
+    L[15] = fmaf(6.0f * mzz, t104, L[15]);
+    L[19] = fmaf(4.0f * mzx, t104, L[19]);
+    const float t050 = -fmaf(c1_5, ry * t040, c2_5 * (t030));
+    L[16] = fmaf(10.0f * myy, t050, L[16]);
+    const float t041 = -fmaf(c1_5, ry * t031 + rz * t040, c2_5 * (t021));
+    L[17] = fmaf(6.0f * myy, t041, L[17]);
+    L[16] = fmaf(4.0f * myz, t041, L[16]);
+    const float t032 = -fmaf(c1_5, ry * t022 + rz * t031, c2_5 * (t012 + t030));
+    L[18] = fmaf(3.0f * myy, t032, L[18]);
+    L[16] = fmaf(mzz, t032, L[16]);
+    L[17] = fmaf(6.0f * myz, t032, L[17]);
+    const float t023 = -fmaf(c1_5, ry * t013 + rz * t022, c2_5 * (t003 + t021));
+    L[19] = fmaf(myy, t023, L[19]);
+    L[17] = fmaf(3.0f * mzz, t023, L[17]);
+    L[18] = fmaf(6.0f * myz, t023, L[18]);
+    const float t014 = -fmaf(c1_5, ry * t004 + rz * t013, c2_5 * (t012));
+    L[18] = fmaf(6.0f * mzz, t014, L[18]);
+    L[19] = fmaf(4.0f * myz, t014, L[19]);
+    const float t005 = -fmaf(c1_5, rz * t004, c2_5 * (t003));
+    L[19] = fmaf(10.0f * mzz, t005, L[19]);
+}
+
+// potential and field at offset y from the expansion centre (order P = 3)
+__device__ inline void l2p_p3(const float (&L)[20], float x, float y, float z, float &phi, float &ex, float &ey, float &ez)
+{
+    const float x0 = 1.0f, x1 = x;
+    const float x2 = x1 * x;
+    const float x3 = x2 * x;
+    const float y0 = 1.0f, y1 = y;
+    const float y2 = y1 * y;
+    const float y3 = y2 * y;
+    const float z0 = 1.0f, z1 = z;
+    const float z2 = z1 * z;
+    const float z3 = z2 * z;
+    phi = L[0] * (x0 * y0 * z0)
+        + L[1] * (x1 * y0 * z0)
+        + L[2] * (x0 * y1 * z0)
+        + L[3] * (x0 * y0 * z1)
+        + L[4] * (x2 * y0 * z0)
+        + L[5] * (x1 * y1 * z0)
+        + L[6] * (x1 * y0 * z1)
+        + L[7] * (x0 * y2 * z0)
+        + L[8] * (x0 * y1 * z1)
+        + L[9] * (x0 * y0 * z2)
+        + L[10] * (x3 * y0 * z0)
+        + L[11] * (x2 * y1 * z0)
+        + L[12] * (x2 * y0 * z1)
+        + L[13] * (x1 * y2 * z0)
+        + L[14] * (x1 * y1 * z1)
+        + L[15] * (x1 * y0 * z2)
+        + L[16] * (x0 * y3 * z0)
+        + L[17] * (x0 * y2 * z1)
+        + L[18] * (x0 * y1 * z2)
+        + L[19] * (x0 * y0 * z3);
+    ex = -(1.0f * L[1] * (x0 * y0 * z0)
+        + 2.0f * L[4] * (x1 * y0 * z0)
+        + 1.0f * L[5] * (x0 * y1 * z0)
+        + 1.0f * L[6] * (x0 * y0 * z1)
+        + 3.0f * L[10] * (x2 * y0 * z0)
+        + 2.0f * L[11] * (x1 * y1 * z0)
+        + 2.0f * L[12] * (x1 * y0 * z1)
+        + 1.0f * L[13] * (x0 * y2 * z0)
+        + 1.0f * L[14] * (x0 * y1 * z1)
+        + 1.0f * L[15] * (x0 * y0 * z2));
+    ey = -(1.0f * L[2] * (x0 * y0 * z0)
+        + 1.0f * L[5] * (x1 * y0 * z0)
+        + 2.0f * L[7] * (x0 * y1 * z0)
+        + 1.0f * L[8] * (x0 * y0 * z1)
+        + 1.0f * L[11] * (x2 * y0 * z0)
+        + 2.0f * L[13] * (x1 * y1 * z0)
+        + 1.0f * L[14] * (x1 * y0 * z1)
+        + 3.0f * L[16] * (x0 * y2 * z0)
+        + 2.0f * L[17] * (x0 * y1 * z1)
+        + 1.0f * L[18] * (x0 * y0 * z2));
+    ez = -(1.0f * L[3] * (x0 * y0 * z0)
+        + 1.0f * L[6] * (x1 * y0 * z0)
+        + 1.0f * L[8] * (x0 * y1 * z0)
+        + 2.0f * L[9] * (x0 * y0 * z1)
+        + 1.0f * L[12] * (x2 * y0 * z0)
+        + 1.0f * L[14] * (x1 * y1 * z0)
+        + 2.0f * L[15] * (x1 * y0 * z1)
+        + 1.0f * L[17] * (x0 * y2 * z0)
+        + 2.0f * L[18] * (x0 * y1 * z1)
+        + 3.0f * L[19] * (x0 * y0 * z2));
+}
+
+// adds one cell's local coefficients about the expansion centre to L[25]
+// (order P = 4; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)
+__device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float dx, float dy, float dz,
+                                  float qxx, float qyy, float qxy, float qyz, float qzx, float (&L)[25])
+{
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float mdx = -dx, mdy = -dy, mdz = -dz;
+    const float mxx = qxx * (1.0f / 3.0f), myy = qyy * (1.0f / 3.0f), mzz = -(qxx + qyy) * (1.0f / 3.0f);
+    const float mxy = qxy * (1.0f / 3.0f), myz = qyz * (1.0f / 3.0f), mzx = qzx * (1.0f / 3.0f);
+    const float t000 = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = t000 * t000;
+    L[0] = fmaf(q, t000, L[0]);
+    const float c1_1 = 1.0f * ir2, c2_1 = 0.0f * ir2;
+    const float t100 = -c1_1 * (rx * t000);
+    L[1] = fmaf(q, t100, L[1]);
+    L[0] = fmaf(mdx, t100, L[0]);
+    const float t010 = -c1_1 * (ry * t000);
+    L[2] = fmaf(q, t010, L[2]);
+    L[0] = fmaf(mdy, t010, L[0]);
+    const float t001 = -c1_1 * (rz * t000);
+    L[3] = fmaf(q, t001, L[3]);
+    L[0] = fmaf(mdz, t001, L[0]);
+    const float c1_2 = 1.5f * ir2, c2_2 = 0.5f * ir2;
+    const float t200 = -fmaf(c1_2, rx * t100, c2_2 * (t000));
+    L[4] = fmaf(q, t200, L[4]);
+    L[1] = fmaf(2.0f * mdx, t200, L[1]);
+    L[0] = fmaf(mxx, t200, L[0]);
+    const float t110 = -c1_2 * (rx * t010 + ry * t100);
+    L[5] = fmaf(q, t110, L[5]);
+    L[2] = fmaf(mdx, t110, L[2]);
+    L[1] = fmaf(mdy, t110, L[1]);
+    L[0] = fmaf(mxy, t110, L[0]);
+    const float t101 = -c1_2 * (rx * t001 + rz * t100);
+    L[6] = fmaf(q, t101, L[6]);
+    L[3] = fmaf(mdx, t101, L[3]);
+    L[1] = fmaf(mdz, t101, L[1]);
+    L[0] = fmaf(mzx, t101, L[0]);
+    const float t020 = -fmaf(c1_2, ry * t010, c2_2 * (t000));
+    L[7] = fmaf(q, t020, L[7]);
+    L[2] = fmaf(2.0f * mdy, t020, L[2]);
+    L[0] = fmaf(myy, t020, L[0]);
+    const float t011 = -c1_2 * (ry * t001 + rz * t010);
+    L[8] = fmaf(q, t011, L[8]);
+    L[3] = fmaf(mdy, t011, L[3]);
+    L[2] = fmaf(mdz, t011, L[2]);
+    L[0] = fmaf(myz, t011, L[0]);
+    const float t002 = -fmaf(c1_2, rz * t001, c2_2 * (t000));
+    L[3] = fmaf(2.0f * mdz, t002, L[3]);
+    L[0] = fmaf(mzz, t002, L[0]);
+    const float c1_3 = 1.6666666666666667f * ir2, c2_3 = 0.6666666666666666f * ir2;
+    const float t300 = -fmaf(c1_3, rx * t200, c2_3 * (t100));
+    L[9] = fmaf(q, t300, L[9]);
+    L[4] = fmaf(3.0f * mdx, t300, L[4]);
+    L[1] = fmaf(3.0f * mxx, t300, L[1]);
+    const float t210 = -fmaf(c1_3, rx * t110 + ry * t200, c2_3 * (t010));
+    L[10] = fmaf(q, t210, L[10]);
+    L[5] = fmaf(2.0f * mdx, t210, L[5]);
+    L[4] = fmaf(mdy, t210, L[4]);
+    L[2] = fmaf(mxx, t210, L[2]);
+    L[1] = fmaf(2.0f * mxy, t210, L[1]);
+    const float t201 = -fmaf(c1_3, rx * t101 + rz * t200, c2_3 * (t001));
+    L[11] = fmaf(q, t201, L[11]);
+    L[6] = fmaf(2.0f * mdx, t201, L[6]);
+    L[4] = fmaf(mdz, t201, L[4]);
+    L[3] = fmaf(mxx, t201, L[3]);
+    L[1] = fmaf(2.0f * mzx, t201, L[1]);
+    const float t120 = -fmaf(c1_3, rx * t020 + ry * t110, c2_3 * (t100));
+    L[12] = fmaf(q, t120, L[12]);
+    L[7] = fmaf(mdx, t120, L[7]);
+    L[5] = fmaf(2.0f * mdy, t120, L[5]);
+    L[1] = fmaf(myy, t120, L[1]);
+    L[2] = fmaf(2.0f * mxy, t120, L[2]);
+    const float t111 = -c1_3 * (rx * t011 + ry * t101 + rz * t110);
+    L[13] = fmaf(q, t111, L[13]);
+    L[8] = fmaf(mdx, t111, L[8]);
+    L[6] = fmaf(mdy, t111, L[6]);
+    L[5] = fmaf(mdz, t111, L[5]);
+    L[3] = fmaf(mxy, t111, L[3]);
+    L[1] = fmaf(myz, t111, L[1]);
+    L[2] = fmaf(mzx, t111, L[2]);
+    const float t102 = -fmaf(c1_3, rx * t002 + rz * t101, c2_3 * (t100));
+    L[6] = fmaf(2.0f * mdz, t102, L[6]);
+    L[1] = fmaf(mzz, t102, L[1]);
+    L[3] = fmaf(2.0f * mzx, t102, L[3]);
+    const float t030 = -fmaf(c1_3, ry * t020, c2_3 * (t010));
+    L[14] = fmaf(q, t030, L[14]);
+    L[7] = fmaf(3.0f * mdy, t030, L[7]);
+    L[2] = fmaf(3.0f * myy, t030, L[2]);
+    const float t021 = -fmaf(c1_3, ry * t011 + rz * t020, c2_3 * (t001));
+    L[15] = fmaf(q, t021, L[15]);
+    L[8] = fmaf(2.0f * mdy, t021, L[8]);
+    L[7] = fmaf(mdz, t021, L[7]);
+    L[3] = fmaf(myy, t021, L[3]);
+    L[2] = fmaf(2.0f * myz, t021, L[2]);
+    const float t012 = -fmaf(c1_3, ry * t002 + rz * t011, c2_3 * (t010));
+    L[8] = fmaf(2.0f * mdz, t012, L[8]);
+    L[2] = fmaf(mzz, t012, L[2]);
+    L[3] = fmaf(2.0f * myz, t012, L[3]);
+    const float t003 = -fmaf(c1_3, rz * t002, c2_3 * (t001));
+    L[3] = fmaf(3.0f * mzz, t003, L[3]);
+    const float c1_4 = 1.75f * ir2, c2_4 = 0.75f * ir2;
+    const float t400 = -fmaf(c1_4, rx * t300, c2_4 * (t200));
+    L[16] = fmaf(q, t400, L[16]);
+    L[9] = fmaf(4.0f * mdx, t400, L[9]);
+    L[4] = fmaf(6.0f * mxx, t400, L[4]);
+    const float t310 = -fmaf(c1_4, rx * t210 + ry * t300, c2_4 * (t110));
+    L[17] = fmaf(q, t310, L[17]);
+    L[10] = fmaf(3.0f * mdx, t310, L[10]);
+    L[9] = fmaf(mdy, t310, L[9]);
+    L[5] = fmaf(3.0f * mxx, t310, L[5]);
+    L[4] = fmaf(3.0f * mxy, t310, L[4]);
+    const float t301 = -fmaf(c1_4, rx * t201 + rz * t300, c2_4 * (t101));
+    L[18] = fmaf(q, t301, L[18]);
+    L[11] = fmaf(3.0f * mdx, t301, L[11]);
+    L[9] = fmaf(mdz, t301, L[9]);
+    L[6] = fmaf(3.0f * mxx, t301, L[6]);
+    L[4] = fmaf(3.0f * mzx, t301, L[4]);
+    const float t220 = -fmaf(c1_4, rx * t120 + ry * t210, c2_4 * (t020 + t200));
+    L[19] = fmaf(q, t220, L[19]);
+    L[12] = fmaf(2.0f * mdx, t220, L[12]);
+    L[10] = fmaf(2.0f * mdy, t220, L[10]);
+    L[7] = fmaf(mxx, t220, L[7]);
+    L[4] = fmaf(myy, t220, L[4]);
+    L[5] = fmaf(4.0f * mxy, t220, L[5]);
+    const float t211 = -fmaf(c1_4, rx * t111 + ry * t201 + rz * t210, c2_4 * (t011));
+    L[20] = fmaf(q, t211, L[20]);
+    L[13] = fmaf(2.0f * mdx, t211, L[13]);
+    L[11] = fmaf(mdy, t211, L[11]);
+    L[10] = fmaf(mdz, t211, L[10]);
+    L[8] = fmaf(mxx, t211, L[8]);
+    L[6] = fmaf(2.0f * mxy, t211, L[6]);
+    L[4] = fmaf(myz, t211, L[4]);
+    L[5] = fmaf(2.0f * mzx, t211, L[5]);
+    const float t202 = -fmaf(c1_4, rx * t102 + rz * t201, c2_4 * (t002 + t200));
+    L[11] = fmaf(2.0f * mdz, t202, L[11]);
+    L[4] = fmaf(mzz, t202, L[4]);
+    L[6] = fmaf(4.0f * mzx, t202, L[6]);
+    const float t130 = -fmaf(c1_4, rx * t030 + ry * t120, c2_4 * (t110));
+    L[21] = fmaf(q, t130, L[21]);
+    L[14] = fmaf(mdx, t130, L[14]);
+    L[12] = fmaf(3.0f * mdy, t130, L[12]);
+    L[5] = fmaf(3.0f * myy, t130, L[5]);
+    L[7] = fmaf(3.0f * mxy, t130, L[7]);
+    const float t121 = -fmaf(c1_4, rx * t021 + ry * t111 + rz * t120, c2_4 * (t101));
+    L[22] = fmaf(q, t121, L[22]);
+    L[15] = fmaf(mdx, t121, L[15]);
+    L[13] = fmaf(2.0f * mdy, t121, L[13]);
+    L[12] = fmaf(mdz, t121, L[12]);
+    L[6] = fmaf(myy, t121, L[6]);
+    L[8] = fmaf(2.0f * mxy, t121, L[8]);
+    L[5] = fmaf(2.0f * myz, t121, L[5]);
+    L[7] = fmaf(mzx, t121, L[7]);
+    const float t112 = -fmaf(c1_4, rx * t012 + ry * t102 + rz * t111, c2_4 * (t110));
+    L[13] = fmaf(2.0f * mdz, t112, L[13]);
+    L[5] = fmaf(mzz, t112, L[5]);
+    L[6] = fmaf(2.0f * myz, t112, L[6]);
+    L[8] = fmaf(2.0f * mzx, t112, L[8]);
+    const float t103 = -fmaf(c1_4, rx * t003 + rz * t102, c2_4 * (t101));
+    L[6] = fmaf(3.0f * mzz, t103, L[6]);
+    const float t040 = -fmaf(c1_4, ry * t030, c2_4 * (t020));
+    L[23] = fmaf(q, t040, L[23]);
+    L[14] = fmaf(4.0f * mdy, t040, L[14]);
+    L[7] = fmaf(6.0f * myy, t040, L[7]);
+    const float t031 = -fmaf(c1_4, ry * t021 + rz * t030, c2_4 * (t011));
+    L[24] = fmaf(q, t031, L[24]);
+    L[15] = fmaf(3.0f * mdy, t031, L[15]);
+    L[14] = fmaf(mdz, t031, L[14]);
+    L[8] = fmaf(3.0f * myy, t031, L[8]);
+    L[7] = fmaf(3.0f * myz, t031, L[7]);
+    const float t022 = -fmaf(c1_4, ry * t012 + rz * t021, c2_4 * (t002 + t020));
+    L[15] = fmaf(2.0f * mdz, t022, L[15]);
+    L[7] = fmaf(mzz, t022, L[7]);
+    L[8] = fmaf(4.0f * myz, t022, L[8]);
+    const float t013 = -fmaf(c1_4, ry * t003 + rz * t012, c2_4 * (t011));
+    L[8] = fmaf(3.0f * mzz, t013, L[8]);
+    const float c1_5 = 1.8f * ir2, c2_5 = 0.8f * ir2;
+    const float t500 = -fmaf(c1_5, rx * t400, c2_5 * (t300));
+    L[16] = fmaf(5.0f * mdx, t500, L[16]);
+    L[9] = fmaf(10.0f * mxx, t500, L[9]);
+    const float t410 = -fmaf(c1_5, rx * t310 + ry * t400, c2_5 * (t210));
+    L[17] = fmaf(4.0f * mdx, t410, L[17]);
+    L[16] = fmaf(mdy, t410, L[16]);
+    L[10] = fmaf(6.0f * mxx, t410, L[10]);
+    L[9] = fmaf(4.0f * mxy, t410, L[9]);
+    const float t401 = -fmaf(c1_5, rx * t301 + rz * t400, c2_5 * (t201));
+    L[18] = fmaf(4.0f * mdx, t401, L[18]);
+    L[16] = fmaf(mdz, t401, L[16]);
+    L[11] = fmaf(6.0f * mxx, t401, L[11]);
+    L[9] = fmaf(4.0f * mzx, t401, L[9]);
+    const float t320 = -fmaf(c1_5, rx * t220 + ry * t310, c2_5 * (t120 + t300));
+    L[19] = fmaf(3.0f * mdx, t320, L[19]);
+    L[17] = fmaf(2.0f * mdy, t320, L[17]);
+    L[12] = fmaf(3.0f * mxx, t320, L[12]);
+    L[9] = fmaf(myy, t320, L[9]);
+    L[10] = fmaf(6.0f * mxy, t320, L[10]);
+    const float t311 = -fmaf(c1_5, rx * t211 + ry * t301 + rz * t310, c2_5 * (t111));
+    L[20] = fmaf(3.0f * mdx, t311, L[20]);
+    L[18] = fmaf(mdy, t311, L[18]);
+    L[17] = fmaf(mdz, t311, L[17]);
+    L[13] = fmaf(3.0f * mxx, t311, L[13]);
+    L[11] = fmaf(3.0f * mxy, t311, L[11]);
+    L[9] = fmaf(myz, t311, L[9]);
+    L[10] = fmaf(3.0f * mzx, t311, L[10]);
+    const float t302 = -fmaf(c1_5, rx * t202 + rz * t301, c2_5 * (t102 + t300));
+    L[18] = fmaf(2.0f * mdz, t302, L[18]);
+    L[9] = fmaf(mzz, t302, L[9]);
+    L[11] = fmaf(6.0f * mzx, t302, L[11]);
+    const float t230 = -fmaf(c1_5, rx * t130 + ry * t220, c2_5 * (t030 + t210));
+    L[21] = fmaf(2.0f * mdx, t230, L[21]);
+    L[19] = fmaf(3.0f * mdy, t230, L[19]);
+    L[14] = fmaf(mxx, t230, L[14]);
+    L[10] = fmaf(3.0f * myy, t230, L[10]);
+    L[12] = fmaf(6.0f * mxy, t230, L[12]);
+    const float t221 = -fmaf(c1_5, rx * t121 + ry * t211 + rz * t220, c2_5 * (t021 + t201));
+    L[22] = fmaf(2.0f * mdx, t221, L[22]);
+    L[20] = fmaf(2.0f * mdy, t221, L[20]);
+    L[19] = fmaf(mdz, t221, L[19]);
+    L[15] = fmaf(mxx, t221, L[15]);
+    L[11] = fmaf(myy, t221, L[11]);
+    L[13] = fmaf(4.0f * mxy, t221, L[13]);
+    L[10] = fmaf(2.0f * myz, t221, L[10]);
+    L[12] = fmaf(2.0f * mzx, t221, L[12]);
+    const float t212 = -fmaf(c1_5, rx * t112 + ry * t202 + rz * t211, c2_5 * (t012 + t210));
+    L[20] = fmaf(2.0f * mdz, t212, L[20]);
+    L[10] = fmaf(mzz, t212, L[10]);
+    L[11] = fmaf(2.0f * myz, t212, L[11]);
+    L[13] = fmaf(4.0f * mzx, t212, L[13]);
+    const float t203 = -fmaf(c1_5, rx * t103 + rz * t202, c2_5 * (t003 + t201));
+    L[11] = fmaf(3.0f * mzz, t203, L[11]);
+    const float t140 = -fmaf(c1_5, rx * t040 + ry * t130, c2_5 * (t120));
+    L[23] = fmaf(mdx, t140, L[23]);
+    L[21] = fmaf(4.0f * mdy, t140, L[21]);
+    L[12] = fmaf(6.0f * myy, t140, L[12]);
+    L[14] = fmaf(4.0f * mxy, t140, L[14]);
+    const float t131 = -fmaf(c1_5, rx * t031 + ry * t121 + rz * t130, c2_5 * (t111));
+    L[24] = fmaf(mdx, t131, L[24]);
+    L[22] = fmaf(3.0f * mdy, t131, L[22]);
+    L[21] = fmaf(mdz, t131, L[21]);
+    L[13] = fmaf(3.0f * myy, t131, L[13]);
+    L[15] = fmaf(3.0f * mxy, t131, L[15]);
+    L[12] = fmaf(3.0f * myz, t131, L[12]);
+    L[14] = fmaf(mzx, t131, L[14]);
+    const float t122 = -fmaf(c1_5, rx * t022 + ry * t112 + rz * t121, c2_5 * (t102 + t120));
+    L[22] = fmaf(2.0f * mdz, t122, L[22]);
+    L[12] = fmaf(mzz, t122, L[12]);
+    L[13] = fmaf(4.0f * myz, t122, L[13]);
+    L[15] = fmaf(2.0f * mzx, t122, L[15]);
+    const float t113 = -fmaf(c1_5, rx * t013 + ry * t103 + rz * t112, c2_5 * (t111));
+    L[13] = fmaf(3.0f * mzz, t113, L[13]);
+    const float t050 = -fmaf(c1_5, ry * t040, c2_5 * (t030));
+    L[23] = fmaf(5.0f * mdy, t050, L[23]);
+    L[14] = fmaf(10.0f * myy, t050, L[14]);
+    const float t041 = -fmaf(c1_5, ry * t031 + rz * t040, c2_5 * (t021));
+    L[24] = fmaf(4.0f * mdy, t041, L[24]);
+    L[23] = fmaf(mdz, t041, L[23]);
+    L[15] = fmaf(6.0f * myy, t041, L[15]);
+    L[14] = fmaf(4.0f * myz, t041, L[14]);
+    const float t032 = -fmaf(c1_5, ry * t022 + rz * t031, c2_5 * (t012 + t030));
+    L[24] = fmaf(2.0f * mdz, t032, L[24]);
+    L[14] = fmaf(mzz, t032, L[14]);
+    L[15] = fmaf(6.0f * myz, t032, L[15]);
+    const float t023 = -fmaf(c1_5, ry * t013 + rz * t022, c2_5 * (t003 + t021));
+    L[15] = fmaf(3.0f * mzz, t023, L[15]);
+    const float c1_6 = 1.8333333333333333f * ir2, c2_6 = 0.8333333333333334f * ir2;
+    const float t600 = -fmaf(c1_6, rx * t500, c2_6 * (t400));
+    L[16] = fmaf(15.0f * mxx, t600, L[16]);
+    const float t510 = -fmaf(c1_6, rx * t410 + ry * t500, c2_6 * (t310));
+    L[17] = fmaf(10.0f * mxx, t510, L[17]);
+    L[16] = fmaf(5.0f * mxy, t510, L[16]);
+    const float t501 = -fmaf(c1_6, rx * t401 + rz * t500, c2_6 * (t301));
+    L[18] = fmaf(10.0f * mxx, t501, L[18]);
+    L[16] = fmaf(5.0f * mzx, t501, L[16]);
+    const float t420 = -fmaf(c1_6, rx * t320 + ry * t410, c2_6 * (t220 + t400));
+    L[19] = fmaf(6.0f * mxx, t420, L[19]);
+    L[16] = fmaf(myy, t420, L[16]);
+    L[17] = fmaf(8.0f * mxy, t420, L[17]);
+    const float t411 = -fmaf(c1_6, rx * t311 + ry * t401 + rz * t410, c2_6 * (t211));
+    L[20] = fmaf(6.0f * mxx, t411, L[20]);
+    L[18] = fmaf(4.0f * mxy, t411, L[18]);
+    L[16] = fmaf(myz, t411, L[16]);
+    L[17] = fmaf(4.0f * mzx, t411, L[17]);
+    const float t402 = -fmaf(c1_6, rx * t302 + rz * t401, c2_6 * (t202 + t400));
+    L[16] = fmaf(mzz, t402, L[16]);
+    L[18] = fmaf(8.0f * mzx, t402, L[18]);
+    const float t330 = -fmaf(c1_6, rx * t230 + ry * t320, c2_6 * (t130 + t310));
+    L[21] = fmaf(3.0f * mxx, t330, L[21]);
+    L[17] = fmaf(3.0f * myy, t330, L[17]);
+    L[19] = fmaf(9.0f * mxy, t330, L[19]);
+    const float t321 = -fmaf(c1_6, rx * t221 + ry * t311 + rz * t320, c2_6 * (t121 + t301));
+    L[22] = fmaf(3.0f * mxx, t321, L[22]);
+    L[18] = fmaf(myy, t321, L[18]);
+    L[20] = fmaf(6.0f * mxy, t321, L[20]);
+    L[17] = fmaf(2.0f * myz, t321, L[17]);
+    L[19] = fmaf(3.0f * mzx, t321, L[19]);
+    const float t312 = -fmaf(c1_6, rx * t212 + ry * t302 + rz * t311, c2_6 * (t112 + t310));
+    L[17] = fmaf(mzz, t312, L[17]);
+    L[18] = fmaf(2.0f * myz, t312, L[18]);
+    L[20] = fmaf(6.0f * mzx, t312, L[20]);
+    const float t303 = -fmaf(c1_6, rx * t203 + rz * t302, c2_6 * (t103 + t301));
+    L[18] = fmaf(3.0f * mzz, t303, L[18]);
+    const float t240 = -fmaf(c1_6, rx * t140 + ry * t230, c2_6 * (t040 + t220));
+    L[23] = fmaf(mxx, t240, L[23]);
+    L[19] = fmaf(6.0f * myy, t240, L[19]);
+    L[21] = fmaf(8.0f * mxy, t240, L[21]);
+    const float t231 = -fmaf(c1_6, rx * t131 + ry * t221 + rz * t230, c2_6 * (t031 + t211));
+    L[24] = fmaf(mxx, t231, L[24]);
+    L[20] = fmaf(3.0f * myy, t231, L[20]);
+    L[22] = fmaf(6.0f * mxy, t231, L[22]);
+    L[19] = fmaf(3.0f * myz, t231, L[19]);
+    L[21] = fmaf(2.0f * mzx, t231, L[21]);
+    const float t222 = -fmaf(c1_6, rx * t122 + ry * t212 + rz * t221, c2_6 * (t022 + t202 + t220));
+    L[19] = fmaf(mzz, t222, L[19]);
+    L[20] = fmaf(4.0f * myz, t222, L[20]);
+    L[22] = fmaf(4.0f * mzx, t222, L[22]);
+    const float t213 = -fmaf(c1_6, rx * t113 + ry * t203 + rz * t212, c2_6 * (t013 + t211));
+    L[20] = fmaf(3.0f * mzz, t213, L[20]);
+    const float t150 = -fmaf(c1_6, rx * t050 + ry * t140, c2_6 * (t130));
+    L[21] = fmaf(10.0f * myy, t150, L[21]);
+    L[23] = fmaf(5.0f * mxy, t150, L[23]);
+    const float t141 = -fmaf(c1_6, rx * t041 + ry * t131 + rz * t140, c2_6 * (t121));
+    L[22] = fmaf(6.0f * myy, t141, L[22]);
+    L[24] = fmaf(4.0f * mxy, t141, L[24]);
+    L[21] = fmaf(4.0f * myz, t141, L[21]);
+    L[23] = fmaf(mzx, t141, L[23]);
+    const float t132 = -fmaf(c1_6, rx * t032 + ry * t122 + rz * t131, c2_6 * (t112 + t130));
+    L[21] = fmaf(mzz, t132, L[21]);
+    L[22] = fmaf(6.0f * myz, t132, L[22]);
+    L[24] = fmaf(2.0f * mzx, t132, L[24]);
+    const float t123 = -fmaf(c1_6, rx * t023 + ry * t113 + rz * t122, c2_6 * (t103 + t121));
+    L[22] = fmaf(3.0f * mzz, t123, L[22]);
+    const float t060 = -fmaf(c1_6, ry * t050, c2_6 * (t040));
+    L[23] = fmaf(15.0f * myy, t060, L[23]);
+    const float t051 = -fmaf(c1_6, ry * t041 + rz * t050, c2_6 * (t031));
+    L[24] = fmaf(10.0f * myy, t051, L[24]);
+    L[23] = fmaf(5.0f * myz, t051, L[23]);
+    const float t042 = -fmaf(c1_6, ry * t032 + rz * t041, c2_6 * (t022 + t040));
+    L[23] = fmaf(mzz, t042, L[23]);
+    L[24] = fmaf(8.0f * myz, t042, L[24]);
+    const float t033 = -fmaf(c1_6, ry * t023 + rz * t032, c2_6 * (t013 + t031));
+    L[24] = fmaf(3.0f * mzz, t033, L[24]);
+}
+
+// potential and field at offset y from the expansion centre (order P = 4, from the
+// independent coefficients b_z <= 1; each homogeneous part of a harmonic
+// expansion is harmonic, which fixes the others)
+__device__ inline void l2p_p4h(const float (&Lh)[25], float x, float y, float z, float &phi, float &ex, float &ey, float &ez)
+{
+    float L[35];
+    L[0] = Lh[0];
+    L[1] = Lh[1];
+    L[2] = Lh[2];
+    L[3] = Lh[3];
+    L[4] = Lh[4];
+    L[5] = Lh[5];
+    L[6] = Lh[6];
+    L[7] = Lh[7];
+    L[8] = Lh[8];
+    L[10] = Lh[9];
+    L[11] = Lh[10];
+    L[12] = Lh[11];
+    L[13] = Lh[12];
+    L[14] = Lh[13];
+    L[16] = Lh[14];
+    L[17] = Lh[15];
+    L[20] = Lh[16];
+    L[21] = Lh[17];
+    L[22] = Lh[18];
+    L[23] = Lh[19];
+    L[24] = Lh[20];
+    L[26] = Lh[21];
+    L[27] = Lh[22];
+    L[30] = Lh[23];
+    L[31] = Lh[24];
+    L[9] = -(1.0f * L[4] + 1.0f * L[7]);
+    L[15] = -(3.0f * L[10] + 1.0f * L[13]);
+    L[18] = -(1.0f * L[11] + 3.0f * L[16]);
+    L[25] = -(6.0f * L[20] + 1.0f * L[23]);
+    L[28] = -(3.0f * L[21] + 3.0f * L[26]);
+    L[32] = -(1.0f * L[23] + 6.0f * L[30]);
+    L[19] = -(0.3333333333333333f * L[12] + 0.3333333333333333f * L[17]);
+    L[29] = -(1.0f * L[22] + 0.3333333333333333f * L[27]);
+    L[33] = -(0.3333333333333333f * L[24] + 1.0f * L[31]);
+    L[34] = -(0.16666666666666666f * L[25] + 0.16666666666666666f * L[32]);
+    const float x0 = 1.0f, x1 = x;
+    const float x2 = x1 * x;
+    const float x3 = x2 * x;
+    const float x4 = x3 * x;
+    const float y0 = 1.0f, y1 = y;
+    const float y2 = y1 * y;
+    const float y3 = y2 * y;
+    const float y4 = y3 * y;
+    const float z0 = 1.0f, z1 = z;
+    const float z2 = z1 * z;
+    const float z3 = z2 * z;
+    const float z4 = z3 * z;
+    phi = L[0] * (x0 * y0 * z0)
+        + L[1] * (x1 * y0 * z0)
+        + L[2] * (x0 * y1 * z0)
+        + L[3] * (x0 * y0 * z1)
+        + L[4] * (x2 * y0 * z0)
+        + L[5] * (x1 * y1 * z0)
+        + L[6] * (x1 * y0 * z1)
+        + L[7] * (x0 * y2 * z0)
+        + L[8] * (x0 * y1 * z1)
+        + L[9] * (x0 * y0 * z2)
+        + L[10] * (x3 * y0 * z0)
+        + L[11] * (x2 * y1 * z0)
+        + L[12] * (x2 * y0 * z1)
+        + L[13] * (x1 * y2 * z0)
+        + L[14] * (x1 * y1 * z1)
+        + L[15] * (x1 * y0 * z2)
+        + L[16] * (x0 * y3 * z0)
+        + L[17] * (x0 * y2 * z1)
+        + L[18] * (x0 * y1 * z2)
+        + L[19] * (x0 * y0 * z3)
+        + L[20] * (x4 * y0 * z0)
+        + L[21] * (x3 * y1 * z0)
+        + L[22] * (x3 * y0 * z1)
+        + L[23] * (x2 * y2 * z0)
+        + L[24] * (x2 * y1 * z1)
+        + L[25] * (x2 * y0 * z2)
+        + L[26] * (x1 * y3 * z0)
+        + L[27] * (x1 * y2 * z1)
+        + L[28] * (x1 * y1 * z2)
+        + L[29] * (x1 * y0 * z3)
+        + L[30] * (x0 * y4 * z0)
+        + L[31] * (x0 * y3 * z1)
+        + L[32] * (x0 * y2 * z2)
+        + L[33] * (x0 * y1 * z3)
+        + L[34] * (x0 * y0 * z4);
+    ex = -(1.0f * L[1] * (x0 * y0 * z0)
+        + 2.0f * L[4] * (x1 * y0 * z0)
+        + 1.0f * L[5] * (x0 * y1 * z0)
+        + 1.0f * L[6] * (x0 * y0 * z1)
+        + 3.0f * L[10] * (x2 * y0 * z0)
+        + 2.0f * L[11] * (x1 * y1 * z0)
+        + 2.0f * L[12] * (x1 * y0 * z1)
+        + 1.0f * L[13] * (x0 * y2 * z0)
+        + 1.0f * L[14] * (x0 * y1 * z1)
+        + 1.0f * L[15] * (x0 * y0 * z2)
+        + 4.0f * L[20] * (x3 * y0 * z0)
+        + 3.0f * L[21] * (x2 * y1 * z0)
+        + 3.0f * L[22] * (x2 * y0 * z1)
+        + 2.0f * L[23] * (x1 * y2 * z0)
+        + 2.0f * L[24] * (x1 * y1 * z1)
+        + 2.0f * L[25] * (x1 * y0 * z2)
+        + 1.0f * L[26] * (x0 * y3 * z0)
+        + 1.0f * L[27] * (x0 * y2 * z1)
+        + 1.0f * L[28] * (x0 * y1 * z2)
+        + 1.0f * L[29] * (x0 * y0 * z3));
+    ey = -(1.0f * L[2] * (x0 * y0 * z0)
+        + 1.0f * L[5] * (x1 * y0 * z0)
+        + 2.0f * L[7] * (x0 * y1 * z0)
+        + 1.0f * L[8] * (x0 * y0 * z1)
+        + 1.0f * L[11] * (x2 * y0 * z0)
+        + 2.0f * L[13] * (x1 * y1 * z0)
+        + 1.0f * L[14] * (x1 * y0 * z1)
+        + 3.0f * L[16] * (x0 * y2 * z0)
+        + 2.0f * L[17] * (x0 * y1 * z1)
+        + 1.0f * L[18] * (x0 * y0 * z2)
+        + 1.0f * L[21] * (x3 * y0 * z0)
+        + 2.0f * L[23] * (x2 * y1 * z0)
+        + 1.0f * L[24] * (x2 * y0 * z1)
+        + 3.0f * L[26] * (x1 * y2 * z0)
+        + 2.0f * L[27] * (x1 * y1 * z1)
+        + 1.0f * L[28] * (x1 * y0 * z2)
+        + 4.0f * L[30] * (x0 * y3 * z0)
+        + 3.0f * L[31] * (x0 * y2 * z1)
+        + 2.0f * L[32] * (x0 * y1 * z2)
+        + 1.0f * L[33] * (x0 * y0 * z3));
+    ez = -(1.0f * L[3] * (x0 * y0 * z0)
+        + 1.0f * L[6] * (x1 * y0 * z0)
+        + 1.0f * L[8] * (x0 * y1 * z0)
+        + 2.0f * L[9] * (x0 * y0 * z1)
+        + 1.0f * L[12] * (x2 * y0 * z0)
+        + 1.0f * L[14] * (x1 * y1 * z0)
+        + 2.0f * L[15] * (x1 * y0 * z1)
+        + 1.0f * L[17] * (x0 * y2 * z0)
+        + 2.0f * L[18] * (x0 * y1 * z1)
+        + 3.0f * L[19] * (x0 * y0 * z2)
+        + 1.0f * L[22] * (x3 * y0 * z0)
+        + 1.0f * L[24] * (x2 * y1 * z0)
+        + 2.0f * L[25] * (x2 * y0 * z1)
+        + 1.0f * L[27] * (x1 * y2 * z0)
+        + 2.0f * L[28] * (x1 * y1 * z1)
+        + 3.0f * L[29] * (x1 * y0 * z2)
+        + 1.0f * L[31] * (x0 * y3 * z0)
+        + 2.0f * L[32] * (x0 * y2 * z1)
+        + 3.0f * L[33] * (x0 * y1 * z2)
+        + 4.0f * L[34] * (x0 * y0 * z3));
+}

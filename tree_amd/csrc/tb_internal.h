@@ -119,6 +119,9 @@ void tb_reduce_partials(hipStream_t s, int nparts, int width, const double *part
 void tb_walk64_list(hipStream_t s, int nt, const int32_t *tsp, const int32_t *idx, int stride, const SpeciesDev &E,
                     const SpeciesDev &I, double rs, double *fout);
 int tb_walk64_blocks(int n);
+// local.hip: the far-field local expansion of cells about one centre, evaluated at offsets
+int tb_local_run(hipStream_t s, int order, int ncell, const float *cells, int ny, const float *y, float *out);
+
 void tb_direct(hipStream_t s, int nt, const int32_t *tsp, const int32_t *tidx_sorted, const SpeciesDev &E,
                const SpeciesDev &I, double rs, double *fout);
 void tb_vv1(hipStream_t s, SpeciesDev &S, double xm, double dt);

@@ -27,6 +27,7 @@
 // group of every domain.
 #include <hip/hip_runtime.h>
 
+#include "local_exp.h"
 #include "tb_mixed.h"
 
 #include <algorithm>
@@ -38,21 +39,40 @@ namespace {
 constexpr int BT = 256;
 constexpr int WPB = BT / 64;
 constexpr int STACK = 1024;
-constexpr int LCAP = 128;   // list capacity; flushed once 64 entries are pending
-constexpr int FLUSH = 64;
+// Interaction lists are flushed once FLUSH entries are pending.  A traversal
+// step adds at most 64 accepted cells, and a learned band decision at most 64
+// more after a flush check, so LCAP = FLUSH - 1 + 64 holds any list.  The
+// queues (band cells, far cells) are flushed 64 at a time: QCAP = 63 + 64.
+// The LDS per wave (12.5 KB) keeps three 4-wave blocks on a CU.
+constexpr int FLUSH = 48;
+constexpr int LCAP = FLUSH - 1 + 64;
+constexpr int QCAP = 128;
 constexpr int NXCD = 8;       // XCDs of an MI355X (work chunks of the walk queue)
 constexpr int CTR_CHUNK = 8;  // first chunk counter in the counter block (16 ints)
 constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decided by the network
 
 const char *g_err = "";
 
+// far-field local expansion: order and coefficient count
+constexpr int FAR_ORDER = 4;
+constexpr int NLOC = FAR_ORDER == 3 ? TB_LOCAL_N3 : TB_LOCAL_N4H;
+// default threshold: cell distance / group radius.  The field's relative
+// truncation error of the order-4 expansion is about 5 (1/K)^4 at the
+// group's edge (4e-3 at K = 6), below the M2P's own error at theta ~ 0.15;
+// measured at N = 4M, K = 6 gave the fastest walk at 1e-4 rms (K = 5 needs
+// a smaller theta).
+constexpr double FAR_RATIO_DEFAULT = 6.0;
+
 struct WaveLDS {
     uint32_t stack[STACK];
     uint32_t batch[64];
-    uint32_t bandq[LCAP];                 // learned: band cells waiting for a decision
+    uint32_t bandq[QCAP];                 // learned: band cells waiting for a decision
+    uint32_t farq[QCAP];                  // far accepted cells waiting for the local expansion
     float4 m0[LCAP], m1[LCAP], m2[LCAP];  // x y z q | Qxx Qyy Qzz Qxy | Qyz Qzx - -
     float4 p[LCAP];                       // x y z -
 };
+// measured: 4 x 13056 bytes per block kept 3 blocks on a CU, 4 x 13568 did not
+static_assert(sizeof(WaveLDS) * 4 <= 4 * 13056, "walk LDS would drop a CU to two blocks");
 
 __device__ inline void wave_sync()
 {
@@ -133,6 +153,7 @@ struct WalkArgs {
     int ndom, ngt;        // domains, groups in all domains
     float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
+    float far_k2;         // square of the local-expansion threshold (distance / group radius); 0: off
     const TbMlpPack *mlpk;  // learned: the bf16 parameters packed as matrix-core operands
     BandRec *rec;         // collect: band pairs
     int rec_cap;
@@ -267,6 +288,63 @@ __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     }
 }
 
+template <int N>
+__device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q1, float4 q2, float (&L)[N])
+{
+    if constexpr (N == TB_LOCAL_N3)
+        m2l_acc_p3(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, L);
+    else
+        m2l_acc_p4h(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, L);
+}
+
+template <int N>
+__device__ inline void l2p(const float (&L)[N], float x, float y, float z, float &phi, float &ex, float &ey, float &ez)
+{
+    if constexpr (N == TB_LOCAL_N3)
+        l2p_p3(L, x, y, z, phi, ex, ey, ez);
+    else
+        l2p_p4h(L, x, y, z, phi, ex, ey, ez);
+}
+
+// Up to 64 far cells (one per lane; lanes past cnt add nothing) into the
+// group's local expansion about its centre (gx, gy, gz).  DIST = false:
+// every lane keeps its own partial coefficients Lf (summed across the
+// wavefront once, at the group's end).  DIST = true (the variants short of
+// registers): each flush is summed across the wavefront at once and lane k
+// keeps coefficient k in lacc.
+template <bool DIST>
+__device__ inline void flush_far(const WaveLDS &L, int cnt, const Domain &D, double gx, double gy, double gz,
+                                 float (&Lf)[NLOC], float &lacc)
+{
+    const int lane = lane_id();
+    float rx = 1.0f, ry = 0.0f, rz = 0.0f;
+    float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0;
+    if (lane < cnt) {
+        const uint32_t nd = L.farq[lane];
+        const double4 m = D.bmac[nd];
+        rx = (float)(gx - m.x);
+        ry = (float)(gy - m.y);
+        rz = (float)(gz - m.z);
+        q0 = D.bq0[nd];
+        q1 = D.bq1[nd];
+        q2 = D.bq2[nd];
+    }
+    if constexpr (DIST) {
+        float Lp[NLOC];
+#pragma unroll
+        for (int k = 0; k < NLOC; k++) Lp[k] = 0.0f;
+        m2l_acc(rx, ry, rz, q0, q1, q2, Lp);
+#pragma unroll
+        for (int k = 0; k < NLOC; k++) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) Lp[k] += __shfl_xor(Lp[k], o, 64);
+            lacc += lane == k ? Lp[k] : 0.0f;
+        }
+    } else {
+        m2l_acc(rx, ry, rz, q0, q1, q2, Lf);
+    }
+}
+
 // At most 3 blocks of 4 waves fit a CU's LDS, i.e. 3 waves per SIMD: the
 // register budget is capped to match (512 / 3 = 170 VGPRs).  The learned
 // walk spills a few registers around its batched band decision under that
@@ -281,6 +359,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
     const int G = 64 * TPL;
     // per target species: cells tested, M2P and like / unlike P2P interactions
     unsigned long long c_visit[2] = {0, 0}, c_m2p[2] = {0, 0}, c_like[2] = {0, 0}, c_unlike[2] = {0, 0};
+    unsigned long long c_m2l[2] = {0, 0}, c_far[2] = {0, 0};
 
     // Work queue, XCD-aware: workgroups go to the 8 XCDs round-robin, so
     // block b's waves first take groups from chunk b % 8 of the group range
@@ -374,8 +453,16 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             for (int k = 0; k < 3; k++) A.s[t][k] = A.c[t][k] = 0.0f;
             A.ps[t] = A.pc[t] = 0.0f;
         }
+        // accepted cells farther than far_k2^(1/2) group radii from the
+        // centre go into the group's local expansion (lane partials Lf)
+        const float far_r2 = W.far_k2 > 0.0f ? W.far_k2 * hdiag * hdiag : 3.0e38f;
+        constexpr bool DIST = FAR_ORDER > 3 || MODE != MODE_FIXED;
+        float Lf[NLOC], lacc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NLOC; k++) Lf[k] = 0.0f;
+        int nfar = 0;  // far cells taken by this group
 
-        int sp = 1, nm = 0, np = 0, nb = 0;  // stack, M2P, P2P, band-queue lengths
+        int sp = 1, nm = 0, np = 0, nb = 0, nf = 0;  // stack, M2P, P2P, band-queue, far-queue lengths
         L.stack[0] = 0u;  // root: range of one cell at 0
         wave_sync();
         while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
@@ -426,10 +513,13 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         const float rlo = q2.y * W.inv_theta_lo;
                         if (d2 > rlo * rlo * 1.000004f) acc = true;
                         else if (!far_hi) open = true;
-                        else if (MODE == MODE_COLLECT || q2.z <= (float)BAND_MAXCNT) band = true;
+                        else if (MODE == MODE_COLLECT ||
+                                 (q2.z <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2))
+                            band = true;
                         // larger cells are never labelled (the label sums the
-                        // cell's particles): in the learned walk the band's
-                        // geometric-mean radius decides them
+                        // cell's particles), and far cells are cheap to accept
+                        // (the group's local expansion): in the learned walk
+                        // the band's geometric-mean radius decides both
                         else if (d2 > rlo * rc * 1.000004f) acc = true;
                         else open = true;
                     }
@@ -457,7 +547,17 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 }
             }
             if (COUNT) c_visit[tsp] += (unsigned long long)__popcll(__ballot(have));
-            // accepted cells -> M2P list
+            // accepted cells -> M2P list, or the far queue
+            const bool far = acc && cx * cx + cy * cy + cz * cz > far_r2;
+            const uint64_t fk = __ballot(far);
+            if (far) L.farq[nf + __popcll(fk & lanemask_lt())] = node;
+            nf += __popcll(fk);
+            nfar += __popcll(fk);
+            if (COUNT) {
+                c_m2l[tsp] += (unsigned long long)__popcll(fk) * nact;
+                c_far[tsp] += (unsigned long long)__popcll(fk);
+            }
+            acc = acc && !far;
             uint64_t mk = __ballot(acc);
             if (acc) {
                 const int k = nm + __popcll(mk & lanemask_lt());
@@ -510,6 +610,15 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 np = 0;
                 wave_sync();
             }
+            if (nf >= 64) {
+                flush_far<DIST>(L, 64, D, gx, gy, gz, Lf, lacc);
+                // the rest of the queue (at most 63 entries) moves down
+                const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
+                wave_sync();
+                if (lane + 64 < nf) L.farq[lane] = rest;
+                nf -= 64;
+                wave_sync();
+            }
             if (MODE == MODE_LEARNED) {
                 const uint64_t bm = __ballot(band);
                 if (band) L.bandq[nb + __popcll(bm & lanemask_lt())] = node;
@@ -545,14 +654,31 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     // previous force (emin = 0) band cells are opened
                     const bool mine = lane < ne;
                     const uint32_t nd = mine ? L.bandq[lane] : 0u;
-                    const bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < W.log_err;
+                    bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < W.log_err;
                     bool bopen = mine && !bacc;
+                    // accepted band cells: far ones to the far queue, the rest to M2P
+                    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+                    if (bacc) {
+                        const double4 m = D.bmac[nd];
+                        bx = (float)(m.x - gx);
+                        by = (float)(m.y - gy);
+                        bz = (float)(m.z - gz);
+                    }
+                    const bool bfar = bacc && bx * bx + by * by + bz * bz > far_r2;
+                    const uint64_t bf = __ballot(bfar);
+                    if (bfar) L.farq[nf + __popcll(bf & lanemask_lt())] = nd;
+                    nf += __popcll(bf);
+                    nfar += __popcll(bf);
+                    if (COUNT) {
+                        c_m2l[tsp] += (unsigned long long)__popcll(bf) * nact;
+                        c_far[tsp] += (unsigned long long)__popcll(bf);
+                    }
+                    bacc = bacc && !bfar;
                     uint64_t bk = __ballot(bacc);
                     if (bacc) {
                         const int k = nm + __popcll(bk & lanemask_lt());
-                        const double4 m = D.bmac[nd];
                         const float4 q0 = D.bq0[nd], q1 = D.bq1[nd], q2 = D.bq2[nd];
-                        L.m0[k] = make_float4((float)(m.x - gx), (float)(m.y - gy), (float)(m.z - gz), q0.x);
+                        L.m0[k] = make_float4(bx, by, bz, q0.x);
                         L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
                         L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
                     }
@@ -587,6 +713,14 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         nm = 0;
                         wave_sync();
                     }
+                    if (nf >= 64) {
+                        flush_far<DIST>(L, 64, D, gx, gy, gz, Lf, lacc);
+                        const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
+                        wave_sync();
+                        if (lane + 64 < nf) L.farq[lane] = rest;
+                        nf -= 64;
+                        wave_sync();
+                    }
                 }
             }
         }
@@ -596,6 +730,28 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         }
         if (np > 0) {
             flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
+        }
+        if (nf > 0) flush_far<DIST>(L, nf, D, gx, gy, gz, Lf, lacc);
+        if (nfar > 0) {
+            // the group's local expansion, evaluated at each target
+            if constexpr (DIST) {
+#pragma unroll
+                for (int k = 0; k < NLOC; k++) Lf[k] = __shfl(lacc, k, 64);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NLOC; k++)
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) Lf[k] += __shfl_xor(Lf[k], o, 64);
+            }
+#pragma unroll
+            for (int t = 0; t < TPL; t++) {
+                float phi, ex, ey, ez;
+                l2p(Lf, rx[t], ry[t], rz[t], phi, ex, ey, ez);
+                nadd(A.s[t][0], A.c[t][0], ex);
+                nadd(A.s[t][1], A.c[t][1], ey);
+                nadd(A.s[t][2], A.c[t][2], ez);
+                nadd(A.ps[t], A.pc[t], phi);
+            }
         }
         wave_sync();
         // forces F = q_t eps E; energy 0.5 q_t eps phi (FP64 from here on)
@@ -626,6 +782,10 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         atomicAdd(&W.cnt[9], c_visit[1]);
         atomicAdd(&W.cnt[10], c_m2p[1]);
         atomicAdd(&W.cnt[11], c_like[1]);
+        atomicAdd(&W.cnt[12], c_m2l[0]);
+        atomicAdd(&W.cnt[13], c_m2l[1]);
+        atomicAdd(&W.cnt[14], c_far[0]);
+        atomicAdd(&W.cnt[15], c_far[1]);
     }
 }
 
@@ -926,6 +1086,8 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : (cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta);
     W.inv_theta_lo = (float)(1.0 / th_lo);
     W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
+    const double kfar = cfg.far_ratio == 0.0 ? FAR_RATIO_DEFAULT : cfg.far_ratio;
+    W.far_k2 = kfar > 0.0 ? (float)(kfar * kfar) : 0.0f;
     W.mlpk = M.mlp.Ppk;
     W.rec = M.rec;
     W.rec_cap = M.rec_cap;
@@ -1025,7 +1187,7 @@ int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &
 int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums,
                    int64_t *d_cnt)
 {
-    hipMemsetAsync(d_cnt, 0, 12 * sizeof(int64_t), s);
+    hipMemsetAsync(d_cnt, 0, 16 * sizeof(int64_t), s);
     return launch(s, M, sp, cfg, d_sums, d_cnt, walk_mode(M, cfg));
 }
 

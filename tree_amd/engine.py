@@ -57,6 +57,7 @@ class Config(ctypes.Structure):
         ("group", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("mem_cap", ctypes.c_int64),
+        ("far_ratio", ctypes.c_double),
     ]
 
 
@@ -72,6 +73,8 @@ class Stats(ctypes.Structure):
         ("ms_force", ctypes.c_double),
         ("ms_vv", ctypes.c_double),
         ("ms_walk", ctypes.c_double),
+        ("m2l", ctypes.c_int64 * 4),
+        ("far_pairs", ctypes.c_int64 * 2),
     ]
 
 
@@ -94,6 +97,7 @@ SIGNATURES = [
     ("tb_step", ctypes.c_int, [_P, _D, _I, _P]),
     ("tb_export_tree", ctypes.c_int, [_P, _I, _I, _P] + [_P] * 12),
     ("tb_direct_sample", ctypes.c_int, [_P, _I, _P, _P, _P]),
+    ("tb_local_eval", ctypes.c_int, [_P, _I, _I, _P, _I, _P, _P]),
     ("tb_force_targets", ctypes.c_int, [_P, _I, _P, _P, _P]),
     ("tb_time_walk", ctypes.c_int, [_P, _I, _P]),
     ("tb_count_interactions", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
@@ -164,7 +168,7 @@ class Engine:
     """One resident domain on one GPU (a tb_handle)."""
 
     def __init__(self, rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed",
-                 theta_lo=0.0, err_target=0.0, group=0, device=0, mem_cap=0):
+                 theta_lo=0.0, err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0):
         self.lib = load_library()
         self.cfg = Config()
         self.cfg.xm[0], self.cfg.xm[1] = xm
@@ -178,6 +182,7 @@ class Engine:
         self.cfg.group = group
         self.cfg.device = device
         self.cfg.mem_cap = mem_cap
+        self.cfg.far_ratio = far_ratio
         h = ctypes.c_void_p()
         _check(self.lib.tb_create(ctypes.byref(h), ctypes.byref(self.cfg)))
         self.h = h
@@ -264,6 +269,15 @@ class Engine:
         _check(self.lib.tb_direct_sample(self.h, len(tidx), _ptr(tspecies), _ptr(tidx), _ptr(f)))
         return f
 
+    def local_eval(self, order, cells, y):
+        """The far-field local expansion check: cells [n][12], offsets y [m][3] ->
+        [m][4] potential and field (FP32)."""
+        cells = np.ascontiguousarray(cells, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        out = np.zeros((len(y), 4), np.float32)
+        _check(self.lib.tb_local_eval(self.h, order, len(cells), _ptr(cells), len(y), _ptr(y), _ptr(out)))
+        return out
+
     def force_targets(self, tspecies, tidx):
         tspecies = np.ascontiguousarray(tspecies, np.int32)
         tidx = np.ascontiguousarray(tidx, np.int32)
@@ -345,11 +359,12 @@ class Engine:
 
 
 def make_config(rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed", theta_lo=0.0,
-                err_target=0.0, group=0, device=0, mem_cap=0):
+                err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0):
     c = Config()
     c.xm[0], c.xm[1] = xm
     c.rs, c.theta, c.theta_mixed, c.theta_lo, c.err_target = rs, theta, theta_mixed, theta_lo, err_target
     c.walk, c.mac, c.group, c.device, c.mem_cap = WALK[walk], MAC[mac], group, device, mem_cap
+    c.far_ratio = far_ratio
     return c
 
 
@@ -417,6 +432,7 @@ def _stats_dict(s):
         "nodes": list(s.nodes), "max_depth": list(s.max_depth),
         "visits": list(s.visits), "m2p": list(s.m2p), "p2p": list(s.p2p),
         "ms_build": s.ms_build, "ms_force": s.ms_force, "ms_vv": s.ms_vv, "ms_walk": s.ms_walk,
+        "m2l": list(s.m2l), "far_pairs": list(s.far_pairs),
     }
 
 

@@ -11,6 +11,7 @@ MODULE TREEMI_CAPI
      REAL(C_DOUBLE) :: xm(2), rs, theta, theta_mixed, theta_lo, err_target
      INTEGER(C_INT32_T) :: walk, mac, group, device
      INTEGER(C_INT64_T) :: mem_cap
+     REAL(C_DOUBLE) :: far_ratio
   END TYPE TB_CONFIG
 
   INTERFACE

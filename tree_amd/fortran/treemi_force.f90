@@ -37,6 +37,7 @@ CONTAINS
     cfg%group = 0
     cfg%device = 0
     cfg%mem_cap = 0
+    cfg%far_ratio = 0.0_C_DOUBLE
     CALL tb_check(tb_create(hnd, cfg), "Engine_Init")
   END SUBROUTINE Engine_Init
 
