@@ -82,6 +82,19 @@ __device__ inline void wave_sync()
 
 __device__ inline int lane_id() { return threadIdx.x & 63; }
 
+// A value that every lane holds alike, moved to a scalar register: the
+// per-group constants of the walk then take no vector registers, and values
+// derived from a scalar group index (the domain descriptor) load as scalars.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ inline double uni(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ inline uint64_t lanemask_lt()
 {
     int l = lane_id();
@@ -151,7 +164,7 @@ __device__ inline void kelbg_gh(float u, float &g, float &h)
 struct WalkArgs {
     const Domain *dom;    // ndom descriptors, ordered by gbase
     int ndom, ngt;        // domains, groups in all domains
-    float inv_theta_lo;   // learned / collect: cells beyond s/theta_lo are always accepted
+    float lo_scale;       // learned / collect: theta / theta_lo; cells beyond lo_scale * s/theta are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
     float far_k2;         // square of the local-expansion threshold (distance / group radius); 0: off
     const TbMlpPack *mlpk;  // learned: the bf16 parameters packed as matrix-core operands
@@ -376,7 +389,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         while (tried < NXCD) {
             const int c0 = (int)((long)W.ngt * chunk / NXCD), c1 = (int)((long)W.ngt * (chunk + 1) / NXCD);
             int v = atomicAdd(&W.ctr[CTR_CHUNK + chunk], lane == 0 ? 1 : 0);
-            v = __shfl(v, 0, 64);
+            v = uni(v);  // lane 0's count (every lane is active here)
             if (c0 + v < c1) {
                 g = c0 + v;
                 break;
@@ -431,15 +444,16 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             }
             emin = (float)(sqrt(wave_min(e2)) / TB_EPS);
         }
-        const double gx = 0.5 * (mnx + mxx), gy = 0.5 * (mny + mxy), gz = 0.5 * (mnz + mxz);
+        emin = uni(emin);
+        const double gx = uni(0.5 * (mnx + mxx)), gy = uni(0.5 * (mny + mxy)), gz = uni(0.5 * (mnz + mxz));
         // half extents, rounded outwards
-        const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f;
-        const float hy = (float)(0.5 * (mxy - mny)) * 1.000001f + 1e-30f;
-        const float hz = (float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f;
+        const float hx = uni((float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f);
+        const float hy = uni((float)(0.5 * (mxy - mny)) * 1.000001f + 1e-30f);
+        const float hz = uni((float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f);
         // per-group terms of the learned criterion's features
-        const float hdiag = sqrtf(hx * hx + hy * hy + hz * hz);
-        const float klemin = 0.125f * __log2f(fmaxf(emin, 1e-30f));
-        const float knact = 0.125f * __log2f((float)nact);
+        const float hdiag = uni(sqrtf(hx * hx + hy * hy + hz * hz));
+        const float klemin = uni(0.125f * __log2f(fmaxf(emin, 1e-30f)));
+        const float knact = uni(0.125f * __log2f((float)nact));
         float rx[TPL], ry[TPL], rz[TPL];
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
@@ -455,7 +469,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         }
         // accepted cells farther than far_k2^(1/2) group radii from the
         // centre go into the group's local expansion (lane partials Lf)
-        const float far_r2 = W.far_k2 > 0.0f ? W.far_k2 * hdiag * hdiag : 3.0e38f;
+        const float far_r2 = uni(W.far_k2 > 0.0f ? W.far_k2 * hdiag * hdiag : 3.0e38f);
         constexpr bool DIST = FAR_ORDER > 3 || MODE != MODE_FIXED;
         float Lf[NLOC], lacc = 0.0f;
 #pragma unroll
@@ -509,12 +523,11 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         acc = far_hi;
                         open = !far_hi;
                     } else {
-                        const float4 q2 = D.bq2[node];
-                        const float rlo = q2.y * W.inv_theta_lo;
+                        const float rlo = rc * W.lo_scale;  // s / theta_lo
                         if (d2 > rlo * rlo * 1.000004f) acc = true;
                         else if (!far_hi) open = true;
                         else if (MODE == MODE_COLLECT ||
-                                 (q2.z <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2))
+                                 (D.bq2[node].z <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2))
                             band = true;
                         // larger cells are never labelled (the label sums the
                         // cell's particles), and far cells are cheap to accept
@@ -1083,8 +1096,9 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     W.dom = M.ddom;
     W.ndom = nd;
     W.ngt = ngt;
-    const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : (cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta);
-    W.inv_theta_lo = (float)(1.0 / th_lo);
+    const double th_hi = cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta;
+    const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : th_hi;
+    W.lo_scale = (float)(th_hi / th_lo);
     W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
     const double kfar = cfg.far_ratio == 0.0 ? FAR_RATIO_DEFAULT : cfg.far_ratio;
     W.far_k2 = kfar > 0.0 ? (float)(kfar * kfar) : 0.0f;
