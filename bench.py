@@ -468,7 +468,7 @@ def main():
     ap.add_argument("--group", type=int, default=0, help="mixed-walk targets per group (0 = autotuned)")
     ap.add_argument("--err-target", type=float, default=0.0,
                     help="fixed error threshold for the learned criterion (skip tuning)")
-    ap.add_argument("--band", type=float, nargs=2, default=(0.7, 1.6), metavar=("LO", "HI"),
+    ap.add_argument("--band", type=float, nargs=2, default=(0.85, 1.25), metavar=("LO", "HI"),
                     help="learned band: theta_lo = LO * theta, theta_hi = HI * theta (theta: the tuned fixed walk's)")
     ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "build"), default=None,
                     help="measure another configuration instead of the headline (one JSON line)")
