@@ -74,7 +74,7 @@ typedef struct tb_config {
     double far_ratio;  /* TB_WALK_MIXED: an accepted cell farther than far_ratio    */
                        /* times the target group's radius from the group's centre */
                        /* goes through the group's local expansion instead of     */
-                       /* being evaluated at every target (0: default 6, < 0:   */
+                       /* being evaluated at every target (0: default 5, < 0:   */
                        /* off, else >= 2)                                        */
 } tb_config;
 
@@ -121,11 +121,12 @@ int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const in
 
 /*
  * Check of the mixed walk's far-field local expansion (no reference
- * counterpart; tests only).  cells [ncell][12]: R = expansion centre - cell
- * centre (3), q, D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx (traceless quadrupole).
- * The cells' local coefficients of order `order` (3 or 4) are summed across a
- * wavefront as in the walk and evaluated at the offsets y [ny][3]:
- * out [ny][4] = potential, field x, y, z (FP32).
+ * counterpart; tests only).  cells [ncell][19]: R = expansion centre - cell
+ * centre (3), q, D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx (traceless quadrupole),
+ * O'xxx, O'xxy, O'xxz, O'xyy, O'xyz, O'yyy, O'yyz (traceless third moment;
+ * order 4 only).  The cells' local coefficients of order `order` (3 or 4)
+ * are summed across a wavefront as in the walk and evaluated at the offsets
+ * y [ny][3]: out [ny][4] = potential, field x, y, z (FP32).
  */
 int tb_local_eval(tb_handle *h, int32_t order, int32_t ncell, const float *cells, int32_t ny, const float *y,
                   float *out);

@@ -16,16 +16,19 @@ index dropped):
 
     n |R|^2 T_a = -(2n - 1) sum_i R_i T_{a - e_i} - (n - 1) sum_i T_{a - 2 e_i}
 
-A cell with centre X and moments q (net charge), D (dipole) and traceless
-quadrupole Q' = 3 M - tr(M) I (M the second moment about X) produces at a
-point g + y, with R = g - X:
+A cell with centre X and moments q (net charge), D (dipole), traceless
+quadrupole Q' = 3 M - tr(M) I (M the second moment about X) and, optionally,
+the third moment O (O_ijk = sum q s_i s_j s_k, s the offset from X)
+produces at a point g + y, with R = g - X:
 
     phi(y) = sum_b L_b y^b,
-    L_b = sum_c C(b + c, b) (-1)^|c| T_{b+c}(R) M_c,   |c| <= 2,
+    L_b = sum_c C(b + c, b) (-1)^|c| T_{b+c}(R) M_c,   |c| <= 2 (or 3),
 
-with M_0 = q, M_{e_i} = D_i, M_c = Q'_c / 3 for |c| = 2 (the trace of M
-drops out because 1/r is harmonic).  The field is E = -grad phi.  Keeping
-|b| <= P gives the field through order P - 1 in y.
+with M_0 = q, M_{e_i} = D_i, M_c = Q'_c / 3 for |c| = 2 and M_c = O_c for
+|c| = 3.  Trace parts of the moment tensors drop out because 1/r is
+harmonic, so Q'/3 may stand for M and the traceless part of O for O.  The
+field is E = -grad phi.  Keeping |b| <= P gives the field through order
+P - 1 in y.
 """
 from math import comb, factorial
 
@@ -70,8 +73,9 @@ def _c(b, c):
     return comb(b[0] + c[0], b[0]) * comb(b[1] + c[1], b[1]) * comb(b[2] + c[2], b[2])
 
 
-def cell_moments(q, D, Qp):
-    """M_c for |c| <= 2 from q, D[3] and the traceless Q' (3x3, symmetric)."""
+def cell_moments(q, D, Qp, O=None):
+    """M_c for |c| <= 2 (3 with O) from q, D[3], the traceless Q' (3x3,
+    symmetric) and the third moment O (3x3x3, symmetric)."""
     Qp = np.asarray(Qp, dtype=np.float64)
     M = {(0, 0, 0): float(q)}
     for i in range(3):
@@ -81,14 +85,32 @@ def cell_moments(q, D, Qp):
     for c in multi_indices(2):
         idx = [i for i in range(3) for _ in range(c[i])]
         M[c] = Qp[idx[0], idx[1]] / 3.0
+    if O is not None:
+        O = np.asarray(O, dtype=np.float64)
+        for c in multi_indices(3):
+            idx = [i for i in range(3) for _ in range(c[i])]
+            M[c] = O[idx[0], idx[1], idx[2]]
     return M
 
 
-def m2l(X, q, D, Qp, g, P):
-    """Local coefficients L_b (|b| <= P) about g of a cell at X with moments q, D, Q'."""
+def third_moment(s, q):
+    """O_ijk = sum_j q_j s_ji s_jj s_jk of point charges at offsets s."""
+    return np.einsum("j,ji,jk,jl->ikl", q, s, s, s)
+
+
+def traceless3(O):
+    """The traceless part of a symmetric rank-3 tensor."""
+    t = np.einsum("iik->k", O)
+    d = np.eye(3)
+    return O - (np.einsum("ij,k->ijk", d, t) + np.einsum("ik,j->ijk", d, t) + np.einsum("jk,i->ijk", d, t)) / 5.0
+
+
+def m2l(X, q, D, Qp, g, P, O=None):
+    """Local coefficients L_b (|b| <= P) about g of a cell at X with moments
+    q, D, Q' (and the third moment O)."""
     R = np.asarray(g, dtype=np.float64) - np.asarray(X, dtype=np.float64)
-    T = taylor_coeffs(R, P + 2)
-    M = cell_moments(q, D, Qp)
+    T = taylor_coeffs(R, P + (3 if O is not None else 2))
+    M = cell_moments(q, D, Qp, O)
     L = {}
     for b in all_indices(P):
         s = 0.0

@@ -149,3 +149,32 @@ def test_local_coefficients_harmonic():
             c[i] += 2
             terms.append((b[i] + 2) * (b[i] + 1) * L[tuple(c)])
         assert abs(sum(terms)) <= 1e-12 * sum(abs(t) for t in terms)
+
+
+def test_octupole_local_expansion_vs_point_charges():
+    # with the third moment the cell's truncation falls to (b/d)^4
+    rng = np.random.default_rng(11)
+    X, xs, qs, q, D, Qp = _cell(rng, n=60, b=0.25)
+    O = lr.third_moment(xs - X, qs)
+    for d_scale in (1.0, 2.0):
+        g = X + d_scale * np.array([-4.0, 2.5, 1.5])
+        d = np.linalg.norm(g - X)
+        L2 = lr.m2l(X, q, D, Qp, g, 4)
+        L3 = lr.m2l(X, q, D, Qp, g, 4, O=O)
+        e2 = e3 = 0.0
+        for _ in range(10):
+            u = rng.normal(size=3)
+            x = g + 0.02 * d * u / np.linalg.norm(u)
+            dr = x - xs
+            r = np.linalg.norm(dr, axis=1)
+            E_dir = (qs[:, None] * dr / r[:, None] ** 3).sum(0)
+            e2 = max(e2, np.linalg.norm(lr.l2p(L2, x - g)[1] - E_dir))
+            e3 = max(e3, np.linalg.norm(lr.l2p(L3, x - g)[1] - E_dir))
+        scale = np.abs(qs).sum() / d ** 2
+        b = 0.25 * np.sqrt(3)
+        assert e3 < 0.5 * e2                      # the third moment helps
+        assert e3 < 20 * scale * (b / d) ** 4     # and leaves an order-4 remainder
+    # the trace of O drops out: its traceless part gives the same coefficients
+    L3t = lr.m2l(X, q, D, Qp, g, 4, O=lr.traceless3(O))
+    for k in L3:
+        assert abs(L3t[k] - L3[k]) <= 1e-12 * max(1.0, abs(L3[k]))

@@ -186,7 +186,7 @@ void free_joint(tb_handle *h, JointDev &J)
     dfree(h, J.cnt, n * 4);
     dfree(h, J.offs, n * 4);
     dfree(h, J.root, 64 * 8);
-    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 80);
+    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 8 * TB_JQ);
     dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
     J = JointDev();
 }
@@ -215,14 +215,14 @@ int ensure_joint_nodes(tb_handle *h, int nn)
     if (nn <= J.cap) return TB_OK;
     size_t c = (size_t)J.cap;
     for (int b = 0; b < 2; b++) { dfree(h, J.dkey[b], c * 8); dfree(h, J.dval[b], c * 4); }
-    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 80);
+    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 8 * TB_JQ);
     dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
     J.cap = 0;
     int rc;
     if ((rc = dalloc_t(h, &J.nc, cap))) return rc;
     if ((rc = dalloc_t(h, &J.nnext, cap))) return rc;
-    if ((rc = dalloc_t(h, &J.nq, (size_t)cap * 10))) return rc;
+    if ((rc = dalloc_t(h, &J.nq, (size_t)cap * TB_JQ))) return rc;
     if ((rc = dalloc_t(h, &J.nstart, cap))) return rc;
     if ((rc = dalloc_t(h, &J.ncount, cap))) return rc;
     if ((rc = dalloc_t(h, &J.ndepth, cap))) return rc;
@@ -874,12 +874,12 @@ int tb_local_eval(tb_handle *h, int32_t order, int32_t ncell, const float *cells
     if (ncell < 1 || ny < 1 || ncell > (1 << 20) || ny > (1 << 20))
         return fail(TB_ERR_ARG, "ncell and ny must be in 1 .. 2^20");
     HIPCHK(hipSetDevice(h->cfg.device));
-    const size_t nb = (size_t)ncell * 48 + (size_t)ny * 12 + (size_t)ny * 16 + 64;
+    const size_t nb = (size_t)ncell * TB_LOCAL_CELL * 4 + (size_t)ny * 12 + (size_t)ny * 16 + 64;
     void *tmp = nullptr;
     int rc = dalloc(h, &tmp, nb);
     if (rc) return rc;
-    float *d_c = (float *)tmp, *d_y = d_c + (size_t)ncell * 12, *d_o = d_y + (size_t)ny * 3;
-    hipError_t e = hipMemcpy(d_c, cells, (size_t)ncell * 48, hipMemcpyHostToDevice);
+    float *d_c = (float *)tmp, *d_y = d_c + (size_t)ncell * TB_LOCAL_CELL, *d_o = d_y + (size_t)ny * 3;
+    hipError_t e = hipMemcpy(d_c, cells, (size_t)ncell * TB_LOCAL_CELL * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_y, y, (size_t)ny * 12, hipMemcpyHostToDevice);
     if (e == hipSuccess && tb_local_run(h->s, order, ncell, d_c, ny, d_y, d_o) != TB_OK) e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->s);

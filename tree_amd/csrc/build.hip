@@ -682,17 +682,18 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
             nnext[id] = nxt | TB_LEAF;
             const double4 p = pos[i];
             nc[id] = make_double4(p.x, p.y, p.z, 0.0);
-            double *q = nq + (size_t)id * 10;
+            double *q = nq + (size_t)id * TB_JQ;
             q[0] = p.w;
-            for (int c = 1; c < 10; c++) q[c] = 0.0;
+            for (int c = 1; c < TB_JQ; c++) q[c] = 0.0;
         } else {
             nnext[id] = nxt;
         }
     }
 }
 
-// net charge q, dipole D and second moment M of a cell about its geometric
-// centre, shifted up from the children's own expansion centres
+// net charge q, dipole D, second moment M and third moment O of a cell about
+// its geometric centre, shifted up from the children's own expansion centres
+// (a child's offsets s become s + a, a = child centre - cell centre)
 __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restrict__ seg,
                                                 const uint64_t *__restrict__ key, const double *__restrict__ root,
                                                 double4 *__restrict__ nc, const uint32_t *__restrict__ nnext,
@@ -718,6 +719,7 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
         lz = lz * 0.5;
     }
     double q = 0.0, D[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double O[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t c = p + 1; c < endp;) {
         const uint32_t nxc = nnext[c] & ~TB_LEAF;
         if (nxc <= c) {
@@ -725,9 +727,24 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
             return;
         }
         const double4 cc = nc[c];
-        const double *cq = nq + (size_t)c * 10;
+        const double *cq = nq + (size_t)c * TB_JQ;
         const double a = cc.x - ox, b = cc.y - oy, e = cc.z - oz;
         const double qc = cq[0], dx = cq[1], dy = cq[2], dz = cq[3];
+        // O_ijk += O_c,ijk + (M_ij a_k + M_ik a_j + M_jk a_i) + (D_i a_j a_k + D_j a_i a_k + D_k a_i a_j)
+        //          + q a_i a_j a_k, with the child's M and D (before this child's own shift)
+        {
+            const double av[3] = {a, b, e}, dv[3] = {dx, dy, dz};
+            const double mm[3][3] = {{cq[4], cq[7], cq[9]}, {cq[7], cq[5], cq[8]}, {cq[9], cq[8], cq[6]}};
+            const int oi[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                   {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+#pragma unroll
+            for (int t = 0; t < 10; t++) {
+                const int i = oi[t][0], j = oi[t][1], k = oi[t][2];
+                O[t] += cq[10 + t] + mm[i][j] * av[k] + mm[i][k] * av[j] + mm[j][k] * av[i] +
+                        dv[i] * av[j] * av[k] + dv[j] * av[i] * av[k] + dv[k] * av[i] * av[j] +
+                        qc * av[i] * av[j] * av[k];
+            }
+        }
         q += qc;
         D[0] += dx + qc * a;
         D[1] += dy + qc * b;
@@ -744,10 +761,11 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
     if (ly > s) s = ly;
     if (lz > s) s = lz;
     nc[p] = make_double4(ox, oy, oz, s);
-    double *o = nq + (size_t)p * 10;
+    double *o = nq + (size_t)p * TB_JQ;
     o[0] = q;
     o[1] = D[0]; o[2] = D[1]; o[3] = D[2];
     for (int c = 0; c < 6; c++) o[4 + c] = M[c];
+    for (int c = 0; c < 10; c++) o[10 + c] = O[c];
 }
 
 // ------------------------------------------------ mixed-walk target groups

@@ -7,8 +7,9 @@ centre g, with R = g - X:
 
   T_a   Taylor coefficients of 1/|R + y| (a multi-index, |a| <= P + 2), by
         n r^2 T_a = -(2n - 1) sum_i R_i T_{a-e_i} - (n - 1) sum_i T_{a-2e_i};
-  L_b  += sum_c C(b + c, b) m_c T_{b+c}   (|b| <= P, |c| <= 2), with
-        m_0 = q, m_{e_i} = -D_i, m_c = Q'_c / 3 for |c| = 2;
+  L_b  += sum_c C(b + c, b) m_c T_{b+c}   (|b| <= P, |c| <= 2 or 3), with
+        m_0 = q, m_{e_i} = -D_i, m_c = Q'_c / 3 for |c| = 2 and
+        m_c = -O'_c for |c| = 3 (O' the traceless third moment);
   phi(y) = sum_b L_b y^b,  E = -grad phi.
 
 The same mathematics in FP64, with loops instead of unrolled code, is the
@@ -45,10 +46,11 @@ def harmonic_indices(P):
     return [b for b in all_indices(P) if b[2] <= 1]
 
 
-def emit_m2l(P, harmonic=False):
+def emit_m2l(P, harmonic=False, octupole=False):
     L = harmonic_indices(P) if harmonic else all_indices(P)
     lidx = {b: i for i, b in enumerate(L)}
-    sfx = "h" if harmonic else ""
+    sfx = ("h" if harmonic else "") + ("o" if octupole else "")
+    cmax = 3 if octupole else 2
     # moments m_c: name and expression
     mom = {(0, 0, 0): "q", (1, 0, 0): "mdx", (0, 1, 0): "mdy", (0, 0, 1): "mdz",
            (2, 0, 0): "mxx", (0, 2, 0): "myy", (0, 0, 2): "mzz",
@@ -58,14 +60,26 @@ def emit_m2l(P, harmonic=False):
     out.append("// (order P = %d; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)" % P)
     out.append("__device__ inline void m2l_acc_p%d%s(float rx, float ry, float rz, float q, float dx, float dy, "
                "float dz," % (P, sfx))
-    out.append("                                  float qxx, float qyy, float qxy, float qyz, float qzx, "
-               "float (&L)[%d])" % len(L))
+    if octupole:
+        out.append("                                  float qxx, float qyy, float qxy, float qyz, float qzx,")
+        out.append("                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, "
+                   "float oyyy, float oyyz, float (&L)[%d])" % len(L))
+    else:
+        out.append("                                  float qxx, float qyy, float qxy, float qyz, float qzx, "
+                   "float (&L)[%d])" % len(L))
     out.append("{")
     out.append("    const float r2 = rx * rx + ry * ry + rz * rz;")
     out.append("    const float mdx = -dx, mdy = -dy, mdz = -dz;")
     out.append("    const float mxx = qxx * (1.0f / 3.0f), myy = qyy * (1.0f / 3.0f), "
                "mzz = -(qxx + qyy) * (1.0f / 3.0f);")
     out.append("    const float mxy = qxy * (1.0f / 3.0f), myz = qyz * (1.0f / 3.0f), mzx = qzx * (1.0f / 3.0f);")
+    if octupole:
+        # m_c = -O'_c; the traceless third moment's z-heavy components follow from the others
+        out.append("    const float m300 = -oxxx, m210 = -oxxy, m201 = -oxxz, m120 = -oxyy, m111 = -oxyz;")
+        out.append("    const float m030 = -oyyy, m021 = -oyyz, m102 = oxxx + oxyy, m012 = oxxy + oyyy, "
+                   "m003 = oxxz + oyyz;")
+        for c in multi_indices(3):
+            mom[c] = "m%d%d%d" % c
     out.append("    const float t000 = __builtin_amdgcn_rsqf(r2);")
     out.append("    const float ir2 = t000 * t000;")
     R = ["rx", "ry", "rz"]
@@ -85,13 +99,13 @@ def emit_m2l(P, harmonic=False):
         return lines
 
     out += contrib((0, 0, 0))
-    for n in range(1, P + 3):
+    for n in range(1, P + cmax + 1):
         c1 = "%s * ir2" % fl((2 * n - 1) / n)
         c2 = "%s * ir2" % fl((n - 1) / n)
         out.append("    const float c1_%d = %s, c2_%d = %s;" % (n, c1, n, c2))
         for a in multi_indices(n):
-            if harmonic and a[2] > 3:
-                continue  # only b_z <= 1 coefficients are formed: they need T with a_z <= 3
+            if harmonic and a[2] > 1 + cmax:
+                continue  # only b_z <= 1 coefficients are formed: they need T with a_z <= 1 + |c|max
             s1 = []
             s2 = []
             for i in range(3):
@@ -189,7 +203,7 @@ def main():
     lines.append("")
     lines += emit_l2p(3)
     lines.append("")
-    lines += emit_m2l(4, harmonic=True)
+    lines += emit_m2l(4, harmonic=True, octupole=True)
     lines.append("")
     lines += emit_l2p(4, harmonic=True)
     lines.append("")

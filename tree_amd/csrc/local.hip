@@ -1,8 +1,9 @@
 // local.hip -- a direct check of the far-field local expansion
 // (local_exp.h) outside the walk: the local coefficients of a set of cells
 // about one centre, summed across a wavefront the way the walk sums them,
-// then evaluated at given offsets.  Order 3 keeps every coefficient; order 4
-// (the walk's) keeps the independent ones of the harmonic expansion.
+// then evaluated at given offsets.  Order 3 keeps every coefficient and the
+// moments up to the quadrupole; order 4 (the walk's) keeps the independent
+// coefficients of the harmonic expansion and adds the third moment.
 #include <hip/hip_runtime.h>
 
 #include "local_exp.h"
@@ -19,8 +20,10 @@ __device__ inline void wave_sum_all(float (&L)[N])
         for (int o = 32; o > 0; o >>= 1) L[k] += __shfl_xor(L[k], o, 64);
 }
 
-// cells: 12 floats each, R = centre - cell centre (3), q, D (3), Q'xx,
-// Q'yy, Q'xy, Q'yz, Q'zx.  out: phi, Ex, Ey, Ez per offset.
+// cells: 19 floats each, R = centre - cell centre (3), q, D (3), Q'xx,
+// Q'yy, Q'xy, Q'yz, Q'zx, then the traceless third moment O'xxx, O'xxy,
+// O'xxz, O'xyy, O'xyz, O'yyy, O'yyz (order 4 only).  out: phi, Ex, Ey, Ez
+// per offset.
 template <int P>
 __global__ __launch_bounds__(64) void k_local_eval(int ncell, const float *__restrict__ cells, int ny,
                                                    const float *__restrict__ y, float *__restrict__ out)
@@ -30,13 +33,13 @@ __global__ __launch_bounds__(64) void k_local_eval(int ncell, const float *__res
 #pragma unroll
     for (int k = 0; k < N; k++) L[k] = 0.0f;
     for (int c = threadIdx.x; c < ncell; c += 64) {
-        const float *e = cells + 12 * c;
+        const float *e = cells + TB_LOCAL_CELL * c;
         if constexpr (P == 3)
             m2l_acc_p3(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11],
                        reinterpret_cast<float(&)[TB_LOCAL_N3]>(L));
         else
-            m2l_acc_p4h(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11],
-                        reinterpret_cast<float(&)[TB_LOCAL_N4H]>(L));
+            m2l_acc_p4ho(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
+                         e[14], e[15], e[16], e[17], e[18], reinterpret_cast<float(&)[TB_LOCAL_N4H]>(L));
     }
     wave_sum_all(L);
     for (int j = threadIdx.x; j < ny; j += 64) {

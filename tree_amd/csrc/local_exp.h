@@ -351,13 +351,16 @@ __device__ inline void l2p_p3(const float (&L)[20], float x, float y, float z, f
 
 // adds one cell's local coefficients about the expansion centre to L[25]
 // (order P = 4; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)
-__device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float dx, float dy, float dz,
-                                  float qxx, float qyy, float qxy, float qyz, float qzx, float (&L)[25])
+__device__ inline void m2l_acc_p4ho(float rx, float ry, float rz, float q, float dx, float dy, float dz,
+                                  float qxx, float qyy, float qxy, float qyz, float qzx,
+                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, float oyyy, float oyyz, float (&L)[25])
 {
     const float r2 = rx * rx + ry * ry + rz * rz;
     const float mdx = -dx, mdy = -dy, mdz = -dz;
     const float mxx = qxx * (1.0f / 3.0f), myy = qyy * (1.0f / 3.0f), mzz = -(qxx + qyy) * (1.0f / 3.0f);
     const float mxy = qxy * (1.0f / 3.0f), myz = qyz * (1.0f / 3.0f), mzx = qzx * (1.0f / 3.0f);
+    const float m300 = -oxxx, m210 = -oxxy, m201 = -oxxz, m120 = -oxyy, m111 = -oxyz;
+    const float m030 = -oyyy, m021 = -oyyz, m102 = oxxx + oxyy, m012 = oxxy + oyyy, m003 = oxxz + oyyz;
     const float t000 = __builtin_amdgcn_rsqf(r2);
     const float ir2 = t000 * t000;
     L[0] = fmaf(q, t000, L[0]);
@@ -403,24 +406,28 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[9] = fmaf(q, t300, L[9]);
     L[4] = fmaf(3.0f * mdx, t300, L[4]);
     L[1] = fmaf(3.0f * mxx, t300, L[1]);
+    L[0] = fmaf(m300, t300, L[0]);
     const float t210 = -fmaf(c1_3, rx * t110 + ry * t200, c2_3 * (t010));
     L[10] = fmaf(q, t210, L[10]);
     L[5] = fmaf(2.0f * mdx, t210, L[5]);
     L[4] = fmaf(mdy, t210, L[4]);
     L[2] = fmaf(mxx, t210, L[2]);
     L[1] = fmaf(2.0f * mxy, t210, L[1]);
+    L[0] = fmaf(m210, t210, L[0]);
     const float t201 = -fmaf(c1_3, rx * t101 + rz * t200, c2_3 * (t001));
     L[11] = fmaf(q, t201, L[11]);
     L[6] = fmaf(2.0f * mdx, t201, L[6]);
     L[4] = fmaf(mdz, t201, L[4]);
     L[3] = fmaf(mxx, t201, L[3]);
     L[1] = fmaf(2.0f * mzx, t201, L[1]);
+    L[0] = fmaf(m201, t201, L[0]);
     const float t120 = -fmaf(c1_3, rx * t020 + ry * t110, c2_3 * (t100));
     L[12] = fmaf(q, t120, L[12]);
     L[7] = fmaf(mdx, t120, L[7]);
     L[5] = fmaf(2.0f * mdy, t120, L[5]);
     L[1] = fmaf(myy, t120, L[1]);
     L[2] = fmaf(2.0f * mxy, t120, L[2]);
+    L[0] = fmaf(m120, t120, L[0]);
     const float t111 = -c1_3 * (rx * t011 + ry * t101 + rz * t110);
     L[13] = fmaf(q, t111, L[13]);
     L[8] = fmaf(mdx, t111, L[8]);
@@ -429,43 +436,54 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[3] = fmaf(mxy, t111, L[3]);
     L[1] = fmaf(myz, t111, L[1]);
     L[2] = fmaf(mzx, t111, L[2]);
+    L[0] = fmaf(m111, t111, L[0]);
     const float t102 = -fmaf(c1_3, rx * t002 + rz * t101, c2_3 * (t100));
     L[6] = fmaf(2.0f * mdz, t102, L[6]);
     L[1] = fmaf(mzz, t102, L[1]);
     L[3] = fmaf(2.0f * mzx, t102, L[3]);
+    L[0] = fmaf(m102, t102, L[0]);
     const float t030 = -fmaf(c1_3, ry * t020, c2_3 * (t010));
     L[14] = fmaf(q, t030, L[14]);
     L[7] = fmaf(3.0f * mdy, t030, L[7]);
     L[2] = fmaf(3.0f * myy, t030, L[2]);
+    L[0] = fmaf(m030, t030, L[0]);
     const float t021 = -fmaf(c1_3, ry * t011 + rz * t020, c2_3 * (t001));
     L[15] = fmaf(q, t021, L[15]);
     L[8] = fmaf(2.0f * mdy, t021, L[8]);
     L[7] = fmaf(mdz, t021, L[7]);
     L[3] = fmaf(myy, t021, L[3]);
     L[2] = fmaf(2.0f * myz, t021, L[2]);
+    L[0] = fmaf(m021, t021, L[0]);
     const float t012 = -fmaf(c1_3, ry * t002 + rz * t011, c2_3 * (t010));
     L[8] = fmaf(2.0f * mdz, t012, L[8]);
     L[2] = fmaf(mzz, t012, L[2]);
     L[3] = fmaf(2.0f * myz, t012, L[3]);
+    L[0] = fmaf(m012, t012, L[0]);
     const float t003 = -fmaf(c1_3, rz * t002, c2_3 * (t001));
     L[3] = fmaf(3.0f * mzz, t003, L[3]);
+    L[0] = fmaf(m003, t003, L[0]);
     const float c1_4 = 1.75f * ir2, c2_4 = 0.75f * ir2;
     const float t400 = -fmaf(c1_4, rx * t300, c2_4 * (t200));
     L[16] = fmaf(q, t400, L[16]);
     L[9] = fmaf(4.0f * mdx, t400, L[9]);
     L[4] = fmaf(6.0f * mxx, t400, L[4]);
+    L[1] = fmaf(4.0f * m300, t400, L[1]);
     const float t310 = -fmaf(c1_4, rx * t210 + ry * t300, c2_4 * (t110));
     L[17] = fmaf(q, t310, L[17]);
     L[10] = fmaf(3.0f * mdx, t310, L[10]);
     L[9] = fmaf(mdy, t310, L[9]);
     L[5] = fmaf(3.0f * mxx, t310, L[5]);
     L[4] = fmaf(3.0f * mxy, t310, L[4]);
+    L[2] = fmaf(m300, t310, L[2]);
+    L[1] = fmaf(3.0f * m210, t310, L[1]);
     const float t301 = -fmaf(c1_4, rx * t201 + rz * t300, c2_4 * (t101));
     L[18] = fmaf(q, t301, L[18]);
     L[11] = fmaf(3.0f * mdx, t301, L[11]);
     L[9] = fmaf(mdz, t301, L[9]);
     L[6] = fmaf(3.0f * mxx, t301, L[6]);
     L[4] = fmaf(3.0f * mzx, t301, L[4]);
+    L[3] = fmaf(m300, t301, L[3]);
+    L[1] = fmaf(3.0f * m201, t301, L[1]);
     const float t220 = -fmaf(c1_4, rx * t120 + ry * t210, c2_4 * (t020 + t200));
     L[19] = fmaf(q, t220, L[19]);
     L[12] = fmaf(2.0f * mdx, t220, L[12]);
@@ -473,6 +491,8 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[7] = fmaf(mxx, t220, L[7]);
     L[4] = fmaf(myy, t220, L[4]);
     L[5] = fmaf(4.0f * mxy, t220, L[5]);
+    L[2] = fmaf(2.0f * m210, t220, L[2]);
+    L[1] = fmaf(2.0f * m120, t220, L[1]);
     const float t211 = -fmaf(c1_4, rx * t111 + ry * t201 + rz * t210, c2_4 * (t011));
     L[20] = fmaf(q, t211, L[20]);
     L[13] = fmaf(2.0f * mdx, t211, L[13]);
@@ -482,16 +502,23 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[6] = fmaf(2.0f * mxy, t211, L[6]);
     L[4] = fmaf(myz, t211, L[4]);
     L[5] = fmaf(2.0f * mzx, t211, L[5]);
+    L[3] = fmaf(m210, t211, L[3]);
+    L[2] = fmaf(m201, t211, L[2]);
+    L[1] = fmaf(2.0f * m111, t211, L[1]);
     const float t202 = -fmaf(c1_4, rx * t102 + rz * t201, c2_4 * (t002 + t200));
     L[11] = fmaf(2.0f * mdz, t202, L[11]);
     L[4] = fmaf(mzz, t202, L[4]);
     L[6] = fmaf(4.0f * mzx, t202, L[6]);
+    L[3] = fmaf(2.0f * m201, t202, L[3]);
+    L[1] = fmaf(2.0f * m102, t202, L[1]);
     const float t130 = -fmaf(c1_4, rx * t030 + ry * t120, c2_4 * (t110));
     L[21] = fmaf(q, t130, L[21]);
     L[14] = fmaf(mdx, t130, L[14]);
     L[12] = fmaf(3.0f * mdy, t130, L[12]);
     L[5] = fmaf(3.0f * myy, t130, L[5]);
     L[7] = fmaf(3.0f * mxy, t130, L[7]);
+    L[2] = fmaf(3.0f * m120, t130, L[2]);
+    L[1] = fmaf(m030, t130, L[1]);
     const float t121 = -fmaf(c1_4, rx * t021 + ry * t111 + rz * t120, c2_4 * (t101));
     L[22] = fmaf(q, t121, L[22]);
     L[15] = fmaf(mdx, t121, L[15]);
@@ -501,49 +528,74 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[8] = fmaf(2.0f * mxy, t121, L[8]);
     L[5] = fmaf(2.0f * myz, t121, L[5]);
     L[7] = fmaf(mzx, t121, L[7]);
+    L[3] = fmaf(m120, t121, L[3]);
+    L[2] = fmaf(2.0f * m111, t121, L[2]);
+    L[1] = fmaf(m021, t121, L[1]);
     const float t112 = -fmaf(c1_4, rx * t012 + ry * t102 + rz * t111, c2_4 * (t110));
     L[13] = fmaf(2.0f * mdz, t112, L[13]);
     L[5] = fmaf(mzz, t112, L[5]);
     L[6] = fmaf(2.0f * myz, t112, L[6]);
     L[8] = fmaf(2.0f * mzx, t112, L[8]);
+    L[3] = fmaf(2.0f * m111, t112, L[3]);
+    L[2] = fmaf(m102, t112, L[2]);
+    L[1] = fmaf(m012, t112, L[1]);
     const float t103 = -fmaf(c1_4, rx * t003 + rz * t102, c2_4 * (t101));
     L[6] = fmaf(3.0f * mzz, t103, L[6]);
+    L[3] = fmaf(3.0f * m102, t103, L[3]);
+    L[1] = fmaf(m003, t103, L[1]);
     const float t040 = -fmaf(c1_4, ry * t030, c2_4 * (t020));
     L[23] = fmaf(q, t040, L[23]);
     L[14] = fmaf(4.0f * mdy, t040, L[14]);
     L[7] = fmaf(6.0f * myy, t040, L[7]);
+    L[2] = fmaf(4.0f * m030, t040, L[2]);
     const float t031 = -fmaf(c1_4, ry * t021 + rz * t030, c2_4 * (t011));
     L[24] = fmaf(q, t031, L[24]);
     L[15] = fmaf(3.0f * mdy, t031, L[15]);
     L[14] = fmaf(mdz, t031, L[14]);
     L[8] = fmaf(3.0f * myy, t031, L[8]);
     L[7] = fmaf(3.0f * myz, t031, L[7]);
+    L[3] = fmaf(m030, t031, L[3]);
+    L[2] = fmaf(3.0f * m021, t031, L[2]);
     const float t022 = -fmaf(c1_4, ry * t012 + rz * t021, c2_4 * (t002 + t020));
     L[15] = fmaf(2.0f * mdz, t022, L[15]);
     L[7] = fmaf(mzz, t022, L[7]);
     L[8] = fmaf(4.0f * myz, t022, L[8]);
+    L[3] = fmaf(2.0f * m021, t022, L[3]);
+    L[2] = fmaf(2.0f * m012, t022, L[2]);
     const float t013 = -fmaf(c1_4, ry * t003 + rz * t012, c2_4 * (t011));
     L[8] = fmaf(3.0f * mzz, t013, L[8]);
+    L[3] = fmaf(3.0f * m012, t013, L[3]);
+    L[2] = fmaf(m003, t013, L[2]);
+    const float t004 = -fmaf(c1_4, rz * t003, c2_4 * (t002));
+    L[3] = fmaf(4.0f * m003, t004, L[3]);
     const float c1_5 = 1.8f * ir2, c2_5 = 0.8f * ir2;
     const float t500 = -fmaf(c1_5, rx * t400, c2_5 * (t300));
     L[16] = fmaf(5.0f * mdx, t500, L[16]);
     L[9] = fmaf(10.0f * mxx, t500, L[9]);
+    L[4] = fmaf(10.0f * m300, t500, L[4]);
     const float t410 = -fmaf(c1_5, rx * t310 + ry * t400, c2_5 * (t210));
     L[17] = fmaf(4.0f * mdx, t410, L[17]);
     L[16] = fmaf(mdy, t410, L[16]);
     L[10] = fmaf(6.0f * mxx, t410, L[10]);
     L[9] = fmaf(4.0f * mxy, t410, L[9]);
+    L[5] = fmaf(4.0f * m300, t410, L[5]);
+    L[4] = fmaf(6.0f * m210, t410, L[4]);
     const float t401 = -fmaf(c1_5, rx * t301 + rz * t400, c2_5 * (t201));
     L[18] = fmaf(4.0f * mdx, t401, L[18]);
     L[16] = fmaf(mdz, t401, L[16]);
     L[11] = fmaf(6.0f * mxx, t401, L[11]);
     L[9] = fmaf(4.0f * mzx, t401, L[9]);
+    L[6] = fmaf(4.0f * m300, t401, L[6]);
+    L[4] = fmaf(6.0f * m201, t401, L[4]);
     const float t320 = -fmaf(c1_5, rx * t220 + ry * t310, c2_5 * (t120 + t300));
     L[19] = fmaf(3.0f * mdx, t320, L[19]);
     L[17] = fmaf(2.0f * mdy, t320, L[17]);
     L[12] = fmaf(3.0f * mxx, t320, L[12]);
     L[9] = fmaf(myy, t320, L[9]);
     L[10] = fmaf(6.0f * mxy, t320, L[10]);
+    L[7] = fmaf(m300, t320, L[7]);
+    L[5] = fmaf(6.0f * m210, t320, L[5]);
+    L[4] = fmaf(3.0f * m120, t320, L[4]);
     const float t311 = -fmaf(c1_5, rx * t211 + ry * t301 + rz * t310, c2_5 * (t111));
     L[20] = fmaf(3.0f * mdx, t311, L[20]);
     L[18] = fmaf(mdy, t311, L[18]);
@@ -552,16 +604,25 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[11] = fmaf(3.0f * mxy, t311, L[11]);
     L[9] = fmaf(myz, t311, L[9]);
     L[10] = fmaf(3.0f * mzx, t311, L[10]);
+    L[8] = fmaf(m300, t311, L[8]);
+    L[6] = fmaf(3.0f * m210, t311, L[6]);
+    L[5] = fmaf(3.0f * m201, t311, L[5]);
+    L[4] = fmaf(3.0f * m111, t311, L[4]);
     const float t302 = -fmaf(c1_5, rx * t202 + rz * t301, c2_5 * (t102 + t300));
     L[18] = fmaf(2.0f * mdz, t302, L[18]);
     L[9] = fmaf(mzz, t302, L[9]);
     L[11] = fmaf(6.0f * mzx, t302, L[11]);
+    L[6] = fmaf(6.0f * m201, t302, L[6]);
+    L[4] = fmaf(3.0f * m102, t302, L[4]);
     const float t230 = -fmaf(c1_5, rx * t130 + ry * t220, c2_5 * (t030 + t210));
     L[21] = fmaf(2.0f * mdx, t230, L[21]);
     L[19] = fmaf(3.0f * mdy, t230, L[19]);
     L[14] = fmaf(mxx, t230, L[14]);
     L[10] = fmaf(3.0f * myy, t230, L[10]);
     L[12] = fmaf(6.0f * mxy, t230, L[12]);
+    L[7] = fmaf(3.0f * m210, t230, L[7]);
+    L[5] = fmaf(6.0f * m120, t230, L[5]);
+    L[4] = fmaf(m030, t230, L[4]);
     const float t221 = -fmaf(c1_5, rx * t121 + ry * t211 + rz * t220, c2_5 * (t021 + t201));
     L[22] = fmaf(2.0f * mdx, t221, L[22]);
     L[20] = fmaf(2.0f * mdy, t221, L[20]);
@@ -571,18 +632,31 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[13] = fmaf(4.0f * mxy, t221, L[13]);
     L[10] = fmaf(2.0f * myz, t221, L[10]);
     L[12] = fmaf(2.0f * mzx, t221, L[12]);
+    L[8] = fmaf(2.0f * m210, t221, L[8]);
+    L[7] = fmaf(m201, t221, L[7]);
+    L[6] = fmaf(2.0f * m120, t221, L[6]);
+    L[5] = fmaf(4.0f * m111, t221, L[5]);
+    L[4] = fmaf(m021, t221, L[4]);
     const float t212 = -fmaf(c1_5, rx * t112 + ry * t202 + rz * t211, c2_5 * (t012 + t210));
     L[20] = fmaf(2.0f * mdz, t212, L[20]);
     L[10] = fmaf(mzz, t212, L[10]);
     L[11] = fmaf(2.0f * myz, t212, L[11]);
     L[13] = fmaf(4.0f * mzx, t212, L[13]);
+    L[8] = fmaf(2.0f * m201, t212, L[8]);
+    L[6] = fmaf(4.0f * m111, t212, L[6]);
+    L[5] = fmaf(2.0f * m102, t212, L[5]);
+    L[4] = fmaf(m012, t212, L[4]);
     const float t203 = -fmaf(c1_5, rx * t103 + rz * t202, c2_5 * (t003 + t201));
     L[11] = fmaf(3.0f * mzz, t203, L[11]);
+    L[6] = fmaf(6.0f * m102, t203, L[6]);
+    L[4] = fmaf(m003, t203, L[4]);
     const float t140 = -fmaf(c1_5, rx * t040 + ry * t130, c2_5 * (t120));
     L[23] = fmaf(mdx, t140, L[23]);
     L[21] = fmaf(4.0f * mdy, t140, L[21]);
     L[12] = fmaf(6.0f * myy, t140, L[12]);
     L[14] = fmaf(4.0f * mxy, t140, L[14]);
+    L[7] = fmaf(6.0f * m120, t140, L[7]);
+    L[5] = fmaf(4.0f * m030, t140, L[5]);
     const float t131 = -fmaf(c1_5, rx * t031 + ry * t121 + rz * t130, c2_5 * (t111));
     L[24] = fmaf(mdx, t131, L[24]);
     L[22] = fmaf(3.0f * mdy, t131, L[22]);
@@ -591,104 +665,320 @@ __device__ inline void m2l_acc_p4h(float rx, float ry, float rz, float q, float 
     L[15] = fmaf(3.0f * mxy, t131, L[15]);
     L[12] = fmaf(3.0f * myz, t131, L[12]);
     L[14] = fmaf(mzx, t131, L[14]);
+    L[8] = fmaf(3.0f * m120, t131, L[8]);
+    L[7] = fmaf(3.0f * m111, t131, L[7]);
+    L[6] = fmaf(m030, t131, L[6]);
+    L[5] = fmaf(3.0f * m021, t131, L[5]);
     const float t122 = -fmaf(c1_5, rx * t022 + ry * t112 + rz * t121, c2_5 * (t102 + t120));
     L[22] = fmaf(2.0f * mdz, t122, L[22]);
     L[12] = fmaf(mzz, t122, L[12]);
     L[13] = fmaf(4.0f * myz, t122, L[13]);
     L[15] = fmaf(2.0f * mzx, t122, L[15]);
+    L[8] = fmaf(4.0f * m111, t122, L[8]);
+    L[7] = fmaf(m102, t122, L[7]);
+    L[6] = fmaf(2.0f * m021, t122, L[6]);
+    L[5] = fmaf(2.0f * m012, t122, L[5]);
     const float t113 = -fmaf(c1_5, rx * t013 + ry * t103 + rz * t112, c2_5 * (t111));
     L[13] = fmaf(3.0f * mzz, t113, L[13]);
+    L[8] = fmaf(3.0f * m102, t113, L[8]);
+    L[6] = fmaf(3.0f * m012, t113, L[6]);
+    L[5] = fmaf(m003, t113, L[5]);
+    const float t104 = -fmaf(c1_5, rx * t004 + rz * t103, c2_5 * (t102));
+    L[6] = fmaf(4.0f * m003, t104, L[6]);
     const float t050 = -fmaf(c1_5, ry * t040, c2_5 * (t030));
     L[23] = fmaf(5.0f * mdy, t050, L[23]);
     L[14] = fmaf(10.0f * myy, t050, L[14]);
+    L[7] = fmaf(10.0f * m030, t050, L[7]);
     const float t041 = -fmaf(c1_5, ry * t031 + rz * t040, c2_5 * (t021));
     L[24] = fmaf(4.0f * mdy, t041, L[24]);
     L[23] = fmaf(mdz, t041, L[23]);
     L[15] = fmaf(6.0f * myy, t041, L[15]);
     L[14] = fmaf(4.0f * myz, t041, L[14]);
+    L[8] = fmaf(4.0f * m030, t041, L[8]);
+    L[7] = fmaf(6.0f * m021, t041, L[7]);
     const float t032 = -fmaf(c1_5, ry * t022 + rz * t031, c2_5 * (t012 + t030));
     L[24] = fmaf(2.0f * mdz, t032, L[24]);
     L[14] = fmaf(mzz, t032, L[14]);
     L[15] = fmaf(6.0f * myz, t032, L[15]);
+    L[8] = fmaf(6.0f * m021, t032, L[8]);
+    L[7] = fmaf(3.0f * m012, t032, L[7]);
     const float t023 = -fmaf(c1_5, ry * t013 + rz * t022, c2_5 * (t003 + t021));
     L[15] = fmaf(3.0f * mzz, t023, L[15]);
+    L[8] = fmaf(6.0f * m012, t023, L[8]);
+    L[7] = fmaf(m003, t023, L[7]);
+    const float t014 = -fmaf(c1_5, ry * t004 + rz * t013, c2_5 * (t012));
+    L[8] = fmaf(4.0f * m003, t014, L[8]);
     const float c1_6 = 1.8333333333333333f * ir2, c2_6 = 0.8333333333333334f * ir2;
     const float t600 = -fmaf(c1_6, rx * t500, c2_6 * (t400));
     L[16] = fmaf(15.0f * mxx, t600, L[16]);
+    L[9] = fmaf(20.0f * m300, t600, L[9]);
     const float t510 = -fmaf(c1_6, rx * t410 + ry * t500, c2_6 * (t310));
     L[17] = fmaf(10.0f * mxx, t510, L[17]);
     L[16] = fmaf(5.0f * mxy, t510, L[16]);
+    L[10] = fmaf(10.0f * m300, t510, L[10]);
+    L[9] = fmaf(10.0f * m210, t510, L[9]);
     const float t501 = -fmaf(c1_6, rx * t401 + rz * t500, c2_6 * (t301));
     L[18] = fmaf(10.0f * mxx, t501, L[18]);
     L[16] = fmaf(5.0f * mzx, t501, L[16]);
+    L[11] = fmaf(10.0f * m300, t501, L[11]);
+    L[9] = fmaf(10.0f * m201, t501, L[9]);
     const float t420 = -fmaf(c1_6, rx * t320 + ry * t410, c2_6 * (t220 + t400));
     L[19] = fmaf(6.0f * mxx, t420, L[19]);
     L[16] = fmaf(myy, t420, L[16]);
     L[17] = fmaf(8.0f * mxy, t420, L[17]);
+    L[12] = fmaf(4.0f * m300, t420, L[12]);
+    L[10] = fmaf(12.0f * m210, t420, L[10]);
+    L[9] = fmaf(4.0f * m120, t420, L[9]);
     const float t411 = -fmaf(c1_6, rx * t311 + ry * t401 + rz * t410, c2_6 * (t211));
     L[20] = fmaf(6.0f * mxx, t411, L[20]);
     L[18] = fmaf(4.0f * mxy, t411, L[18]);
     L[16] = fmaf(myz, t411, L[16]);
     L[17] = fmaf(4.0f * mzx, t411, L[17]);
+    L[13] = fmaf(4.0f * m300, t411, L[13]);
+    L[11] = fmaf(6.0f * m210, t411, L[11]);
+    L[10] = fmaf(6.0f * m201, t411, L[10]);
+    L[9] = fmaf(4.0f * m111, t411, L[9]);
     const float t402 = -fmaf(c1_6, rx * t302 + rz * t401, c2_6 * (t202 + t400));
     L[16] = fmaf(mzz, t402, L[16]);
     L[18] = fmaf(8.0f * mzx, t402, L[18]);
+    L[11] = fmaf(12.0f * m201, t402, L[11]);
+    L[9] = fmaf(4.0f * m102, t402, L[9]);
     const float t330 = -fmaf(c1_6, rx * t230 + ry * t320, c2_6 * (t130 + t310));
     L[21] = fmaf(3.0f * mxx, t330, L[21]);
     L[17] = fmaf(3.0f * myy, t330, L[17]);
     L[19] = fmaf(9.0f * mxy, t330, L[19]);
+    L[14] = fmaf(m300, t330, L[14]);
+    L[12] = fmaf(9.0f * m210, t330, L[12]);
+    L[10] = fmaf(9.0f * m120, t330, L[10]);
+    L[9] = fmaf(m030, t330, L[9]);
     const float t321 = -fmaf(c1_6, rx * t221 + ry * t311 + rz * t320, c2_6 * (t121 + t301));
     L[22] = fmaf(3.0f * mxx, t321, L[22]);
     L[18] = fmaf(myy, t321, L[18]);
     L[20] = fmaf(6.0f * mxy, t321, L[20]);
     L[17] = fmaf(2.0f * myz, t321, L[17]);
     L[19] = fmaf(3.0f * mzx, t321, L[19]);
+    L[15] = fmaf(m300, t321, L[15]);
+    L[13] = fmaf(6.0f * m210, t321, L[13]);
+    L[12] = fmaf(3.0f * m201, t321, L[12]);
+    L[11] = fmaf(3.0f * m120, t321, L[11]);
+    L[10] = fmaf(6.0f * m111, t321, L[10]);
+    L[9] = fmaf(m021, t321, L[9]);
     const float t312 = -fmaf(c1_6, rx * t212 + ry * t302 + rz * t311, c2_6 * (t112 + t310));
     L[17] = fmaf(mzz, t312, L[17]);
     L[18] = fmaf(2.0f * myz, t312, L[18]);
     L[20] = fmaf(6.0f * mzx, t312, L[20]);
+    L[13] = fmaf(6.0f * m201, t312, L[13]);
+    L[11] = fmaf(6.0f * m111, t312, L[11]);
+    L[10] = fmaf(3.0f * m102, t312, L[10]);
+    L[9] = fmaf(m012, t312, L[9]);
     const float t303 = -fmaf(c1_6, rx * t203 + rz * t302, c2_6 * (t103 + t301));
     L[18] = fmaf(3.0f * mzz, t303, L[18]);
+    L[11] = fmaf(9.0f * m102, t303, L[11]);
+    L[9] = fmaf(m003, t303, L[9]);
     const float t240 = -fmaf(c1_6, rx * t140 + ry * t230, c2_6 * (t040 + t220));
     L[23] = fmaf(mxx, t240, L[23]);
     L[19] = fmaf(6.0f * myy, t240, L[19]);
     L[21] = fmaf(8.0f * mxy, t240, L[21]);
+    L[14] = fmaf(4.0f * m210, t240, L[14]);
+    L[12] = fmaf(12.0f * m120, t240, L[12]);
+    L[10] = fmaf(4.0f * m030, t240, L[10]);
     const float t231 = -fmaf(c1_6, rx * t131 + ry * t221 + rz * t230, c2_6 * (t031 + t211));
     L[24] = fmaf(mxx, t231, L[24]);
     L[20] = fmaf(3.0f * myy, t231, L[20]);
     L[22] = fmaf(6.0f * mxy, t231, L[22]);
     L[19] = fmaf(3.0f * myz, t231, L[19]);
     L[21] = fmaf(2.0f * mzx, t231, L[21]);
+    L[15] = fmaf(3.0f * m210, t231, L[15]);
+    L[14] = fmaf(m201, t231, L[14]);
+    L[13] = fmaf(6.0f * m120, t231, L[13]);
+    L[12] = fmaf(6.0f * m111, t231, L[12]);
+    L[11] = fmaf(m030, t231, L[11]);
+    L[10] = fmaf(3.0f * m021, t231, L[10]);
     const float t222 = -fmaf(c1_6, rx * t122 + ry * t212 + rz * t221, c2_6 * (t022 + t202 + t220));
     L[19] = fmaf(mzz, t222, L[19]);
     L[20] = fmaf(4.0f * myz, t222, L[20]);
     L[22] = fmaf(4.0f * mzx, t222, L[22]);
+    L[15] = fmaf(2.0f * m201, t222, L[15]);
+    L[13] = fmaf(8.0f * m111, t222, L[13]);
+    L[12] = fmaf(2.0f * m102, t222, L[12]);
+    L[11] = fmaf(2.0f * m021, t222, L[11]);
+    L[10] = fmaf(2.0f * m012, t222, L[10]);
     const float t213 = -fmaf(c1_6, rx * t113 + ry * t203 + rz * t212, c2_6 * (t013 + t211));
     L[20] = fmaf(3.0f * mzz, t213, L[20]);
+    L[13] = fmaf(6.0f * m102, t213, L[13]);
+    L[11] = fmaf(3.0f * m012, t213, L[11]);
+    L[10] = fmaf(m003, t213, L[10]);
+    const float t204 = -fmaf(c1_6, rx * t104 + rz * t203, c2_6 * (t004 + t202));
+    L[11] = fmaf(4.0f * m003, t204, L[11]);
     const float t150 = -fmaf(c1_6, rx * t050 + ry * t140, c2_6 * (t130));
     L[21] = fmaf(10.0f * myy, t150, L[21]);
     L[23] = fmaf(5.0f * mxy, t150, L[23]);
+    L[14] = fmaf(10.0f * m120, t150, L[14]);
+    L[12] = fmaf(10.0f * m030, t150, L[12]);
     const float t141 = -fmaf(c1_6, rx * t041 + ry * t131 + rz * t140, c2_6 * (t121));
     L[22] = fmaf(6.0f * myy, t141, L[22]);
     L[24] = fmaf(4.0f * mxy, t141, L[24]);
     L[21] = fmaf(4.0f * myz, t141, L[21]);
     L[23] = fmaf(mzx, t141, L[23]);
+    L[15] = fmaf(6.0f * m120, t141, L[15]);
+    L[14] = fmaf(4.0f * m111, t141, L[14]);
+    L[13] = fmaf(4.0f * m030, t141, L[13]);
+    L[12] = fmaf(6.0f * m021, t141, L[12]);
     const float t132 = -fmaf(c1_6, rx * t032 + ry * t122 + rz * t131, c2_6 * (t112 + t130));
     L[21] = fmaf(mzz, t132, L[21]);
     L[22] = fmaf(6.0f * myz, t132, L[22]);
     L[24] = fmaf(2.0f * mzx, t132, L[24]);
+    L[15] = fmaf(6.0f * m111, t132, L[15]);
+    L[14] = fmaf(m102, t132, L[14]);
+    L[13] = fmaf(6.0f * m021, t132, L[13]);
+    L[12] = fmaf(3.0f * m012, t132, L[12]);
     const float t123 = -fmaf(c1_6, rx * t023 + ry * t113 + rz * t122, c2_6 * (t103 + t121));
     L[22] = fmaf(3.0f * mzz, t123, L[22]);
+    L[15] = fmaf(3.0f * m102, t123, L[15]);
+    L[13] = fmaf(6.0f * m012, t123, L[13]);
+    L[12] = fmaf(m003, t123, L[12]);
+    const float t114 = -fmaf(c1_6, rx * t014 + ry * t104 + rz * t113, c2_6 * (t112));
+    L[13] = fmaf(4.0f * m003, t114, L[13]);
     const float t060 = -fmaf(c1_6, ry * t050, c2_6 * (t040));
     L[23] = fmaf(15.0f * myy, t060, L[23]);
+    L[14] = fmaf(20.0f * m030, t060, L[14]);
     const float t051 = -fmaf(c1_6, ry * t041 + rz * t050, c2_6 * (t031));
     L[24] = fmaf(10.0f * myy, t051, L[24]);
     L[23] = fmaf(5.0f * myz, t051, L[23]);
+    L[15] = fmaf(10.0f * m030, t051, L[15]);
+    L[14] = fmaf(10.0f * m021, t051, L[14]);
     const float t042 = -fmaf(c1_6, ry * t032 + rz * t041, c2_6 * (t022 + t040));
     L[23] = fmaf(mzz, t042, L[23]);
     L[24] = fmaf(8.0f * myz, t042, L[24]);
+    L[15] = fmaf(12.0f * m021, t042, L[15]);
+    L[14] = fmaf(4.0f * m012, t042, L[14]);
     const float t033 = -fmaf(c1_6, ry * t023 + rz * t032, c2_6 * (t013 + t031));
     L[24] = fmaf(3.0f * mzz, t033, L[24]);
+    L[15] = fmaf(9.0f * m012, t033, L[15]);
+    L[14] = fmaf(m003, t033, L[14]);
+    const float t024 = -fmaf(c1_6, ry * t014 + rz * t023, c2_6 * (t004 + t022));
+    L[15] = fmaf(4.0f * m003, t024, L[15]);
+    const float c1_7 = 1.8571428571428572f * ir2, c2_7 = 0.8571428571428571f * ir2;
+    const float t700 = -fmaf(c1_7, rx * t600, c2_7 * (t500));
+    L[16] = fmaf(35.0f * m300, t700, L[16]);
+    const float t610 = -fmaf(c1_7, rx * t510 + ry * t600, c2_7 * (t410));
+    L[17] = fmaf(20.0f * m300, t610, L[17]);
+    L[16] = fmaf(15.0f * m210, t610, L[16]);
+    const float t601 = -fmaf(c1_7, rx * t501 + rz * t600, c2_7 * (t401));
+    L[18] = fmaf(20.0f * m300, t601, L[18]);
+    L[16] = fmaf(15.0f * m201, t601, L[16]);
+    const float t520 = -fmaf(c1_7, rx * t420 + ry * t510, c2_7 * (t320 + t500));
+    L[19] = fmaf(10.0f * m300, t520, L[19]);
+    L[17] = fmaf(20.0f * m210, t520, L[17]);
+    L[16] = fmaf(5.0f * m120, t520, L[16]);
+    const float t511 = -fmaf(c1_7, rx * t411 + ry * t501 + rz * t510, c2_7 * (t311));
+    L[20] = fmaf(10.0f * m300, t511, L[20]);
+    L[18] = fmaf(10.0f * m210, t511, L[18]);
+    L[17] = fmaf(10.0f * m201, t511, L[17]);
+    L[16] = fmaf(5.0f * m111, t511, L[16]);
+    const float t502 = -fmaf(c1_7, rx * t402 + rz * t501, c2_7 * (t302 + t500));
+    L[18] = fmaf(20.0f * m201, t502, L[18]);
+    L[16] = fmaf(5.0f * m102, t502, L[16]);
+    const float t430 = -fmaf(c1_7, rx * t330 + ry * t420, c2_7 * (t230 + t410));
+    L[21] = fmaf(4.0f * m300, t430, L[21]);
+    L[19] = fmaf(18.0f * m210, t430, L[19]);
+    L[17] = fmaf(12.0f * m120, t430, L[17]);
+    L[16] = fmaf(m030, t430, L[16]);
+    const float t421 = -fmaf(c1_7, rx * t321 + ry * t411 + rz * t420, c2_7 * (t221 + t401));
+    L[22] = fmaf(4.0f * m300, t421, L[22]);
+    L[20] = fmaf(12.0f * m210, t421, L[20]);
+    L[19] = fmaf(6.0f * m201, t421, L[19]);
+    L[18] = fmaf(4.0f * m120, t421, L[18]);
+    L[17] = fmaf(8.0f * m111, t421, L[17]);
+    L[16] = fmaf(m021, t421, L[16]);
+    const float t412 = -fmaf(c1_7, rx * t312 + ry * t402 + rz * t411, c2_7 * (t212 + t410));
+    L[20] = fmaf(12.0f * m201, t412, L[20]);
+    L[18] = fmaf(8.0f * m111, t412, L[18]);
+    L[17] = fmaf(4.0f * m102, t412, L[17]);
+    L[16] = fmaf(m012, t412, L[16]);
+    const float t403 = -fmaf(c1_7, rx * t303 + rz * t402, c2_7 * (t203 + t401));
+    L[18] = fmaf(12.0f * m102, t403, L[18]);
+    L[16] = fmaf(m003, t403, L[16]);
+    const float t340 = -fmaf(c1_7, rx * t240 + ry * t330, c2_7 * (t140 + t320));
+    L[23] = fmaf(m300, t340, L[23]);
+    L[21] = fmaf(12.0f * m210, t340, L[21]);
+    L[19] = fmaf(18.0f * m120, t340, L[19]);
+    L[17] = fmaf(4.0f * m030, t340, L[17]);
+    const float t331 = -fmaf(c1_7, rx * t231 + ry * t321 + rz * t330, c2_7 * (t131 + t311));
+    L[24] = fmaf(m300, t331, L[24]);
+    L[22] = fmaf(9.0f * m210, t331, L[22]);
+    L[21] = fmaf(3.0f * m201, t331, L[21]);
+    L[20] = fmaf(9.0f * m120, t331, L[20]);
+    L[19] = fmaf(9.0f * m111, t331, L[19]);
+    L[18] = fmaf(m030, t331, L[18]);
+    L[17] = fmaf(3.0f * m021, t331, L[17]);
+    const float t322 = -fmaf(c1_7, rx * t222 + ry * t312 + rz * t321, c2_7 * (t122 + t302 + t320));
+    L[22] = fmaf(6.0f * m201, t322, L[22]);
+    L[20] = fmaf(12.0f * m111, t322, L[20]);
+    L[19] = fmaf(3.0f * m102, t322, L[19]);
+    L[18] = fmaf(2.0f * m021, t322, L[18]);
+    L[17] = fmaf(2.0f * m012, t322, L[17]);
+    const float t313 = -fmaf(c1_7, rx * t213 + ry * t303 + rz * t312, c2_7 * (t113 + t311));
+    L[20] = fmaf(9.0f * m102, t313, L[20]);
+    L[18] = fmaf(3.0f * m012, t313, L[18]);
+    L[17] = fmaf(m003, t313, L[17]);
+    const float t304 = -fmaf(c1_7, rx * t204 + rz * t303, c2_7 * (t104 + t302));
+    L[18] = fmaf(4.0f * m003, t304, L[18]);
+    const float t250 = -fmaf(c1_7, rx * t150 + ry * t240, c2_7 * (t050 + t230));
+    L[23] = fmaf(5.0f * m210, t250, L[23]);
+    L[21] = fmaf(20.0f * m120, t250, L[21]);
+    L[19] = fmaf(10.0f * m030, t250, L[19]);
+    const float t241 = -fmaf(c1_7, rx * t141 + ry * t231 + rz * t240, c2_7 * (t041 + t221));
+    L[24] = fmaf(4.0f * m210, t241, L[24]);
+    L[23] = fmaf(m201, t241, L[23]);
+    L[22] = fmaf(12.0f * m120, t241, L[22]);
+    L[21] = fmaf(8.0f * m111, t241, L[21]);
+    L[20] = fmaf(4.0f * m030, t241, L[20]);
+    L[19] = fmaf(6.0f * m021, t241, L[19]);
+    const float t232 = -fmaf(c1_7, rx * t132 + ry * t222 + rz * t231, c2_7 * (t032 + t212 + t230));
+    L[24] = fmaf(2.0f * m201, t232, L[24]);
+    L[22] = fmaf(12.0f * m111, t232, L[22]);
+    L[21] = fmaf(2.0f * m102, t232, L[21]);
+    L[20] = fmaf(6.0f * m021, t232, L[20]);
+    L[19] = fmaf(3.0f * m012, t232, L[19]);
+    const float t223 = -fmaf(c1_7, rx * t123 + ry * t213 + rz * t222, c2_7 * (t023 + t203 + t221));
+    L[22] = fmaf(6.0f * m102, t223, L[22]);
+    L[20] = fmaf(6.0f * m012, t223, L[20]);
+    L[19] = fmaf(m003, t223, L[19]);
+    const float t214 = -fmaf(c1_7, rx * t114 + ry * t204 + rz * t213, c2_7 * (t014 + t212));
+    L[20] = fmaf(4.0f * m003, t214, L[20]);
+    const float t160 = -fmaf(c1_7, rx * t060 + ry * t150, c2_7 * (t140));
+    L[23] = fmaf(15.0f * m120, t160, L[23]);
+    L[21] = fmaf(20.0f * m030, t160, L[21]);
+    const float t151 = -fmaf(c1_7, rx * t051 + ry * t141 + rz * t150, c2_7 * (t131));
+    L[24] = fmaf(10.0f * m120, t151, L[24]);
+    L[23] = fmaf(5.0f * m111, t151, L[23]);
+    L[22] = fmaf(10.0f * m030, t151, L[22]);
+    L[21] = fmaf(10.0f * m021, t151, L[21]);
+    const float t142 = -fmaf(c1_7, rx * t042 + ry * t132 + rz * t141, c2_7 * (t122 + t140));
+    L[24] = fmaf(8.0f * m111, t142, L[24]);
+    L[23] = fmaf(m102, t142, L[23]);
+    L[22] = fmaf(12.0f * m021, t142, L[22]);
+    L[21] = fmaf(4.0f * m012, t142, L[21]);
+    const float t133 = -fmaf(c1_7, rx * t033 + ry * t123 + rz * t132, c2_7 * (t113 + t131));
+    L[24] = fmaf(3.0f * m102, t133, L[24]);
+    L[22] = fmaf(9.0f * m012, t133, L[22]);
+    L[21] = fmaf(m003, t133, L[21]);
+    const float t124 = -fmaf(c1_7, rx * t024 + ry * t114 + rz * t123, c2_7 * (t104 + t122));
+    L[22] = fmaf(4.0f * m003, t124, L[22]);
+    const float t070 = -fmaf(c1_7, ry * t060, c2_7 * (t050));
+    L[23] = fmaf(35.0f * m030, t070, L[23]);
+    const float t061 = -fmaf(c1_7, ry * t051 + rz * t060, c2_7 * (t041));
+    L[24] = fmaf(20.0f * m030, t061, L[24]);
+    L[23] = fmaf(15.0f * m021, t061, L[23]);
+    const float t052 = -fmaf(c1_7, ry * t042 + rz * t051, c2_7 * (t032 + t050));
+    L[24] = fmaf(20.0f * m021, t052, L[24]);
+    L[23] = fmaf(5.0f * m012, t052, L[23]);
+    const float t043 = -fmaf(c1_7, ry * t033 + rz * t042, c2_7 * (t023 + t041));
+    L[24] = fmaf(12.0f * m012, t043, L[24]);
+    L[23] = fmaf(m003, t043, L[23]);
+    const float t034 = -fmaf(c1_7, ry * t024 + rz * t033, c2_7 * (t014 + t032));
+    L[24] = fmaf(4.0f * m003, t034, L[24]);
 }
 
 // potential and field at offset y from the expansion centre (order P = 4, from the

@@ -49,6 +49,10 @@ struct SpeciesDev {
 
 // The joint tree over both species (mixed walk): nodes in preorder, expansion
 // about each cell's geometric centre (a leaf: its particle).
+// joint-tree moments per cell: net charge, dipole, second moment (xx yy zz
+// xy yz zx) and third moment (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
+constexpr int TB_JQ = 20;
+
 struct JointDev {
     int n = 0;                  // ne + ni
     uint64_t *key[2] = {};
@@ -59,7 +63,7 @@ struct JointDev {
     int nn = 0, cap = 0, maxdepth = 0;
     double4 *nc = nullptr;      // expansion centre, cell size max(lx) (leaf: particle, 0)
     uint32_t *nnext = nullptr;
-    double *nq = nullptr;       // [nn][10] q, D(3), M(6) (xx yy zz xy yz zx) about the centre
+    double *nq = nullptr;       // [nn][TB_JQ] q, D(3), M(6) (xx yy zz xy yz zx), O(10) about the centre
     int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
     uint64_t *dkey[2] = {};
     uint32_t *dval[2] = {};
@@ -120,6 +124,8 @@ void tb_walk64_list(hipStream_t s, int nt, const int32_t *tsp, const int32_t *id
                     const SpeciesDev &I, double rs, double *fout);
 int tb_walk64_blocks(int n);
 // local.hip: the far-field local expansion of cells about one centre, evaluated at offsets
+// (TB_LOCAL_CELL floats per cell)
+constexpr int TB_LOCAL_CELL = 19;
 int tb_local_run(hipStream_t s, int order, int ncell, const float *cells, int ny, const float *y, float *out);
 
 void tb_direct(hipStream_t s, int nt, const int32_t *tsp, const int32_t *tidx_sorted, const SpeciesDev &E,

@@ -9,6 +9,8 @@
 //   bq2   float4   Q'zx, s, count, log2(s)/8      (Q'zz = -Q'xx - Q'yy)
 //   bf    float4   the learned criterion's per-cell features (log2 / 8 of
 //                  |q|/count, |D|/(count s), |Q'|/(count s^2), count)
+//   bq3   float4   O'xxx, O'xxy, O'xxz, O'xyy   traceless third moment
+//   bq4   float4   O'xyz, O'yyy, O'yyz, 0       (the others follow from tr = 0)
 #pragma once
 #include "tb_internal.h"
 #include "tb_mlp.h"
@@ -18,6 +20,7 @@ struct MixedTree {
     double4 *bmac = nullptr;
     uint32_t *binfo = nullptr;
     float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr, *bf = nullptr;
+    float4 *bq3 = nullptr, *bq4 = nullptr;   // traceless third moment (far-field local expansion)
     uint32_t *inv = nullptr;
 };
 
@@ -35,7 +38,7 @@ struct Domain {
     int n[2], ng[2];
     const double4 *bmac;
     const uint32_t *binfo;
-    const float4 *bq0, *bq1, *bq2, *bf;
+    const float4 *bq0, *bq1, *bq2, *bf, *bq3, *bq4;
     float inv_rs, rs;
     int gbase;        // first global group index of this domain
     int nn;           // cells in the tree
@@ -50,6 +53,10 @@ struct MixedDev {
     int ddom_cap = 0;
     double *esum = nullptr;       // 4 energy sums (ee, ei, ie, ii) per domain
     int esum_cap = 0;
+    // far cells of the last walk: chunk lists per target group (k_far_eval)
+    uint32_t *far_cells = nullptr;
+    int32_t *far_link = nullptr, *far_head = nullptr;
+    int far_cap = 0, head_cap = 0;
     // the learned opening criterion
     MlpDev mlp;
     BandRec *rec = nullptr;       // collected band pairs

@@ -49,6 +49,8 @@ constexpr int LCAP = FLUSH - 1 + 64;
 constexpr int QCAP = 128;
 constexpr int NXCD = 8;       // XCDs of an MI355X (work chunks of the walk queue)
 constexpr int CTR_CHUNK = 8;  // first chunk counter in the counter block (16 ints)
+constexpr int CTR_FAR = 4;    // far-chunk allocation counter
+constexpr int FAR_SLAB = 256; // far chunks a wave reserves at a time
 constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decided by the network
 
 const char *g_err = "";
@@ -58,10 +60,10 @@ constexpr int FAR_ORDER = 4;
 constexpr int NLOC = FAR_ORDER == 3 ? TB_LOCAL_N3 : TB_LOCAL_N4H;
 // default threshold: cell distance / group radius.  The field's relative
 // truncation error of the order-4 expansion is about 5 (1/K)^4 at the
-// group's edge (4e-3 at K = 6), below the M2P's own error at theta ~ 0.15;
-// measured at N = 4M, K = 6 gave the fastest walk at 1e-4 rms (K = 5 needs
-// a smaller theta).
-constexpr double FAR_RATIO_DEFAULT = 6.0;
+// group's edge (8e-3 at K = 5), and the far cells' third moments remove
+// most of their M2P truncation error; measured at N = 4M and 1e-4 rms,
+// K = 5 gave the fastest walk (K = 4 needs a much smaller theta).
+constexpr double FAR_RATIO_DEFAULT = 5.0;
 
 struct WaveLDS {
     uint32_t stack[STACK];
@@ -167,10 +169,14 @@ struct WalkArgs {
     float lo_scale;       // learned / collect: theta / theta_lo; cells beyond lo_scale * s/theta are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
     float far_k2;         // square of the local-expansion threshold (distance / group radius); 0: off
+    uint32_t *far_cells;  // far cells, 64 per chunk (0xffffffff: none)
+    int32_t *far_link;    // per chunk: the group's previous chunk, or -1
+    int32_t *far_head;    // per global group: its last chunk, or -1
+    int far_cap;          // chunks in far_cells
     const TbMlpPack *mlpk;  // learned: the bf16 parameters packed as matrix-core operands
     BandRec *rec;         // collect: band pairs
     int rec_cap;
-    int32_t *ctr;         // [1] error flag, [2] prep error, [3] band pair count, [8..15] chunk counters
+    int32_t *ctr;         // [1] error flag, [2] prep error, [3] band pair count, [4] far chunks, [8..15] work chunks
     double *gen;          // 2 per global group: energy of electron targets / of ion targets
     unsigned long long *cnt;
 };
@@ -301,13 +307,16 @@ __device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     }
 }
 
+// q0..q2: q, D, traceless quadrupole; q3, q4: traceless third moment (order 4 only)
 template <int N>
-__device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q1, float4 q2, float (&L)[N])
+__device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4,
+                               float (&L)[N])
 {
     if constexpr (N == TB_LOCAL_N3)
         m2l_acc_p3(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, L);
     else
-        m2l_acc_p4h(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, L);
+        m2l_acc_p4ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
+                     q4.y, q4.z, L);
 }
 
 template <int N>
@@ -319,42 +328,31 @@ __device__ inline void l2p(const float (&L)[N], float x, float y, float z, float
         l2p_p4h(L, x, y, z, phi, ex, ey, ez);
 }
 
-// Up to 64 far cells (one per lane; lanes past cnt add nothing) into the
-// group's local expansion about its centre (gx, gy, gz).  DIST = false:
-// every lane keeps its own partial coefficients Lf (summed across the
-// wavefront once, at the group's end).  DIST = true (the variants short of
-// registers): each flush is summed across the wavefront at once and lane k
-// keeps coefficient k in lacc.
-template <bool DIST>
-__device__ inline void flush_far(const WaveLDS &L, int cnt, const Domain &D, double gx, double gy, double gz,
-                                 float (&Lf)[NLOC], float &lacc)
+// Far cells are evaluated after the walk (k_far_eval), so their local
+// expansion takes no registers here: the walk appends each group's far cells
+// to a linked list of 64-entry chunks in global memory.  A wave reserves
+// chunks FAR_SLAB at a time; a chunk past the buffer's capacity is not
+// written, and the host re-runs the force with a larger buffer.
+struct FarSlab {
+    int next = 0, end = 0;  // the wave's reserved chunk range
+    int head = -1;          // the group's last chunk (lists run backwards)
+};
+
+__device__ inline void push_far(const WaveLDS &L, int cnt, uint32_t *__restrict__ cells, int32_t *__restrict__ link,
+                                int cap, int32_t *ctr, FarSlab &F)
 {
     const int lane = lane_id();
-    float rx = 1.0f, ry = 0.0f, rz = 0.0f;
-    float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0;
-    if (lane < cnt) {
-        const uint32_t nd = L.farq[lane];
-        const double4 m = D.bmac[nd];
-        rx = (float)(gx - m.x);
-        ry = (float)(gy - m.y);
-        rz = (float)(gz - m.z);
-        q0 = D.bq0[nd];
-        q1 = D.bq1[nd];
-        q2 = D.bq2[nd];
+    if (F.next >= F.end) {
+        // every lane issues the atomic (lane 0 adds, the rest add 0): no lane-index branch
+        const int v = uni(atomicAdd(ctr, lane == 0 ? FAR_SLAB : 0));
+        F.next = v;
+        F.end = v + FAR_SLAB;
     }
-    if constexpr (DIST) {
-        float Lp[NLOC];
-#pragma unroll
-        for (int k = 0; k < NLOC; k++) Lp[k] = 0.0f;
-        m2l_acc(rx, ry, rz, q0, q1, q2, Lp);
-#pragma unroll
-        for (int k = 0; k < NLOC; k++) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) Lp[k] += __shfl_xor(Lp[k], o, 64);
-            lacc += lane == k ? Lp[k] : 0.0f;
-        }
-    } else {
-        m2l_acc(rx, ry, rz, q0, q1, q2, Lf);
+    const int c = F.next++;
+    if (c < cap) {
+        cells[(size_t)c * 64 + lane] = lane < cnt ? L.farq[lane] : 0xffffffffu;
+        link[c] = F.head;  // every lane stores the same value
+        F.head = c;
     }
 }
 
@@ -379,6 +377,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
     // (contiguous in key order, hence in space: that XCD's L2 keeps the tree
     // region its groups share), then from the other chunks in turn.
     int chunk = blockIdx.x & (NXCD - 1), tried = 0;
+    FarSlab F;
     for (;;) {
         // No branch on the lane index inside this loop: the compiler may
         // thread such a (loop-invariant) branch across the back edge and split
@@ -468,13 +467,9 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             A.ps[t] = A.pc[t] = 0.0f;
         }
         // accepted cells farther than far_k2^(1/2) group radii from the
-        // centre go into the group's local expansion (lane partials Lf)
+        // centre go to the group's far list (k_far_eval)
         const float far_r2 = uni(W.far_k2 > 0.0f ? W.far_k2 * hdiag * hdiag : 3.0e38f);
-        constexpr bool DIST = FAR_ORDER > 3 || MODE != MODE_FIXED;
-        float Lf[NLOC], lacc = 0.0f;
-#pragma unroll
-        for (int k = 0; k < NLOC; k++) Lf[k] = 0.0f;
-        int nfar = 0;  // far cells taken by this group
+        F.head = -1;
 
         int sp = 1, nm = 0, np = 0, nb = 0, nf = 0;  // stack, M2P, P2P, band-queue, far-queue lengths
         L.stack[0] = 0u;  // root: range of one cell at 0
@@ -565,7 +560,6 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             const uint64_t fk = __ballot(far);
             if (far) L.farq[nf + __popcll(fk & lanemask_lt())] = node;
             nf += __popcll(fk);
-            nfar += __popcll(fk);
             if (COUNT) {
                 c_m2l[tsp] += (unsigned long long)__popcll(fk) * nact;
                 c_far[tsp] += (unsigned long long)__popcll(fk);
@@ -624,7 +618,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 wave_sync();
             }
             if (nf >= 64) {
-                flush_far<DIST>(L, 64, D, gx, gy, gz, Lf, lacc);
+                push_far(L, 64, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
                 // the rest of the queue (at most 63 entries) moves down
                 const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
                 wave_sync();
@@ -681,7 +675,6 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     const uint64_t bf = __ballot(bfar);
                     if (bfar) L.farq[nf + __popcll(bf & lanemask_lt())] = nd;
                     nf += __popcll(bf);
-                    nfar += __popcll(bf);
                     if (COUNT) {
                         c_m2l[tsp] += (unsigned long long)__popcll(bf) * nact;
                         c_far[tsp] += (unsigned long long)__popcll(bf);
@@ -727,7 +720,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         wave_sync();
                     }
                     if (nf >= 64) {
-                        flush_far<DIST>(L, 64, D, gx, gy, gz, Lf, lacc);
+                        push_far(L, 64, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
                         const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
                         wave_sync();
                         if (lane + 64 < nf) L.farq[lane] = rest;
@@ -744,28 +737,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         if (np > 0) {
             flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
         }
-        if (nf > 0) flush_far<DIST>(L, nf, D, gx, gy, gz, Lf, lacc);
-        if (nfar > 0) {
-            // the group's local expansion, evaluated at each target
-            if constexpr (DIST) {
-#pragma unroll
-                for (int k = 0; k < NLOC; k++) Lf[k] = __shfl(lacc, k, 64);
-            } else {
-#pragma unroll
-                for (int k = 0; k < NLOC; k++)
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) Lf[k] += __shfl_xor(Lf[k], o, 64);
-            }
-#pragma unroll
-            for (int t = 0; t < TPL; t++) {
-                float phi, ex, ey, ez;
-                l2p(Lf, rx[t], ry[t], rz[t], phi, ex, ey, ez);
-                nadd(A.s[t][0], A.c[t][0], ex);
-                nadd(A.s[t][1], A.c[t][1], ey);
-                nadd(A.s[t][2], A.c[t][2], ez);
-                nadd(A.ps[t], A.pc[t], phi);
-            }
-        }
+        if (nf > 0) push_far(L, nf, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
+        W.far_head[g] = F.head;  // every lane stores the same value
         wave_sync();
         // forces F = q_t eps E; energy 0.5 q_t eps phi (FP64 from here on)
         const double ff = (double)qt * TB_EPS;
@@ -802,6 +775,87 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
     }
 }
 
+// The far cells of every group, after the walk: one wavefront per group,
+// one far cell per lane per chunk into the lane's partial local expansion
+// about the group's centre (the centre the walk used: the middle of the
+// targets' bounding box), summed across the wavefront and evaluated at each
+// target; field and energy are added to the walk's.
+struct FarArgs {
+    const Domain *dom;
+    int ndom, ngt;
+    const uint32_t *far_cells;
+    const int32_t *far_link, *far_head;
+    double *gen;
+};
+
+__global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
+{
+    const int lane = lane_id();
+    const int g = uni((int)(blockIdx.x * WPB + (threadIdx.x >> 6)));
+    if (g >= A.ngt) return;
+    int c = uni(A.far_head[g]);
+    if (c < 0) return;
+    int lo = 0, hi = A.ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.dom[mid].gbase <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const Domain &D = A.dom[lo];
+    const int gl = g - D.gbase;
+    const int tsp = gl < D.ng[0] ? 0 : 1;
+    const int2 gr = D.grp[tsp][tsp ? gl - D.ng[0] : gl];
+    const double *px = D.px[tsp], *py = D.py[tsp], *pz = D.pz[tsp];
+    double mnx = 1e300, mny = 1e300, mnz = 1e300, mxx = -1e300, mxy = -1e300, mxz = -1e300;
+    for (int t = lane; t < gr.y; t += 64) {
+        const double x = px[gr.x + t], y = py[gr.x + t], z = pz[gr.x + t];
+        mnx = fmin(mnx, x); mxx = fmax(mxx, x);
+        mny = fmin(mny, y); mxy = fmax(mxy, y);
+        mnz = fmin(mnz, z); mxz = fmax(mxz, z);
+    }
+    mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
+    mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
+    const double gx = uni(0.5 * (mnx + mxx)), gy = uni(0.5 * (mny + mxy)), gz = uni(0.5 * (mnz + mxz));
+    float Lp[NLOC];
+#pragma unroll
+    for (int k = 0; k < NLOC; k++) Lp[k] = 0.0f;
+    while (c >= 0) {
+        const uint32_t nd = A.far_cells[(size_t)c * 64 + lane];
+        float rx = 1.0f, ry = 0.0f, rz = 0.0f;
+        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0;
+        if (nd != 0xffffffffu) {
+            const double4 m = D.bmac[nd];
+            rx = (float)(gx - m.x);
+            ry = (float)(gy - m.y);
+            rz = (float)(gz - m.z);
+            q0 = D.bq0[nd];
+            q1 = D.bq1[nd];
+            q2 = D.bq2[nd];
+            q3 = D.bq3[nd];
+            q4 = D.bq4[nd];
+        }
+        m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, Lp);
+        c = uni(A.far_link[c]);
+    }
+#pragma unroll
+    for (int k = 0; k < NLOC; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) Lp[k] += __shfl_xor(Lp[k], o, 64);
+    const double ff = (tsp ? 1.0 : -1.0) * TB_EPS;
+    double eg = 0.0;
+    for (int t = lane; t < gr.y; t += 64) {
+        const int i = gr.x + t;
+        float phi, ex, ey, ez;
+        l2p(Lp, (float)(px[i] - gx), (float)(py[i] - gy), (float)(pz[i] - gz), phi, ex, ey, ez);
+        D.fx[tsp][i] += ff * (double)ex;
+        D.fy[tsp][i] += ff * (double)ey;
+        D.fz[tsp][i] += ff * (double)ez;
+        eg += 0.5 * ff * (double)phi;
+    }
+    eg = __shfl(wave_sum(eg), 0, 64);
+    A.gen[2 * g + tsp] += eg;  // every lane: the same value to the same address
+}
+
 template <int MODE, bool COUNT>
 void dispatch(hipStream_t s, int tpl, int blocks, const WalkArgs &W)
 {
@@ -829,7 +883,8 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
                                                     const int32_t *__restrict__ ncount, double theta,
                                                     double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
                                                     float4 *__restrict__ bq0, float4 *__restrict__ bq1,
-                                                    float4 *__restrict__ bq2, float4 *__restrict__ bf,
+                                                    float4 *__restrict__ bq2, float4 *__restrict__ bq3,
+                                                    float4 *__restrict__ bq4, float4 *__restrict__ bf,
                                                     int32_t *__restrict__ err)
 {
     const int b = blockIdx.x * BT + threadIdx.x;
@@ -837,13 +892,14 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
     const uint32_t p = bfs[b];
     const uint32_t nx = nnext[p];
     const double4 c = nc[p];
-    const double *Q = nq + (size_t)p * 10;
+    const double *Q = nq + (size_t)p * TB_JQ;
     if (nx & TB_LEAF) {
         binfo[b] = TB_LEAF | (uint32_t)nstart[p];
         bmac[b] = make_double4(c.x, c.y, c.z, 0.0);
         bq0[b] = make_float4((float)Q[0], 0.f, 0.f, 0.f);
         bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         bq2[b] = make_float4(0.f, 0.f, 1.f, 0.f);
+        bq3[b] = bq4[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         bf[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
@@ -861,6 +917,13 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
     const double tr = Q[4] + Q[5] + Q[6];
     bq0[b] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
     bq1[b] = make_float4((float)(3.0 * Q[4] - tr), (float)(3.0 * Q[5] - tr), (float)(3.0 * Q[7]), (float)(3.0 * Q[8]));
+    // traceless third moment O' = O - (d_ij t_k + d_ik t_j + d_jk t_i) / 5, t_k = O_iik
+    // (raw O: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz); the far path's local expansion uses it
+    const double *O = Q + 10;
+    const double tx = O[0] + O[3] + O[5], ty = O[1] + O[6] + O[8], tz = O[2] + O[7] + O[9];
+    bq3[b] = make_float4((float)(O[0] - 0.6 * tx), (float)(O[1] - 0.2 * ty), (float)(O[2] - 0.2 * tz),
+                         (float)(O[3] - 0.2 * tx));
+    bq4[b] = make_float4((float)O[4], (float)(O[6] - 0.6 * ty), (float)(O[7] - 0.2 * tz), 0.0f);
     // the learned criterion's per-cell features, FP64 then rounded (see band_features)
     const double k = 0.125, cn = (double)ncount[p], sz = c.w, tiny = 9.5367431640625e-07;
     const double qxx = 3.0 * Q[4] - tr, qyy = 3.0 * Q[5] - tr, qzz = 3.0 * Q[6] - tr;
@@ -948,7 +1011,7 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const int2 gr = A.grp[tsp][(int)R.group - (tsp ? A.ng0 : 0)];
     const uint32_t p = A.bfs[R.cell];
     const double4 c = A.nc[p];
-    const double *Q = A.nq + (size_t)p * 10;
+    const double *Q = A.nq + (size_t)p * TB_JQ;
     const double q = Q[0], Dx = Q[1], Dy = Q[2], Dz = Q[3];
     const double tr = Q[4] + Q[5] + Q[6];
     const double Qxx = 3.0 * Q[4] - tr, Qyy = 3.0 * Q[5] - tr, Qzz = 3.0 * Q[6] - tr;
@@ -1053,11 +1116,33 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int g
         d.grp[k] = sp[k]->grp;
     }
     d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2; d.bf = t->bf;
+    d.bq3 = t->bq3; d.bq4 = t->bq4;
     d.inv_rs = (float)(1.0 / rs);
     d.rs = (float)rs;
     d.gbase = gbase;
     d.nn = t->nn;
     return d;
+}
+
+// far-cell chunk buffer of at least `chunks` chunks (64 cells each)
+int grow_far(MixedDev &M, long chunks)
+{
+    if (chunks > (long)INT32_MAX - (1L << 20)) {
+        g_err = "mixed walk: far-cell buffer would exceed 2^31 chunks";
+        return TB_ERR_NOMEM;
+    }
+    if (M.far_cells) hipFree(M.far_cells);
+    if (M.far_link) hipFree(M.far_link);
+    M.far_cells = nullptr;
+    M.far_link = nullptr;
+    M.far_cap = 0;
+    if (hipMalloc((void **)&M.far_cells, (size_t)chunks * 64 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&M.far_link, (size_t)chunks * sizeof(int32_t)) != hipSuccess) {
+        g_err = "mixed walk: far-cell buffer allocation failed";
+        return TB_ERR_NOMEM;
+    }
+    M.far_cap = (int)chunks;
+    return TB_OK;
 }
 
 // Uploads the domain descriptors (synchronously: they change only after a
@@ -1115,18 +1200,70 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         g_err = "mixed walk: event creation failed";
         return TB_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
-    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_CHUNK, CTR_CHUNK + NXCD);
-    hipEventRecord(M.ev[0], s);
-    if (mode == MODE_COLLECT) {
-        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
-        dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
-    } else if (mode == MODE_LEARNED) {
-        if (d_cnt) dispatch<MODE_LEARNED, true>(s, tpl, blocks, W);
-        else dispatch<MODE_LEARNED, false>(s, tpl, blocks, W);
-    } else {
-        if (d_cnt) dispatch<MODE_FIXED, true>(s, tpl, blocks, W);
-        else dispatch<MODE_FIXED, false>(s, tpl, blocks, W);
+    if (ngt > M.head_cap) {
+        if (M.far_head) hipFree(M.far_head);
+        M.far_head = nullptr;
+        M.head_cap = 0;
+        if (hipMalloc((void **)&M.far_head, (size_t)ngt * sizeof(int32_t)) != hipSuccess) {
+            g_err = "mixed walk: far-list heads allocation failed";
+            return TB_ERR_NOMEM;
+        }
+        M.head_cap = ngt;
+    }
+    // Far-cell chunks: a first guess, then what the last walk asked for,
+    // each time plus every wave's partly used slab.  A walk that asked for
+    // more than the buffer holds is run again with a larger one (its far
+    // lists are incomplete; the near forces it wrote are overwritten).
+    const long slack = (long)blocks * WPB * FAR_SLAB;
+    const bool far_on = W.far_k2 > 0.0f;
+    if (far_on && M.far_cap == 0) {
+        const int rc = grow_far(M, (long)ngt * 512 + slack);
+        if (rc) return rc;
+    }
+    for (int attempt = 0;; attempt++) {
+        W.far_cells = M.far_cells;
+        W.far_link = M.far_link;
+        W.far_head = M.far_head;
+        W.far_cap = M.far_cap;
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_FAR, CTR_FAR + 1);
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_CHUNK, CTR_CHUNK + NXCD);
+        hipEventRecord(M.ev[0], s);
+        if (mode == MODE_COLLECT) {
+            hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
+            dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
+        } else if (mode == MODE_LEARNED) {
+            if (d_cnt) dispatch<MODE_LEARNED, true>(s, tpl, blocks, W);
+            else dispatch<MODE_LEARNED, false>(s, tpl, blocks, W);
+        } else {
+            if (d_cnt) dispatch<MODE_FIXED, true>(s, tpl, blocks, W);
+            else dispatch<MODE_FIXED, false>(s, tpl, blocks, W);
+        }
+        if (!far_on) break;
+        int32_t used = 0;
+        if (hipMemcpyAsync(&used, M.ctr + CTR_FAR, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            g_err = "mixed walk: far-chunk count read failed";
+            return TB_ERR_HIP;
+        }
+        if (used <= M.far_cap) break;
+        if (attempt > 0) {
+            g_err = "mixed walk: far-cell buffer still too small after growing";
+            return TB_ERR_NOMEM;
+        }
+        const int rc = grow_far(M, (long)used + used / 4 + slack);
+        if (rc) return rc;
+    }
+    if (far_on) {
+        FarArgs FA;
+        FA.dom = M.ddom;
+        FA.ndom = nd;
+        FA.ngt = ngt;
+        FA.far_cells = M.far_cells;
+        FA.far_link = M.far_link;
+        FA.far_head = M.far_head;
+        FA.gen = W.gen;
+        hipLaunchKernelGGL(k_far_eval, dim3(nblk(ngt, WPB)), dim3(BT), 0, s, FA);
     }
     hipEventRecord(M.ev[1], s);
     hipLaunchKernelGGL(k_domain_energy, dim3(nd), dim3(256), 0, s, M.ddom, W.gen, M.esum);
@@ -1173,7 +1310,7 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, cons
     const JointDev &J = *jt;
     MixedTree &T = M.t[0];
     int cap = T.cap, rc;
-    int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap;
+    int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap, c8 = cap, c9 = cap;
     if ((rc = grow(&T.bmac, &c1, J.nn, bytes))) return rc;
     if ((rc = grow(&T.binfo, &c2, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq0, &c3, J.nn, bytes))) return rc;
@@ -1181,11 +1318,15 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, cons
     if ((rc = grow(&T.bq2, &c5, J.nn, bytes))) return rc;
     if ((rc = grow(&T.inv, &c6, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bf, &c7, J.nn, bytes))) return rc;
-    T.cap = std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(std::min(c5, c6), c7));
+    if ((rc = grow(&T.bq3, &c8, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bq4, &c9, J.nn, bytes))) return rc;
+    T.cap = std::min(std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(std::min(c5, c6), c7)),
+                     std::min(c8, c9));
     T.nn = J.nn;
     hipLaunchKernelGGL(k_inv, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv);
     hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv, J.nc, J.nnext, J.nq,
-                       J.nstart, J.ncount, theta, T.bmac, T.binfo, T.bq0, T.bq1, T.bq2, T.bf, M.ctr + 2);
+                       J.nstart, J.ncount, theta, T.bmac, T.binfo, T.bq0, T.bq1, T.bq2, T.bq3, T.bq4, T.bf,
+                       M.ctr + 2);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: prep launch failed";
         return TB_ERR_HIP;
@@ -1302,8 +1443,14 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
 {
     MixedTree &T = M.t[0];
     hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.bq2); hipFree(T.inv);
-    hipFree(T.bf);
+    hipFree(T.bf); hipFree(T.bq3); hipFree(T.bq4);
     T = MixedTree();
+    if (M.far_cells) hipFree(M.far_cells);
+    if (M.far_link) hipFree(M.far_link);
+    if (M.far_head) hipFree(M.far_head);
+    M.far_cells = nullptr;
+    M.far_link = M.far_head = nullptr;
+    M.far_cap = M.head_cap = 0;
     hipFree(M.ctr);
     if (M.ddom) hipFree(M.ddom);
     if (M.esum) hipFree(M.esum);

@@ -270,7 +270,7 @@ class Engine:
         return f
 
     def local_eval(self, order, cells, y):
-        """The far-field local expansion check: cells [n][12], offsets y [m][3] ->
+        """The far-field local expansion check: cells [n][19], offsets y [m][3] ->
         [m][4] potential and field (FP32)."""
         cells = np.ascontiguousarray(cells, np.float32)
         y = np.ascontiguousarray(y, np.float32)
