@@ -137,19 +137,25 @@ def l2p(L, y):
     return phi, E
 
 
-def m2p(X, q, D, Qp, x):
+def m2p(X, q, D, Qp, x, Op=None):
     """The walk's cell expansion evaluated directly at x (FP64):
-    phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5), E = -grad phi, r = x - X."""
+    phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5) [+ 5/2 O'rrr / r^7], E = -grad phi,
+    r = x - X, O' the traceless third moment (the walk's M2P keeps it)."""
     r = np.asarray(x, dtype=np.float64) - np.asarray(X, dtype=np.float64)
     Qp = np.asarray(Qp, dtype=np.float64)
     r2 = float(r @ r)
     ri = 1.0 / np.sqrt(r2)
-    ri3, ri5, ri7 = ri ** 3, ri ** 5, ri ** 7
+    ri3, ri5, ri7, ri9 = ri ** 3, ri ** 5, ri ** 7, ri ** 9
     dp = float(np.dot(D, r))
     Qr = Qp @ r
     rQr = float(r @ Qr)
     phi = q * ri + dp * ri3 + 0.5 * rQr * ri5
     E = r * (q * ri3 + 3.0 * dp * ri5 + 2.5 * rQr * ri7) - np.asarray(D) * ri3 - Qr * ri5
+    if Op is not None:
+        v = np.einsum("ijk,j,k->i", np.asarray(Op, dtype=np.float64), r, r)
+        s3 = float(r @ v)
+        phi += 2.5 * s3 * ri7
+        E += -7.5 * v * ri7 + 17.5 * s3 * r * ri9
     return phi, E
 
 

@@ -72,10 +72,10 @@ def test_walk_far_cells_through_local_expansion():
     """The same accepted cells with and without the far path: the M2P cells
     with it plus its far cells equal the M2P cells without it (the route
     changes, the acceptance does not); forces differ by less than the walk's
-    own error; both stay close to the FP64 direct sum.  The far path adds the
-    cells' third moments, so its total potential energy is no farther from
-    an accurate one (the walk without the far path at theta 0.1) than the
-    energy without it."""
+    own error; both stay close to the FP64 direct sum; the total potential
+    energy stays within the local expansion's truncation (1e-4 relative at
+    K = 8, 1e-3 at K = 4) of an accurate one (the walk without the far path at
+    theta 0.1)."""
     announce("walk_far_path")
     n = 1 << 16
     qel, qion = ta.gen_plasma(31, n, n, 300.0)
@@ -107,7 +107,8 @@ def test_walk_far_cells_through_local_expansion():
         d = np.linalg.norm(f1 - f0, axis=1) / np.linalg.norm(f0, axis=1)
         bound = 2e-3 if far == 8.0 else 1e-2
         assert np.sqrt(np.mean(d ** 2)) < bound, (far, np.sqrt(np.mean(d ** 2)))
-        assert abs(sum(ep1) - ep_ref) <= abs(sum(ep0) - ep_ref) + 1e-6 * abs(ep_ref), (ep0, ep1, ep_ref)
+        ebound = 1e-4 if far == 8.0 else 1e-3
+        assert abs(sum(ep1) - ep_ref) <= ebound * abs(ep_ref), (far, ep0, ep1, ep_ref)
     idx = np.where(tsp == 0, tix, n + tix)
     for far in (-1.0, 8.0):
         f = out[far][0][idx]

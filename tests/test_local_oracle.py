@@ -178,3 +178,34 @@ def test_octupole_local_expansion_vs_point_charges():
     L3t = lr.m2l(X, q, D, Qp, g, 4, O=lr.traceless3(O))
     for k in L3:
         assert abs(L3t[k] - L3[k]) <= 1e-12 * max(1.0, abs(L3[k]))
+
+
+def test_octupole_m2p_vs_point_charges_and_gradient():
+    # the walk's M2P with the traceless third moment: E = -grad phi (finite
+    # differences), and against point charges the error falls as (b/d)^4
+    # instead of (b/d)^3
+    rng = np.random.default_rng(12)
+    X, xs, qs, q, D, Qp = _cell(rng, n=80, b=0.25)
+    Op = lr.traceless3(lr.third_moment(xs - X, qs))
+    x0 = X + np.array([1.3, -0.9, 1.1])
+    h = 1e-5
+    g = np.zeros(3)
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = h
+        g[i] = (lr.m2p(X, q, D, Qp, x0 + e, Op)[0] - lr.m2p(X, q, D, Qp, x0 - e, Op)[0]) / (2 * h)
+    E = lr.m2p(X, q, D, Qp, x0, Op)[1]
+    assert np.allclose(E, -g, rtol=1e-6, atol=1e-9 * np.abs(E).max())
+    errs = []
+    for d in (2.0, 4.0):
+        u = np.array([0.6, -0.48, 0.64])
+        x = X + d * u
+        dr = x - xs
+        r = np.linalg.norm(dr, axis=1)
+        E_dir = (qs[:, None] * dr / r[:, None] ** 3).sum(0)
+        e2 = np.linalg.norm(lr.m2p(X, q, D, Qp, x)[1] - E_dir)
+        e3 = np.linalg.norm(lr.m2p(X, q, D, Qp, x, Op)[1] - E_dir)
+        assert e3 < e2
+        errs.append(e3)
+    # order: doubling d divides the octupole-truncated error by about 2^6 (field ~ d^-2 * (b/d)^4)
+    assert errs[0] / errs[1] > 0.5 * 2 ** 6

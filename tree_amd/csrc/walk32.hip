@@ -39,13 +39,17 @@ namespace {
 constexpr int BT = 256;
 constexpr int WPB = BT / 64;
 constexpr int STACK = 1024;
-// Interaction lists are flushed once FLUSH entries are pending.  A traversal
-// step adds at most 64 accepted cells, and a learned band decision at most 64
-// more after a flush check, so LCAP = FLUSH - 1 + 64 holds any list.  The
-// queues (band cells, far cells) are flushed 64 at a time: QCAP = 63 + 64.
-// The LDS per wave (12.5 KB) keeps three 4-wave blocks on a CU.
-constexpr int FLUSH = 48;
-constexpr int LCAP = FLUSH - 1 + 64;
+// Interaction lists are flushed once FLUSH_M (cells) or FLUSH_P (leaves)
+// entries are pending.  A traversal step adds at most 64 entries to a list,
+// and a learned band decision at most 64 more cells after a flush check, so
+// a list of capacity FLUSH - 1 + 64 holds them.  The queues (band cells, far
+// cells) are flushed 64 at a time: QCAP = 63 + 64.  The LDS per wave
+// (13.0 KB) keeps three 4-wave blocks on a CU; cell entries (80 bytes, up to
+// the third moment) are the largest part.
+constexpr int FLUSH_M = 15;
+constexpr int LCAP_M = FLUSH_M - 1 + 64;
+constexpr int FLUSH_P = 24;
+constexpr int LCAP_P = FLUSH_P - 1 + 64;
 constexpr int QCAP = 128;
 constexpr int NXCD = 8;       // XCDs of an MI355X (work chunks of the walk queue)
 constexpr int CTR_CHUNK = 8;  // first chunk counter in the counter block (16 ints)
@@ -70,8 +74,10 @@ struct WaveLDS {
     uint32_t batch[64];
     uint32_t bandq[QCAP];                 // learned: band cells waiting for a decision
     uint32_t farq[QCAP];                  // far accepted cells waiting for the local expansion
-    float4 m0[LCAP], m1[LCAP], m2[LCAP];  // x y z q | Qxx Qyy Qzz Qxy | Qyz Qzx - -
-    float4 p[LCAP];                       // x y z -
+    // accepted cells: x y z q | Dx Dy Dz Q'xx | Q'yy Q'xy Q'yz Q'zx |
+    // O'xxx O'xxy O'xxz O'xyy | O'xyz O'yyy O'yyz -
+    float4 m0[LCAP_M], m1[LCAP_M], m2[LCAP_M], m3[LCAP_M], m4[LCAP_M];
+    float4 p[LCAP_P];                     // leaves: x y z q
 };
 // measured: 4 x 13056 bytes per block kept 3 blocks on a CU, 4 x 13568 did not
 static_assert(sizeof(WaveLDS) * 4 <= 4 * 13056, "walk LDS would drop a CU to two blocks");
@@ -214,7 +220,9 @@ struct Acc {
     float ps[TPL], pc[TPL];      // compensated potential
 };
 
-// monopole + dipole + traceless quadrupole about the cell centre
+// monopole + dipole + traceless quadrupole + traceless third moment about
+// the cell centre: phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5) + 5/2 O'rrr/r^7,
+// E = -grad phi (the O' part: -15/2 v/r^7 + 35/2 (r.v) r/r^9, v_i = O'_ijk r_j r_k)
 template <int TPL>
 __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
                                  const float (&rz)[TPL], Acc<TPL> &A)
@@ -223,8 +231,14 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
 #pragma unroll
     for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
     for (int j = 0; j < cnt; j++) {
-        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j];
+        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j], o1 = L.m3[j], o2 = L.m4[j];
         const float qxx = b.w, qyy = c.x, qzz = -b.w - c.x, qxy = c.y, qyz = c.z, qzx = c.w;
+        // O' components (tr = 0 fixes the z-heavy ones); the mixed ones doubled
+        const float oxxx = o1.x, oxyy = o1.w, oxzz = -(o1.x + o1.w);
+        const float oxxy = o1.y, oyyy = o2.y, oyzz = -(o1.y + o2.y);
+        const float oxxz = o1.z, oyyz = o2.z, ozzz = -(o1.z + o2.z);
+        const float o2xxy = 2.0f * oxxy, o2xxz = 2.0f * oxxz, o2xyz = 2.0f * o2.x;
+        const float o2xyy = 2.0f * oxyy, o2yyz = 2.0f * oyyz, o2xzz = 2.0f * oxzz, o2yzz = 2.0f * oyzz;
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - a.x, dy = ry[t] - a.y, dz = rz[t] - a.z;
@@ -238,14 +252,20 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
             const float qy = qxy * dx + qyy * dy + qyz * dz;
             const float qz = qzx * dx + qyz * dy + qzz * dz;
             const float xqx = dx * qx + dy * qy + dz * qz;
-            // E = dx ri3 (q + u (3 dp + 2.5 xqx u)) - D ri3 - Q.dx ri5
-            // phi = ri (q + u (dp + 0.5 xqx u))
-            const float a5 = 2.5f * u * xqx;
+            const float xx = dx * dx, yy = dy * dy, zz = dz * dz, xy = dx * dy, xz = dx * dz, yz = dy * dz;
+            const float vx = oxxx * xx + oxyy * yy + oxzz * zz + o2xxy * xy + o2xxz * xz + o2xyz * yz;
+            const float vy = oxxy * xx + oyyy * yy + oyzz * zz + o2xyy * xy + o2xyz * xz + o2yyz * yz;
+            const float vz = oxxz * xx + oyyz * yy + ozzz * zz + o2xyz * xy + o2xzz * xz + o2yzz * yz;
+            const float t3 = u * (dx * vx + dy * vy + dz * vz);
+            // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3))) - ri3 (D + u (Q.dx + 7.5 u v))
+            // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3)))
+            const float a5 = u * fmaf(17.5f, t3, 2.5f * xqx);
             const float w = ri3 * fmaf(u, fmaf(3.0f, dp, a5), a.w);
-            ax[t] = fmaf(-qx, ri5, fmaf(-b.x, ri3, fmaf(dx, w, ax[t])));
-            ay[t] = fmaf(-qy, ri5, fmaf(-b.y, ri3, fmaf(dy, w, ay[t])));
-            az[t] = fmaf(-qz, ri5, fmaf(-b.z, ri3, fmaf(dz, w, az[t])));
-            ap[t] = fmaf(ri, fmaf(u, fmaf(0.2f, a5, dp), a.w), ap[t]);
+            const float u75 = 7.5f * u;
+            ax[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vx, qx), b.x), fmaf(dx, w, ax[t]));
+            ay[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vy, qy), b.y), fmaf(dy, w, ay[t]));
+            az[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vz, qz), b.z), fmaf(dz, w, az[t]));
+            ap[t] = fmaf(ri, fmaf(u, dp + u * fmaf(2.5f, t3, 0.5f * xqx), a.w), ap[t]);
         }
     }
 #pragma unroll
@@ -572,6 +592,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 L.m0[k] = make_float4(cx, cy, cz, q0.x);
                 L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
                 L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
+                L.m3[k] = D.bq3[node];
+                L.m4[k] = D.bq4[node];
             }
             nm += __popcll(mk);
             // leaves -> P2P list with their charge
@@ -606,13 +628,13 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
             sp += npush;
             wave_sync();
-            if (nm >= FLUSH) {
+            if (nm >= FLUSH_M) {
                 if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
                 flush_m2p<TPL>(L, nm, rx, ry, rz, A);
                 nm = 0;
                 wave_sync();
             }
-            if (np >= FLUSH) {
+            if (np >= FLUSH_P) {
                 flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
                 np = 0;
                 wave_sync();
@@ -687,6 +709,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         L.m0[k] = make_float4(bx, by, bz, q0.x);
                         L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
                         L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
+                        L.m3[k] = D.bq3[nd];
+                        L.m4[k] = D.bq4[nd];
                     }
                     nm += __popcll(bk);
                     const uint32_t binf = bopen ? D.binfo[nd] : 0u;
@@ -713,7 +737,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     if (lane + 64 < nb) L.bandq[lane] = rest;
                     nb -= ne;
                     wave_sync();
-                    if (nm >= FLUSH) {
+                    if (nm >= FLUSH_M) {
                         if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
                         flush_m2p<TPL>(L, nm, rx, ry, rz, A);
                         nm = 0;
@@ -997,7 +1021,7 @@ struct LabelArgs {
 
 // label of a band pair: log10 of the rms, over the group's targets, of
 // |F_M2P - F_exact| / |F_target| with F_M2P the cell's monopole + dipole +
-// quadrupole field and F_exact the FP64 sum over the cell's particles with the
+// quadrupole + third-moment field and F_exact the FP64 sum over the cell's particles with the
 // reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg).  The
 // rms (not the largest) is the cell's share of the rms force error that the
 // walk is tuned to.
@@ -1016,6 +1040,12 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const double tr = Q[4] + Q[5] + Q[6];
     const double Qxx = 3.0 * Q[4] - tr, Qyy = 3.0 * Q[5] - tr, Qzz = 3.0 * Q[6] - tr;
     const double Qxy = 3.0 * Q[7], Qyz = 3.0 * Q[8], Qzx = 3.0 * Q[9];
+    // traceless third moment (raw O: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
+    const double *O = Q + 10;
+    const double tx = O[0] + O[3] + O[5], ty = O[1] + O[6] + O[8], tz = O[2] + O[7] + O[9];
+    const double oxxx = O[0] - 0.6 * tx, oxxy = O[1] - 0.2 * ty, oxxz = O[2] - 0.2 * tz, oxyy = O[3] - 0.2 * tx;
+    const double oxyz = O[4], oxzz = O[5] - 0.2 * tx, oyyy = O[6] - 0.6 * ty, oyyz = O[7] - 0.2 * tz;
+    const double oyzz = O[8] - 0.2 * ty, ozzz = O[9] - 0.6 * tz;
     const int s0 = A.nstart[p], cnt = A.ncount[p];
     double sq = 0.0;  // sum over the group's targets of the squared relative error
     for (int t = threadIdx.x; t < gr.y; t += 64) {
@@ -1029,8 +1059,14 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
         const double qy = Qxy * dx + Qyy * dy + Qyz * dz;
         const double qz = Qzx * dx + Qyz * dy + Qzz * dz;
         const double xqx = dx * qx + dy * qy + dz * qz;
-        const double k = q * ri3 + 3.0 * dp * ri5 + 2.5 * xqx * ri7;
-        double ex = dx * k - Dx * ri3 - qx * ri5, ey = dy * k - Dy * ri3 - qy * ri5, ez = dz * k - Dz * ri3 - qz * ri5;
+        const double vx = oxxx * dx * dx + oxyy * dy * dy + oxzz * dz * dz + 2.0 * (oxxy * dx * dy + oxxz * dx * dz + oxyz * dy * dz);
+        const double vy = oxxy * dx * dx + oyyy * dy * dy + oyzz * dz * dz + 2.0 * (oxyy * dx * dy + oxyz * dx * dz + oyyz * dy * dz);
+        const double vz = oxxz * dx * dx + oyyz * dy * dy + ozzz * dz * dz + 2.0 * (oxyz * dx * dy + oxzz * dx * dz + oyzz * dy * dz);
+        const double s3 = dx * vx + dy * vy + dz * vz, ri9 = ri7 * ri2;
+        const double k = q * ri3 + 3.0 * dp * ri5 + 2.5 * xqx * ri7 + 17.5 * s3 * ri9;
+        double ex = dx * k - Dx * ri3 - qx * ri5 - 7.5 * vx * ri7;
+        double ey = dy * k - Dy * ri3 - qy * ri5 - 7.5 * vy * ri7;
+        double ez = dz * k - Dz * ri3 - qz * ri5 - 7.5 * vz * ri7;
         double gx = 0.0, gy = 0.0, gz = 0.0;
         for (int j = s0; j < s0 + cnt; j++) {
             const double4 sj = A.pos[j];
