@@ -115,3 +115,28 @@ def test_walk_far_cells_through_local_expansion():
         rel = np.linalg.norm(f - fd, axis=1) / np.linalg.norm(fd, axis=1)
         out[far] = out[far] + (np.sqrt(np.mean(rel ** 2)),)
     assert out[8.0][3] < 1.3 * out[-1.0][3], (out[-1.0][3], out[8.0][3])
+
+
+def test_far_buffer_regrows(monkeypatch):
+    """A far-cell chunk buffer too small for the walk (one chunk) is detected
+    after the walk, grown, and the walk re-run: forces and energies are
+    bit-identical to a run whose first buffer was large enough."""
+    announce("far_buffer_regrow")
+    n = 1 << 14
+    qel, qion = ta.gen_plasma(33, n, n, 200.0)
+    out = []
+    for chunks in (None, "1"):
+        if chunks:
+            monkeypatch.setenv("TREEMI_FAR_CHUNKS", chunks)
+        else:
+            monkeypatch.delenv("TREEMI_FAR_CHUNKS", raising=False)
+        with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.4, walk="mixed") as eng:
+            eng.set_particles(qel, qion)
+            eng.build()
+            ep = eng.force()
+            e, i = eng.get_particles()
+            st = eng.count_interactions()
+            out.append((np.r_[e["ff"], i["ff"]], ep, st))
+    assert sum(out[0][2]["m2l"]) > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]

@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace {
@@ -1253,7 +1254,10 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     const long slack = (long)blocks * WPB * FAR_SLAB;
     const bool far_on = W.far_k2 > 0.0f;
     if (far_on && M.far_cap == 0) {
-        const int rc = grow_far(M, (long)ngt * 512 + slack);
+        // TREEMI_FAR_CHUNKS overrides the first guess (tests use it to take the re-run path)
+        const char *env = getenv("TREEMI_FAR_CHUNKS");
+        const long first = env && atol(env) > 0 ? atol(env) : (long)ngt * 512 + slack;
+        const int rc = grow_far(M, first);
         if (rc) return rc;
     }
     for (int attempt = 0;; attempt++) {
