@@ -659,27 +659,31 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     // the matrix cores (tb_mlp_eval32_mfma, two tiles of 32).
                     // Entry e's features are recomputed from its records.
                     const int ne = min(nb, 64);
-                    float pr[2];
-#pragma unroll 1
-                    for (int tile = 0; tile < 2; tile++) {
-                        const int e = 32 * tile + (lane & 31);
-                        float f[TB_MLP_NIN];
+                    // lane l forms entry l's features; each tile's MFMA
+                    // operand takes features 0..3 of its entry in lanes 0..31
+                    // and 4..7 in lanes 32..63, exchanged across the halves
+                    float f[TB_MLP_NIN];
 #pragma unroll
-                        for (int j = 0; j < TB_MLP_NIN; j++) f[j] = 0.0f;
-                        if (e < ne) {
-                            const uint32_t nd = L.bandq[e];
-                            const double4 m = D.bmac[nd];
-                            const float ex = fmaxf(fabsf((float)(m.x - gx)) - hx, 0.0f);
-                            const float ey = fmaxf(fabsf((float)(m.y - gy)) - hy, 0.0f);
-                            const float ez = fmaxf(fabsf((float)(m.z - gz)) - hz, 0.0f);
-                            band_features(D.bf[nd], D.bq2[nd].w, sqrtf(ex * ex + ey * ey + ez * ez), hdiag, klemin,
-                                          D.bq0[nd].x * qt >= 0.0f, knact, f);
-                        }
-                        float xq[4];
-#pragma unroll
-                        for (int q = 0; q < 4; q++) xq[q] = (lane >> 5) ? f[4 + q] : f[q];
-                        pr[tile] = tb_mlp_eval32_mfma(W.mlpk, xq);
+                    for (int j = 0; j < TB_MLP_NIN; j++) f[j] = 0.0f;
+                    if (lane < ne) {
+                        const uint32_t nd = L.bandq[lane];
+                        const double4 m = D.bmac[nd];
+                        const float ex = fmaxf(fabsf((float)(m.x - gx)) - hx, 0.0f);
+                        const float ey = fmaxf(fabsf((float)(m.y - gy)) - hy, 0.0f);
+                        const float ez = fmaxf(fabsf((float)(m.z - gz)) - hz, 0.0f);
+                        band_features(D.bf[nd], D.bq2[nd].w, sqrtf(ex * ex + ey * ey + ez * ez), hdiag, klemin,
+                                      D.bq0[nd].x * qt >= 0.0f, knact, f);
                     }
+                    float xa[4], xb[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const float lo4 = __shfl_xor(f[q], 32, 64), hi4 = __shfl_xor(f[4 + q], 32, 64);
+                        xa[q] = lane < 32 ? f[q] : hi4;   // tile 0: entries 0..31
+                        xb[q] = lane < 32 ? lo4 : f[4 + q];  // tile 1: entries 32..63
+                    }
+                    float pr[2];
+                    pr[0] = tb_mlp_eval32_mfma(W.mlpk, xa);
+                    pr[1] = tb_mlp_eval32_mfma(W.mlpk, xb);
                     // lane l decides entry l (tile l / 32 holds it); without a
                     // previous force (emin = 0) band cells are opened
                     const bool mine = lane < ne;
