@@ -74,7 +74,7 @@ typedef struct tb_config {
     double far_ratio;  /* TB_WALK_MIXED: an accepted cell farther than far_ratio    */
                        /* times the target group's radius from the group's centre */
                        /* goes through the group's local expansion instead of     */
-                       /* being evaluated at every target (0: default 5, < 0:   */
+                       /* being evaluated at every target (0: default 6, < 0:   */
                        /* off, else >= 2)                                        */
 } tb_config;
 

@@ -65,10 +65,12 @@ constexpr int FAR_ORDER = 4;
 constexpr int NLOC = FAR_ORDER == 3 ? TB_LOCAL_N3 : TB_LOCAL_N4H;
 // default threshold: cell distance / group radius.  The field's relative
 // truncation error of the order-4 expansion is about 5 (1/K)^4 at the
-// group's edge (8e-3 at K = 5), and the far cells' third moments remove
-// most of their M2P truncation error; measured at N = 4M and 1e-4 rms,
-// K = 5 gave the fastest walk (K = 4 needs a much smaller theta).
-constexpr double FAR_RATIO_DEFAULT = 5.0;
+// group's edge (4e-3 at K = 6), and the far cells' third moments remove
+// most of their M2P truncation error.  Measured at N = 4M, groups of 128,
+// 1e-4 rms: K = 6 lets theta reach its value without the far path (0.275)
+// and gave the fastest walk, 72 ms (K = 5: theta 0.262, 74 ms; K = 8:
+// 76 ms; off: 96 ms; K = 4 needs theta below 0.18).
+constexpr double FAR_RATIO_DEFAULT = 6.0;
 
 struct WaveLDS {
     uint32_t stack[STACK];
