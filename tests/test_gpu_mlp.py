@@ -74,36 +74,37 @@ def _direct_err(eng, qel, qion, n):
     return float(np.sqrt(np.mean(rel ** 2)))
 
 
-def test_learned_mac_vs_fixed():
+@pytest.mark.parametrize("theta", [0.26, 0.5])
+def test_learned_mac_gain_over_fixed(theta):
     """Trained on labels the method generates, the learned criterion is tuned
     (err_target bisected in log10) to the fixed-theta walk's rms error,
     sampled against the FP64 direct sum, and its work (M2P + P2P
-    interactions) is compared there: 2^17 + 2^17 particles, fixed walk at
-    theta 0.26 (the 1e-4 regime of the walk with third moments); learned walk
-    with cells beyond s/0.208 always accepted, cells within s/0.325 always
-    opened, the network deciding near band cells in between (far ones are
-    decided at the band's geometric mean, 0.26, as the fixed walk decides
-    them).  The tuning must reach the fixed walk's error, and the work must
-    stay within 10% of the fixed walk's.  With the third moment in the M2P the
-    fixed criterion is the better one here (measured: 1.05x its work); with
-    quadrupole-only cells the network saved 4-13% (earlier builds)."""
-    announce("learned_mac_vs_fixed")
+    interactions) is compared there: 2^17 + 2^17 particles; cells beyond
+    s/(0.97 theta) always accepted, cells within s/(1.03 theta) always opened,
+    the network deciding the near band cells in between (far ones are decided
+    at the band's geometric mean, as the fixed walk decides them).  The
+    learned walk must reach the fixed walk's error with less work.  Measured:
+    0.985x (theta 0.26) and 0.977x (theta 0.5) at this seed, 0.995x / 0.989x
+    at another.  Wider bands lose with the third moment in the M2P: theta x
+    0.8 .. 1.25 needed 1.04-1.16x the fixed walk's work."""
+    announce("learned_mac_gain theta=%g" % theta)
     n = 1 << 17
     qel, qion = ta.gen_plasma(31, n, n, 300.0)
-    with _eng(theta=0.26, theta_mixed=0.325, theta_lo=0.208, walk="mixed", err_target=1e-7, mac="learned") as eng:
+    with _eng(theta=theta, theta_mixed=1.03 * theta, theta_lo=0.97 * theta, walk="mixed", err_target=1e-7,
+              mac="learned") as eng:
         eng.set_particles(qel, qion)
         eng.build()
         X, y = eng.mlp_collect(cap=1 << 20)
         assert len(X) > 5000
         eng.mlp_set(mlp_ref.init_params(0).astype(np.float32))
         eng.mlp_train(X, y, steps=3000, batch=8192, lr=2e-3)
-        eng.configure(mac="fixed", theta_mixed=0.26)
+        eng.configure(mac="fixed", theta_mixed=theta)
         err_fix = _direct_err(eng, qel, qion, 2048)
         st = eng.count_interactions()
         work_fix = sum(st["m2p"]) + sum(st["p2p"])
-        eng.configure(mac="learned", theta_mixed=0.325)
-        lo, hi, best, rows = -10.0, -3.0, None, []
-        for _ in range(10):
+        eng.configure(mac="learned", theta_mixed=1.03 * theta)
+        lo, hi, best, rows = -10.0, -2.0, None, []
+        for _ in range(12):
             mid = 0.5 * (lo + hi)
             eng.configure(err_target=10.0 ** mid)
             err_l = _direct_err(eng, qel, qion, 2048)
@@ -117,4 +118,4 @@ def test_learned_mac_vs_fixed():
             else:
                 hi = mid
     assert best is not None, rows
-    assert best[2] < 1.10 * work_fix, (best, work_fix, rows)
+    assert best[2] < work_fix, (best, work_fix, rows)
