@@ -44,9 +44,9 @@ import tree_amd as ta  # noqa: E402
 W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # mixed walk, per target-entry interaction, counted from walk32.hip's compiled
 # FP32 code (monopole + dipole + traceless quadrupole + traceless third moment
-# M2P 122, Coulomb P2P 18, Kelbg P2P 30); MAC tests are per cell and group and
+# M2P 122, Coulomb P2P 18, Kelbg P2P 22); MAC tests are per cell and group and
 # are not counted
-W32_M2P, W32_P2P_C, W32_P2P_K = 122, 18, 30
+W32_M2P, W32_P2P_C, W32_P2P_K = 122, 18, 22
 # far cells: flops per (cell, target group) pair of the order-4 local
 # expansion (FMA = 2), counted from the compiled code of m2l_acc_p4h
 W32_M2L_PAIR = 1003

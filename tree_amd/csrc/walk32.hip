@@ -57,6 +57,13 @@ constexpr int CTR_CHUNK = 8;  // first chunk counter in the counter block (16 in
 constexpr int CTR_FAR = 4;    // far-chunk allocation counter
 constexpr int FAR_SLAB = 256; // far chunks a wave reserves at a time
 constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decided by the network
+// A cell of at most SMALL_OPEN particles is always opened: its particles'
+// P2P is exact and, for two particles, cheaper than one M2P with the third
+// moment (~80 VALU instructions per target).  Measured at N = 4M, 1e-4 rms,
+// theta re-tuned: walk 70.0 ms without the rule, 67.7 ms with it (3 and 4
+// particles: no further gain).  Far cells are opened too, so that the far
+// path changes how accepted cells are evaluated, never which are accepted.
+constexpr int SMALL_OPEN = 2;
 
 const char *g_err = "";
 
@@ -80,7 +87,7 @@ struct WaveLDS {
     // accepted cells: x y z q | Dx Dy Dz Q'xx | Q'yy Q'xy Q'yz Q'zx |
     // O'xxx O'xxy O'xxz O'xyy | O'xyz O'yyy O'yyz -
     float4 m0[LCAP_M], m1[LCAP_M], m2[LCAP_M], m3[LCAP_M], m4[LCAP_M];
-    float4 p[LCAP_P];                     // leaves: x y z q
+    float4 p[LCAP_P];                     // leaves x y z q: like charges from 0 up, unlike from LCAP_P - 1 down
 };
 // measured: 4 x 13056 bytes per block kept 3 blocks on a CU, 4 x 13568 did not
 static_assert(sizeof(WaveLDS) * 4 <= 4 * 13056, "walk LDS would drop a CU to two blocks");
@@ -280,53 +287,59 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     }
 }
 
-// leaves: p.w is the source charge; like charges (same species as the
-// group's targets) use Coulomb beyond the 0.1 um cut, unlike ones Kelbg
+// Leaves (one particle each, charge +-1).  Like charges (the targets'
+// species) sit at the bottom of L.p, [0, nl), unlike ones at the top,
+// [LCAP_P - nu, LCAP_P): the Coulomb loop (with the 0.1 um cut) and the Kelbg
+// loop then run without a per-entry branch on the charge.  Every like source
+// has charge qt and every unlike one -qt, so both loops sum q-free terms and
+// the sign goes on once at the end.
 template <int TPL>
-__device__ inline void flush_p2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
+__device__ inline void flush_p2p(const WaveLDS &L, int nl, int nu, const float (&rx)[TPL], const float (&ry)[TPL],
                                  const float (&rz)[TPL], float qt, float inv_rs, Acc<TPL> &A)
 {
     float ax[TPL], ay[TPL], az[TPL], ap[TPL];
 #pragma unroll
     for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
-    for (int j = 0; j < cnt; j++) {
+#pragma unroll 4
+    for (int j = 0; j < nl; j++) {
         const float4 q = L.p[j];
-        if (q.w == qt) {
 #pragma unroll
-            for (int t = 0; t < TPL; t++) {
-                const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                // also drops the target itself; beyond the cut r2 is normal (raw v_rsq_f32)
-                const float ri = r2 > 0.01f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-                const float w = q.w * ri * ri * ri;
-                ax[t] += dx * w;
-                ay[t] += dy * w;
-                az[t] += dz * w;
-                ap[t] += q.w * ri;
-            }
-        } else {
+        for (int t = 0; t < TPL; t++) {
+            const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            // also drops the target itself; beyond the cut r2 is normal (raw v_rsq_f32)
+            const float ri = r2 > 0.01f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+            const float w = ri * ri * ri;
+            ax[t] = fmaf(dx, w, ax[t]);
+            ay[t] = fmaf(dy, w, ay[t]);
+            az[t] = fmaf(dz, w, az[t]);
+            ap[t] += ri;
+        }
+    }
+#pragma unroll 2
+    for (int j = LCAP_P - nu; j < LCAP_P; j++) {
+        const float4 q = L.p[j];
 #pragma unroll
-            for (int t = 0; t < TPL; t++) {
-                const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float ri = rsqrtf(r2);
-                const float u = r2 * ri * inv_rs;
-                float g, h;
-                kelbg_gh(u, g, h);
-                const float w = q.w * g * ri * ri * ri;
-                ax[t] += dx * w;
-                ay[t] += dy * w;
-                az[t] += dz * w;
-                ap[t] += q.w * h * ri;
-            }
+        for (int t = 0; t < TPL; t++) {
+            const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const float u = r2 * ri * inv_rs;  // r / rs
+            float g = 1.0f, h = 1.0f;
+            if (u < 22.0f) kelbg_gh(u, g, h);  // a close pair: the Kelbg factors differ from 1
+            const float w = -g * ri * ri * ri;
+            ax[t] = fmaf(dx, w, ax[t]);
+            ay[t] = fmaf(dy, w, ay[t]);
+            az[t] = fmaf(dz, w, az[t]);
+            ap[t] = fmaf(-h, ri, ap[t]);
         }
     }
 #pragma unroll
     for (int t = 0; t < TPL; t++) {
-        nadd(A.s[t][0], A.c[t][0], ax[t]);
-        nadd(A.s[t][1], A.c[t][1], ay[t]);
-        nadd(A.s[t][2], A.c[t][2], az[t]);
-        nadd(A.ps[t], A.pc[t], ap[t]);
+        nadd(A.s[t][0], A.c[t][0], qt * ax[t]);
+        nadd(A.s[t][1], A.c[t][1], qt * ay[t]);
+        nadd(A.s[t][2], A.c[t][2], qt * az[t]);
+        nadd(A.ps[t], A.pc[t], qt * ap[t]);
     }
 }
 
@@ -494,7 +507,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         const float far_r2 = uni(W.far_k2 > 0.0f ? W.far_k2 * hdiag * hdiag : 3.0e38f);
         F.head = -1;
 
-        int sp = 1, nm = 0, np = 0, nb = 0, nf = 0;  // stack, M2P, P2P, band-queue, far-queue lengths
+        // stack, M2P, P2P (like, unlike), band-queue, far-queue lengths
+        int sp = 1, nm = 0, npl = 0, npu = 0, nb = 0, nf = 0;
         L.stack[0] = 0u;  // root: range of one cell at 0
         wave_sync();
         while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
@@ -537,7 +551,9 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     d2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     const float rc = (float)m.w;  // s / theta_mixed
                     const bool far_hi = d2 > rc * rc * 1.000004f;
-                    if (MODE == MODE_FIXED) {
+                    if (D.bq2[node].z <= (float)SMALL_OPEN) {
+                        open = true;  // its particles' P2P is exact and cheaper than its M2P
+                    } else if (MODE == MODE_FIXED) {
                         acc = far_hi;
                         open = !far_hi;
                     } else {
@@ -599,15 +615,17 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 L.m4[k] = D.bq4[node];
             }
             nm += __popcll(mk);
-            // leaves -> P2P list with their charge
-            mk = __ballot(leaf);
+            // leaves -> P2P list: like charges from the bottom, unlike from the top
             const float qleaf = leaf ? D.bq0[node].x : 0.0f;
-            if (leaf) L.p[np + __popcll(mk & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
-            np += __popcll(mk);
+            const bool lk = leaf && qleaf == qt;
+            const uint64_t ml = __ballot(lk), mu = __ballot(leaf && !lk);
+            if (lk) L.p[npl + __popcll(ml & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
+            else if (leaf) L.p[LCAP_P - 1 - npu - __popcll(mu & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
+            npl += __popcll(ml);
+            npu += __popcll(mu);
             if (COUNT) {
-                const int nl = __popcll(__ballot(leaf && qleaf == qt));
-                c_like[tsp] += (unsigned long long)nl * nact;
-                c_unlike[tsp] += (unsigned long long)(__popcll(mk) - nl) * nact;
+                c_like[tsp] += (unsigned long long)__popcll(ml) * nact;
+                c_unlike[tsp] += (unsigned long long)__popcll(mu) * nact;
             }
             // opened cells -> their child ranges.  Children follow their
             // parent in breadth-first order, so a range not strictly after the
@@ -625,7 +643,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             if (sp + npush > STACK) {
                 atomicOr(&W.ctr[1], 1);
                 sp = 0;
-                nm = np = 0;
+                nm = npl = npu = 0;
                 break;
             }
             if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
@@ -637,9 +655,9 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 nm = 0;
                 wave_sync();
             }
-            if (np >= FLUSH_P) {
-                flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
-                np = 0;
+            if (npl + npu >= FLUSH_P) {
+                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A);
+                npl = npu = 0;
                 wave_sync();
             }
             if (nf >= 64) {
@@ -733,7 +751,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     if (sp + bpush > STACK) {
                         atomicOr(&W.ctr[1], 1);
                         sp = 0;
-                        nm = np = nb = 0;
+                        nm = npl = npu = nb = 0;
                         break;
                     }
                     if (bopen) L.stack[sp + __popcll(bk & lanemask_lt())] = binf;
@@ -765,8 +783,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
             flush_m2p<TPL>(L, nm, rx, ry, rz, A);
         }
-        if (np > 0) {
-            flush_p2p<TPL>(L, np, rx, ry, rz, qt, D.inv_rs, A);
+        if (npl + npu > 0) {
+            flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A);
         }
         if (nf > 0) push_far(L, nf, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
         W.far_head[g] = F.head;  // every lane stores the same value
