@@ -21,8 +21,8 @@ def _err(f, fd):
     return np.sqrt(np.mean(rel ** 2)), rel.max()
 
 
-def _run(qel, qion, theta, group=0, walk="mixed"):
-    with ta.Engine(rs=RS10, theta=theta, theta_mixed=theta, walk=walk, group=group) as eng:
+def _run(qel, qion, theta, group=0, walk="mixed", rs=RS10):
+    with ta.Engine(rs=rs, theta=theta, theta_mixed=theta, walk=walk, group=group) as eng:
         eng.set_particles(qel, qion)
         eng.build()
         ep = eng.force()
@@ -42,6 +42,24 @@ def test_mixed_tiny_theta_is_fp32_direct():
     f, fd, _ = _run(qel, qion, 1e-9)
     rms, mx = _err(f, fd)
     assert rms < 3e-6 and mx < 5e-5, (rms, mx)
+
+
+@pytest.mark.parametrize("theta", [1e-9, 0.3])
+def test_mixed_close_kelbg_pairs(theta):
+    """A Kelbg length of 5 um puts most unlike pairs of a 100 um cloud within
+    22 rs, where the Kelbg factors differ from 1 and the P2P flush adds them
+    in its close-pair pass.  theta -> 0 is then the FP32 direct sum with the
+    reference's pair rules: rms < 3e-6, max < 5e-5; theta 0.3 stays within
+    the walk's usual error there: rms < 1e-3."""
+    announce("mixed_close_kelbg theta=%g" % theta)
+    qel, qion = ta.gen_plasma(24, 3000, 3000, 100.0)
+    f, fd, _ = _run(qel, qion, theta, rs=5.0)
+    rms, mx = _err(f, fd)
+    print("rs 5 um, theta %g: rms %.3e max %.3e" % (theta, rms, mx))
+    if theta < 1e-6:
+        assert rms < 3e-6 and mx < 5e-5, (rms, mx)
+    else:
+        assert rms < 1e-3, (rms, mx)
 
 
 @pytest.mark.parametrize("group", [64, 128, 256])

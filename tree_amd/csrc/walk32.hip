@@ -316,22 +316,50 @@ __device__ inline void flush_p2p(const WaveLDS &L, int nl, int nu, const float (
             ap[t] += ri;
         }
     }
-#pragma unroll 2
+    // unlike pairs: beyond 22 rs the Kelbg factors are 1 in FP32 and the term
+    // is Coulomb's; a closer pair (rare: at 1e14-1e15 m^-3 about one per
+    // thousand flushes) is left out here and added by the pass below, which
+    // runs only when some lane of the wave has one
+    bool close = false;
+#pragma unroll 4
     for (int j = LCAP_P - nu; j < LCAP_P; j++) {
         const float4 q = L.p[j];
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float ri = __builtin_amdgcn_rsqf(r2);
-            const float u = r2 * ri * inv_rs;  // r / rs
-            float g = 1.0f, h = 1.0f;
-            if (u < 22.0f) kelbg_gh(u, g, h);  // a close pair: the Kelbg factors differ from 1
-            const float w = -g * ri * ri * ri;
-            ax[t] = fmaf(dx, w, ax[t]);
-            ay[t] = fmaf(dy, w, ay[t]);
-            az[t] = fmaf(dz, w, az[t]);
-            ap[t] = fmaf(-h, ri, ap[t]);
+            // the same operations as in the pass below, so both passes split
+            // the pairs alike
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float rq = __builtin_amdgcn_rsqf(r2);
+            const bool cl = r2 * rq * inv_rs < 22.0f;  // r / rs
+            close = close || cl;
+            const float ri = cl ? 0.0f : rq;
+            const float w = ri * ri * ri;
+            ax[t] = fmaf(-dx, w, ax[t]);
+            ay[t] = fmaf(-dy, w, ay[t]);
+            az[t] = fmaf(-dz, w, az[t]);
+            ap[t] -= ri;
+        }
+    }
+    if (__ballot(close)) {
+        for (int j = LCAP_P - nu; j < LCAP_P; j++) {
+            const float4 q = L.p[j];
+#pragma unroll
+            for (int t = 0; t < TPL; t++) {
+                const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float rq = __builtin_amdgcn_rsqf(r2);
+                const float u = r2 * rq * inv_rs;
+                if (u < 22.0f) {
+                    float g, h;
+                    kelbg_gh(u, g, h);
+                    const float w = g * rq * rq * rq;
+                    ax[t] = fmaf(-dx, w, ax[t]);
+                    ay[t] = fmaf(-dy, w, ay[t]);
+                    az[t] = fmaf(-dz, w, az[t]);
+                    ap[t] = fmaf(-h, rq, ap[t]);
+                }
+            }
         }
     }
 #pragma unroll
