@@ -47,9 +47,10 @@ W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # M2P 122, Coulomb P2P 18, Kelbg P2P 22); MAC tests are per cell and group and
 # are not counted
 W32_M2P, W32_P2P_C, W32_P2P_K = 122, 18, 22
-# far cells: flops per (cell, target group) pair of the order-4 local
-# expansion (FMA = 2), counted from the compiled code of m2l_acc_p4h
-W32_M2L_PAIR = 1003
+# far cells: flops per (cell, target group) pair of the order-6 local
+# expansion with the cell's third moment (FMA = 2), counted from the compiled
+# code of m2l_acc_p6ho in k_far_eval (2065 VALU instructions)
+W32_M2L_PAIR = 3460
 # TFLOP/s, MI355X published vector peaks.  The FP32 figure counts packed FP32
 # (two lanes per instruction); this build keeps packed FP32 off, so one
 # instruction per lane per cycle (78.6) is its own ceiling, stated beside it.
@@ -548,6 +549,10 @@ def main():
                 out[k] = v[k]
         if "st" in v:
             out["interactions_per_s"] = interactions(v["st"]) / v["s"]
+            st = v["st"]
+            out["counts_per_force"] = {"m2p": int(sum(st["m2p"])), "p2p": int(sum(st["p2p"])),
+                                       "far_m2l": int(sum(st.get("m2l", [0]))),
+                                       "far_pairs": int(sum(st.get("far_pairs", [0])))}
         return out
 
     res = {

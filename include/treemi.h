@@ -74,7 +74,7 @@ typedef struct tb_config {
     double far_ratio;  /* TB_WALK_MIXED: an accepted cell farther than far_ratio    */
                        /* times the target group's radius from the group's centre */
                        /* goes through the group's local expansion instead of     */
-                       /* being evaluated at every target (0: default 6, < 0:   */
+                       /* being evaluated at every target (0: default 3, < 0:   */
                        /* off, else >= 2)                                        */
 } tb_config;
 
@@ -124,7 +124,7 @@ int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const in
  * counterpart; tests only).  cells [ncell][19]: R = expansion centre - cell
  * centre (3), q, D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx (traceless quadrupole),
  * O'xxx, O'xxy, O'xxz, O'xyy, O'xyz, O'yyy, O'yyz (traceless third moment;
- * order 4 only).  The cells' local coefficients of order `order` (3 or 4)
+ * orders 4 to 6).  The cells' local coefficients of order `order` (3 to 6)
  * are summed across a wavefront as in the walk and evaluated at the offsets
  * y [ny][3]: out [ny][4] = potential, field x, y, z (FP32).
  */

@@ -870,7 +870,7 @@ int tb_local_eval(tb_handle *h, int32_t order, int32_t ncell, const float *cells
                   float *out)
 {
     if (!h || !cells || !y || !out) return fail(TB_ERR_ARG, "null argument");
-    if (order != 3 && order != 4) return fail(TB_ERR_ARG, "order must be 3 or 4 (got %d)", order);
+    if (order < 3 || order > 6) return fail(TB_ERR_ARG, "order must be 3 to 6 (got %d)", order);
     if (ncell < 1 || ny < 1 || ncell > (1 << 20) || ny > (1 << 20))
         return fail(TB_ERR_ARG, "ncell and ny must be in 1 .. 2^20");
     HIPCHK(hipSetDevice(h->cfg.device));

@@ -189,24 +189,28 @@ def main():
              "//",
              "// Far-field local expansion of the mixed walk: Taylor coefficients of",
              "// 1/|R + y| by their three-term recurrence, contracted with a cell's",
-             "// net charge, dipole and traceless quadrupole into the local",
-             "// coefficients L_b of the potential about a target group's centre",
-             "// (|b| <= P, multi-indices in the order of all_indices), and the",
-             "// evaluation of potential and field at a target.",
+             "// net charge, dipole, traceless quadrupole and (orders 4 to 6, the",
+             "// harmonic forms) traceless third moment into the local coefficients",
+             "// L_b of the potential about a target group's centre (|b| <= P,",
+             "// multi-indices in the order of all_indices; the harmonic forms keep",
+             "// b_z <= 1), and the evaluation of potential and field at a target.",
              "#pragma once",
              "#include <hip/hip_runtime.h>",
              ""]
     lines.append("constexpr int TB_LOCAL_N3 = %d;   // order 3, all coefficients" % len(all_indices(3)))
     lines.append("constexpr int TB_LOCAL_N4H = %d;  // order 4, independent coefficients only" % len(harmonic_indices(4)))
+    lines.append("constexpr int TB_LOCAL_N5H = %d;  // order 5, independent coefficients only" % len(harmonic_indices(5)))
+    lines.append("constexpr int TB_LOCAL_N6H = %d;  // order 6, independent coefficients only" % len(harmonic_indices(6)))
     lines.append("")
     lines += emit_m2l(3)
     lines.append("")
     lines += emit_l2p(3)
     lines.append("")
-    lines += emit_m2l(4, harmonic=True, octupole=True)
-    lines.append("")
-    lines += emit_l2p(4, harmonic=True)
-    lines.append("")
+    for P in (4, 5, 6):
+        lines += emit_m2l(P, harmonic=True, octupole=True)
+        lines.append("")
+        lines += emit_l2p(P, harmonic=True)
+        lines.append("")
     with open(os.path.join(HERE, "local_exp.h"), "w") as f:
         f.write("\n".join(lines))
 

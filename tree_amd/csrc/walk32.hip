@@ -68,16 +68,19 @@ constexpr int SMALL_OPEN = 2;
 const char *g_err = "";
 
 // far-field local expansion: order and coefficient count
-constexpr int FAR_ORDER = 4;
-constexpr int NLOC = FAR_ORDER == 3 ? TB_LOCAL_N3 : TB_LOCAL_N4H;
+constexpr int FAR_ORDER = 6;
+constexpr int NLOC = FAR_ORDER == 3   ? TB_LOCAL_N3
+                     : FAR_ORDER == 4 ? TB_LOCAL_N4H
+                     : FAR_ORDER == 5 ? TB_LOCAL_N5H
+                                      : TB_LOCAL_N6H;
 // default threshold: cell distance / group radius.  The field's relative
-// truncation error of the order-4 expansion is about 5 (1/K)^4 at the
-// group's edge (4e-3 at K = 6), and the far cells' third moments remove
-// most of their M2P truncation error.  Measured at N = 4M, groups of 128,
-// 1e-4 rms: K = 6 lets theta reach its value without the far path (0.275)
-// and gave the fastest walk, 72 ms (K = 5: theta 0.262, 74 ms; K = 8:
-// 76 ms; off: 96 ms; K = 4 needs theta below 0.18).
-constexpr double FAR_RATIO_DEFAULT = 6.0;
+// truncation error of the order-P expansion falls as (1/K)^P at the group's
+// edge, and a higher order costs more per far cell.  Measured at N = 4M,
+// groups of 128, theta re-tuned to 1e-4 rms for each (order, K), walk + far
+// evaluation: order 4 at K = 6, 64.5 ms; order 5 at K = 4, 58.4 ms; order 6
+// at K = 3, 55.5 ms (K = 3.5: 57.1; K = 2.5 needs theta below 0.22); order 7
+// at K = 2.5 or 3, 65 ms (its far evaluation costs more than it saves).
+constexpr double FAR_RATIO_DEFAULT = 3.0;
 
 struct WaveLDS {
     uint32_t stack[STACK];
@@ -371,15 +374,21 @@ __device__ inline void flush_p2p(const WaveLDS &L, int nl, int nu, const float (
     }
 }
 
-// q0..q2: q, D, traceless quadrupole; q3, q4: traceless third moment (order 4 only)
+// q0..q2: q, D, traceless quadrupole; q3, q4: traceless third moment (orders 4 to 6)
 template <int N>
 __device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4,
                                float (&L)[N])
 {
     if constexpr (N == TB_LOCAL_N3)
         m2l_acc_p3(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, L);
-    else
+    else if constexpr (N == TB_LOCAL_N4H)
         m2l_acc_p4ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
+                     q4.y, q4.z, L);
+    else if constexpr (N == TB_LOCAL_N5H)
+        m2l_acc_p5ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
+                     q4.y, q4.z, L);
+    else
+        m2l_acc_p6ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
                      q4.y, q4.z, L);
 }
 
@@ -388,8 +397,12 @@ __device__ inline void l2p(const float (&L)[N], float x, float y, float z, float
 {
     if constexpr (N == TB_LOCAL_N3)
         l2p_p3(L, x, y, z, phi, ex, ey, ez);
-    else
+    else if constexpr (N == TB_LOCAL_N4H)
         l2p_p4h(L, x, y, z, phi, ex, ey, ez);
+    else if constexpr (N == TB_LOCAL_N5H)
+        l2p_p5h(L, x, y, z, phi, ex, ey, ez);
+    else
+        l2p_p6h(L, x, y, z, phi, ex, ey, ez);
 }
 
 // Far cells are evaluated after the walk (k_far_eval), so their local
