@@ -73,7 +73,7 @@ def _cell(rng, n=40, b=0.3):
     return X, X + s, q, q.sum(), q @ s, Qp
 
 
-@pytest.mark.parametrize("P", [3, 4])
+@pytest.mark.parametrize("P", [3, 4, 5, 6])
 def test_local_expansion_converges_to_cell_field(P):
     rng = np.random.default_rng(10 + P)
     X, _, _, q, D, Qp = _cell(rng)
@@ -135,14 +135,15 @@ def test_moment_trace_drops_out():
         assert abs(extra) <= 1e-12 * max(1.0, abs(L1[b]))
 
 
-def test_local_coefficients_harmonic():
+@pytest.mark.parametrize("P", [4, 6])
+def test_local_coefficients_harmonic(P):
     # each homogeneous part of the local expansion is harmonic:
     # sum_i (b_i + 2)(b_i + 1) L_{b + 2 e_i} = 0 for |b| <= P - 2.  The walk
     # forms only the coefficients with b_z <= 1 and recovers the others from this.
     rng = np.random.default_rng(8)
-    X, _, _, q, D, Qp = _cell(rng)
-    L = lr.m2l(X, q, D, Qp, X + np.array([4.0, -2.0, 3.0]), 4)
-    for b in lr.all_indices(2):
+    X, xs, qs, q, D, Qp = _cell(rng)
+    L = lr.m2l(X, q, D, Qp, X + np.array([4.0, -2.0, 3.0]), P, O=lr.third_moment(xs - X, qs))
+    for b in lr.all_indices(P - 2):
         terms = []
         for i in range(3):
             c = list(b)
