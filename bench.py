@@ -48,9 +48,10 @@ W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # are not counted
 W32_M2P, W32_P2P_C, W32_P2P_K = 122, 18, 22
 # far cells: flops per (cell, target group) pair of the order-6 local
-# expansion with the cell's third moment (FMA = 2), counted from the compiled
-# code of m2l_acc_p6ho in k_far_eval (2065 VALU instructions)
-W32_M2L_PAIR = 3460
+# expansion with the cell's third moment, terms to total order 7 (FMA = 2),
+# counted from the compiled loop of k_far_eval (m2l_acc_p6hot7; 1321 VALU
+# instructions with the cell-record loads)
+W32_M2L_PAIR = 2177
 # TFLOP/s, MI355X published vector peaks.  The FP32 figure counts packed FP32
 # (two lanes per instruction); this build keeps packed FP32 off, so one
 # instruction per lane per cycle (78.6) is its own ceiling, stated beside it.

@@ -28,7 +28,10 @@ with M_0 = q, M_{e_i} = D_i, M_c = Q'_c / 3 for |c| = 2 and M_c = O_c for
 |c| = 3.  Trace parts of the moment tensors drop out because 1/r is
 harmonic, so Q'/3 may stand for M and the traceless part of O for O.  The
 field is E = -grad phi.  Keeping |b| <= P gives the field through order
-P - 1 in y.
+P - 1 in y.  With tmax, only the terms of total order |b| + |c| <= tmax are
+kept (the walk's order-6 expansion keeps tmax = 7: the dropped terms, a
+higher moment times a high power of y, are below the order-P truncation for
+cells accepted by the walk).
 """
 from math import comb, factorial
 
@@ -105,9 +108,10 @@ def traceless3(O):
     return O - (np.einsum("ij,k->ijk", d, t) + np.einsum("ik,j->ijk", d, t) + np.einsum("jk,i->ijk", d, t)) / 5.0
 
 
-def m2l(X, q, D, Qp, g, P, O=None):
+def m2l(X, q, D, Qp, g, P, O=None, tmax=None):
     """Local coefficients L_b (|b| <= P) about g of a cell at X with moments
-    q, D, Q' (and the third moment O)."""
+    q, D, Q' (and the third moment O); with tmax, only the terms with
+    |b| + |c| <= tmax."""
     R = np.asarray(g, dtype=np.float64) - np.asarray(X, dtype=np.float64)
     T = taylor_coeffs(R, P + (3 if O is not None else 2))
     M = cell_moments(q, D, Qp, O)
@@ -115,6 +119,8 @@ def m2l(X, q, D, Qp, g, P, O=None):
     for b in all_indices(P):
         s = 0.0
         for c, mc in M.items():
+            if tmax is not None and sum(b) + sum(c) > tmax:
+                continue
             a = (b[0] + c[0], b[1] + c[1], b[2] + c[2])
             s += _c(b, c) * (-1.0) ** sum(c) * T[a] * mc
         L[b] = s

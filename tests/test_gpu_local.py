@@ -34,7 +34,8 @@ def test_local_expansion_vs_oracle(order):
     """FP32 coefficients summed over 150 cells (a partial last pass of the
     wavefront) and evaluated at 70 offsets: within 2e-5 of the FP64 oracle,
     relative to the field's scale.  Orders 4 to 6 (the harmonic forms; 6 is
-    the walk's) include the cells' third moments."""
+    the walk's, with the terms to total order 7) include the cells' third
+    moments."""
     announce("local_eval order %d" % order)
     rng = np.random.default_rng(order)
     cells, R, q, D, Qp, O = _cells(rng, 150, 5.0, 20.0)
@@ -43,7 +44,8 @@ def test_local_expansion_vs_oracle(order):
         out = eng.local_eval(order, cells, y)
     L = {}
     for k in range(len(cells)):
-        Lk = lr.m2l(-R[k], q[k], D[k], Qp[k], np.zeros(3), order, O=O[k] if order >= 4 else None)
+        Lk = lr.m2l(-R[k], q[k], D[k], Qp[k], np.zeros(3), order, O=O[k] if order >= 4 else None,
+                    tmax=7 if order == 6 else None)
         for b, v in Lk.items():
             L[b] = L.get(b, 0.0) + v
     ref = np.array([[*lr.l2p(L, yy)[:1], *lr.l2p(L, yy)[1]] for yy in y])

@@ -210,3 +210,35 @@ def test_octupole_m2p_vs_point_charges_and_gradient():
         errs.append(e3)
     # order: doubling d divides the octupole-truncated error by about 2^6 (field ~ d^-2 * (b/d)^4)
     assert errs[0] / errs[1] > 0.5 * 2 ** 6
+
+
+def test_total_order_truncation():
+    # The walk's order-6 expansion keeps only the terms with |b| + |c| <= 7.
+    # tmax = P + 3 keeps every term; and for a cell at the walk's opening
+    # angle (extent / distance ~ 0.27) seen from a third of the distance,
+    # what the truncation drops stays below the order-6 truncation itself.
+    rng = np.random.default_rng(12)
+    X, xs, qs, q, D, Qp = _cell(rng, n=60, b=0.25)
+    O = lr.third_moment(xs - X, qs)
+    g = X + np.array([1.1, -0.9, 0.8])
+    d = np.linalg.norm(g - X)
+    assert 0.25 * np.sqrt(3) / d < 0.35
+    full = lr.m2l(X, q, D, Qp, g, 6, O=O)
+    same = lr.m2l(X, q, D, Qp, g, 6, O=O, tmax=9)
+    for k in full:
+        assert same[k] == full[k]
+    t7 = lr.m2l(X, q, D, Qp, g, 6, O=O, tmax=7)
+    t6 = lr.m2l(X, q, D, Qp, g, 6, O=O, tmax=6)
+    worst_cut = worst_t6 = worst_trunc = 0.0
+    for _ in range(20):
+        u = rng.normal(size=3)
+        y = d / 3.0 * u / np.linalg.norm(u)
+        dr = g + y - xs
+        r = np.linalg.norm(dr, axis=1)
+        E_dir = (qs[:, None] * dr / r[:, None] ** 3).sum(0)
+        E_full = lr.l2p(full, y)[1]
+        worst_trunc = max(worst_trunc, np.linalg.norm(E_full - E_dir))
+        worst_cut = max(worst_cut, np.linalg.norm(lr.l2p(t7, y)[1] - E_full))
+        worst_t6 = max(worst_t6, np.linalg.norm(lr.l2p(t6, y)[1] - E_full))
+    assert worst_cut < worst_trunc, (worst_cut, worst_trunc)
+    assert worst_cut < worst_t6  # each further order dropped costs more

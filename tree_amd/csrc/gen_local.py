@@ -46,11 +46,15 @@ def harmonic_indices(P):
     return [b for b in all_indices(P) if b[2] <= 1]
 
 
-def emit_m2l(P, harmonic=False, octupole=False):
+def emit_m2l(P, harmonic=False, octupole=False, tmax=None):
+    """Keeps the terms m_c T_{b+c} with |b| <= P and total order |b| + |c| <=
+    tmax (Taylor coefficients to order tmax); tmax None: every moment with
+    every |b| <= P (to order P + 3)."""
     L = harmonic_indices(P) if harmonic else all_indices(P)
     lidx = {b: i for i, b in enumerate(L)}
-    sfx = ("h" if harmonic else "") + ("o" if octupole else "")
+    sfx = ("h" if harmonic else "") + ("o" if octupole else "") + ("t%d" % tmax if tmax else "")
     cmax = 3 if octupole else 2
+    tmax = tmax or P + cmax
     # moments m_c: name and expression
     mom = {(0, 0, 0): "q", (1, 0, 0): "mdx", (0, 1, 0): "mdy", (0, 0, 1): "mdz",
            (2, 0, 0): "mxx", (0, 2, 0): "myy", (0, 0, 2): "mzz",
@@ -99,7 +103,7 @@ def emit_m2l(P, harmonic=False, octupole=False):
         return lines
 
     out += contrib((0, 0, 0))
-    for n in range(1, P + cmax + 1):
+    for n in range(1, tmax + 1):
         c1 = "%s * ir2" % fl((2 * n - 1) / n)
         c2 = "%s * ir2" % fl((n - 1) / n)
         out.append("    const float c1_%d = %s, c2_%d = %s;" % (n, c1, n, c2))
@@ -193,24 +197,31 @@ def main():
              "// harmonic forms) traceless third moment into the local coefficients",
              "// L_b of the potential about a target group's centre (|b| <= P,",
              "// multi-indices in the order of all_indices; the harmonic forms keep",
-             "// b_z <= 1), and the evaluation of potential and field at a target.",
+             "// b_z <= 1; the order-6 form keeps the terms of total order |b| + |c|",
+             "// <= 7 only), and the evaluation of potential and field at a target.",
              "#pragma once",
              "#include <hip/hip_runtime.h>",
              ""]
     lines.append("constexpr int TB_LOCAL_N3 = %d;   // order 3, all coefficients" % len(all_indices(3)))
     lines.append("constexpr int TB_LOCAL_N4H = %d;  // order 4, independent coefficients only" % len(harmonic_indices(4)))
     lines.append("constexpr int TB_LOCAL_N5H = %d;  // order 5, independent coefficients only" % len(harmonic_indices(5)))
-    lines.append("constexpr int TB_LOCAL_N6H = %d;  // order 6, independent coefficients only" % len(harmonic_indices(6)))
+    lines.append("constexpr int TB_LOCAL_N6H = %d;  // order 6 (terms to total order 7), independent coefficients only"
+                 % len(harmonic_indices(6)))
     lines.append("")
     lines += emit_m2l(3)
     lines.append("")
     lines += emit_l2p(3)
     lines.append("")
-    for P in (4, 5, 6):
+    for P in (4, 5):
         lines += emit_m2l(P, harmonic=True, octupole=True)
         lines.append("")
         lines += emit_l2p(P, harmonic=True)
         lines.append("")
+    # the walk's: order 6, terms to total order 7
+    lines += emit_m2l(6, harmonic=True, octupole=True, tmax=7)
+    lines.append("")
+    lines += emit_l2p(6, harmonic=True)
+    lines.append("")
     with open(os.path.join(HERE, "local_exp.h"), "w") as f:
         f.write("\n".join(lines))
 

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64) void k_local_eval(int ncell, const float *__res
             m2l_acc_p5ho(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
                          e[14], e[15], e[16], e[17], e[18], L);
         else
-            m2l_acc_p6ho(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
+            m2l_acc_p6hot7(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
                          e[14], e[15], e[16], e[17], e[18], L);
     }
     wave_sum_all(L);

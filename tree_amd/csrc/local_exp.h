@@ -6,14 +6,15 @@
 // harmonic forms) traceless third moment into the local coefficients
 // L_b of the potential about a target group's centre (|b| <= P,
 // multi-indices in the order of all_indices; the harmonic forms keep
-// b_z <= 1), and the evaluation of potential and field at a target.
+// b_z <= 1; the order-6 form keeps the terms of total order |b| + |c|
+// <= 7 only), and the evaluation of potential and field at a target.
 #pragma once
 #include <hip/hip_runtime.h>
 
 constexpr int TB_LOCAL_N3 = 20;   // order 3, all coefficients
 constexpr int TB_LOCAL_N4H = 25;  // order 4, independent coefficients only
 constexpr int TB_LOCAL_N5H = 36;  // order 5, independent coefficients only
-constexpr int TB_LOCAL_N6H = 49;  // order 6, independent coefficients only
+constexpr int TB_LOCAL_N6H = 49;  // order 6 (terms to total order 7), independent coefficients only
 
 // adds one cell's local coefficients about the expansion centre to L[20]
 // (order P = 3; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)
@@ -2264,7 +2265,7 @@ __device__ inline void l2p_p5h(const float (&Lh)[36], float x, float y, float z,
 
 // adds one cell's local coefficients about the expansion centre to L[49]
 // (order P = 6; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)
-__device__ inline void m2l_acc_p6ho(float rx, float ry, float rz, float q, float dx, float dy, float dz,
+__device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, float dx, float dy, float dz,
                                   float qxx, float qyy, float qxy, float qyz, float qzx,
                                   float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, float oyyy, float oyyz, float (&L)[49])
 {
@@ -3054,401 +3055,6 @@ __device__ inline void m2l_acc_p6ho(float rx, float ry, float rz, float q, float
     L[23] = fmaf(m003, t043, L[23]);
     const float t034 = -fmaf(c1_7, ry * t024 + rz * t033, c2_7 * (t014 + t032));
     L[24] = fmaf(4.0f * m003, t034, L[24]);
-    const float c1_8 = 1.875f * ir2, c2_8 = 0.875f * ir2;
-    const float t800 = -fmaf(c1_8, rx * t700, c2_8 * (t600));
-    L[36] = fmaf(28.0f * mxx, t800, L[36]);
-    L[25] = fmaf(56.0f * m300, t800, L[25]);
-    const float t710 = -fmaf(c1_8, rx * t610 + ry * t700, c2_8 * (t510));
-    L[37] = fmaf(21.0f * mxx, t710, L[37]);
-    L[36] = fmaf(7.0f * mxy, t710, L[36]);
-    L[26] = fmaf(35.0f * m300, t710, L[26]);
-    L[25] = fmaf(21.0f * m210, t710, L[25]);
-    const float t701 = -fmaf(c1_8, rx * t601 + rz * t700, c2_8 * (t501));
-    L[38] = fmaf(21.0f * mxx, t701, L[38]);
-    L[36] = fmaf(7.0f * mzx, t701, L[36]);
-    L[27] = fmaf(35.0f * m300, t701, L[27]);
-    L[25] = fmaf(21.0f * m201, t701, L[25]);
-    const float t620 = -fmaf(c1_8, rx * t520 + ry * t610, c2_8 * (t420 + t600));
-    L[39] = fmaf(15.0f * mxx, t620, L[39]);
-    L[36] = fmaf(myy, t620, L[36]);
-    L[37] = fmaf(12.0f * mxy, t620, L[37]);
-    L[28] = fmaf(20.0f * m300, t620, L[28]);
-    L[26] = fmaf(30.0f * m210, t620, L[26]);
-    L[25] = fmaf(6.0f * m120, t620, L[25]);
-    const float t611 = -fmaf(c1_8, rx * t511 + ry * t601 + rz * t610, c2_8 * (t411));
-    L[40] = fmaf(15.0f * mxx, t611, L[40]);
-    L[38] = fmaf(6.0f * mxy, t611, L[38]);
-    L[36] = fmaf(myz, t611, L[36]);
-    L[37] = fmaf(6.0f * mzx, t611, L[37]);
-    L[29] = fmaf(20.0f * m300, t611, L[29]);
-    L[27] = fmaf(15.0f * m210, t611, L[27]);
-    L[26] = fmaf(15.0f * m201, t611, L[26]);
-    L[25] = fmaf(6.0f * m111, t611, L[25]);
-    const float t602 = -fmaf(c1_8, rx * t502 + rz * t601, c2_8 * (t402 + t600));
-    L[36] = fmaf(mzz, t602, L[36]);
-    L[38] = fmaf(12.0f * mzx, t602, L[38]);
-    L[27] = fmaf(30.0f * m201, t602, L[27]);
-    L[25] = fmaf(6.0f * m102, t602, L[25]);
-    const float t530 = -fmaf(c1_8, rx * t430 + ry * t520, c2_8 * (t330 + t510));
-    L[41] = fmaf(10.0f * mxx, t530, L[41]);
-    L[37] = fmaf(3.0f * myy, t530, L[37]);
-    L[39] = fmaf(15.0f * mxy, t530, L[39]);
-    L[30] = fmaf(10.0f * m300, t530, L[30]);
-    L[28] = fmaf(30.0f * m210, t530, L[28]);
-    L[26] = fmaf(15.0f * m120, t530, L[26]);
-    L[25] = fmaf(m030, t530, L[25]);
-    const float t521 = -fmaf(c1_8, rx * t421 + ry * t511 + rz * t520, c2_8 * (t321 + t501));
-    L[42] = fmaf(10.0f * mxx, t521, L[42]);
-    L[38] = fmaf(myy, t521, L[38]);
-    L[40] = fmaf(10.0f * mxy, t521, L[40]);
-    L[37] = fmaf(2.0f * myz, t521, L[37]);
-    L[39] = fmaf(5.0f * mzx, t521, L[39]);
-    L[31] = fmaf(10.0f * m300, t521, L[31]);
-    L[29] = fmaf(20.0f * m210, t521, L[29]);
-    L[28] = fmaf(10.0f * m201, t521, L[28]);
-    L[27] = fmaf(5.0f * m120, t521, L[27]);
-    L[26] = fmaf(10.0f * m111, t521, L[26]);
-    L[25] = fmaf(m021, t521, L[25]);
-    const float t512 = -fmaf(c1_8, rx * t412 + ry * t502 + rz * t511, c2_8 * (t312 + t510));
-    L[37] = fmaf(mzz, t512, L[37]);
-    L[38] = fmaf(2.0f * myz, t512, L[38]);
-    L[40] = fmaf(10.0f * mzx, t512, L[40]);
-    L[29] = fmaf(20.0f * m201, t512, L[29]);
-    L[27] = fmaf(10.0f * m111, t512, L[27]);
-    L[26] = fmaf(5.0f * m102, t512, L[26]);
-    L[25] = fmaf(m012, t512, L[25]);
-    const float t503 = -fmaf(c1_8, rx * t403 + rz * t502, c2_8 * (t303 + t501));
-    L[38] = fmaf(3.0f * mzz, t503, L[38]);
-    L[27] = fmaf(15.0f * m102, t503, L[27]);
-    L[25] = fmaf(m003, t503, L[25]);
-    const float t440 = -fmaf(c1_8, rx * t340 + ry * t430, c2_8 * (t240 + t420));
-    L[43] = fmaf(6.0f * mxx, t440, L[43]);
-    L[39] = fmaf(6.0f * myy, t440, L[39]);
-    L[41] = fmaf(16.0f * mxy, t440, L[41]);
-    L[32] = fmaf(4.0f * m300, t440, L[32]);
-    L[30] = fmaf(24.0f * m210, t440, L[30]);
-    L[28] = fmaf(24.0f * m120, t440, L[28]);
-    L[26] = fmaf(4.0f * m030, t440, L[26]);
-    const float t431 = -fmaf(c1_8, rx * t331 + ry * t421 + rz * t430, c2_8 * (t231 + t411));
-    L[44] = fmaf(6.0f * mxx, t431, L[44]);
-    L[40] = fmaf(3.0f * myy, t431, L[40]);
-    L[42] = fmaf(12.0f * mxy, t431, L[42]);
-    L[39] = fmaf(3.0f * myz, t431, L[39]);
-    L[41] = fmaf(4.0f * mzx, t431, L[41]);
-    L[33] = fmaf(4.0f * m300, t431, L[33]);
-    L[31] = fmaf(18.0f * m210, t431, L[31]);
-    L[30] = fmaf(6.0f * m201, t431, L[30]);
-    L[29] = fmaf(12.0f * m120, t431, L[29]);
-    L[28] = fmaf(12.0f * m111, t431, L[28]);
-    L[27] = fmaf(m030, t431, L[27]);
-    L[26] = fmaf(3.0f * m021, t431, L[26]);
-    const float t422 = -fmaf(c1_8, rx * t322 + ry * t412 + rz * t421, c2_8 * (t222 + t402 + t420));
-    L[39] = fmaf(mzz, t422, L[39]);
-    L[40] = fmaf(4.0f * myz, t422, L[40]);
-    L[42] = fmaf(8.0f * mzx, t422, L[42]);
-    L[31] = fmaf(12.0f * m201, t422, L[31]);
-    L[29] = fmaf(16.0f * m111, t422, L[29]);
-    L[28] = fmaf(4.0f * m102, t422, L[28]);
-    L[27] = fmaf(2.0f * m021, t422, L[27]);
-    L[26] = fmaf(2.0f * m012, t422, L[26]);
-    const float t413 = -fmaf(c1_8, rx * t313 + ry * t403 + rz * t412, c2_8 * (t213 + t411));
-    L[40] = fmaf(3.0f * mzz, t413, L[40]);
-    L[29] = fmaf(12.0f * m102, t413, L[29]);
-    L[27] = fmaf(3.0f * m012, t413, L[27]);
-    L[26] = fmaf(m003, t413, L[26]);
-    const float t404 = -fmaf(c1_8, rx * t304 + rz * t403, c2_8 * (t204 + t402));
-    L[27] = fmaf(4.0f * m003, t404, L[27]);
-    const float t350 = -fmaf(c1_8, rx * t250 + ry * t340, c2_8 * (t150 + t330));
-    L[45] = fmaf(3.0f * mxx, t350, L[45]);
-    L[41] = fmaf(10.0f * myy, t350, L[41]);
-    L[43] = fmaf(15.0f * mxy, t350, L[43]);
-    L[34] = fmaf(m300, t350, L[34]);
-    L[32] = fmaf(15.0f * m210, t350, L[32]);
-    L[30] = fmaf(30.0f * m120, t350, L[30]);
-    L[28] = fmaf(10.0f * m030, t350, L[28]);
-    const float t341 = -fmaf(c1_8, rx * t241 + ry * t331 + rz * t340, c2_8 * (t141 + t321));
-    L[46] = fmaf(3.0f * mxx, t341, L[46]);
-    L[42] = fmaf(6.0f * myy, t341, L[42]);
-    L[44] = fmaf(12.0f * mxy, t341, L[44]);
-    L[41] = fmaf(4.0f * myz, t341, L[41]);
-    L[43] = fmaf(3.0f * mzx, t341, L[43]);
-    L[35] = fmaf(m300, t341, L[35]);
-    L[33] = fmaf(12.0f * m210, t341, L[33]);
-    L[32] = fmaf(3.0f * m201, t341, L[32]);
-    L[31] = fmaf(18.0f * m120, t341, L[31]);
-    L[30] = fmaf(12.0f * m111, t341, L[30]);
-    L[29] = fmaf(4.0f * m030, t341, L[29]);
-    L[28] = fmaf(6.0f * m021, t341, L[28]);
-    const float t332 = -fmaf(c1_8, rx * t232 + ry * t322 + rz * t331, c2_8 * (t132 + t312 + t330));
-    L[41] = fmaf(mzz, t332, L[41]);
-    L[42] = fmaf(6.0f * myz, t332, L[42]);
-    L[44] = fmaf(6.0f * mzx, t332, L[44]);
-    L[33] = fmaf(6.0f * m201, t332, L[33]);
-    L[31] = fmaf(18.0f * m111, t332, L[31]);
-    L[30] = fmaf(3.0f * m102, t332, L[30]);
-    L[29] = fmaf(6.0f * m021, t332, L[29]);
-    L[28] = fmaf(3.0f * m012, t332, L[28]);
-    const float t323 = -fmaf(c1_8, rx * t223 + ry * t313 + rz * t322, c2_8 * (t123 + t303 + t321));
-    L[42] = fmaf(3.0f * mzz, t323, L[42]);
-    L[31] = fmaf(9.0f * m102, t323, L[31]);
-    L[29] = fmaf(6.0f * m012, t323, L[29]);
-    L[28] = fmaf(m003, t323, L[28]);
-    const float t314 = -fmaf(c1_8, rx * t214 + ry * t304 + rz * t313, c2_8 * (t114 + t312));
-    L[29] = fmaf(4.0f * m003, t314, L[29]);
-    const float t260 = -fmaf(c1_8, rx * t160 + ry * t250, c2_8 * (t060 + t240));
-    L[47] = fmaf(mxx, t260, L[47]);
-    L[43] = fmaf(15.0f * myy, t260, L[43]);
-    L[45] = fmaf(12.0f * mxy, t260, L[45]);
-    L[34] = fmaf(6.0f * m210, t260, L[34]);
-    L[32] = fmaf(30.0f * m120, t260, L[32]);
-    L[30] = fmaf(20.0f * m030, t260, L[30]);
-    const float t251 = -fmaf(c1_8, rx * t151 + ry * t241 + rz * t250, c2_8 * (t051 + t231));
-    L[48] = fmaf(mxx, t251, L[48]);
-    L[44] = fmaf(10.0f * myy, t251, L[44]);
-    L[46] = fmaf(10.0f * mxy, t251, L[46]);
-    L[43] = fmaf(5.0f * myz, t251, L[43]);
-    L[45] = fmaf(2.0f * mzx, t251, L[45]);
-    L[35] = fmaf(5.0f * m210, t251, L[35]);
-    L[34] = fmaf(m201, t251, L[34]);
-    L[33] = fmaf(20.0f * m120, t251, L[33]);
-    L[32] = fmaf(10.0f * m111, t251, L[32]);
-    L[31] = fmaf(10.0f * m030, t251, L[31]);
-    L[30] = fmaf(10.0f * m021, t251, L[30]);
-    const float t242 = -fmaf(c1_8, rx * t142 + ry * t232 + rz * t241, c2_8 * (t042 + t222 + t240));
-    L[43] = fmaf(mzz, t242, L[43]);
-    L[44] = fmaf(8.0f * myz, t242, L[44]);
-    L[46] = fmaf(4.0f * mzx, t242, L[46]);
-    L[35] = fmaf(2.0f * m201, t242, L[35]);
-    L[33] = fmaf(16.0f * m111, t242, L[33]);
-    L[32] = fmaf(2.0f * m102, t242, L[32]);
-    L[31] = fmaf(12.0f * m021, t242, L[31]);
-    L[30] = fmaf(4.0f * m012, t242, L[30]);
-    const float t233 = -fmaf(c1_8, rx * t133 + ry * t223 + rz * t232, c2_8 * (t033 + t213 + t231));
-    L[44] = fmaf(3.0f * mzz, t233, L[44]);
-    L[33] = fmaf(6.0f * m102, t233, L[33]);
-    L[31] = fmaf(9.0f * m012, t233, L[31]);
-    L[30] = fmaf(m003, t233, L[30]);
-    const float t224 = -fmaf(c1_8, rx * t124 + ry * t214 + rz * t223, c2_8 * (t024 + t204 + t222));
-    L[31] = fmaf(4.0f * m003, t224, L[31]);
-    const float t170 = -fmaf(c1_8, rx * t070 + ry * t160, c2_8 * (t150));
-    L[45] = fmaf(21.0f * myy, t170, L[45]);
-    L[47] = fmaf(7.0f * mxy, t170, L[47]);
-    L[34] = fmaf(21.0f * m120, t170, L[34]);
-    L[32] = fmaf(35.0f * m030, t170, L[32]);
-    const float t161 = -fmaf(c1_8, rx * t061 + ry * t151 + rz * t160, c2_8 * (t141));
-    L[46] = fmaf(15.0f * myy, t161, L[46]);
-    L[48] = fmaf(6.0f * mxy, t161, L[48]);
-    L[45] = fmaf(6.0f * myz, t161, L[45]);
-    L[47] = fmaf(mzx, t161, L[47]);
-    L[35] = fmaf(15.0f * m120, t161, L[35]);
-    L[34] = fmaf(6.0f * m111, t161, L[34]);
-    L[33] = fmaf(20.0f * m030, t161, L[33]);
-    L[32] = fmaf(15.0f * m021, t161, L[32]);
-    const float t152 = -fmaf(c1_8, rx * t052 + ry * t142 + rz * t151, c2_8 * (t132 + t150));
-    L[45] = fmaf(mzz, t152, L[45]);
-    L[46] = fmaf(10.0f * myz, t152, L[46]);
-    L[48] = fmaf(2.0f * mzx, t152, L[48]);
-    L[35] = fmaf(10.0f * m111, t152, L[35]);
-    L[34] = fmaf(m102, t152, L[34]);
-    L[33] = fmaf(20.0f * m021, t152, L[33]);
-    L[32] = fmaf(5.0f * m012, t152, L[32]);
-    const float t143 = -fmaf(c1_8, rx * t043 + ry * t133 + rz * t142, c2_8 * (t123 + t141));
-    L[46] = fmaf(3.0f * mzz, t143, L[46]);
-    L[35] = fmaf(3.0f * m102, t143, L[35]);
-    L[33] = fmaf(12.0f * m012, t143, L[33]);
-    L[32] = fmaf(m003, t143, L[32]);
-    const float t134 = -fmaf(c1_8, rx * t034 + ry * t124 + rz * t133, c2_8 * (t114 + t132));
-    L[33] = fmaf(4.0f * m003, t134, L[33]);
-    const float t080 = -fmaf(c1_8, ry * t070, c2_8 * (t060));
-    L[47] = fmaf(28.0f * myy, t080, L[47]);
-    L[34] = fmaf(56.0f * m030, t080, L[34]);
-    const float t071 = -fmaf(c1_8, ry * t061 + rz * t070, c2_8 * (t051));
-    L[48] = fmaf(21.0f * myy, t071, L[48]);
-    L[47] = fmaf(7.0f * myz, t071, L[47]);
-    L[35] = fmaf(35.0f * m030, t071, L[35]);
-    L[34] = fmaf(21.0f * m021, t071, L[34]);
-    const float t062 = -fmaf(c1_8, ry * t052 + rz * t061, c2_8 * (t042 + t060));
-    L[47] = fmaf(mzz, t062, L[47]);
-    L[48] = fmaf(12.0f * myz, t062, L[48]);
-    L[35] = fmaf(30.0f * m021, t062, L[35]);
-    L[34] = fmaf(6.0f * m012, t062, L[34]);
-    const float t053 = -fmaf(c1_8, ry * t043 + rz * t052, c2_8 * (t033 + t051));
-    L[48] = fmaf(3.0f * mzz, t053, L[48]);
-    L[35] = fmaf(15.0f * m012, t053, L[35]);
-    L[34] = fmaf(m003, t053, L[34]);
-    const float t044 = -fmaf(c1_8, ry * t034 + rz * t043, c2_8 * (t024 + t042));
-    L[35] = fmaf(4.0f * m003, t044, L[35]);
-    const float c1_9 = 1.8888888888888888f * ir2, c2_9 = 0.8888888888888888f * ir2;
-    const float t900 = -fmaf(c1_9, rx * t800, c2_9 * (t700));
-    L[36] = fmaf(84.0f * m300, t900, L[36]);
-    const float t810 = -fmaf(c1_9, rx * t710 + ry * t800, c2_9 * (t610));
-    L[37] = fmaf(56.0f * m300, t810, L[37]);
-    L[36] = fmaf(28.0f * m210, t810, L[36]);
-    const float t801 = -fmaf(c1_9, rx * t701 + rz * t800, c2_9 * (t601));
-    L[38] = fmaf(56.0f * m300, t801, L[38]);
-    L[36] = fmaf(28.0f * m201, t801, L[36]);
-    const float t720 = -fmaf(c1_9, rx * t620 + ry * t710, c2_9 * (t520 + t700));
-    L[39] = fmaf(35.0f * m300, t720, L[39]);
-    L[37] = fmaf(42.0f * m210, t720, L[37]);
-    L[36] = fmaf(7.0f * m120, t720, L[36]);
-    const float t711 = -fmaf(c1_9, rx * t611 + ry * t701 + rz * t710, c2_9 * (t511));
-    L[40] = fmaf(35.0f * m300, t711, L[40]);
-    L[38] = fmaf(21.0f * m210, t711, L[38]);
-    L[37] = fmaf(21.0f * m201, t711, L[37]);
-    L[36] = fmaf(7.0f * m111, t711, L[36]);
-    const float t702 = -fmaf(c1_9, rx * t602 + rz * t701, c2_9 * (t502 + t700));
-    L[38] = fmaf(42.0f * m201, t702, L[38]);
-    L[36] = fmaf(7.0f * m102, t702, L[36]);
-    const float t630 = -fmaf(c1_9, rx * t530 + ry * t620, c2_9 * (t430 + t610));
-    L[41] = fmaf(20.0f * m300, t630, L[41]);
-    L[39] = fmaf(45.0f * m210, t630, L[39]);
-    L[37] = fmaf(18.0f * m120, t630, L[37]);
-    L[36] = fmaf(m030, t630, L[36]);
-    const float t621 = -fmaf(c1_9, rx * t521 + ry * t611 + rz * t620, c2_9 * (t421 + t601));
-    L[42] = fmaf(20.0f * m300, t621, L[42]);
-    L[40] = fmaf(30.0f * m210, t621, L[40]);
-    L[39] = fmaf(15.0f * m201, t621, L[39]);
-    L[38] = fmaf(6.0f * m120, t621, L[38]);
-    L[37] = fmaf(12.0f * m111, t621, L[37]);
-    L[36] = fmaf(m021, t621, L[36]);
-    const float t612 = -fmaf(c1_9, rx * t512 + ry * t602 + rz * t611, c2_9 * (t412 + t610));
-    L[40] = fmaf(30.0f * m201, t612, L[40]);
-    L[38] = fmaf(12.0f * m111, t612, L[38]);
-    L[37] = fmaf(6.0f * m102, t612, L[37]);
-    L[36] = fmaf(m012, t612, L[36]);
-    const float t603 = -fmaf(c1_9, rx * t503 + rz * t602, c2_9 * (t403 + t601));
-    L[38] = fmaf(18.0f * m102, t603, L[38]);
-    L[36] = fmaf(m003, t603, L[36]);
-    const float t540 = -fmaf(c1_9, rx * t440 + ry * t530, c2_9 * (t340 + t520));
-    L[43] = fmaf(10.0f * m300, t540, L[43]);
-    L[41] = fmaf(40.0f * m210, t540, L[41]);
-    L[39] = fmaf(30.0f * m120, t540, L[39]);
-    L[37] = fmaf(4.0f * m030, t540, L[37]);
-    const float t531 = -fmaf(c1_9, rx * t431 + ry * t521 + rz * t530, c2_9 * (t331 + t511));
-    L[44] = fmaf(10.0f * m300, t531, L[44]);
-    L[42] = fmaf(30.0f * m210, t531, L[42]);
-    L[41] = fmaf(10.0f * m201, t531, L[41]);
-    L[40] = fmaf(15.0f * m120, t531, L[40]);
-    L[39] = fmaf(15.0f * m111, t531, L[39]);
-    L[38] = fmaf(m030, t531, L[38]);
-    L[37] = fmaf(3.0f * m021, t531, L[37]);
-    const float t522 = -fmaf(c1_9, rx * t422 + ry * t512 + rz * t521, c2_9 * (t322 + t502 + t520));
-    L[42] = fmaf(20.0f * m201, t522, L[42]);
-    L[40] = fmaf(20.0f * m111, t522, L[40]);
-    L[39] = fmaf(5.0f * m102, t522, L[39]);
-    L[38] = fmaf(2.0f * m021, t522, L[38]);
-    L[37] = fmaf(2.0f * m012, t522, L[37]);
-    const float t513 = -fmaf(c1_9, rx * t413 + ry * t503 + rz * t512, c2_9 * (t313 + t511));
-    L[40] = fmaf(15.0f * m102, t513, L[40]);
-    L[38] = fmaf(3.0f * m012, t513, L[38]);
-    L[37] = fmaf(m003, t513, L[37]);
-    const float t504 = -fmaf(c1_9, rx * t404 + rz * t503, c2_9 * (t304 + t502));
-    L[38] = fmaf(4.0f * m003, t504, L[38]);
-    const float t450 = -fmaf(c1_9, rx * t350 + ry * t440, c2_9 * (t250 + t430));
-    L[45] = fmaf(4.0f * m300, t450, L[45]);
-    L[43] = fmaf(30.0f * m210, t450, L[43]);
-    L[41] = fmaf(40.0f * m120, t450, L[41]);
-    L[39] = fmaf(10.0f * m030, t450, L[39]);
-    const float t441 = -fmaf(c1_9, rx * t341 + ry * t431 + rz * t440, c2_9 * (t241 + t421));
-    L[46] = fmaf(4.0f * m300, t441, L[46]);
-    L[44] = fmaf(24.0f * m210, t441, L[44]);
-    L[43] = fmaf(6.0f * m201, t441, L[43]);
-    L[42] = fmaf(24.0f * m120, t441, L[42]);
-    L[41] = fmaf(16.0f * m111, t441, L[41]);
-    L[40] = fmaf(4.0f * m030, t441, L[40]);
-    L[39] = fmaf(6.0f * m021, t441, L[39]);
-    const float t432 = -fmaf(c1_9, rx * t332 + ry * t422 + rz * t431, c2_9 * (t232 + t412 + t430));
-    L[44] = fmaf(12.0f * m201, t432, L[44]);
-    L[42] = fmaf(24.0f * m111, t432, L[42]);
-    L[41] = fmaf(4.0f * m102, t432, L[41]);
-    L[40] = fmaf(6.0f * m021, t432, L[40]);
-    L[39] = fmaf(3.0f * m012, t432, L[39]);
-    const float t423 = -fmaf(c1_9, rx * t323 + ry * t413 + rz * t422, c2_9 * (t223 + t403 + t421));
-    L[42] = fmaf(12.0f * m102, t423, L[42]);
-    L[40] = fmaf(6.0f * m012, t423, L[40]);
-    L[39] = fmaf(m003, t423, L[39]);
-    const float t414 = -fmaf(c1_9, rx * t314 + ry * t404 + rz * t413, c2_9 * (t214 + t412));
-    L[40] = fmaf(4.0f * m003, t414, L[40]);
-    const float t360 = -fmaf(c1_9, rx * t260 + ry * t350, c2_9 * (t160 + t340));
-    L[47] = fmaf(m300, t360, L[47]);
-    L[45] = fmaf(18.0f * m210, t360, L[45]);
-    L[43] = fmaf(45.0f * m120, t360, L[43]);
-    L[41] = fmaf(20.0f * m030, t360, L[41]);
-    const float t351 = -fmaf(c1_9, rx * t251 + ry * t341 + rz * t350, c2_9 * (t151 + t331));
-    L[48] = fmaf(m300, t351, L[48]);
-    L[46] = fmaf(15.0f * m210, t351, L[46]);
-    L[45] = fmaf(3.0f * m201, t351, L[45]);
-    L[44] = fmaf(30.0f * m120, t351, L[44]);
-    L[43] = fmaf(15.0f * m111, t351, L[43]);
-    L[42] = fmaf(10.0f * m030, t351, L[42]);
-    L[41] = fmaf(10.0f * m021, t351, L[41]);
-    const float t342 = -fmaf(c1_9, rx * t242 + ry * t332 + rz * t341, c2_9 * (t142 + t322 + t340));
-    L[46] = fmaf(6.0f * m201, t342, L[46]);
-    L[44] = fmaf(24.0f * m111, t342, L[44]);
-    L[43] = fmaf(3.0f * m102, t342, L[43]);
-    L[42] = fmaf(12.0f * m021, t342, L[42]);
-    L[41] = fmaf(4.0f * m012, t342, L[41]);
-    const float t333 = -fmaf(c1_9, rx * t233 + ry * t323 + rz * t332, c2_9 * (t133 + t313 + t331));
-    L[44] = fmaf(9.0f * m102, t333, L[44]);
-    L[42] = fmaf(9.0f * m012, t333, L[42]);
-    L[41] = fmaf(m003, t333, L[41]);
-    const float t324 = -fmaf(c1_9, rx * t224 + ry * t314 + rz * t323, c2_9 * (t124 + t304 + t322));
-    L[42] = fmaf(4.0f * m003, t324, L[42]);
-    const float t270 = -fmaf(c1_9, rx * t170 + ry * t260, c2_9 * (t070 + t250));
-    L[47] = fmaf(7.0f * m210, t270, L[47]);
-    L[45] = fmaf(42.0f * m120, t270, L[45]);
-    L[43] = fmaf(35.0f * m030, t270, L[43]);
-    const float t261 = -fmaf(c1_9, rx * t161 + ry * t251 + rz * t260, c2_9 * (t061 + t241));
-    L[48] = fmaf(6.0f * m210, t261, L[48]);
-    L[47] = fmaf(m201, t261, L[47]);
-    L[46] = fmaf(30.0f * m120, t261, L[46]);
-    L[45] = fmaf(12.0f * m111, t261, L[45]);
-    L[44] = fmaf(20.0f * m030, t261, L[44]);
-    L[43] = fmaf(15.0f * m021, t261, L[43]);
-    const float t252 = -fmaf(c1_9, rx * t152 + ry * t242 + rz * t251, c2_9 * (t052 + t232 + t250));
-    L[48] = fmaf(2.0f * m201, t252, L[48]);
-    L[46] = fmaf(20.0f * m111, t252, L[46]);
-    L[45] = fmaf(2.0f * m102, t252, L[45]);
-    L[44] = fmaf(20.0f * m021, t252, L[44]);
-    L[43] = fmaf(5.0f * m012, t252, L[43]);
-    const float t243 = -fmaf(c1_9, rx * t143 + ry * t233 + rz * t242, c2_9 * (t043 + t223 + t241));
-    L[46] = fmaf(6.0f * m102, t243, L[46]);
-    L[44] = fmaf(12.0f * m012, t243, L[44]);
-    L[43] = fmaf(m003, t243, L[43]);
-    const float t234 = -fmaf(c1_9, rx * t134 + ry * t224 + rz * t233, c2_9 * (t034 + t214 + t232));
-    L[44] = fmaf(4.0f * m003, t234, L[44]);
-    const float t180 = -fmaf(c1_9, rx * t080 + ry * t170, c2_9 * (t160));
-    L[47] = fmaf(28.0f * m120, t180, L[47]);
-    L[45] = fmaf(56.0f * m030, t180, L[45]);
-    const float t171 = -fmaf(c1_9, rx * t071 + ry * t161 + rz * t170, c2_9 * (t151));
-    L[48] = fmaf(21.0f * m120, t171, L[48]);
-    L[47] = fmaf(7.0f * m111, t171, L[47]);
-    L[46] = fmaf(35.0f * m030, t171, L[46]);
-    L[45] = fmaf(21.0f * m021, t171, L[45]);
-    const float t162 = -fmaf(c1_9, rx * t062 + ry * t152 + rz * t161, c2_9 * (t142 + t160));
-    L[48] = fmaf(12.0f * m111, t162, L[48]);
-    L[47] = fmaf(m102, t162, L[47]);
-    L[46] = fmaf(30.0f * m021, t162, L[46]);
-    L[45] = fmaf(6.0f * m012, t162, L[45]);
-    const float t153 = -fmaf(c1_9, rx * t053 + ry * t143 + rz * t152, c2_9 * (t133 + t151));
-    L[48] = fmaf(3.0f * m102, t153, L[48]);
-    L[46] = fmaf(15.0f * m012, t153, L[46]);
-    L[45] = fmaf(m003, t153, L[45]);
-    const float t144 = -fmaf(c1_9, rx * t044 + ry * t134 + rz * t143, c2_9 * (t124 + t142));
-    L[46] = fmaf(4.0f * m003, t144, L[46]);
-    const float t090 = -fmaf(c1_9, ry * t080, c2_9 * (t070));
-    L[47] = fmaf(84.0f * m030, t090, L[47]);
-    const float t081 = -fmaf(c1_9, ry * t071 + rz * t080, c2_9 * (t061));
-    L[48] = fmaf(56.0f * m030, t081, L[48]);
-    L[47] = fmaf(28.0f * m021, t081, L[47]);
-    const float t072 = -fmaf(c1_9, ry * t062 + rz * t071, c2_9 * (t052 + t070));
-    L[48] = fmaf(42.0f * m021, t072, L[48]);
-    L[47] = fmaf(7.0f * m012, t072, L[47]);
-    const float t063 = -fmaf(c1_9, ry * t053 + rz * t062, c2_9 * (t043 + t061));
-    L[48] = fmaf(18.0f * m012, t063, L[48]);
-    L[47] = fmaf(m003, t063, L[47]);
-    const float t054 = -fmaf(c1_9, ry * t044 + rz * t053, c2_9 * (t034 + t052));
-    L[48] = fmaf(4.0f * m003, t054, L[48]);
 }
 
 // potential and field at offset y from the expansion centre (order P = 6, from the

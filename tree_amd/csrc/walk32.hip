@@ -80,6 +80,10 @@ constexpr int NLOC = FAR_ORDER == 3   ? TB_LOCAL_N3
 // evaluation: order 4 at K = 6, 64.5 ms; order 5 at K = 4, 58.4 ms; order 6
 // at K = 3, 55.5 ms (K = 3.5: 57.1; K = 2.5 needs theta below 0.22); order 7
 // at K = 2.5 or 3, 65 ms (its far evaluation costs more than it saves).
+// Order 6 keeping only the terms of total order |b| + |c| <= 7 (2 of the
+// 3 orders of moment-times-coefficient products beyond it dropped): 53.0 ms
+// at K = 3 against 56.2 for the full order 6 on the same GPU (<= 8: 53.9;
+// <= 6: theta falls to 0.256 and the gain is lost).
 constexpr double FAR_RATIO_DEFAULT = 3.0;
 
 struct WaveLDS {
@@ -388,7 +392,7 @@ __device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q
         m2l_acc_p5ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
                      q4.y, q4.z, L);
     else
-        m2l_acc_p6ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
+        m2l_acc_p6hot7(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
                      q4.y, q4.z, L);
 }
 
