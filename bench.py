@@ -248,7 +248,7 @@ def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
     else:
         log("tuning the mixed walk")
         th, rms, mx = bisect(lambda v: eng.configure(theta_mixed=v), lambda: err_full(eng, tsp, tix, fdir),
-                             0.05, 1.0, 8)
+                             0.05, 1.0, 10)
     eng.build()
     g = a.group
     if a.group == 0:
