@@ -62,6 +62,21 @@ def test_mixed_close_kelbg_pairs(theta):
         assert rms < 1e-3, (rms, mx)
 
 
+@pytest.mark.parametrize("ne,ni,group", [(1, 1, 64), (1, 5, 128), (3, 2, 256), (64, 65, 64), (130, 1, 128),
+                                          (257, 300, 256)])
+def test_mixed_tiny_and_ragged(ne, ni, group):
+    """Tiny and ragged systems (one particle of a species, a group of one
+    target, partial groups): theta -> 0 is the FP32 direct sum, rms < 3e-6;
+    at theta 0.5 every force stays within 2e-2 of the direct sum."""
+    announce("mixed_tiny %d+%d group %d" % (ne, ni, group))
+    qel, qion = ta.gen_plasma(25, ne, ni, 30.0)
+    for theta, bound in ((1e-9, 3e-6), (0.5, 2e-2)):
+        f, fd, _ = _run(qel, qion, theta, group=group)
+        rms, mx = _err(f, fd)
+        assert np.all(np.isfinite(f))
+        assert rms < bound and mx < 10 * bound, (theta, rms, mx)
+
+
 @pytest.mark.parametrize("group", [64, 128, 256])
 def test_mixed_group_sizes_agree(group):
     """All walk-group sizes stay within one regression bound at theta 0.4: rms < 2.5e-3
