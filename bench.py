@@ -494,10 +494,11 @@ def main():
     ap.add_argument("--group", type=int, default=0, help="mixed-walk targets per group (0 = autotuned)")
     ap.add_argument("--err-target", type=float, default=0.0,
                     help="fixed error threshold for the learned criterion (skip tuning)")
-    # measured at N = 4M, 1e-4 rms, groups of 128: band 0.8 .. 1.25 -> 10.6
-    # steps/s, 0.9 .. 1.12 -> 11.5, 0.95 .. 1.06 -> 12.4, 0.97 .. 1.03 ->
-    # 12.7 (the fixed-theta walk: 12.7); with the third-moment M2P the
-    # learned decisions away from theta cost more work than they save
+    # measured at N = 4M, 1e-4 rms, groups of 128 (far path at order 4,
+    # K = 6): band 0.8 .. 1.25 -> 10.6 steps/s, 0.9 .. 1.12 -> 11.5,
+    # 0.95 .. 1.06 -> 12.4, 0.97 .. 1.03 -> 12.7 (the fixed-theta walk:
+    # 12.7); with the third-moment M2P the learned decisions away from theta
+    # cost more work than they save
     ap.add_argument("--band", type=float, nargs=2, default=(0.97, 1.03), metavar=("LO", "HI"),
                     help="learned band: theta_lo = LO * theta, theta_hi = HI * theta (theta: the tuned fixed walk's)")
     ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "build"), default=None,
