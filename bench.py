@@ -428,7 +428,7 @@ def run_c4(a):
 
 def run_c5(a):
     """2^23 + 2^23, sigma 1 mm: the learned criterion (trained at 2^20) against the fixed-theta walk at
-    theta 0.5, its fallback and judge; err_target bisected to the fixed walk's rms error."""
+    theta 0.5, its fallback and yardstick; err_target bisected to the fixed walk's rms error."""
     global TARGET
     n, dt = 1 << 23, 0.005 / ta.TPS
     rs = ta.kelbg_length(10.0)

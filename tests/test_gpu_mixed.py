@@ -1,7 +1,7 @@
 """The mixed-precision group walk against FP64 references.
 
 The mixed walk is a different approximation from the reference's (group
-MAC, exact quadrupoles, FP32 arithmetic), so it is judged by its error
+MAC, exact moments, FP32 arithmetic), so it is measured by its error
 against the FP64 direct sum: rms and max relative force error are stated in
 each assertion.
 """

@@ -1,6 +1,6 @@
 """The GPU MLP kernels against the FP64 oracle (oracle/mlp_ref.py), the
 GPU trainer, and the learned opening criterion against the fixed-theta walk
-(its fallback and judge)."""
+(its fallback and its yardstick)."""
 import numpy as np
 import pytest
 
