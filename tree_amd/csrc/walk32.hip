@@ -41,9 +41,9 @@ constexpr int BT = 256;
 constexpr int WPB = BT / 64;
 constexpr int STACK = 1024;
 // Interaction lists are flushed once FLUSH_M (cells) or FLUSH_P (leaves)
-// entries are pending.  A traversal step adds at most 64 entries to a list,
-// and a learned band decision at most 64 more cells after a flush check, so
-// a list of capacity FLUSH - 1 + 64 holds them.  The queues (band cells, far
+// entries are pending.  Each pass of the walk loop (a traversal step, or a
+// learned band decision) adds at most 64 entries to a list before the flush
+// checks, so a list of capacity FLUSH - 1 + 64 holds them.  The queues (band cells, far
 // cells) are flushed 64 at a time: QCAP = 63 + 64.  The LDS per wave
 // (13.0 KB) keeps three 4-wave blocks on a CU; cell entries (80 bytes, up to
 // the third moment) are the largest part.
@@ -557,143 +557,244 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         L.stack[0] = 0u;  // root: range of one cell at 0
         wave_sync();
         while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
-            // a batch of <= 64 cells from the top ranges of the stack
-            uint32_t R = 0;
-            int len = 0;
-            if (lane < sp) {
-                R = L.stack[sp - 1 - lane];
-                len = (int)(R & 7u) + 1;
-            }
-            const int incl = wave_incl_scan(len);
-            const bool take = lane < sp && incl <= 64;
-            const int K = __popcll(__ballot(take));
-            const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
-            if (take) {
-                const uint32_t st = R >> 3;
-                const int ex = incl - len;
-                for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
-            }
-            sp -= K;
-            wave_sync();
-            const bool have = lane < total;
-            const uint32_t node = have ? L.batch[lane] : 0u;
-            wave_sync();
-            bool leaf = false, acc = false, open = false, band = false;
-            float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
-            uint32_t info = 0;
-            if (have) {
-                const double4 m = D.bmac[node];
-                info = D.binfo[node];
-                cx = (float)(m.x - gx);
-                cy = (float)(m.y - gy);
-                cz = (float)(m.z - gz);
-                if (info & TB_LEAF) {
-                    leaf = true;
-                } else {
-                    const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
-                    const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
-                    const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
-                    d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const float rc = (float)m.w;  // s / theta_mixed
-                    const bool far_hi = d2 > rc * rc * 1.000004f;
-                    if (D.bq2[node].z <= (float)SMALL_OPEN) {
-                        open = true;  // its particles' P2P is exact and cheaper than its M2P
-                    } else if (MODE == MODE_FIXED) {
-                        acc = far_hi;
-                        open = !far_hi;
-                    } else {
-                        const float rlo = rc * W.lo_scale;  // s / theta_lo
-                        if (d2 > rlo * rlo * 1.000004f) acc = true;
-                        else if (!far_hi) open = true;
-                        else if (MODE == MODE_COLLECT ||
-                                 (D.bq2[node].z <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2))
-                            band = true;
-                        // larger cells are never labelled (the label sums the
-                        // cell's particles), and far cells are cheap to accept
-                        // (the group's local expansion): in the learned walk
-                        // the band's geometric-mean radius decides both
-                        else if (d2 > rlo * rc * 1.000004f) acc = true;
-                        else open = true;
-                    }
+            // learned: a pass either decides the queued band cells (once 64
+            // wait, or the stack is empty) or takes a traversal step; the
+            // flushes after it serve both, so a pass adds at most 64 entries
+            // to each list
+            const bool decide = MODE == MODE_LEARNED && (nb >= 64 || (sp == 0 && nb > 0));
+            if (!decide) {
+                // a batch of <= 64 cells from the top ranges of the stack
+                uint32_t R = 0;
+                int len = 0;
+                if (lane < sp) {
+                    R = L.stack[sp - 1 - lane];
+                    len = (int)(R & 7u) + 1;
                 }
-            }
-            // learned: band cells are queued and decided 64 at a time below
-            if (MODE == MODE_COLLECT && band) {
-                const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
-                float f[TB_MLP_NIN];
-                band_features(D.bf[node], q2.w, sqrtf(d2), hdiag, klemin, q0.x * qt >= 0.0f, knact, f);
-                {
-                    open = true;  // the conservative walk opens it; the pair is recorded
-                    if (q2.z <= (float)BAND_MAXCNT) {
-                        const int k = atomicAdd(&W.ctr[3], 1);
-                        if (k < W.rec_cap) {
-                            BandRec r;
-                            r.cell = node;
-                            r.group = (uint32_t)g;
-                            r.tree = 0;
-                            r.pad = 0;
-                            for (int j = 0; j < TB_MLP_NIN; j++) r.f[j] = f[j];
-                            W.rec[k] = r;
+                const int incl = wave_incl_scan(len);
+                const bool take = lane < sp && incl <= 64;
+                const int K = __popcll(__ballot(take));
+                const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
+                if (take) {
+                    const uint32_t st = R >> 3;
+                    const int ex = incl - len;
+                    for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
+                }
+                sp -= K;
+                wave_sync();
+                const bool have = lane < total;
+                const uint32_t node = have ? L.batch[lane] : 0u;
+                wave_sync();
+                bool leaf = false, acc = false, open = false, band = false;
+                float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
+                uint32_t info = 0;
+                if (have) {
+                    const double4 m = D.bmac[node];
+                    info = D.binfo[node];
+                    cx = (float)(m.x - gx);
+                    cy = (float)(m.y - gy);
+                    cz = (float)(m.z - gz);
+                    if (info & TB_LEAF) {
+                        leaf = true;
+                    } else {
+                        const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
+                        const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
+                        const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
+                        d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const float rc = (float)m.w;  // s / theta_mixed
+                        const bool far_hi = d2 > rc * rc * 1.000004f;
+                        if (D.bq2[node].z <= (float)SMALL_OPEN) {
+                            open = true;  // its particles' P2P is exact and cheaper than its M2P
+                        } else if (MODE == MODE_FIXED) {
+                            acc = far_hi;
+                            open = !far_hi;
+                        } else {
+                            const float rlo = rc * W.lo_scale;  // s / theta_lo
+                            if (d2 > rlo * rlo * 1.000004f) acc = true;
+                            else if (!far_hi) open = true;
+                            else if (MODE == MODE_COLLECT ||
+                                     (D.bq2[node].z <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2))
+                                band = true;
+                            // larger cells are never labelled (the label sums the
+                            // cell's particles), and far cells are cheap to accept
+                            // (the group's local expansion): in the learned walk
+                            // the band's geometric-mean radius decides both
+                            else if (d2 > rlo * rc * 1.000004f) acc = true;
+                            else open = true;
                         }
                     }
                 }
-            }
-            if (COUNT) c_visit[tsp] += (unsigned long long)__popcll(__ballot(have));
-            // accepted cells -> M2P list, or the far queue
-            const bool far = acc && cx * cx + cy * cy + cz * cz > far_r2;
-            const uint64_t fk = __ballot(far);
-            if (far) L.farq[nf + __popcll(fk & lanemask_lt())] = node;
-            nf += __popcll(fk);
-            if (COUNT) {
-                c_m2l[tsp] += (unsigned long long)__popcll(fk) * nact;
-                c_far[tsp] += (unsigned long long)__popcll(fk);
-            }
-            acc = acc && !far;
-            uint64_t mk = __ballot(acc);
-            if (acc) {
-                const int k = nm + __popcll(mk & lanemask_lt());
-                const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
-                L.m0[k] = make_float4(cx, cy, cz, q0.x);
-                L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
-                L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
-                L.m3[k] = D.bq3[node];
-                L.m4[k] = D.bq4[node];
-            }
-            nm += __popcll(mk);
-            // leaves -> P2P list: like charges from the bottom, unlike from the top
-            const float qleaf = leaf ? D.bq0[node].x : 0.0f;
-            const bool lk = leaf && qleaf == qt;
-            const uint64_t ml = __ballot(lk), mu = __ballot(leaf && !lk);
-            if (lk) L.p[npl + __popcll(ml & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
-            else if (leaf) L.p[LCAP_P - 1 - npu - __popcll(mu & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
-            npl += __popcll(ml);
-            npu += __popcll(mu);
-            if (COUNT) {
-                c_like[tsp] += (unsigned long long)__popcll(ml) * nact;
-                c_unlike[tsp] += (unsigned long long)__popcll(mu) * nact;
-            }
-            // opened cells -> their child ranges.  Children follow their
-            // parent in breadth-first order, so a range not strictly after the
-            // cell (or past the table) is a malformed tree: flagged, never
-            // pushed, which bounds every path through the traversal.
-            if (open) {
-                const uint32_t st = info >> 3, len = (info & 7u) + 1u;
-                if (st <= node || st + len > (uint32_t)D.nn) {
-                    atomicOr(&W.ctr[1], 2);
-                    open = false;
+                // learned: band cells are queued and decided 64 at a time below
+                if (MODE == MODE_COLLECT && band) {
+                    const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
+                    float f[TB_MLP_NIN];
+                    band_features(D.bf[node], q2.w, sqrtf(d2), hdiag, klemin, q0.x * qt >= 0.0f, knact, f);
+                    {
+                        open = true;  // the conservative walk opens it; the pair is recorded
+                        if (q2.z <= (float)BAND_MAXCNT) {
+                            const int k = atomicAdd(&W.ctr[3], 1);
+                            if (k < W.rec_cap) {
+                                BandRec r;
+                                r.cell = node;
+                                r.group = (uint32_t)g;
+                                r.tree = 0;
+                                r.pad = 0;
+                                for (int j = 0; j < TB_MLP_NIN; j++) r.f[j] = f[j];
+                                W.rec[k] = r;
+                            }
+                        }
+                    }
                 }
+                if (COUNT) c_visit[tsp] += (unsigned long long)__popcll(__ballot(have));
+                // accepted cells -> M2P list, or the far queue
+                const bool far = acc && cx * cx + cy * cy + cz * cz > far_r2;
+                const uint64_t fk = __ballot(far);
+                if (far) L.farq[nf + __popcll(fk & lanemask_lt())] = node;
+                nf += __popcll(fk);
+                if (COUNT) {
+                    c_m2l[tsp] += (unsigned long long)__popcll(fk) * nact;
+                    c_far[tsp] += (unsigned long long)__popcll(fk);
+                }
+                acc = acc && !far;
+                uint64_t mk = __ballot(acc);
+                if (acc) {
+                    const int k = nm + __popcll(mk & lanemask_lt());
+                    const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
+                    L.m0[k] = make_float4(cx, cy, cz, q0.x);
+                    L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
+                    L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
+                    L.m3[k] = D.bq3[node];
+                    L.m4[k] = D.bq4[node];
+                }
+                nm += __popcll(mk);
+                // leaves -> P2P list: like charges from the bottom, unlike from the top
+                const float qleaf = leaf ? D.bq0[node].x : 0.0f;
+                const bool lk = leaf && qleaf == qt;
+                const uint64_t ml = __ballot(lk), mu = __ballot(leaf && !lk);
+                if (lk) L.p[npl + __popcll(ml & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
+                else if (leaf) L.p[LCAP_P - 1 - npu - __popcll(mu & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
+                npl += __popcll(ml);
+                npu += __popcll(mu);
+                if (COUNT) {
+                    c_like[tsp] += (unsigned long long)__popcll(ml) * nact;
+                    c_unlike[tsp] += (unsigned long long)__popcll(mu) * nact;
+                }
+                // opened cells -> their child ranges.  Children follow their
+                // parent in breadth-first order, so a range not strictly after the
+                // cell (or past the table) is a malformed tree: flagged, never
+                // pushed, which bounds every path through the traversal.
+                if (open) {
+                    const uint32_t st = info >> 3, len = (info & 7u) + 1u;
+                    if (st <= node || st + len > (uint32_t)D.nn) {
+                        atomicOr(&W.ctr[1], 2);
+                        open = false;
+                    }
+                }
+                mk = __ballot(open);
+                const int npush = __popcll(mk);
+                if (sp + npush > STACK) {
+                    atomicOr(&W.ctr[1], 1);
+                    sp = 0;
+                    nm = npl = npu = 0;
+                    break;
+                }
+                if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
+                sp += npush;
+                wave_sync();
+                if (MODE == MODE_LEARNED) {
+                    const uint64_t bm = __ballot(band);
+                    if (band) L.bandq[nb + __popcll(bm & lanemask_lt())] = node;
+                    nb += __popcll(bm);
+                    wave_sync();
+                }
+            } else {
+                // Decide up to 64 queued cells with one batched MLP pass on
+                // the matrix cores (tb_mlp_eval32_mfma, two tiles of 32).
+                // Entry e's features are recomputed from its records.
+                const int ne = min(nb, 64);
+                // lane l forms entry l's features; each tile's MFMA
+                // operand takes features 0..3 of its entry in lanes 0..31
+                // and 4..7 in lanes 32..63, exchanged across the halves
+                float f[TB_MLP_NIN];
+#pragma unroll
+                for (int j = 0; j < TB_MLP_NIN; j++) f[j] = 0.0f;
+                if (lane < ne) {
+                    const uint32_t nd = L.bandq[lane];
+                    const double4 m = D.bmac[nd];
+                    const float ex = fmaxf(fabsf((float)(m.x - gx)) - hx, 0.0f);
+                    const float ey = fmaxf(fabsf((float)(m.y - gy)) - hy, 0.0f);
+                    const float ez = fmaxf(fabsf((float)(m.z - gz)) - hz, 0.0f);
+                    band_features(D.bf[nd], D.bq2[nd].w, sqrtf(ex * ex + ey * ey + ez * ez), hdiag, klemin,
+                                  D.bq0[nd].x * qt >= 0.0f, knact, f);
+                }
+                float xa[4], xb[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float lo4 = __shfl_xor(f[q], 32, 64), hi4 = __shfl_xor(f[4 + q], 32, 64);
+                    xa[q] = lane < 32 ? f[q] : hi4;   // tile 0: entries 0..31
+                    xb[q] = lane < 32 ? lo4 : f[4 + q];  // tile 1: entries 32..63
+                }
+                float pr[2];
+                pr[0] = tb_mlp_eval32_mfma(W.mlpk, xa);
+                pr[1] = tb_mlp_eval32_mfma(W.mlpk, xb);
+                // lane l decides entry l (tile l / 32 holds it); without a
+                // previous force (emin = 0) band cells are opened
+                const bool mine = lane < ne;
+                const uint32_t nd = mine ? L.bandq[lane] : 0u;
+                bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < W.log_err;
+                bool bopen = mine && !bacc;
+                // accepted band cells: far ones to the far queue, the rest to M2P
+                float bx = 0.0f, by = 0.0f, bz = 0.0f;
+                if (bacc) {
+                    const double4 m = D.bmac[nd];
+                    bx = (float)(m.x - gx);
+                    by = (float)(m.y - gy);
+                    bz = (float)(m.z - gz);
+                }
+                const bool bfar = bacc && bx * bx + by * by + bz * bz > far_r2;
+                const uint64_t bf = __ballot(bfar);
+                if (bfar) L.farq[nf + __popcll(bf & lanemask_lt())] = nd;
+                nf += __popcll(bf);
+                if (COUNT) {
+                    c_m2l[tsp] += (unsigned long long)__popcll(bf) * nact;
+                    c_far[tsp] += (unsigned long long)__popcll(bf);
+                }
+                bacc = bacc && !bfar;
+                uint64_t bk = __ballot(bacc);
+                if (bacc) {
+                    const int k = nm + __popcll(bk & lanemask_lt());
+                    const float4 q0 = D.bq0[nd], q1 = D.bq1[nd], q2 = D.bq2[nd];
+                    L.m0[k] = make_float4(bx, by, bz, q0.x);
+                    L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
+                    L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
+                    L.m3[k] = D.bq3[nd];
+                    L.m4[k] = D.bq4[nd];
+                }
+                nm += __popcll(bk);
+                const uint32_t binf = bopen ? D.binfo[nd] : 0u;
+                if (bopen) {
+                    const uint32_t st = binf >> 3, len = (binf & 7u) + 1u;
+                    if (st <= nd || st + len > (uint32_t)D.nn) {
+                        atomicOr(&W.ctr[1], 2);
+                        bopen = false;
+                    }
+                }
+                bk = __ballot(bopen);
+                const int bpush = __popcll(bk);
+                if (sp + bpush > STACK) {
+                    atomicOr(&W.ctr[1], 1);
+                    sp = 0;
+                    nm = npl = npu = nb = 0;
+                    break;
+                }
+                if (bopen) L.stack[sp + __popcll(bk & lanemask_lt())] = binf;
+                sp += bpush;
+                // the rest of the queue (at most 63 entries) moves down
+                const uint32_t rest = lane + 64 < nb ? L.bandq[lane + 64] : 0u;
+                wave_sync();
+                if (lane + 64 < nb) L.bandq[lane] = rest;
+                nb -= ne;
+                wave_sync();
             }
-            mk = __ballot(open);
-            const int npush = __popcll(mk);
-            if (sp + npush > STACK) {
-                atomicOr(&W.ctr[1], 1);
-                sp = 0;
-                nm = npl = npu = 0;
-                break;
-            }
-            if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
-            sp += npush;
-            wave_sync();
             if (nm >= FLUSH_M) {
                 if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
                 flush_m2p<TPL>(L, nm, rx, ry, rz, A);
@@ -713,115 +814,6 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 if (lane + 64 < nf) L.farq[lane] = rest;
                 nf -= 64;
                 wave_sync();
-            }
-            if (MODE == MODE_LEARNED) {
-                const uint64_t bm = __ballot(band);
-                if (band) L.bandq[nb + __popcll(bm & lanemask_lt())] = node;
-                nb += __popcll(bm);
-                wave_sync();
-                if (nb >= 64 || (sp == 0 && nb > 0)) {
-                    // Decide up to 64 queued cells with one batched MLP pass on
-                    // the matrix cores (tb_mlp_eval32_mfma, two tiles of 32).
-                    // Entry e's features are recomputed from its records.
-                    const int ne = min(nb, 64);
-                    // lane l forms entry l's features; each tile's MFMA
-                    // operand takes features 0..3 of its entry in lanes 0..31
-                    // and 4..7 in lanes 32..63, exchanged across the halves
-                    float f[TB_MLP_NIN];
-#pragma unroll
-                    for (int j = 0; j < TB_MLP_NIN; j++) f[j] = 0.0f;
-                    if (lane < ne) {
-                        const uint32_t nd = L.bandq[lane];
-                        const double4 m = D.bmac[nd];
-                        const float ex = fmaxf(fabsf((float)(m.x - gx)) - hx, 0.0f);
-                        const float ey = fmaxf(fabsf((float)(m.y - gy)) - hy, 0.0f);
-                        const float ez = fmaxf(fabsf((float)(m.z - gz)) - hz, 0.0f);
-                        band_features(D.bf[nd], D.bq2[nd].w, sqrtf(ex * ex + ey * ey + ez * ez), hdiag, klemin,
-                                      D.bq0[nd].x * qt >= 0.0f, knact, f);
-                    }
-                    float xa[4], xb[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const float lo4 = __shfl_xor(f[q], 32, 64), hi4 = __shfl_xor(f[4 + q], 32, 64);
-                        xa[q] = lane < 32 ? f[q] : hi4;   // tile 0: entries 0..31
-                        xb[q] = lane < 32 ? lo4 : f[4 + q];  // tile 1: entries 32..63
-                    }
-                    float pr[2];
-                    pr[0] = tb_mlp_eval32_mfma(W.mlpk, xa);
-                    pr[1] = tb_mlp_eval32_mfma(W.mlpk, xb);
-                    // lane l decides entry l (tile l / 32 holds it); without a
-                    // previous force (emin = 0) band cells are opened
-                    const bool mine = lane < ne;
-                    const uint32_t nd = mine ? L.bandq[lane] : 0u;
-                    bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < W.log_err;
-                    bool bopen = mine && !bacc;
-                    // accepted band cells: far ones to the far queue, the rest to M2P
-                    float bx = 0.0f, by = 0.0f, bz = 0.0f;
-                    if (bacc) {
-                        const double4 m = D.bmac[nd];
-                        bx = (float)(m.x - gx);
-                        by = (float)(m.y - gy);
-                        bz = (float)(m.z - gz);
-                    }
-                    const bool bfar = bacc && bx * bx + by * by + bz * bz > far_r2;
-                    const uint64_t bf = __ballot(bfar);
-                    if (bfar) L.farq[nf + __popcll(bf & lanemask_lt())] = nd;
-                    nf += __popcll(bf);
-                    if (COUNT) {
-                        c_m2l[tsp] += (unsigned long long)__popcll(bf) * nact;
-                        c_far[tsp] += (unsigned long long)__popcll(bf);
-                    }
-                    bacc = bacc && !bfar;
-                    uint64_t bk = __ballot(bacc);
-                    if (bacc) {
-                        const int k = nm + __popcll(bk & lanemask_lt());
-                        const float4 q0 = D.bq0[nd], q1 = D.bq1[nd], q2 = D.bq2[nd];
-                        L.m0[k] = make_float4(bx, by, bz, q0.x);
-                        L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
-                        L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
-                        L.m3[k] = D.bq3[nd];
-                        L.m4[k] = D.bq4[nd];
-                    }
-                    nm += __popcll(bk);
-                    const uint32_t binf = bopen ? D.binfo[nd] : 0u;
-                    if (bopen) {
-                        const uint32_t st = binf >> 3, len = (binf & 7u) + 1u;
-                        if (st <= nd || st + len > (uint32_t)D.nn) {
-                            atomicOr(&W.ctr[1], 2);
-                            bopen = false;
-                        }
-                    }
-                    bk = __ballot(bopen);
-                    const int bpush = __popcll(bk);
-                    if (sp + bpush > STACK) {
-                        atomicOr(&W.ctr[1], 1);
-                        sp = 0;
-                        nm = npl = npu = nb = 0;
-                        break;
-                    }
-                    if (bopen) L.stack[sp + __popcll(bk & lanemask_lt())] = binf;
-                    sp += bpush;
-                    // the rest of the queue (at most 63 entries) moves down
-                    const uint32_t rest = lane + 64 < nb ? L.bandq[lane + 64] : 0u;
-                    wave_sync();
-                    if (lane + 64 < nb) L.bandq[lane] = rest;
-                    nb -= ne;
-                    wave_sync();
-                    if (nm >= FLUSH_M) {
-                        if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
-                        flush_m2p<TPL>(L, nm, rx, ry, rz, A);
-                        nm = 0;
-                        wave_sync();
-                    }
-                    if (nf >= 64) {
-                        push_far(L, 64, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
-                        const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
-                        wave_sync();
-                        if (lane + 64 < nf) L.farq[lane] = rest;
-                        nf -= 64;
-                        wave_sync();
-                    }
-                }
             }
         }
         if (nm > 0) {
