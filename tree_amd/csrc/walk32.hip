@@ -439,9 +439,10 @@ __device__ inline void push_far(const WaveLDS &L, int cnt, uint32_t *__restrict_
 
 // At most 3 blocks of 4 waves fit a CU's LDS, i.e. 3 waves per SIMD: the
 // register budget is capped to match (512 / 3 = 170 VGPRs).  The learned
-// walk spills a few registers around its batched band decision under that
-// cap; measured at N = 4M that is still faster (257 ms) than giving it 256
-// registers at 2 waves per SIMD (290 ms).
+// walk fits it without spilling since its band decisions and traversal steps
+// share one set of flushes; when it spilled (an earlier loop shape) it was
+// still faster under the cap (257 ms at N = 4M) than with 256 registers at 2
+// waves per SIMD (290 ms).
 template <int TPL, int MODE, bool COUNT>
 __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
 {
