@@ -703,9 +703,11 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 wave_sync();
                 if (MODE == MODE_LEARNED) {
                     const uint64_t bm = __ballot(band);
-                    if (band) L.bandq[nb + __popcll(bm & lanemask_lt())] = node;
-                    nb += __popcll(bm);
-                    wave_sync();
+                    if (bm) {  // wave-uniform: most steps meet no band cell
+                        if (band) L.bandq[nb + __popcll(bm & lanemask_lt())] = node;
+                        nb += __popcll(bm);
+                        wave_sync();
+                    }
                 }
             } else {
                 // Decide up to 64 queued cells with one batched MLP pass on
