@@ -351,35 +351,41 @@ __global__ __launch_bounds__(BT) void k_scan_final(int n, const int32_t *__restr
 
 // ------------------------------------------------------------------- lcp
 
+// A block walks chunks of BT keys (block-strided, so every wave runs the same
+// number of passes) and keeps its depth histogram in LDS across them: the
+// grid is capped, and the global histogram takes one set of atomics per
+// block instead of one per BT keys.
 __global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ key, int32_t *__restrict__ cnt,
                                             int32_t *__restrict__ hist, int32_t *__restrict__ err)
 {
     __shared__ int32_t lh[2][TB_LEVELS + 2];
     if (threadIdx.x < 2 * (TB_LEVELS + 2)) (&lh[0][0])[threadIdx.x] = 0;
     __syncthreads();
-    int i = blockIdx.x * BT + threadIdx.x;
-    int lm = TB_LEVELS + 2, leafd = 0;   // no depth counted for i >= n
-    if (i < n) {
-        uint64_t k = key[i];
-        lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
-        int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
-        int mx = max(lm, lp);
-        if (mx >= TB_LEVELS) {
-            atomicOr(err, 1);
-            mx = TB_LEVELS - 1;
-            lm = min(lm, mx);
+    for (long base = (long)blockIdx.x * BT; base < n; base += (long)gridDim.x * BT) {
+        const long i = base + threadIdx.x;
+        int lm = TB_LEVELS + 2, leafd = 0;  // no depth counted for i >= n
+        if (i < n) {
+            uint64_t k = key[i];
+            lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
+            int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
+            int mx = max(lm, lp);
+            if (mx >= TB_LEVELS) {
+                atomicOr(err, 1);
+                mx = TB_LEVELS - 1;
+                lm = min(lm, mx);
+            }
+            leafd = 1 + mx;
+            cnt[i] = leafd - lm;
         }
-        leafd = 1 + mx;
-        cnt[i] = leafd - lm;
-    }
-    // particle i opens the twigs at depths lm+1 .. leafd-1 and a leaf at
-    // depth leafd: counted per wavefront by ballots, one LDS add per depth
-    for (int d = 1; d < TB_LEVELS + 2; d++) {
-        const uint64_t tw = __ballot(lm < d && d < leafd);
-        const uint64_t lf = __ballot(leafd == d);
-        if ((threadIdx.x & 63) == 0) {
-            if (tw) atomicAdd(&lh[0][d], (int32_t)__popcll(tw));
-            if (lf) atomicAdd(&lh[1][d], (int32_t)__popcll(lf));
+        // particle i opens the twigs at depths lm+1 .. leafd-1 and a leaf at
+        // depth leafd: counted per wavefront by ballots, one LDS add per depth
+        for (int d = 1; d < TB_LEVELS + 2; d++) {
+            const uint64_t tw = __ballot(lm < d && d < leafd);
+            const uint64_t lf = __ballot(leafd == d);
+            if ((threadIdx.x & 63) == 0) {
+                if (tw) atomicAdd(&lh[0][d], (int32_t)__popcll(tw));
+                if (lf) atomicAdd(&lh[1][d], (int32_t)__popcll(lf));
+            }
         }
     }
     __syncthreads();
@@ -938,7 +944,7 @@ void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err
 
 void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist, int32_t *err)
 {
-    hipLaunchKernelGGL(k_lcp, dim3(nblk(n, BT)), dim3(BT), 0, s, n, key, cnt, hist, err);
+    hipLaunchKernelGGL(k_lcp, dim3(std::min(nblk(n, BT), 1024)), dim3(BT), 0, s, n, key, cnt, hist, err);
 }
 
 void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t *total, void *tmp)
