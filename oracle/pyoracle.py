@@ -36,6 +36,7 @@ def lib():
         L.orc_vv2.argtypes = [I, P, P, P, D, D, I, P]
         L.orc_vv2.restype = None
         L.orc_time_sample.argtypes = [I, P, I, P, D, D, I, I, P, P, P]
+        L.orc_force_targets.argtypes = [I, P, I, P, D, D, I, P, P, P, I]
         _lib = L
     return _lib
 
@@ -89,6 +90,19 @@ def direct(xe, xi, rs, tspecies, tidx, nthreads=0):
     lib().orc_direct(len(xe), _p(xe), len(xi), _p(xi), rs, len(tidx), _p(tspecies), _p(tidx), _p(f), _p(pot),
                      nthreads)
     return f, pot
+
+
+def force_targets(xe, xi, theta, rs, tspecies, tidx, nthreads=0):
+    """The tree walk's forces on listed targets only (same trees, same order as local_coulomb_sum)."""
+    xe, xi = _aos(xe), _aos(xi)
+    tspecies = np.ascontiguousarray(tspecies, np.int32)
+    tidx = np.ascontiguousarray(tidx, np.int32)
+    f = np.zeros((len(tidx), 3))
+    rc = lib().orc_force_targets(len(xe), _p(xe), len(xi), _p(xi), theta, rs, len(tidx), _p(tspecies), _p(tidx),
+                                 _p(f), nthreads)
+    if rc:
+        raise RuntimeError("oracle: coincident particles")
+    return f
 
 
 def vv1(x, v, f, xm, dt):

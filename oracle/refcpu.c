@@ -594,3 +594,45 @@ int orc_time_sample(int ne, const double *xe, int ni, const double *xi, double t
     free(Te); free(Ti);
     return err ? -1 : (sink != sink ? 1 : 0);
 }
+
+/*
+ * The per-target walk of orc_local_coulomb_sum for a list of targets only
+ * (tspecies[t] 0 electron / 1 ion, tidx[t] index in that species): the same
+ * trees and the same search order, so each listed target gets exactly the
+ * force the full walk gives it.  Lets tests pin the FP64 walk at sizes where
+ * walking every target on the CPU would take minutes.
+ */
+int orc_force_targets(int ne, const double *xe, int ni, const double *xi, double theta, double rs, int nt,
+                      const int *tspecies, const int *tidx, double *f, int nthreads)
+{
+    otree *Te = build(ne, xe, -1);
+    otree *Ti = build(ni, xi, +1);
+    int err = Te->error || Ti->error;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    if (!err) {
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < nt; t++) {
+            wctx W = {0};
+            W.theta = theta; W.rs = rs;
+            if (tspecies[t] == 0) {
+                W.t = xe + 3 * tidx[t];
+                W.like = 1; W.ff_m2p = -ORC_EPS; W.ff_p2p = ORC_EPS;
+                search(&W, Te->root);
+                W.like = 0; W.ff_m2p = -ORC_EPS; W.ff_p2p = -ORC_EPS;
+                search(&W, Ti->root);
+            } else {
+                W.t = xi + 3 * tidx[t];
+                W.like = 0; W.ff_m2p = ORC_EPS; W.ff_p2p = -ORC_EPS;
+                search(&W, Te->root);
+                W.like = 1; W.ff_m2p = ORC_EPS; W.ff_p2p = ORC_EPS;
+                search(&W, Ti->root);
+            }
+            for (int k = 0; k < 3; k++) f[3 * t + k] = W.f[k];
+        }
+    }
+    free_nodes(Te->root); free_nodes(Ti->root);
+    free(Te); free(Ti);
+    return err;
+}

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import tree_amd as ta
+from oracle import pyoracle
 from tests.conftest import announce
 
 pytestmark = pytest.mark.gpu
@@ -21,17 +22,28 @@ def _err(f, fd):
     return np.sqrt(np.mean(rel ** 2)), rel.max()
 
 
+_DIRECT = {}
+
+
+def _direct(qel, qion, rs):
+    """The oracle's FP64 direct sum (reference pair rules) on every particle, cached per plasma."""
+    key = (qel["xx"].tobytes(), qion["xx"].tobytes(), rs)
+    if key not in _DIRECT:
+        _DIRECT.clear()
+        ne, ni = len(qel), len(qion)
+        tsp = np.r_[np.zeros(ne, np.int32), np.ones(ni, np.int32)]
+        tix = np.r_[np.arange(ne), np.arange(ni)].astype(np.int32)
+        _DIRECT[key] = pyoracle.direct(qel["xx"], qion["xx"], rs, tsp, tix)[0]
+    return _DIRECT[key]
+
+
 def _run(qel, qion, theta, group=0, walk="mixed", rs=RS10):
     with ta.Engine(rs=rs, theta=theta, theta_mixed=theta, walk=walk, group=group) as eng:
         eng.set_particles(qel, qion)
         eng.build()
         ep = eng.force()
         e, i = eng.get_particles()
-        ne, ni = len(qel), len(qion)
-        tsp = np.r_[np.zeros(ne, np.int32), np.ones(ni, np.int32)]
-        tix = np.r_[np.arange(ne), np.arange(ni)].astype(np.int32)
-        fd = eng.direct_sample(tsp, tix)
-    return np.r_[e["ff"], i["ff"]], fd, ep
+    return np.r_[e["ff"], i["ff"]], _direct(qel, qion, rs), ep
 
 
 def test_mixed_tiny_theta_is_fp32_direct():
@@ -123,3 +135,38 @@ def test_mixed_step_runs_and_conserves():
         out = eng.step(dt, 5)
         e1 = 0.5 * (out[2] + out[3]) + out[0] + out[1]
     assert abs(e1 - e0) < 1e-3 * abs(e0), (e0, e1)
+
+
+def test_mixed_close_unlike_pairs_share_the_finest_cell():
+    """Electron-ion pairs 0.05 nm apart fall into one finest cell of the joint
+    tree (all 21 key digits equal).  The joint tree keeps that cell as a twig
+    over the particles' leaves one level deeper (the tree's depth is then 22),
+    and the walk matches the FP64 direct sum as for any other system (theta ->
+    0: rms < 3e-6; theta 0.4: rms < 2e-3).  Nine particles in one finest cell
+    are refused (TB_ERR_DEPTH)."""
+    announce("mixed_close_unlike_pairs")
+    qel, qion = ta.gen_plasma(26, 2000, 2000, 100.0)
+    for k in range(6):  # six electrons placed 5e-5 um from six ions
+        qel["xx"][k] = qion["xx"][k] + np.array([5e-5, 0.0, 0.0])
+    for theta, bound in ((1e-9, 3e-6), (0.4, 2e-3)):
+        with ta.Engine(rs=RS10, theta=0.5, theta_mixed=theta, walk="mixed") as eng:
+            eng.set_particles(qel, qion)
+            eng.build()
+            depth = eng.stats()["max_depth"][0]
+            eng.force()
+            e, i = eng.get_particles()
+        f = np.r_[e["ff"], i["ff"]]
+        rms, mx = _err(f, _direct(qel, qion, RS10))
+        print("close pairs theta %g: depth %d rms %.3e max %.3e" % (theta, depth, rms, mx))
+        assert depth == 22, depth
+        assert np.all(np.isfinite(f)) and rms < bound, (theta, rms, mx)
+    # nine particles (5 e-, 4 ions) within 1e-6 um: more than one finest cell holds
+    for k in range(5):
+        qel["xx"][10 + k] = qion["xx"][10] + np.array([1e-7 * (k + 1), 0.0, 0.0])
+    for k in range(3):
+        qion["xx"][11 + k] = qion["xx"][10] + np.array([0.0, 1e-7 * (k + 1), 0.0])
+    with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.4, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        with pytest.raises(ta.TreeError) as ex:
+            eng.build()
+    assert ex.value.code == 3, ex.value

@@ -324,7 +324,7 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0)
         const double *px[2] = {E.x[ce][0], I.x[ci][0]}, *py[2] = {E.x[ce][1], I.x[ci][1]},
                      *pz[2] = {E.x[ce][2], I.x[ci][2]};
         tb_jgather(s, nj, J.val[which[2]], px, py, pz, J.pos);
-        tb_lcp(s, nj, J.key[which[2]], J.cnt, h->d_misc + MISC_J + 2, h->d_misc + MISC_J);
+        tb_lcp(s, nj, J.key[which[2]], J.cnt, h->d_misc + MISC_J + 2, h->d_misc + MISC_J, true);
         tb_scan_excl(s, nj, J.cnt, J.offs, h->d_misc + MISC_J + 1, h->scan_tmp);
     }
     HIPCHK(hipGetLastError());
@@ -342,9 +342,12 @@ int build_phase_b(tb_handle *h, const int *which)
         int e = sp < 2 ? h->h_misc[sp] : (mixed ? h->h_misc[MISC_J] : 0);
         if (e & 4) return fail(TB_ERR_HIP, "species %d: internal sort check failed", sp);
         if (e & 2) return fail(TB_ERR_NONFINITE, "non-finite position in species %d", sp % 2);
-        if (e & 1)
+        if (e & 1) {
+            if (sp == 2)
+                return fail(TB_ERR_DEPTH, "joint tree: more than 8 particles share all %d key levels", TB_LEVELS);
             return fail(TB_ERR_DEPTH, "%s: two particles share all %d key levels (coincident)",
-                        sp == 0 ? "electrons" : (sp == 1 ? "ions" : "joint tree"), TB_LEVELS);
+                        sp == 0 ? "electrons" : "ions", TB_LEVELS);
+        }
     }
     if (mixed) {
         for (int sp = 0; sp < 2; sp++) h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];

@@ -355,8 +355,15 @@ __global__ __launch_bounds__(BT) void k_scan_final(int n, const int32_t *__restr
 // number of passes) and keeps its depth histogram in LDS across them: the
 // grid is capped, and the global histogram takes one set of atomics per
 // block instead of one per BT keys.
+//
+// Particles whose keys agree in all TB_LEVELS digits: a species tree (the
+// reference's one particle per leaf) raises the coincidence flag.  The joint
+// tree of the mixed walk instead keeps the finest cell as a twig whose
+// children are its particles' leaves at depth TB_LEVELS + 1, for up to 8
+// particles (an electron-ion pair a few nm apart is physical: the Kelbg
+// potential is finite at r = 0); 9 or more raise the flag.
 __global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ key, int32_t *__restrict__ cnt,
-                                            int32_t *__restrict__ hist, int32_t *__restrict__ err)
+                                            int32_t *__restrict__ hist, int32_t *__restrict__ err, int joint)
 {
     __shared__ int32_t lh[2][TB_LEVELS + 2];
     if (threadIdx.x < 2 * (TB_LEVELS + 2)) (&lh[0][0])[threadIdx.x] = 0;
@@ -370,9 +377,14 @@ __global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ 
             int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
             int mx = max(lm, lp);
             if (mx >= TB_LEVELS) {
-                atomicOr(err, 1);
-                mx = TB_LEVELS - 1;
-                lm = min(lm, mx);
+                if (joint) {
+                    if (i >= 8 && lcp_digits(key[i - 8], k) >= TB_LEVELS) atomicOr(err, 1);
+                    mx = TB_LEVELS;
+                } else {
+                    atomicOr(err, 1);
+                    mx = TB_LEVELS - 1;
+                    lm = min(lm, mx);
+                }
             }
             leafd = 1 + mx;
             cnt[i] = leafd - lm;
@@ -653,7 +665,8 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
     uint64_t k = key[i];
     int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
     int lp = i < n - 1 ? lcp_digits(k, key[i + 1]) : 0;
-    int mx = min(max(lm, lp), TB_LEVELS - 1);
+    // identical keys: leaves one level below the finest cell (see k_lcp)
+    int mx = min(max(lm, lp), TB_LEVELS);
     lm = min(lm, mx);
     int leafd = 1 + mx;
     int base = offs[i] + 1;
@@ -942,9 +955,10 @@ void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err
     S.cur = o;
 }
 
-void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist, int32_t *err)
+void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist, int32_t *err, bool joint)
 {
-    hipLaunchKernelGGL(k_lcp, dim3(std::min(nblk(n, BT), 1024)), dim3(BT), 0, s, n, key, cnt, hist, err);
+    hipLaunchKernelGGL(k_lcp, dim3(std::min(nblk(n, BT), 1024)), dim3(BT), 0, s, n, key, cnt, hist, err,
+                       joint ? 1 : 0);
 }
 
 void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t *total, void *tmp)

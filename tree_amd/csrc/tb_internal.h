@@ -87,8 +87,9 @@ void tb_keys(hipStream_t s, int n, const double *x, const double *y, const doubl
 void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int lo_bit, int hi_bit, void *tmp,
                    int *which, bool init_iota);
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err);
+// joint: the mixed walk's joint tree (up to 8 particles may share a full key)
 void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist /*2*(TB_LEVELS+2)*/,
-            int32_t *err);
+            int32_t *err, bool joint = false);
 void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t *total, void *tmp);
 size_t tb_scan_temp_bytes(int n);
 void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign);

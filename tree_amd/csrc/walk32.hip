@@ -52,6 +52,11 @@ constexpr int LCAP_M = FLUSH_M - 1 + 64;
 constexpr int FLUSH_P = 24;
 constexpr int LCAP_P = FLUSH_P - 1 + 64;
 constexpr int QCAP = 128;
+// Unlike leaves within 22 rs of some target of the group: up to NEXACT of
+// them per group are evaluated after the walk with separations from the FP64
+// positions; more (a cluster of bound pairs near one group) fall back to
+// FP32 group-relative coordinates, accurate to ~6e-8 of the group's size.
+constexpr int NEXACT = 8;
 constexpr int NXCD = 8;       // XCDs of an MI355X (work chunks of the walk queue)
 constexpr int CTR_CHUNK = 8;  // first chunk counter in the counter block (16 ints)
 constexpr int CTR_FAR = 4;    // far-chunk allocation counter
@@ -94,7 +99,8 @@ struct WaveLDS {
     // accepted cells: x y z q | Dx Dy Dz Q'xx | Q'yy Q'xy Q'yz Q'zx |
     // O'xxx O'xxy O'xxz O'xyy | O'xyz O'yyy O'yyz -
     float4 m0[LCAP_M], m1[LCAP_M], m2[LCAP_M], m3[LCAP_M], m4[LCAP_M];
-    float4 p[LCAP_P];                     // leaves x y z q: like charges from 0 up, unlike from LCAP_P - 1 down
+    float4 p[LCAP_P];                     // leaves x y z, record index: like charges from 0 up, unlike from LCAP_P - 1 down
+    uint32_t exact[NEXACT];               // leaves within 22 rs of some target: FP64 separations after the walk
 };
 // measured: 4 x 13056 bytes per block kept 3 blocks on a CU, 4 x 13568 did not
 static_assert(sizeof(WaveLDS) * 4 <= 4 * 13056, "walk LDS would drop a CU to two blocks");
@@ -299,10 +305,21 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
 // [LCAP_P - nu, LCAP_P): the Coulomb loop (with the 0.1 um cut) and the Kelbg
 // loop then run without a per-entry branch on the charge.  Every like source
 // has charge qt and every unlike one -qt, so both loops sum q-free terms and
-// the sign goes on once at the end.
+// the sign goes on once at the end.  An entry's w holds its leaf's record
+// index (uint32 bits): the charge follows from the list it is in.
+//
+// The close test of an unlike pair (r < 22 rs) is one expression, repeated
+// alike wherever a pair is classified, so every pair is counted once.
+__device__ inline bool close_pair(float dx, float dy, float dz, float inv_rs, float &r2, float &rq)
+{
+    r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    rq = __builtin_amdgcn_rsqf(r2);
+    return r2 * rq * inv_rs < 22.0f;  // r / rs
+}
+
 template <int TPL>
-__device__ inline void flush_p2p(const WaveLDS &L, int nl, int nu, const float (&rx)[TPL], const float (&ry)[TPL],
-                                 const float (&rz)[TPL], float qt, float inv_rs, Acc<TPL> &A)
+__device__ inline void flush_p2p(WaveLDS &L, int nl, int nu, const float (&rx)[TPL], const float (&ry)[TPL],
+                                 const float (&rz)[TPL], float qt, float inv_rs, Acc<TPL> &A, int &nx)
 {
     float ax[TPL], ay[TPL], az[TPL], ap[TPL];
 #pragma unroll
@@ -334,11 +351,8 @@ __device__ inline void flush_p2p(const WaveLDS &L, int nl, int nu, const float (
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
-            // the same operations as in the pass below, so both passes split
-            // the pairs alike
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const float rq = __builtin_amdgcn_rsqf(r2);
-            const bool cl = r2 * rq * inv_rs < 22.0f;  // r / rs
+            float r2, rq;
+            const bool cl = close_pair(dx, dy, dz, inv_rs, r2, rq);
             close = close || cl;
             const float ri = cl ? 0.0f : rq;
             const float w = ri * ri * ri;
@@ -349,23 +363,83 @@ __device__ inline void flush_p2p(const WaveLDS &L, int nl, int nu, const float (
         }
     }
     if (__ballot(close)) {
+        // leaves close to some target: queued for the exact pass after the
+        // walk while it has room, else evaluated here in FP32
         for (int j = LCAP_P - nu; j < LCAP_P; j++) {
             const float4 q = L.p[j];
+            bool any = false;
+#pragma unroll
+            for (int t = 0; t < TPL; t++) {
+                float r2, rq;
+                any = any || close_pair(rx[t] - q.x, ry[t] - q.y, rz[t] - q.z, inv_rs, r2, rq);
+            }
+            if (!__ballot(any)) continue;  // wave-uniform
+            if (nx < NEXACT) {
+                L.exact[nx] = __float_as_uint(q.w);  // every lane stores the same value
+                nx++;
+                continue;
+            }
 #pragma unroll
             for (int t = 0; t < TPL; t++) {
                 const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float rq = __builtin_amdgcn_rsqf(r2);
-                const float u = r2 * rq * inv_rs;
-                if (u < 22.0f) {
+                float r2, rq;
+                if (close_pair(dx, dy, dz, inv_rs, r2, rq)) {
                     float g, h;
-                    kelbg_gh(u, g, h);
+                    kelbg_gh(r2 * rq * inv_rs, g, h);
                     const float w = g * rq * rq * rq;
                     ax[t] = fmaf(-dx, w, ax[t]);
                     ay[t] = fmaf(-dy, w, ay[t]);
                     az[t] = fmaf(-dz, w, az[t]);
                     ap[t] = fmaf(-h, rq, ap[t]);
                 }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        nadd(A.s[t][0], A.c[t][0], qt * ax[t]);
+        nadd(A.s[t][1], A.c[t][1], qt * ay[t]);
+        nadd(A.s[t][2], A.c[t][2], qt * az[t]);
+        nadd(A.ps[t], A.pc[t], qt * ap[t]);
+    }
+}
+
+// The exact pass over the group's queued close leaves, after the walk: the
+// same close test as in flush_p2p (same FP32 operands: the leaf's offset from
+// the group centre rounded as when it was listed), then the pair's Kelbg term
+// from the FP64 positions of the leaf and the target.  An electron bound a
+// few nm from an ion is far below the resolution of FP32 coordinates relative
+// to a group centre some um away.
+template <int TPL>
+__device__ inline void exact_pairs(const WaveLDS &L, int nx, const float (&rx)[TPL], const float (&ry)[TPL],
+                                   const float (&rz)[TPL], float qt, float inv_rs, double gx, double gy, double gz,
+                                   const double4 *__restrict__ bmac, const double *__restrict__ px,
+                                   const double *__restrict__ py, const double *__restrict__ pz, int tbase, int nact,
+                                   Acc<TPL> &A)
+{
+    float ax[TPL], ay[TPL], az[TPL], ap[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
+    for (int e = 0; e < nx; e++) {
+        const double4 m = bmac[L.exact[e]];
+        const float cx = (float)(m.x - gx), cy = (float)(m.y - gy), cz = (float)(m.z - gz);
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            float r2, rq;
+            if (close_pair(rx[t] - cx, ry[t] - cy, rz[t] - cz, inv_rs, r2, rq)) {
+                const int k = lane_id() + 64 * t;
+                const int ti = tbase + (k < nact ? k : 0);
+                const float ex = (float)(px[ti] - m.x), ey = (float)(py[ti] - m.y), ez = (float)(pz[ti] - m.z);
+                const float e2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                const float eq = e2 > 0.0f ? __builtin_amdgcn_rsqf(e2) : 0.0f;
+                float g, h;
+                kelbg_gh(e2 * eq * inv_rs, g, h);
+                const float w = g * eq * eq * eq;
+                ax[t] = fmaf(-ex, w, ax[t]);
+                ay[t] = fmaf(-ey, w, ay[t]);
+                az[t] = fmaf(-ez, w, az[t]);
+                // coincident (r = 0): phi = 1 / rs and no force
+                ap[t] = e2 > 0.0f ? fmaf(-h, eq, ap[t]) : ap[t] - inv_rs;
             }
         }
     }
@@ -491,8 +565,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         const int tsp = gl < D.ng[0] ? 0 : 1;
         const float qt = tsp ? 1.0f : -1.0f;
         const int2 gr = D.grp[tsp][tsp ? gl - D.ng[0] : gl];
-        const int base = gr.x;
-        const int nact = gr.y;  // targets in this group (at most G)
+        const int base = uni(gr.x);
+        const int nact = uni(gr.y);  // targets in this group (at most G)
         const double *px = D.px[tsp], *py = D.py[tsp], *pz = D.pz[tsp];
 
         // targets and the group's bounding box (FP64)
@@ -554,7 +628,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         F.head = -1;
 
         // stack, M2P, P2P (like, unlike), band-queue, far-queue lengths
-        int sp = 1, nm = 0, npl = 0, npu = 0, nb = 0, nf = 0;
+        int sp = 1, nm = 0, npl = 0, npu = 0, nb = 0, nf = 0, nx = 0;
         L.stack[0] = 0u;  // root: range of one cell at 0
         wave_sync();
         while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
@@ -671,8 +745,9 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 const float qleaf = leaf ? D.bq0[node].x : 0.0f;
                 const bool lk = leaf && qleaf == qt;
                 const uint64_t ml = __ballot(lk), mu = __ballot(leaf && !lk);
-                if (lk) L.p[npl + __popcll(ml & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
-                else if (leaf) L.p[LCAP_P - 1 - npu - __popcll(mu & lanemask_lt())] = make_float4(cx, cy, cz, qleaf);
+                const float nodebits = __uint_as_float(node);
+                if (lk) L.p[npl + __popcll(ml & lanemask_lt())] = make_float4(cx, cy, cz, nodebits);
+                else if (leaf) L.p[LCAP_P - 1 - npu - __popcll(mu & lanemask_lt())] = make_float4(cx, cy, cz, nodebits);
                 npl += __popcll(ml);
                 npu += __popcll(mu);
                 if (COUNT) {
@@ -805,7 +880,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 wave_sync();
             }
             if (npl + npu >= FLUSH_P) {
-                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A);
+                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A, nx);
                 npl = npu = 0;
                 wave_sync();
             }
@@ -824,7 +899,11 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             flush_m2p<TPL>(L, nm, rx, ry, rz, A);
         }
         if (npl + npu > 0) {
-            flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A);
+            flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A, nx);
+        }
+        if (nx > 0) {
+            wave_sync();
+            exact_pairs<TPL>(L, nx, rx, ry, rz, qt, D.inv_rs, gx, gy, gz, D.bmac, px, py, pz, base, nact, A);
         }
         if (nf > 0) push_far(L, nf, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
         W.far_head[g] = F.head;  // every lane stores the same value
