@@ -120,6 +120,20 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
 int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
 
 /*
+ * tb_direct_sample without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * target list and the forces are staged in a fixed per-GPU array of the
+ * library (2.25 MiB, held until the process ends) instead of memory allocated
+ * per call, so nt is at most TB_NOALLOC_MAX_TARGETS (TB_ERR_ARG beyond).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (handles on the same device driven from several host threads) race.  Same
+ * forces as tb_direct_sample, bit for bit.
+ */
+#define TB_NOALLOC_MAX_TARGETS 65536
+int tb_direct_sample_noalloc(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
+
+/*
  * Check of the mixed walk's far-field local expansion (no reference
  * counterpart; tests only).  cells [ncell][19]: R = expansion centre - cell
  * centre (3), q, D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx (traceless quadrupole),

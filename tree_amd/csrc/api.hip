@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,52 @@ int fail(int code, const char *fmt, ...)
         hipError_t e_ = (expr);                                                               \
         if (e_ != hipSuccess) return fail(TB_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
+
+}  // namespace
+
+// ---------------------------------------------- allocation-free direct sum
+//
+// tb_direct_sample_noalloc stages its target list and forces in `scratch`, a
+// fixed device array (one copy per GPU, held until the process ends), instead
+// of device memory allocated per call.  Host code never names `scratch` as
+// an address: its device address is looked up once per device, with that
+// device current (scratch_ptr), and only that address reaches kernels and
+// copies.  Layout: tspecies, reference indices and key-order indices (int32,
+// TB_NOALLOC_MAX_TARGETS each), then forces (3 doubles per target).
+constexpr size_t NOALLOC_INT_BYTES = (size_t)TB_NOALLOC_MAX_TARGETS * 4;
+constexpr size_t NOALLOC_BYTES = 3 * NOALLOC_INT_BYTES + (size_t)TB_NOALLOC_MAX_TARGETS * 24;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of this device's `scratch` (the device must be current);
+// nullptr if it cannot be looked up
+void *scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// reference index -> key-order index of each listed target, through the
+// species' inverse permutations
+__global__ __launch_bounds__(256) void k_map_targets(int nt, const int32_t *__restrict__ tsp,
+                                                     const int32_t *__restrict__ tref,
+                                                     const int32_t *__restrict__ inv_e,
+                                                     const int32_t *__restrict__ inv_i, int32_t *__restrict__ tout)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < nt) tout[t] = tsp[t] == 0 ? inv_e[tref[t]] : inv_i[tref[t]];
+}
 
 }  // namespace
 
@@ -543,6 +590,8 @@ int tb_create(tb_handle **out, const tb_config *cfg)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(TB_ERR_NODEV, "no HIP device");
     if (cfg->device < 0 || cfg->device >= ndev) return fail(TB_ERR_ARG, "device %d out of range", cfg->device);
     HIPCHK(hipSetDevice(cfg->device));
+    // this device's scratch array, looked up once per device with it current
+    if (!scratch_ptr(cfg->device)) return fail(TB_ERR_HIP, "device %d: scratch array address lookup failed", cfg->device);
     tb_handle *h = new tb_handle();
     h->cfg = *cfg;
     if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess) {
@@ -866,6 +915,38 @@ int tb_direct_sample(tb_handle *h, int32_t nt, const int32_t *tspecies, const in
     if (e == hipSuccess) e = hipMemcpy(f, d_f, (size_t)nt * 24, hipMemcpyDeviceToHost);
     dfree(h, tmp, (size_t)nt * (8 + 24) + 64);
     if (e != hipSuccess) return fail(TB_ERR_HIP, "tb_direct_sample: %s", hipGetErrorString(e));
+    return TB_OK;
+}
+
+int tb_direct_sample_noalloc(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f)
+{
+    if (!h || !tspecies || !tidx || !f) return fail(TB_ERR_ARG, "null argument");
+    if (nt < 1 || nt > TB_NOALLOC_MAX_TARGETS)
+        return fail(TB_ERR_ARG, "nt must be in 1 .. %d (got %d)", TB_NOALLOC_MAX_TARGETS, nt);
+    if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "no particles set");
+    for (int t = 0; t < nt; t++) {
+        const int sp = tspecies[t];
+        if (sp != 0 && sp != 1) return fail(TB_ERR_ARG, "tspecies[%d] = %d", t, sp);
+        if (tidx[t] < 0 || tidx[t] >= h->sp[sp].n) return fail(TB_ERR_ARG, "tidx[%d] = %d out of range", t, tidx[t]);
+    }
+    HIPCHK(hipSetDevice(h->cfg.device));
+    unsigned char *base = (unsigned char *)scratch_ptr(h->cfg.device);
+    if (!base) return fail(TB_ERR_HIP, "scratch array address lookup failed");
+    int32_t *d_sp = (int32_t *)base, *d_ref = (int32_t *)(base + NOALLOC_INT_BYTES),
+            *d_ix = (int32_t *)(base + 2 * NOALLOC_INT_BYTES);
+    double *d_f = (double *)(base + 3 * NOALLOC_INT_BYTES);
+    hipStream_t s = h->s;
+    HIPCHK(hipMemcpyAsync(d_sp, tspecies, (size_t)nt * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_ref, tidx, (size_t)nt * 4, hipMemcpyHostToDevice, s));
+    // inverse permutations of the current key order, into each species'
+    // build-time scratch column (rewritten by every build)
+    for (int sp = 0; sp < 2; sp++) tb_inverse_perm(s, h->sp[sp].n, h->sp[sp].orig[h->sp[sp].cur], h->sp[sp].cnt);
+    hipLaunchKernelGGL(k_map_targets, dim3((nt + 255) / 256), dim3(256), 0, s, nt, d_sp, d_ref, h->sp[0].cnt,
+                       h->sp[1].cnt, d_ix);
+    tb_direct(s, nt, d_sp, d_ix, h->sp[0], h->sp[1], h->cfg.rs, d_f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(f, d_f, (size_t)nt * 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return TB_OK;
 }
 

@@ -30,6 +30,7 @@ TPS = 0.10180505305371923        # ns per code time unit, data.f90:39
 XM_E = 5.4858e-4                 # amu
 XM_I = 85.0                      # amu (Rb-85)
 
+NOALLOC_MAX_TARGETS = 65536  # TB_NOALLOC_MAX_TARGETS, include/treemi.h
 NDIST = 10000  # histogram bins of the reference's RDF, less one (data.f90:38)
 WALK = {"ref64": 0, "mixed": 1}
 MLP_NPARAMS = 1377
@@ -97,6 +98,7 @@ SIGNATURES = [
     ("tb_step", ctypes.c_int, [_P, _D, _I, _P]),
     ("tb_export_tree", ctypes.c_int, [_P, _I, _I, _P] + [_P] * 12),
     ("tb_direct_sample", ctypes.c_int, [_P, _I, _P, _P, _P]),
+    ("tb_direct_sample_noalloc", ctypes.c_int, [_P, _I, _P, _P, _P]),
     ("tb_local_eval", ctypes.c_int, [_P, _I, _I, _P, _I, _P, _P]),
     ("tb_force_targets", ctypes.c_int, [_P, _I, _P, _P, _P]),
     ("tb_time_walk", ctypes.c_int, [_P, _I, _P]),
@@ -267,6 +269,14 @@ class Engine:
         tidx = np.ascontiguousarray(tidx, np.int32)
         f = np.zeros((len(tidx), 3))
         _check(self.lib.tb_direct_sample(self.h, len(tidx), _ptr(tspecies), _ptr(tidx), _ptr(f)))
+        return f
+
+    def direct_sample_noalloc(self, tspecies, tidx):
+        """tb_direct_sample with no device allocation (at most NOALLOC_MAX_TARGETS targets)."""
+        tspecies = np.ascontiguousarray(tspecies, np.int32)
+        tidx = np.ascontiguousarray(tidx, np.int32)
+        f = np.zeros((len(tidx), 3))
+        _check(self.lib.tb_direct_sample_noalloc(self.h, len(tidx), _ptr(tspecies), _ptr(tidx), _ptr(f)))
         return f
 
     def local_eval(self, order, cells, y):
