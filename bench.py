@@ -198,7 +198,17 @@ def cpu_baseline(qel, qion, theta, rs, threads):
         stride = max(1, int(stride * max(tw, 1e-3) / 15.0))
         tb, tw, nw = run(stride)
     t_step = tb + tw * n / nw
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": kind,
+            "cpu_model": model, "host_cpus": os.cpu_count(), "cpus_available": len(os.sched_getaffinity(0)),
             "sample": "one serial tree build of both species (%.1f s) + walk of every %d-th target (%d targets, "
                       "%.1f s) at the plain walk's tuned theta=%.4f, scaled to all %d targets"
                       % (tb, stride, nw, tw, theta, n)}
@@ -217,13 +227,12 @@ def run_plain(qel, qion, rs, tsp, tix, fdir, a):
                              0.01, 0.3, 7)
     eng.build()
     ms_build = eng.stats()["ms_build"]
-    # sampled walk timing: every stride-th target of each species, at least
-    # 2^17 per species so that the sample fills the GPU (the scaled time is a
-    # throughput, not one wave's latency)
+    # sampled walk timing: every stride-th block of 64 consecutive targets of
+    # each species, at least 2^17 targets per species so that the sample
+    # fills the GPU (the scaled time is a throughput, not one wave's latency)
     n = len(qel) + len(qion)
     stride = max(1, min(len(qel), len(qion)) // (1 << 17))
-    ms = eng.time_walk(stride)
-    nsamp = -(-len(qel) // stride) + -(-len(qion) // stride)
+    ms, nsamp = eng.time_walk(stride)
     ms_walk = ms * n / nsamp
     t0 = time.perf_counter()
     eng.vv2(0.0, False)
@@ -267,7 +276,6 @@ def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
 def train_mlp(a, th_lo, th_hi):
     """Labels from the method itself on a 2^20-particle plasma (the band the
     learned walk will decide), then GPU training."""
-    from oracle import mlp_ref  # the seeded initial parameter vector only
     n = 1 << 19
     qel, qion = ta.gen_plasma(a.seed + 1, n, n, 400.0, te=a.te, ti=1e-3)
     rs = ta.kelbg_length(a.te)
@@ -275,7 +283,7 @@ def train_mlp(a, th_lo, th_hi):
         eng.set_particles(qel, qion)
         eng.build()
         X, y = eng.mlp_collect(cap=1 << 20)
-        eng.mlp_set(mlp_ref.init_params(0).astype(np.float32))
+        eng.mlp_init(0)
         hist = eng.mlp_train(X, y, steps=4000, batch=8192, lr=2e-3, seed=1)
         p = eng.mlp_get()
     log("  MLP: %d labelled band pairs, loss %.3f -> %.3f" % (len(X), hist[:20].mean(), hist[-50:].mean()))
@@ -335,7 +343,10 @@ def _steps_per_s(eng, dt, steps):
 
 
 def run_c2(a):
-    """2^19 + 2^19, sigma 400 um, theta 0.5: the plain FP64 walk and the mixed walk at the same theta."""
+    """2^19 + 2^19, sigma 400 um, theta 0.5: the plain FP64 walk and the mixed walk at the same theta,
+    each stepped a.steps times (steps/s and energy drift); the FP64 walk's counted flops against its
+    kernel time (FP64 VALU roofline); the sampled walk timing that the headline's plain-walk figure uses,
+    checked against a full force."""
     n, dt = 1 << 19, 0.005 / ta.TPS
     rs = ta.kelbg_length(10.0)
     qel, qion = ta.gen_plasma(2, n, n, 400.0, te=10.0, ti=1e-3)
@@ -349,11 +360,36 @@ def run_c2(a):
             rms, mx = err_full(eng, tsp, tix, fdir)
             st = eng.count_interactions()
             eng.set_particles(qel, qion)
-            sps, stt = _steps_per_s(eng, dt, a.steps)
-            out[walk] = {"steps_per_s": sps, "rms": rms, "max": mx, "ms_build": stt["ms_build"],
-                         "ms_walk": stt["ms_walk"],
-                         "interactions_per_s": interactions(st) * sps}
-            log("C2 %s: %.2f steps/s rms %.2e" % (walk, sps, rms))
+            r = timed_steps(eng, dt, a.steps, 0)
+            sps = 1.0 / r["s"]
+            row = {"steps_per_s": sps, "rms": rms, "max": mx, "ms_build": r["ms_build"], "ms_walk": r["ms_walk"],
+                   "energy_drift": r["drift"], "interactions_per_s": interactions(st) * sps}
+            if walk == "ref64":
+                fl = walk_flops64(st)
+                row["walk_tflops"] = fl / (r["ms_walk"] * 1e-3) / 1e12
+                row["walk_frac_fp64_peak"] = row["walk_tflops"] / PEAK["fp64"]
+                row["flops_per_force"] = fl
+                # the sampled timing of the headline's plain walk, against the full force just timed
+                eng.set_particles(qel, qion)
+                eng.build()
+                stride = max(1, n // (1 << 17))
+                runs = [eng.time_walk(stride) for _ in range(3)]
+                ms, nsamp = min(x[0] for x in runs), runs[0][1]
+                row["sampled_walk_check"] = {"stride": stride, "scaled_ms": ms * 2 * n / nsamp,
+                                             "full_ms": r["ms_walk"]}
+                # and at the headline's small theta, where the full walk takes tens of seconds
+                if a.theta64 > 0:
+                    eng.configure(theta=a.theta64)
+                    eng.build()
+                    t0 = time.perf_counter()
+                    eng.force()
+                    full = eng.stats()["ms_force"]
+                    log("C2 FP64 full walk at theta %.4f: %.1f s" % (a.theta64, time.perf_counter() - t0))
+                    ms, nsamp = eng.time_walk(stride)
+                    row["sampled_walk_check_small_theta"] = {"theta": a.theta64, "stride": stride,
+                                                             "scaled_ms": ms * 2 * n / nsamp, "full_ms": full}
+            out[walk] = row
+            log("C2 %s: %.2f steps/s rms %.2e drift %.2e" % (walk, sps, rms, r["drift"]))
     print(json.dumps(out))
 
 
@@ -586,7 +622,10 @@ def main():
     }
     if not a.no_cpu:
         log("timing the reference CPU path on a bounded sample")
-        threads = min(16, os.cpu_count() or 1)
+        # every CPU this job may use: the OpenMP thread count the host sets
+        # for it (the CPU share of one GPU), else the CPUs it is bound to
+        avail = len(os.sched_getaffinity(0))
+        threads = min(avail, int(os.environ.get("OMP_NUM_THREADS") or avail))
         th_cpu = plain["theta"] if plain is not None else 0.5
         res["cpu_baseline"] = cpu_baseline(qel, qion, th_cpu, rs, threads)
     print(json.dumps(res))

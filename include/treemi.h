@@ -84,7 +84,8 @@ int tb_create(tb_handle **h, const tb_config *cfg);
 int tb_destroy(tb_handle *h);
 int tb_set_config(tb_handle *h, const tb_config *cfg);
 
-/* positions, velocities (forces ignored) in the reference's index order */
+/* positions, velocities in the reference's index order; forces are not read
+ * (the device forces start at zero until the next tb_force) */
 int tb_set_particles(tb_handle *h, int32_t ne, const tb_pt *e, int32_t ni, const tb_pt *i);
 /* x, v, f back in the order given to tb_set_particles */
 int tb_get_particles(tb_handle *h, tb_pt *e, tb_pt *i);
@@ -150,9 +151,11 @@ int tb_local_eval(tb_handle *h, int32_t order, int32_t ncell, const float *cells
  * order): f [nt][3].  Same forces as tb_force gives those targets.
  */
 int tb_force_targets(tb_handle *h, int32_t nt, const int32_t *tspecies, const int32_t *tidx, double *f);
-/* device time of the FP64 walk over every stride-th target of each species
- * (sampled timing; per-target walks are independent) */
-int tb_time_walk(tb_handle *h, int32_t stride, float *ms);
+/* device time of the FP64 walk over every stride-th block of 64 consecutive
+ * (key-ordered) targets of each species, so that each wavefront walks
+ * neighbouring targets as in the full walk (sampled timing; per-target walks
+ * are independent); *nwalked receives the number of targets walked */
+int tb_time_walk(tb_handle *h, int32_t stride, float *ms, int64_t *nwalked);
 
 typedef struct tb_stats {
     int64_t nodes[2];        /* node count per species tree                     */
@@ -186,6 +189,8 @@ int tb_get_stats(tb_handle *h, tb_stats *s);
 #define TB_MLP_NPARAMS 1377
 int tb_mlp_set(tb_handle *h, const float *params, int32_t n);
 int tb_mlp_get(tb_handle *h, float *params, int32_t n);
+/* seeded initial network (He-uniform weights, zero biases) for tb_mlp_train */
+int tb_mlp_init(tb_handle *h, uint64_t seed);
 /* forward pass on n samples X[n][8] (bf16 != 0: weights rounded to BF16) */
 int tb_mlp_forward(tb_handle *h, int32_t n, const float *X, int32_t bf16, float *out);
 /* mean squared error and its gradient (fp32 master weights) */

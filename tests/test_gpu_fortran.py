@@ -73,3 +73,64 @@ def test_fortran_driver_matches_capi(tmp_path):
         assert np.allclose(got_rdf[:, 1], hei / 5.0, rtol=1e-6, atol=0)
         assert np.allclose(got_rdf[:, 3], hii / 5.0, rtol=1e-6, atol=0)
     assert np.array_equal(f_i["id"], np.arange(1, ni + 1))
+    _check_records(tmp_path)
+
+
+def _check_records(d):
+    """Every record of the driver's ener / res / rdf files is laid out as the
+    reference's FORMAT statements write it (plasma_io's formatters, pinned to
+    amdflang's output by tests/test_formats.py)."""
+    for path in sorted(d.glob("ener*.dat")):
+        for ln in path.read_text().splitlines()[1:]:
+            vals = [plasma_io.fortran_float(v) for v in ln.split()]
+            assert ln == plasma_io.format_ener(vals), (path.name, ln)
+    for path in sorted(d.glob("rdf*.dat")):
+        for ln in path.read_text().splitlines()[1:50]:
+            vals = [plasma_io.fortran_float(v) for v in ln.split()]
+            assert ln == plasma_io.format_rdf(vals), (path.name, ln)
+    for path in sorted(d.glob("res*.xyz")):
+        for ln in path.read_text().splitlines()[2:]:
+            t = ln.split()
+            v = [plasma_io.fortran_float(x) for x in t[1:7]]
+            assert ln == plasma_io.format_res(t[0], v[:3], v[3:], int(t[7])), (path.name, ln)
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="Fortran driver not built")
+def test_fortran_driver_resume(tmp_path):
+    """Resume (rest /= 'OFF', main.f90:333-380 reading, Vinit_REST main.f90:796-834):
+    10 steps in one run against 5 steps, res001.xyz, and 5 more steps resumed
+    from it.  The resumed run starts from the 7-digit positions and velocities
+    of the restart file, so it matches the uninterrupted run to that
+    precision: positions within 1e-6 of the largest coordinate, velocities
+    and energies (T_e, T_i, Epot, Etot) within 1e-4 relative."""
+    announce("fortran_driver_resume")
+    ne = ni = 1200
+    rs = 6.6493e-3
+    qel, qion = ta.gen_plasma(9, ne, ni, 100.0)
+    full, first, second = tmp_path / "full", tmp_path / "first", tmp_path / "second"
+    for d, tmax in ((full, 0.05), (first, 0.025)):
+        d.mkdir()
+        plasma_io.write_control(d / "control.prm", ne, ni, tmax=tmax, trest=0.025, tdump=0.025, tsamp=0.005,
+                                dt=0.005, rs=rs, theta=0.5)
+        plasma_io.write_xyz(d / "input.xyz", qel, qion)
+        r = subprocess.run([BIN], cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        _check_records(d)
+    second.mkdir()
+    plasma_io.write_control(second / "control.prm", ne, ni, tmax=0.025, trest=0.025, tdump=0.025, tsamp=0.005,
+                            dt=0.005, rs=rs, theta=0.5, rest="ON")
+    (second / "input.xyz").write_text((first / "res001.xyz").read_text())
+    r = subprocess.run([BIN], cwd=second, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _check_records(second)
+    a_e, a_i = plasma_io.read_res(full / "res002.xyz")
+    b_e, b_i = plasma_io.read_res(second / "res001.xyz")
+    assert np.array_equal(a_e["id"], b_e["id"]) and np.array_equal(a_i["id"], b_i["id"])
+    for a, b in ((a_e, b_e), (a_i, b_i)):
+        assert np.abs(a["xx"] - b["xx"]).max() <= 1e-6 * np.abs(a["xx"]).max()
+        assert np.abs(a["xv"] - b["xv"]).max() <= 1e-4 * np.abs(a["xv"]).max()
+    ea = plasma_io.read_ener(full / "ener001.dat")    # steps 6..10
+    eb = plasma_io.read_ener(second / "ener000.dat")  # the resumed run's steps 1..5
+    assert ea.shape == eb.shape == (5, 9)
+    for c in (1, 2, 7, 8):
+        assert np.allclose(ea[:, c], eb[:, c], rtol=1e-4, atol=0), (c, ea[:, c], eb[:, c])

@@ -243,7 +243,9 @@ def test_force_targets_and_sampled_timing():
         eng.force()
         e, i = eng.get_particles()
         f = eng.force_targets(tsp, tix)
-        ms = eng.time_walk(8)
+        ms, nw = eng.time_walk(8)
+        # blocks 0, 8, 16, ... of 64 targets: 10 of 79 electron blocks, 8 of 63 ion blocks (all whole)
+        assert nw == 10 * 64 + 8 * 64, nw
     fr = np.where(tsp[:, None] == 0, e["ff"][np.where(tsp == 0, tix, 0)], i["ff"][np.where(tsp == 1, tix, 0)])
     assert np.array_equal(f, fr)
     assert ms > 0

@@ -53,6 +53,8 @@ constexpr size_t NOALLOC_INT_BYTES = (size_t)TB_NOALLOC_MAX_TARGETS * 4;
 constexpr size_t NOALLOC_BYTES = 3 * NOALLOC_INT_BYTES + (size_t)TB_NOALLOC_MAX_TARGETS * 24;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
+extern "C" void tb_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);  // gen.cpp
+
 namespace {
 
 constexpr int MAX_DEVICES = 64;
@@ -668,10 +670,12 @@ int tb_set_particles(tb_handle *h, int32_t ne, const tb_pt *e, int32_t ni, const
         S.cur = 0;
         const int n = nn[sp];
         std::vector<double> col((size_t)n);
+        // forces are not taken from the caller: they start at zero (the
+        // learned criterion reads the previous force, and none exists yet)
         for (int k = 0; k < 9; k++) {
             for (int j = 0; j < n; j++) {
                 const tb_pt &p = src[sp][j];
-                col[j] = k < 3 ? p.xx[k] : (k < 6 ? p.xv[k - 3] : p.ff[k - 6]);
+                col[j] = k < 3 ? p.xx[k] : (k < 6 ? p.xv[k - 3] : 0.0);
             }
             double *dst = k < 3 ? S.x[0][k] : (k < 6 ? S.v[0][k - 3] : S.f[0][k - 6]);
             HIPCHK(hipMemcpy(dst, col.data(), (size_t)n * 8, hipMemcpyHostToDevice));
@@ -796,8 +800,12 @@ int tb_step(tb_handle *h, double dt, int32_t nsteps, double *out)
     for (int k = 0; k < nsteps; k++) {
         HIPCHK(hipEventRecord(h->ev[0], h->s));
         if ((rc = build_all(h))) return rc;
+        // the build synchronised: the walk flag of the previous step's force
+        // (copied after it) is on the host; stop before integrating past it
+        if (k > 0 && (rc = check_walk_flags(h))) return rc;
         HIPCHK(hipEventRecord(h->ev[1], h->s));
         if ((rc = force_all(h, false))) return rc;
+        HIPCHK(hipMemcpyAsync(h->h_sums + 15, h->d_sums + 15, sizeof(double), hipMemcpyDeviceToHost, h->s));
         HIPCHK(hipEventRecord(h->ev[3], h->s));
         const bool last = k == nsteps - 1;
         if ((rc = vv2_all(h, dt, last && out ? 1 : 0, !last))) return rc;
@@ -1021,13 +1029,20 @@ int tb_force_targets(tb_handle *h, int32_t nt, const int32_t *tspecies, const in
     return e == hipSuccess ? TB_OK : fail(TB_ERR_HIP, "tb_force_targets: %s", hipGetErrorString(e));
 }
 
-int tb_time_walk(tb_handle *h, int32_t stride, float *ms)
+int tb_time_walk(tb_handle *h, int32_t stride, float *ms, int64_t *nwalked)
 {
-    if (!h || !ms || stride < 1) return fail(TB_ERR_ARG, "bad argument");
+    if (!h || !ms || !nwalked || stride < 1) return fail(TB_ERR_ARG, "bad argument");
     if (!h->built || h->tree_walk != TB_WALK_REF64)
         return fail(TB_ERR_STATE, "tb_time_walk needs trees built for the FP64 walk");
     HIPCHK(hipSetDevice(h->cfg.device));
-    const int nt = (h->sp[0].n + stride - 1) / stride + (h->sp[1].n + stride - 1) / stride;
+    // slots: every stride-th block of 64 targets per species; the last block may be short
+    int nt = 0;
+    int64_t walked = 0;
+    for (int sp = 0; sp < 2; sp++) {
+        const int n = h->sp[sp].n, blocks = (n + 63) / 64, sampled = (blocks + stride - 1) / stride;
+        nt += sampled * 64;
+        for (int b = 0; b < blocks; b += stride) walked += std::min(64, n - 64 * b);
+    }
     int32_t *dummy = nullptr;
     int rc = dalloc_t(h, &dummy, 16);
     if (rc) return rc;
@@ -1037,6 +1052,7 @@ int tb_time_walk(tb_handle *h, int32_t stride, float *ms)
     HIPCHK(hipEventSynchronize(h->ev[3]));
     HIPCHK(hipEventElapsedTime(ms, h->ev[2], h->ev[3]));
     dfree(h, dummy, 64);
+    *nwalked = walked;
     return TB_OK;
 }
 
@@ -1080,6 +1096,36 @@ int tb_mlp_set(tb_handle *h, const float *params, int32_t n)
     if (tb_mlp_upload(h->s, h->mx.mlp, params)) return fail(TB_ERR_NOMEM, "MLP upload failed");
     HIPCHK(hipStreamSynchronize(h->s));
     return TB_OK;
+}
+
+int tb_mlp_init(tb_handle *h, uint64_t seed)
+{
+    if (!h) return fail(TB_ERR_ARG, "null handle");
+    // He-uniform weights (limit sqrt(6 / fan_in)), zero biases, from
+    // Philox4x32-10 keyed by the seed: counter (k, 0x4d4c5000, 0, 0) gives
+    // the uniforms of weights 4k .. 4k+3 in layout order
+    const int nin = TB_MLP_NIN, nh = 32;
+    std::vector<float> p(TB_MLP_NPARAMS, 0.0f);
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int k = 0;
+    uint32_t buf[4] = {0, 0, 0, 0};
+    auto uniform = [&](void) {
+        if ((k & 3) == 0) {
+            const uint32_t ctr[4] = {(uint32_t)(k >> 2), 0x4d4c5000u, 0u, 0u};
+            tb_philox4x32_10(ctr, key, buf);
+        }
+        const uint32_t b = buf[k & 3];
+        k++;
+        return (double)(b >> 8) * (1.0 / 16777216.0);  // [0, 1)
+    };
+    const int fan_in[3] = {nin, nh, nh}, fan_out[3] = {nh, nh, 1};
+    int o = 0;
+    for (int l = 0; l < 3; l++) {
+        const double lim = std::sqrt(6.0 / fan_in[l]);
+        for (int j = 0; j < fan_in[l] * fan_out[l]; j++) p[o + j] = (float)(lim * (2.0 * uniform() - 1.0));
+        o += fan_in[l] * fan_out[l] + fan_out[l];  // biases stay zero
+    }
+    return tb_mlp_set(h, p.data(), TB_MLP_NPARAMS);
 }
 
 int tb_mlp_get(tb_handle *h, float *params, int32_t n)
@@ -1263,6 +1309,9 @@ int ens_finish(tb_ensemble *E, double *out, bool sampled)
     for (int r = 0; r < n; r++) {
         HIPCHK(hipMemcpy(fl, E->h[r]->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
         if ((fl[0] | fl[1]) & 12) return fail(TB_ERR_HIP, "realisation %d: tree integrity check failed", r);
+        // the member's walk-record flag (k_mixed_nodes: a malformed joint table)
+        HIPCHK(hipMemcpy(fl, E->h[r]->mx.ctr + 2, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (fl[0]) return fail(TB_ERR_HIP, "realisation %d: malformed joint node table (flag %d)", r, fl[0]);
         const double *v = E->h_buf + 4 * r;
         tb_handle *h = E->h[r];
         epot_from_sums(v, h->epot);

@@ -56,6 +56,7 @@ struct MixedDev {
     // far cells of the last walk: chunk lists per target group (k_far_eval)
     uint32_t *far_cells = nullptr;
     int32_t *far_link = nullptr, *far_head = nullptr;
+    float *gemin = nullptr;       // per target group: the learned walk's emin (kept for a re-run)
     int far_cap = 0, head_cap = 0;
     // the learned opening criterion
     MlpDev mlp;

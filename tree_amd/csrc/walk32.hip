@@ -201,6 +201,8 @@ struct WalkArgs {
     uint32_t *far_cells;  // far cells, 64 per chunk (0xffffffff: none)
     int32_t *far_link;    // per chunk: the group's previous chunk, or -1
     int32_t *far_head;    // per global group: its last chunk, or -1
+    float *gemin;         // learned / collect: per global group, emin of the first attempt
+    int emin_saved;       // 1 on a re-run: emin comes from gemin (the forces were overwritten)
     int far_cap;          // chunks in far_cells
     const TbMlpPack *mlpk;  // learned: the bf16 parameters packed as matrix-core operands
     BandRec *rec;         // collect: band pairs
@@ -589,15 +591,21 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         // learned / collect: the group's weakest net field from the previous force
         float emin = 0.0f;
         if (MODE != MODE_FIXED) {
-            double e2 = 1e300;
+            if (W.emin_saved) {
+                emin = W.gemin[g];
+            } else {
+                double e2 = 1e300;
 #pragma unroll
-            for (int t = 0; t < TPL; t++) {
-                if (!act[t]) continue;
-                const int i = base + lane + 64 * t;
-                const double fx = D.fx[tsp][i], fy = D.fy[tsp][i], fz = D.fz[tsp][i];
-                e2 = fmin(e2, fx * fx + fy * fy + fz * fz);
+                for (int t = 0; t < TPL; t++) {
+                    if (!act[t]) continue;
+                    const int i = base + lane + 64 * t;
+                    const double fx = D.fx[tsp][i], fy = D.fy[tsp][i], fz = D.fz[tsp][i];
+                    e2 = fmin(e2, fx * fx + fy * fy + fz * fz);
+                }
+                emin = (float)(sqrt(wave_min(e2)) / TB_EPS);
+                // kept for a re-run of this walk, which finds the forces overwritten
+                W.gemin[g] = uni(emin);  // every lane stores the same value
             }
-            emin = (float)(sqrt(wave_min(e2)) / TB_EPS);
         }
         emin = uni(emin);
         const double gx = uni(0.5 * (mnx + mxx)), gy = uni(0.5 * (mny + mxy)), gz = uni(0.5 * (mnz + mxz));
@@ -1382,10 +1390,13 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     }
     if (ngt > M.head_cap) {
         if (M.far_head) hipFree(M.far_head);
+        if (M.gemin) hipFree(M.gemin);
         M.far_head = nullptr;
+        M.gemin = nullptr;
         M.head_cap = 0;
-        if (hipMalloc((void **)&M.far_head, (size_t)ngt * sizeof(int32_t)) != hipSuccess) {
-            g_err = "mixed walk: far-list heads allocation failed";
+        if (hipMalloc((void **)&M.far_head, (size_t)ngt * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc((void **)&M.gemin, (size_t)ngt * sizeof(float)) != hipSuccess) {
+            g_err = "mixed walk: per-group buffers allocation failed";
             return TB_ERR_NOMEM;
         }
         M.head_cap = ngt;
@@ -1408,6 +1419,8 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         W.far_link = M.far_link;
         W.far_head = M.far_head;
         W.far_cap = M.far_cap;
+        W.gemin = M.gemin;
+        W.emin_saved = attempt > 0 ? 1 : 0;
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 0, 2);
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_FAR, CTR_FAR + 1);
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_CHUNK, CTR_CHUNK + NXCD);
@@ -1631,6 +1644,8 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
     if (M.far_cells) hipFree(M.far_cells);
     if (M.far_link) hipFree(M.far_link);
     if (M.far_head) hipFree(M.far_head);
+    if (M.gemin) hipFree(M.gemin);
+    M.gemin = nullptr;
     M.far_cells = nullptr;
     M.far_link = M.far_head = nullptr;
     M.far_cap = M.head_cap = 0;

@@ -224,11 +224,15 @@ __global__ __launch_bounds__(BT) void k_walk64_list(int nt, const int32_t *__res
         tsp = tsp_list[t];
         j = idx[t];
     } else {
-        // strided sample over electrons then ions
-        const int ne = (n0 + stride - 1) / stride;
+        // every stride-th run of 64 consecutive key-ordered targets (one
+        // wavefront's worth), electrons then ions: each wavefront walks
+        // neighbouring targets, as in the full walk, so the sample's time per
+        // target is the full walk's
+        const int ne = ((n0 + 63) / 64 + stride - 1) / stride * 64;
         tsp = t < ne ? 0 : 1;
-        j = (tsp ? t - ne : t) * stride;
-        (void)n1;
+        const int k = tsp ? t - ne : t;
+        j = (k >> 6) * 64 * stride + (k & 63);
+        if (j >= (tsp ? n1 : n0)) return;
     }
     const double x = tsp ? ix[j] : ex[j], y = tsp ? iy[j] : ey[j], z = tsp ? iz[j] : ez[j];
     const double tq = tsp ? 1.0 : -1.0;

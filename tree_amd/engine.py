@@ -101,13 +101,14 @@ SIGNATURES = [
     ("tb_direct_sample_noalloc", ctypes.c_int, [_P, _I, _P, _P, _P]),
     ("tb_local_eval", ctypes.c_int, [_P, _I, _I, _P, _I, _P, _P]),
     ("tb_force_targets", ctypes.c_int, [_P, _I, _P, _P, _P]),
-    ("tb_time_walk", ctypes.c_int, [_P, _I, _P]),
+    ("tb_time_walk", ctypes.c_int, [_P, _I, _P, _P]),
     ("tb_count_interactions", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_get_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("tb_rdf_enable", ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _I]),
     ("tb_rdf_get", ctypes.c_int, [_P, _I, _P, _P, _I]),
     ("tb_mlp_set", ctypes.c_int, [_P, _P, _I]),
     ("tb_mlp_get", ctypes.c_int, [_P, _P, _I]),
+    ("tb_mlp_init", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("tb_mlp_forward", ctypes.c_int, [_P, _I, _P, _I, _P]),
     ("tb_mlp_loss_grad", ctypes.c_int, [_P, _I, _P, _P, _P, _P]),
     ("tb_mlp_train", ctypes.c_int, [_P, _I, _P, _P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
@@ -296,9 +297,11 @@ class Engine:
         return f
 
     def time_walk(self, stride):
+        """(ms, targets walked) of the FP64 walk over every stride-th block of 64 targets."""
         ms = ctypes.c_float()
-        _check(self.lib.tb_time_walk(self.h, stride, ctypes.byref(ms)))
-        return ms.value
+        nw = ctypes.c_int64()
+        _check(self.lib.tb_time_walk(self.h, stride, ctypes.byref(ms), ctypes.byref(nw)))
+        return ms.value, nw.value
 
     def count_interactions(self):
         s = Stats()
@@ -309,6 +312,10 @@ class Engine:
     def mlp_set(self, params):
         p = np.ascontiguousarray(params, np.float32)
         _check(self.lib.tb_mlp_set(self.h, _ptr(p), len(p)))
+
+    def mlp_init(self, seed=0):
+        """Seeded initial network (He-uniform weights, zero biases)."""
+        _check(self.lib.tb_mlp_init(self.h, seed))
 
     def mlp_get(self):
         p = np.zeros(MLP_NPARAMS, np.float32)

@@ -16,6 +16,12 @@ MODULE DATASTR
      INTEGER(C_INT32_T) :: id, pad
   END TYPE PT
 
+  ! position-only record (data.f90:54-56): what PTEgathering2 / PTIgathering2
+  ! hand to BuildLocalTree
+  TYPE GS
+     REAL(C_DOUBLE) :: xx(3)
+  END TYPE GS
+
   ! time control
   TYPE TM
      INTEGER :: Nloop, Nrest, Ndump, Nsamp

@@ -1,8 +1,8 @@
 ! main.f90 -- Fortran driver of the GPU engine for ultracold-plasma MD.
 ! Same program flow, input files and output files as the reference's
 ! PROGRAM OCTREE_HYBRID (main.f90:44-187) on one domain: control.prm and
-! input.xyz in, ener###.dat (TEMPORAL, print.f90:55-96) and res###.xyz
-! (Restart, sample.f90:5-109) out.  Each step is VVerletNotemp1, Force,
+! input.xyz in, ener###.dat (TEMPORAL, sample.f90:55-96) and res###.xyz
+! (Restart, print.f90:5-109) out.  Each step is VVerletNotemp1, Force,
 ! VVerletNotemp2; Force builds both trees and runs LocalCoulombSum on the GPU.
 ! The walk is the reference-order FP64 walk unless TREEMI_WALK=mixed.
 PROGRAM OCTREE_GPU
@@ -89,69 +89,94 @@ CONTAINS
     TYPE(ST), INTENT(INOUT) :: sys
     REAL(C_DOUBLE), INTENT(INOUT) :: dt
     CHARACTER(LEN=5) :: dummy
-    INTEGER :: i, nid
+    CHARACTER(LEN=256) :: line
+    INTEGER :: i, nid, ios, nhead
     REAL(C_DOUBLE) :: x, y, z, xv, yv, zv
-    OPEN(UNIT=15, FILE="control.prm", STATUS="OLD")
-    READ(15, *) dummy
-    READ(15, *) dummy
+    OPEN(UNIT=15, FILE="control.prm", STATUS="OLD", IOSTAT=ios)
+    IF (ios /= 0) STOP "control.prm: cannot open"
+    READ(15, *, IOSTAT=ios) dummy
+    READ(15, *, IOSTAT=ios) dummy
     DO i = 1, Nparam
-       READ(15, *) param%q(i)
+       IF (ios == 0) READ(15, *, IOSTAT=ios) param%q(i)
     END DO
-    READ(15, *) dummy
+    CALL prm_check(ios, "charges")
+    READ(15, *, IOSTAT=ios) dummy
     DO i = 1, Nparam
-       READ(15, *) param%xm(i)
+       IF (ios == 0) READ(15, *, IOSTAT=ios) param%xm(i)
     END DO
-    READ(15, *) dummy
-    READ(15, *) NS%NTel
-    READ(15, *) NS%NTion
+    CALL prm_check(ios, "masses")
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) NS%NTel
+    IF (ios == 0) READ(15, *, IOSTAT=ios) NS%NTion
+    CALL prm_check(ios, "particle numbers")
+    IF (NS%NTel < 1 .OR. NS%NTion < 1) STOP "control.prm: particle numbers must be positive"
     NS%NTpt = NS%NTel + NS%NTion
-    READ(15, *) dummy
-    READ(15, *) param%Te
-    READ(15, *) param%Ti
-    READ(15, *) dummy
-    READ(15, *) ttime%tmax, ttime%trest, ttime%tdump, ttime%tsamp, dt
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) param%Te
+    IF (ios == 0) READ(15, *, IOSTAT=ios) param%Ti
+    CALL prm_check(ios, "temperatures")
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) ttime%tmax, ttime%trest, ttime%tdump, ttime%tsamp, dt
+    CALL prm_check(ios, "time control")
+    IF (.NOT. (dt > 0.D0)) STOP "control.prm: the time step must be positive"
     ttime%tmax = ttime%tmax/tps
     ttime%trest = ttime%trest/tps
     ttime%tdump = ttime%tdump/tps
     ttime%tsamp = ttime%tsamp/tps
     dt = dt/tps
-    READ(15, *) dummy
-    READ(15, *) sys%kelbg, param%rs
-    READ(15, *) dummy
-    READ(15, *) param%rest
-    READ(15, *) dummy
-    READ(15, *) NS%dx_ee, NS%dx_ei, NS%dx_ii
-    READ(15, *) dummy
-    READ(15, *) param%sd_ratio
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) sys%kelbg, param%rs
+    CALL prm_check(ios, "Kelbg flag and length")
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) param%rest
+    CALL prm_check(ios, "restart flag")
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) NS%dx_ee, NS%dx_ei, NS%dx_ii
+    CALL prm_check(ios, "RDF bin widths")
+    READ(15, *, IOSTAT=ios) dummy
+    IF (ios == 0) READ(15, *, IOSTAT=ios) param%sd_ratio
+    CALL prm_check(ios, "opening angle")
     CLOSE(15)
 
     NS%Nel = NS%NTel
     NS%Nion = NS%NTion
     NS%Npt = NS%NTpt
     ALLOCATE(qel(NS%Nel), qion(NS%Nion))
-    OPEN(UNIT=11, FILE="input.xyz", STATUS="OLD")
-    READ(11, *) dummy
-    READ(11, *) dummy
+    OPEN(UNIT=11, FILE="input.xyz", STATUS="OLD", IOSTAT=ios)
+    IF (ios /= 0) STOP "input.xyz: cannot open"
+    ! the first line holds the particle count (as Restart writes it): it must
+    ! match control.prm, as the reference's count check does (main.f90:415-421)
+    READ(11, '(A)', IOSTAT=ios) line
+    IF (ios /= 0) STOP "input.xyz: empty file"
+    READ(line, *, IOSTAT=ios) nhead
+    IF (ios == 0 .AND. nhead /= NS%NTpt) THEN
+       PRINT *, "parsing error", nhead, "while given number is", NS%NTpt
+       STOP "input.xyz: particle count differs from control.prm"
+    END IF
+    READ(11, *, IOSTAT=ios) dummy
+    IF (ios /= 0) STOP "input.xyz: missing second header line"
     sys%mv2el = 0.D0
     sys%mv2ion = 0.D0
     DO i = 1, NS%Nel
        IF (param%rest == 'OFF') THEN
-          READ(11, *) dummy, x, y, z, nid
+          READ(11, *, IOSTAT=ios) dummy, x, y, z, nid
           xv = 0.D0; yv = 0.D0; zv = 0.D0
        ELSE
-          READ(11, *) dummy, x, y, z, xv, yv, zv, nid
+          READ(11, *, IOSTAT=ios) dummy, x, y, z, xv, yv, zv, nid
        END IF
+       CALL xyz_check(ios, i, NS%NTpt)
        qel(i)%xx = (/x, y, z/); qel(i)%xv = (/xv, yv, zv/); qel(i)%ff = 0.D0
        qel(i)%id = nid; qel(i)%pad = 0
        sys%mv2el = sys%mv2el + param%xm(1)*(xv**2 + yv**2 + zv**2)
     END DO
     DO i = 1, NS%Nion
        IF (param%rest == 'OFF') THEN
-          READ(11, *) dummy, x, y, z, nid
+          READ(11, *, IOSTAT=ios) dummy, x, y, z, nid
           xv = 0.D0; yv = 0.D0; zv = 0.D0
        ELSE
-          READ(11, *) dummy, x, y, z, xv, yv, zv, nid
+          READ(11, *, IOSTAT=ios) dummy, x, y, z, xv, yv, zv, nid
        END IF
+       CALL xyz_check(ios, NS%Nel + i, NS%NTpt)
        qion(i)%xx = (/x, y, z/); qion(i)%xv = (/xv, yv, zv/); qion(i)%ff = 0.D0
        qion(i)%id = nid; qion(i)%pad = 0
        sys%mv2ion = sys%mv2ion + param%xm(2)*(xv**2 + yv**2 + zv**2)
@@ -167,6 +192,25 @@ CONTAINS
     sys%kbm_i_cir = kboltz*DBLE(2*NS%NTion - Nref)
     IF (param%rest /= 'OFF') sys%temp = (sys%mv2el + sys%mv2ion)/sys%kbm
   END SUBROUTINE Init
+
+  ! a failed control.prm read STOPs with the field's name
+  SUBROUTINE prm_check(ios, what)
+    INTEGER, INTENT(IN) :: ios
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    IF (ios /= 0) THEN
+       PRINT *, "control.prm: cannot read ", what
+       STOP "control.prm: malformed"
+    END IF
+  END SUBROUTINE prm_check
+
+  ! a short or malformed input.xyz STOPs with the particle it stopped at
+  SUBROUTINE xyz_check(ios, k, n)
+    INTEGER, INTENT(IN) :: ios, k, n
+    IF (ios /= 0) THEN
+       PRINT *, "parsing error at particle", k, "of", n
+       STOP "input.xyz: short or malformed"
+    END IF
+  END SUBROUTINE xyz_check
 
   ! Maxwell-Boltzmann velocities at exactly Te / Ti (main.f90:709-790), from
   ! the engine's seeded generator, then the initial force
@@ -206,14 +250,16 @@ CONTAINS
     CALL Force(NS, qel, qion, param, ttime)
   END SUBROUTINE Vinit_REST
 
-  ! main.f90:506-541 on one domain: build both trees, then the local sum
+  ! main.f90:506-541 on one domain: gather, build both trees, then the local sum
   SUBROUTINE Force(NS, qel, qion, param, ttime)
     TYPE(NM), INTENT(INOUT) :: NS
     TYPE(PT), INTENT(INOUT) :: qel(NS%Nel), qion(NS%Nion)
     TYPE(PM), INTENT(IN) :: param
     TYPE(TM), INTENT(INOUT) :: ttime
     TYPE(TREE_REF) :: Roote, Rooti
-    CALL BuildLocalTree(NS, qel, qion, Roote, Rooti, ttime)
+    TYPE(GS), ALLOCATABLE, SAVE :: gel(:), gion(:)
+    CALL Engine_Gather(NS, gel, gion)
+    CALL BuildLocalTree(NS, gel, gion, Roote, Rooti, ttime)
     CALL LocalCoulombSum(NS, Roote, Rooti, qel, qion, sys, param, ttime)
   END SUBROUTINE Force
 
@@ -242,7 +288,7 @@ CONTAINS
 200 FORMAT(5(ES14.6, 1X))
   END SUBROUTINE RDF
 
-  ! print.f90:55-96 on one domain
+  ! sample.f90:55-96 on one domain
   SUBROUTINE TEMPORAL(NS, sys, ttime)
     TYPE(NM), INTENT(IN) :: NS
     TYPE(ST), INTENT(INOUT) :: sys
@@ -262,7 +308,7 @@ CONTAINS
 500 FORMAT(9(ES14.6, 1X))
   END SUBROUTINE TEMPORAL
 
-  ! sample.f90:5-109 on one domain: res###.xyz, P = electron, O = ion
+  ! print.f90:5-109 on one domain: res###.xyz, P = electron, O = ion
   SUBROUTINE Restart(NS, qel, qion, ttime)
     TYPE(NM), INTENT(IN) :: NS
     TYPE(PT), INTENT(IN) :: qel(NS%Nel), qion(NS%Nion)

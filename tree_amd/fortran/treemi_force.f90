@@ -67,15 +67,39 @@ CONTAINS
     CALL tb_check(tb_rdf_get(hnd, INT(Ndist + 1, C_INT32_T), rdf_ei, rdf_ii, 1_C_INT32_T), "Engine_RDF_Take")
   END SUBROUTINE Engine_RDF_Take
 
+  ! PTEgathering2 / PTIgathering2 (main.f90:430-502) on one domain: every
+  ! particle is local, so the totals NS%NTel / NS%NTion are the local counts;
+  ! the position copies are sized for BuildLocalTree's argument list but not
+  ! filled (the positions live on the GPU)
+  SUBROUTINE Engine_Gather(NS, gel, gion)
+    TYPE(NM), INTENT(INOUT) :: NS
+    TYPE(GS), ALLOCATABLE, INTENT(INOUT) :: gel(:), gion(:)
+    NS%NTel = NS%Nel
+    NS%NTion = NS%Nion
+    IF (ALLOCATED(gel)) THEN
+       IF (SIZE(gel) /= NS%NTel) DEALLOCATE(gel)
+    END IF
+    IF (ALLOCATED(gion)) THEN
+       IF (SIZE(gion) /= NS%NTion) DEALLOCATE(gion)
+    END IF
+    IF (.NOT. ALLOCATED(gel)) ALLOCATE(gel(NS%NTel))
+    IF (.NOT. ALLOCATED(gion)) ALLOCATE(gion(NS%NTion))
+  END SUBROUTINE Engine_Gather
+
   SUBROUTINE Engine_Done()
     INTEGER(C_INT) :: rc
     rc = tb_destroy(hnd)
     hnd = C_NULL_PTR
   END SUBROUTINE Engine_Done
 
+  ! The reference's argument list (tree.f90:39-43): gel / gion are the
+  ! position copies PTEgathering2 / PTIgathering2 make.  Here the positions
+  ! are already resident on the GPU (the last VVerletNotemp1 moved them
+  ! there), so the trees are built from the device state and gel / gion are
+  ! not read.
   SUBROUTINE BuildLocalTree(NS, gel, gion, Roote, Rooti, ttime)
     TYPE(NM), INTENT(IN) :: NS
-    TYPE(PT), INTENT(IN) :: gel(NS%Nel), gion(NS%Nion)   ! positions already resident on the GPU
+    TYPE(GS), INTENT(IN) :: gel(NS%NTel), gion(NS%NTion)
     TYPE(TREE_REF), INTENT(OUT) :: Roote, Rooti
     TYPE(TM), INTENT(IN) :: ttime
     CALL tb_check(tb_build(hnd), "BuildLocalTree")
