@@ -333,9 +333,22 @@ constexpr int MISC_J = MISC_HIST + 2 * 2 * HW;  // joint tree: err, total, 2 x H
 constexpr int MISC_G = MISC_J + 2 + 2 * HW;      // per species: segments, target groups
 constexpr int MISC_N = MISC_G + 4;               // ints read back after phase A
 
+// A batched ensemble builds all its realisations as one combined state:
+// realisation r holds slice [off[k][r], off[k][r+1]) of species k's arrays,
+// keys carry r as their first T octal digits (tb_ekeys), each realisation's
+// boxes are its own (d_roots: electrons, ions, joint, [nreal][6] each), and
+// its cells form the subtree at depth T of one joint table.
+struct EnsBuild {
+    int nreal = 0, T = 0, chunks = 1;
+    int32_t *d_off[2] = {nullptr, nullptr};
+    double *d_roots = nullptr;   // [3][nreal][6]
+    double *d_part = nullptr;    // [2][nreal][chunks][6] box partials
+    int32_t *d_gfirst = nullptr; // [2][nreal] first target group of each realisation
+};
+
 // Phase A.  tpl = walk-group size / 64 for the mixed walk's target groups
 // (0: chosen from the configuration and the autotuner's table).
-int build_phase_a(tb_handle *h, int *which, int tpl = 0)
+int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nullptr)
 {
     hipStream_t s = h->s;
     int rc;
@@ -344,16 +357,28 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0)
     if ((rc = ensure_tmp(h, mixed ? nj : std::max(h->sp[0].n, h->sp[1].n)))) return rc;
     if (mixed && (rc = ensure_joint(h, nj))) return rc;
     HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * MISC_N, s));
+    const size_t pstride = eb ? (size_t)eb->nreal * eb->chunks * 6 : 0;
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
         const int c = S.cur;
-        tb_bbox(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, h->bbox_tmp);
-        tb_keys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
+        if (eb) {
+            double *roots = eb->d_roots + (size_t)sp * eb->nreal * 6;
+            tb_sbbox(s, eb->nreal, eb->chunks, eb->d_off[sp], S.x[c][0], S.x[c][1], S.x[c][2],
+                     eb->d_part + sp * pstride);
+            tb_sbbox_final(s, eb->nreal, eb->chunks, eb->d_part + sp * pstride, nullptr, roots);
+            tb_ekeys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], eb->d_off[sp], eb->nreal, roots, eb->T, S.key[0],
+                     S.val[0], 0u, h->d_misc + sp);
+        } else {
+            tb_bbox(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, h->bbox_tmp);
+            tb_keys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
+        }
         tb_radix_sort(s, S.n, S.key, S.val, 0, 64, h->sort_tmp, &which[sp], false);
         tb_permute(s, S, S.val[which[sp]], h->d_misc + sp);
         S.skey = S.key[which[sp]];
-        if (mixed) tb_groups(s, S, 64 * (tpl ? tpl : tb_mixed_tpl(h->cfg, nj)), h->d_misc + MISC_G + 2 * sp,
-                             h->scan_tmp);
+        if (mixed)
+            tb_groups(s, S, 64 * (tpl ? tpl : tb_mixed_tpl(h->cfg, eb ? nj / eb->nreal : nj)),
+                      h->d_misc + MISC_G + 2 * sp, h->scan_tmp, eb ? eb->T : 0,
+                      eb ? eb->d_gfirst + (size_t)sp * eb->nreal : nullptr);
         if (!mixed) {
             tb_lcp(s, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
             tb_scan_excl(s, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, h->scan_tmp);
@@ -364,11 +389,22 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0)
         JointDev &J = h->jt;
         SpeciesDev &E = h->sp[0], &I = h->sp[1];
         const int ce = E.cur, ci = I.cur;
-        tb_jbbox(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root,
-                 h->bbox_tmp);
-        tb_jkeys(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], J.root, J.key[0], J.val[0], 0u, h->d_misc + MISC_J);
-        tb_jkeys(s, I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root, J.key[0] + E.n, J.val[0] + E.n, 0x80000000u,
-                 h->d_misc + MISC_J);
+        if (eb) {
+            // each realisation's joint box from both species' partial boxes
+            double *roots = eb->d_roots + (size_t)2 * eb->nreal * 6;
+            tb_sbbox_final(s, eb->nreal, eb->chunks, eb->d_part, eb->d_part + pstride, roots);
+            tb_ekeys(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], eb->d_off[0], eb->nreal, roots, eb->T, J.key[0],
+                     J.val[0], 0u, h->d_misc + MISC_J);
+            tb_ekeys(s, I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], eb->d_off[1], eb->nreal, roots, eb->T,
+                     J.key[0] + E.n, J.val[0] + E.n, 0x80000000u, h->d_misc + MISC_J);
+        } else {
+            tb_jbbox(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root,
+                     h->bbox_tmp);
+            tb_jkeys(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], J.root, J.key[0], J.val[0], 0u,
+                     h->d_misc + MISC_J);
+            tb_jkeys(s, I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root, J.key[0] + E.n, J.val[0] + E.n,
+                     0x80000000u, h->d_misc + MISC_J);
+        }
         tb_radix_sort(s, nj, J.key, J.val, 0, 64, h->sort_tmp, &which[2], false);
         const double *px[2] = {E.x[ce][0], I.x[ci][0]}, *py[2] = {E.x[ce][1], I.x[ci][1]},
                      *pz[2] = {E.x[ce][2], I.x[ci][2]};
@@ -382,7 +418,7 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0)
 }
 
 // after the stream has been synchronised
-int build_phase_b(tb_handle *h, const int *which)
+int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
 {
     hipStream_t s = h->s;
     int rc;
@@ -421,7 +457,15 @@ int build_phase_b(tb_handle *h, const int *which)
         int w2;
         tb_radix_sort(s, J.nn, J.dkey, J.dval, 0, 8, h->sort_tmp, &w2, true);
         J.bfs = J.dval[w2];
-        tb_jupward(s, J, key, h->d_misc);
+        if (eb) {
+            // realisation r's root cell is the r-th cell at depth T (the tag digits)
+            if (J.lvl_off[eb->T + 1] - J.lvl_off[eb->T] != eb->nreal)
+                return fail(TB_ERR_HIP, "ensemble tree: %d cells at depth %d for %d realisations",
+                            J.lvl_off[eb->T + 1] - J.lvl_off[eb->T], eb->T, eb->nreal);
+            tb_jupward(s, J, key, h->d_misc, eb->T, eb->d_roots + (size_t)2 * eb->nreal * 6);
+        } else {
+            tb_jupward(s, J, key, h->d_misc);
+        }
         h->st.nodes[0] = J.nn;
         h->st.nodes[1] = 0;
         h->st.max_depth[0] = J.maxdepth;
@@ -1233,95 +1277,151 @@ int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out)
 
 }  // extern "C"
 
+// The batched ensemble: every realisation in one combined state (one
+// tb_handle C), built by one pipeline per step (EnsBuild: keys tagged by
+// realisation, boxes per realisation) and walked by one persistent launch
+// whose domains are the realisations' subtrees.
 struct tb_ensemble {
-    std::vector<tb_handle *> h;
-    hipStream_t s = nullptr;
-    tb_config cfg{};
-    MixedDev mx;                 // shared launch state: descriptors, counters, energy sums
-    double *d_esum = nullptr;    // 4 per realisation
-    double *h_buf = nullptr;     // pinned: 4 energies + 4 kinetic sums per realisation
+    int nreal = 0, T = 0;
+    tb_handle *C = nullptr;
+    std::vector<double> rs;                 // Kelbg length per realisation
+    std::vector<std::vector<tb_pt>> pe, pi; // the realisations' particles as last set (host)
+    std::vector<int32_t> off[2];            // slices of the combined arrays [nreal + 1]
+    bool dirty = true;                      // set particles not yet on the device
     bool have_force = false;
-    size_t bytes = 0;
+    EnsBuild eb;
+    int32_t *h_gfirst = nullptr;            // pinned [2][nreal]
+    std::vector<int32_t> gfirst[2], gcount[2];
+    std::vector<uint32_t> roots;            // root cell of each realisation
+    double *d_esum = nullptr;               // [nreal][4]
+    double *d_ke = nullptr;                 // [2][nreal][2] mv2, radial sum
+    double *d_kpart = nullptr;              // [nreal][chunks][2]
+    double *h_buf = nullptr;                // pinned: 4 energies + 4 kinetic sums per realisation
 };
 
 namespace {
 
-int ens_tpl(const tb_ensemble *E)
+constexpr int ENS_MAX_CHUNKS = 64;
+
+// group ranges (first, count) of every realisation after its groups were cut
+void ens_groups_from_host(tb_ensemble *E)
 {
-    int ntot = 0;
-    for (auto *h : E->h) ntot += h->sp[0].n + h->sp[1].n;
-    return tb_mixed_tpl(E->cfg, ntot / (int)E->h.size());
+    for (int k = 0; k < 2; k++) {
+        E->gfirst[k].assign(E->h_gfirst + (size_t)k * E->nreal, E->h_gfirst + (size_t)(k + 1) * E->nreal);
+        E->gcount[k].resize(E->nreal);
+        for (int r = 0; r < E->nreal; r++) {
+            const int end = r + 1 < E->nreal ? E->gfirst[k][r + 1] : E->C->sp[k].ng;
+            E->gcount[k][r] = end - E->gfirst[k][r];
+        }
+    }
 }
 
-int ens_regroup(tb_ensemble *E, int tpl)
+int ens_upload(tb_ensemble *E)
 {
-    int rc;
-    for (auto *h : E->h)
-        if ((rc = regroup(h, tpl))) return rc;
+    std::vector<tb_pt> all[2];
+    int nmax = 1;
+    for (int k = 0; k < 2; k++) {
+        E->off[k].assign(E->nreal + 1, 0);
+        for (int r = 0; r < E->nreal; r++) {
+            const std::vector<tb_pt> &v = k == 0 ? E->pe[r] : E->pi[r];
+            if (v.empty()) return fail(TB_ERR_STATE, "realisation %d has no particles", r);
+            E->off[k][r + 1] = E->off[k][r] + (int32_t)v.size();
+            nmax = std::max(nmax, (int)v.size());
+            all[k].insert(all[k].end(), v.begin(), v.end());
+        }
+    }
+    int rc = tb_set_particles(E->C, E->off[0][E->nreal], all[0].data(), E->off[1][E->nreal], all[1].data());
+    if (rc) return rc;
+    for (int k = 0; k < 2; k++)
+        HIPCHK(hipMemcpy(E->eb.d_off[k], E->off[k].data(), (E->nreal + 1) * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    E->eb.chunks = std::min(ENS_MAX_CHUNKS, tb_sbbox_chunks(nmax));
+    E->dirty = false;
     return TB_OK;
 }
 
-int ens_build_all(tb_ensemble *E)
+// one build of every realisation: keys, sorts, target groups and the joint
+// table, with one host read-back (node totals, group starts)
+int ens_build(tb_ensemble *E)
 {
-    const int n = (int)E->h.size();
-    std::vector<int> which(3 * (size_t)n);
-    int rc;
-    // one walk launch serves every realisation, so all are grouped for one size
-    const int tpl = ens_tpl(E);
-    for (int r = 0; r < n; r++)
-        if ((rc = build_phase_a(E->h[r], &which[3 * r], tpl))) return rc;
-    HIPCHK(hipStreamSynchronize(E->s));
-    for (int r = 0; r < n; r++)
-        if ((rc = build_phase_b(E->h[r], &which[3 * r]))) return rc;
+    tb_handle *C = E->C;
+    int which[3], rc;
+    if ((rc = build_phase_a(C, which, 0, &E->eb))) return rc;
+    HIPCHK(hipMemcpyAsync(E->h_gfirst, E->eb.d_gfirst, 2 * E->nreal * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          C->s));
+    HIPCHK(hipStreamSynchronize(C->s));
+    if ((rc = build_phase_b(C, which, &E->eb))) return rc;
+    if ((rc = build_check(C))) return rc;
+    ens_groups_from_host(E);
+    E->roots.resize(E->nreal);
+    for (int r = 0; r < E->nreal; r++) E->roots[r] = (uint32_t)(C->jt.lvl_off[E->T] + r);
     return TB_OK;
 }
 
 int ens_force(tb_ensemble *E)
 {
-    const int n = (int)E->h.size();
-    std::vector<SpeciesDev *> sp(2 * (size_t)n);
-    std::vector<const MixedTree *> tr((size_t)n);
-    std::vector<double> rs((size_t)n);
-    for (int r = 0; r < n; r++) {
-        for (int k = 0; k < 2; k++) sp[2 * r + k] = &E->h[r]->sp[k];
-        tr[r] = &E->h[r]->mx.t[0];
-        rs[r] = E->h[r]->cfg.rs;  // each realisation's own Kelbg length
-    }
-    int rc = tb_mixed_force_multi(E->s, E->mx, sp.data(), tr.data(), rs.data(), n, E->cfg, E->d_esum);
+    const int32_t *gf[2] = {E->gfirst[0].data(), E->gfirst[1].data()};
+    const int32_t *gc[2] = {E->gcount[0].data(), E->gcount[1].data()};
+    int rc = tb_mixed_force_ens(E->C->s, E->C->mx, E->C->sp, E->nreal, E->roots.data(), E->rs.data(), gf, gc,
+                                E->C->cfg, E->d_esum);
     if (rc) return fail(rc, "%s", tb_mixed_error());
-    for (int r = 0; r < n; r++) E->h[r]->have_force = true;
+    E->C->have_force = true;
     return TB_OK;
 }
 
-// energies (and kinetic sums when sampled) of every realisation back to the host
+// cuts the target groups again for walk-group size 64 * tpl (synchronises)
+int ens_regroup(tb_ensemble *E, int tpl)
+{
+    tb_handle *C = E->C;
+    for (int k = 0; k < 2; k++)
+        tb_groups(C->s, C->sp[k], 64 * tpl, C->d_misc + MISC_G + 2 * k, C->scan_tmp, E->T,
+                  E->eb.d_gfirst + (size_t)k * E->nreal);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(C->h_misc + MISC_G, C->d_misc + MISC_G, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+    HIPCHK(hipMemcpyAsync(E->h_gfirst, E->eb.d_gfirst, 2 * E->nreal * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          C->s));
+    HIPCHK(hipStreamSynchronize(C->s));
+    for (int k = 0; k < 2; k++) C->sp[k].ng = C->h_misc[MISC_G + 2 * k + 1];
+    ens_groups_from_host(E);
+    return TB_OK;
+}
+
+// energies (and, sampled, the kinetic sums of the current state) of every
+// realisation back to the host; the walk's and the build's integrity flags
 int ens_finish(tb_ensemble *E, double *out, bool sampled)
 {
-    const int n = (int)E->h.size();
-    HIPCHK(hipMemcpyAsync(E->h_buf, E->d_esum, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, E->s));
+    tb_handle *C = E->C;
+    const int n = E->nreal;
     if (sampled)
-        for (int r = 0; r < n; r++)
-            HIPCHK(hipMemcpyAsync(E->h_buf + 4 * n + 4 * r, E->h[r]->d_sums + 4, 4 * sizeof(double),
-                                  hipMemcpyDeviceToHost, E->s));
-    HIPCHK(hipStreamSynchronize(E->s));
-    int32_t fl[2];
-    HIPCHK(hipMemcpy(fl, E->mx.ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (fl[1]) return fail(TB_ERR_HIP, "ensemble walk: traversal stack overflow");
+        for (int k = 0; k < 2; k++)
+            tb_ke_seg(C->s, C->sp[k], C->cfg.xm[k], n, E->eb.chunks, E->eb.d_off[k], E->d_kpart,
+                      E->d_ke + (size_t)k * n * 2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(E->h_buf, E->d_esum, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, C->s));
+    if (sampled)
+        HIPCHK(hipMemcpyAsync(E->h_buf + 4 * n, E->d_ke, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost,
+                              C->s));
+    HIPCHK(hipStreamSynchronize(C->s));
+    int32_t fl[3];
+    HIPCHK(hipMemcpy(fl, C->mx.ctr, 3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (fl[1] || fl[2])
+        return fail(TB_ERR_HIP, "ensemble walk: traversal stack overflow or malformed node table (flags %d %d)",
+                    fl[1], fl[2]);
+    HIPCHK(hipMemcpy(fl, C->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if ((fl[0] | fl[1]) & 12) return fail(TB_ERR_HIP, "ensemble: tree integrity check failed");
     for (int r = 0; r < n; r++) {
-        HIPCHK(hipMemcpy(fl, E->h[r]->d_misc, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
-        if ((fl[0] | fl[1]) & 12) return fail(TB_ERR_HIP, "realisation %d: tree integrity check failed", r);
-        // the member's walk-record flag (k_mixed_nodes: a malformed joint table)
-        HIPCHK(hipMemcpy(fl, E->h[r]->mx.ctr + 2, sizeof(int32_t), hipMemcpyDeviceToHost));
-        if (fl[0]) return fail(TB_ERR_HIP, "realisation %d: malformed joint node table (flag %d)", r, fl[0]);
-        const double *v = E->h_buf + 4 * r;
-        tb_handle *h = E->h[r];
-        epot_from_sums(v, h->epot);
+        double ep[2];
+        epot_from_sums(E->h_buf + 4 * r, ep);
         if (out) {
             double *o = out + 6 * r;
-            o[0] = h->epot[0];
-            o[1] = h->epot[1];
+            o[0] = ep[0];
+            o[1] = ep[1];
             if (sampled) {
-                const double *k = E->h_buf + 4 * n + 4 * r;
-                o[2] = k[0]; o[3] = k[2]; o[4] = k[1]; o[5] = k[3];
+                const double *ke = E->h_buf + 4 * n;
+                o[2] = ke[2 * r];            // mv2el
+                o[3] = ke[2 * n + 2 * r];    // mv2ion
+                o[4] = ke[2 * r + 1];        // Te_rad
+                o[5] = ke[2 * n + 2 * r + 1];// Ti_rad
             } else {
                 o[2] = o[3] = o[4] = o[5] = 0.0;
             }
@@ -1338,39 +1438,50 @@ int tb_ensemble_create(tb_ensemble **out, int32_t n, const tb_config *cfgs)
 {
     if (!out || !cfgs || n < 1) return fail(TB_ERR_ARG, "bad argument");
     *out = nullptr;
+    int T = 0;
+    while ((1L << (3 * T)) < n) T++;
+    if (T > 4) return fail(TB_ERR_ARG, "at most 4096 realisations (got %d)", n);
+    const tb_config &c0 = cfgs[0];
     for (int r = 0; r < n; r++) {
-        int rc = check_cfg(&cfgs[r]);
+        const tb_config &c = cfgs[r];
+        int rc = check_cfg(&c);
         if (rc) return rc;
-        if (cfgs[r].walk != TB_WALK_MIXED) return fail(TB_ERR_ARG, "ensemble realisations use the mixed walk");
-        if (cfgs[r].device != cfgs[0].device) return fail(TB_ERR_ARG, "ensemble realisations share one device");
+        if (c.walk != TB_WALK_MIXED) return fail(TB_ERR_ARG, "ensemble realisations use the mixed walk");
+        if (c.device != c0.device) return fail(TB_ERR_ARG, "ensemble realisations share one device");
+        // one walk launch over one combined table: these are shared, only rs may differ
+        if (c.xm[0] != c0.xm[0] || c.xm[1] != c0.xm[1] || c.theta != c0.theta || c.theta_mixed != c0.theta_mixed ||
+            c.group != c0.group || c.far_ratio != c0.far_ratio)
+            return fail(TB_ERR_ARG, "realisation %d: masses, theta, group and far_ratio must equal realisation 0's", r);
     }
     tb_ensemble *E = new tb_ensemble();
-    E->cfg = cfgs[0];
-    E->cfg.mac = TB_MAC_FIXED;
-    for (int r = 0; r < n; r++) {
-        tb_handle *h = nullptr;
-        int rc = tb_create(&h, &cfgs[r]);
-        if (rc) {
-            std::string msg = g_err;
-            tb_ensemble_destroy(E);
-            return fail(rc, "%s", msg.c_str());
-        }
-        E->h.push_back(h);
+    E->nreal = n;
+    E->T = T;
+    tb_config cfg = c0;
+    cfg.mac = TB_MAC_FIXED;
+    int rc = tb_create(&E->C, &cfg);
+    if (rc) {
+        delete E;
+        return rc;
     }
-    // one stream for the whole ensemble
-    E->s = E->h[0]->s;
-    for (int r = 1; r < n; r++) {
-        hipStreamDestroy(E->h[r]->s);
-        E->h[r]->s = E->s;
-    }
-    bool ok = hipMalloc((void **)&E->mx.ctr, 64) == hipSuccess &&
-              hipMalloc((void **)&E->d_esum, (size_t)4 * n * sizeof(double)) == hipSuccess &&
-              hipHostMalloc((void **)&E->h_buf, (size_t)8 * n * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    E->rs.resize(n);
+    for (int r = 0; r < n; r++) E->rs[r] = cfgs[r].rs;
+    E->pe.resize(n);
+    E->pi.resize(n);
+    E->eb.nreal = n;
+    E->eb.T = T;
+    tb_handle *C = E->C;
+    bool ok = dalloc_t(C, &E->eb.d_off[0], n + 1) == TB_OK && dalloc_t(C, &E->eb.d_off[1], n + 1) == TB_OK &&
+              dalloc_t(C, &E->eb.d_roots, (size_t)3 * n * 6) == TB_OK &&
+              dalloc_t(C, &E->eb.d_part, (size_t)2 * n * ENS_MAX_CHUNKS * 6) == TB_OK &&
+              dalloc_t(C, &E->eb.d_gfirst, (size_t)2 * n) == TB_OK && dalloc_t(C, &E->d_esum, (size_t)4 * n) == TB_OK &&
+              dalloc_t(C, &E->d_ke, (size_t)4 * n) == TB_OK &&
+              dalloc_t(C, &E->d_kpart, (size_t)2 * n * ENS_MAX_CHUNKS) == TB_OK;
+    ok = ok && hipHostMalloc((void **)&E->h_gfirst, (size_t)2 * n * sizeof(int32_t), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&E->h_buf, (size_t)8 * n * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
         tb_ensemble_destroy(E);
         return fail(TB_ERR_NOMEM, "ensemble allocation failed");
     }
-    hipMemset(E->mx.ctr, 0, 64);
     *out = E;
     return TB_OK;
 }
@@ -1378,45 +1489,89 @@ int tb_ensemble_create(tb_ensemble **out, int32_t n, const tb_config *cfgs)
 int tb_ensemble_destroy(tb_ensemble *E)
 {
     if (!E) return TB_OK;
-    if (E->s) hipStreamSynchronize(E->s);
-    // every member shares member 0's stream: only that one is destroyed with its handle
-    for (size_t r = 1; r < E->h.size(); r++) E->h[r]->s = nullptr;
-    for (auto *h : E->h) tb_destroy(h);
-    size_t b = 0;
-    tb_mixed_free(E->mx, &b);
-    if (E->d_esum) hipFree(E->d_esum);
+    if (E->h_gfirst) hipHostFree(E->h_gfirst);
     if (E->h_buf) hipHostFree(E->h_buf);
+    if (E->C) {
+        hipSetDevice(E->C->cfg.device);
+        hipStreamSynchronize(E->C->s);
+        void *dev[8] = {E->eb.d_off[0], E->eb.d_off[1], E->eb.d_roots, E->eb.d_part, E->eb.d_gfirst, E->d_esum,
+                        E->d_ke, E->d_kpart};
+        for (void *p : dev) dfree(E->C, p, 0);
+        tb_destroy(E->C);
+    }
     delete E;
     return TB_OK;
 }
 
 int tb_ensemble_set_particles(tb_ensemble *E, int32_t r, int32_t ne, const tb_pt *e, int32_t ni, const tb_pt *i)
 {
-    if (!E || r < 0 || r >= (int)E->h.size()) return fail(TB_ERR_ARG, "bad realisation index");
+    if (!E || r < 0 || r >= E->nreal) return fail(TB_ERR_ARG, "bad realisation index");
+    if (!e || !i) return fail(TB_ERR_ARG, "null argument");
+    if (ne < 1 || ni < 1) return fail(TB_ERR_ARG, "need at least one particle per species (ne=%d ni=%d)", ne, ni);
+    E->pe[r].assign(e, e + ne);
+    E->pi[r].assign(i, i + ni);
+    for (auto &p : E->pe[r]) p.ff[0] = p.ff[1] = p.ff[2] = 0.0;
+    for (auto &p : E->pi[r]) p.ff[0] = p.ff[1] = p.ff[2] = 0.0;
+    E->dirty = true;
     E->have_force = false;
-    return tb_set_particles(E->h[r], ne, e, ni, i);
+    return TB_OK;
 }
 
 int tb_ensemble_get_particles(tb_ensemble *E, int32_t r, tb_pt *e, tb_pt *i)
 {
-    if (!E || r < 0 || r >= (int)E->h.size()) return fail(TB_ERR_ARG, "bad realisation index");
-    return tb_get_particles(E->h[r], e, i);
+    if (!E || r < 0 || r >= E->nreal) return fail(TB_ERR_ARG, "bad realisation index");
+    if (!e || !i) return fail(TB_ERR_ARG, "null argument");
+    if (E->dirty) {
+        std::copy(E->pe[r].begin(), E->pe[r].end(), e);
+        std::copy(E->pi[r].begin(), E->pi[r].end(), i);
+        return TB_OK;
+    }
+    tb_handle *C = E->C;
+    HIPCHK(hipSetDevice(C->cfg.device));
+    HIPCHK(hipStreamSynchronize(C->s));
+    tb_pt *dst[2] = {e, i};
+    for (int k = 0; k < 2; k++) {
+        // realisation r's slice: its caller-order slice, and (keys tagged by
+        // r first) its slice of the key order
+        SpeciesDev &S = C->sp[k];
+        const int b = E->off[k][r], n = E->off[k][r + 1] - b, c = S.cur;
+        std::vector<int32_t> orig((size_t)n);
+        HIPCHK(hipMemcpy(orig.data(), S.orig[c] + b, (size_t)n * 4, hipMemcpyDeviceToHost));
+        std::vector<double> col((size_t)n);
+        for (int q = 0; q < 9; q++) {
+            const double *src = q < 3 ? S.x[c][q] : (q < 6 ? S.v[c][q - 3] : S.f[c][q - 6]);
+            HIPCHK(hipMemcpy(col.data(), src + b, (size_t)n * 8, hipMemcpyDeviceToHost));
+            for (int j = 0; j < n; j++) {
+                const int at = orig[j] - b;
+                if (at < 0 || at >= n) return fail(TB_ERR_HIP, "ensemble: realisation %d slice is not its own", r);
+                tb_pt &p = dst[k][at];
+                if (q < 3) p.xx[q] = col[j];
+                else if (q < 6) p.xv[q - 3] = col[j];
+                else p.ff[q - 6] = col[j];
+            }
+        }
+        const std::vector<tb_pt> &src = k == 0 ? E->pe[r] : E->pi[r];
+        for (int j = 0; j < n; j++) {
+            dst[k][j].id = src[j].id;
+            dst[k][j].pad_ = 0;
+        }
+    }
+    return TB_OK;
 }
 
 int tb_ensemble_force(tb_ensemble *E, double *out)
 {
     if (!E) return fail(TB_ERR_ARG, "null ensemble");
-    for (auto *h : E->h)
-        if (h->sp[0].n == 0) return fail(TB_ERR_STATE, "a realisation has no particles");
-    HIPCHK(hipSetDevice(E->cfg.device));
+    HIPCHK(hipSetDevice(E->C->cfg.device));
     int rc;
-    if ((rc = ens_build_all(E))) return rc;
+    if (E->dirty && (rc = ens_upload(E))) return rc;
+    if ((rc = ens_build(E))) return rc;
     if ((rc = ens_force(E))) return rc;
     E->have_force = true;
-    std::vector<double> o(6 * E->h.size());
+    std::vector<double> o(6 * (size_t)E->nreal);
     if ((rc = ens_finish(E, o.data(), false))) return rc;
     if (out)
-        for (size_t r = 0; r < E->h.size(); r++) {
+        for (int r = 0; r < E->nreal; r++) {
             out[2 * r] = o[6 * r];
             out[2 * r + 1] = o[6 * r + 1];
         }
@@ -1426,51 +1581,42 @@ int tb_ensemble_force(tb_ensemble *E, double *out)
 int tb_ensemble_step(tb_ensemble *E, double dt, int32_t nsteps, double *out)
 {
     if (!E || nsteps < 1) return fail(TB_ERR_ARG, "bad argument");
-    if (!E->have_force) return fail(TB_ERR_STATE, "tb_ensemble_step needs forces: call tb_ensemble_force first");
-    HIPCHK(hipSetDevice(E->cfg.device));
+    if (!E->have_force || E->dirty)
+        return fail(TB_ERR_STATE, "tb_ensemble_step needs forces: call tb_ensemble_force first");
+    tb_handle *C = E->C;
+    HIPCHK(hipSetDevice(C->cfg.device));
     int rc;
-    for (auto *h : E->h)
-        for (int sp = 0; sp < 2; sp++) tb_vv1(E->s, h->sp[sp], h->cfg.xm[sp], dt);
+    for (int k = 0; k < 2; k++) tb_vv1(C->s, C->sp[k], C->cfg.xm[k], dt);
     for (int k = 0; k < nsteps; k++) {
-        if ((rc = ens_build_all(E))) return rc;
+        if ((rc = ens_build(E))) return rc;
         if ((rc = ens_force(E))) return rc;
         const bool last = k == nsteps - 1;
-        for (auto *h : E->h)
-            if ((rc = vv2_all(h, dt, last && out ? 1 : 0, !last))) return rc;
+        if ((rc = vv2_all(C, dt, 0, !last))) return rc;
     }
-    for (auto *h : E->h) h->built = false;
+    C->built = false;
     return ens_finish(E, out, out != nullptr);
 }
 
 int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
 {
     if (!E || !E->have_force) return fail(TB_ERR_STATE, "tb_ensemble_autotune needs tb_ensemble_force first");
-    HIPCHK(hipSetDevice(E->cfg.device));
-    const int saved = E->cfg.group;
+    tb_handle *C = E->C;
+    HIPCHK(hipSetDevice(C->cfg.device));
     float best = 1e30f;
-    int best_tpl = 2, ntot = 0;
-    for (auto *h : E->h) ntot += h->sp[0].n + h->sp[1].n;
-    hipEvent_t a = E->h[0]->ev[4], b = E->h[0]->ev[5];
+    int best_tpl = 2;
+    const int ntot = C->sp[0].n + C->sp[1].n;
     int rc;
     for (int tpl = 1; tpl <= 4; tpl *= 2) {
-        E->cfg.group = 64 * tpl;
-        if ((rc = ens_regroup(E, tpl))) {
-            E->cfg.group = saved;
-            return rc;
-        }
+        if ((rc = ens_regroup(E, tpl))) return rc;
         float ms = 0.0f;
         for (int rep = 0; rep < 3; rep++) {
-            HIPCHK(hipEventRecord(a, E->s));
-            rc = ens_force(E);
-            if (rc) {
-                E->cfg.group = saved;
-                return rc;
-            }
-            HIPCHK(hipEventRecord(b, E->s));
-            HIPCHK(hipEventSynchronize(b));
+            HIPCHK(hipEventRecord(C->ev[4], C->s));
+            if ((rc = ens_force(E))) return rc;
+            HIPCHK(hipEventRecord(C->ev[5], C->s));
+            HIPCHK(hipEventSynchronize(C->ev[5]));
             float t = 0.0f;
-            HIPCHK(hipEventElapsedTime(&t, a, b));
-            if (rep > 0) ms += t;
+            HIPCHK(hipEventElapsedTime(&t, C->ev[4], C->ev[5]));
+            if (rep > 0) ms += t;  // the first run warms caches
         }
         ms *= 0.5f;
         if (ms_out) ms_out[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)] = ms;
@@ -1479,9 +1625,10 @@ int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
             best_tpl = tpl;
         }
     }
-    E->cfg.group = saved;
-    tb_autotune_record(ntot / (int)E->h.size(), best_tpl);
-    if ((rc = ens_regroup(E, ens_tpl(E)))) return rc;
+    tb_autotune_record(ntot / E->nreal, best_tpl);
+    if ((rc = ens_regroup(E, tb_mixed_tpl(C->cfg, ntot / E->nreal)))) return rc;
+    // the forces on the host side belong to the last timed launch, same trees
+    if ((rc = ens_force(E))) return rc;
     if (group_out) *group_out = 64 * best_tpl;
     return TB_OK;
 }

@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "tb_internal.h"
@@ -713,9 +714,14 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
 // net charge q, dipole D, second moment M and third moment O of a cell about
 // its geometric centre, shifted up from the children's own expansion centres
 // (a child's offsets s become s + a, a = child centre - cell centre)
+//
+// A batched ensemble's keys start with T octal digits naming the realisation
+// (its tag): the cells of realisation r start at depth T from its own box
+// root[6 r ..]; the T levels above hold no cell of any realisation and are
+// never walked (their records are zero).  One domain: T = 0.
 __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restrict__ seg,
                                                 const uint64_t *__restrict__ key, const double *__restrict__ root,
-                                                double4 *__restrict__ nc, const uint32_t *__restrict__ nnext,
+                                                int T, double4 *__restrict__ nc, const uint32_t *__restrict__ nnext,
                                                 double *__restrict__ nq, const int32_t *__restrict__ nstart,
                                                 const int32_t *__restrict__ ndepth, int32_t *__restrict__ err)
 {
@@ -725,10 +731,16 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
     uint32_t endp = nnext[p];
     if (endp & TB_LEAF) return;
     int d = ndepth[p];
-    double ox = root[0], oy = root[1], oz = root[2];
-    double lx = root[3], ly = root[4], lz = root[5];
     uint64_t k = key[nstart[p]];
-    for (int l = 0; l < d; l++) {
+    if (d < T) {
+        nc[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+        for (int c = 0; c < TB_JQ; c++) nq[(size_t)p * TB_JQ + c] = 0.0;
+        return;
+    }
+    const double *rt = root + (T > 0 ? 6 * (size_t)(k >> (3 * (TB_LEVELS - T))) : 0);
+    double ox = rt[0], oy = rt[1], oz = rt[2];
+    double lx = rt[3], ly = rt[4], lz = rt[5];
+    for (int l = T; l < d; l++) {
         uint32_t dig = (uint32_t)(k >> (3 * (TB_LEVELS - 1 - l))) & 7u;
         ox = ox + ((dig & 1u) ? 0.25 : -0.25) * lx;
         oy = oy + ((dig & 2u) ? 0.25 : -0.25) * ly;
@@ -846,9 +858,11 @@ __global__ __launch_bounds__(BT) void k_gwin(int n, int W, const uint64_t *__res
 }
 
 // flag[j] = 1 where a segment starts
+// tagmask: key bits naming the realisation of a batched ensemble (0: one
+// domain); a segment never spans two realisations
 __global__ __launch_bounds__(BT) void k_gflag(int n, int W, const uint64_t *__restrict__ key,
                                               const int8_t *__restrict__ pre, const int8_t *__restrict__ suf,
-                                              int32_t *__restrict__ flag)
+                                              uint64_t tagmask, int32_t *__restrict__ flag)
 {
     const int j = blockIdx.x * BT + threadIdx.x;
     if (j >= n) return;
@@ -856,9 +870,21 @@ __global__ __launch_bounds__(BT) void k_gflag(int n, int W, const uint64_t *__re
     if (j > 0) {
         int mx = pre[j];
         if (j >= W) mx = max(mx, (int)suf[j - W]);
-        f = lcp_digits(key[j - 1], key[j]) <= mx ? 1 : 0;
+        f = (lcp_digits(key[j - 1], key[j]) <= mx || ((key[j - 1] ^ key[j]) & tagmask) != 0) ? 1 : 0;
     }
     flag[j] = f;
+}
+
+// batched ensemble: first target group of each realisation (its tag, the key
+// bits from tagshift up, of the group's first particle)
+__global__ __launch_bounds__(BT) void k_gfirst(int n, const int32_t *__restrict__ ng, const int2 *__restrict__ grp,
+                                               const uint64_t *__restrict__ key, int tagshift,
+                                               int32_t *__restrict__ first)
+{
+    const int g = blockIdx.x * BT + threadIdx.x;
+    if (g >= n || g >= *ng) return;
+    const uint64_t r = key[grp[g].x] >> tagshift;
+    if (g == 0 || (key[grp[g - 1].x] >> tagshift) != r) first[r] = g;
 }
 
 __global__ __launch_bounds__(BT) void k_gseg(int n, const int32_t *__restrict__ flag,
@@ -894,6 +920,103 @@ __global__ __launch_bounds__(BT) void k_gemit(int n, const int32_t *__restrict__
         const int a = st + (int)((long)q * m / p), b = st + (int)((long)(q + 1) * m / p);
         grp[o + q] = make_int2(a, b - a);
     }
+}
+
+// ----------------------------------------------- batched ensemble (segments)
+//
+// Realisation r holds particles [off[r], off[r+1]) of a combined array.
+
+// partial min/max of each realisation: block (c, r) strides over r's slice
+__global__ __launch_bounds__(BT) void k_sbbox_part(const int32_t *__restrict__ off, const double *__restrict__ x,
+                                                   const double *__restrict__ y, const double *__restrict__ z,
+                                                   double *__restrict__ part)
+{
+    const int r = blockIdx.y, b = off[r], e = off[r + 1];
+    double mn0 = DBL_MAX, mn1 = DBL_MAX, mn2 = DBL_MAX, mx0 = -DBL_MAX, mx1 = -DBL_MAX, mx2 = -DBL_MAX;
+    for (int i = b + blockIdx.x * BT + threadIdx.x; i < e; i += gridDim.x * BT) {
+        double a = x[i], bb = y[i], c = z[i];
+        mn0 = fmin(mn0, a); mx0 = fmax(mx0, a);
+        mn1 = fmin(mn1, bb); mx1 = fmax(mx1, bb);
+        mn2 = fmin(mn2, c); mx2 = fmax(mx2, c);
+    }
+    __shared__ double sm[6][BT];
+    sm[0][threadIdx.x] = mn0; sm[1][threadIdx.x] = mn1; sm[2][threadIdx.x] = mn2;
+    sm[3][threadIdx.x] = mx0; sm[4][threadIdx.x] = mx1; sm[5][threadIdx.x] = mx2;
+    __syncthreads();
+    for (int w = BT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) sm[k][threadIdx.x] = fmin(sm[k][threadIdx.x], sm[k][threadIdx.x + w]);
+            for (int k = 3; k < 6; k++) sm[k][threadIdx.x] = fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[((size_t)r * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// box of each realisation (block r) from its partials of one or two arrays;
+// min and max are exact, so the box is the one a single domain computes
+__global__ __launch_bounds__(64) void k_sbbox_final(int chunks, const double *__restrict__ pa,
+                                                    const double *__restrict__ pb, double *__restrict__ root)
+{
+    const int r = blockIdx.x;
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int src = 0; src < 2; src++) {
+        const double *p = src == 0 ? pa : pb;
+        if (!p) continue;
+        for (int c = threadIdx.x; c < chunks; c += 64)
+            for (int k = 0; k < 3; k++) {
+                v[k] = fmin(v[k], p[((size_t)r * chunks + c) * 6 + k]);
+                v[3 + k] = fmax(v[3 + k], p[((size_t)r * chunks + c) * 6 + 3 + k]);
+            }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 3; k++) {
+            v[k] = fmin(v[k], __shfl_xor(v[k], o));
+            v[3 + k] = fmax(v[3 + k], __shfl_xor(v[3 + k], o));
+        }
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; k++) {
+        root[6 * r + k] = (v[3 + k] + v[k]) / 2.0;   // as k_bbox_final
+        root[6 * r + 3 + k] = v[3 + k] - v[k];
+    }
+}
+
+// keys of a batched ensemble: the realisation r as T octal digits, then 21 - T
+// levels of the centre recurrence (k_keys) from r's own box
+__global__ __launch_bounds__(BT) void k_ekeys(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                              const double *__restrict__ z, const int32_t *__restrict__ off,
+                                              int nreal, const double *__restrict__ roots, int T,
+                                              uint64_t *__restrict__ key, uint32_t *__restrict__ val, uint32_t tag,
+                                              int32_t *__restrict__ err)
+{
+    int i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nreal - 1;  // the realisation holding i: off[lo] <= i < off[lo + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const double *rt = roots + 6 * (size_t)lo;
+    double px = x[i], py = y[i], pz = z[i];
+    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(err, 2);
+    double cx = rt[0], cy = rt[1], cz = rt[2];
+    double lx = rt[3], ly = rt[4], lz = rt[5];
+    uint64_t k = (uint64_t)lo;
+    for (int l = T; l < TB_LEVELS; l++) {
+        uint32_t bx = (px < cx) ? 0u : 1u;
+        uint32_t by = (py < cy) ? 0u : 1u;
+        uint32_t bz = (pz < cz) ? 0u : 1u;
+        k = (k << 3) | (uint64_t)(bx | (by << 1) | (bz << 2));
+        cx = cx + (bx ? 0.25 : -0.25) * lx;
+        cy = cy + (by ? 0.25 : -0.25) * ly;
+        cz = cz + (bz ? 0.25 : -0.25) * lz;
+        lx = lx * 0.5;
+        ly = ly * 0.5;
+        lz = lz * 0.5;
+    }
+    key[i] = k;
+    val[i] = tag | (uint32_t)i;
 }
 
 inline int nblk(long n, int per) { return (int)((n + per - 1) / per); }
@@ -1024,17 +1147,42 @@ void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x
     hipLaunchKernelGGL(k_jgather, dim3(nblk(n, BT)), dim3(BT), 0, s, n, val, src, pos);
 }
 
-void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp)
+void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp, int T, int32_t *first)
 {
     const int n = S.n, W = 8 * G;
+    const int tagshift = 3 * (TB_LEVELS - T);
+    const uint64_t tagmask = T > 0 ? ~0ull << tagshift : 0ull;
     hipLaunchKernelGGL(k_gwin, dim3(nblk(n, W)), dim3(BT), 0, s, n, W, S.skey, S.gpre, S.gsuf);
-    hipLaunchKernelGGL(k_gflag, dim3(nblk(n, BT)), dim3(BT), 0, s, n, W, S.skey, S.gpre, S.gsuf, S.cnt);
+    hipLaunchKernelGGL(k_gflag, dim3(nblk(n, BT)), dim3(BT), 0, s, n, W, S.skey, S.gpre, S.gsuf, tagmask, S.cnt);
     tb_scan_excl(s, n, S.cnt, S.offs, totals, scan_tmp);
     hipLaunchKernelGGL(k_gseg, dim3(nblk(n, BT)), dim3(BT), 0, s, n, S.cnt, S.offs, S.gseg);
     hipLaunchKernelGGL(k_gcount, dim3(nblk(n, BT)), dim3(BT), 0, s, n, G, totals, S.gseg, S.cnt);
     tb_scan_excl(s, n, S.cnt, S.offs, totals + 1, scan_tmp);
     hipLaunchKernelGGL(k_gemit, dim3(nblk(n, BT)), dim3(BT), 0, s, n, totals, S.gseg, S.cnt, S.offs, S.grp);
+    if (first)
+        hipLaunchKernelGGL(k_gfirst, dim3(nblk(n, BT)), dim3(BT), 0, s, n, totals + 1, S.grp, S.skey,
+                           T > 0 ? tagshift : 63, first);
     S.gG = G;
+}
+
+int tb_sbbox_chunks(int n_max) { return std::max(1, std::min(64, nblk(n_max, 4 * BT))); }
+
+void tb_sbbox(hipStream_t s, int nreal, int chunks, const int32_t *off, const double *x, const double *y,
+              const double *z, double *part)
+{
+    hipLaunchKernelGGL(k_sbbox_part, dim3(chunks, nreal), dim3(BT), 0, s, off, x, y, z, part);
+}
+
+void tb_sbbox_final(hipStream_t s, int nreal, int chunks, const double *pa, const double *pb, double *roots)
+{
+    hipLaunchKernelGGL(k_sbbox_final, dim3(nreal), dim3(64), 0, s, chunks, pa, pb, roots);
+}
+
+void tb_ekeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const int32_t *off,
+              int nreal, const double *roots, int T, uint64_t *key, uint32_t *val, uint32_t tag, int32_t *err)
+{
+    hipLaunchKernelGGL(k_ekeys, dim3(nblk(n, BT)), dim3(BT), 0, s, n, x, y, z, off, nreal, roots, T, key, val, tag,
+                       err);
 }
 
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)
@@ -1043,12 +1191,12 @@ void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)
                        J.nq, J.nstart, J.ncount, J.ndepth, J.dkey[0]);
 }
 
-void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err)
+void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err, int T, const double *roots)
 {
     for (int d = J.maxdepth; d >= 0; d--) {
         int beg = J.lvl_off[d], cnt = J.lvl_off[d + 1] - beg;
         if (cnt <= 0 || J.twig_hist[d] == 0) continue;
-        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, cnt, J.bfs + beg, key, J.root, J.nc,
-                           J.nnext, J.nq, J.nstart, J.ndepth, err);
+        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, cnt, J.bfs + beg, key,
+                           T > 0 ? roots : J.root, T, J.nc, J.nnext, J.nq, J.nstart, J.ndepth, err);
     }
 }

@@ -105,10 +105,22 @@ void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const doub
 void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x[2], const double *const y[2],
                 const double *const z[2], double4 *pos);
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key);
-void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err);
+// T > 0: a batched ensemble's joint tree (keys tagged by tb_ekeys), roots [nreal][6]
+void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err, int T = 0,
+                const double *roots = nullptr);
 // target groups of at most G particles over S.skey; totals[0] = segments,
-// totals[1] = groups (device); uses S.cnt and S.offs as scratch
-void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp);
+// totals[1] = groups (device); uses S.cnt and S.offs as scratch.  A batched
+// ensemble (keys tagged with T digits) never groups across realisations, and
+// first[r] (device) receives realisation r's first group.
+void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_tmp, int T = 0,
+               int32_t *first = nullptr);
+// batched ensemble: realisation r holds [off[r], off[r+1]) of a combined array
+int tb_sbbox_chunks(int n_max);
+void tb_sbbox(hipStream_t s, int nreal, int chunks, const int32_t *off, const double *x, const double *y,
+              const double *z, double *part /*[nreal][chunks][6]*/);
+void tb_sbbox_final(hipStream_t s, int nreal, int chunks, const double *pa, const double *pb, double *roots);
+void tb_ekeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const int32_t *off,
+              int nreal, const double *roots, int T, uint64_t *key, uint32_t *val, uint32_t tag, int32_t *err);
 
 // ----- walk64.hip
 // RDF accumulation of the FP64 walk: hist = [2][nbin] int64 (e-i, i-i) on the device
@@ -136,3 +148,7 @@ void tb_vv2(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, doub
 void tb_kdk(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, double *part);
 int tb_vv_blocks(int n);
 void tb_inverse_perm(hipStream_t s, int n, const int32_t *orig, int32_t *inv);
+// batched ensemble: mv2 and radial sums per realisation (slices off[r] .. off[r+1]), out [nreal][2];
+// part holds nreal * chunks * 2 doubles
+void tb_ke_seg(hipStream_t s, SpeciesDev &S, double xm, int nreal, int chunks, const int32_t *off, double *part,
+               double *out);

@@ -42,6 +42,7 @@ struct Domain {
     float inv_rs, rs;
     int gbase;        // first global group index of this domain
     int nn;           // cells in the tree
+    uint32_t root;    // the domain's root cell (a batched ensemble's trees share one table)
 };
 
 struct MixedDev {
@@ -79,12 +80,13 @@ const char *tb_mixed_error();
 // duration of the last walk dispatch (after the stream has synchronised); -1 if none
 float tb_mixed_walk_ms(MixedDev &M);
 
-// one persistent launch over several domains (the batched ensemble): fills
-// esum[4 * d] per domain; rs[d] is domain d's Kelbg length; rec/collect are
-// not used
-int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees,
-                         const double *rs, int ndom,
-                         const tb_config &cfg, double *esum_out_dev);
+// one persistent launch over the realisations of a batched ensemble, all in
+// one combined state: species arrays sp[2] (realisation r's targets in groups
+// [gfirst[k][r], gfirst[k][r] + gcount[k][r]) of species k) and one table of
+// cells (r's root at roots[r]); rs[r] its Kelbg length; fills esum[4 r]
+int tb_mixed_force_ens(hipStream_t s, MixedDev &M, SpeciesDev *sp, int nreal, const uint32_t *roots,
+                       const double *rs, const int32_t *const gfirst[2], const int32_t *const gcount[2],
+                       const tb_config &cfg, double *esum_out_dev);
 
 // walk-group size chosen by the autotuner for a problem size (targets per lane: 1, 2 or 4)
 int tb_autotune_tpl(int n_total);

@@ -637,7 +637,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
 
         // stack, M2P, P2P (like, unlike), band-queue, far-queue lengths
         int sp = 1, nm = 0, npl = 0, npu = 0, nb = 0, nf = 0, nx = 0;
-        L.stack[0] = 0u;  // root: range of one cell at 0
+        L.stack[0] = D.root << 3;  // the domain's root: a range of one cell
         wave_sync();
         while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
             // learned: a pass either decides the queued band cells (once 64
@@ -1309,6 +1309,7 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int g
     d.rs = (float)rs;
     d.gbase = gbase;
     d.nn = t->nn;
+    d.root = 0;
     return d;
 }
 
@@ -1604,30 +1605,30 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev 
     return TB_OK;
 }
 
-int tb_mixed_force_multi(hipStream_t s, MixedDev &M, SpeciesDev *const *sp, const MixedTree *const *trees,
-                         const double *rs, int ndom,
-                         const tb_config &cfg, double *esum_out_dev)
+int tb_mixed_force_ens(hipStream_t s, MixedDev &M, SpeciesDev *sp, int nreal, const uint32_t *roots,
+                       const double *rs, const int32_t *const gfirst[2], const int32_t *const gcount[2],
+                       const tb_config &cfg, double *esum_out_dev)
 {
-    // one kernel instantiation for every domain: all were grouped for one size
-    const int tpl = group_tpl(*sp[0], *sp[1]);
-    for (int d = 1; d < ndom && tpl; d++)
-        if (group_tpl(*sp[2 * d], *sp[2 * d + 1]) != tpl) {
-            g_err = "mixed walk: realisations grouped for different walk-group sizes";
-            return TB_ERR_STATE;
-        }
+    const int tpl = group_tpl(sp[0], sp[1]);
     if (!tpl) return TB_ERR_STATE;
+    SpeciesDev *const spp[2] = {&sp[0], &sp[1]};
     std::vector<Domain> doms;
-    doms.reserve(ndom);
+    doms.reserve(nreal);
     int gb = 0;
-    for (int d = 0; d < ndom; d++) {
-        SpeciesDev *const spp[2] = {sp[2 * d], sp[2 * d + 1]};
-        doms.push_back(make_domain(spp, trees[d], rs[d], gb));
-        gb += doms.back().ng[0] + doms.back().ng[1];
+    for (int r = 0; r < nreal; r++) {
+        Domain d = make_domain(spp, &M.t[0], rs[r], gb);
+        for (int k = 0; k < 2; k++) {
+            d.grp[k] = sp[k].grp + gfirst[k][r];
+            d.ng[k] = gcount[k][r];
+        }
+        d.root = roots[r];
+        doms.push_back(d);
+        gb += d.ng[0] + d.ng[1];
     }
     int rc = launch_domains(s, M, doms, tpl, cfg, nullptr, MODE_FIXED);
     if (rc) return rc;
     if (esum_out_dev &&
-        hipMemcpyAsync(esum_out_dev, M.esum, (size_t)4 * ndom * sizeof(double), hipMemcpyDeviceToDevice, s) !=
+        hipMemcpyAsync(esum_out_dev, M.esum, (size_t)4 * nreal * sizeof(double), hipMemcpyDeviceToDevice, s) !=
             hipSuccess) {
         g_err = "mixed walk: energy copy failed";
         return TB_ERR_HIP;

@@ -380,6 +380,47 @@ __global__ __launch_bounds__(BT) void k_vv2(int n, StateCols S, double xm, doubl
     }
 }
 
+// kinetic and radial sums of each realisation of a batched ensemble
+// (vverlet.f90:83-86 per particle): block (c, r) strides over realisation
+// r's slice [off[r], off[r+1]); partials [r][c][2]
+__global__ __launch_bounds__(BT) void k_ke_seg(const int32_t *__restrict__ off, StateCols S, double xm,
+                                               double *__restrict__ part)
+{
+    __shared__ double lds[BT];
+    const int r = blockIdx.y;
+    double a = 0.0, b = 0.0;
+    for (int i = off[r] + blockIdx.x * BT + threadIdx.x; i < off[r + 1]; i += gridDim.x * BT) {
+        double x[3], v[3], mv2, rad;
+        for (int k = 0; k < 3; k++) {
+            x[k] = S.x[k][i];
+            v[k] = S.v[k][i];
+        }
+        sample_sums(xm, x, v, mv2, rad);
+        a += mv2;
+        b += rad;
+    }
+    a = block_sum(a, lds);
+    b = block_sum(b, lds);
+    if (threadIdx.x == 0) {
+        part[((size_t)r * gridDim.x + blockIdx.x) * 2] = a;
+        part[((size_t)r * gridDim.x + blockIdx.x) * 2 + 1] = b;
+    }
+}
+
+// per realisation, its partials in a fixed order: out[2 r], out[2 r + 1]
+__global__ void k_ke_seg_final(int nreal, int chunks, const double *__restrict__ part, double *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreal) return;
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < chunks; c++) {
+        a += part[((size_t)r * chunks + c) * 2];
+        b += part[((size_t)r * chunks + c) * 2 + 1];
+    }
+    out[2 * r] = a;
+    out[2 * r + 1] = b;
+}
+
 __global__ __launch_bounds__(BT) void k_inverse(int n, const int32_t *__restrict__ orig, int32_t *__restrict__ inv)
 {
     const int i = blockIdx.x * BT + threadIdx.x;
@@ -457,6 +498,13 @@ void tb_vv2(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, doub
 void tb_kdk(hipStream_t s, SpeciesDev &S, double xm, double dt, int sample, double *part)
 {
     hipLaunchKernelGGL(k_vv2<true>, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, cols(S), xm, dt, sample, part);
+}
+
+void tb_ke_seg(hipStream_t s, SpeciesDev &S, double xm, int nreal, int chunks, const int32_t *off, double *part,
+               double *out)
+{
+    hipLaunchKernelGGL(k_ke_seg, dim3(chunks, nreal), dim3(BT), 0, s, off, cols(S), xm, part);
+    hipLaunchKernelGGL(k_ke_seg_final, dim3((nreal + 63) / 64), dim3(64), 0, s, nreal, chunks, part, out);
 }
 
 void tb_inverse_perm(hipStream_t s, int n, const int32_t *orig, int32_t *inv)
