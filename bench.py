@@ -262,7 +262,9 @@ def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
     g = a.group
     if a.group == 0:
         g, ms = eng.autotune()
-        log("  autotuned group size %d (ms per walk for 64/128/256: %s)" % (g, np.round(ms, 2)))
+        log("  autotuned group size %d, lists %s (ms per walk for 64/128/256: %s)"
+            % (g, eng.walk_lists(), np.round(ms, 2)))
+    lists = eng.walk_lists()
     eng.build()
     eng.force()
     st = eng.count_interactions()
@@ -270,7 +272,8 @@ def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
     r = timed_steps(eng, dt, a.steps, a.warmup)
     eng.close()
     log("mixed: theta=%.4f rms=%.2e %.3f steps/s walk %.2f ms" % (th, rms, 1 / r["s"], r["ms_walk"]))
-    return {"variant": "mixed", "theta": th, "rms": rms, "max": mx, "st": st, "prec": "fp32", "group": g, **r}
+    return {"variant": "mixed", "theta": th, "rms": rms, "max": mx, "st": st, "prec": "fp32", "group": g,
+            "lists": lists, **r}
 
 
 def train_mlp(a, th_lo, th_hi):
@@ -307,7 +310,9 @@ def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
         eng.build()
         eng.force()
         g, ms = eng.autotune()
-        log("  learned walk: autotuned group size %d (ms per walk for 64/128/256: %s)" % (g, np.round(ms, 2)))
+        log("  learned walk: autotuned group size %d, lists %s (ms per walk for 64/128/256: %s)"
+            % (g, eng.walk_lists(), np.round(ms, 2)))
+    lists = eng.walk_lists()
     if a.err_target > 0:
         et = a.err_target
         eng.configure(err_target=et)
@@ -325,7 +330,7 @@ def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
     log("mixed+learned: err_target=%.2e rms=%.2e %.3f steps/s walk %.2f ms"
         % (et, rms_l, 1 / r["s"], r["ms_walk"]))
     return {"variant": "mixed+learned", "theta": th_hi, "theta_lo": th_lo, "err_target": et, "rms": rms_l,
-            "max": mx_l, "st": st_l, "prec": "fp32", "group": g, **r}
+            "max": mx_l, "st": st_l, "prec": "fp32", "group": g, "lists": lists, **r}
 
 
 # ------------------------------------------------------------------ other configurations
@@ -439,6 +444,7 @@ def run_c4(a):
             ens.set_particles(r, qe, qi)
         ens.force()
         g, ms = ens.autotune()
+        lists = ens.walk_lists()
         ens.step(dt, 2)
         t0 = time.perf_counter()
         ens.step(dt, steps)
@@ -454,7 +460,7 @@ def run_c4(a):
             eng.step(dt, steps)
             t_seq += time.perf_counter() - t0
     out = {"config": "C4: 64 realisations, N_total log-uniform 16k..256k (sum %d), T_e 1..100 K, theta 0.5, "
-                     "mixed walk" % int(ntot.sum()), "steps": steps, "group": int(g),
+                     "mixed walk" % int(ntot.sum()), "steps": steps, "group": int(g), "lists": list(lists),
            "ensemble_steps_per_s": steps / t_ens, "particle_steps_per_s": steps * float(ntot.sum()) / t_ens,
            "one_engine_at_a_time_s": t_seq, "ensemble_s": t_ens, "batching_speedup": t_seq / t_ens}
     log("C4: %.2f ensemble steps/s, %.3g particle-steps/s, batching x%.2f" %
@@ -582,7 +588,7 @@ def main():
                "ms_force": v["ms_force"], "ms_build": v["ms_build"]}
         if "ms_walk" in v:
             out["ms_walk_kernel"] = v["ms_walk"]
-        for k in ("drift", "stride", "err_target", "theta_lo", "group"):
+        for k in ("drift", "stride", "err_target", "theta_lo", "group", "lists"):
             if k in v:
                 out[k] = v[k]
         if "st" in v:

@@ -209,8 +209,12 @@ int tb_mlp_train(tb_handle *h, int32_t n, const float *X, const float *y, int32_
 int tb_mlp_collect(tb_handle *h, int32_t cap, float *X, float *y, int32_t *n);
 
 /* time the mixed walk on the current tree for 64, 128 and 256 targets per
- * group, keep the fastest for this problem size; ms_out[3] (optional) */
+ * group, each with interaction lists evaluated at 15 / 11 / 7 cells (24 / 16 /
+ * 12 leaves), keep the fastest pair for this problem size; ms_out[3]
+ * (optional): the three group sizes at the chosen list lengths */
 int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out);
+/* the list lengths (cells, leaves) the mixed walk of this handle flushes at */
+int tb_walk_lists(tb_handle *h, int32_t *flush_m, int32_t *flush_p);
 
 /* ------------------------------------------------------- batched ensemble
  * Many independent realisations (each its own electron and ion trees) on one
@@ -229,6 +233,7 @@ int tb_ensemble_force(tb_ensemble *e, double *epot);
 /* nsteps for all; out (optional) [n][6] as tb_step after the last step */
 int tb_ensemble_step(tb_ensemble *e, double dt, int32_t nsteps, double *out);
 int tb_ensemble_autotune(tb_ensemble *e, int32_t *group_out, float *ms_out);
+int tb_ensemble_walk_lists(tb_ensemble *e, int32_t *flush_m, int32_t *flush_p);
 
 /* ------------------------------------------------ radial distribution
  * Leaf-pair RDF histograms, as the reference accumulates them in every

@@ -26,8 +26,6 @@ def test_ensemble_matches_single_runs():
         got = [ens.get_particles(r) for r in range(len(sizes))]
         out = ens.step(dt, 3)
         after = [ens.get_particles(r) for r in range(len(sizes))]
-        g, ms = ens.autotune()
-    assert g in (64, 128, 256) and np.all(ms > 0)
     for r, (qe, qi) in enumerate(data):
         with ta.Engine(rs=ta.kelbg_length(tes[r]), theta=0.5, theta_mixed=0.5, walk="mixed", group=128) as eng:
             eng.set_particles(qe, qi)
@@ -41,3 +39,11 @@ def test_ensemble_matches_single_runs():
         assert np.allclose(ep[r], ep1, rtol=1e-12)
         assert np.array_equal(after[r][0]["xx"], e2["xx"]) and np.array_equal(after[r][1]["xv"], i2["xv"])
         assert np.allclose(out[r], out1, rtol=1e-12)
+    # the autotuner (last: it sets the list lengths later walks of these sizes use)
+    with ta.Ensemble(cfgs) as ens:
+        for r, (qe, qi) in enumerate(data):
+            ens.set_particles(r, qe, qi)
+        ens.force()
+        g, ms = ens.autotune()
+        assert g in (64, 128, 256) and np.all(ms > 0)
+        assert ens.walk_lists() in ((15, 24), (11, 16), (7, 12))

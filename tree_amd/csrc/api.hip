@@ -471,6 +471,7 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         h->st.max_depth[0] = J.maxdepth;
         h->st.max_depth[1] = 0;
         HIPCHK(hipGetLastError());
+        h->mx.lists = tb_autotune_lists(eb ? (h->sp[0].n + h->sp[1].n) / eb->nreal : h->sp[0].n + h->sp[1].n);
         if ((rc = tb_mixed_prep(h->s, h->mx, h->sp, &h->jt, h->cfg, &h->bytes)))
             return fail(rc, "%s", tb_mixed_error());
         h->built = true;
@@ -1237,37 +1238,47 @@ int tb_autotune(tb_handle *h, int32_t *group_out, float *ms_out)
     HIPCHK(hipSetDevice(h->cfg.device));
     const int saved = h->cfg.group;
     float best = 1e30f;
-    int best_tpl = 2;
+    int best_tpl = 2, best_lv = 0;
     int rc;
+    float tab[3][TB_LIST_VARIANTS];
     for (int tpl = 1; tpl <= 4; tpl *= 2) {
         h->cfg.group = 64 * tpl;
         if ((rc = regroup(h, tpl))) {
             h->cfg.group = saved;
             return rc;
         }
-        float ms = 0.0f;
-        for (int rep = 0; rep < 3; rep++) {
-            HIPCHK(hipEventRecord(h->ev[2], h->s));
-            rc = force_all(h, false);
-            if (rc) {
-                h->cfg.group = saved;
-                return rc;
+        for (int lv = 0; lv < TB_LIST_VARIANTS; lv++) {
+            h->mx.lists = lv;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 3; rep++) {
+                HIPCHK(hipEventRecord(h->ev[2], h->s));
+                rc = force_all(h, false);
+                if (rc) {
+                    h->cfg.group = saved;
+                    return rc;
+                }
+                HIPCHK(hipEventRecord(h->ev[3], h->s));
+                HIPCHK(hipEventSynchronize(h->ev[3]));
+                float t = 0.0f;
+                HIPCHK(hipEventElapsedTime(&t, h->ev[2], h->ev[3]));
+                if (rep > 0) ms += t;  // first run warms caches
             }
-            HIPCHK(hipEventRecord(h->ev[3], h->s));
-            HIPCHK(hipEventSynchronize(h->ev[3]));
-            float t = 0.0f;
-            HIPCHK(hipEventElapsedTime(&t, h->ev[2], h->ev[3]));
-            if (rep > 0) ms += t;  // first run warms caches
-        }
-        ms *= 0.5f;
-        if (ms_out) ms_out[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)] = ms;
-        if (ms < best) {
-            best = ms;
-            best_tpl = tpl;
+            ms *= 0.5f;
+            tab[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)][lv] = ms;
+            if (ms < best) {
+                best = ms;
+                best_tpl = tpl;
+                best_lv = lv;
+            }
         }
     }
+    // per group size: its time at the chosen list lengths
+    if (ms_out)
+        for (int g = 0; g < 3; g++) ms_out[g] = tab[g][best_lv];
     h->cfg.group = saved;
     tb_autotune_record(h->sp[0].n + h->sp[1].n, best_tpl);
+    tb_autotune_record_lists(h->sp[0].n + h->sp[1].n, best_lv);
+    h->mx.lists = best_lv;
     if ((rc = regroup(h, tb_mixed_tpl(h->cfg, h->sp[0].n + h->sp[1].n)))) return rc;
     if (group_out) *group_out = 64 * best_tpl;
     return TB_OK;
@@ -1603,34 +1614,59 @@ int tb_ensemble_autotune(tb_ensemble *E, int32_t *group_out, float *ms_out)
     tb_handle *C = E->C;
     HIPCHK(hipSetDevice(C->cfg.device));
     float best = 1e30f;
-    int best_tpl = 2;
+    int best_tpl = 2, best_lv = 0;
     const int ntot = C->sp[0].n + C->sp[1].n;
     int rc;
+    float tab[3][TB_LIST_VARIANTS];
     for (int tpl = 1; tpl <= 4; tpl *= 2) {
         if ((rc = ens_regroup(E, tpl))) return rc;
-        float ms = 0.0f;
-        for (int rep = 0; rep < 3; rep++) {
-            HIPCHK(hipEventRecord(C->ev[4], C->s));
-            if ((rc = ens_force(E))) return rc;
-            HIPCHK(hipEventRecord(C->ev[5], C->s));
-            HIPCHK(hipEventSynchronize(C->ev[5]));
-            float t = 0.0f;
-            HIPCHK(hipEventElapsedTime(&t, C->ev[4], C->ev[5]));
-            if (rep > 0) ms += t;  // the first run warms caches
-        }
-        ms *= 0.5f;
-        if (ms_out) ms_out[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)] = ms;
-        if (ms < best) {
-            best = ms;
-            best_tpl = tpl;
+        for (int lv = 0; lv < TB_LIST_VARIANTS; lv++) {
+            C->mx.lists = lv;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 3; rep++) {
+                HIPCHK(hipEventRecord(C->ev[4], C->s));
+                if ((rc = ens_force(E))) return rc;
+                HIPCHK(hipEventRecord(C->ev[5], C->s));
+                HIPCHK(hipEventSynchronize(C->ev[5]));
+                float t = 0.0f;
+                HIPCHK(hipEventElapsedTime(&t, C->ev[4], C->ev[5]));
+                if (rep > 0) ms += t;  // the first run warms caches
+            }
+            ms *= 0.5f;
+            tab[tpl == 1 ? 0 : (tpl == 2 ? 1 : 2)][lv] = ms;
+            if (ms < best) {
+                best = ms;
+                best_tpl = tpl;
+                best_lv = lv;
+            }
         }
     }
+    if (ms_out)
+        for (int g = 0; g < 3; g++) ms_out[g] = tab[g][best_lv];
     tb_autotune_record(ntot / E->nreal, best_tpl);
+    tb_autotune_record_lists(ntot / E->nreal, best_lv);
+    C->mx.lists = best_lv;
     if ((rc = ens_regroup(E, tb_mixed_tpl(C->cfg, ntot / E->nreal)))) return rc;
     // the forces on the host side belong to the last timed launch, same trees
     if ((rc = ens_force(E))) return rc;
     if (group_out) *group_out = 64 * best_tpl;
     return TB_OK;
+}
+
+int tb_walk_lists(tb_handle *h, int32_t *flush_m, int32_t *flush_p)
+{
+    if (!h || !flush_m || !flush_p) return fail(TB_ERR_ARG, "null argument");
+    int m, p;
+    tb_list_lengths(h->mx.lists, &m, &p);
+    *flush_m = m;
+    *flush_p = p;
+    return TB_OK;
+}
+
+int tb_ensemble_walk_lists(tb_ensemble *E, int32_t *flush_m, int32_t *flush_p)
+{
+    if (!E) return fail(TB_ERR_ARG, "null ensemble");
+    return tb_walk_lists(E->C, flush_m, flush_p);
 }
 
 // ------------------------------------------------------------------ RDF

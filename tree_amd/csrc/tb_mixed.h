@@ -45,8 +45,12 @@ struct Domain {
     uint32_t root;    // the domain's root cell (a batched ensemble's trees share one table)
 };
 
+// list-length variants of the walk (walk32.hip LIST_M / LIST_P)
+#define TB_LIST_VARIANTS 3
+
 struct MixedDev {
     MixedTree t[1];
+    int lists = 0;                // list-length variant of the next walks
     int32_t *ctr = nullptr;       // 16 ints: error flags, band pair count, work-queue chunk counters
     int tpl = 0;                  // targets per lane used by the last launch
     hipEvent_t ev[2] = {};        // around the last walk dispatch
@@ -93,3 +97,7 @@ int tb_autotune_tpl(int n_total);
 // targets per lane for a configuration: its group size, else the autotuner's choice
 int tb_mixed_tpl(const tb_config &cfg, int n_total);
 void tb_autotune_record(int n_total, int tpl);
+// list-length variant chosen by the autotuner for a problem size, its record, and its lengths
+int tb_autotune_lists(int n_total);
+void tb_autotune_record_lists(int n_total, int variant);
+void tb_list_lengths(int variant, int *m, int *p);

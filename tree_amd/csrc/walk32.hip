@@ -47,10 +47,16 @@ constexpr int STACK = 1024;
 // cells) are flushed 64 at a time: QCAP = 63 + 64.  The LDS per wave
 // (13.0 KB) keeps three 4-wave blocks on a CU; cell entries (80 bytes, up to
 // the third moment) are the largest part.
+// FLUSH_M / FLUSH_P are the largest list lengths (they size the lists); the
+// walk flushes at the lengths of one of TB_LIST_VARIANTS, chosen by the
+// autotuner with the group size (shorter lists: more flushes of fewer
+// entries, each flush's compensated sums and loop set-up paid more often).
 constexpr int FLUSH_M = 15;
 constexpr int LCAP_M = FLUSH_M - 1 + 64;
 constexpr int FLUSH_P = 24;
 constexpr int LCAP_P = FLUSH_P - 1 + 64;
+constexpr int LIST_M[TB_LIST_VARIANTS] = {15, 11, 7};
+constexpr int LIST_P[TB_LIST_VARIANTS] = {24, 16, 12};
 constexpr int QCAP = 128;
 // Unlike leaves within 22 rs of some target of the group: up to NEXACT of
 // them per group are evaluated after the walk with separations from the FP64
@@ -208,6 +214,7 @@ struct WalkArgs {
     BandRec *rec;         // collect: band pairs
     int rec_cap;
     int32_t *ctr;         // [1] error flag, [2] prep error, [3] band pair count, [4] far chunks, [8..15] work chunks
+    int flush_m, flush_p; // list lengths at which the M2P / P2P lists are evaluated (<= FLUSH_M / FLUSH_P)
     double *gen;          // 2 per global group: energy of electron targets / of ion targets
     unsigned long long *cnt;
 };
@@ -881,13 +888,13 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 nb -= ne;
                 wave_sync();
             }
-            if (nm >= FLUSH_M) {
+            if (nm >= W.flush_m) {
                 if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
                 flush_m2p<TPL>(L, nm, rx, ry, rz, A);
                 nm = 0;
                 wave_sync();
             }
-            if (npl + npu >= FLUSH_P) {
+            if (npl + npu >= W.flush_p) {
                 flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A, nx);
                 npl = npu = 0;
                 wave_sync();
@@ -1377,6 +1384,9 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     const double kfar = cfg.far_ratio == 0.0 ? FAR_RATIO_DEFAULT : cfg.far_ratio;
     W.far_k2 = kfar > 0.0 ? (float)(kfar * kfar) : 0.0f;
     W.mlpk = M.mlp.Ppk;
+    const int lv = M.lists >= 0 && M.lists < TB_LIST_VARIANTS ? M.lists : 0;
+    W.flush_m = LIST_M[lv];
+    W.flush_p = LIST_P[lv];
     W.rec = M.rec;
     W.rec_cap = M.rec_cap;
     W.ctr = M.ctr;
@@ -1701,3 +1711,17 @@ int tb_autotune_tpl(int n_total)
 }
 
 void tb_autotune_record(int n_total, int tpl) { g_tpl_table[size_bucket(n_total)] = tpl; }
+
+// list-length variant per size bucket (0, the longest lists, until autotuned)
+static int g_list_table[64] = {0};
+
+int tb_autotune_lists(int n_total) { return g_list_table[size_bucket(n_total)]; }
+
+void tb_autotune_record_lists(int n_total, int variant) { g_list_table[size_bucket(n_total)] = variant; }
+
+void tb_list_lengths(int variant, int *m, int *p)
+{
+    const int v = variant >= 0 && variant < TB_LIST_VARIANTS ? variant : 0;
+    *m = LIST_M[v];
+    *p = LIST_P[v];
+}

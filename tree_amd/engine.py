@@ -121,6 +121,8 @@ SIGNATURES = [
     ("tb_ensemble_force", ctypes.c_int, [_P, _P]),
     ("tb_ensemble_step", ctypes.c_int, [_P, _D, _I, _P]),
     ("tb_ensemble_autotune", ctypes.c_int, [_P, _P, _P]),
+    ("tb_ensemble_walk_lists", ctypes.c_int, [_P, _P, _P]),
+    ("tb_walk_lists", ctypes.c_int, [_P, _P, _P]),
     ("tb_last_error", ctypes.c_char_p, []),
     ("tb_abi_sizes", ctypes.c_int, [_P, _P, _P]),
     ("tb_kelbg_length", ctypes.c_double, [_D, _D, _D]),
@@ -357,6 +359,12 @@ class Engine:
         _check(self.lib.tb_autotune(self.h, ctypes.byref(g), _ptr(ms)))
         return g.value, ms
 
+    def walk_lists(self):
+        """(cells, leaves): the list lengths at which the mixed walk evaluates its interaction lists."""
+        m, p = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.tb_walk_lists(self.h, ctypes.byref(m), ctypes.byref(p)))
+        return m.value, p.value
+
     def rdf_enable(self, dx_ei, dx_ii, nbin=NDIST + 1):
         """Accumulate the leaf-pair RDF histograms in every force (FP64 walk); nbin = 0 disables."""
         _check(self.lib.tb_rdf_enable(self.h, dx_ei, dx_ii, nbin))
@@ -442,6 +450,11 @@ class Ensemble:
         ms = np.zeros(3, np.float32)
         _check(self.lib.tb_ensemble_autotune(self.h, ctypes.byref(g), _ptr(ms)))
         return g.value, ms
+
+    def walk_lists(self):
+        m, p = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.tb_ensemble_walk_lists(self.h, ctypes.byref(m), ctypes.byref(p)))
+        return m.value, p.value
 
 
 def _stats_dict(s):
