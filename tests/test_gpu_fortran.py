@@ -134,3 +134,38 @@ def test_fortran_driver_resume(tmp_path):
     assert ea.shape == eb.shape == (5, 9)
     for c in (1, 2, 7, 8):
         assert np.allclose(ea[:, c], eb[:, c], rtol=1e-4, atol=0), (c, ea[:, c], eb[:, c])
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="Fortran driver not built")
+def test_fortran_driver_mixed_walk_writes_rdf(tmp_path):
+    """TREEMI_WALK=mixed: the driver steps with the mixed walk and still writes
+    rdf###.dat (every pair in range, see rdf32.hip), in the reference's record
+    format, with the histograms of the same steps driven through the C-ABI."""
+    announce("fortran_driver_mixed")
+    ne = ni = 1000
+    rs = 6.6493e-3
+    qel, qion = ta.gen_plasma(10, ne, ni, 100.0)
+    plasma_io.write_control(tmp_path / "control.prm", ne, ni, tmax=0.025, trest=0.025, tdump=0.025, tsamp=0.005,
+                            dt=0.005, rs=rs, theta=0.4)
+    plasma_io.write_xyz(tmp_path / "input.xyz", qel, qion)
+    env = dict(os.environ, TREEMI_WALK="mixed")
+    r = subprocess.run([BIN], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _check_records(tmp_path)
+    got = plasma_io.read_rdf(tmp_path / "rdf001.dat")
+    v_e, v_i = ta.gen_plasma(1, ne, ni, 1.0, te=10.0, ti=1e-3)
+    qel["xv"], qion["xv"] = v_e["xv"], v_i["xv"]
+    dt = 0.005 / ta.TPS
+    with ta.Engine(rs=rs, theta=0.4, theta_mixed=0.4, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        eng.build()
+        eng.force()
+        eng.rdf_enable(0.01, 0.01)
+        for k in range(5):
+            eng.vv1(dt)
+            eng.build()
+            eng.force()
+            eng.vv2(dt, True)
+        hei, hii = eng.rdf_get()
+    assert hei.sum() > 0 and hii.sum() > 0
+    assert np.allclose(got[:, 1], hei / 5.0, rtol=1e-6, atol=0) and np.allclose(got[:, 3], hii / 5.0, rtol=1e-6, atol=0)

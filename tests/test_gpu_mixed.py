@@ -170,3 +170,59 @@ def test_mixed_close_unlike_pairs_share_the_finest_cell():
         with pytest.raises(ta.TreeError) as ex:
             eng.build()
     assert ex.value.code == 3, ex.value
+
+
+def _pair_hist(xa, xb, dx, nbin, cut):
+    """Histogram of the pairs (a in xa, b in xb) with r / dx < nbin, r formed as
+    the FP64 walk forms it (x x + y y + z z of target - source), found with a
+    k-d tree; cut: drop pairs with r^2 <= 0.01 (like species)."""
+    from scipy.spatial import cKDTree
+    ta_, tb_ = cKDTree(xa), cKDTree(xb)
+    pairs = ta_.query_ball_tree(tb_, nbin * dx * 1.001)
+    ia = np.repeat(np.arange(len(xa)), [len(p) for p in pairs])
+    ib = np.concatenate([np.asarray(p, np.int64) for p in pairs]) if len(ia) else np.zeros(0, np.int64)
+    d = xa[ia] - xb[ib]
+    r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+    if cut:
+        r2 = r2[r2 > 0.01]
+    idx = (np.sqrt(r2) / dx).astype(np.int64) + 1
+    idx = idx[(idx >= 1) & (idx <= nbin)]
+    return np.bincount(idx - 1, minlength=nbin)
+
+
+def test_mixed_rdf_is_the_pair_histogram():
+    """The mixed walk's RDF holds every pair within the histograms' range, the
+    reference's leaf-pair histograms (tree.f90:916-919, 1021-1024, 1125-1129)
+    in their theta -> 0 limit: bit for bit the oracle's theta -> 0 histograms
+    at 1500 + 1500 (dx 0.01 um, 10001 bins), and at 2^19 + 2^19 (sigma 400 um,
+    dx 0.001 / 0.0015 um) the exact pair enumeration of a k-d tree: 2 x the e-i
+    pairs (seen from both sides), the ordered i-i pairs beyond the 0.1 um cut.
+    Two forces add twice; the counting pass adds nothing."""
+    announce("mixed_rdf")
+    qel, qion = ta.gen_plasma(27, 1500, 1500, 100.0)
+    *_, oei, oii = pyoracle.local_coulomb_sum_rdf(qel["xx"], qion["xx"], 1e-12, RS10, 0.01, 0.01, nthreads=0)
+    with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.4, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        eng.rdf_enable(0.01, 0.01)
+        eng.build()
+        eng.force()
+        ei, ii = eng.rdf_get()
+        eng.force()
+        eng.count_interactions()
+        ei2, ii2 = eng.rdf_get(reset=True)
+    assert np.array_equal(ei, oei) and np.array_equal(ii, oii)
+    assert np.array_equal(ei2, 2 * oei) and np.array_equal(ii2, 2 * oii)
+    n = 1 << 19
+    qel, qion = ta.gen_plasma(28, n, n, 400.0)
+    nb = 10001
+    with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.3, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        eng.rdf_enable(0.001, 0.0015)
+        eng.build()
+        eng.force()
+        ei, ii = eng.rdf_get()
+    hei = _pair_hist(qel["xx"], qion["xx"], 0.001, nb, False)
+    hii = _pair_hist(qion["xx"], qion["xx"], 0.0015, nb, True)
+    print("2^19: e-i pairs %d, i-i pairs %d" % (hei.sum(), hii.sum()))
+    assert hei.sum() > 1000 and hii.sum() > 1000
+    assert np.array_equal(ei, 2 * hei) and np.array_equal(ii, hii)

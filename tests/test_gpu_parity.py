@@ -195,8 +195,8 @@ def test_rdf_matches_reference_and_oracle():
     assert np.array_equal(ei, oei) and np.array_equal(ii, oii)
     assert ei[2048:].sum() > ei[:2048].sum() > 0
     with _engine(theta=0.5, walk="mixed", theta_mixed=0.5) as eng:
-        with pytest.raises(ta.TreeError):
-            eng.rdf_enable(dx, dx)
+        with pytest.raises(ta.TreeError):  # the mixed walk's histograms live in LDS: at most 10240 bins
+            eng.rdf_enable(dx, dx, nbin=10241)
 
 
 def test_errors():

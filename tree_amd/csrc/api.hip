@@ -332,6 +332,8 @@ int check_cfg(const tb_config *c)
 constexpr int MISC_J = MISC_HIST + 2 * 2 * HW;  // joint tree: err, total, 2 x HW histograms
 constexpr int MISC_G = MISC_J + 2 + 2 * HW;      // per species: segments, target groups
 constexpr int MISC_N = MISC_G + 4;               // ints read back after phase A
+constexpr int MISC_RDF = 240;                    // the mixed-walk RDF's error flag (not read back)
+static_assert(MISC_N <= MISC_RDF, "d_misc layout");
 
 // A batched ensemble builds all its realisations as one combined state:
 // realisation r holds slice [off[k][r], off[k][r+1]) of species k's arrays,
@@ -563,6 +565,9 @@ int force_all(tb_handle *h, bool counting)
     if (h->cfg.walk == TB_WALK_MIXED && !counting) {
         int rc = tb_mixed_force(s, h->mx, h->sp, h->cfg, h->epart, h->d_sums);
         if (rc) return fail(rc, "%s", tb_mixed_error());
+        // the mixed walk's RDF: every pair within the histograms' range
+        if (h->rdf_on && (rc = tb_rdf_pairs(s, h->mx.t[0], h->sp[1], h->rdf, h->d_misc + MISC_RDF)))
+            return fail(rc, "mixed-walk RDF launch failed (nbin %d, ion groups of %d)", h->rdf.nbin, h->sp[1].gG);
     } else {
         if (counting) HIPCHK(hipMemsetAsync(h->d_cnt, 0, 16 * sizeof(int64_t), s));
         const int nb0 = tb_walk64_blocks(h->sp[0].n);
@@ -690,8 +695,9 @@ int tb_set_config(tb_handle *h, const tb_config *cfg)
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (cfg->device != h->cfg.device) return fail(TB_ERR_ARG, "device cannot change on a live handle");
-    if (h->rdf_on && cfg->walk != TB_WALK_REF64)
-        return fail(TB_ERR_STATE, "the RDF is accumulated by the FP64 walk: disable it before switching walks");
+    if (h->rdf_on && cfg->walk == TB_WALK_MIXED && h->rdf.nbin > TB_RDF32_MAXBIN)
+        return fail(TB_ERR_STATE, "the mixed walk's RDF holds at most %d bins: disable it before switching walks",
+                    TB_RDF32_MAXBIN);
     h->cfg = *cfg;
     h->built = false;  // rcrit and walk records depend on theta / walk
     return TB_OK;
@@ -1684,7 +1690,8 @@ int tb_rdf_enable(tb_handle *h, double dx_ei, double dx_ii, int32_t nbin)
     }
     if (!(dx_ei > 0.0) || !(dx_ii > 0.0) || nbin < 1)
         return fail(TB_ERR_ARG, "RDF needs dx_ei > 0, dx_ii > 0 and nbin >= 1");
-    if (h->cfg.walk != TB_WALK_REF64) return fail(TB_ERR_STATE, "the RDF is accumulated by the FP64 walk");
+    if (h->cfg.walk == TB_WALK_MIXED && nbin > TB_RDF32_MAXBIN)
+        return fail(TB_ERR_ARG, "the mixed walk's RDF holds at most %d bins (got %d)", TB_RDF32_MAXBIN, nbin);
     HIPCHK(hipSetDevice(h->cfg.device));
     HIPCHK(hipStreamSynchronize(h->s));
     dfree(h, h->rdf.hist, 0);
@@ -1693,6 +1700,7 @@ int tb_rdf_enable(tb_handle *h, double dx_ei, double dx_ii, int32_t nbin)
     int rc = dalloc_t(h, &h->rdf.hist, (size_t)2 * nbin);
     if (rc) return fail(rc, "RDF allocation failed");
     HIPCHK(hipMemsetAsync(h->rdf.hist, 0, (size_t)2 * nbin * sizeof(int64_t), h->s));
+    HIPCHK(hipMemsetAsync(h->d_misc + MISC_RDF, 0, sizeof(int32_t), h->s));
     HIPCHK(hipStreamSynchronize(h->s));
     h->rdf.dx[0] = dx_ei;
     h->rdf.dx[1] = dx_ii;
@@ -1709,8 +1717,11 @@ int tb_rdf_get(tb_handle *h, int32_t nbin, int64_t *rdf_ei, int64_t *rdf_ii, int
     HIPCHK(hipSetDevice(h->cfg.device));
     HIPCHK(hipMemcpyAsync(rdf_ei, h->rdf.hist, (size_t)nbin * 8, hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipMemcpyAsync(rdf_ii, h->rdf.hist + nbin, (size_t)nbin * 8, hipMemcpyDeviceToHost, h->s));
+    int32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, h->d_misc + MISC_RDF, sizeof(int32_t), hipMemcpyDeviceToHost, h->s));
     if (reset) HIPCHK(hipMemsetAsync(h->rdf.hist, 0, (size_t)2 * nbin * sizeof(int64_t), h->s));
     HIPCHK(hipStreamSynchronize(h->s));
+    if (flag) return fail(TB_ERR_HIP, "mixed-walk RDF: pair search stack overflow or malformed table (%d)", flag);
     return TB_OK;
 }
 

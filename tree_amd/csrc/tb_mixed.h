@@ -92,6 +92,11 @@ int tb_mixed_force_ens(hipStream_t s, MixedDev &M, SpeciesDev *sp, int nreal, co
                        const double *rs, const int32_t *const gfirst[2], const int32_t *const gcount[2],
                        const tb_config &cfg, double *esum_out_dev);
 
+// rdf32.hip: the mixed walk's RDF, every pair within nbin dx of an ion target
+// (nbin <= TB_RDF32_MAXBIN) added to R.hist; flag gets 1 (stack) / 2 (table)
+#define TB_RDF32_MAXBIN 10240
+int tb_rdf_pairs(hipStream_t s, const MixedTree &T, const SpeciesDev &I, const TbRdf &R, int32_t *flag);
+
 // walk-group size chosen by the autotuner for a problem size (targets per lane: 1, 2 or 4)
 int tb_autotune_tpl(int n_total);
 // targets per lane for a configuration: its group size, else the autotuner's choice

@@ -42,8 +42,9 @@ PROGRAM OCTREE_GPU
      CALL Vinit_REST(NS, qel, qion, param, ttime, sys, dt)
   END IF
 
-  ! the RDF accumulates from here (main.f90:118-120); the mixed walk keeps none
-  IF (walk == TB_WALK_REF64) CALL Engine_RDF_Init(NS)
+  ! the RDF accumulates from here (main.f90:118-120): the FP64 walk's leaf
+  ! pairs; the mixed walk's histograms hold every pair in their range
+  CALL Engine_RDF_Init(NS)
   ttime%Ndump = NINT(ttime%tdump/dt)
   ttime%Nrest = NINT(ttime%trest/dt)
   ttime%Nsamp = NINT(ttime%tsamp/dt)
@@ -58,7 +59,7 @@ PROGRAM OCTREE_GPU
      CALL VVerletNotemp1(NS, qel, qion, param, dt, ttime)
      CALL Force(NS, qel, qion, param, ttime)
      CALL VVerletNotemp2(NS, qel, qion, param, sys, dt, ttime)
-     IF (walk == TB_WALK_REF64 .AND. MOD(ttime%Nloop, ttime%Ndump) == 0) CALL RDF(NS, ttime)
+     IF (MOD(ttime%Nloop, ttime%Ndump) == 0) CALL RDF(NS, ttime)
      sys%Epot = sys%Epot_e + sys%Epot_i
      IF (MOD(ttime%Nloop, ttime%Nsamp) == 0) CALL TEMPORAL(NS, sys, ttime)
      IF (MOD(ttime%Nloop, ttime%Nrest) == 0) THEN
