@@ -123,7 +123,8 @@ def walk_flops32(st):
     for p in range(4):
         wp = W32_P2P_C if p in (0, 3) else W32_P2P_K
         fl += W32_M2P * st["m2p"][p] + wp * st["p2p"][p]
-    return fl + W32_M2L_PAIR * sum(st.get("far_pairs", [0, 0]))
+    # far cells: (cell, group) pairs, and (cell, segment) pairs of the two-level far field
+    return fl + W32_M2L_PAIR * (sum(st.get("far_pairs", [0, 0])) + sum(st.get("seg_pairs", [0, 0])))
 
 
 def profiled_traffic(kernels, path=os.path.join(ROOT, "profiles", "hbm_traffic_4M.md")):

@@ -76,6 +76,13 @@ typedef struct tb_config {
                        /* goes through the group's local expansion instead of     */
                        /* being evaluated at every target (0: default 3, < 0:   */
                        /* off, else >= 2)                                        */
+    int32_t far_levels;/* TB_WALK_MIXED with the far path: 0 or 1 (default) the   */
+                       /* group level only; 2: two levels, a segment of up to 8   */
+                       /* neighbouring groups walks the top of the tree once, and */
+                       /* cells far from all of it share the segment's local      */
+                       /* expansion (same acceptance, fewer visits and M2L pairs; */
+                       /* no faster at 4M, sigma 600 um, theta_mixed 0.27)        */
+    int32_t pad_;
 } tb_config;
 
 typedef struct tb_handle tb_handle;
@@ -171,6 +178,9 @@ typedef struct tb_stats {
                              /* group's local expansion (counted per target)    */
     int64_t far_pairs[2];    /* the same as (far cell, target group) pairs, by  */
                              /* target species                                  */
+    int64_t seg_pairs[2];    /* two-level far field: (far cell, segment) pairs,  */
+                             /* by target species (their (cell, target) pairs   */
+                             /* are in m2l)                                     */
 } tb_stats;
 /* counting pass: re-runs the last force with interaction counters */
 int tb_count_interactions(tb_handle *h, tb_stats *s);

@@ -142,3 +142,69 @@ def test_far_buffer_regrows(monkeypatch):
     assert sum(out[0][2]["m2l"]) > 0
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("mac", ["fixed", "learned"])
+def test_two_level_far_field(mac):
+    """far_levels 2 (segments of up to 8 groups walk the top of the tree once;
+    cells far from the whole segment go through the segment's local
+    expansion) against the group level alone: the same M2P, P2P and per-target
+    far counts (acceptance and route per target are unchanged), fewer cells
+    visited, forces within the local expansions' truncation of each other, and
+    no further from the FP64 direct sum.  The learned criterion runs a
+    network from tb_mlp_init (any network: both runs take the same decisions)."""
+    announce("two_level_far %s" % mac)
+    n = 1 << 16
+    qel, qion = ta.gen_plasma(35, n, n, 300.0)
+    rng = np.random.default_rng(6)
+    tsp = np.r_[np.zeros(512, np.int32), np.ones(512, np.int32)]
+    tix = np.r_[rng.choice(n, 512, replace=False), rng.choice(n, 512, replace=False)].astype(np.int32)
+    idx = np.where(tsp == 0, tix, n + tix)
+    out = {}
+    for lv in (1, 2):
+        with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.3, walk="mixed", mac=mac, group=128,
+                       far_levels=lv) as eng:
+            if mac == "learned":
+                eng.mlp_init(3)
+            eng.set_particles(qel, qion)
+            if lv == 1:
+                fd = eng.direct_sample(tsp, tix)
+            eng.build()
+            ep = eng.force()
+            out[lv] = (_forces(eng, n, n), ep, eng.count_interactions())
+    (f1, ep1, s1), (f2, ep2, s2) = out[1], out[2]
+    assert sum(s1["seg_pairs"]) == 0 and sum(s2["seg_pairs"]) > 0
+    assert s2["m2p"] == s1["m2p"] and s2["p2p"] == s1["p2p"] and s2["m2l"] == s1["m2l"], (s1, s2)
+    assert sum(s2["far_pairs"]) < sum(s1["far_pairs"])
+    assert sum(s2["visits"]) < sum(s1["visits"])
+    d = np.linalg.norm(f2 - f1, axis=1) / np.linalg.norm(f1, axis=1)
+    assert np.sqrt(np.mean(d ** 2)) < 2e-4, np.sqrt(np.mean(d ** 2))
+    assert abs(sum(ep2) - sum(ep1)) <= 1e-5 * abs(sum(ep1)), (ep1, ep2)
+    e = []
+    for f in (f1, f2):
+        rel = np.linalg.norm(f[idx] - fd, axis=1) / np.linalg.norm(fd, axis=1)
+        e.append(np.sqrt(np.mean(rel ** 2)))
+    assert e[1] < 1.1 * e[0], e
+
+
+def test_two_level_far_buffer_regrows(monkeypatch):
+    """With two levels the segments' frontier and far cells share the far-cell
+    chunk buffer: one chunk to start with is detected, grown and re-run,
+    bit-identical to a run whose buffer was large enough."""
+    announce("two_level_far_regrow")
+    n = 1 << 14
+    qel, qion = ta.gen_plasma(37, n, n, 200.0)
+    out = []
+    for chunks in (None, "1"):
+        if chunks:
+            monkeypatch.setenv("TREEMI_FAR_CHUNKS", chunks)
+        else:
+            monkeypatch.delenv("TREEMI_FAR_CHUNKS", raising=False)
+        with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.4, walk="mixed", far_levels=2) as eng:
+            eng.set_particles(qel, qion)
+            eng.build()
+            ep = eng.force()
+            out.append((_forces(eng, n, n), ep, eng.count_interactions()))
+    assert sum(out[0][2]["seg_pairs"]) > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]

@@ -164,6 +164,8 @@ void free_species(tb_handle *h, SpeciesDev &S)
     dfree(h, S.gpre, n);
     dfree(h, S.gsuf, n);
     dfree(h, S.gseg, n * 4);
+    dfree(h, S.sgrp, n * 8);
+    dfree(h, S.gsid, n * 4);
     dfree(h, S.grp, n * 8);
     size_t c = (size_t)S.cap;
     dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
@@ -192,6 +194,8 @@ int alloc_species(tb_handle *h, SpeciesDev &S, int n)
     if ((rc = dalloc_t(h, &S.gpre, n))) return rc;
     if ((rc = dalloc_t(h, &S.gsuf, n))) return rc;
     if ((rc = dalloc_t(h, &S.gseg, n))) return rc;
+    if ((rc = dalloc_t(h, &S.sgrp, n))) return rc;
+    if ((rc = dalloc_t(h, &S.gsid, n))) return rc;
     if ((rc = dalloc_t(h, &S.grp, n))) return rc;
     return TB_OK;
 }
@@ -323,6 +327,8 @@ int check_cfg(const tb_config *c)
         return fail(TB_ERR_ARG, "group must be 0, 64, 128 or 256 (got %d)", c->group);
     if (!std::isfinite(c->far_ratio) || (c->far_ratio > 0.0 && c->far_ratio < 2.0))
         return fail(TB_ERR_ARG, "far_ratio must be 0 (default), negative (off) or >= 2 (got %g)", c->far_ratio);
+    if (c->far_levels < 0 || c->far_levels > 2)
+        return fail(TB_ERR_ARG, "far_levels must be 0 or 1 (default, group level) or 2 (got %d)", c->far_levels);
     return TB_OK;
 }
 
@@ -437,7 +443,10 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         }
     }
     if (mixed) {
-        for (int sp = 0; sp < 2; sp++) h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];
+        for (int sp = 0; sp < 2; sp++) {
+            h->sp[sp].nseg = h->h_misc[MISC_G + 2 * sp];
+            h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];
+        }
         JointDev &J = h->jt;
         const int32_t *hist = h->h_misc + MISC_J + 2;
         J.nn = 1 + h->h_misc[MISC_J + 1];
@@ -526,7 +535,10 @@ int regroup(tb_handle *h, int tpl)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->h_misc + MISC_G, h->d_misc + MISC_G, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
-    for (int sp = 0; sp < 2; sp++) h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];
+    for (int sp = 0; sp < 2; sp++) {
+        h->sp[sp].nseg = h->h_misc[MISC_G + 2 * sp];
+        h->sp[sp].ng = h->h_misc[MISC_G + 2 * sp + 1];
+    }
     return TB_OK;
 }
 
@@ -569,7 +581,7 @@ int force_all(tb_handle *h, bool counting)
         if (h->rdf_on && (rc = tb_rdf_pairs(s, h->mx.t[0], h->sp[1], h->rdf, h->d_misc + MISC_RDF)))
             return fail(rc, "mixed-walk RDF launch failed (nbin %d, ion groups of %d)", h->rdf.nbin, h->sp[1].gG);
     } else {
-        if (counting) HIPCHK(hipMemsetAsync(h->d_cnt, 0, 16 * sizeof(int64_t), s));
+        if (counting) HIPCHK(hipMemsetAsync(h->d_cnt, 0, TB_NCNT * sizeof(int64_t), s));
         const int nb0 = tb_walk64_blocks(h->sp[0].n);
         for (int tsp = 0; tsp < 2; tsp++) {
             double *part = h->epart + (tsp == 0 ? 0 : 2 * nb0);
@@ -651,7 +663,7 @@ int tb_create(tb_handle **out, const tb_config *cfg)
         return fail(TB_ERR_HIP, "stream creation failed");
     }
     bool ok = dalloc_t(h, &h->bbox_tmp, 2048 * 6) == TB_OK && dalloc_t(h, &h->d_misc, 256) == TB_OK &&
-              dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, 16) == TB_OK;
+              dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, TB_NCNT) == TB_OK;
     ok = ok && hipHostMalloc((void **)&h->h_misc, 256 * sizeof(int32_t), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&h->h_sums, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     for (int k = 0; k < 6 && ok; k++) ok = hipEventCreate(&h->ev[k]) == hipSuccess;
@@ -1112,7 +1124,7 @@ int tb_count_interactions(tb_handle *h, tb_stats *st)
     if (!h || !st) return fail(TB_ERR_ARG, "null argument");
     if (!h->built) return fail(TB_ERR_STATE, "tb_count_interactions before tb_build");
     HIPCHK(hipSetDevice(h->cfg.device));
-    int64_t c[16] = {};
+    int64_t c[TB_NCNT] = {};
     if (h->cfg.walk == TB_WALK_MIXED) {
         int rc = tb_mixed_count(h->s, h->mx, h->sp, h->cfg, h->epart, h->d_sums, h->d_cnt);
         if (rc) return fail(rc, "%s", tb_mixed_error());
@@ -1135,6 +1147,9 @@ int tb_count_interactions(tb_handle *h, tb_stats *st)
     h->st.m2l[3] = c[13];
     h->st.far_pairs[0] = c[14];
     h->st.far_pairs[1] = c[15];
+    // two-level far field: (far cell, segment) pairs by target species
+    h->st.seg_pairs[0] = c[16];
+    h->st.seg_pairs[1] = c[17];
     *st = h->st;
     return TB_OK;
 }
@@ -1398,7 +1413,10 @@ int ens_regroup(tb_ensemble *E, int tpl)
     HIPCHK(hipMemcpyAsync(E->h_gfirst, E->eb.d_gfirst, 2 * E->nreal * sizeof(int32_t), hipMemcpyDeviceToHost,
                           C->s));
     HIPCHK(hipStreamSynchronize(C->s));
-    for (int k = 0; k < 2; k++) C->sp[k].ng = C->h_misc[MISC_G + 2 * k + 1];
+    for (int k = 0; k < 2; k++) {
+        C->sp[k].nseg = C->h_misc[MISC_G + 2 * k];
+        C->sp[k].ng = C->h_misc[MISC_G + 2 * k + 1];
+    }
     ens_groups_from_host(E);
     return TB_OK;
 }

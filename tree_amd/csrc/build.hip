@@ -911,7 +911,8 @@ __global__ __launch_bounds__(BT) void k_gcount(int n, int G, const int32_t *__re
 
 __global__ __launch_bounds__(BT) void k_gemit(int n, const int32_t *__restrict__ nseg,
                                               const int32_t *__restrict__ seg, const int32_t *__restrict__ cnt,
-                                              const int32_t *__restrict__ off, int2 *__restrict__ grp)
+                                              const int32_t *__restrict__ off, int2 *__restrict__ grp,
+                                              int2 *__restrict__ sgrp, int32_t *__restrict__ gsid)
 {
     const int k = blockIdx.x * BT + threadIdx.x;
     if (k >= n || k >= *nseg) return;
@@ -919,7 +920,9 @@ __global__ __launch_bounds__(BT) void k_gemit(int n, const int32_t *__restrict__
     for (int q = 0; q < p; q++) {
         const int a = st + (int)((long)q * m / p), b = st + (int)((long)(q + 1) * m / p);
         grp[o + q] = make_int2(a, b - a);
+        gsid[o + q] = k;
     }
+    sgrp[k] = make_int2(o, p);
 }
 
 // ----------------------------------------------- batched ensemble (segments)
@@ -1158,7 +1161,8 @@ void tb_groups(hipStream_t s, SpeciesDev &S, int G, int32_t *totals, void *scan_
     hipLaunchKernelGGL(k_gseg, dim3(nblk(n, BT)), dim3(BT), 0, s, n, S.cnt, S.offs, S.gseg);
     hipLaunchKernelGGL(k_gcount, dim3(nblk(n, BT)), dim3(BT), 0, s, n, G, totals, S.gseg, S.cnt);
     tb_scan_excl(s, n, S.cnt, S.offs, totals + 1, scan_tmp);
-    hipLaunchKernelGGL(k_gemit, dim3(nblk(n, BT)), dim3(BT), 0, s, n, totals, S.gseg, S.cnt, S.offs, S.grp);
+    hipLaunchKernelGGL(k_gemit, dim3(nblk(n, BT)), dim3(BT), 0, s, n, totals, S.gseg, S.cnt, S.offs, S.grp, S.sgrp,
+                       S.gsid);
     if (first)
         hipLaunchKernelGGL(k_gfirst, dim3(nblk(n, BT)), dim3(BT), 0, s, n, totals + 1, S.grp, S.skey,
                            T > 0 ? tagshift : 63, first);

@@ -10,6 +10,8 @@
 #define TB_LEVELS 21
 // Leaf marker in the preorder skip-pointer array.
 #define TB_LEAF 0x80000000u
+// Interaction counters of a counting pass (int64 each).
+#define TB_NCNT 32
 // Coulomb constant eV*um (data.f90:40).
 #define TB_EPS 0.0014399835661145824
 
@@ -45,6 +47,11 @@ struct SpeciesDev {
     int32_t *gseg = nullptr;    // segment starts
     int2 *grp = nullptr;        // group start, length
     int ng = 0, gG = 0;         // groups, and the size bound they were cut for
+    // segments (cells of at most 8 gG particles, each split into <= 8 groups):
+    // first group and group count per segment, segment of each group
+    int2 *sgrp = nullptr;
+    int32_t *gsid = nullptr;
+    int nseg = 0;
 };
 
 // The joint tree over both species (mixed walk): nodes in preorder, expansion

@@ -43,6 +43,8 @@ struct Domain {
     int gbase;        // first global group index of this domain
     int nn;           // cells in the tree
     uint32_t root;    // the domain's root cell (a batched ensemble's trees share one table)
+    int gfirst[2];    // combined index of the domain's first group per species (grp[k] = combined + gfirst[k])
+    const int32_t *gsid[2];  // combined: segment of each group
 };
 
 // list-length variants of the walk (walk32.hip LIST_M / LIST_P)
@@ -61,6 +63,12 @@ struct MixedDev {
     // far cells of the last walk: chunk lists per target group (k_far_eval)
     uint32_t *far_cells = nullptr;
     int32_t *far_link = nullptr, *far_head = nullptr;
+    // two-level far field, per segment: frontier and far-chunk heads, centre
+    // and half-diagonal, local expansion coefficients
+    int32_t *sfront = nullptr, *sfar = nullptr;
+    double4 *scen = nullptr;
+    float *sloc = nullptr;
+    int seg_cap = 0;
     float *gemin = nullptr;       // per target group: the learned walk's emin (kept for a re-run)
     int far_cap = 0, head_cap = 0;
     // the learned opening criterion

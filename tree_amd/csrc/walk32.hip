@@ -178,6 +178,20 @@ __device__ inline void nadd(float &s, float &c, float v)
     s = t;
 }
 
+// A group's box: half extents from its min / max, rounded outwards, and the
+// squared distance of a point (offset from the box centre) to the box.  The
+// walk and the segment walk form them alike (no contraction left to the
+// compiler), so their acceptance tests agree bit for bit.
+__device__ inline float half_extent(double mn, double mx) { return fmaf((float)(0.5 * (mx - mn)), 1.000001f, 1e-30f); }
+
+__device__ inline float box_d2(float cx, float cy, float cz, float hx, float hy, float hz)
+{
+    const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
+    const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
+    const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
+    return fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+}
+
 // 1 - e^{-u}(1+u) and 1 - e^{-u}, with series below u = 1/8 where the
 // subtraction would cancel; both round to 1 in FP32 from u = 22 on
 // (e^{-22} (1 + 22) < 2^-25, half an ulp below 1), which skips the
@@ -215,6 +229,17 @@ struct WalkArgs {
     int rec_cap;
     int32_t *ctr;         // [1] error flag, [2] prep error, [3] band pair count, [4] far chunks, [8..15] work chunks
     int flush_m, flush_p; // list lengths at which the M2P / P2P lists are evaluated (<= FLUSH_M / FLUSH_P)
+    // two-level far field: segments (super-groups of <= 8 groups inside one
+    // cell) walk the top of the tree once (k_swalk); their groups start from
+    // the segment's frontier, and cells far from the whole segment go through
+    // the segment's own local expansion
+    int two_level;
+    int nseg0, nsegt;     // electron segments, all segments
+    const int2 *sgrp[2];  // per species: segment -> first group, group count (combined indices)
+    const int2 *cgrp[2];  // per species: the combined target groups
+    int32_t *sfront;      // per segment: last frontier chunk (-1: none)
+    int32_t *sfar;        // per segment: last chunk of its far cells (-1: none)
+    double4 *scen;        // per segment: centre, half-diagonal
     double *gen;          // 2 per global group: energy of electron targets / of ion targets
     unsigned long long *cnt;
 };
@@ -502,6 +527,24 @@ struct FarSlab {
     int head = -1;          // the group's last chunk (lists run backwards)
 };
 
+__device__ inline void push_chunk(const uint32_t *q, int cnt, uint32_t *__restrict__ cells,
+                                  int32_t *__restrict__ link, int cap, int32_t *ctr, FarSlab &F)
+{
+    const int lane = lane_id();
+    if (F.next >= F.end) {
+        // every lane issues the atomic (lane 0 adds, the rest add 0): no lane-index branch
+        const int v = uni(atomicAdd(ctr, lane == 0 ? FAR_SLAB : 0));
+        F.next = v;
+        F.end = v + FAR_SLAB;
+    }
+    const int c = F.next++;
+    if (c < cap) {
+        cells[(size_t)c * 64 + lane] = lane < cnt ? q[lane] : 0xffffffffu;
+        link[c] = F.head;  // every lane stores the same value
+        F.head = c;
+    }
+}
+
 __device__ inline void push_far(const WaveLDS &L, int cnt, uint32_t *__restrict__ cells, int32_t *__restrict__ link,
                                 int cap, int32_t *ctr, FarSlab &F)
 {
@@ -617,9 +660,9 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         emin = uni(emin);
         const double gx = uni(0.5 * (mnx + mxx)), gy = uni(0.5 * (mny + mxy)), gz = uni(0.5 * (mnz + mxz));
         // half extents, rounded outwards
-        const float hx = uni((float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f);
-        const float hy = uni((float)(0.5 * (mxy - mny)) * 1.000001f + 1e-30f);
-        const float hz = uni((float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f);
+        const float hx = uni(half_extent(mnx, mxx));
+        const float hy = uni(half_extent(mny, mxy));
+        const float hz = uni(half_extent(mnz, mxz));
         // per-group terms of the learned criterion's features
         const float hdiag = uni(sqrtf(hx * hx + hy * hy + hz * hz));
         const float klemin = uni(0.125f * __log2f(fmaxf(emin, 1e-30f)));
@@ -644,36 +687,54 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
 
         // stack, M2P, P2P (like, unlike), band-queue, far-queue lengths
         int sp = 1, nm = 0, npl = 0, npu = 0, nb = 0, nf = 0, nx = 0;
+        // two-level: start from the segment's frontier (chunks of cells its
+        // segment walk left to the groups) instead of the root
+        int fc = -1;
+        if (W.two_level) {
+            const int gc = D.gfirst[tsp] + (tsp ? gl - D.ng[0] : gl);  // combined group index
+            fc = uni(W.sfront[D.gsid[tsp][gc] + (tsp ? W.nseg0 : 0)]);
+            sp = 0;
+        }
         L.stack[0] = D.root << 3;  // the domain's root: a range of one cell
         wave_sync();
-        while (sp > 0 || (MODE == MODE_LEARNED && nb > 0)) {
+        while (sp > 0 || fc >= 0 || (MODE == MODE_LEARNED && nb > 0)) {
             // learned: a pass either decides the queued band cells (once 64
-            // wait, or the stack is empty) or takes a traversal step; the
+            // wait, or nothing else is left) or takes a traversal step; the
             // flushes after it serve both, so a pass adds at most 64 entries
             // to each list
-            const bool decide = MODE == MODE_LEARNED && (nb >= 64 || (sp == 0 && nb > 0));
+            const bool decide = MODE == MODE_LEARNED && (nb >= 64 || (sp == 0 && fc < 0 && nb > 0));
             if (!decide) {
-                // a batch of <= 64 cells from the top ranges of the stack
-                uint32_t R = 0;
-                int len = 0;
-                if (lane < sp) {
-                    R = L.stack[sp - 1 - lane];
-                    len = (int)(R & 7u) + 1;
+                bool have;
+                uint32_t node;
+                if (sp > 0) {
+                    // a batch of <= 64 cells from the top ranges of the stack
+                    uint32_t R = 0;
+                    int len = 0;
+                    if (lane < sp) {
+                        R = L.stack[sp - 1 - lane];
+                        len = (int)(R & 7u) + 1;
+                    }
+                    const int incl = wave_incl_scan(len);
+                    const bool take = lane < sp && incl <= 64;
+                    const int K = __popcll(__ballot(take));
+                    const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
+                    if (take) {
+                        const uint32_t st = R >> 3;
+                        const int ex = incl - len;
+                        for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
+                    }
+                    sp -= K;
+                    wave_sync();
+                    have = lane < total;
+                    node = have ? L.batch[lane] : 0u;
+                    wave_sync();
+                } else {
+                    // the next frontier chunk: up to 64 cells
+                    const uint32_t v = W.far_cells[(size_t)fc * 64 + lane];
+                    have = v != 0xffffffffu;
+                    node = have ? v : 0u;
+                    fc = uni(W.far_link[fc]);
                 }
-                const int incl = wave_incl_scan(len);
-                const bool take = lane < sp && incl <= 64;
-                const int K = __popcll(__ballot(take));
-                const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
-                if (take) {
-                    const uint32_t st = R >> 3;
-                    const int ex = incl - len;
-                    for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
-                }
-                sp -= K;
-                wave_sync();
-                const bool have = lane < total;
-                const uint32_t node = have ? L.batch[lane] : 0u;
-                wave_sync();
                 bool leaf = false, acc = false, open = false, band = false;
                 float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
                 uint32_t info = 0;
@@ -686,10 +747,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     if (info & TB_LEAF) {
                         leaf = true;
                     } else {
-                        const float ddx = fmaxf(fabsf(cx) - hx, 0.0f);
-                        const float ddy = fmaxf(fabsf(cy) - hy, 0.0f);
-                        const float ddz = fmaxf(fabsf(cz) - hz, 0.0f);
-                        d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        d2 = box_d2(cx, cy, cz, hx, hy, hz);
                         const float rc = (float)m.w;  // s / theta_mixed
                         const bool far_hi = d2 > rc * rc * 1.000004f;
                         if (D.bq2[node].z <= (float)SMALL_OPEN) {
@@ -963,12 +1021,256 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
 // about the group's centre (the centre the walk used: the middle of the
 // targets' bounding box), summed across the wavefront and evaluated at each
 // target; field and energy are added to the walk's.
+// ------------------------------------------------------ segment walk
+//
+// One wavefront per segment S (a cell of at most 8 G targets of one species,
+// cut into at most 8 groups).  It walks the top of the tree once for all its
+// groups, with every group's own acceptance test:
+//   - a cell that every group of S would open (its test fails for each group
+//     box) is opened here;
+//   - a cell far from the whole of S (accepted for S's box, which implies
+//     every group accepts it, and beyond K times S's half-diagonal from S's
+//     centre) goes to S's far list: its field comes from S's own local
+//     expansion, evaluated at each of S's targets (k_far_eval);
+//   - every other cell (some group may accept it, or a leaf, or a small cell
+//     the groups open themselves) goes to S's frontier, where the groups'
+//     walks start.
+// So each group accepts and opens exactly the cells it would from the root,
+// except that cells far from all of S share one expansion; the local
+// expansion's truncation bound is the group level's, (1/K)^7 of the cell's
+// field at the targets, with S's half-diagonal for the group's.
+struct SegLDS {
+    uint32_t stack[STACK];
+    uint32_t batch[64];
+    uint32_t qf[QCAP];      // frontier cells waiting for a chunk
+    uint32_t qs[QCAP];      // far cells waiting for a chunk
+    double gc[8][3];        // group centres
+    float gh[8][3];         // group half extents
+};
+
+template <int MODE, bool COUNT>
+__global__ __launch_bounds__(BT) void k_swalk(WalkArgs W)
+{
+    __shared__ SegLDS lds[WPB];
+    const int lane = lane_id();
+    SegLDS &L = lds[threadIdx.x >> 6];
+    unsigned long long c_pairs[2] = {0, 0}, c_tgt[2] = {0, 0};
+    for (int sg = uni((int)(blockIdx.x * WPB + (threadIdx.x >> 6))); sg < W.nsegt; sg += gridDim.x * WPB) {
+        const int k = sg < W.nseg0 ? 0 : 1;
+        const int2 sr = W.sgrp[k][sg - (k ? W.nseg0 : 0)];
+        const int g0 = uni(sr.x), ng = uni(sr.y);
+        // the domain holding the segment (its groups' combined indices)
+        int lo = 0, hi = W.ndom - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (W.dom[mid].gfirst[k] <= g0) lo = mid;
+            else hi = mid - 1;
+        }
+        const Domain &D = W.dom[lo];
+        const double *px = D.px[k], *py = D.py[k], *pz = D.pz[k];
+        // each group's box, formed as the walk forms it; the segment's box
+        double smn[3] = {1e300, 1e300, 1e300}, smx[3] = {-1e300, -1e300, -1e300};
+        int ntgt = 0;
+        for (int j = 0; j < ng; j++) {
+            const int2 gr = W.cgrp[k][g0 + j];
+            const int base = uni(gr.x), nact = uni(gr.y);
+            ntgt += nact;
+            double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+            for (int t = 0; t < 4; t++) {
+                const int ii = lane + 64 * t < nact ? base + lane + 64 * t : base;
+                const double v[3] = {px[ii], py[ii], pz[ii]};
+                for (int a = 0; a < 3; a++) {
+                    mn[a] = fmin(mn[a], v[a]);
+                    mx[a] = fmax(mx[a], v[a]);
+                }
+            }
+            for (int a = 0; a < 3; a++) {
+                mn[a] = wave_min(mn[a]);
+                mx[a] = wave_max(mx[a]);
+                smn[a] = fmin(smn[a], mn[a]);
+                smx[a] = fmax(smx[a], mx[a]);
+                const double cen = uni(0.5 * (mn[a] + mx[a]));
+                const float he = uni(half_extent(mn[a], mx[a]));
+                L.gc[j][a] = cen;  // every lane stores the same value
+                L.gh[j][a] = he;
+            }
+        }
+        const double sx = uni(0.5 * (smn[0] + smx[0])), sy = uni(0.5 * (smn[1] + smx[1])),
+                     sz = uni(0.5 * (smn[2] + smx[2]));
+        const float shx = uni(half_extent(smn[0], smx[0])), shy = uni(half_extent(smn[1], smx[1])),
+                    shz = uni(half_extent(smn[2], smx[2]));
+        const float sdiag = uni(sqrtf(shx * shx + shy * shy + shz * shz));
+        const float far_r2 = W.far_k2 * sdiag * sdiag;
+        W.scen[sg] = make_double4(sx, sy, sz, (double)sdiag);  // every lane stores the same value
+        FarSlab Ff, Fs;
+        Ff.head = Fs.head = -1;
+        int sp = 1, nq = 0, ns = 0;
+        L.stack[0] = D.root << 3;
+        wave_sync();
+        while (sp > 0) {
+            uint32_t R = 0;
+            int len = 0;
+            if (lane < sp) {
+                R = L.stack[sp - 1 - lane];
+                len = (int)(R & 7u) + 1;
+            }
+            const int incl = wave_incl_scan(len);
+            const bool take = lane < sp && incl <= 64;
+            const int K = __popcll(__ballot(take));
+            const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
+            if (take)
+                for (int o = 0; o < len; o++) L.batch[incl - len + o] = (R >> 3) + (uint32_t)o;
+            sp -= K;
+            wave_sync();
+            const bool have = lane < total;
+            const uint32_t node = have ? L.batch[lane] : 0u;
+            wave_sync();
+            bool front = false, sfar = false, open = false;
+            uint32_t info = 0;
+            if (have) {
+                const double4 m = D.bmac[node];
+                info = D.binfo[node];
+                if ((info & TB_LEAF) || D.bq2[node].z <= (float)SMALL_OPEN) {
+                    front = true;
+                } else {
+                    const float rc = (float)m.w;  // s / theta_mixed
+                    // the radius beyond which every group accepts outright
+                    const float ra = MODE == MODE_FIXED ? rc : rc * W.lo_scale;
+                    const float cx = (float)(m.x - sx), cy = (float)(m.y - sy), cz = (float)(m.z - sz);
+                    if (box_d2(cx, cy, cz, shx, shy, shz) > ra * ra * 1.000004f &&
+                        cx * cx + cy * cy + cz * cz > far_r2) {
+                        sfar = true;
+                    } else {
+                        bool any = false;
+                        for (int j = 0; j < ng; j++) {
+                            const float gx = (float)(m.x - L.gc[j][0]), gy = (float)(m.y - L.gc[j][1]),
+                                        gz = (float)(m.z - L.gc[j][2]);
+                            any = any || box_d2(gx, gy, gz, L.gh[j][0], L.gh[j][1], L.gh[j][2]) >
+                                             rc * rc * 1.000004f;
+                        }
+                        if (any) front = true;
+                        else open = true;
+                    }
+                }
+            }
+            if (open) {
+                const uint32_t st = info >> 3, n = (info & 7u) + 1u;
+                if (st <= node || st + n > (uint32_t)D.nn) {
+                    atomicOr(&W.ctr[1], 2);
+                    open = false;
+                }
+            }
+            const uint64_t om = __ballot(open);
+            const int npush = __popcll(om);
+            if (sp + npush > STACK) {
+                atomicOr(&W.ctr[1], 1);
+                break;
+            }
+            if (open) L.stack[sp + __popcll(om & lanemask_lt())] = info;
+            sp += npush;
+            const uint64_t fm = __ballot(front), sm = __ballot(sfar);
+            if (front) L.qf[nq + __popcll(fm & lanemask_lt())] = node;
+            if (sfar) L.qs[ns + __popcll(sm & lanemask_lt())] = node;
+            nq += __popcll(fm);
+            ns += __popcll(sm);
+            if (COUNT) {
+                c_pairs[k] += (unsigned long long)__popcll(sm);
+                c_tgt[k] += (unsigned long long)__popcll(sm) * ntgt;
+            }
+            wave_sync();
+            for (int pass = 0; pass < 2; pass++) {
+                uint32_t *q = pass ? L.qs : L.qf;
+                int &n = pass ? ns : nq;
+                FarSlab &F = pass ? Fs : Ff;
+                if (n >= 64) {
+                    push_chunk(q, 64, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
+                    const uint32_t rest = lane + 64 < n ? q[lane + 64] : 0u;
+                    wave_sync();
+                    if (lane + 64 < n) q[lane] = rest;
+                    n -= 64;
+                    wave_sync();
+                }
+            }
+        }
+        if (nq > 0) push_chunk(L.qf, nq, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, Ff);
+        if (ns > 0) push_chunk(L.qs, ns, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, Fs);
+        W.sfront[sg] = Ff.head;  // every lane stores the same value
+        W.sfar[sg] = Fs.head;
+        wave_sync();
+    }
+    // segment far cells: (cell, segment) pairs [16, 17] and (cell, target) pairs into [12, 13]
+    if (COUNT && lane == 0) {
+        atomicAdd(&W.cnt[16], c_pairs[0]);
+        atomicAdd(&W.cnt[17], c_pairs[1]);
+        atomicAdd(&W.cnt[12], c_tgt[0]);
+        atomicAdd(&W.cnt[13], c_tgt[1]);
+    }
+}
+
+// each segment's far cells into its local expansion about its centre
+struct SegFarArgs {
+    const Domain *dom;
+    int ndom, nseg0, nsegt;
+    const int2 *sgrp[2];
+    const uint32_t *far_cells;
+    const int32_t *far_link, *sfar;
+    const double4 *scen;
+    float *sloc;
+};
+
+__global__ __launch_bounds__(BT) void k_sfar_eval(SegFarArgs A)
+{
+    const int lane = lane_id();
+    const int sg = uni((int)(blockIdx.x * WPB + (threadIdx.x >> 6)));
+    if (sg >= A.nsegt) return;
+    const int k = sg < A.nseg0 ? 0 : 1;
+    const int g0 = uni(A.sgrp[k][sg - (k ? A.nseg0 : 0)].x);
+    int lo = 0, hi = A.ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.dom[mid].gfirst[k] <= g0) lo = mid;
+        else hi = mid - 1;
+    }
+    const Domain &D = A.dom[lo];
+    const double4 sc = A.scen[sg];
+    float Lp[NLOC];
+#pragma unroll
+    for (int q = 0; q < NLOC; q++) Lp[q] = 0.0f;
+    for (int c = uni(A.sfar[sg]); c >= 0; c = uni(A.far_link[c])) {
+        const uint32_t nd = A.far_cells[(size_t)c * 64 + lane];
+        float rx = 1.0f, ry = 0.0f, rz = 0.0f;
+        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0;
+        if (nd != 0xffffffffu) {
+            const double4 m = D.bmac[nd];
+            rx = (float)(sc.x - m.x);
+            ry = (float)(sc.y - m.y);
+            rz = (float)(sc.z - m.z);
+            q0 = D.bq0[nd];
+            q1 = D.bq1[nd];
+            q2 = D.bq2[nd];
+            q3 = D.bq3[nd];
+            q4 = D.bq4[nd];
+        }
+        m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, Lp);
+    }
+#pragma unroll
+    for (int q = 0; q < NLOC; q++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) Lp[q] += __shfl_xor(Lp[q], o, 64);
+        A.sloc[(size_t)sg * NLOC + q] = Lp[q];  // every lane stores the same value
+    }
+}
+
 struct FarArgs {
     const Domain *dom;
     int ndom, ngt;
     const uint32_t *far_cells;
     const int32_t *far_link, *far_head;
     double *gen;
+    // two-level: each target also evaluates its segment's expansion
+    int two_level, nseg0;
+    const double4 *scen;
+    const float *sloc;
 };
 
 __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
@@ -977,7 +1279,7 @@ __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
     const int g = uni((int)(blockIdx.x * WPB + (threadIdx.x >> 6)));
     if (g >= A.ngt) return;
     int c = uni(A.far_head[g]);
-    if (c < 0) return;
+    if (c < 0 && !A.two_level) return;
     int lo = 0, hi = A.ndom - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -1034,6 +1336,23 @@ __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
         D.fy[tsp][i] += ff * (double)ey;
         D.fz[tsp][i] += ff * (double)ez;
         eg += 0.5 * ff * (double)phi;
+    }
+    if (A.two_level) {
+        // the segment's expansion (cells far from every group of the segment)
+        const int gc = D.gfirst[tsp] + (tsp ? gl - D.ng[0] : gl);
+        const int sg = uni(D.gsid[tsp][gc] + (tsp ? A.nseg0 : 0));
+        const double4 sc = A.scen[sg];
+#pragma unroll
+        for (int q = 0; q < NLOC; q++) Lp[q] = A.sloc[(size_t)sg * NLOC + q];
+        for (int t = lane; t < gr.y; t += 64) {
+            const int i = gr.x + t;
+            float phi, ex, ey, ez;
+            l2p(Lp, (float)(px[i] - sc.x), (float)(py[i] - sc.y), (float)(pz[i] - sc.z), phi, ex, ey, ez);
+            D.fx[tsp][i] += ff * (double)ex;
+            D.fy[tsp][i] += ff * (double)ey;
+            D.fz[tsp][i] += ff * (double)ez;
+            eg += 0.5 * ff * (double)phi;
+        }
     }
     eg = __shfl(wave_sum(eg), 0, 64);
     A.gen[2 * g + tsp] += eg;  // every lane: the same value to the same address
@@ -1317,6 +1636,10 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int g
     d.gbase = gbase;
     d.nn = t->nn;
     d.root = 0;
+    for (int k = 0; k < 2; k++) {
+        d.gfirst[k] = 0;
+        d.gsid[k] = sp[k]->gsid;
+    }
     return d;
 }
 
@@ -1343,8 +1666,38 @@ int grow_far(MixedDev &M, long chunks)
 
 // Uploads the domain descriptors (synchronously: they change only after a
 // build, which leaves the stream idle) and launches one persistent walk.
+// per-segment buffers of the two-level far field for nseg segments
+int grow_seg(MixedDev &M, int nseg)
+{
+    if (nseg <= M.seg_cap) return TB_OK;
+    hipFree(M.sfront); hipFree(M.sfar); hipFree(M.scen); hipFree(M.sloc);
+    M.sfront = M.sfar = nullptr;
+    M.scen = nullptr;
+    M.sloc = nullptr;
+    M.seg_cap = 0;
+    const int cap = nseg + nseg / 8 + 64;
+    if (hipMalloc((void **)&M.sfront, (size_t)cap * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&M.sfar, (size_t)cap * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&M.scen, (size_t)cap * sizeof(double4)) != hipSuccess ||
+        hipMalloc((void **)&M.sloc, (size_t)cap * NLOC * sizeof(float)) != hipSuccess) {
+        g_err = "mixed walk: segment buffers allocation failed";
+        return TB_ERR_NOMEM;
+    }
+    M.seg_cap = cap;
+    return TB_OK;
+}
+
+template <int MODE, bool COUNT>
+void dispatch_seg(hipStream_t s, int nsegt, const WalkArgs &W)
+{
+    const int blocks = std::max(1, std::min(nblk(nsegt, WPB), nblk(resident_waves(), WPB)));
+    hipLaunchKernelGGL((k_swalk<MODE, COUNT>), dim3(blocks), dim3(BT), 0, s, W);
+}
+
+// sp: the species arrays the domains' groups and segments index (a batched
+// ensemble's combined arrays)
 int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, int tpl, const tb_config &cfg,
-                   int64_t *d_cnt, int mode)
+                   int64_t *d_cnt, int mode, const SpeciesDev *sp)
 {
     const int nd = (int)doms.size();
     if (nd > M.ddom_cap) {
@@ -1416,8 +1769,25 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     // each time plus every wave's partly used slab.  A walk that asked for
     // more than the buffer holds is run again with a larger one (its far
     // lists are incomplete; the near forces it wrote are overwritten).
-    const long slack = (long)blocks * WPB * FAR_SLAB;
     const bool far_on = W.far_k2 > 0.0f;
+    // two-level far field: segments walk the top of the tree for their groups
+    const bool two = far_on && mode != MODE_COLLECT && cfg.far_levels == 2 && sp && sp[0].nseg > 0 && sp[1].nseg > 0;
+    W.two_level = two ? 1 : 0;
+    W.nseg0 = two ? sp[0].nseg : 0;
+    W.nsegt = two ? sp[0].nseg + sp[1].nseg : 0;
+    for (int k = 0; k < 2; k++) {
+        W.sgrp[k] = sp ? sp[k].sgrp : nullptr;
+        W.cgrp[k] = sp ? sp[k].grp : nullptr;
+    }
+    if (two) {
+        const int rc = grow_seg(M, W.nsegt);
+        if (rc) return rc;
+    }
+    W.sfront = M.sfront;
+    W.sfar = M.sfar;
+    W.scen = M.scen;
+    // every wave of the segment walk and of the group walk may hold a part-used slab
+    const long slack = (long)(blocks + (two ? nblk(std::min(W.nsegt, resident_waves()), WPB) : 0)) * WPB * FAR_SLAB;
     if (far_on && M.far_cap == 0) {
         // TREEMI_FAR_CHUNKS overrides the first guess (tests use it to take the re-run path)
         const char *env = getenv("TREEMI_FAR_CHUNKS");
@@ -1436,6 +1806,15 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_FAR, CTR_FAR + 1);
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, CTR_CHUNK, CTR_CHUNK + NXCD);
         hipEventRecord(M.ev[0], s);
+        if (two) {
+            if (mode == MODE_LEARNED) {
+                if (d_cnt) dispatch_seg<MODE_LEARNED, true>(s, W.nsegt, W);
+                else dispatch_seg<MODE_LEARNED, false>(s, W.nsegt, W);
+            } else {
+                if (d_cnt) dispatch_seg<MODE_FIXED, true>(s, W.nsegt, W);
+                else dispatch_seg<MODE_FIXED, false>(s, W.nsegt, W);
+            }
+        }
         if (mode == MODE_COLLECT) {
             hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 3, 4);
             dispatch<MODE_COLLECT, false>(s, tpl, blocks, W);
@@ -1461,6 +1840,20 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         const int rc = grow_far(M, (long)used + used / 4 + slack);
         if (rc) return rc;
     }
+    if (two) {
+        SegFarArgs SA;
+        SA.dom = M.ddom;
+        SA.ndom = nd;
+        SA.nseg0 = W.nseg0;
+        SA.nsegt = W.nsegt;
+        for (int k = 0; k < 2; k++) SA.sgrp[k] = W.sgrp[k];
+        SA.far_cells = M.far_cells;
+        SA.far_link = M.far_link;
+        SA.sfar = M.sfar;
+        SA.scen = M.scen;
+        SA.sloc = M.sloc;
+        hipLaunchKernelGGL(k_sfar_eval, dim3(nblk(W.nsegt, WPB)), dim3(BT), 0, s, SA);
+    }
     if (far_on) {
         FarArgs FA;
         FA.dom = M.ddom;
@@ -1470,6 +1863,10 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
         FA.far_link = M.far_link;
         FA.far_head = M.far_head;
         FA.gen = W.gen;
+        FA.two_level = W.two_level;
+        FA.nseg0 = W.nseg0;
+        FA.scen = M.scen;
+        FA.sloc = M.sloc;
         hipLaunchKernelGGL(k_far_eval, dim3(nblk(ngt, WPB)), dim3(BT), 0, s, FA);
     }
     hipEventRecord(M.ev[1], s);
@@ -1488,7 +1885,7 @@ int launch(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, dou
     if (!tpl) return TB_ERR_STATE;
     SpeciesDev *const spp[2] = {&sp[0], &sp[1]};
     std::vector<Domain> doms(1, make_domain(spp, &M.t[0], cfg.rs, 0));
-    int rc = launch_domains(s, M, doms, tpl, cfg, d_cnt, mode);
+    int rc = launch_domains(s, M, doms, tpl, cfg, d_cnt, mode, sp);
     if (rc) return rc;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, M.esum, M.ctr, d_sums);
     return hipGetLastError() == hipSuccess ? TB_OK : TB_ERR_HIP;
@@ -1549,7 +1946,7 @@ int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &
 int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums,
                    int64_t *d_cnt)
 {
-    hipMemsetAsync(d_cnt, 0, 16 * sizeof(int64_t), s);
+    hipMemsetAsync(d_cnt, 0, TB_NCNT * sizeof(int64_t), s);
     return launch(s, M, sp, cfg, d_sums, d_cnt, walk_mode(M, cfg));
 }
 
@@ -1630,12 +2027,13 @@ int tb_mixed_force_ens(hipStream_t s, MixedDev &M, SpeciesDev *sp, int nreal, co
         for (int k = 0; k < 2; k++) {
             d.grp[k] = sp[k].grp + gfirst[k][r];
             d.ng[k] = gcount[k][r];
+            d.gfirst[k] = gfirst[k][r];
         }
         d.root = roots[r];
         doms.push_back(d);
         gb += d.ng[0] + d.ng[1];
     }
-    int rc = launch_domains(s, M, doms, tpl, cfg, nullptr, MODE_FIXED);
+    int rc = launch_domains(s, M, doms, tpl, cfg, nullptr, MODE_FIXED, sp);
     if (rc) return rc;
     if (esum_out_dev &&
         hipMemcpyAsync(esum_out_dev, M.esum, (size_t)4 * nreal * sizeof(double), hipMemcpyDeviceToDevice, s) !=
@@ -1657,6 +2055,11 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
     if (M.far_head) hipFree(M.far_head);
     if (M.gemin) hipFree(M.gemin);
     M.gemin = nullptr;
+    hipFree(M.sfront); hipFree(M.sfar); hipFree(M.scen); hipFree(M.sloc);
+    M.sfront = M.sfar = nullptr;
+    M.scen = nullptr;
+    M.sloc = nullptr;
+    M.seg_cap = 0;
     M.far_cells = nullptr;
     M.far_link = M.far_head = nullptr;
     M.far_cap = M.head_cap = 0;

@@ -59,6 +59,8 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("mem_cap", ctypes.c_int64),
         ("far_ratio", ctypes.c_double),
+        ("far_levels", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 
@@ -76,6 +78,7 @@ class Stats(ctypes.Structure):
         ("ms_walk", ctypes.c_double),
         ("m2l", ctypes.c_int64 * 4),
         ("far_pairs", ctypes.c_int64 * 2),
+        ("seg_pairs", ctypes.c_int64 * 2),
     ]
 
 
@@ -173,7 +176,7 @@ class Engine:
     """One resident domain on one GPU (a tb_handle)."""
 
     def __init__(self, rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed",
-                 theta_lo=0.0, err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0):
+                 theta_lo=0.0, err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0, far_levels=0):
         self.lib = load_library()
         self.cfg = Config()
         self.cfg.xm[0], self.cfg.xm[1] = xm
@@ -188,6 +191,7 @@ class Engine:
         self.cfg.device = device
         self.cfg.mem_cap = mem_cap
         self.cfg.far_ratio = far_ratio
+        self.cfg.far_levels = far_levels
         h = ctypes.c_void_p()
         _check(self.lib.tb_create(ctypes.byref(h), ctypes.byref(self.cfg)))
         self.h = h
@@ -384,12 +388,13 @@ class Engine:
 
 
 def make_config(rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed", theta_lo=0.0,
-                err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0):
+                err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0, far_levels=0):
     c = Config()
     c.xm[0], c.xm[1] = xm
     c.rs, c.theta, c.theta_mixed, c.theta_lo, c.err_target = rs, theta, theta_mixed, theta_lo, err_target
     c.walk, c.mac, c.group, c.device, c.mem_cap = WALK[walk], MAC[mac], group, device, mem_cap
     c.far_ratio = far_ratio
+    c.far_levels = far_levels
     return c
 
 
@@ -462,7 +467,7 @@ def _stats_dict(s):
         "nodes": list(s.nodes), "max_depth": list(s.max_depth),
         "visits": list(s.visits), "m2p": list(s.m2p), "p2p": list(s.p2p),
         "ms_build": s.ms_build, "ms_force": s.ms_force, "ms_vv": s.ms_vv, "ms_walk": s.ms_walk,
-        "m2l": list(s.m2l), "far_pairs": list(s.far_pairs),
+        "m2l": list(s.m2l), "far_pairs": list(s.far_pairs), "seg_pairs": list(s.seg_pairs),
     }
 
 

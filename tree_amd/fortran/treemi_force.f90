@@ -38,6 +38,8 @@ CONTAINS
     cfg%device = 0
     cfg%mem_cap = 0
     cfg%far_ratio = 0.0_C_DOUBLE
+    cfg%far_levels = 0
+    cfg%pad = 0
     CALL tb_check(tb_create(hnd, cfg), "Engine_Init")
   END SUBROUTINE Engine_Init
 
