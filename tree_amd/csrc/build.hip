@@ -513,10 +513,13 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
         }
         c = nc;
     }
+    // a child's qeff is sign * count (what k_emit and this kernel store in
+    // nmom[0]), read from the dense count array rather than the 80-byte
+    // moment records
     double qsum = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
     for (uint32_t c = p + 1; c < endp; c = nnext[c] & ~TB_LEAF) {
         double4 m = nmac[c];
-        double q = nmom[(long)c * 10];
+        double q = sign * (double)ncount[c];
         qsum = qsum + q;
         X = X + q * m.x;
         Y = Y + q * m.y;
@@ -529,9 +532,10 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
     double dx = sqrt(ax * ax + ay * ay + az * az);
     double D0 = 0.0, D1 = 0.0, D2 = 0.0, Q0 = 0.0, Q1 = 0.0, Q2 = 0.0, R0 = 0.0, R1 = 0.0, R2 = 0.0;
     double M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t c = p + 1; c < endp; c = nnext[c] & ~TB_LEAF) {
+    for (uint32_t c = p + 1; c < endp;) {
+        const uint32_t nxr = nnext[c];
         double4 m = nmac[c];
-        double q = nmom[(long)c * 10];
+        double q = sign * (double)ncount[c];
         double d0 = m.x - X, d1 = m.y - Y, d2 = m.z - Z;
         D0 = D0 + q * d0;
         D1 = D1 + q * d1;
@@ -542,13 +546,24 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
         R0 = R0 + q * d0 * d1;
         R1 = R1 + q * d1 * d2;
         R2 = R2 + q * d2 * d0;
-        const double *cm = nm2 + (long)c * 6;
-        M[0] = M[0] + (cm[0] + q * (d0 * d0));
-        M[1] = M[1] + (cm[1] + q * (d1 * d1));
-        M[2] = M[2] + (cm[2] + q * (d2 * d2));
-        M[3] = M[3] + (cm[3] + q * (d0 * d1));
-        M[4] = M[4] + (cm[4] + q * (d1 * d2));
-        M[5] = M[5] + (cm[5] + q * (d2 * d0));
+        if (nxr & TB_LEAF) {
+            // a leaf's second moments are zero: cm + x is x exactly
+            M[0] = M[0] + q * (d0 * d0);
+            M[1] = M[1] + q * (d1 * d1);
+            M[2] = M[2] + q * (d2 * d2);
+            M[3] = M[3] + q * (d0 * d1);
+            M[4] = M[4] + q * (d1 * d2);
+            M[5] = M[5] + q * (d2 * d0);
+        } else {
+            const double *cm = nm2 + (long)c * 6;
+            M[0] = M[0] + (cm[0] + q * (d0 * d0));
+            M[1] = M[1] + (cm[1] + q * (d1 * d1));
+            M[2] = M[2] + (cm[2] + q * (d2 * d2));
+            M[3] = M[3] + (cm[3] + q * (d0 * d1));
+            M[4] = M[4] + (cm[4] + q * (d1 * d2));
+            M[5] = M[5] + (cm[5] + q * (d2 * d0));
+        }
+        c = nxr & ~TB_LEAF;
     }
     double s = lx;
     if (ly > s) s = ly;
@@ -699,12 +714,10 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
         ncount[id] = end - i;
         uint32_t nxt = end >= n ? (uint32_t)total : (uint32_t)(offs[end] + 1);
         if (d == leafd) {
+            // a leaf: its particle's position and charge; it has no moment
+            // record (nq of a leaf is never read)
             nnext[id] = nxt | TB_LEAF;
-            const double4 p = pos[i];
-            nc[id] = make_double4(p.x, p.y, p.z, 0.0);
-            double *q = nq + (size_t)id * TB_JQ;
-            q[0] = p.w;
-            for (int c = 1; c < TB_JQ; c++) q[c] = 0.0;
+            nc[id] = pos[i];
         } else {
             nnext[id] = nxt;
         }
@@ -752,14 +765,35 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
     double q = 0.0, D[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double O[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t c = p + 1; c < endp;) {
-        const uint32_t nxc = nnext[c] & ~TB_LEAF;
+        const uint32_t nxr = nnext[c], nxc = nxr & ~TB_LEAF;
         if (nxc <= c) {
             atomicOr(err, 8);
             return;
         }
         const double4 cc = nc[c];
-        const double *cq = nq + (size_t)c * TB_JQ;
         const double a = cc.x - ox, b = cc.y - oy, e = cc.z - oz;
+        if (nxr & TB_LEAF) {
+            // a particle (charge cc.w): the shift below with D = M = O = 0,
+            // term for term (the dropped terms are exact zeros)
+            const double qc = cc.w, av[3] = {a, b, e};
+            const int oi[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                   {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+#pragma unroll
+            for (int t = 0; t < 10; t++) O[t] += qc * av[oi[t][0]] * av[oi[t][1]] * av[oi[t][2]];
+            q += qc;
+            D[0] += qc * a;
+            D[1] += qc * b;
+            D[2] += qc * e;
+            M[0] += qc * a * a;
+            M[1] += qc * b * b;
+            M[2] += qc * e * e;
+            M[3] += qc * a * b;
+            M[4] += qc * b * e;
+            M[5] += qc * e * a;
+            c = nxc;
+            continue;
+        }
+        const double *cq = nq + (size_t)c * TB_JQ;
         const double qc = cq[0], dx = cq[1], dy = cq[2], dz = cq[3];
         // O_ijk += O_c,ijk + (M_ij a_k + M_ik a_j + M_jk a_i) + (D_i a_j a_k + D_j a_i a_k + D_k a_i a_j)
         //          + q a_i a_j a_k, with the child's M and D (before this child's own shift)

@@ -68,9 +68,10 @@ struct JointDev {
     int32_t *cnt = nullptr, *offs = nullptr;
     double *root = nullptr;     // joint box c0[3], L0[3]
     int nn = 0, cap = 0, maxdepth = 0;
-    double4 *nc = nullptr;      // expansion centre, cell size max(lx) (leaf: particle, 0)
+    double4 *nc = nullptr;      // expansion centre, cell size max(lx) (leaf: particle, charge)
     uint32_t *nnext = nullptr;
-    double *nq = nullptr;       // [nn][TB_JQ] q, D(3), M(6) (xx yy zz xy yz zx), O(10) about the centre
+    double *nq = nullptr;       // [nn][TB_JQ] q, D(3), M(6) (xx yy zz xy yz zx), O(10) about the
+                                // centre; cells only (a leaf's entry is never written or read)
     int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
     uint64_t *dkey[2] = {};
     uint32_t *dval[2] = {};

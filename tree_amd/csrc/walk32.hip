@@ -1394,11 +1394,11 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
     const uint32_t p = bfs[b];
     const uint32_t nx = nnext[p];
     const double4 c = nc[p];
-    const double *Q = nq + (size_t)p * TB_JQ;
     if (nx & TB_LEAF) {
+        // a particle: position and charge
         binfo[b] = TB_LEAF | (uint32_t)nstart[p];
         bmac[b] = make_double4(c.x, c.y, c.z, 0.0);
-        bq0[b] = make_float4((float)Q[0], 0.f, 0.f, 0.f);
+        bq0[b] = make_float4((float)c.w, 0.f, 0.f, 0.f);
         bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         bq2[b] = make_float4(0.f, 0.f, 1.f, 0.f);
         bq3[b] = bq4[b] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1414,6 +1414,7 @@ __global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__re
         }
         k = k2;
     }
+    const double *Q = nq + (size_t)p * TB_JQ;
     binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nch - 1);
     bmac[b] = make_double4(c.x, c.y, c.z, c.w / theta);
     const double tr = Q[4] + Q[5] + Q[6];
