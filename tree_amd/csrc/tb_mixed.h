@@ -40,6 +40,7 @@ struct Domain {
     const uint32_t *binfo;
     const float4 *bq0, *bq1, *bq2, *bf, *bq3, *bq4;
     float inv_rs, rs;
+    float close2;     // (22 rs)^2: unlike pairs closer than this take the Kelbg term
     int gbase;        // first global group index of this domain
     int nn;           // cells in the tree
     uint32_t root;    // the domain's root cell (a batched ensemble's trees share one table)

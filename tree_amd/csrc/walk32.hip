@@ -290,12 +290,11 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
     for (int j = 0; j < cnt; j++) {
         const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j], o1 = L.m3[j], o2 = L.m4[j];
         const float qxx = b.w, qyy = c.x, qzz = -b.w - c.x, qxy = c.y, qyz = c.z, qzx = c.w;
-        // O' components (tr = 0 fixes the z-heavy ones); the mixed ones doubled
-        const float oxxx = o1.x, oxyy = o1.w, oxzz = -(o1.x + o1.w);
-        const float oxxy = o1.y, oyyy = o2.y, oyzz = -(o1.y + o2.y);
-        const float oxxz = o1.z, oyyz = o2.z, ozzz = -(o1.z + o2.z);
-        const float o2xxy = 2.0f * oxxy, o2xxz = 2.0f * oxxz, o2xyz = 2.0f * o2.x;
-        const float o2xyy = 2.0f * oxyy, o2yyz = 2.0f * oyyz, o2xzz = 2.0f * oxzz, o2yzz = 2.0f * oyzz;
+        // O' from its 7 independent components (tr = 0: O'_xzz = -(O'_xxx +
+        // O'_xyy), O'_yzz = -(O'_xxy + O'_yyy), O'_zzz = -(O'_xxz + O'_yyz)),
+        // so v = O'rr needs xx - zz, yy - zz and the doubled mixed products
+        const float oxxx = o1.x, oxxy = o1.y, oxxz = o1.z, oxyy = o1.w, oxyz = o2.x, oyyy = o2.y, oyyz = o2.z;
+        const float sxy = oxxx + oxyy, syz = oxxy + oyyy;
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - a.x, dy = ry[t] - a.y, dz = rz[t] - a.z;
@@ -309,10 +308,11 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
             const float qy = qxy * dx + qyy * dy + qyz * dz;
             const float qz = qzx * dx + qyz * dy + qzz * dz;
             const float xqx = dx * qx + dy * qy + dz * qz;
-            const float xx = dx * dx, yy = dy * dy, zz = dz * dz, xy = dx * dy, xz = dx * dz, yz = dy * dz;
-            const float vx = oxxx * xx + oxyy * yy + oxzz * zz + o2xxy * xy + o2xxz * xz + o2xyz * yz;
-            const float vy = oxxy * xx + oyyy * yy + oyzz * zz + o2xyy * xy + o2xyz * xz + o2yyz * yz;
-            const float vz = oxxz * xx + oyyz * yy + ozzz * zz + o2xyz * xy + o2xzz * xz + o2yzz * yz;
+            const float zz = dz * dz, xmz = dx * dx - zz, ymz = dy * dy - zz;
+            const float dx2 = dx + dx, xy2 = dx2 * dy, xz2 = dx2 * dz, yz2 = (dy + dy) * dz;
+            const float vx = oxxx * xmz + oxyy * ymz + oxxy * xy2 + oxxz * xz2 + oxyz * yz2;
+            const float vy = oxxy * xmz + oyyy * ymz + oxyy * xy2 + oxyz * xz2 + oyyz * yz2;
+            const float vz = oxxz * xmz + oyyz * ymz + oxyz * xy2 - sxy * xz2 - syz * yz2;
             const float t3 = u * (dx * vx + dy * vy + dz * vz);
             // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3))) - ri3 (D + u (Q.dx + 7.5 u v))
             // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3)))
@@ -344,16 +344,17 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
 //
 // The close test of an unlike pair (r < 22 rs) is one expression, repeated
 // alike wherever a pair is classified, so every pair is counted once.
-__device__ inline bool close_pair(float dx, float dy, float dz, float inv_rs, float &r2, float &rq)
+__device__ inline bool close_pair(float dx, float dy, float dz, float close2, float &r2, float &rq)
 {
     r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     rq = __builtin_amdgcn_rsqf(r2);
-    return r2 * rq * inv_rs < 22.0f;  // r / rs
+    return r2 < close2;  // r < 22 rs: close2 = (22 rs)^2
 }
 
 template <int TPL>
 __device__ inline void flush_p2p(WaveLDS &L, int nl, int nu, const float (&rx)[TPL], const float (&ry)[TPL],
-                                 const float (&rz)[TPL], float qt, float inv_rs, Acc<TPL> &A, int &nx)
+                                 const float (&rz)[TPL], float qt, float inv_rs, float close2, Acc<TPL> &A,
+                                 int &nx)
 {
     float ax[TPL], ay[TPL], az[TPL], ap[TPL];
 #pragma unroll
@@ -386,7 +387,7 @@ __device__ inline void flush_p2p(WaveLDS &L, int nl, int nu, const float (&rx)[T
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
             float r2, rq;
-            const bool cl = close_pair(dx, dy, dz, inv_rs, r2, rq);
+            const bool cl = close_pair(dx, dy, dz, close2, r2, rq);
             close = close || cl;
             const float ri = cl ? 0.0f : rq;
             const float w = ri * ri * ri;
@@ -405,7 +406,7 @@ __device__ inline void flush_p2p(WaveLDS &L, int nl, int nu, const float (&rx)[T
 #pragma unroll
             for (int t = 0; t < TPL; t++) {
                 float r2, rq;
-                any = any || close_pair(rx[t] - q.x, ry[t] - q.y, rz[t] - q.z, inv_rs, r2, rq);
+                any = any || close_pair(rx[t] - q.x, ry[t] - q.y, rz[t] - q.z, close2, r2, rq);
             }
             if (!__ballot(any)) continue;  // wave-uniform
             if (nx < NEXACT) {
@@ -417,7 +418,7 @@ __device__ inline void flush_p2p(WaveLDS &L, int nl, int nu, const float (&rx)[T
             for (int t = 0; t < TPL; t++) {
                 const float dx = rx[t] - q.x, dy = ry[t] - q.y, dz = rz[t] - q.z;
                 float r2, rq;
-                if (close_pair(dx, dy, dz, inv_rs, r2, rq)) {
+                if (close_pair(dx, dy, dz, close2, r2, rq)) {
                     float g, h;
                     kelbg_gh(r2 * rq * inv_rs, g, h);
                     const float w = g * rq * rq * rq;
@@ -446,7 +447,8 @@ __device__ inline void flush_p2p(WaveLDS &L, int nl, int nu, const float (&rx)[T
 // to a group centre some um away.
 template <int TPL>
 __device__ inline void exact_pairs(const WaveLDS &L, int nx, const float (&rx)[TPL], const float (&ry)[TPL],
-                                   const float (&rz)[TPL], float qt, float inv_rs, double gx, double gy, double gz,
+                                   const float (&rz)[TPL], float qt, float inv_rs, float close2, double gx,
+                                   double gy, double gz,
                                    const double4 *__restrict__ bmac, const double *__restrict__ px,
                                    const double *__restrict__ py, const double *__restrict__ pz, int tbase, int nact,
                                    Acc<TPL> &A)
@@ -460,7 +462,7 @@ __device__ inline void exact_pairs(const WaveLDS &L, int nx, const float (&rx)[T
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             float r2, rq;
-            if (close_pair(rx[t] - cx, ry[t] - cy, rz[t] - cz, inv_rs, r2, rq)) {
+            if (close_pair(rx[t] - cx, ry[t] - cy, rz[t] - cz, close2, r2, rq)) {
                 const int k = lane_id() + 64 * t;
                 const int ti = tbase + (k < nact ? k : 0);
                 const float ex = (float)(px[ti] - m.x), ey = (float)(py[ti] - m.y), ez = (float)(pz[ti] - m.z);
@@ -953,7 +955,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 wave_sync();
             }
             if (npl + npu >= W.flush_p) {
-                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A, nx);
+                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, D.close2, A, nx);
                 npl = npu = 0;
                 wave_sync();
             }
@@ -972,11 +974,11 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             flush_m2p<TPL>(L, nm, rx, ry, rz, A);
         }
         if (npl + npu > 0) {
-            flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, A, nx);
+            flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, D.close2, A, nx);
         }
         if (nx > 0) {
             wave_sync();
-            exact_pairs<TPL>(L, nx, rx, ry, rz, qt, D.inv_rs, gx, gy, gz, D.bmac, px, py, pz, base, nact, A);
+            exact_pairs<TPL>(L, nx, rx, ry, rz, qt, D.inv_rs, D.close2, gx, gy, gz, D.bmac, px, py, pz, base, nact, A);
         }
         if (nf > 0) push_far(L, nf, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
         W.far_head[g] = F.head;  // every lane stores the same value
@@ -1633,6 +1635,7 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int g
     d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2; d.bf = t->bf;
     d.bq3 = t->bq3; d.bq4 = t->bq4;
     d.inv_rs = (float)(1.0 / rs);
+    d.close2 = (float)(484.0 * rs * rs);  // (22 rs)^2: closer unlike pairs take the Kelbg term
     d.rs = (float)rs;
     d.gbase = gbase;
     d.nn = t->nn;
