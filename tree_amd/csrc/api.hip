@@ -91,14 +91,18 @@ __global__ __launch_bounds__(256) void k_map_targets(int nt, const int32_t *__re
 struct tb_handle {
     tb_config cfg{};
     hipStream_t s = nullptr;
+    // the ions' build work runs on s1 beside the electrons' on s (joined
+    // back with fork / join events); each stream has its own temporaries
+    hipStream_t s1 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     SpeciesDev sp[2];
     JointDev jt;                   // mixed walk: one tree over both species
     MixedDev mx;
-    void *sort_tmp = nullptr;
+    void *sort_tmp = nullptr, *sort_tmp1 = nullptr;
     size_t sort_tmp_bytes = 0;
-    void *scan_tmp = nullptr;
+    void *scan_tmp = nullptr, *scan_tmp1 = nullptr;
     size_t scan_tmp_bytes = 0;
-    double *bbox_tmp = nullptr;
+    double *bbox_tmp = nullptr, *bbox_tmp1 = nullptr;
     int32_t *d_misc = nullptr;     // [0..1] err per species, [2..3] totals, [4..] hist 2 x 2 x (L+2)
     int32_t *h_misc = nullptr;     // pinned mirror
     double *epart = nullptr;       // per-block partial sums
@@ -170,6 +174,8 @@ void free_species(tb_handle *h, SpeciesDev &S)
     size_t c = (size_t)S.cap;
     dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
+    dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
+    dfree(h, S.bm2, c * 48);
     for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
     S = SpeciesDev();
 }
@@ -206,6 +212,8 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     size_t c = (size_t)S.cap;
     dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
+    dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
+    dfree(h, S.bm2, c * 48);
     for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
     S.cap = 0;
@@ -218,6 +226,11 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     if ((rc = dalloc_t(h, &S.nstart, cap))) return rc;
     if ((rc = dalloc_t(h, &S.ncount, cap))) return rc;
     if ((rc = dalloc_t(h, &S.ndepth, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.binv, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.bflag, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.bkids, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.bxq, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.bm2, (size_t)cap * 6))) return rc;
     for (int b = 0; b < 2; b++) {
         if ((rc = dalloc_t(h, &S.dkey[b], cap))) return rc;
         if ((rc = dalloc_t(h, &S.dval[b], cap))) return rc;
@@ -293,15 +306,17 @@ int ensure_tmp(tb_handle *h, int n)
     size_t sb = tb_sort_temp_bytes(n);
     if (sb > h->sort_tmp_bytes) {
         dfree(h, h->sort_tmp, h->sort_tmp_bytes);
-        h->sort_tmp = nullptr; h->sort_tmp_bytes = 0;
-        if ((rc = dalloc(h, &h->sort_tmp, sb))) return rc;
+        dfree(h, h->sort_tmp1, h->sort_tmp_bytes);
+        h->sort_tmp = h->sort_tmp1 = nullptr; h->sort_tmp_bytes = 0;
+        if ((rc = dalloc(h, &h->sort_tmp, sb)) || (rc = dalloc(h, &h->sort_tmp1, sb))) return rc;
         h->sort_tmp_bytes = sb;
     }
     size_t cb = tb_scan_temp_bytes(n);
     if (cb > h->scan_tmp_bytes) {
         dfree(h, h->scan_tmp, h->scan_tmp_bytes);
-        h->scan_tmp = nullptr; h->scan_tmp_bytes = 0;
-        if ((rc = dalloc(h, &h->scan_tmp, cb))) return rc;
+        dfree(h, h->scan_tmp1, h->scan_tmp_bytes);
+        h->scan_tmp = h->scan_tmp1 = nullptr; h->scan_tmp_bytes = 0;
+        if ((rc = dalloc(h, &h->scan_tmp, cb)) || (rc = dalloc(h, &h->scan_tmp1, cb))) return rc;
         h->scan_tmp_bytes = cb;
     }
     size_t pe = (size_t)std::max(tb_walk64_blocks(n), tb_vv_blocks(n)) * 4 + 4096;
@@ -366,32 +381,39 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nu
     if (mixed && (rc = ensure_joint(h, nj))) return rc;
     HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * MISC_N, s));
     const size_t pstride = eb ? (size_t)eb->nreal * eb->chunks * 6 : 0;
+    // the two species' keys, sorts and groups are independent: ions on s1
+    HIPCHK(hipEventRecord(h->fork, s));
+    HIPCHK(hipStreamWaitEvent(h->s1, h->fork, 0));
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
         const int c = S.cur;
+        hipStream_t ss = sp ? h->s1 : s;
+        void *sort_t = sp ? h->sort_tmp1 : h->sort_tmp, *scan_t = sp ? h->scan_tmp1 : h->scan_tmp;
         if (eb) {
             double *roots = eb->d_roots + (size_t)sp * eb->nreal * 6;
-            tb_sbbox(s, eb->nreal, eb->chunks, eb->d_off[sp], S.x[c][0], S.x[c][1], S.x[c][2],
+            tb_sbbox(ss, eb->nreal, eb->chunks, eb->d_off[sp], S.x[c][0], S.x[c][1], S.x[c][2],
                      eb->d_part + sp * pstride);
-            tb_sbbox_final(s, eb->nreal, eb->chunks, eb->d_part + sp * pstride, nullptr, roots);
-            tb_ekeys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], eb->d_off[sp], eb->nreal, roots, eb->T, S.key[0],
+            tb_sbbox_final(ss, eb->nreal, eb->chunks, eb->d_part + sp * pstride, nullptr, roots);
+            tb_ekeys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], eb->d_off[sp], eb->nreal, roots, eb->T, S.key[0],
                      S.val[0], 0u, h->d_misc + sp);
         } else {
-            tb_bbox(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, h->bbox_tmp);
-            tb_keys(s, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
+            tb_bbox(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, sp ? h->bbox_tmp1 : h->bbox_tmp);
+            tb_keys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
         }
-        tb_radix_sort(s, S.n, S.key, S.val, 0, 64, h->sort_tmp, &which[sp], false);
-        tb_permute(s, S, S.val[which[sp]], h->d_misc + sp);
+        tb_radix_sort(ss, S.n, S.key, S.val, 0, 64, sort_t, &which[sp], false);
+        tb_permute(ss, S, S.val[which[sp]], h->d_misc + sp);
         S.skey = S.key[which[sp]];
         if (mixed)
-            tb_groups(s, S, 64 * (tpl ? tpl : tb_mixed_tpl(h->cfg, eb ? nj / eb->nreal : nj)),
-                      h->d_misc + MISC_G + 2 * sp, h->scan_tmp, eb ? eb->T : 0,
+            tb_groups(ss, S, 64 * (tpl ? tpl : tb_mixed_tpl(h->cfg, eb ? nj / eb->nreal : nj)),
+                      h->d_misc + MISC_G + 2 * sp, scan_t, eb ? eb->T : 0,
                       eb ? eb->d_gfirst + (size_t)sp * eb->nreal : nullptr);
         if (!mixed) {
-            tb_lcp(s, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
-            tb_scan_excl(s, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, h->scan_tmp);
+            tb_lcp(ss, S.n, S.key[which[sp]], S.cnt, h->d_misc + MISC_HIST + sp * 2 * HW, h->d_misc + sp);
+            tb_scan_excl(ss, S.n, S.cnt, S.offs, h->d_misc + 2 + sp, scan_t);
         }
     }
+    HIPCHK(hipEventRecord(h->join, h->s1));
+    HIPCHK(hipStreamWaitEvent(s, h->join, 0));
     if (mixed) {
         // joint tree over both species, on the freshly permuted positions
         JointDev &J = h->jt;
@@ -496,8 +518,12 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         if ((rc = ensure_nodes(h, S, S.nn))) return rc;
     }
     if ((rc = ensure_tmp(h, std::max(h->sp[0].nn, h->sp[1].nn)))) return rc;
+    // each species' linking, depth sort and upward pass: ions on s1
+    HIPCHK(hipEventRecord(h->fork, s));
+    HIPCHK(hipStreamWaitEvent(h->s1, h->fork, 0));
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
+        hipStream_t ss = sp ? h->s1 : s;
         const int32_t *hist = h->h_misc + MISC_HIST + sp * 2 * HW;
         int all[HW + 1];
         for (int d = 0; d < HW; d++) {
@@ -512,14 +538,16 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         }
         const uint64_t *key = S.key[which[sp]];
         const int sign = sp == 0 ? -1 : 1;
-        tb_emit(s, S, key, sign);
+        tb_emit(ss, S, key, sign);
         int w2;
-        tb_radix_sort(s, S.nn, S.dkey, S.dval, 0, 8, h->sort_tmp, &w2, true);
+        tb_radix_sort(ss, S.nn, S.dkey, S.dval, 0, 8, sp ? h->sort_tmp1 : h->sort_tmp, &w2, true);
         S.bfs = S.dval[w2];
-        tb_upward(s, S, key, sign, h->cfg.theta, h->d_misc + sp);
+        tb_upward(ss, S, key, sign, h->cfg.theta, h->d_misc + sp);
         h->st.nodes[sp] = S.nn;
         h->st.max_depth[sp] = S.maxdepth;
     }
+    HIPCHK(hipEventRecord(h->join, h->s1));
+    HIPCHK(hipStreamWaitEvent(s, h->join, 0));
     HIPCHK(hipGetLastError());
     h->built = true;
     h->tree_walk = TB_WALK_REF64;
@@ -662,7 +690,11 @@ int tb_create(tb_handle **out, const tb_config *cfg)
         delete h;
         return fail(TB_ERR_HIP, "stream creation failed");
     }
-    bool ok = dalloc_t(h, &h->bbox_tmp, 2048 * 6) == TB_OK && dalloc_t(h, &h->d_misc, 256) == TB_OK &&
+    bool ok = hipStreamCreateWithFlags(&h->s1, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&h->join, hipEventDisableTiming) == hipSuccess;
+    ok = ok && dalloc_t(h, &h->bbox_tmp, 2048 * 6) == TB_OK && dalloc_t(h, &h->bbox_tmp1, 2048 * 6) == TB_OK &&
+         dalloc_t(h, &h->d_misc, 256) == TB_OK &&
               dalloc_t(h, &h->d_sums, 16) == TB_OK && dalloc_t(h, &h->d_cnt, TB_NCNT) == TB_OK;
     ok = ok && hipHostMalloc((void **)&h->h_misc, 256 * sizeof(int32_t), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&h->h_sums, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
@@ -681,12 +713,16 @@ int tb_destroy(tb_handle *h)
     if (!h) return TB_OK;
     hipSetDevice(h->cfg.device);
     if (h->s) hipStreamSynchronize(h->s);
+    if (h->s1) hipStreamSynchronize(h->s1);
     for (int sp = 0; sp < 2; sp++) free_species(h, h->sp[sp]);
     free_joint(h, h->jt);
     tb_mixed_free(h->mx, &h->bytes);
     dfree(h, h->sort_tmp, h->sort_tmp_bytes);
+    dfree(h, h->sort_tmp1, h->sort_tmp_bytes);
     dfree(h, h->scan_tmp, h->scan_tmp_bytes);
+    dfree(h, h->scan_tmp1, h->scan_tmp_bytes);
     dfree(h, h->bbox_tmp, 0);
+    dfree(h, h->bbox_tmp1, 0);
     dfree(h, h->d_misc, 0);
     dfree(h, h->epart, 0);
     dfree(h, h->d_sums, 0);
@@ -696,6 +732,9 @@ int tb_destroy(tb_handle *h)
     if (h->h_sums) hipHostFree(h->h_sums);
     for (int k = 0; k < 6; k++)
         if (h->ev[k]) hipEventDestroy(h->ev[k]);
+    if (h->fork) hipEventDestroy(h->fork);
+    if (h->join) hipEventDestroy(h->join);
+    if (h->s1) hipStreamDestroy(h->s1);
     if (h->s) hipStreamDestroy(h->s);
     delete h;
     return TB_OK;

@@ -475,20 +475,94 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
 
 // ---------------------------------------------------------------- upward
 
-__global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restrict__ seg,
-                                               const uint64_t *__restrict__ key, const double *__restrict__ root,
-                                               double sign, double theta, double4 *__restrict__ nmac,
-                                               const uint32_t *__restrict__ nnext, double *__restrict__ nmom,
+// Breadth-first companions for the upward pass (one thread per node in
+// breadth-first order): the inverse of the order; whether the node is its
+// parent's first child (in preorder a first child directly follows its
+// parent, one level up; any other node follows a deeper one) and whether it
+// is a leaf; a leaf's position and qeff (its particle, the species' sign).
+__global__ __launch_bounds__(BT) void k_bfs_prep(int nn, const uint32_t *__restrict__ bfs,
+                                                 const uint32_t *__restrict__ nnext,
+                                                 const int32_t *__restrict__ ndepth,
+                                                 const double4 *__restrict__ nmac, double sign,
+                                                 uint32_t *__restrict__ binv, uint8_t *__restrict__ bflag,
+                                                 double4 *__restrict__ bxq)
+{
+    const int j = blockIdx.x * BT + threadIdx.x;
+    if (j >= nn) return;
+    const uint32_t q = bfs[j];
+    binv[q] = (uint32_t)j;
+    const bool leaf = (nnext[q] & TB_LEAF) != 0;
+    const bool first = q > 0 && ndepth[q - 1] == ndepth[q] - 1;
+    bflag[j] = (uint8_t)((first ? 1 : 0) | (leaf ? 2 : 0));
+    if (leaf) {
+        const double4 m = nmac[q];
+        bxq[j] = make_double4(m.x, m.y, m.z, sign);
+    }
+}
+
+// Each twig's children: the run of breadth-first nodes from its first child
+// (p + 1 in preorder) to the next first child.  The first node of every
+// level is a first child, so a run never crosses into the next level.  At
+// most 8 children, the first one flagged so, else a malformed table (flag 8).
+__global__ __launch_bounds__(BT) void k_bfs_kids(int nn, const uint32_t *__restrict__ bfs,
+                                                 const uint32_t *__restrict__ binv,
+                                                 const uint8_t *__restrict__ bflag, uint32_t *__restrict__ bkids,
+                                                 int32_t *__restrict__ err)
+{
+    const int b = blockIdx.x * BT + threadIdx.x;
+    if (b >= nn || (bflag[b] & 2)) return;
+    const uint32_t p = bfs[b];
+    if (p + 1 >= (uint32_t)nn) {
+        atomicOr(err, 8);
+        return;
+    }
+    const int j0 = (int)binv[p + 1];
+    if (j0 <= b || !(bflag[j0] & 1)) {
+        atomicOr(err, 8);
+        return;
+    }
+    int nch = 1;
+    while (j0 + nch < nn && !(bflag[j0 + nch] & 1)) {
+        if (++nch > 8) {
+            atomicOr(err, 8);
+            return;
+        }
+    }
+    bkids[b] = ((uint32_t)j0 << 3) | (uint32_t)(nch - 1);
+}
+
+// One level of twigs, deepest level first.  A twig's children are the run
+// of the next level [binv[p + 1], next first child): preorder order, which
+// is the reference's child order 1..8.  Each twig
+// - replays its geometric centre from its key (Insert_Tree's placement);
+// - computes centroid, dx, D, Q, QQ exactly as PSeudoE/I (tree.f90:450-553),
+//   from its children's centroids and qeff = sign * count;
+// - and the exact second moment (parallel-axis M2M) for the mixed walk
+//   (a leaf child's is zero: cm + x is x, so it is not read);
+// - writes the preorder table and its own breadth-first companion record.
+// Children outside the next level are a malformed table: flag 8.
+__global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int cend,
+                                               const uint32_t *__restrict__ bfs, const uint32_t *__restrict__ bkids,
+                                               const uint8_t *__restrict__ bflag, const uint64_t *__restrict__ key,
+                                               const double *__restrict__ root, double sign, double theta,
+                                               double4 *__restrict__ nmac, double *__restrict__ nmom,
                                                double *__restrict__ nm2, double *__restrict__ ndx,
                                                const int32_t *__restrict__ nstart, const int32_t *__restrict__ ndepth,
-                                               const int32_t *__restrict__ ncount, int32_t *__restrict__ err)
+                                               const int32_t *__restrict__ ncount, double4 *__restrict__ bxq,
+                                               double *__restrict__ bm2, int32_t *__restrict__ err)
 {
-    int t = blockIdx.x * BT + threadIdx.x;
+    const int t = blockIdx.x * BT + threadIdx.x;
     if (t >= cnt) return;
-    uint32_t p = seg[t];
-    uint32_t endp = nnext[p];
-    if (endp & TB_LEAF) return;
-    int d = ndepth[p];
+    const int b = beg + t;
+    if (bflag[b] & 2) return;  // a leaf
+    const uint32_t p = bfs[b];
+    const uint32_t kids = bkids[b];
+    const int j0 = (int)(kids >> 3), nch = (int)(kids & 7u) + 1;
+    if (j0 < cbeg || j0 + nch > cend) {
+        atomicOr(err, 8);
+        return;
+    }
+    const int d = ndepth[p];
     // geometric centre and size of the cell, replaying Insert_Tree's placement
     double ox = root[0], oy = root[1], oz = root[2];
     double lx = root[3], ly = root[4], lz = root[5];
@@ -503,23 +577,10 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
         lz = lz * 0.5;
     }
     // PSeudoE: centroid over the children, children in order 1..8
-    // children are p+1 and then successive skip pointers; a pointer that does
-    // not advance would be a malformed table: flag it and stop
-    for (uint32_t c = p + 1; c < endp;) {
-        uint32_t nc = nnext[c] & ~TB_LEAF;
-        if (nc <= c) {
-            atomicOr(err, 8);
-            return;
-        }
-        c = nc;
-    }
-    // a child's qeff is sign * count (what k_emit and this kernel store in
-    // nmom[0]), read from the dense count array rather than the 80-byte
-    // moment records
     double qsum = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-    for (uint32_t c = p + 1; c < endp; c = nnext[c] & ~TB_LEAF) {
-        double4 m = nmac[c];
-        double q = sign * (double)ncount[c];
+    for (int c = 0; c < nch; c++) {
+        const double4 m = bxq[j0 + c];
+        const double q = m.w;
         qsum = qsum + q;
         X = X + q * m.x;
         Y = Y + q * m.y;
@@ -532,10 +593,9 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
     double dx = sqrt(ax * ax + ay * ay + az * az);
     double D0 = 0.0, D1 = 0.0, D2 = 0.0, Q0 = 0.0, Q1 = 0.0, Q2 = 0.0, R0 = 0.0, R1 = 0.0, R2 = 0.0;
     double M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t c = p + 1; c < endp;) {
-        const uint32_t nxr = nnext[c];
-        double4 m = nmac[c];
-        double q = sign * (double)ncount[c];
+    for (int c = 0; c < nch; c++) {
+        const double4 m = bxq[j0 + c];
+        const double q = m.w;
         double d0 = m.x - X, d1 = m.y - Y, d2 = m.z - Z;
         D0 = D0 + q * d0;
         D1 = D1 + q * d1;
@@ -546,7 +606,7 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
         R0 = R0 + q * d0 * d1;
         R1 = R1 + q * d1 * d2;
         R2 = R2 + q * d2 * d0;
-        if (nxr & TB_LEAF) {
+        if (bflag[j0 + c] & 2) {
             // a leaf's second moments are zero: cm + x is x exactly
             M[0] = M[0] + q * (d0 * d0);
             M[1] = M[1] + q * (d1 * d1);
@@ -555,7 +615,7 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
             M[4] = M[4] + q * (d1 * d2);
             M[5] = M[5] + q * (d2 * d0);
         } else {
-            const double *cm = nm2 + (long)c * 6;
+            const double *cm = bm2 + (size_t)(j0 + c) * 6;
             M[0] = M[0] + (cm[0] + q * (d0 * d0));
             M[1] = M[1] + (cm[1] + q * (d1 * d1));
             M[2] = M[2] + (cm[2] + q * (d2 * d2));
@@ -563,21 +623,23 @@ __global__ __launch_bounds__(BT) void k_upward(int cnt, const uint32_t *__restri
             M[4] = M[4] + (cm[4] + q * (d1 * d2));
             M[5] = M[5] + (cm[5] + q * (d2 * d0));
         }
-        c = nxr & ~TB_LEAF;
     }
     double s = lx;
     if (ly > s) s = ly;
     if (lz > s) s = lz;
     double rcrit = s / theta + dx;   // MAC left side, tree.f90:742
     nmac[p] = make_double4(X, Y, Z, rcrit);
-    double *mo = nmom + (long)p * 10;
-    mo[0] = sign * (double)ncount[p];
+    const double qeff = sign * (double)ncount[p];
+    double *mo = nmom + (size_t)p * 10;
+    mo[0] = qeff;
     mo[1] = D0; mo[2] = D1; mo[3] = D2;
     mo[4] = Q0; mo[5] = Q1; mo[6] = Q2;
     mo[7] = R0; mo[8] = R1; mo[9] = R2;
-    double *m2 = nm2 + (long)p * 6;
-    for (int c = 0; c < 6; c++) m2[c] = M[c];
+    double *m2 = nm2 + (size_t)p * 6;
+    double *b2 = bm2 + (size_t)b * 6;
+    for (int c = 0; c < 6; c++) m2[c] = b2[c] = M[c];
     ndx[p] = dx;
+    bxq[b] = make_double4(X, Y, Z, qeff);
 }
 
 // ------------------------------------------------------------ export
@@ -1140,11 +1202,15 @@ void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign)
 
 void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err)
 {
+    hipLaunchKernelGGL(k_bfs_prep, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.nnext, S.ndepth, S.nmac,
+                       (double)sign, S.binv, S.bflag, S.bxq);
+    hipLaunchKernelGGL(k_bfs_kids, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.binv, S.bflag, S.bkids, err);
     for (int d = S.maxdepth; d >= 0; d--) {
         int beg = S.lvl_off[d], cnt = S.lvl_off[d + 1] - beg;
         if (cnt <= 0 || S.twig_hist[d] == 0) continue;
-        hipLaunchKernelGGL(k_upward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, cnt, S.bfs + beg, key, S.root,
-                           (double)sign, theta, S.nmac, S.nnext, S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, S.ncount, err);
+        hipLaunchKernelGGL(k_upward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, beg, cnt, S.lvl_off[d + 1],
+                           S.lvl_off[d + 2], S.bfs, S.bkids, S.bflag, key, S.root, (double)sign, theta, S.nmac,
+                           S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, S.ncount, S.bxq, S.bm2, err);
     }
 }
 

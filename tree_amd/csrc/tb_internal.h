@@ -40,6 +40,16 @@ struct SpeciesDev {
     uint32_t *bfs = nullptr;    // preorder ids grouped by depth, preorder within a depth
     int lvl_off[TB_LEVELS + 3] = {};
     int twig_hist[TB_LEVELS + 2] = {};
+    // breadth-first companions of the upward pass, so that a cell reads its
+    // children (a contiguous run of the next level) without chasing pointers:
+    // preorder -> breadth-first index; per breadth-first node bit 0 = first
+    // child of its parent, bit 1 = leaf; per cell, first child << 3 |
+    // children - 1; centroid and qeff; exact second moment (cells)
+    uint32_t *binv = nullptr;
+    uint8_t *bflag = nullptr;
+    uint32_t *bkids = nullptr;
+    double4 *bxq = nullptr;
+    double *bm2 = nullptr;
     // mixed walk: target groups of at most gG consecutive key-ordered
     // particles, each inside one cell of at most 8 gG particles (tb_groups)
     const uint64_t *skey = nullptr;  // this build's sorted keys
