@@ -252,7 +252,7 @@ void free_joint(tb_handle *h, JointDev &J)
     dfree(h, J.cnt, n * 4);
     dfree(h, J.offs, n * 4);
     dfree(h, J.root, 64 * 8);
-    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 8 * TB_JQ);
+    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4);
     dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
     J = JointDev();
 }
@@ -281,14 +281,13 @@ int ensure_joint_nodes(tb_handle *h, int nn)
     if (nn <= J.cap) return TB_OK;
     size_t c = (size_t)J.cap;
     for (int b = 0; b < 2; b++) { dfree(h, J.dkey[b], c * 8); dfree(h, J.dval[b], c * 4); }
-    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4); dfree(h, J.nq, c * 8 * TB_JQ);
+    dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4);
     dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
     J.cap = 0;
     int rc;
     if ((rc = dalloc_t(h, &J.nc, cap))) return rc;
     if ((rc = dalloc_t(h, &J.nnext, cap))) return rc;
-    if ((rc = dalloc_t(h, &J.nq, (size_t)cap * TB_JQ))) return rc;
     if ((rc = dalloc_t(h, &J.nstart, cap))) return rc;
     if ((rc = dalloc_t(h, &J.ncount, cap))) return rc;
     if ((rc = dalloc_t(h, &J.ndepth, cap))) return rc;
@@ -490,23 +489,21 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         int w2;
         tb_radix_sort(s, J.nn, J.dkey, J.dval, 0, 8, h->sort_tmp, &w2, true);
         J.bfs = J.dval[w2];
-        if (eb) {
+        if (eb && J.lvl_off[eb->T + 1] - J.lvl_off[eb->T] != eb->nreal)
             // realisation r's root cell is the r-th cell at depth T (the tag digits)
-            if (J.lvl_off[eb->T + 1] - J.lvl_off[eb->T] != eb->nreal)
-                return fail(TB_ERR_HIP, "ensemble tree: %d cells at depth %d for %d realisations",
-                            J.lvl_off[eb->T + 1] - J.lvl_off[eb->T], eb->T, eb->nreal);
-            tb_jupward(s, J, key, h->d_misc, eb->T, eb->d_roots + (size_t)2 * eb->nreal * 6);
-        } else {
-            tb_jupward(s, J, key, h->d_misc);
-        }
+            return fail(TB_ERR_HIP, "ensemble tree: %d cells at depth %d for %d realisations",
+                        J.lvl_off[eb->T + 1] - J.lvl_off[eb->T], eb->T, eb->nreal);
+        // the walk's breadth-first records, filled by the upward pass
+        if ((rc = tb_mixed_prep(h->s, h->mx, &h->jt, &h->bytes))) return fail(rc, "%s", tb_mixed_error());
+        const CellRecs R = tb_mixed_recs(h->mx, h->cfg);
+        if (eb) tb_jupward(s, J, key, R, eb->T, eb->d_roots + (size_t)2 * eb->nreal * 6);
+        else tb_jupward(s, J, key, R);
         h->st.nodes[0] = J.nn;
         h->st.nodes[1] = 0;
         h->st.max_depth[0] = J.maxdepth;
         h->st.max_depth[1] = 0;
         HIPCHK(hipGetLastError());
         h->mx.lists = tb_autotune_lists(eb ? (h->sp[0].n + h->sp[1].n) / eb->nreal : h->sp[0].n + h->sp[1].n);
-        if ((rc = tb_mixed_prep(h->s, h->mx, h->sp, &h->jt, h->cfg, &h->bytes)))
-            return fail(rc, "%s", tb_mixed_error());
         h->built = true;
         h->tree_walk = TB_WALK_MIXED;
         return TB_OK;

@@ -475,22 +475,28 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
 
 // ---------------------------------------------------------------- upward
 
-// Breadth-first companions for the upward pass (one thread per node in
-// breadth-first order): the inverse of the order; whether the node is its
-// parent's first child (in preorder a first child directly follows its
-// parent, one level up; any other node follows a deeper one) and whether it
-// is a leaf; a leaf's position and qeff (its particle, the species' sign).
-__global__ __launch_bounds__(BT) void k_bfs_prep(int nn, const uint32_t *__restrict__ bfs,
+// The inverse of the breadth-first order (preorder node -> breadth-first index).
+__global__ __launch_bounds__(BT) void k_bfs_inv(int nn, const uint32_t *__restrict__ bfs, uint32_t *__restrict__ binv)
+{
+    const int j = blockIdx.x * BT + threadIdx.x;
+    if (j < nn) binv[bfs[j]] = (uint32_t)j;
+}
+
+// Breadth-first companions for the upward pass, one thread per node in
+// preorder (reads in order, writes to the node's breadth-first slot):
+// whether the node is its parent's first child (in preorder a first child
+// directly follows its parent, one level up; any other node follows a deeper
+// one) and whether it is a leaf; a leaf's position and qeff (its particle,
+// the species' sign).
+__global__ __launch_bounds__(BT) void k_bfs_prep(int nn, const uint32_t *__restrict__ binv,
                                                  const uint32_t *__restrict__ nnext,
                                                  const int32_t *__restrict__ ndepth,
                                                  const double4 *__restrict__ nmac, double sign,
-                                                 uint32_t *__restrict__ binv, uint8_t *__restrict__ bflag,
-                                                 double4 *__restrict__ bxq)
+                                                 uint8_t *__restrict__ bflag, double4 *__restrict__ bxq)
 {
-    const int j = blockIdx.x * BT + threadIdx.x;
-    if (j >= nn) return;
-    const uint32_t q = bfs[j];
-    binv[q] = (uint32_t)j;
+    const int q = blockIdx.x * BT + threadIdx.x;
+    if (q >= nn) return;
+    const uint32_t j = binv[q];
     const bool leaf = (nnext[q] & TB_LEAF) != 0;
     const bool first = q > 0 && ndepth[q - 1] == ndepth[q] - 1;
     bflag[j] = (uint8_t)((first ? 1 : 0) | (leaf ? 2 : 0));
@@ -731,7 +737,7 @@ __global__ __launch_bounds__(BT) void k_jgather(int n, const uint32_t *__restric
 __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict__ key, const int32_t *__restrict__ offs,
                                               int total, const double4 *__restrict__ pos,
                                               double4 *__restrict__ nc, uint32_t *__restrict__ nnext,
-                                              double *__restrict__ nq, int32_t *__restrict__ nstart,
+                                              int32_t *__restrict__ nstart,
                                               int32_t *__restrict__ ncount, int32_t *__restrict__ ndepth,
                                               uint64_t *__restrict__ dkey)
 {
@@ -776,8 +782,7 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
         ncount[id] = end - i;
         uint32_t nxt = end >= n ? (uint32_t)total : (uint32_t)(offs[end] + 1);
         if (d == leafd) {
-            // a leaf: its particle's position and charge; it has no moment
-            // record (nq of a leaf is never read)
+            // a leaf: its particle's position and charge
             nnext[id] = nxt | TB_LEAF;
             nc[id] = pos[i];
         } else {
@@ -786,32 +791,130 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
     }
 }
 
-// net charge q, dipole D, second moment M and third moment O of a cell about
-// its geometric centre, shifted up from the children's own expansion centres
-// (a child's offsets s become s + a, a = child centre - cell centre)
+// The mixed walk's breadth-first records.  A leaf: its particle and charge.
+// A cell: centre and s / theta, first child (breadth-first) << 3 | children
+// - 1, q, D, the traceless quadrupole 3 M - tr M, the traceless third moment
+// O' = O - (d_ij t_k + d_ik t_j + d_jk t_i) / 5 with t_k = O_iik, and the
+// learned criterion's per-cell features (FP64, then rounded; see
+// band_features in walk32.hip).
+// (A leaf's other records, bq1..bq4 and bf, are never read.)
+__device__ inline void leaf_record(const CellRecs &R, int b, double4 c, int32_t start)
+{
+    R.binfo[b] = TB_LEAF | (uint32_t)start;
+    R.bmac[b] = make_double4(c.x, c.y, c.z, 0.0);
+    R.bq0[b] = make_float4((float)c.w, 0.f, 0.f, 0.f);
+}
+
+__device__ inline void cell_record(const CellRecs &R, int b, double cx, double cy, double cz, double sz,
+                                   const double (&Q)[TB_JQ], int32_t count)
+{
+#pragma clang fp contract(fast)
+    R.bmac[b] = make_double4(cx, cy, cz, sz / R.theta);
+    const double tr = Q[4] + Q[5] + Q[6];
+    R.bq0[b] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
+    R.bq1[b] = make_float4((float)(3.0 * Q[4] - tr), (float)(3.0 * Q[5] - tr), (float)(3.0 * Q[7]),
+                           (float)(3.0 * Q[8]));
+    // raw O: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+    const double *O = Q + 10;
+    const double tx = O[0] + O[3] + O[5], ty = O[1] + O[6] + O[8], tz = O[2] + O[7] + O[9];
+    R.bq3[b] = make_float4((float)(O[0] - 0.6 * tx), (float)(O[1] - 0.2 * ty), (float)(O[2] - 0.2 * tz),
+                           (float)(O[3] - 0.2 * tx));
+    R.bq4[b] = make_float4((float)O[4], (float)(O[6] - 0.6 * ty), (float)(O[7] - 0.2 * tz), 0.0f);
+    const double kf = 0.125, cn = (double)count, tiny = 9.5367431640625e-07;
+    const double qxx = 3.0 * Q[4] - tr, qyy = 3.0 * Q[5] - tr, qzz = 3.0 * Q[6] - tr;
+    const double qn = sqrt(qxx * qxx + qyy * qyy + qzz * qzz +
+                           2.0 * (9.0 * Q[7] * Q[7] + 9.0 * Q[8] * Q[8] + 9.0 * Q[9] * Q[9]));
+    const double dn = sqrt(Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]);
+    R.bq2[b] = make_float4((float)(3.0 * Q[9]), (float)sz, (float)cn, (float)(kf * log2(sz)));
+    R.bf[b] = make_float4((float)(kf * log2(fabs(Q[0]) / cn + tiny)), (float)(kf * log2(dn / (cn * sz) + tiny)),
+                          (float)(kf * log2(qn / (cn * sz * sz) + tiny)), (float)(kf * log2(cn)));
+}
+
+// Breadth-first companions of the joint tree, one thread per node in
+// preorder (reads in order, writes to the node's breadth-first slot b):
+// first-child and leaf flags (a first child directly follows its parent in
+// preorder); a leaf's record; a cell's first particle's key and particle
+// count.
+__global__ __launch_bounds__(BT) void k_jprep(int nn, const uint32_t *__restrict__ nnext,
+                                              const int32_t *__restrict__ ndepth,
+                                              const int32_t *__restrict__ nstart,
+                                              const int32_t *__restrict__ ncount, const uint64_t *__restrict__ key,
+                                              const double4 *__restrict__ nc, CellRecs R)
+{
+    const int q = blockIdx.x * BT + threadIdx.x;
+    if (q >= nn) return;
+    const int b = (int)R.inv[q];
+    const bool leaf = (nnext[q] & TB_LEAF) != 0;
+    const bool first = q > 0 && ndepth[q - 1] == ndepth[q] - 1;
+    R.bflag[b] = (uint8_t)((first ? 1 : 0) | (leaf ? 2 : 0));
+    const int32_t st = nstart[q];
+    if (leaf) {
+        leaf_record(R, b, nc[q], st);
+    } else {
+        R.bkey[b] = key[st];
+        R.bcnt[b] = ncount[q];
+    }
+}
+
+// Each cell's children: the run of breadth-first nodes from its first child
+// (p + 1 in preorder) to the next first child (the first node of a level is
+// a first child, so a run stays in its level); 1 to 8 children, else a
+// malformed table.
+__global__ __launch_bounds__(BT) void k_jkids(int nn, const uint32_t *__restrict__ bfs, CellRecs R)
+{
+    const int b = blockIdx.x * BT + threadIdx.x;
+    if (b >= nn || (R.bflag[b] & 2)) return;
+    const uint32_t p = bfs[b];
+    if (p + 1 >= (uint32_t)nn) {
+        atomicOr(R.flag, 16);
+        return;
+    }
+    const int j0 = (int)R.inv[p + 1];
+    if (j0 <= b || !(R.bflag[j0] & 1)) {
+        atomicOr(R.flag, 16);
+        return;
+    }
+    int nch = 1;
+    while (j0 + nch < nn && !(R.bflag[j0 + nch] & 1)) {
+        if (++nch > 8) {
+            atomicOr(R.flag, 16);
+            return;
+        }
+    }
+    R.binfo[b] = ((uint32_t)j0 << 3) | (uint32_t)(nch - 1);
+}
+
+// net charge q, dipole D, second moment M and third moment O of the cells of
+// one level (deepest first) about their geometric centres, shifted up from
+// the children's own expansion centres (a child's offsets s become s + a,
+// a = child centre - cell centre); the children are a contiguous run of the
+// next level, read from their breadth-first records.
 //
 // A batched ensemble's keys start with T octal digits naming the realisation
 // (its tag): the cells of realisation r start at depth T from its own box
 // root[6 r ..]; the T levels above hold no cell of any realisation and are
-// never walked (their records are zero).  One domain: T = 0.
-__global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restrict__ seg,
-                                                const uint64_t *__restrict__ key, const double *__restrict__ root,
-                                                int T, double4 *__restrict__ nc, const uint32_t *__restrict__ nnext,
-                                                double *__restrict__ nq, const int32_t *__restrict__ nstart,
-                                                const int32_t *__restrict__ ndepth, int32_t *__restrict__ err)
+// never walked (their moments and records are zero).  One domain: T = 0.
+__global__ __launch_bounds__(BT) void k_jupward(int beg, int cnt, int cbeg, int cend, int d, int T,
+                                                const double *__restrict__ root, CellRecs R)
 {
-    int t = blockIdx.x * BT + threadIdx.x;
+    const int t = blockIdx.x * BT + threadIdx.x;
     if (t >= cnt) return;
-    uint32_t p = seg[t];
-    uint32_t endp = nnext[p];
-    if (endp & TB_LEAF) return;
-    int d = ndepth[p];
-    uint64_t k = key[nstart[p]];
-    if (d < T) {
-        nc[p] = make_double4(0.0, 0.0, 0.0, 0.0);
-        for (int c = 0; c < TB_JQ; c++) nq[(size_t)p * TB_JQ + c] = 0.0;
+    const int b = beg + t;
+    if (R.bflag[b] & 2) return;  // a leaf: its record is written
+    const uint32_t kids = R.binfo[b];
+    const int j0 = (int)(kids >> 3), nch = (int)(kids & 7u) + 1;
+    if (j0 < cbeg || j0 + nch > cend) {
+        atomicOr(R.flag, 16);
         return;
     }
+    double *o = R.bnq + (size_t)b * TB_JQ;
+    if (d < T) {
+        for (int c = 0; c < TB_JQ; c++) o[c] = 0.0;
+        R.bmac[b] = make_double4(0.0, 0.0, 0.0, 0.0);
+        R.bq0[b] = R.bq1[b] = R.bq2[b] = R.bq3[b] = R.bq4[b] = R.bf[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const uint64_t k = R.bkey[b];
     const double *rt = root + (T > 0 ? 6 * (size_t)(k >> (3 * (TB_LEVELS - T))) : 0);
     double ox = rt[0], oy = rt[1], oz = rt[2];
     double lx = rt[3], ly = rt[4], lz = rt[5];
@@ -826,73 +929,66 @@ __global__ __launch_bounds__(BT) void k_jupward(int cnt, const uint32_t *__restr
     }
     double q = 0.0, D[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double O[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t c = p + 1; c < endp;) {
-        const uint32_t nxr = nnext[c], nxc = nxr & ~TB_LEAF;
-        if (nxc <= c) {
-            atomicOr(err, 8);
-            return;
-        }
-        const double4 cc = nc[c];
-        const double a = cc.x - ox, b = cc.y - oy, e = cc.z - oz;
-        if (nxr & TB_LEAF) {
-            // a particle (charge cc.w): the shift below with D = M = O = 0,
-            // term for term (the dropped terms are exact zeros)
-            const double qc = cc.w, av[3] = {a, b, e};
-            const int oi[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
-                                   {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+    const int oi[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                           {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+    for (int j = j0; j < j0 + nch; j++) {
+        const double4 cc = R.bmac[j];
+        const double a = cc.x - ox, b_ = cc.y - oy, e = cc.z - oz;
+        if (R.bflag[j] & 2) {
+            // a particle (charge +-1, exact in the FP32 record): the shift
+            // below with D = M = O = 0, term for term (the dropped terms are
+            // exact zeros)
+            const double qc = (double)R.bq0[j].x, av[3] = {a, b_, e};
 #pragma unroll
-            for (int t = 0; t < 10; t++) O[t] += qc * av[oi[t][0]] * av[oi[t][1]] * av[oi[t][2]];
+            for (int t2 = 0; t2 < 10; t2++) O[t2] += qc * av[oi[t2][0]] * av[oi[t2][1]] * av[oi[t2][2]];
             q += qc;
             D[0] += qc * a;
-            D[1] += qc * b;
+            D[1] += qc * b_;
             D[2] += qc * e;
             M[0] += qc * a * a;
-            M[1] += qc * b * b;
+            M[1] += qc * b_ * b_;
             M[2] += qc * e * e;
-            M[3] += qc * a * b;
-            M[4] += qc * b * e;
+            M[3] += qc * a * b_;
+            M[4] += qc * b_ * e;
             M[5] += qc * e * a;
-            c = nxc;
             continue;
         }
-        const double *cq = nq + (size_t)c * TB_JQ;
+        const double *cq = R.bnq + (size_t)j * TB_JQ;
         const double qc = cq[0], dx = cq[1], dy = cq[2], dz = cq[3];
         // O_ijk += O_c,ijk + (M_ij a_k + M_ik a_j + M_jk a_i) + (D_i a_j a_k + D_j a_i a_k + D_k a_i a_j)
         //          + q a_i a_j a_k, with the child's M and D (before this child's own shift)
         {
-            const double av[3] = {a, b, e}, dv[3] = {dx, dy, dz};
+            const double av[3] = {a, b_, e}, dv[3] = {dx, dy, dz};
             const double mm[3][3] = {{cq[4], cq[7], cq[9]}, {cq[7], cq[5], cq[8]}, {cq[9], cq[8], cq[6]}};
-            const int oi[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
-                                   {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
 #pragma unroll
-            for (int t = 0; t < 10; t++) {
-                const int i = oi[t][0], j = oi[t][1], k = oi[t][2];
-                O[t] += cq[10 + t] + mm[i][j] * av[k] + mm[i][k] * av[j] + mm[j][k] * av[i] +
-                        dv[i] * av[j] * av[k] + dv[j] * av[i] * av[k] + dv[k] * av[i] * av[j] +
-                        qc * av[i] * av[j] * av[k];
+            for (int t2 = 0; t2 < 10; t2++) {
+                const int i = oi[t2][0], jj = oi[t2][1], kk = oi[t2][2];
+                O[t2] += cq[10 + t2] + mm[i][jj] * av[kk] + mm[i][kk] * av[jj] + mm[jj][kk] * av[i] +
+                         dv[i] * av[jj] * av[kk] + dv[jj] * av[i] * av[kk] + dv[kk] * av[i] * av[jj] +
+                         qc * av[i] * av[jj] * av[kk];
             }
         }
         q += qc;
         D[0] += dx + qc * a;
-        D[1] += dy + qc * b;
+        D[1] += dy + qc * b_;
         D[2] += dz + qc * e;
         M[0] += cq[4] + 2.0 * dx * a + qc * a * a;
-        M[1] += cq[5] + 2.0 * dy * b + qc * b * b;
+        M[1] += cq[5] + 2.0 * dy * b_ + qc * b_ * b_;
         M[2] += cq[6] + 2.0 * dz * e + qc * e * e;
-        M[3] += cq[7] + dx * b + dy * a + qc * a * b;
-        M[4] += cq[8] + dy * e + dz * b + qc * b * e;
+        M[3] += cq[7] + dx * b_ + dy * a + qc * a * b_;
+        M[4] += cq[8] + dy * e + dz * b_ + qc * b_ * e;
         M[5] += cq[9] + dz * a + dx * e + qc * e * a;
-        c = nxc;
     }
     double s = lx;
     if (ly > s) s = ly;
     if (lz > s) s = lz;
-    nc[p] = make_double4(ox, oy, oz, s);
-    double *o = nq + (size_t)p * TB_JQ;
-    o[0] = q;
-    o[1] = D[0]; o[2] = D[1]; o[3] = D[2];
-    for (int c = 0; c < 6; c++) o[4 + c] = M[c];
-    for (int c = 0; c < 10; c++) o[10 + c] = O[c];
+    double Q[TB_JQ];
+    Q[0] = q;
+    for (int c = 0; c < 3; c++) Q[1 + c] = D[c];
+    for (int c = 0; c < 6; c++) Q[4 + c] = M[c];
+    for (int c = 0; c < 10; c++) Q[10 + c] = O[c];
+    for (int c = 0; c < TB_JQ; c++) o[c] = Q[c];
+    cell_record(R, b, ox, oy, oz, s, Q, R.bcnt[b]);
 }
 
 // ------------------------------------------------ mixed-walk target groups
@@ -1202,8 +1298,9 @@ void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign)
 
 void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err)
 {
-    hipLaunchKernelGGL(k_bfs_prep, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.nnext, S.ndepth, S.nmac,
-                       (double)sign, S.binv, S.bflag, S.bxq);
+    hipLaunchKernelGGL(k_bfs_inv, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.binv);
+    hipLaunchKernelGGL(k_bfs_prep, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.binv, S.nnext, S.ndepth, S.nmac,
+                       (double)sign, S.bflag, S.bxq);
     hipLaunchKernelGGL(k_bfs_kids, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.binv, S.bflag, S.bkids, err);
     for (int d = S.maxdepth; d >= 0; d--) {
         int beg = S.lvl_off[d], cnt = S.lvl_off[d + 1] - beg;
@@ -1292,15 +1389,20 @@ void tb_ekeys(hipStream_t s, int n, const double *x, const double *y, const doub
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)
 {
     hipLaunchKernelGGL(k_jemit, dim3(nblk(J.n, BT)), dim3(BT), 0, s, J.n, key, J.offs, J.nn, J.pos, J.nc, J.nnext,
-                       J.nq, J.nstart, J.ncount, J.ndepth, J.dkey[0]);
+                       J.nstart, J.ncount, J.ndepth, J.dkey[0]);
 }
 
-void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err, int T, const double *roots)
+void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, const CellRecs &R, int T, const double *roots)
 {
+    hipLaunchKernelGGL(k_bfs_inv, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, R.inv);
+    hipLaunchKernelGGL(k_jprep, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.nnext, J.ndepth, J.nstart, J.ncount,
+                       key, J.nc, R);
+    hipLaunchKernelGGL(k_jkids, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, R);
     for (int d = J.maxdepth; d >= 0; d--) {
         int beg = J.lvl_off[d], cnt = J.lvl_off[d + 1] - beg;
         if (cnt <= 0 || J.twig_hist[d] == 0) continue;
-        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, cnt, J.bfs + beg, key,
-                           T > 0 ? roots : J.root, T, J.nc, J.nnext, J.nq, J.nstart, J.ndepth, err);
+        const int cend = d + 2 <= TB_LEVELS + 2 ? J.lvl_off[d + 2] : J.nn;
+        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, beg, cnt, J.lvl_off[d + 1], cend, d, T,
+                           T > 0 ? roots : J.root, R);
     }
 }

@@ -78,10 +78,8 @@ struct JointDev {
     int32_t *cnt = nullptr, *offs = nullptr;
     double *root = nullptr;     // joint box c0[3], L0[3]
     int nn = 0, cap = 0, maxdepth = 0;
-    double4 *nc = nullptr;      // expansion centre, cell size max(lx) (leaf: particle, charge)
+    double4 *nc = nullptr;      // leaves: the particle's position and charge (cells: unused)
     uint32_t *nnext = nullptr;
-    double *nq = nullptr;       // [nn][TB_JQ] q, D(3), M(6) (xx yy zz xy yz zx), O(10) about the
-                                // centre; cells only (a leaf's entry is never written or read)
     int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
     uint64_t *dkey[2] = {};
     uint32_t *dval[2] = {};
@@ -123,8 +121,23 @@ void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const doub
 void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x[2], const double *const y[2],
                 const double *const z[2], double4 *pos);
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key);
+// The mixed walk's breadth-first cell records, which the joint tree's upward
+// pass fills, and the breadth-first companions it works from.
+struct CellRecs {
+    uint32_t *inv = nullptr;    // preorder node -> breadth-first index
+    double4 *bmac = nullptr;    // centre (a leaf: its particle), s / theta (a leaf: 0)
+    uint32_t *binfo = nullptr;  // first child << 3 | children - 1, or leaf | joint particle
+    float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr, *bq3 = nullptr, *bq4 = nullptr, *bf = nullptr;
+    uint8_t *bflag = nullptr;   // bit 0: first child of its parent; bit 1: leaf
+    uint64_t *bkey = nullptr;   // a cell's first particle's key (its centre replays from it)
+    int32_t *bcnt = nullptr;    // a cell's particle count
+    double *bnq = nullptr;      // [nn][TB_JQ] a cell's FP64 q, D(3), M(6) (xx yy zz xy yz zx),
+                                // O(10) about its centre
+    double theta = 0.0;
+    int32_t *flag = nullptr;    // 16: malformed node table
+};
 // T > 0: a batched ensemble's joint tree (keys tagged by tb_ekeys), roots [nreal][6]
-void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, int32_t *err, int T = 0,
+void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, const CellRecs &R, int T = 0,
                 const double *roots = nullptr);
 // target groups of at most G particles over S.skey; totals[0] = segments,
 // totals[1] = groups (device); uses S.cnt and S.offs as scratch.  A batched

@@ -22,6 +22,11 @@ struct MixedTree {
     float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr, *bf = nullptr;
     float4 *bq3 = nullptr, *bq4 = nullptr;   // traceless third moment (far-field local expansion)
     uint32_t *inv = nullptr;
+    // the upward pass's breadth-first companions (CellRecs)
+    uint8_t *bflag = nullptr;
+    uint64_t *bkey = nullptr;
+    int32_t *bcnt = nullptr;
+    double *bnq = nullptr;
 };
 
 // a (cell, target group) pair of the learned criterion's band, with its features
@@ -79,7 +84,10 @@ struct MixedDev {
     int rec_cap = 0;
 };
 
-int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, const tb_config &cfg, size_t *bytes);
+// the cell records for the joint tree *jt: allocated and their error flag
+// cleared; tb_jupward with tb_mixed_recs then fills them
+int tb_mixed_prep(hipStream_t s, MixedDev &M, const JointDev *jt, size_t *bytes);
+CellRecs tb_mixed_recs(MixedDev &M, const tb_config &cfg);
 int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *epart, double *d_sums);
 int tb_mixed_count(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *epart, double *d_sums,
                    int64_t *d_cnt);

@@ -1372,74 +1372,6 @@ void dispatch(hipStream_t s, int tpl, int blocks, const WalkArgs &W)
 
 // ---------------------------------------------------------------- prep
 
-__global__ __launch_bounds__(BT) void k_inv(int nn, const uint32_t *__restrict__ bfs, uint32_t *__restrict__ inv)
-{
-    const int b = blockIdx.x * BT + threadIdx.x;
-    if (b < nn) inv[bfs[b]] = (uint32_t)b;
-}
-
-__global__ __launch_bounds__(BT) void k_mixed_nodes(int nn, const uint32_t *__restrict__ bfs,
-                                                    const uint32_t *__restrict__ inv,
-                                                    const double4 *__restrict__ nc,
-                                                    const uint32_t *__restrict__ nnext,
-                                                    const double *__restrict__ nq,
-                                                    const int32_t *__restrict__ nstart,
-                                                    const int32_t *__restrict__ ncount, double theta,
-                                                    double4 *__restrict__ bmac, uint32_t *__restrict__ binfo,
-                                                    float4 *__restrict__ bq0, float4 *__restrict__ bq1,
-                                                    float4 *__restrict__ bq2, float4 *__restrict__ bq3,
-                                                    float4 *__restrict__ bq4, float4 *__restrict__ bf,
-                                                    int32_t *__restrict__ err)
-{
-    const int b = blockIdx.x * BT + threadIdx.x;
-    if (b >= nn) return;
-    const uint32_t p = bfs[b];
-    const uint32_t nx = nnext[p];
-    const double4 c = nc[p];
-    if (nx & TB_LEAF) {
-        // a particle: position and charge
-        binfo[b] = TB_LEAF | (uint32_t)nstart[p];
-        bmac[b] = make_double4(c.x, c.y, c.z, 0.0);
-        bq0[b] = make_float4((float)c.w, 0.f, 0.f, 0.f);
-        bq1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
-        bq2[b] = make_float4(0.f, 0.f, 1.f, 0.f);
-        bq3[b] = bq4[b] = make_float4(0.f, 0.f, 0.f, 0.f);
-        bf[b] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
-    }
-    int nch = 0;
-    for (uint32_t k = p + 1; k < nx; nch++) {
-        const uint32_t k2 = nnext[k] & ~TB_LEAF;
-        if (k2 <= k || nch >= 8) {
-            atomicOr(err, 16);
-            return;
-        }
-        k = k2;
-    }
-    const double *Q = nq + (size_t)p * TB_JQ;
-    binfo[b] = (inv[p + 1] << 3) | (uint32_t)(nch - 1);
-    bmac[b] = make_double4(c.x, c.y, c.z, c.w / theta);
-    const double tr = Q[4] + Q[5] + Q[6];
-    bq0[b] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
-    bq1[b] = make_float4((float)(3.0 * Q[4] - tr), (float)(3.0 * Q[5] - tr), (float)(3.0 * Q[7]), (float)(3.0 * Q[8]));
-    // traceless third moment O' = O - (d_ij t_k + d_ik t_j + d_jk t_i) / 5, t_k = O_iik
-    // (raw O: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz); the far path's local expansion uses it
-    const double *O = Q + 10;
-    const double tx = O[0] + O[3] + O[5], ty = O[1] + O[6] + O[8], tz = O[2] + O[7] + O[9];
-    bq3[b] = make_float4((float)(O[0] - 0.6 * tx), (float)(O[1] - 0.2 * ty), (float)(O[2] - 0.2 * tz),
-                         (float)(O[3] - 0.2 * tx));
-    bq4[b] = make_float4((float)O[4], (float)(O[6] - 0.6 * ty), (float)(O[7] - 0.2 * tz), 0.0f);
-    // the learned criterion's per-cell features, FP64 then rounded (see band_features)
-    const double k = 0.125, cn = (double)ncount[p], sz = c.w, tiny = 9.5367431640625e-07;
-    const double qxx = 3.0 * Q[4] - tr, qyy = 3.0 * Q[5] - tr, qzz = 3.0 * Q[6] - tr;
-    const double qn = sqrt(qxx * qxx + qyy * qyy + qzz * qzz +
-                           2.0 * (9.0 * Q[7] * Q[7] + 9.0 * Q[8] * Q[8] + 9.0 * Q[9] * Q[9]));
-    const double dn = sqrt(Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]);
-    bq2[b] = make_float4((float)(3.0 * Q[9]), (float)sz, (float)cn, (float)(k * log2(sz)));
-    bf[b] = make_float4((float)(k * log2(fabs(Q[0]) / cn + tiny)), (float)(k * log2(dn / (cn * sz) + tiny)),
-                        (float)(k * log2(qn / (cn * sz * sz) + tiny)), (float)(k * log2(cn)));
-}
-
 __global__ void k_zero_ctr(int32_t *ctr, int lo, int hi)
 {
     const int i = lo + (int)threadIdx.x;
@@ -1492,8 +1424,8 @@ struct LabelArgs {
     const double *fx[2], *fy[2], *fz[2];  // forces of the collecting walk
     int n[2];
     const uint32_t *bfs;                  // breadth-first -> preorder (joint tree)
-    const double4 *nc;
-    const double *nq;
+    const double4 *bmac;                  // breadth-first: cell centres
+    const double *bnq;                    // breadth-first: cells' FP64 moments
     const int32_t *nstart, *ncount;
     const double4 *pos;                   // joint particles: x, y, z, charge
     double rs;
@@ -1515,8 +1447,8 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const double qt = tsp ? 1.0 : -1.0;
     const int2 gr = A.grp[tsp][(int)R.group - (tsp ? A.ng0 : 0)];
     const uint32_t p = A.bfs[R.cell];
-    const double4 c = A.nc[p];
-    const double *Q = A.nq + (size_t)p * TB_JQ;
+    const double4 c = A.bmac[R.cell];
+    const double *Q = A.bnq + (size_t)R.cell * TB_JQ;
     const double q = Q[0], Dx = Q[1], Dy = Q[2], Dz = Q[3];
     const double tr = Q[4] + Q[5] + Q[6];
     const double Qxx = 3.0 * Q[4] - tr, Qyy = 3.0 * Q[5] - tr, Qzz = 3.0 * Q[6] - tr;
@@ -1903,9 +1835,8 @@ int walk_mode(const MixedDev &M, const tb_config &cfg)
 
 }  // namespace
 
-int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, const tb_config &cfg, size_t *bytes)
+int tb_mixed_prep(hipStream_t s, MixedDev &M, const JointDev *jt, size_t *bytes)
 {
-    (void)sp;
     if (!M.ctr) {
         if (hipMalloc((void **)&M.ctr, 64) != hipSuccess) {
             g_err = "mixed walk: counter allocation failed";
@@ -1913,12 +1844,12 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, cons
         }
         *bytes += 64;
     }
-    const double theta = cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta;
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 2, 3);
     const JointDev &J = *jt;
     MixedTree &T = M.t[0];
     int cap = T.cap, rc;
     int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap, c8 = cap, c9 = cap;
+    int c10 = cap, c11 = cap, c12 = cap, c13 = cap * TB_JQ;  // bnq: TB_JQ doubles per node
     if ((rc = grow(&T.bmac, &c1, J.nn, bytes))) return rc;
     if ((rc = grow(&T.binfo, &c2, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq0, &c3, J.nn, bytes))) return rc;
@@ -1928,18 +1859,40 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, SpeciesDev *sp, JointDev *jt, cons
     if ((rc = grow(&T.bf, &c7, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq3, &c8, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq4, &c9, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bflag, &c10, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bkey, &c11, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bcnt, &c12, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bnq, &c13, J.nn * TB_JQ, bytes))) return rc;
     T.cap = std::min(std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(std::min(c5, c6), c7)),
-                     std::min(c8, c9));
+                     std::min(std::min(c8, c9), std::min(std::min(c10, c11), std::min(c12, c13 / TB_JQ))));
     T.nn = J.nn;
-    hipLaunchKernelGGL(k_inv, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv);
-    hipLaunchKernelGGL(k_mixed_nodes, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, T.inv, J.nc, J.nnext, J.nq,
-                       J.nstart, J.ncount, theta, T.bmac, T.binfo, T.bq0, T.bq1, T.bq2, T.bq3, T.bq4, T.bf,
-                       M.ctr + 2);
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: prep launch failed";
         return TB_ERR_HIP;
     }
     return TB_OK;
+}
+
+CellRecs tb_mixed_recs(MixedDev &M, const tb_config &cfg)
+{
+    MixedTree &T = M.t[0];
+    CellRecs R;
+    R.inv = T.inv;
+    R.bmac = T.bmac;
+    R.binfo = T.binfo;
+    R.bq0 = T.bq0;
+    R.bq1 = T.bq1;
+    R.bq2 = T.bq2;
+    R.bq3 = T.bq3;
+    R.bq4 = T.bq4;
+    R.bf = T.bf;
+    R.bflag = T.bflag;
+    R.bkey = T.bkey;
+    R.bcnt = T.bcnt;
+    R.bnq = T.bnq;
+    R.theta = cfg.theta_mixed > 0.0 ? cfg.theta_mixed : cfg.theta;
+    R.flag = M.ctr + 2;  // read with the walk's flags after the next force
+    return R;
 }
 
 int tb_mixed_force(hipStream_t s, MixedDev &M, SpeciesDev *sp, const tb_config &cfg, double *, double *d_sums)
@@ -1991,8 +1944,8 @@ int tb_mixed_collect(hipStream_t s, MixedDev &M, SpeciesDev *sp, const JointDev 
         A.grp[k] = sp[k].grp;
     }
     A.bfs = jt->bfs;
-    A.nc = jt->nc;
-    A.nq = jt->nq;
+    A.bmac = M.t[0].bmac;
+    A.bnq = M.t[0].bnq;
     A.nstart = jt->nstart;
     A.ncount = jt->ncount;
     A.pos = jt->pos;
@@ -2053,6 +2006,7 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
     MixedTree &T = M.t[0];
     hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.bq2); hipFree(T.inv);
     hipFree(T.bf); hipFree(T.bq3); hipFree(T.bq4);
+    hipFree(T.bflag); hipFree(T.bkey); hipFree(T.bcnt); hipFree(T.bnq);
     T = MixedTree();
     if (M.far_cells) hipFree(M.far_cells);
     if (M.far_link) hipFree(M.far_link);
