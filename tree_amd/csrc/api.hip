@@ -172,10 +172,9 @@ void free_species(tb_handle *h, SpeciesDev &S)
     dfree(h, S.gsid, n * 4);
     dfree(h, S.grp, n * 8);
     size_t c = (size_t)S.cap;
-    dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
+    dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
     dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
-    dfree(h, S.bm2, c * 48);
     for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
     S = SpeciesDev();
 }
@@ -210,10 +209,9 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
 {
     if (nn <= S.cap) return TB_OK;
     size_t c = (size_t)S.cap;
-    dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80); dfree(h, S.nm2, c * 48);
+    dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
     dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
-    dfree(h, S.bm2, c * 48);
     for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
     S.cap = 0;
@@ -221,7 +219,6 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     if ((rc = dalloc_t(h, &S.nmac, cap))) return rc;
     if ((rc = dalloc_t(h, &S.nnext, cap))) return rc;
     if ((rc = dalloc_t(h, &S.nmom, (size_t)cap * 10))) return rc;
-    if ((rc = dalloc_t(h, &S.nm2, (size_t)cap * 6))) return rc;
     if ((rc = dalloc_t(h, &S.ndx, cap))) return rc;
     if ((rc = dalloc_t(h, &S.nstart, cap))) return rc;
     if ((rc = dalloc_t(h, &S.ncount, cap))) return rc;
@@ -230,7 +227,6 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     if ((rc = dalloc_t(h, &S.bflag, cap))) return rc;
     if ((rc = dalloc_t(h, &S.bkids, cap))) return rc;
     if ((rc = dalloc_t(h, &S.bxq, cap))) return rc;
-    if ((rc = dalloc_t(h, &S.bm2, (size_t)cap * 6))) return rc;
     for (int b = 0; b < 2; b++) {
         if ((rc = dalloc_t(h, &S.dkey[b], cap))) return rc;
         if ((rc = dalloc_t(h, &S.dval[b], cap))) return rc;
@@ -956,7 +952,7 @@ int tb_export_tree(tb_handle *h, int32_t species, int32_t cap, int32_t *nnodes, 
             *d_next = d_cnt + n;
     double *dd = (double *)(((uintptr_t)(d_next + n) + 15) & ~(uintptr_t)15);
     double *d_xx = dd, *d_q = d_xx + 3 * n, *d_dx = d_q + n, *d_D = d_dx + n, *d_Q = d_D + 3 * n, *d_QQ = d_Q + 3 * n;
-    tb_export_gather(h->s, S, d_depth, d_leaf, d_pt, d_cnt, d_next, d_xx, d_q, d_dx, d_D, d_Q, d_QQ);
+    tb_export_gather(h->s, S, species ? 1 : -1, d_depth, d_leaf, d_pt, d_cnt, d_next, d_xx, d_q, d_dx, d_D, d_Q, d_QQ);
     double root[6];
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->s);

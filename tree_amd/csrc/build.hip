@@ -413,9 +413,8 @@ __global__ __launch_bounds__(BT) void k_lcp(int n, const uint64_t *__restrict__ 
 __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__ key, const int32_t *__restrict__ offs,
                                              int total, const double *__restrict__ px, const double *__restrict__ py,
                                              const double *__restrict__ pz, double sign, double4 *__restrict__ nmac,
-                                             uint32_t *__restrict__ nnext, double *__restrict__ nmom,
-                                             double *__restrict__ nm2, double *__restrict__ ndx,
-                                             int32_t *__restrict__ nstart, int32_t *__restrict__ ncount,
+                                             uint32_t *__restrict__ nnext, int32_t *__restrict__ nstart,
+                                             int32_t *__restrict__ ncount,
                                              int32_t *__restrict__ ndepth, uint64_t *__restrict__ dkey)
 {
     int i = blockIdx.x * BT + threadIdx.x;
@@ -460,13 +459,10 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
         uint32_t nxt = end >= n ? (uint32_t)total : (uint32_t)(offs[end] + 1);
         if (d == leafd) {
             nnext[id] = nxt | TB_LEAF;
-            nmac[id] = make_double4(px[i], py[i], pz[i], -1.0);   // leaf xx = particle, tree.f90:461
-            double *m = nmom + (long)id * 10;
-            m[0] = sign;
-            for (int c = 1; c < 10; c++) m[c] = 0.0;
-            double *q = nm2 + (long)id * 6;
-            for (int c = 0; c < 6; c++) q[c] = 0.0;
-            ndx[id] = 0.0;
+            // leaf xx = particle, tree.f90:461; its moments (qeff = sign,
+            // no D, Q, QQ, dx) are implied: the walk takes leaves by P2P and
+            // the upward pass and the export know a leaf's values
+            nmac[id] = make_double4(px[i], py[i], pz[i], -1.0);
         } else {
             nnext[id] = nxt;
         }
@@ -543,8 +539,6 @@ __global__ __launch_bounds__(BT) void k_bfs_kids(int nn, const uint32_t *__restr
 // - replays its geometric centre from its key (Insert_Tree's placement);
 // - computes centroid, dx, D, Q, QQ exactly as PSeudoE/I (tree.f90:450-553),
 //   from its children's centroids and qeff = sign * count;
-// - and the exact second moment (parallel-axis M2M) for the mixed walk
-//   (a leaf child's is zero: cm + x is x, so it is not read);
 // - writes the preorder table and its own breadth-first companion record.
 // Children outside the next level are a malformed table: flag 8.
 __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int cend,
@@ -552,10 +546,10 @@ __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int c
                                                const uint8_t *__restrict__ bflag, const uint64_t *__restrict__ key,
                                                const double *__restrict__ root, double sign, double theta,
                                                double4 *__restrict__ nmac, double *__restrict__ nmom,
-                                               double *__restrict__ nm2, double *__restrict__ ndx,
-                                               const int32_t *__restrict__ nstart, const int32_t *__restrict__ ndepth,
+                                               double *__restrict__ ndx, const int32_t *__restrict__ nstart,
+                                               const int32_t *__restrict__ ndepth,
                                                const int32_t *__restrict__ ncount, double4 *__restrict__ bxq,
-                                               double *__restrict__ bm2, int32_t *__restrict__ err)
+                                               int32_t *__restrict__ err)
 {
     const int t = blockIdx.x * BT + threadIdx.x;
     if (t >= cnt) return;
@@ -598,7 +592,6 @@ __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int c
     double ax = X - ox, ay = Y - oy, az = Z - oz;
     double dx = sqrt(ax * ax + ay * ay + az * az);
     double D0 = 0.0, D1 = 0.0, D2 = 0.0, Q0 = 0.0, Q1 = 0.0, Q2 = 0.0, R0 = 0.0, R1 = 0.0, R2 = 0.0;
-    double M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int c = 0; c < nch; c++) {
         const double4 m = bxq[j0 + c];
         const double q = m.w;
@@ -612,23 +605,6 @@ __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int c
         R0 = R0 + q * d0 * d1;
         R1 = R1 + q * d1 * d2;
         R2 = R2 + q * d2 * d0;
-        if (bflag[j0 + c] & 2) {
-            // a leaf's second moments are zero: cm + x is x exactly
-            M[0] = M[0] + q * (d0 * d0);
-            M[1] = M[1] + q * (d1 * d1);
-            M[2] = M[2] + q * (d2 * d2);
-            M[3] = M[3] + q * (d0 * d1);
-            M[4] = M[4] + q * (d1 * d2);
-            M[5] = M[5] + q * (d2 * d0);
-        } else {
-            const double *cm = bm2 + (size_t)(j0 + c) * 6;
-            M[0] = M[0] + (cm[0] + q * (d0 * d0));
-            M[1] = M[1] + (cm[1] + q * (d1 * d1));
-            M[2] = M[2] + (cm[2] + q * (d2 * d2));
-            M[3] = M[3] + (cm[3] + q * (d0 * d1));
-            M[4] = M[4] + (cm[4] + q * (d1 * d2));
-            M[5] = M[5] + (cm[5] + q * (d2 * d0));
-        }
     }
     double s = lx;
     if (ly > s) s = ly;
@@ -641,16 +617,15 @@ __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int c
     mo[1] = D0; mo[2] = D1; mo[3] = D2;
     mo[4] = Q0; mo[5] = Q1; mo[6] = Q2;
     mo[7] = R0; mo[8] = R1; mo[9] = R2;
-    double *m2 = nm2 + (size_t)p * 6;
-    double *b2 = bm2 + (size_t)b * 6;
-    for (int c = 0; c < 6; c++) m2[c] = b2[c] = M[c];
     ndx[p] = dx;
     bxq[b] = make_double4(X, Y, Z, qeff);
 }
 
 // ------------------------------------------------------------ export
 
-__global__ __launch_bounds__(BT) void k_export(int nn, const double4 *__restrict__ nmac,
+// a leaf's moments are implied (qeff = sign, D = Q = QQ = 0, dx = 0): its
+// records are not written by the build
+__global__ __launch_bounds__(BT) void k_export(int nn, double sign, const double4 *__restrict__ nmac,
                                                const uint32_t *__restrict__ nnext, const double *__restrict__ nmom,
                                                const double *__restrict__ ndx, const int32_t *__restrict__ nstart,
                                                const int32_t *__restrict__ ncount, const int32_t *__restrict__ ndepth,
@@ -669,6 +644,12 @@ __global__ __launch_bounds__(BT) void k_export(int nn, const double4 *__restrict
     next[i] = (int32_t)(nx & ~TB_LEAF);
     double4 m = nmac[i];
     xx[3 * i] = m.x; xx[3 * i + 1] = m.y; xx[3 * i + 2] = m.z;
+    if (lf) {
+        qeff[i] = sign;
+        for (int k = 0; k < 3; k++) D[3 * i + k] = Q[3 * i + k] = QQ[3 * i + k] = 0.0;
+        dx[i] = 0.0;
+        return;
+    }
     const double *mo = nmom + (long)i * 10;
     qeff[i] = mo[0];
     for (int k = 0; k < 3; k++) {
@@ -1292,7 +1273,7 @@ void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign)
 {
     const int c = S.cur;
     hipLaunchKernelGGL(k_emit, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, key, S.offs, S.nn, S.x[c][0], S.x[c][1],
-                       S.x[c][2], (double)sign, S.nmac, S.nnext, S.nmom, S.nm2, S.ndx, S.nstart, S.ncount, S.ndepth,
+                       S.x[c][2], (double)sign, S.nmac, S.nnext, S.nstart, S.ncount, S.ndepth,
                        S.dkey[0]);
 }
 
@@ -1307,15 +1288,16 @@ void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, doub
         if (cnt <= 0 || S.twig_hist[d] == 0) continue;
         hipLaunchKernelGGL(k_upward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, beg, cnt, S.lvl_off[d + 1],
                            S.lvl_off[d + 2], S.bfs, S.bkids, S.bflag, key, S.root, (double)sign, theta, S.nmac,
-                           S.nmom, S.nm2, S.ndx, S.nstart, S.ndepth, S.ncount, S.bxq, S.bm2, err);
+                           S.nmom, S.ndx, S.nstart, S.ndepth, S.ncount, S.bxq, err);
     }
 }
 
-void tb_export_gather(hipStream_t s, const SpeciesDev &S, int32_t *depth, int32_t *leaf, int32_t *pt,
+void tb_export_gather(hipStream_t s, const SpeciesDev &S, int sign, int32_t *depth, int32_t *leaf, int32_t *pt,
                       int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,
                       double *QQ)
 {
-    hipLaunchKernelGGL(k_export, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.nmac, S.nnext, S.nmom, S.ndx,
+    hipLaunchKernelGGL(k_export, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, (double)sign, S.nmac, S.nnext, S.nmom,
+                       S.ndx,
                        S.nstart, S.ncount, S.ndepth, S.orig[S.cur], depth, leaf, pt, count, next, xx, qeff, dx, D, Q,
                        QQ);
 }

@@ -32,7 +32,6 @@ struct SpeciesDev {
     double4 *nmac = nullptr;    // centroid xyz, rcrit = max(lx)/theta + dx (leaf: -1)
     uint32_t *nnext = nullptr;  // preorder index after the subtree | TB_LEAF
     double *nmom = nullptr;     // [nn][10] qeff, D(3), Q(3), QQ(3)  (child-point, tree.f90:537-551)
-    double *nm2 = nullptr;      // [nn][6] exact second moment about the centroid xx,yy,zz,xy,yz,zx
     double *ndx = nullptr;      // centroid offset from the geometric centre
     int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
     uint64_t *dkey[2] = {};     // depth-sort keys / node ids -> breadth-first order
@@ -44,12 +43,11 @@ struct SpeciesDev {
     // children (a contiguous run of the next level) without chasing pointers:
     // preorder -> breadth-first index; per breadth-first node bit 0 = first
     // child of its parent, bit 1 = leaf; per cell, first child << 3 |
-    // children - 1; centroid and qeff; exact second moment (cells)
+    // children - 1; centroid and qeff
     uint32_t *binv = nullptr;
     uint8_t *bflag = nullptr;
     uint32_t *bkids = nullptr;
     double4 *bxq = nullptr;
-    double *bm2 = nullptr;
     // mixed walk: target groups of at most gG consecutive key-ordered
     // particles, each inside one cell of at most 8 gG particles (tb_groups)
     const uint64_t *skey = nullptr;  // this build's sorted keys
@@ -110,7 +108,8 @@ void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t
 size_t tb_scan_temp_bytes(int n);
 void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign);
 void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err);
-void tb_export_gather(hipStream_t s, const SpeciesDev &S, int32_t *depth, int32_t *leaf, int32_t *pt,
+// sign: the species' charge (-1 electrons, +1 ions), a leaf's qeff
+void tb_export_gather(hipStream_t s, const SpeciesDev &S, int sign, int32_t *depth, int32_t *leaf, int32_t *pt,
                       int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,
                       double *QQ);
 
