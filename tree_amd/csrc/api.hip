@@ -175,6 +175,7 @@ void free_species(tb_handle *h, SpeciesDev &S)
     dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
     dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
+    dfree(h, S.bkey, c * 8);
     for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
     S = SpeciesDev();
 }
@@ -212,6 +213,7 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     dfree(h, S.nmac, c * 32); dfree(h, S.nnext, c * 4); dfree(h, S.nmom, c * 80);
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
     dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
+    dfree(h, S.bkey, c * 8);
     for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
     S.cap = 0;
@@ -226,6 +228,7 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     if ((rc = dalloc_t(h, &S.binv, cap))) return rc;
     if ((rc = dalloc_t(h, &S.bflag, cap))) return rc;
     if ((rc = dalloc_t(h, &S.bkids, cap))) return rc;
+    if ((rc = dalloc_t(h, &S.bkey, cap))) return rc;
     if ((rc = dalloc_t(h, &S.bxq, cap))) return rc;
     for (int b = 0; b < 2; b++) {
         if ((rc = dalloc_t(h, &S.dkey[b], cap))) return rc;
@@ -376,14 +379,16 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nu
     if (mixed && (rc = ensure_joint(h, nj))) return rc;
     HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * MISC_N, s));
     const size_t pstride = eb ? (size_t)eb->nreal * eb->chunks * 6 : 0;
-    // the two species' keys, sorts and groups are independent: ions on s1
+    // the two species' keys, sorts and groups are independent: ions on s1,
+    // each stage enqueued for both species before the next, so that both
+    // streams have work from the start
     HIPCHK(hipEventRecord(h->fork, s));
     HIPCHK(hipStreamWaitEvent(h->s1, h->fork, 0));
+    const hipStream_t st2[2] = {s, h->s1};
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
         const int c = S.cur;
-        hipStream_t ss = sp ? h->s1 : s;
-        void *sort_t = sp ? h->sort_tmp1 : h->sort_tmp, *scan_t = sp ? h->scan_tmp1 : h->scan_tmp;
+        hipStream_t ss = st2[sp];
         if (eb) {
             double *roots = eb->d_roots + (size_t)sp * eb->nreal * 6;
             tb_sbbox(ss, eb->nreal, eb->chunks, eb->d_off[sp], S.x[c][0], S.x[c][1], S.x[c][2],
@@ -395,7 +400,18 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nu
             tb_bbox(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, sp ? h->bbox_tmp1 : h->bbox_tmp);
             tb_keys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
         }
-        tb_radix_sort(ss, S.n, S.key, S.val, 0, 64, sort_t, &which[sp], false);
+    }
+    {
+        const int n2[2] = {h->sp[0].n, h->sp[1].n};
+        uint64_t **k2[2] = {h->sp[0].key, h->sp[1].key};
+        uint32_t **v2[2] = {h->sp[0].val, h->sp[1].val};
+        void *const t2[2] = {h->sort_tmp, h->sort_tmp1};
+        tb_radix_sort2(st2, n2, k2, v2, 0, 64, t2, which, false);
+    }
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        hipStream_t ss = st2[sp];
+        void *scan_t = sp ? h->scan_tmp1 : h->scan_tmp;
         tb_permute(ss, S, S.val[which[sp]], h->d_misc + sp);
         S.skey = S.key[which[sp]];
         if (mixed)
@@ -511,12 +527,14 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         if ((rc = ensure_nodes(h, S, S.nn))) return rc;
     }
     if ((rc = ensure_tmp(h, std::max(h->sp[0].nn, h->sp[1].nn)))) return rc;
-    // each species' linking, depth sort and upward pass: ions on s1
+    // each species' linking, depth sort and upward pass: ions on s1, stage
+    // by stage for both species
     HIPCHK(hipEventRecord(h->fork, s));
     HIPCHK(hipStreamWaitEvent(h->s1, h->fork, 0));
+    const hipStream_t st2[2] = {s, h->s1};
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
-        hipStream_t ss = sp ? h->s1 : s;
+        hipStream_t ss = st2[sp];
         const int32_t *hist = h->h_misc + MISC_HIST + sp * 2 * HW;
         int all[HW + 1];
         for (int d = 0; d < HW; d++) {
@@ -529,16 +547,23 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
             S.lvl_off[d + 1] = S.lvl_off[d] + all[d];
             if (all[d]) S.maxdepth = d;
         }
-        const uint64_t *key = S.key[which[sp]];
-        const int sign = sp == 0 ? -1 : 1;
-        tb_emit(ss, S, key, sign);
-        int w2;
-        tb_radix_sort(ss, S.nn, S.dkey, S.dval, 0, 8, sp ? h->sort_tmp1 : h->sort_tmp, &w2, true);
-        S.bfs = S.dval[w2];
-        tb_upward(ss, S, key, sign, h->cfg.theta, h->d_misc + sp);
+        tb_emit(ss, S, S.key[which[sp]], sp == 0 ? -1 : 1);
         h->st.nodes[sp] = S.nn;
         h->st.max_depth[sp] = S.maxdepth;
     }
+    {
+        const int n2[2] = {h->sp[0].nn, h->sp[1].nn};
+        uint64_t **k2[2] = {h->sp[0].dkey, h->sp[1].dkey};
+        uint32_t **v2[2] = {h->sp[0].dval, h->sp[1].dval};
+        void *const t2[2] = {h->sort_tmp, h->sort_tmp1};
+        int w2[2];
+        tb_radix_sort2(st2, n2, k2, v2, 0, 8, t2, w2, true);
+        for (int sp = 0; sp < 2; sp++) h->sp[sp].bfs = h->sp[sp].dval[w2[sp]];
+    }
+    for (int sp = 0; sp < 2; sp++)
+        tb_upward_prep(st2[sp], h->sp[sp], h->sp[sp].key[which[sp]], sp == 0 ? -1 : 1, h->d_misc + sp);
+    for (int d = std::max(h->sp[0].maxdepth, h->sp[1].maxdepth); d >= 0; d--)
+        for (int sp = 0; sp < 2; sp++) tb_upward_level(st2[sp], h->sp[sp], d, h->cfg.theta, h->d_misc + sp);
     HIPCHK(hipEventRecord(h->join, h->s1));
     HIPCHK(hipStreamWaitEvent(s, h->join, 0));
     HIPCHK(hipGetLastError());

@@ -483,12 +483,15 @@ __global__ __launch_bounds__(BT) void k_bfs_inv(int nn, const uint32_t *__restri
 // whether the node is its parent's first child (in preorder a first child
 // directly follows its parent, one level up; any other node follows a deeper
 // one) and whether it is a leaf; a leaf's position and qeff (its particle,
-// the species' sign).
+// the species' sign); a cell's first particle's key.
 __global__ __launch_bounds__(BT) void k_bfs_prep(int nn, const uint32_t *__restrict__ binv,
                                                  const uint32_t *__restrict__ nnext,
                                                  const int32_t *__restrict__ ndepth,
+                                                 const int32_t *__restrict__ nstart,
+                                                 const uint64_t *__restrict__ key,
                                                  const double4 *__restrict__ nmac, double sign,
-                                                 uint8_t *__restrict__ bflag, double4 *__restrict__ bxq)
+                                                 uint8_t *__restrict__ bflag, uint64_t *__restrict__ bkey,
+                                                 double4 *__restrict__ bxq)
 {
     const int q = blockIdx.x * BT + threadIdx.x;
     if (q >= nn) return;
@@ -499,6 +502,8 @@ __global__ __launch_bounds__(BT) void k_bfs_prep(int nn, const uint32_t *__restr
     if (leaf) {
         const double4 m = nmac[q];
         bxq[j] = make_double4(m.x, m.y, m.z, sign);
+    } else {
+        bkey[j] = key[nstart[q]];
     }
 }
 
@@ -536,19 +541,19 @@ __global__ __launch_bounds__(BT) void k_bfs_kids(int nn, const uint32_t *__restr
 // One level of twigs, deepest level first.  A twig's children are the run
 // of the next level [binv[p + 1], next first child): preorder order, which
 // is the reference's child order 1..8.  Each twig
-// - replays its geometric centre from its key (Insert_Tree's placement);
+// - replays its geometric centre from its first particle's key (Insert_Tree's
+//   placement);
 // - computes centroid, dx, D, Q, QQ exactly as PSeudoE/I (tree.f90:450-553),
-//   from its children's centroids and qeff = sign * count;
+//   from its children's centroids and qeff = sign * count (its own qeff is
+//   their sum: integers, exact);
 // - writes the preorder table and its own breadth-first companion record.
 // Children outside the next level are a malformed table: flag 8.
-__global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int cend,
+__global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int cend, int d,
                                                const uint32_t *__restrict__ bfs, const uint32_t *__restrict__ bkids,
-                                               const uint8_t *__restrict__ bflag, const uint64_t *__restrict__ key,
-                                               const double *__restrict__ root, double sign, double theta,
+                                               const uint8_t *__restrict__ bflag, const uint64_t *__restrict__ bkey,
+                                               const double *__restrict__ root, double theta,
                                                double4 *__restrict__ nmac, double *__restrict__ nmom,
-                                               double *__restrict__ ndx, const int32_t *__restrict__ nstart,
-                                               const int32_t *__restrict__ ndepth,
-                                               const int32_t *__restrict__ ncount, double4 *__restrict__ bxq,
+                                               double *__restrict__ ndx, double4 *__restrict__ bxq,
                                                int32_t *__restrict__ err)
 {
     const int t = blockIdx.x * BT + threadIdx.x;
@@ -562,11 +567,10 @@ __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int c
         atomicOr(err, 8);
         return;
     }
-    const int d = ndepth[p];
     // geometric centre and size of the cell, replaying Insert_Tree's placement
     double ox = root[0], oy = root[1], oz = root[2];
     double lx = root[3], ly = root[4], lz = root[5];
-    uint64_t k = key[nstart[p]];
+    const uint64_t k = bkey[b];
     for (int l = 0; l < d; l++) {
         uint32_t dig = (uint32_t)(k >> (3 * (TB_LEVELS - 1 - l))) & 7u;
         ox = ox + ((dig & 1u) ? 0.25 : -0.25) * lx;
@@ -611,7 +615,7 @@ __global__ __launch_bounds__(BT) void k_upward(int beg, int cnt, int cbeg, int c
     if (lz > s) s = lz;
     double rcrit = s / theta + dx;   // MAC left side, tree.f90:742
     nmac[p] = make_double4(X, Y, Z, rcrit);
-    const double qeff = sign * (double)ncount[p];
+    const double qeff = qsum;  // sign * count
     double *mo = nmom + (size_t)p * 10;
     mo[0] = qeff;
     mo[1] = D0; mo[2] = D1; mo[3] = D2;
@@ -1241,6 +1245,29 @@ void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int
     *which = src;
 }
 
+void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], uint32_t **val[2], int lo_bit,
+                    int hi_bit, void *const tmp[2], int which[2], bool init_iota)
+{
+    int nb[2], src[2] = {0, 0};
+    for (int k = 0; k < 2; k++) nb[k] = nblk(n[k], RS_TILE);
+    bool iota = init_iota;
+    for (int sh = lo_bit; sh < hi_bit; sh += 8) {
+        for (int k = 0; k < 2; k++) {
+            uint32_t *hist = (uint32_t *)tmp[k];
+            uint32_t *totals = hist + (size_t)256 * nb[k];
+            uint64_t **kk = key[k];
+            uint32_t **vv = val[k];
+            hipLaunchKernelGGL(k_rs_up, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], sh, hist, nb[k]);
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s[k], nb[k], hist, totals);
+            hipLaunchKernelGGL(k_rs_down, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], vv[src[k]],
+                               kk[1 - src[k]], vv[1 - src[k]], sh, hist, totals, nb[k], iota ? 1 : 0);
+            src[k] = 1 - src[k];
+        }
+        iota = false;
+    }
+    for (int k = 0; k < 2; k++) which[k] = src[k];
+}
+
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err)
 {
     const int c = S.cur, o = 1 - S.cur;
@@ -1277,20 +1304,23 @@ void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign)
                        S.dkey[0]);
 }
 
-void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err)
+void tb_upward_prep(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, int32_t *err)
 {
     hipLaunchKernelGGL(k_bfs_inv, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.binv);
-    hipLaunchKernelGGL(k_bfs_prep, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.binv, S.nnext, S.ndepth, S.nmac,
-                       (double)sign, S.bflag, S.bxq);
+    hipLaunchKernelGGL(k_bfs_prep, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.binv, S.nnext, S.ndepth, S.nstart,
+                       key, S.nmac, (double)sign, S.bflag, S.bkey, S.bxq);
     hipLaunchKernelGGL(k_bfs_kids, dim3(nblk(S.nn, BT)), dim3(BT), 0, s, S.nn, S.bfs, S.binv, S.bflag, S.bkids, err);
-    for (int d = S.maxdepth; d >= 0; d--) {
-        int beg = S.lvl_off[d], cnt = S.lvl_off[d + 1] - beg;
-        if (cnt <= 0 || S.twig_hist[d] == 0) continue;
-        hipLaunchKernelGGL(k_upward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, beg, cnt, S.lvl_off[d + 1],
-                           S.lvl_off[d + 2], S.bfs, S.bkids, S.bflag, key, S.root, (double)sign, theta, S.nmac,
-                           S.nmom, S.ndx, S.nstart, S.ndepth, S.ncount, S.bxq, err);
-    }
 }
+
+void tb_upward_level(hipStream_t s, SpeciesDev &S, int d, double theta, int32_t *err)
+{
+    if (d < 0 || d > S.maxdepth) return;
+    int beg = S.lvl_off[d], cnt = S.lvl_off[d + 1] - beg;
+    if (cnt <= 0 || S.twig_hist[d] == 0) return;
+    hipLaunchKernelGGL(k_upward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, beg, cnt, S.lvl_off[d + 1], S.lvl_off[d + 2], d,
+                       S.bfs, S.bkids, S.bflag, S.bkey, S.root, theta, S.nmac, S.nmom, S.ndx, S.bxq, err);
+}
+
 
 void tb_export_gather(hipStream_t s, const SpeciesDev &S, int sign, int32_t *depth, int32_t *leaf, int32_t *pt,
                       int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,

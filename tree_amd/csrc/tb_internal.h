@@ -47,6 +47,7 @@ struct SpeciesDev {
     uint32_t *binv = nullptr;
     uint8_t *bflag = nullptr;
     uint32_t *bkids = nullptr;
+    uint64_t *bkey = nullptr;   // a cell's first particle's key (its centre replays from it)
     double4 *bxq = nullptr;
     // mixed walk: target groups of at most gG consecutive key-ordered
     // particles, each inside one cell of at most 8 gG particles (tb_groups)
@@ -100,6 +101,10 @@ void tb_keys(hipStream_t s, int n, const double *x, const double *y, const doubl
 // sorts key[0]/val[0] by bits [lo_bit, hi_bit); result ends in key[*which]/val[*which]
 void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int lo_bit, int hi_bit, void *tmp,
                    int *which, bool init_iota);
+// two independent sorts (key sets k = 0, 1 with their own temporaries), their
+// passes enqueued alternately on two streams so both run from the start
+void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], uint32_t **val[2], int lo_bit,
+                    int hi_bit, void *const tmp[2], int which[2], bool init_iota);
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err);
 // joint: the mixed walk's joint tree (up to 8 particles may share a full key)
 void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist /*2*(TB_LEVELS+2)*/,
@@ -107,7 +112,9 @@ void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hi
 void tb_scan_excl(hipStream_t s, int n, const int32_t *in, int32_t *out, int32_t *total, void *tmp);
 size_t tb_scan_temp_bytes(int n);
 void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign);
-void tb_upward(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, double theta, int32_t *err);
+// the upward pass: the breadth-first companions, then each level (deepest first)
+void tb_upward_prep(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, int32_t *err);
+void tb_upward_level(hipStream_t s, SpeciesDev &S, int d, double theta, int32_t *err);
 // sign: the species' charge (-1 electrons, +1 ions), a leaf's qeff
 void tb_export_gather(hipStream_t s, const SpeciesDev &S, int sign, int32_t *depth, int32_t *leaf, int32_t *pt,
                       int32_t *count, int32_t *next, double *xx, double *qeff, double *dx, double *D, double *Q,
