@@ -77,15 +77,18 @@ def _direct_err(eng, qel, qion, n):
 @pytest.mark.parametrize("theta", [0.26, 0.5])
 def test_learned_mac_matches_fixed(theta):
     """Trained on labels the method generates, the learned criterion is tuned
-    (err_target bisected in log10) to the fixed-theta walk's rms error,
-    sampled against the FP64 direct sum, and its work (M2P + P2P
+    (err_target bisected in log10) to the fixed-theta walk's rms error within
+    1%, as in the other learned-criterion tests (opening every band cell does
+    not lower the error monotonically, so an exact match is not always
+    reachable), sampled against the FP64 direct sum, and its work (M2P + P2P
     interactions) is compared there: 2^17 + 2^17 particles; cells beyond
     s/(0.97 theta) always accepted, cells within s/(1.03 theta) always opened,
     the network deciding the near band cells in between (far ones are decided
     at the band's geometric mean, as the fixed walk decides them).  The
     learned walk must reach the fixed walk's error within 3% of its work.
-    Measured at this seed: 1.001x (theta 0.26) and 0.988x (theta 0.5); over
-    two seeds and three bands 0.96-1.05x.  With quadrupole cells the network
+    Measured at this seed: 1.014x (theta 0.26, where every band cell opened
+    gives rms 8.15e-5 against the fixed walk's 8.10e-5) and 0.986x (theta
+    0.5); over two seeds and three bands 0.96-1.05x.  With quadrupole cells the network
     saved 4-13%; the cells' third moments and the far path through order-6
     local expansions leave it no systematic gain."""
     announce("learned_mac_vs_fixed theta=%g" % theta)
@@ -114,7 +117,7 @@ def test_learned_mac_matches_fixed(theta):
             rows.append((mid, err_l, work_l))
             print("err_target 1e%.2f: learned rms %.3e work %d | fixed rms %.3e work %d"
                   % (mid, err_l, work_l, err_fix, work_fix))
-            if err_l <= err_fix:
+            if err_l <= 1.01 * err_fix:
                 best, lo = (mid, err_l, work_l), mid
             else:
                 hi = mid

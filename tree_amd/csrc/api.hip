@@ -379,6 +379,24 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nu
     if (mixed && (rc = ensure_joint(h, nj))) return rc;
     HIPCHK(hipMemsetAsync(h->d_misc, 0, sizeof(int32_t) * MISC_N, s));
     const size_t pstride = eb ? (size_t)eb->nreal * eb->chunks * 6 : 0;
+    // The mixed walk keys both species over the joint box (each
+    // realisation's, in an ensemble), so that the joint order is a merge of
+    // the two species' sorted keys.
+    double *jroots = eb ? eb->d_roots + (size_t)2 * eb->nreal * 6 : nullptr;
+    if (mixed) {
+        SpeciesDev &E = h->sp[0], &I = h->sp[1];
+        if (eb) {
+            for (int sp = 0; sp < 2; sp++) {
+                SpeciesDev &S = h->sp[sp];
+                tb_sbbox(s, eb->nreal, eb->chunks, eb->d_off[sp], S.x[S.cur][0], S.x[S.cur][1], S.x[S.cur][2],
+                         eb->d_part + sp * pstride);
+            }
+            tb_sbbox_final(s, eb->nreal, eb->chunks, eb->d_part, eb->d_part + pstride, jroots);
+        } else {
+            tb_jbbox(s, E.n, E.x[E.cur][0], E.x[E.cur][1], E.x[E.cur][2], I.n, I.x[I.cur][0], I.x[I.cur][1],
+                     I.x[I.cur][2], h->jt.root, h->bbox_tmp);
+        }
+    }
     // the two species' keys, sorts and groups are independent: ions on s1,
     // each stage enqueued for both species before the next, so that both
     // streams have work from the start
@@ -390,12 +408,10 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nu
         const int c = S.cur;
         hipStream_t ss = st2[sp];
         if (eb) {
-            double *roots = eb->d_roots + (size_t)sp * eb->nreal * 6;
-            tb_sbbox(ss, eb->nreal, eb->chunks, eb->d_off[sp], S.x[c][0], S.x[c][1], S.x[c][2],
-                     eb->d_part + sp * pstride);
-            tb_sbbox_final(ss, eb->nreal, eb->chunks, eb->d_part + sp * pstride, nullptr, roots);
-            tb_ekeys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], eb->d_off[sp], eb->nreal, roots, eb->T, S.key[0],
+            tb_ekeys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], eb->d_off[sp], eb->nreal, jroots, eb->T, S.key[0],
                      S.val[0], 0u, h->d_misc + sp);
+        } else if (mixed) {
+            tb_keys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], h->jt.root, S.key[0], S.val[0], h->d_misc + sp);
         } else {
             tb_bbox(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, sp ? h->bbox_tmp1 : h->bbox_tmp);
             tb_keys(ss, S.n, S.x[c][0], S.x[c][1], S.x[c][2], S.root, S.key[0], S.val[0], h->d_misc + sp);
@@ -426,27 +442,13 @@ int build_phase_a(tb_handle *h, int *which, int tpl = 0, const EnsBuild *eb = nu
     HIPCHK(hipEventRecord(h->join, h->s1));
     HIPCHK(hipStreamWaitEvent(s, h->join, 0));
     if (mixed) {
-        // joint tree over both species, on the freshly permuted positions
+        // joint tree over both species, on the freshly permuted positions:
+        // its key order merged from the species' sorted keys
         JointDev &J = h->jt;
         SpeciesDev &E = h->sp[0], &I = h->sp[1];
         const int ce = E.cur, ci = I.cur;
-        if (eb) {
-            // each realisation's joint box from both species' partial boxes
-            double *roots = eb->d_roots + (size_t)2 * eb->nreal * 6;
-            tb_sbbox_final(s, eb->nreal, eb->chunks, eb->d_part, eb->d_part + pstride, roots);
-            tb_ekeys(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], eb->d_off[0], eb->nreal, roots, eb->T, J.key[0],
-                     J.val[0], 0u, h->d_misc + MISC_J);
-            tb_ekeys(s, I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], eb->d_off[1], eb->nreal, roots, eb->T,
-                     J.key[0] + E.n, J.val[0] + E.n, 0x80000000u, h->d_misc + MISC_J);
-        } else {
-            tb_jbbox(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root,
-                     h->bbox_tmp);
-            tb_jkeys(s, E.n, E.x[ce][0], E.x[ce][1], E.x[ce][2], J.root, J.key[0], J.val[0], 0u,
-                     h->d_misc + MISC_J);
-            tb_jkeys(s, I.n, I.x[ci][0], I.x[ci][1], I.x[ci][2], J.root, J.key[0] + E.n, J.val[0] + E.n,
-                     0x80000000u, h->d_misc + MISC_J);
-        }
-        tb_radix_sort(s, nj, J.key, J.val, 0, 64, h->sort_tmp, &which[2], false);
+        tb_jmerge(s, E.n, I.n, E.key[which[0]], I.key[which[1]], J.key[0], J.val[0]);
+        which[2] = 0;
         const double *px[2] = {E.x[ce][0], I.x[ci][0]}, *py[2] = {E.x[ce][1], I.x[ci][1]},
                      *pz[2] = {E.x[ce][2], I.x[ci][2]};
         tb_jgather(s, nj, J.val[which[2]], px, py, pz, J.pos);

@@ -671,41 +671,54 @@ __global__ __launch_bounds__(BT) void k_export(int nn, double sign, const double
 // geometric centre.  In a quasi-neutral plasma the two species' far fields
 // nearly cancel, so expanding the net charge distribution keeps the far-field
 // error proportional to the (small) net field instead of to each species'
-// (large) field.  Keys use the same FP64 centre recurrence over the joint box.
+// (large) field.  Both species are keyed over the joint box with the same FP64
+// centre recurrence (k_keys), so the joint order is a merge of their sorted
+// keys.
 
-__global__ __launch_bounds__(BT) void k_jkeys(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                              const double *__restrict__ z, const double *__restrict__ root,
-                                              uint64_t *__restrict__ key, uint32_t *__restrict__ val,
-                                              uint32_t tag, int32_t *__restrict__ err)
-{
-    int i = blockIdx.x * BT + threadIdx.x;
-    if (i >= n) return;
-    double px = x[i], py = y[i], pz = z[i];
-    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(err, 2);
-    double cx = root[0], cy = root[1], cz = root[2];
-    double lx = root[3], ly = root[4], lz = root[5];
-    uint64_t k = 0;
-#pragma unroll
-    for (int l = 0; l < TB_LEVELS; l++) {
-        uint32_t bx = (px < cx) ? 0u : 1u;
-        uint32_t by = (py < cy) ? 0u : 1u;
-        uint32_t bz = (pz < cz) ? 0u : 1u;
-        k = (k << 3) | (uint64_t)(bx | (by << 1) | (bz << 2));
-        cx = cx + (bx ? 0.25 : -0.25) * lx;
-        cy = cy + (by ? 0.25 : -0.25) * ly;
-        cz = cz + (bz ? 0.25 : -0.25) * lz;
-        lx = lx * 0.5;
-        ly = ly * 0.5;
-        lz = lz * 0.5;
-    }
-    key[i] = k;
-    val[i] = tag | (uint32_t)i;
-}
-
-// joint box from the two species' partial min/max (k_bbox_part output)
+// the two species' position columns (k_jgather's sources)
 struct JSrc {
     const double *x[2], *y[2], *z[2];
 };
+
+// The joint key order from the two species' sorted keys (both formed over the
+// joint box): a merge.  Electron e at rank i goes to i + (ions with a smaller
+// key), ion j to j + (electrons with a key not larger), so equal keys keep
+// the electrons first and each species' own order: the order a stable sort
+// of the electrons' keys followed by the ions' gives.  The payload is the
+// species-tagged index in the species' key order.  One block merges 256
+// consecutive entries of one species: two searches bracket the other
+// species' range its keys fall in, then each lane searches that bracket.
+__global__ __launch_bounds__(BT) void k_jmerge(int ne, int ni, const uint64_t *__restrict__ ke,
+                                               const uint64_t *__restrict__ ki, uint64_t *__restrict__ kout,
+                                               uint32_t *__restrict__ vout)
+{
+    __shared__ int brk[2];
+    const int nbe = (ne + BT - 1) / BT;
+    const bool ion = (int)blockIdx.x >= nbe;
+    const int base = (ion ? (int)blockIdx.x - nbe : (int)blockIdx.x) * BT;
+    const int nself = ion ? ni : ne, nother = ion ? ne : ni;
+    const uint64_t *self = ion ? ki : ke, *other = ion ? ke : ki;
+    const int last = min(base + BT, nself) - 1;
+    // count of other keys < key (electrons) or <= key (ions) in [lo, hi)
+    auto rank = [&](uint64_t key, int lo, int hi) {
+        while (lo < hi) {
+            const int mid = lo + ((hi - lo) >> 1);
+            const uint64_t o = other[mid];
+            if (ion ? (o <= key) : (o < key)) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    if (threadIdx.x == 0) brk[0] = rank(self[base], 0, nother);
+    if (threadIdx.x == 1) brk[1] = rank(self[last], 0, nother);
+    __syncthreads();
+    const int i = base + (int)threadIdx.x;
+    if (i > last) return;
+    const uint64_t key = self[i];
+    const int pos = i + rank(key, brk[0], brk[1]);
+    kout[pos] = key;
+    vout[pos] = (ion ? 0x80000000u : 0u) | (uint32_t)i;
+}
 
 // sorted joint particles: position and charge (electrons -1, ions +1)
 __global__ __launch_bounds__(BT) void k_jgather(int n, const uint32_t *__restrict__ val, JSrc src,
@@ -1341,10 +1354,10 @@ void tb_jbbox(hipStream_t s, int ne, const double *ex, const double *ey, const d
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BT), 0, s, nbe + nbi, tmp, root);
 }
 
-void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
-              uint64_t *key, uint32_t *val, uint32_t tag, int32_t *err)
+void tb_jmerge(hipStream_t s, int ne, int ni, const uint64_t *ke, const uint64_t *ki, uint64_t *kout, uint32_t *vout)
 {
-    hipLaunchKernelGGL(k_jkeys, dim3(nblk(n, BT)), dim3(BT), 0, s, n, x, y, z, root, key, val, tag, err);
+    const int blocks = nblk(ne, BT) + nblk(ni, BT);
+    hipLaunchKernelGGL(k_jmerge, dim3(blocks), dim3(BT), 0, s, ne, ni, ke, ki, kout, vout);
 }
 
 void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x[2], const double *const y[2],

@@ -122,8 +122,9 @@ void tb_export_gather(hipStream_t s, const SpeciesDev &S, int sign, int32_t *dep
 
 void tb_jbbox(hipStream_t s, int ne, const double *ex, const double *ey, const double *ez, int ni, const double *ix,
               const double *iy, const double *iz, double *root, double *tmp);
-void tb_jkeys(hipStream_t s, int n, const double *x, const double *y, const double *z, const double *root,
-              uint64_t *key, uint32_t *val, uint32_t tag, int32_t *err);
+// the joint key order as a merge of the two species' sorted keys (formed over
+// the joint box); payload: species << 31 | index in the species' key order
+void tb_jmerge(hipStream_t s, int ne, int ni, const uint64_t *ke, const uint64_t *ki, uint64_t *kout, uint32_t *vout);
 void tb_jgather(hipStream_t s, int n, const uint32_t *val, const double *const x[2], const double *const y[2],
                 const double *const z[2], double4 *pos);
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key);
