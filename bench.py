@@ -43,15 +43,16 @@ import tree_amd as ta  # noqa: E402
 # FP64 walk flop weights per event (FMA = 2): MAC test, M2P on accept, P2P Coulomb / Kelbg
 W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # mixed walk, per target-entry interaction, counted from walk32.hip's compiled
-# FP32 code (monopole + dipole + traceless quadrupole + traceless third moment
-# M2P 122, Coulomb P2P 18, Kelbg P2P 22); MAC tests are per cell and group and
-# are not counted
-W32_M2P, W32_P2P_C, W32_P2P_K = 122, 18, 22
+# FP32 code (monopole + dipole + traceless quadrupole + traceless third and
+# fourth moments M2P 202: 404 flops in 253 VALU instructions per listed cell
+# for a lane's 2 targets; Coulomb P2P 18, Kelbg P2P 22); MAC tests are per
+# cell and group and are not counted
+W32_M2P, W32_P2P_C, W32_P2P_K = 202, 18, 22
 # far cells: flops per (cell, target group) pair of the order-6 local
-# expansion with the cell's third moment, terms to total order 7 (FMA = 2),
-# counted from the compiled loop of k_far_eval (m2l_acc_p6hot7; 1321 VALU
-# instructions with the cell-record loads)
-W32_M2L_PAIR = 2177
+# expansion with the cell's third and fourth moments, terms to total order 7
+# (FMA = 2), counted from the compiled loop of k_far_eval (m2l_acc_p6hxt7;
+# 1632 VALU instructions with the cell-record loads)
+W32_M2L_PAIR = 2776
 # TFLOP/s, MI355X published vector peaks.  The FP32 figure counts packed FP32
 # (two lanes per instruction); this build keeps packed FP32 off, so one
 # instruction per lane per cycle (78.6) is its own ceiling, stated beside it.

@@ -74,7 +74,7 @@ typedef struct tb_config {
     double far_ratio;  /* TB_WALK_MIXED: an accepted cell farther than far_ratio    */
                        /* times the target group's radius from the group's centre */
                        /* goes through the group's local expansion instead of     */
-                       /* being evaluated at every target (0: default 3, < 0:   */
+                       /* being evaluated at every target (0: default 4, < 0:   */
                        /* off, else >= 2)                                        */
     int32_t far_levels;/* TB_WALK_MIXED with the far path: 0 or 1 (default) the   */
                        /* group level only; 2: two levels, a segment of up to 8   */
@@ -143,10 +143,12 @@ int tb_direct_sample_noalloc(tb_handle *h, int32_t nt, const int32_t *tspecies, 
 
 /*
  * Check of the mixed walk's far-field local expansion (no reference
- * counterpart; tests only).  cells [ncell][19]: R = expansion centre - cell
+ * counterpart; tests only).  cells [ncell][28]: R = expansion centre - cell
  * centre (3), q, D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx (traceless quadrupole),
  * O'xxx, O'xxy, O'xxz, O'xyy, O'xyz, O'yyy, O'yyz (traceless third moment;
- * orders 4 to 6).  The cells' local coefficients of order `order` (3 to 6)
+ * orders 4 to 6), H'xxxx, H'xxxy, H'xxxz, H'xxyy, H'xxyz, H'xyyy, H'xyyz,
+ * H'yyyy, H'yyyz (traceless fourth moment; order 6, the walk's).  The cells'
+ * local coefficients of order `order` (3 to 6)
  * are summed across a wavefront as in the walk and evaluated at the offsets
  * y [ny][3]: out [ny][4] = potential, field x, y, z (FP32).
  */

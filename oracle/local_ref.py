@@ -18,15 +18,17 @@ index dropped):
 
 A cell with centre X and moments q (net charge), D (dipole), traceless
 quadrupole Q' = 3 M - tr(M) I (M the second moment about X) and, optionally,
-the third moment O (O_ijk = sum q s_i s_j s_k, s the offset from X)
-produces at a point g + y, with R = g - X:
+the third moment O (O_ijk = sum q s_i s_j s_k, s the offset from X) and the
+fourth moment H (H_ijkl = sum q s_i s_j s_k s_l) produces at a point g + y,
+with R = g - X:
 
     phi(y) = sum_b L_b y^b,
-    L_b = sum_c C(b + c, b) (-1)^|c| T_{b+c}(R) M_c,   |c| <= 2 (or 3),
+    L_b = sum_c C(b + c, b) (-1)^|c| T_{b+c}(R) M_c,   |c| <= 2 (3, or 4),
 
-with M_0 = q, M_{e_i} = D_i, M_c = Q'_c / 3 for |c| = 2 and M_c = O_c for
-|c| = 3.  Trace parts of the moment tensors drop out because 1/r is
-harmonic, so Q'/3 may stand for M and the traceless part of O for O.  The
+with M_0 = q, M_{e_i} = D_i, M_c = Q'_c / 3 for |c| = 2, M_c = O_c for
+|c| = 3 and M_c = H_c for |c| = 4.  Trace parts of the moment tensors drop
+out because 1/r is harmonic, so Q'/3 may stand for M and the traceless
+parts of O and H for O and H.  The
 field is E = -grad phi.  Keeping |b| <= P gives the field through order
 P - 1 in y.  With tmax, only the terms of total order |b| + |c| <= tmax are
 kept (the walk's order-6 expansion keeps tmax = 7: the dropped terms, a
@@ -76,9 +78,10 @@ def _c(b, c):
     return comb(b[0] + c[0], b[0]) * comb(b[1] + c[1], b[1]) * comb(b[2] + c[2], b[2])
 
 
-def cell_moments(q, D, Qp, O=None):
-    """M_c for |c| <= 2 (3 with O) from q, D[3], the traceless Q' (3x3,
-    symmetric) and the third moment O (3x3x3, symmetric)."""
+def cell_moments(q, D, Qp, O=None, H=None):
+    """M_c for |c| <= 2 (3 with O, 4 with H) from q, D[3], the traceless Q'
+    (3x3, symmetric), the third moment O (3x3x3, symmetric) and the fourth
+    moment H (3x3x3x3, symmetric)."""
     Qp = np.asarray(Qp, dtype=np.float64)
     M = {(0, 0, 0): float(q)}
     for i in range(3):
@@ -93,12 +96,35 @@ def cell_moments(q, D, Qp, O=None):
         for c in multi_indices(3):
             idx = [i for i in range(3) for _ in range(c[i])]
             M[c] = O[idx[0], idx[1], idx[2]]
+    if H is not None:
+        H = np.asarray(H, dtype=np.float64)
+        for c in multi_indices(4):
+            idx = [i for i in range(3) for _ in range(c[i])]
+            M[c] = H[idx[0], idx[1], idx[2], idx[3]]
     return M
 
 
 def third_moment(s, q):
     """O_ijk = sum_j q_j s_ji s_jj s_jk of point charges at offsets s."""
     return np.einsum("j,ji,jk,jl->ikl", q, s, s, s)
+
+
+def fourth_moment(s, q):
+    """H_ijkl = sum_j q_j s_ji s_jj s_jk s_jl of point charges at offsets s."""
+    return np.einsum("j,ji,jk,jl,jm->iklm", q, s, s, s, s)
+
+
+def traceless4(H):
+    """The traceless part of a symmetric rank-4 tensor: H - (d_ij T_kl + 5
+    more pairings) / 7 + (d_ij d_kl + d_ik d_jl + d_il d_jk) tr T / 35, with
+    T_kl = H_iikl."""
+    T = np.einsum("iikl->kl", H)
+    t0 = np.trace(T)
+    d = np.eye(3)
+    pair = (np.einsum("ij,kl->ijkl", d, T) + np.einsum("ik,jl->ijkl", d, T) + np.einsum("il,jk->ijkl", d, T) +
+            np.einsum("jk,il->ijkl", d, T) + np.einsum("jl,ik->ijkl", d, T) + np.einsum("kl,ij->ijkl", d, T))
+    dd = np.einsum("ij,kl->ijkl", d, d) + np.einsum("ik,jl->ijkl", d, d) + np.einsum("il,jk->ijkl", d, d)
+    return H - pair / 7.0 + dd * t0 / 35.0
 
 
 def traceless3(O):
@@ -108,13 +134,13 @@ def traceless3(O):
     return O - (np.einsum("ij,k->ijk", d, t) + np.einsum("ik,j->ijk", d, t) + np.einsum("jk,i->ijk", d, t)) / 5.0
 
 
-def m2l(X, q, D, Qp, g, P, O=None, tmax=None):
+def m2l(X, q, D, Qp, g, P, O=None, tmax=None, H=None):
     """Local coefficients L_b (|b| <= P) about g of a cell at X with moments
-    q, D, Q' (and the third moment O); with tmax, only the terms with
-    |b| + |c| <= tmax."""
+    q, D, Q' (and the third moment O, the fourth moment H); with tmax, only
+    the terms with |b| + |c| <= tmax."""
     R = np.asarray(g, dtype=np.float64) - np.asarray(X, dtype=np.float64)
-    T = taylor_coeffs(R, P + (3 if O is not None else 2))
-    M = cell_moments(q, D, Qp, O)
+    T = taylor_coeffs(R, P + (4 if H is not None else 3 if O is not None else 2))
+    M = cell_moments(q, D, Qp, O, H)
     L = {}
     for b in all_indices(P):
         s = 0.0
@@ -143,10 +169,11 @@ def l2p(L, y):
     return phi, E
 
 
-def m2p(X, q, D, Qp, x, Op=None):
+def m2p(X, q, D, Qp, x, Op=None, Hp=None):
     """The walk's cell expansion evaluated directly at x (FP64):
-    phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5) [+ 5/2 O'rrr / r^7], E = -grad phi,
-    r = x - X, O' the traceless third moment (the walk's M2P keeps it)."""
+    phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5) [+ 5/2 O'rrr / r^7] [+ 35/8 H'rrrr / r^9],
+    E = -grad phi, r = x - X, O' and H' the traceless third and fourth
+    moments (the walk's M2P keeps both)."""
     r = np.asarray(x, dtype=np.float64) - np.asarray(X, dtype=np.float64)
     Qp = np.asarray(Qp, dtype=np.float64)
     r2 = float(r @ r)
@@ -162,6 +189,11 @@ def m2p(X, q, D, Qp, x, Op=None):
         s3 = float(r @ v)
         phi += 2.5 * s3 * ri7
         E += -7.5 * v * ri7 + 17.5 * s3 * r * ri9
+    if Hp is not None:
+        w = np.einsum("ijkl,j,k,l->i", np.asarray(Hp, dtype=np.float64), r, r, r)
+        s4 = float(r @ w)
+        phi += 4.375 * s4 * ri9
+        E += -17.5 * w * ri9 + 39.375 * s4 * r * ri9 / r2
     return phi, E
 
 

@@ -46,4 +46,4 @@ def test_ensemble_matches_single_runs():
         ens.force()
         g, ms = ens.autotune()
         assert g in (64, 128, 256) and np.all(ms > 0)
-        assert ens.walk_lists() in ((15, 24), (11, 16), (7, 12))
+        assert ens.walk_lists() in ((11, 24), (9, 16), (7, 12))

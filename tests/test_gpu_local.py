@@ -24,9 +24,14 @@ def _cells(rng, n, dmin, dmax):
     # a traceless symmetric third moment from random point-charge offsets
     O = np.stack([lr.traceless3(lr.third_moment(rng.normal(size=(5, 3)) * 0.3, rng.normal(size=5)))
                   for _ in range(n)])
+    # and a traceless symmetric fourth moment
+    H = np.stack([lr.traceless4(lr.fourth_moment(rng.normal(size=(5, 3)) * 0.3, rng.normal(size=5)))
+                  for _ in range(n)])
     cells = np.c_[R, q, D, Qp[:, 0, 0], Qp[:, 1, 1], Qp[:, 0, 1], Qp[:, 1, 2], Qp[:, 2, 0],
-                  O[:, 0, 0, 0], O[:, 0, 0, 1], O[:, 0, 0, 2], O[:, 0, 1, 1], O[:, 0, 1, 2], O[:, 1, 1, 1], O[:, 1, 1, 2]]
-    return cells, R, q, D, Qp, O
+                  O[:, 0, 0, 0], O[:, 0, 0, 1], O[:, 0, 0, 2], O[:, 0, 1, 1], O[:, 0, 1, 2], O[:, 1, 1, 1], O[:, 1, 1, 2],
+                  H[:, 0, 0, 0, 0], H[:, 0, 0, 0, 1], H[:, 0, 0, 0, 2], H[:, 0, 0, 1, 1], H[:, 0, 0, 1, 2],
+                  H[:, 0, 1, 1, 1], H[:, 0, 1, 1, 2], H[:, 1, 1, 1, 1], H[:, 1, 1, 1, 2]]
+    return cells, R, q, D, Qp, O, H
 
 
 @pytest.mark.parametrize("order", [3, 4, 5, 6])
@@ -35,17 +40,17 @@ def test_local_expansion_vs_oracle(order):
     wavefront) and evaluated at 70 offsets: within 2e-5 of the FP64 oracle,
     relative to the field's scale.  Orders 4 to 6 (the harmonic forms; 6 is
     the walk's, with the terms to total order 7) include the cells' third
-    moments."""
+    moments, and order 6 their fourth moments."""
     announce("local_eval order %d" % order)
     rng = np.random.default_rng(order)
-    cells, R, q, D, Qp, O = _cells(rng, 150, 5.0, 20.0)
+    cells, R, q, D, Qp, O, H = _cells(rng, 150, 5.0, 20.0)
     y = rng.uniform(-1.0, 1.0, size=(70, 3))
     with ta.Engine(rs=RS10) as eng:
         out = eng.local_eval(order, cells, y)
     L = {}
     for k in range(len(cells)):
         Lk = lr.m2l(-R[k], q[k], D[k], Qp[k], np.zeros(3), order, O=O[k] if order >= 4 else None,
-                    tmax=7 if order == 6 else None)
+                    tmax=7 if order == 6 else None, H=H[k] if order == 6 else None)
         for b, v in Lk.items():
             L[b] = L.get(b, 0.0) + v
     ref = np.array([[*lr.l2p(L, yy)[:1], *lr.l2p(L, yy)[1]] for yy in y])

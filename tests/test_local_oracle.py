@@ -242,3 +242,112 @@ def test_total_order_truncation():
         worst_t6 = max(worst_t6, np.linalg.norm(lr.l2p(t6, y)[1] - E_full))
     assert worst_cut < worst_trunc, (worst_cut, worst_trunc)
     assert worst_cut < worst_t6  # each further order dropped costs more
+
+
+def test_traceless4_is_traceless_and_idempotent():
+    # every pair trace of the projection vanishes, the projection of a
+    # traceless tensor is itself, and what it removes is a pure trace part
+    rng = np.random.default_rng(21)
+    s = rng.normal(size=(30, 3))
+    qs = rng.choice([-1.0, 1.0], size=30)
+    H = lr.fourth_moment(s, qs)
+    Hp = lr.traceless4(H)
+    scale = np.abs(H).max()
+    for sub in ("iikl->kl", "ikil->kl", "ikli->kl", "kiil->kl", "kili->kl", "klii->kl"):
+        assert np.abs(np.einsum(sub, Hp)).max() <= 1e-13 * scale
+    assert np.allclose(lr.traceless4(Hp), Hp, rtol=0, atol=1e-13 * scale)
+    # symmetric under every index swap
+    for perm in ((1, 0, 2, 3), (0, 2, 1, 3), (0, 1, 3, 2), (3, 1, 2, 0)):
+        assert np.allclose(Hp.transpose(perm), Hp, rtol=0, atol=1e-14 * scale)
+    # r.r.H.r.r of a harmonic quartic: the Laplacian of H'rrrr vanishes at any point
+    x0 = rng.normal(size=3)
+    h = 1e-3
+    f = lambda x: np.einsum("ijkl,i,j,k,l->", Hp, x, x, x, x)
+    lap = sum((f(x0 + h * e) - 2 * f(x0) + f(x0 - h * e)) / h ** 2 for e in np.eye(3))
+    assert abs(lap) <= 1e-6 * scale * (1 + x0 @ x0)
+
+
+def test_hexadecapole_local_expansion_vs_point_charges():
+    # with the fourth moment the cell's truncation falls to (b/d)^5
+    rng = np.random.default_rng(13)
+    X, xs, qs, q, D, Qp = _cell(rng, n=60, b=0.25)
+    O = lr.third_moment(xs - X, qs)
+    H = lr.fourth_moment(xs - X, qs)
+    for d_scale in (1.0, 2.0):
+        g = X + d_scale * np.array([-4.0, 2.5, 1.5])
+        d = np.linalg.norm(g - X)
+        L3 = lr.m2l(X, q, D, Qp, g, 5, O=O)
+        L4 = lr.m2l(X, q, D, Qp, g, 5, O=O, H=H)
+        e3 = e4 = 0.0
+        for _ in range(10):
+            u = rng.normal(size=3)
+            x = g + 0.02 * d * u / np.linalg.norm(u)
+            dr = x - xs
+            r = np.linalg.norm(dr, axis=1)
+            E_dir = (qs[:, None] * dr / r[:, None] ** 3).sum(0)
+            e3 = max(e3, np.linalg.norm(lr.l2p(L3, x - g)[1] - E_dir))
+            e4 = max(e4, np.linalg.norm(lr.l2p(L4, x - g)[1] - E_dir))
+        scale = np.abs(qs).sum() / d ** 2
+        b = 0.25 * np.sqrt(3)
+        assert e4 < 0.5 * e3                      # the fourth moment helps
+        assert e4 < 20 * scale * (b / d) ** 5     # and leaves an order-5 remainder
+    # the traces of H drop out: its traceless part gives the same coefficients
+    L4t = lr.m2l(X, q, D, Qp, g, 5, O=O, H=lr.traceless4(H))
+    for k in L4:
+        assert abs(L4t[k] - L4[k]) <= 1e-12 * max(1.0, abs(L4[k]))
+
+
+def test_hexadecapole_m2p_vs_point_charges_and_gradient():
+    # the walk's M2P with the traceless third and fourth moments: E = -grad
+    # phi (finite differences), and against point charges the error falls as
+    # (b/d)^5 instead of (b/d)^4
+    rng = np.random.default_rng(14)
+    X, xs, qs, q, D, Qp = _cell(rng, n=80, b=0.25)
+    Op = lr.traceless3(lr.third_moment(xs - X, qs))
+    Hp = lr.traceless4(lr.fourth_moment(xs - X, qs))
+    x0 = X + np.array([1.3, -0.9, 1.1])
+    h = 1e-5
+    g = np.zeros(3)
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = h
+        g[i] = (lr.m2p(X, q, D, Qp, x0 + e, Op, Hp)[0] - lr.m2p(X, q, D, Qp, x0 - e, Op, Hp)[0]) / (2 * h)
+    E = lr.m2p(X, q, D, Qp, x0, Op, Hp)[1]
+    assert np.allclose(E, -g, rtol=1e-6, atol=1e-9 * np.abs(E).max())
+    errs = []
+    for d in (2.0, 4.0):
+        u = np.array([0.6, -0.48, 0.64])
+        x = X + d * u
+        dr = x - xs
+        r = np.linalg.norm(dr, axis=1)
+        E_dir = (qs[:, None] * dr / r[:, None] ** 3).sum(0)
+        e3 = np.linalg.norm(lr.m2p(X, q, D, Qp, x, Op)[1] - E_dir)
+        e4 = np.linalg.norm(lr.m2p(X, q, D, Qp, x, Op, Hp)[1] - E_dir)
+        assert e4 < e3
+        errs.append(e4)
+    # order: doubling d divides the error by about 2^7 (field ~ d^-2 * (b/d)^5)
+    assert errs[0] / errs[1] > 0.5 * 2 ** 7
+
+
+def test_hexadecapole_m2l_matches_m2p_near_the_centre():
+    # the local expansion of a cell with all four moments tends to the
+    # cell's own M2P field (same moments) as the offset shrinks
+    rng = np.random.default_rng(15)
+    X, xs, qs, q, D, Qp = _cell(rng)
+    Op = lr.traceless3(lr.third_moment(xs - X, qs))
+    Hp = lr.traceless4(lr.fourth_moment(xs - X, qs))
+    g = X + np.array([5.0, -4.0, 3.0])
+    d = np.linalg.norm(g - X)
+    L = lr.m2l(X, q, D, Qp, g, 6, O=Op, H=Hp)
+    errs = []
+    for rho in (0.08, 0.04):
+        ee = 0.0
+        for _ in range(20):
+            u = rng.normal(size=3)
+            yv = rho * d * u / np.linalg.norm(u)
+            E = lr.l2p(L, yv)[1]
+            E0 = lr.m2p(X, q, D, Qp, g + yv, Op, Hp)[1]
+            ee = max(ee, np.linalg.norm(E - E0) / np.linalg.norm(E0))
+        errs.append(ee)
+    assert errs[0] < 60 * 0.08 ** 6
+    assert errs[0] / errs[1] > 0.4 * 2 ** 6

@@ -789,13 +789,24 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
     }
 }
 
+// index tuples of the raw fourth moment's components, and the raw third
+// moment's component of a (sorted or not) index triple
+__constant__ const int kHi[15][4] = {{0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 1, 1}, {0, 0, 1, 2},
+                                     {0, 0, 2, 2}, {0, 1, 1, 1}, {0, 1, 1, 2}, {0, 1, 2, 2}, {0, 2, 2, 2},
+                                     {1, 1, 1, 1}, {1, 1, 1, 2}, {1, 1, 2, 2}, {1, 2, 2, 2}, {2, 2, 2, 2}};
+__constant__ const int kO3[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}},
+                                       {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                                       {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+
 // The mixed walk's breadth-first records.  A leaf: its particle and charge.
 // A cell: centre and s / theta, first child (breadth-first) << 3 | children
 // - 1, q, D, the traceless quadrupole 3 M - tr M, the traceless third moment
-// O' = O - (d_ij t_k + d_ik t_j + d_jk t_i) / 5 with t_k = O_iik, and the
-// learned criterion's per-cell features (FP64, then rounded; see
+// O' = O - (d_ij t_k + d_ik t_j + d_jk t_i) / 5 with t_k = O_iik, the
+// traceless fourth moment H' = H - (d_ij T_kl + 5 more pairings) / 7 +
+// (d_ij d_kl + d_ik d_jl + d_il d_jk) T0 / 35 with T_kl = H_mmkl, T0 = T_mm,
+// and the learned criterion's per-cell features (FP64, then rounded; see
 // band_features in walk32.hip).
-// (A leaf's other records, bq1..bq4 and bf, are never read.)
+// (A leaf's other records, bq1..bq6 and bf, are never read.)
 __device__ inline void leaf_record(const CellRecs &R, int b, double4 c, int32_t start)
 {
     R.binfo[b] = TB_LEAF | (uint32_t)start;
@@ -804,7 +815,7 @@ __device__ inline void leaf_record(const CellRecs &R, int b, double4 c, int32_t 
 }
 
 __device__ inline void cell_record(const CellRecs &R, int b, double cx, double cy, double cz, double sz,
-                                   const double (&Q)[TB_JQ], int32_t count)
+                                   const double *Q, int32_t count)
 {
 #pragma clang fp contract(fast)
     R.bmac[b] = make_double4(cx, cy, cz, sz / R.theta);
@@ -817,7 +828,11 @@ __device__ inline void cell_record(const CellRecs &R, int b, double cx, double c
     const double tx = O[0] + O[3] + O[5], ty = O[1] + O[6] + O[8], tz = O[2] + O[7] + O[9];
     R.bq3[b] = make_float4((float)(O[0] - 0.6 * tx), (float)(O[1] - 0.2 * ty), (float)(O[2] - 0.2 * tz),
                            (float)(O[3] - 0.2 * tx));
-    R.bq4[b] = make_float4((float)O[4], (float)(O[6] - 0.6 * ty), (float)(O[7] - 0.2 * tz), 0.0f);
+    // bq4.w (H'xxxx) is the fourth-moment record's (cell_record4)
+    float *q4 = (float *)&R.bq4[b];
+    q4[0] = (float)O[4];
+    q4[1] = (float)(O[6] - 0.6 * ty);
+    q4[2] = (float)(O[7] - 0.2 * tz);
     const double kf = 0.125, cn = (double)count, tiny = 9.5367431640625e-07;
     const double qxx = 3.0 * Q[4] - tr, qyy = 3.0 * Q[5] - tr, qzz = 3.0 * Q[6] - tr;
     const double qn = sqrt(qxx * qxx + qyy * qyy + qzz * qzz +
@@ -826,6 +841,21 @@ __device__ inline void cell_record(const CellRecs &R, int b, double cx, double c
     R.bq2[b] = make_float4((float)(3.0 * Q[9]), (float)sz, (float)cn, (float)(kf * log2(sz)));
     R.bf[b] = make_float4((float)(kf * log2(fabs(Q[0]) / cn + tiny)), (float)(kf * log2(dn / (cn * sz) + tiny)),
                           (float)(kf * log2(qn / (cn * sz * sz) + tiny)), (float)(kf * log2(cn)));
+}
+
+// the traceless fourth moment's 9 independent components from the raw
+// H (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz yyyy yyyz yyzz yzzz zzzz)
+__device__ inline void cell_record4(const CellRecs &R, int b, const double (&H)[15])
+{
+#pragma clang fp contract(fast)
+    const double txx = H[0] + H[3] + H[5], tyy = H[3] + H[10] + H[12], tzz = H[5] + H[12] + H[14];
+    const double txy = H[1] + H[6] + H[8], tyz = H[4] + H[11] + H[13], tzx = H[2] + H[7] + H[9];
+    const double t0 = txx + tyy + tzz, s7 = 1.0 / 7.0, s35 = 1.0 / 35.0;
+    ((float *)&R.bq4[b])[3] = (float)(H[0] - 6.0 * s7 * txx + 3.0 * s35 * t0);
+    R.bq5[b] = make_float4((float)(H[1] - 3.0 * s7 * txy), (float)(H[2] - 3.0 * s7 * tzx),
+                           (float)(H[3] - s7 * (txx + tyy) + s35 * t0), (float)(H[4] - s7 * tyz));
+    R.bq6[b] = make_float4((float)(H[6] - 3.0 * s7 * txy), (float)(H[7] - s7 * tzx),
+                           (float)(H[10] - 6.0 * s7 * tyy + 3.0 * s35 * t0), (float)(H[11] - 3.0 * s7 * tyz));
 }
 
 // Breadth-first companions of the joint tree, one thread per node in
@@ -882,20 +912,28 @@ __global__ __launch_bounds__(BT) void k_jkids(int nn, const uint32_t *__restrict
     R.binfo[b] = ((uint32_t)j0 << 3) | (uint32_t)(nch - 1);
 }
 
-// net charge q, dipole D, second moment M and third moment O of the cells of
-// one level (deepest first) about their geometric centres, shifted up from
-// the children's own expansion centres (a child's offsets s become s + a,
-// a = child centre - cell centre); the children are a contiguous run of the
-// next level, read from their breadth-first records.
+// net charge q, dipole D, second moment M and third and fourth moments O, H
+// of the cells of one level (deepest first) about their geometric centres,
+// shifted up from the children's own expansion centres (a child's offsets s
+// become s + a, a = child centre - cell centre); the children are a
+// contiguous run of the next level, read from their breadth-first records.
+// A block takes JUP_CELLS cells: its first half of wavefronts forms q, D, M,
+// O and the records built from them, the second half (the same cells) the
+// fourth moment and its records.  Split by wavefront, the two parts run side
+// by side without divergence, and each thread's serial chain over the
+// children is half as long (the top levels are latency-bound).
 //
 // A batched ensemble's keys start with T octal digits naming the realisation
 // (its tag): the cells of realisation r start at depth T from its own box
 // root[6 r ..]; the T levels above hold no cell of any realisation and are
 // never walked (their moments and records are zero).  One domain: T = 0.
+constexpr int JUP_CELLS = BT / 2;
+
 __global__ __launch_bounds__(BT) void k_jupward(int beg, int cnt, int cbeg, int cend, int d, int T,
                                                 const double *__restrict__ root, CellRecs R)
 {
-    const int t = blockIdx.x * BT + threadIdx.x;
+    const int part = threadIdx.x / JUP_CELLS;  // wave-uniform: 0 q..O, 1 H
+    const int t = blockIdx.x * JUP_CELLS + (int)(threadIdx.x % JUP_CELLS);
     if (t >= cnt) return;
     const int b = beg + t;
     if (R.bflag[b] & 2) return;  // a leaf: its record is written
@@ -907,9 +945,17 @@ __global__ __launch_bounds__(BT) void k_jupward(int beg, int cnt, int cbeg, int 
     }
     double *o = R.bnq + (size_t)b * TB_JQ;
     if (d < T) {
-        for (int c = 0; c < TB_JQ; c++) o[c] = 0.0;
-        R.bmac[b] = make_double4(0.0, 0.0, 0.0, 0.0);
-        R.bq0[b] = R.bq1[b] = R.bq2[b] = R.bq3[b] = R.bq4[b] = R.bf[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (part == 0) {
+            for (int c = 0; c < 20; c++) o[c] = 0.0;
+            R.bmac[b] = make_double4(0.0, 0.0, 0.0, 0.0);
+            R.bq0[b] = R.bq1[b] = R.bq2[b] = R.bq3[b] = R.bf[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            float *q4 = (float *)&R.bq4[b];
+            q4[0] = q4[1] = q4[2] = 0.f;
+        } else {
+            for (int c = 20; c < TB_JQ; c++) o[c] = 0.0;
+            ((float *)&R.bq4[b])[3] = 0.f;
+            R.bq5[b] = R.bq6[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         return;
     }
     const uint64_t k = R.bkey[b];
@@ -924,6 +970,44 @@ __global__ __launch_bounds__(BT) void k_jupward(int beg, int cnt, int cbeg, int 
         lx = lx * 0.5;
         ly = ly * 0.5;
         lz = lz * 0.5;
+    }
+    if (part == 1) {
+        // H_ijkl += H_c + (O_jkl a_i + 3 more) + (M_ij a_k a_l + 5 more) +
+        //           (D_i a_j a_k a_l + 3 more) + q a_i a_j a_k a_l, with the
+        //           child's moments before this child's own shift (a particle:
+        //           q a_i a_j a_k a_l alone, its other terms exact zeros)
+        double H[15];
+#pragma unroll
+        for (int t2 = 0; t2 < 15; t2++) H[t2] = 0.0;
+        for (int j = j0; j < j0 + nch; j++) {
+            const double4 cc = R.bmac[j];
+            const double av[3] = {cc.x - ox, cc.y - oy, cc.z - oz};
+            if (R.bflag[j] & 2) {
+                const double qc = (double)R.bq0[j].x;
+#pragma unroll
+                for (int t2 = 0; t2 < 15; t2++)
+                    H[t2] += qc * av[kHi[t2][0]] * av[kHi[t2][1]] * av[kHi[t2][2]] * av[kHi[t2][3]];
+                continue;
+            }
+            const double *cq = R.bnq + (size_t)j * TB_JQ;
+            const double qc = cq[0], dv[3] = {cq[1], cq[2], cq[3]};
+            const double mm[3][3] = {{cq[4], cq[7], cq[9]}, {cq[7], cq[5], cq[8]}, {cq[9], cq[8], cq[6]}};
+            const double *oc = cq + 10;
+#pragma unroll
+            for (int t2 = 0; t2 < 15; t2++) {
+                const int i = kHi[t2][0], jj = kHi[t2][1], kk = kHi[t2][2], ll = kHi[t2][3];
+                H[t2] += cq[20 + t2] + oc[kO3[jj][kk][ll]] * av[i] + oc[kO3[i][kk][ll]] * av[jj] +
+                         oc[kO3[i][jj][ll]] * av[kk] + oc[kO3[i][jj][kk]] * av[ll] +
+                         mm[i][jj] * av[kk] * av[ll] + mm[i][kk] * av[jj] * av[ll] + mm[i][ll] * av[jj] * av[kk] +
+                         mm[jj][kk] * av[i] * av[ll] + mm[jj][ll] * av[i] * av[kk] + mm[kk][ll] * av[i] * av[jj] +
+                         dv[i] * av[jj] * av[kk] * av[ll] + dv[jj] * av[i] * av[kk] * av[ll] +
+                         dv[kk] * av[i] * av[jj] * av[ll] + dv[ll] * av[i] * av[jj] * av[kk] +
+                         qc * av[i] * av[jj] * av[kk] * av[ll];
+            }
+        }
+        for (int c = 0; c < 15; c++) o[20 + c] = H[c];
+        cell_record4(R, b, H);
+        return;
     }
     double q = 0.0, D[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double O[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -980,12 +1064,12 @@ __global__ __launch_bounds__(BT) void k_jupward(int beg, int cnt, int cbeg, int 
     double s = lx;
     if (ly > s) s = ly;
     if (lz > s) s = lz;
-    double Q[TB_JQ];
+    double Q[20];
     Q[0] = q;
     for (int c = 0; c < 3; c++) Q[1 + c] = D[c];
     for (int c = 0; c < 6; c++) Q[4 + c] = M[c];
     for (int c = 0; c < 10; c++) Q[10 + c] = O[c];
-    for (int c = 0; c < TB_JQ; c++) o[c] = Q[c];
+    for (int c = 0; c < 20; c++) o[c] = Q[c];
     cell_record(R, b, ox, oy, oz, s, Q, R.bcnt[b]);
 }
 
@@ -1427,7 +1511,7 @@ void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, const CellRecs 
         int beg = J.lvl_off[d], cnt = J.lvl_off[d + 1] - beg;
         if (cnt <= 0 || J.twig_hist[d] == 0) continue;
         const int cend = d + 2 <= TB_LEVELS + 2 ? J.lvl_off[d + 2] : J.nn;
-        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, BT)), dim3(BT), 0, s, beg, cnt, J.lvl_off[d + 1], cend, d, T,
+        hipLaunchKernelGGL(k_jupward, dim3(nblk(cnt, JUP_CELLS)), dim3(BT), 0, s, beg, cnt, J.lvl_off[d + 1], cend, d, T,
                            T > 0 ? roots : J.root, R);
     }
 }

@@ -2,14 +2,15 @@
 straight-line FP32 device code (run by hand after a change; the header is
 committed).
 
-For a cell with centre X and moments q, D, traceless Q' and an expansion
-centre g, with R = g - X:
+For a cell with centre X and moments q, D, traceless Q' (and the traceless
+third and fourth moments O', H') and an expansion centre g, with R = g - X:
 
-  T_a   Taylor coefficients of 1/|R + y| (a multi-index, |a| <= P + 2), by
+  T_a   Taylor coefficients of 1/|R + y| (a multi-index, |a| <= P + |c|max), by
         n r^2 T_a = -(2n - 1) sum_i R_i T_{a-e_i} - (n - 1) sum_i T_{a-2e_i};
-  L_b  += sum_c C(b + c, b) m_c T_{b+c}   (|b| <= P, |c| <= 2 or 3), with
+  L_b  += sum_c C(b + c, b) m_c T_{b+c}   (|b| <= P, |c| <= 2, 3 or 4), with
         m_0 = q, m_{e_i} = -D_i, m_c = Q'_c / 3 for |c| = 2 and
-        m_c = -O'_c for |c| = 3 (O' the traceless third moment);
+        m_c = -O'_c for |c| = 3 (O' the traceless third moment) and
+        m_c = H'_c for |c| = 4 (H' the traceless fourth moment);
   phi(y) = sum_b L_b y^b,  E = -grad phi.
 
 The same mathematics in FP64, with loops instead of unrolled code, is the
@@ -46,14 +47,16 @@ def harmonic_indices(P):
     return [b for b in all_indices(P) if b[2] <= 1]
 
 
-def emit_m2l(P, harmonic=False, octupole=False, tmax=None):
+def emit_m2l(P, harmonic=False, octupole=False, tmax=None, hexadecapole=False):
     """Keeps the terms m_c T_{b+c} with |b| <= P and total order |b| + |c| <=
     tmax (Taylor coefficients to order tmax); tmax None: every moment with
-    every |b| <= P (to order P + 3)."""
+    every |b| <= P (to order P + 3).  hexadecapole adds the traceless fourth
+    moment (m_c = H'_c for |c| = 4)."""
     L = harmonic_indices(P) if harmonic else all_indices(P)
     lidx = {b: i for i, b in enumerate(L)}
-    sfx = ("h" if harmonic else "") + ("o" if octupole else "") + ("t%d" % tmax if tmax else "")
-    cmax = 3 if octupole else 2
+    sfx = ("h" if harmonic else "") + ("x" if hexadecapole else "o" if octupole else "") + \
+        ("t%d" % tmax if tmax else "")
+    cmax = 4 if hexadecapole else 3 if octupole else 2
     tmax = tmax or P + cmax
     # moments m_c: name and expression
     mom = {(0, 0, 0): "q", (1, 0, 0): "mdx", (0, 1, 0): "mdy", (0, 0, 1): "mdz",
@@ -66,8 +69,15 @@ def emit_m2l(P, harmonic=False, octupole=False, tmax=None):
                "float dz," % (P, sfx))
     if octupole:
         out.append("                                  float qxx, float qyy, float qxy, float qyz, float qzx,")
-        out.append("                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, "
-                   "float oyyy, float oyyz, float (&L)[%d])" % len(L))
+        if hexadecapole:
+            out.append("                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, "
+                       "float oyyy, float oyyz,")
+            out.append("                                  float hxxxx, float hxxxy, float hxxxz, float hxxyy, "
+                       "float hxxyz, float hxyyy, float hxyyz,")
+            out.append("                                  float hyyyy, float hyyyz, float (&L)[%d])" % len(L))
+        else:
+            out.append("                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, "
+                       "float oyyy, float oyyz, float (&L)[%d])" % len(L))
     else:
         out.append("                                  float qxx, float qyy, float qxy, float qyz, float qzx, "
                    "float (&L)[%d])" % len(L))
@@ -83,6 +93,15 @@ def emit_m2l(P, harmonic=False, octupole=False, tmax=None):
         out.append("    const float m030 = -oyyy, m021 = -oyyz, m102 = oxxx + oxyy, m012 = oxxy + oyyy, "
                    "m003 = oxxz + oyyz;")
         for c in multi_indices(3):
+            mom[c] = "m%d%d%d" % c
+    if hexadecapole:
+        # m_c = H'_c; the traceless fourth moment's components with two or more z from the others
+        out.append("    const float m400 = hxxxx, m310 = hxxxy, m301 = hxxxz, m220 = hxxyy, m211 = hxxyz;")
+        out.append("    const float m130 = hxyyy, m121 = hxyyz, m040 = hyyyy, m031 = hyyyz;")
+        out.append("    const float m202 = -(hxxxx + hxxyy), m112 = -(hxxxy + hxyyy), m022 = -(hxxyy + hyyyy);")
+        out.append("    const float m103 = -(hxxxz + hxyyz), m013 = -(hxxyz + hyyyz), "
+                   "m004 = hxxxx + 2.0f * hxxyy + hyyyy;")
+        for c in multi_indices(4):
             mom[c] = "m%d%d%d" % c
     out.append("    const float t000 = __builtin_amdgcn_rsqf(r2);")
     out.append("    const float ir2 = t000 * t000;")
@@ -194,7 +213,8 @@ def main():
              "// Far-field local expansion of the mixed walk: Taylor coefficients of",
              "// 1/|R + y| by their three-term recurrence, contracted with a cell's",
              "// net charge, dipole, traceless quadrupole and (orders 4 to 6, the",
-             "// harmonic forms) traceless third moment into the local coefficients",
+             "// harmonic forms) traceless third moment (order 6, the walk's: and",
+             "// traceless fourth moment) into the local coefficients",
              "// L_b of the potential about a target group's centre (|b| <= P,",
              "// multi-indices in the order of all_indices; the harmonic forms keep",
              "// b_z <= 1; the order-6 form keeps the terms of total order |b| + |c|",
@@ -217,8 +237,8 @@ def main():
         lines.append("")
         lines += emit_l2p(P, harmonic=True)
         lines.append("")
-    # the walk's: order 6, terms to total order 7
-    lines += emit_m2l(6, harmonic=True, octupole=True, tmax=7)
+    # the walk's: order 6, terms to total order 7, cells to the fourth moment
+    lines += emit_m2l(6, harmonic=True, octupole=True, tmax=7, hexadecapole=True)
     lines.append("")
     lines += emit_l2p(6, harmonic=True)
     lines.append("")

@@ -4,7 +4,7 @@
 // then evaluated at given offsets.  Order 3 keeps every coefficient and the
 // moments up to the quadrupole; orders 4 to 6 (6 is the walk's) keep the
 // independent coefficients of the harmonic expansion and add the third
-// moment.
+// moment, and order 6 the fourth.
 #include <hip/hip_runtime.h>
 
 #include "local_exp.h"
@@ -21,10 +21,11 @@ __device__ inline void wave_sum_all(float (&L)[N])
         for (int o = 32; o > 0; o >>= 1) L[k] += __shfl_xor(L[k], o, 64);
 }
 
-// cells: 19 floats each, R = centre - cell centre (3), q, D (3), Q'xx,
-// Q'yy, Q'xy, Q'yz, Q'zx, then the traceless third moment O'xxx, O'xxy,
-// O'xxz, O'xyy, O'xyz, O'yyy, O'yyz (orders 4 to 6).  out: phi, Ex, Ey, Ez
-// per offset.
+// cells: TB_LOCAL_CELL (28) floats each, R = centre - cell centre (3), q,
+// D (3), Q'xx, Q'yy, Q'xy, Q'yz, Q'zx, then the traceless third moment O'xxx,
+// O'xxy, O'xxz, O'xyy, O'xyz, O'yyy, O'yyz (orders 4 to 6) and the traceless
+// fourth moment H'xxxx, H'xxxy, H'xxxz, H'xxyy, H'xxyz, H'xyyy, H'xyyz,
+// H'yyyy, H'yyyz (order 6).  out: phi, Ex, Ey, Ez per offset.
 template <int P>
 __global__ __launch_bounds__(64) void k_local_eval(int ncell, const float *__restrict__ cells, int ny,
                                                    const float *__restrict__ y, float *__restrict__ out)
@@ -44,8 +45,9 @@ __global__ __launch_bounds__(64) void k_local_eval(int ncell, const float *__res
             m2l_acc_p5ho(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
                          e[14], e[15], e[16], e[17], e[18], L);
         else
-            m2l_acc_p6hot7(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
-                         e[14], e[15], e[16], e[17], e[18], L);
+            m2l_acc_p6hxt7(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8], e[9], e[10], e[11], e[12], e[13],
+                           e[14], e[15], e[16], e[17], e[18], e[19], e[20], e[21], e[22], e[23], e[24], e[25], e[26],
+                           e[27], L);
     }
     wave_sum_all(L);
     for (int j = threadIdx.x; j < ny; j += 64) {

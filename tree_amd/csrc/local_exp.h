@@ -3,7 +3,8 @@
 // Far-field local expansion of the mixed walk: Taylor coefficients of
 // 1/|R + y| by their three-term recurrence, contracted with a cell's
 // net charge, dipole, traceless quadrupole and (orders 4 to 6, the
-// harmonic forms) traceless third moment into the local coefficients
+// harmonic forms) traceless third moment (order 6, the walk's: and
+// traceless fourth moment) into the local coefficients
 // L_b of the potential about a target group's centre (|b| <= P,
 // multi-indices in the order of all_indices; the harmonic forms keep
 // b_z <= 1; the order-6 form keeps the terms of total order |b| + |c|
@@ -2265,9 +2266,11 @@ __device__ inline void l2p_p5h(const float (&Lh)[36], float x, float y, float z,
 
 // adds one cell's local coefficients about the expansion centre to L[49]
 // (order P = 6; R = centre - cell centre; Q' traceless, Q'zz = -Q'xx - Q'yy)
-__device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, float dx, float dy, float dz,
+__device__ inline void m2l_acc_p6hxt7(float rx, float ry, float rz, float q, float dx, float dy, float dz,
                                   float qxx, float qyy, float qxy, float qyz, float qzx,
-                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, float oyyy, float oyyz, float (&L)[49])
+                                  float oxxx, float oxxy, float oxxz, float oxyy, float oxyz, float oyyy, float oyyz,
+                                  float hxxxx, float hxxxy, float hxxxz, float hxxyy, float hxxyz, float hxyyy, float hxyyz,
+                                  float hyyyy, float hyyyz, float (&L)[49])
 {
     const float r2 = rx * rx + ry * ry + rz * rz;
     const float mdx = -dx, mdy = -dy, mdz = -dz;
@@ -2275,6 +2278,10 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     const float mxy = qxy * (1.0f / 3.0f), myz = qyz * (1.0f / 3.0f), mzx = qzx * (1.0f / 3.0f);
     const float m300 = -oxxx, m210 = -oxxy, m201 = -oxxz, m120 = -oxyy, m111 = -oxyz;
     const float m030 = -oyyy, m021 = -oyyz, m102 = oxxx + oxyy, m012 = oxxy + oyyy, m003 = oxxz + oyyz;
+    const float m400 = hxxxx, m310 = hxxxy, m301 = hxxxz, m220 = hxxyy, m211 = hxxyz;
+    const float m130 = hxyyy, m121 = hxyyz, m040 = hyyyy, m031 = hyyyz;
+    const float m202 = -(hxxxx + hxxyy), m112 = -(hxxxy + hxyyy), m022 = -(hxxyy + hyyyy);
+    const float m103 = -(hxxxz + hxyyz), m013 = -(hxxyz + hyyyz), m004 = hxxxx + 2.0f * hxxyy + hyyyy;
     const float t000 = __builtin_amdgcn_rsqf(r2);
     const float ir2 = t000 * t000;
     L[0] = fmaf(q, t000, L[0]);
@@ -2382,6 +2389,7 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[9] = fmaf(4.0f * mdx, t400, L[9]);
     L[4] = fmaf(6.0f * mxx, t400, L[4]);
     L[1] = fmaf(4.0f * m300, t400, L[1]);
+    L[0] = fmaf(m400, t400, L[0]);
     const float t310 = -fmaf(c1_4, rx * t210 + ry * t300, c2_4 * (t110));
     L[17] = fmaf(q, t310, L[17]);
     L[10] = fmaf(3.0f * mdx, t310, L[10]);
@@ -2390,6 +2398,7 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[4] = fmaf(3.0f * mxy, t310, L[4]);
     L[2] = fmaf(m300, t310, L[2]);
     L[1] = fmaf(3.0f * m210, t310, L[1]);
+    L[0] = fmaf(m310, t310, L[0]);
     const float t301 = -fmaf(c1_4, rx * t201 + rz * t300, c2_4 * (t101));
     L[18] = fmaf(q, t301, L[18]);
     L[11] = fmaf(3.0f * mdx, t301, L[11]);
@@ -2398,6 +2407,7 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[4] = fmaf(3.0f * mzx, t301, L[4]);
     L[3] = fmaf(m300, t301, L[3]);
     L[1] = fmaf(3.0f * m201, t301, L[1]);
+    L[0] = fmaf(m301, t301, L[0]);
     const float t220 = -fmaf(c1_4, rx * t120 + ry * t210, c2_4 * (t020 + t200));
     L[19] = fmaf(q, t220, L[19]);
     L[12] = fmaf(2.0f * mdx, t220, L[12]);
@@ -2407,6 +2417,7 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[5] = fmaf(4.0f * mxy, t220, L[5]);
     L[2] = fmaf(2.0f * m210, t220, L[2]);
     L[1] = fmaf(2.0f * m120, t220, L[1]);
+    L[0] = fmaf(m220, t220, L[0]);
     const float t211 = -fmaf(c1_4, rx * t111 + ry * t201 + rz * t210, c2_4 * (t011));
     L[20] = fmaf(q, t211, L[20]);
     L[13] = fmaf(2.0f * mdx, t211, L[13]);
@@ -2419,12 +2430,14 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[3] = fmaf(m210, t211, L[3]);
     L[2] = fmaf(m201, t211, L[2]);
     L[1] = fmaf(2.0f * m111, t211, L[1]);
+    L[0] = fmaf(m211, t211, L[0]);
     const float t202 = -fmaf(c1_4, rx * t102 + rz * t201, c2_4 * (t002 + t200));
     L[11] = fmaf(2.0f * mdz, t202, L[11]);
     L[4] = fmaf(mzz, t202, L[4]);
     L[6] = fmaf(4.0f * mzx, t202, L[6]);
     L[3] = fmaf(2.0f * m201, t202, L[3]);
     L[1] = fmaf(2.0f * m102, t202, L[1]);
+    L[0] = fmaf(m202, t202, L[0]);
     const float t130 = -fmaf(c1_4, rx * t030 + ry * t120, c2_4 * (t110));
     L[21] = fmaf(q, t130, L[21]);
     L[14] = fmaf(mdx, t130, L[14]);
@@ -2433,6 +2446,7 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[7] = fmaf(3.0f * mxy, t130, L[7]);
     L[2] = fmaf(3.0f * m120, t130, L[2]);
     L[1] = fmaf(m030, t130, L[1]);
+    L[0] = fmaf(m130, t130, L[0]);
     const float t121 = -fmaf(c1_4, rx * t021 + ry * t111 + rz * t120, c2_4 * (t101));
     L[22] = fmaf(q, t121, L[22]);
     L[15] = fmaf(mdx, t121, L[15]);
@@ -2445,6 +2459,7 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[3] = fmaf(m120, t121, L[3]);
     L[2] = fmaf(2.0f * m111, t121, L[2]);
     L[1] = fmaf(m021, t121, L[1]);
+    L[0] = fmaf(m121, t121, L[0]);
     const float t112 = -fmaf(c1_4, rx * t012 + ry * t102 + rz * t111, c2_4 * (t110));
     L[13] = fmaf(2.0f * mdz, t112, L[13]);
     L[5] = fmaf(mzz, t112, L[5]);
@@ -2453,15 +2468,18 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[3] = fmaf(2.0f * m111, t112, L[3]);
     L[2] = fmaf(m102, t112, L[2]);
     L[1] = fmaf(m012, t112, L[1]);
+    L[0] = fmaf(m112, t112, L[0]);
     const float t103 = -fmaf(c1_4, rx * t003 + rz * t102, c2_4 * (t101));
     L[6] = fmaf(3.0f * mzz, t103, L[6]);
     L[3] = fmaf(3.0f * m102, t103, L[3]);
     L[1] = fmaf(m003, t103, L[1]);
+    L[0] = fmaf(m103, t103, L[0]);
     const float t040 = -fmaf(c1_4, ry * t030, c2_4 * (t020));
     L[23] = fmaf(q, t040, L[23]);
     L[14] = fmaf(4.0f * mdy, t040, L[14]);
     L[7] = fmaf(6.0f * myy, t040, L[7]);
     L[2] = fmaf(4.0f * m030, t040, L[2]);
+    L[0] = fmaf(m040, t040, L[0]);
     const float t031 = -fmaf(c1_4, ry * t021 + rz * t030, c2_4 * (t011));
     L[24] = fmaf(q, t031, L[24]);
     L[15] = fmaf(3.0f * mdy, t031, L[15]);
@@ -2470,24 +2488,29 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[7] = fmaf(3.0f * myz, t031, L[7]);
     L[3] = fmaf(m030, t031, L[3]);
     L[2] = fmaf(3.0f * m021, t031, L[2]);
+    L[0] = fmaf(m031, t031, L[0]);
     const float t022 = -fmaf(c1_4, ry * t012 + rz * t021, c2_4 * (t002 + t020));
     L[15] = fmaf(2.0f * mdz, t022, L[15]);
     L[7] = fmaf(mzz, t022, L[7]);
     L[8] = fmaf(4.0f * myz, t022, L[8]);
     L[3] = fmaf(2.0f * m021, t022, L[3]);
     L[2] = fmaf(2.0f * m012, t022, L[2]);
+    L[0] = fmaf(m022, t022, L[0]);
     const float t013 = -fmaf(c1_4, ry * t003 + rz * t012, c2_4 * (t011));
     L[8] = fmaf(3.0f * mzz, t013, L[8]);
     L[3] = fmaf(3.0f * m012, t013, L[3]);
     L[2] = fmaf(m003, t013, L[2]);
+    L[0] = fmaf(m013, t013, L[0]);
     const float t004 = -fmaf(c1_4, rz * t003, c2_4 * (t002));
     L[3] = fmaf(4.0f * m003, t004, L[3]);
+    L[0] = fmaf(m004, t004, L[0]);
     const float c1_5 = 1.8f * ir2, c2_5 = 0.8f * ir2;
     const float t500 = -fmaf(c1_5, rx * t400, c2_5 * (t300));
     L[25] = fmaf(q, t500, L[25]);
     L[16] = fmaf(5.0f * mdx, t500, L[16]);
     L[9] = fmaf(10.0f * mxx, t500, L[9]);
     L[4] = fmaf(10.0f * m300, t500, L[4]);
+    L[1] = fmaf(5.0f * m400, t500, L[1]);
     const float t410 = -fmaf(c1_5, rx * t310 + ry * t400, c2_5 * (t210));
     L[26] = fmaf(q, t410, L[26]);
     L[17] = fmaf(4.0f * mdx, t410, L[17]);
@@ -2496,6 +2519,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[9] = fmaf(4.0f * mxy, t410, L[9]);
     L[5] = fmaf(4.0f * m300, t410, L[5]);
     L[4] = fmaf(6.0f * m210, t410, L[4]);
+    L[2] = fmaf(m400, t410, L[2]);
+    L[1] = fmaf(4.0f * m310, t410, L[1]);
     const float t401 = -fmaf(c1_5, rx * t301 + rz * t400, c2_5 * (t201));
     L[27] = fmaf(q, t401, L[27]);
     L[18] = fmaf(4.0f * mdx, t401, L[18]);
@@ -2504,6 +2529,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[9] = fmaf(4.0f * mzx, t401, L[9]);
     L[6] = fmaf(4.0f * m300, t401, L[6]);
     L[4] = fmaf(6.0f * m201, t401, L[4]);
+    L[3] = fmaf(m400, t401, L[3]);
+    L[1] = fmaf(4.0f * m301, t401, L[1]);
     const float t320 = -fmaf(c1_5, rx * t220 + ry * t310, c2_5 * (t120 + t300));
     L[28] = fmaf(q, t320, L[28]);
     L[19] = fmaf(3.0f * mdx, t320, L[19]);
@@ -2514,6 +2541,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[7] = fmaf(m300, t320, L[7]);
     L[5] = fmaf(6.0f * m210, t320, L[5]);
     L[4] = fmaf(3.0f * m120, t320, L[4]);
+    L[2] = fmaf(2.0f * m310, t320, L[2]);
+    L[1] = fmaf(3.0f * m220, t320, L[1]);
     const float t311 = -fmaf(c1_5, rx * t211 + ry * t301 + rz * t310, c2_5 * (t111));
     L[29] = fmaf(q, t311, L[29]);
     L[20] = fmaf(3.0f * mdx, t311, L[20]);
@@ -2527,12 +2556,17 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[6] = fmaf(3.0f * m210, t311, L[6]);
     L[5] = fmaf(3.0f * m201, t311, L[5]);
     L[4] = fmaf(3.0f * m111, t311, L[4]);
+    L[3] = fmaf(m310, t311, L[3]);
+    L[2] = fmaf(m301, t311, L[2]);
+    L[1] = fmaf(3.0f * m211, t311, L[1]);
     const float t302 = -fmaf(c1_5, rx * t202 + rz * t301, c2_5 * (t102 + t300));
     L[18] = fmaf(2.0f * mdz, t302, L[18]);
     L[9] = fmaf(mzz, t302, L[9]);
     L[11] = fmaf(6.0f * mzx, t302, L[11]);
     L[6] = fmaf(6.0f * m201, t302, L[6]);
     L[4] = fmaf(3.0f * m102, t302, L[4]);
+    L[3] = fmaf(2.0f * m301, t302, L[3]);
+    L[1] = fmaf(3.0f * m202, t302, L[1]);
     const float t230 = -fmaf(c1_5, rx * t130 + ry * t220, c2_5 * (t030 + t210));
     L[30] = fmaf(q, t230, L[30]);
     L[21] = fmaf(2.0f * mdx, t230, L[21]);
@@ -2543,6 +2577,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[7] = fmaf(3.0f * m210, t230, L[7]);
     L[5] = fmaf(6.0f * m120, t230, L[5]);
     L[4] = fmaf(m030, t230, L[4]);
+    L[2] = fmaf(3.0f * m220, t230, L[2]);
+    L[1] = fmaf(2.0f * m130, t230, L[1]);
     const float t221 = -fmaf(c1_5, rx * t121 + ry * t211 + rz * t220, c2_5 * (t021 + t201));
     L[31] = fmaf(q, t221, L[31]);
     L[22] = fmaf(2.0f * mdx, t221, L[22]);
@@ -2558,6 +2594,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[6] = fmaf(2.0f * m120, t221, L[6]);
     L[5] = fmaf(4.0f * m111, t221, L[5]);
     L[4] = fmaf(m021, t221, L[4]);
+    L[3] = fmaf(m220, t221, L[3]);
+    L[2] = fmaf(2.0f * m211, t221, L[2]);
+    L[1] = fmaf(2.0f * m121, t221, L[1]);
     const float t212 = -fmaf(c1_5, rx * t112 + ry * t202 + rz * t211, c2_5 * (t012 + t210));
     L[20] = fmaf(2.0f * mdz, t212, L[20]);
     L[10] = fmaf(mzz, t212, L[10]);
@@ -2567,10 +2606,15 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[6] = fmaf(4.0f * m111, t212, L[6]);
     L[5] = fmaf(2.0f * m102, t212, L[5]);
     L[4] = fmaf(m012, t212, L[4]);
+    L[3] = fmaf(2.0f * m211, t212, L[3]);
+    L[2] = fmaf(m202, t212, L[2]);
+    L[1] = fmaf(2.0f * m112, t212, L[1]);
     const float t203 = -fmaf(c1_5, rx * t103 + rz * t202, c2_5 * (t003 + t201));
     L[11] = fmaf(3.0f * mzz, t203, L[11]);
     L[6] = fmaf(6.0f * m102, t203, L[6]);
     L[4] = fmaf(m003, t203, L[4]);
+    L[3] = fmaf(3.0f * m202, t203, L[3]);
+    L[1] = fmaf(2.0f * m103, t203, L[1]);
     const float t140 = -fmaf(c1_5, rx * t040 + ry * t130, c2_5 * (t120));
     L[32] = fmaf(q, t140, L[32]);
     L[23] = fmaf(mdx, t140, L[23]);
@@ -2579,6 +2623,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[14] = fmaf(4.0f * mxy, t140, L[14]);
     L[7] = fmaf(6.0f * m120, t140, L[7]);
     L[5] = fmaf(4.0f * m030, t140, L[5]);
+    L[2] = fmaf(4.0f * m130, t140, L[2]);
+    L[1] = fmaf(m040, t140, L[1]);
     const float t131 = -fmaf(c1_5, rx * t031 + ry * t121 + rz * t130, c2_5 * (t111));
     L[33] = fmaf(q, t131, L[33]);
     L[24] = fmaf(mdx, t131, L[24]);
@@ -2592,6 +2638,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[7] = fmaf(3.0f * m111, t131, L[7]);
     L[6] = fmaf(m030, t131, L[6]);
     L[5] = fmaf(3.0f * m021, t131, L[5]);
+    L[3] = fmaf(m130, t131, L[3]);
+    L[2] = fmaf(3.0f * m121, t131, L[2]);
+    L[1] = fmaf(m031, t131, L[1]);
     const float t122 = -fmaf(c1_5, rx * t022 + ry * t112 + rz * t121, c2_5 * (t102 + t120));
     L[22] = fmaf(2.0f * mdz, t122, L[22]);
     L[12] = fmaf(mzz, t122, L[12]);
@@ -2601,18 +2650,27 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[7] = fmaf(m102, t122, L[7]);
     L[6] = fmaf(2.0f * m021, t122, L[6]);
     L[5] = fmaf(2.0f * m012, t122, L[5]);
+    L[3] = fmaf(2.0f * m121, t122, L[3]);
+    L[2] = fmaf(2.0f * m112, t122, L[2]);
+    L[1] = fmaf(m022, t122, L[1]);
     const float t113 = -fmaf(c1_5, rx * t013 + ry * t103 + rz * t112, c2_5 * (t111));
     L[13] = fmaf(3.0f * mzz, t113, L[13]);
     L[8] = fmaf(3.0f * m102, t113, L[8]);
     L[6] = fmaf(3.0f * m012, t113, L[6]);
     L[5] = fmaf(m003, t113, L[5]);
+    L[3] = fmaf(3.0f * m112, t113, L[3]);
+    L[2] = fmaf(m103, t113, L[2]);
+    L[1] = fmaf(m013, t113, L[1]);
     const float t104 = -fmaf(c1_5, rx * t004 + rz * t103, c2_5 * (t102));
     L[6] = fmaf(4.0f * m003, t104, L[6]);
+    L[3] = fmaf(4.0f * m103, t104, L[3]);
+    L[1] = fmaf(m004, t104, L[1]);
     const float t050 = -fmaf(c1_5, ry * t040, c2_5 * (t030));
     L[34] = fmaf(q, t050, L[34]);
     L[23] = fmaf(5.0f * mdy, t050, L[23]);
     L[14] = fmaf(10.0f * myy, t050, L[14]);
     L[7] = fmaf(10.0f * m030, t050, L[7]);
+    L[2] = fmaf(5.0f * m040, t050, L[2]);
     const float t041 = -fmaf(c1_5, ry * t031 + rz * t040, c2_5 * (t021));
     L[35] = fmaf(q, t041, L[35]);
     L[24] = fmaf(4.0f * mdy, t041, L[24]);
@@ -2621,24 +2679,35 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[14] = fmaf(4.0f * myz, t041, L[14]);
     L[8] = fmaf(4.0f * m030, t041, L[8]);
     L[7] = fmaf(6.0f * m021, t041, L[7]);
+    L[3] = fmaf(m040, t041, L[3]);
+    L[2] = fmaf(4.0f * m031, t041, L[2]);
     const float t032 = -fmaf(c1_5, ry * t022 + rz * t031, c2_5 * (t012 + t030));
     L[24] = fmaf(2.0f * mdz, t032, L[24]);
     L[14] = fmaf(mzz, t032, L[14]);
     L[15] = fmaf(6.0f * myz, t032, L[15]);
     L[8] = fmaf(6.0f * m021, t032, L[8]);
     L[7] = fmaf(3.0f * m012, t032, L[7]);
+    L[3] = fmaf(2.0f * m031, t032, L[3]);
+    L[2] = fmaf(3.0f * m022, t032, L[2]);
     const float t023 = -fmaf(c1_5, ry * t013 + rz * t022, c2_5 * (t003 + t021));
     L[15] = fmaf(3.0f * mzz, t023, L[15]);
     L[8] = fmaf(6.0f * m012, t023, L[8]);
     L[7] = fmaf(m003, t023, L[7]);
+    L[3] = fmaf(3.0f * m022, t023, L[3]);
+    L[2] = fmaf(2.0f * m013, t023, L[2]);
     const float t014 = -fmaf(c1_5, ry * t004 + rz * t013, c2_5 * (t012));
     L[8] = fmaf(4.0f * m003, t014, L[8]);
+    L[3] = fmaf(4.0f * m013, t014, L[3]);
+    L[2] = fmaf(m004, t014, L[2]);
+    const float t005 = -fmaf(c1_5, rz * t004, c2_5 * (t003));
+    L[3] = fmaf(5.0f * m004, t005, L[3]);
     const float c1_6 = 1.8333333333333333f * ir2, c2_6 = 0.8333333333333334f * ir2;
     const float t600 = -fmaf(c1_6, rx * t500, c2_6 * (t400));
     L[36] = fmaf(q, t600, L[36]);
     L[25] = fmaf(6.0f * mdx, t600, L[25]);
     L[16] = fmaf(15.0f * mxx, t600, L[16]);
     L[9] = fmaf(20.0f * m300, t600, L[9]);
+    L[4] = fmaf(15.0f * m400, t600, L[4]);
     const float t510 = -fmaf(c1_6, rx * t410 + ry * t500, c2_6 * (t310));
     L[37] = fmaf(q, t510, L[37]);
     L[26] = fmaf(5.0f * mdx, t510, L[26]);
@@ -2647,6 +2716,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[16] = fmaf(5.0f * mxy, t510, L[16]);
     L[10] = fmaf(10.0f * m300, t510, L[10]);
     L[9] = fmaf(10.0f * m210, t510, L[9]);
+    L[5] = fmaf(5.0f * m400, t510, L[5]);
+    L[4] = fmaf(10.0f * m310, t510, L[4]);
     const float t501 = -fmaf(c1_6, rx * t401 + rz * t500, c2_6 * (t301));
     L[38] = fmaf(q, t501, L[38]);
     L[27] = fmaf(5.0f * mdx, t501, L[27]);
@@ -2655,6 +2726,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[16] = fmaf(5.0f * mzx, t501, L[16]);
     L[11] = fmaf(10.0f * m300, t501, L[11]);
     L[9] = fmaf(10.0f * m201, t501, L[9]);
+    L[6] = fmaf(5.0f * m400, t501, L[6]);
+    L[4] = fmaf(10.0f * m301, t501, L[4]);
     const float t420 = -fmaf(c1_6, rx * t320 + ry * t410, c2_6 * (t220 + t400));
     L[39] = fmaf(q, t420, L[39]);
     L[28] = fmaf(4.0f * mdx, t420, L[28]);
@@ -2665,6 +2738,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[12] = fmaf(4.0f * m300, t420, L[12]);
     L[10] = fmaf(12.0f * m210, t420, L[10]);
     L[9] = fmaf(4.0f * m120, t420, L[9]);
+    L[7] = fmaf(m400, t420, L[7]);
+    L[5] = fmaf(8.0f * m310, t420, L[5]);
+    L[4] = fmaf(6.0f * m220, t420, L[4]);
     const float t411 = -fmaf(c1_6, rx * t311 + ry * t401 + rz * t410, c2_6 * (t211));
     L[40] = fmaf(q, t411, L[40]);
     L[29] = fmaf(4.0f * mdx, t411, L[29]);
@@ -2678,12 +2754,18 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[11] = fmaf(6.0f * m210, t411, L[11]);
     L[10] = fmaf(6.0f * m201, t411, L[10]);
     L[9] = fmaf(4.0f * m111, t411, L[9]);
+    L[8] = fmaf(m400, t411, L[8]);
+    L[6] = fmaf(4.0f * m310, t411, L[6]);
+    L[5] = fmaf(4.0f * m301, t411, L[5]);
+    L[4] = fmaf(6.0f * m211, t411, L[4]);
     const float t402 = -fmaf(c1_6, rx * t302 + rz * t401, c2_6 * (t202 + t400));
     L[27] = fmaf(2.0f * mdz, t402, L[27]);
     L[16] = fmaf(mzz, t402, L[16]);
     L[18] = fmaf(8.0f * mzx, t402, L[18]);
     L[11] = fmaf(12.0f * m201, t402, L[11]);
     L[9] = fmaf(4.0f * m102, t402, L[9]);
+    L[6] = fmaf(8.0f * m301, t402, L[6]);
+    L[4] = fmaf(6.0f * m202, t402, L[4]);
     const float t330 = -fmaf(c1_6, rx * t230 + ry * t320, c2_6 * (t130 + t310));
     L[41] = fmaf(q, t330, L[41]);
     L[30] = fmaf(3.0f * mdx, t330, L[30]);
@@ -2695,6 +2777,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[12] = fmaf(9.0f * m210, t330, L[12]);
     L[10] = fmaf(9.0f * m120, t330, L[10]);
     L[9] = fmaf(m030, t330, L[9]);
+    L[7] = fmaf(3.0f * m310, t330, L[7]);
+    L[5] = fmaf(9.0f * m220, t330, L[5]);
+    L[4] = fmaf(3.0f * m130, t330, L[4]);
     const float t321 = -fmaf(c1_6, rx * t221 + ry * t311 + rz * t320, c2_6 * (t121 + t301));
     L[42] = fmaf(q, t321, L[42]);
     L[31] = fmaf(3.0f * mdx, t321, L[31]);
@@ -2711,6 +2796,11 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[11] = fmaf(3.0f * m120, t321, L[11]);
     L[10] = fmaf(6.0f * m111, t321, L[10]);
     L[9] = fmaf(m021, t321, L[9]);
+    L[8] = fmaf(2.0f * m310, t321, L[8]);
+    L[7] = fmaf(m301, t321, L[7]);
+    L[6] = fmaf(3.0f * m220, t321, L[6]);
+    L[5] = fmaf(6.0f * m211, t321, L[5]);
+    L[4] = fmaf(3.0f * m121, t321, L[4]);
     const float t312 = -fmaf(c1_6, rx * t212 + ry * t302 + rz * t311, c2_6 * (t112 + t310));
     L[29] = fmaf(2.0f * mdz, t312, L[29]);
     L[17] = fmaf(mzz, t312, L[17]);
@@ -2720,10 +2810,16 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[11] = fmaf(6.0f * m111, t312, L[11]);
     L[10] = fmaf(3.0f * m102, t312, L[10]);
     L[9] = fmaf(m012, t312, L[9]);
+    L[8] = fmaf(2.0f * m301, t312, L[8]);
+    L[6] = fmaf(6.0f * m211, t312, L[6]);
+    L[5] = fmaf(3.0f * m202, t312, L[5]);
+    L[4] = fmaf(3.0f * m112, t312, L[4]);
     const float t303 = -fmaf(c1_6, rx * t203 + rz * t302, c2_6 * (t103 + t301));
     L[18] = fmaf(3.0f * mzz, t303, L[18]);
     L[11] = fmaf(9.0f * m102, t303, L[11]);
     L[9] = fmaf(m003, t303, L[9]);
+    L[6] = fmaf(9.0f * m202, t303, L[6]);
+    L[4] = fmaf(3.0f * m103, t303, L[4]);
     const float t240 = -fmaf(c1_6, rx * t140 + ry * t230, c2_6 * (t040 + t220));
     L[43] = fmaf(q, t240, L[43]);
     L[32] = fmaf(2.0f * mdx, t240, L[32]);
@@ -2734,6 +2830,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[14] = fmaf(4.0f * m210, t240, L[14]);
     L[12] = fmaf(12.0f * m120, t240, L[12]);
     L[10] = fmaf(4.0f * m030, t240, L[10]);
+    L[7] = fmaf(6.0f * m220, t240, L[7]);
+    L[5] = fmaf(8.0f * m130, t240, L[5]);
+    L[4] = fmaf(m040, t240, L[4]);
     const float t231 = -fmaf(c1_6, rx * t131 + ry * t221 + rz * t230, c2_6 * (t031 + t211));
     L[44] = fmaf(q, t231, L[44]);
     L[33] = fmaf(2.0f * mdx, t231, L[33]);
@@ -2750,6 +2849,11 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[12] = fmaf(6.0f * m111, t231, L[12]);
     L[11] = fmaf(m030, t231, L[11]);
     L[10] = fmaf(3.0f * m021, t231, L[10]);
+    L[8] = fmaf(3.0f * m220, t231, L[8]);
+    L[7] = fmaf(3.0f * m211, t231, L[7]);
+    L[6] = fmaf(2.0f * m130, t231, L[6]);
+    L[5] = fmaf(6.0f * m121, t231, L[5]);
+    L[4] = fmaf(m031, t231, L[4]);
     const float t222 = -fmaf(c1_6, rx * t122 + ry * t212 + rz * t221, c2_6 * (t022 + t202 + t220));
     L[31] = fmaf(2.0f * mdz, t222, L[31]);
     L[19] = fmaf(mzz, t222, L[19]);
@@ -2760,13 +2864,24 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[12] = fmaf(2.0f * m102, t222, L[12]);
     L[11] = fmaf(2.0f * m021, t222, L[11]);
     L[10] = fmaf(2.0f * m012, t222, L[10]);
+    L[8] = fmaf(4.0f * m211, t222, L[8]);
+    L[7] = fmaf(m202, t222, L[7]);
+    L[6] = fmaf(4.0f * m121, t222, L[6]);
+    L[5] = fmaf(4.0f * m112, t222, L[5]);
+    L[4] = fmaf(m022, t222, L[4]);
     const float t213 = -fmaf(c1_6, rx * t113 + ry * t203 + rz * t212, c2_6 * (t013 + t211));
     L[20] = fmaf(3.0f * mzz, t213, L[20]);
     L[13] = fmaf(6.0f * m102, t213, L[13]);
     L[11] = fmaf(3.0f * m012, t213, L[11]);
     L[10] = fmaf(m003, t213, L[10]);
+    L[8] = fmaf(3.0f * m202, t213, L[8]);
+    L[6] = fmaf(6.0f * m112, t213, L[6]);
+    L[5] = fmaf(2.0f * m103, t213, L[5]);
+    L[4] = fmaf(m013, t213, L[4]);
     const float t204 = -fmaf(c1_6, rx * t104 + rz * t203, c2_6 * (t004 + t202));
     L[11] = fmaf(4.0f * m003, t204, L[11]);
+    L[6] = fmaf(8.0f * m103, t204, L[6]);
+    L[4] = fmaf(m004, t204, L[4]);
     const float t150 = -fmaf(c1_6, rx * t050 + ry * t140, c2_6 * (t130));
     L[45] = fmaf(q, t150, L[45]);
     L[34] = fmaf(mdx, t150, L[34]);
@@ -2775,6 +2890,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[23] = fmaf(5.0f * mxy, t150, L[23]);
     L[14] = fmaf(10.0f * m120, t150, L[14]);
     L[12] = fmaf(10.0f * m030, t150, L[12]);
+    L[7] = fmaf(10.0f * m130, t150, L[7]);
+    L[5] = fmaf(5.0f * m040, t150, L[5]);
     const float t141 = -fmaf(c1_6, rx * t041 + ry * t131 + rz * t140, c2_6 * (t121));
     L[46] = fmaf(q, t141, L[46]);
     L[35] = fmaf(mdx, t141, L[35]);
@@ -2788,6 +2905,10 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[14] = fmaf(4.0f * m111, t141, L[14]);
     L[13] = fmaf(4.0f * m030, t141, L[13]);
     L[12] = fmaf(6.0f * m021, t141, L[12]);
+    L[8] = fmaf(4.0f * m130, t141, L[8]);
+    L[7] = fmaf(6.0f * m121, t141, L[7]);
+    L[6] = fmaf(m040, t141, L[6]);
+    L[5] = fmaf(4.0f * m031, t141, L[5]);
     const float t132 = -fmaf(c1_6, rx * t032 + ry * t122 + rz * t131, c2_6 * (t112 + t130));
     L[33] = fmaf(2.0f * mdz, t132, L[33]);
     L[21] = fmaf(mzz, t132, L[21]);
@@ -2797,18 +2918,32 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[14] = fmaf(m102, t132, L[14]);
     L[13] = fmaf(6.0f * m021, t132, L[13]);
     L[12] = fmaf(3.0f * m012, t132, L[12]);
+    L[8] = fmaf(6.0f * m121, t132, L[8]);
+    L[7] = fmaf(3.0f * m112, t132, L[7]);
+    L[6] = fmaf(2.0f * m031, t132, L[6]);
+    L[5] = fmaf(3.0f * m022, t132, L[5]);
     const float t123 = -fmaf(c1_6, rx * t023 + ry * t113 + rz * t122, c2_6 * (t103 + t121));
     L[22] = fmaf(3.0f * mzz, t123, L[22]);
     L[15] = fmaf(3.0f * m102, t123, L[15]);
     L[13] = fmaf(6.0f * m012, t123, L[13]);
     L[12] = fmaf(m003, t123, L[12]);
+    L[8] = fmaf(6.0f * m112, t123, L[8]);
+    L[7] = fmaf(m103, t123, L[7]);
+    L[6] = fmaf(3.0f * m022, t123, L[6]);
+    L[5] = fmaf(2.0f * m013, t123, L[5]);
     const float t114 = -fmaf(c1_6, rx * t014 + ry * t104 + rz * t113, c2_6 * (t112));
     L[13] = fmaf(4.0f * m003, t114, L[13]);
+    L[8] = fmaf(4.0f * m103, t114, L[8]);
+    L[6] = fmaf(4.0f * m013, t114, L[6]);
+    L[5] = fmaf(m004, t114, L[5]);
+    const float t105 = -fmaf(c1_6, rx * t005 + rz * t104, c2_6 * (t103));
+    L[6] = fmaf(5.0f * m004, t105, L[6]);
     const float t060 = -fmaf(c1_6, ry * t050, c2_6 * (t040));
     L[47] = fmaf(q, t060, L[47]);
     L[34] = fmaf(6.0f * mdy, t060, L[34]);
     L[23] = fmaf(15.0f * myy, t060, L[23]);
     L[14] = fmaf(20.0f * m030, t060, L[14]);
+    L[7] = fmaf(15.0f * m040, t060, L[7]);
     const float t051 = -fmaf(c1_6, ry * t041 + rz * t050, c2_6 * (t031));
     L[48] = fmaf(q, t051, L[48]);
     L[35] = fmaf(5.0f * mdy, t051, L[35]);
@@ -2817,23 +2952,34 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[23] = fmaf(5.0f * myz, t051, L[23]);
     L[15] = fmaf(10.0f * m030, t051, L[15]);
     L[14] = fmaf(10.0f * m021, t051, L[14]);
+    L[8] = fmaf(5.0f * m040, t051, L[8]);
+    L[7] = fmaf(10.0f * m031, t051, L[7]);
     const float t042 = -fmaf(c1_6, ry * t032 + rz * t041, c2_6 * (t022 + t040));
     L[35] = fmaf(2.0f * mdz, t042, L[35]);
     L[23] = fmaf(mzz, t042, L[23]);
     L[24] = fmaf(8.0f * myz, t042, L[24]);
     L[15] = fmaf(12.0f * m021, t042, L[15]);
     L[14] = fmaf(4.0f * m012, t042, L[14]);
+    L[8] = fmaf(8.0f * m031, t042, L[8]);
+    L[7] = fmaf(6.0f * m022, t042, L[7]);
     const float t033 = -fmaf(c1_6, ry * t023 + rz * t032, c2_6 * (t013 + t031));
     L[24] = fmaf(3.0f * mzz, t033, L[24]);
     L[15] = fmaf(9.0f * m012, t033, L[15]);
     L[14] = fmaf(m003, t033, L[14]);
+    L[8] = fmaf(9.0f * m022, t033, L[8]);
+    L[7] = fmaf(3.0f * m013, t033, L[7]);
     const float t024 = -fmaf(c1_6, ry * t014 + rz * t023, c2_6 * (t004 + t022));
     L[15] = fmaf(4.0f * m003, t024, L[15]);
+    L[8] = fmaf(8.0f * m013, t024, L[8]);
+    L[7] = fmaf(m004, t024, L[7]);
+    const float t015 = -fmaf(c1_6, ry * t005 + rz * t014, c2_6 * (t013));
+    L[8] = fmaf(5.0f * m004, t015, L[8]);
     const float c1_7 = 1.8571428571428572f * ir2, c2_7 = 0.8571428571428571f * ir2;
     const float t700 = -fmaf(c1_7, rx * t600, c2_7 * (t500));
     L[36] = fmaf(7.0f * mdx, t700, L[36]);
     L[25] = fmaf(21.0f * mxx, t700, L[25]);
     L[16] = fmaf(35.0f * m300, t700, L[16]);
+    L[9] = fmaf(35.0f * m400, t700, L[9]);
     const float t610 = -fmaf(c1_7, rx * t510 + ry * t600, c2_7 * (t410));
     L[37] = fmaf(6.0f * mdx, t610, L[37]);
     L[36] = fmaf(mdy, t610, L[36]);
@@ -2841,6 +2987,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[25] = fmaf(6.0f * mxy, t610, L[25]);
     L[17] = fmaf(20.0f * m300, t610, L[17]);
     L[16] = fmaf(15.0f * m210, t610, L[16]);
+    L[10] = fmaf(15.0f * m400, t610, L[10]);
+    L[9] = fmaf(20.0f * m310, t610, L[9]);
     const float t601 = -fmaf(c1_7, rx * t501 + rz * t600, c2_7 * (t401));
     L[38] = fmaf(6.0f * mdx, t601, L[38]);
     L[36] = fmaf(mdz, t601, L[36]);
@@ -2848,6 +2996,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[25] = fmaf(6.0f * mzx, t601, L[25]);
     L[18] = fmaf(20.0f * m300, t601, L[18]);
     L[16] = fmaf(15.0f * m201, t601, L[16]);
+    L[11] = fmaf(15.0f * m400, t601, L[11]);
+    L[9] = fmaf(20.0f * m301, t601, L[9]);
     const float t520 = -fmaf(c1_7, rx * t420 + ry * t510, c2_7 * (t320 + t500));
     L[39] = fmaf(5.0f * mdx, t520, L[39]);
     L[37] = fmaf(2.0f * mdy, t520, L[37]);
@@ -2857,6 +3007,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[19] = fmaf(10.0f * m300, t520, L[19]);
     L[17] = fmaf(20.0f * m210, t520, L[17]);
     L[16] = fmaf(5.0f * m120, t520, L[16]);
+    L[12] = fmaf(5.0f * m400, t520, L[12]);
+    L[10] = fmaf(20.0f * m310, t520, L[10]);
+    L[9] = fmaf(10.0f * m220, t520, L[9]);
     const float t511 = -fmaf(c1_7, rx * t411 + ry * t501 + rz * t510, c2_7 * (t311));
     L[40] = fmaf(5.0f * mdx, t511, L[40]);
     L[38] = fmaf(mdy, t511, L[38]);
@@ -2869,12 +3022,18 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[18] = fmaf(10.0f * m210, t511, L[18]);
     L[17] = fmaf(10.0f * m201, t511, L[17]);
     L[16] = fmaf(5.0f * m111, t511, L[16]);
+    L[13] = fmaf(5.0f * m400, t511, L[13]);
+    L[11] = fmaf(10.0f * m310, t511, L[11]);
+    L[10] = fmaf(10.0f * m301, t511, L[10]);
+    L[9] = fmaf(10.0f * m211, t511, L[9]);
     const float t502 = -fmaf(c1_7, rx * t402 + rz * t501, c2_7 * (t302 + t500));
     L[38] = fmaf(2.0f * mdz, t502, L[38]);
     L[25] = fmaf(mzz, t502, L[25]);
     L[27] = fmaf(10.0f * mzx, t502, L[27]);
     L[18] = fmaf(20.0f * m201, t502, L[18]);
     L[16] = fmaf(5.0f * m102, t502, L[16]);
+    L[11] = fmaf(20.0f * m301, t502, L[11]);
+    L[9] = fmaf(10.0f * m202, t502, L[9]);
     const float t430 = -fmaf(c1_7, rx * t330 + ry * t420, c2_7 * (t230 + t410));
     L[41] = fmaf(4.0f * mdx, t430, L[41]);
     L[39] = fmaf(3.0f * mdy, t430, L[39]);
@@ -2885,6 +3044,10 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[19] = fmaf(18.0f * m210, t430, L[19]);
     L[17] = fmaf(12.0f * m120, t430, L[17]);
     L[16] = fmaf(m030, t430, L[16]);
+    L[14] = fmaf(m400, t430, L[14]);
+    L[12] = fmaf(12.0f * m310, t430, L[12]);
+    L[10] = fmaf(18.0f * m220, t430, L[10]);
+    L[9] = fmaf(4.0f * m130, t430, L[9]);
     const float t421 = -fmaf(c1_7, rx * t321 + ry * t411 + rz * t420, c2_7 * (t221 + t401));
     L[42] = fmaf(4.0f * mdx, t421, L[42]);
     L[40] = fmaf(2.0f * mdy, t421, L[40]);
@@ -2900,6 +3063,12 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[18] = fmaf(4.0f * m120, t421, L[18]);
     L[17] = fmaf(8.0f * m111, t421, L[17]);
     L[16] = fmaf(m021, t421, L[16]);
+    L[15] = fmaf(m400, t421, L[15]);
+    L[13] = fmaf(8.0f * m310, t421, L[13]);
+    L[12] = fmaf(4.0f * m301, t421, L[12]);
+    L[11] = fmaf(6.0f * m220, t421, L[11]);
+    L[10] = fmaf(12.0f * m211, t421, L[10]);
+    L[9] = fmaf(4.0f * m121, t421, L[9]);
     const float t412 = -fmaf(c1_7, rx * t312 + ry * t402 + rz * t411, c2_7 * (t212 + t410));
     L[40] = fmaf(2.0f * mdz, t412, L[40]);
     L[26] = fmaf(mzz, t412, L[26]);
@@ -2909,10 +3078,16 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[18] = fmaf(8.0f * m111, t412, L[18]);
     L[17] = fmaf(4.0f * m102, t412, L[17]);
     L[16] = fmaf(m012, t412, L[16]);
+    L[13] = fmaf(8.0f * m301, t412, L[13]);
+    L[11] = fmaf(12.0f * m211, t412, L[11]);
+    L[10] = fmaf(6.0f * m202, t412, L[10]);
+    L[9] = fmaf(4.0f * m112, t412, L[9]);
     const float t403 = -fmaf(c1_7, rx * t303 + rz * t402, c2_7 * (t203 + t401));
     L[27] = fmaf(3.0f * mzz, t403, L[27]);
     L[18] = fmaf(12.0f * m102, t403, L[18]);
     L[16] = fmaf(m003, t403, L[16]);
+    L[11] = fmaf(18.0f * m202, t403, L[11]);
+    L[9] = fmaf(4.0f * m103, t403, L[9]);
     const float t340 = -fmaf(c1_7, rx * t240 + ry * t330, c2_7 * (t140 + t320));
     L[43] = fmaf(3.0f * mdx, t340, L[43]);
     L[41] = fmaf(4.0f * mdy, t340, L[41]);
@@ -2923,6 +3098,10 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[21] = fmaf(12.0f * m210, t340, L[21]);
     L[19] = fmaf(18.0f * m120, t340, L[19]);
     L[17] = fmaf(4.0f * m030, t340, L[17]);
+    L[14] = fmaf(4.0f * m310, t340, L[14]);
+    L[12] = fmaf(18.0f * m220, t340, L[12]);
+    L[10] = fmaf(12.0f * m130, t340, L[10]);
+    L[9] = fmaf(m040, t340, L[9]);
     const float t331 = -fmaf(c1_7, rx * t231 + ry * t321 + rz * t330, c2_7 * (t131 + t311));
     L[44] = fmaf(3.0f * mdx, t331, L[44]);
     L[42] = fmaf(3.0f * mdy, t331, L[42]);
@@ -2939,6 +3118,13 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[19] = fmaf(9.0f * m111, t331, L[19]);
     L[18] = fmaf(m030, t331, L[18]);
     L[17] = fmaf(3.0f * m021, t331, L[17]);
+    L[15] = fmaf(3.0f * m310, t331, L[15]);
+    L[14] = fmaf(m301, t331, L[14]);
+    L[13] = fmaf(9.0f * m220, t331, L[13]);
+    L[12] = fmaf(9.0f * m211, t331, L[12]);
+    L[11] = fmaf(3.0f * m130, t331, L[11]);
+    L[10] = fmaf(9.0f * m121, t331, L[10]);
+    L[9] = fmaf(m031, t331, L[9]);
     const float t322 = -fmaf(c1_7, rx * t222 + ry * t312 + rz * t321, c2_7 * (t122 + t302 + t320));
     L[42] = fmaf(2.0f * mdz, t322, L[42]);
     L[28] = fmaf(mzz, t322, L[28]);
@@ -2949,13 +3135,25 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[19] = fmaf(3.0f * m102, t322, L[19]);
     L[18] = fmaf(2.0f * m021, t322, L[18]);
     L[17] = fmaf(2.0f * m012, t322, L[17]);
+    L[15] = fmaf(2.0f * m301, t322, L[15]);
+    L[13] = fmaf(12.0f * m211, t322, L[13]);
+    L[12] = fmaf(3.0f * m202, t322, L[12]);
+    L[11] = fmaf(6.0f * m121, t322, L[11]);
+    L[10] = fmaf(6.0f * m112, t322, L[10]);
+    L[9] = fmaf(m022, t322, L[9]);
     const float t313 = -fmaf(c1_7, rx * t213 + ry * t303 + rz * t312, c2_7 * (t113 + t311));
     L[29] = fmaf(3.0f * mzz, t313, L[29]);
     L[20] = fmaf(9.0f * m102, t313, L[20]);
     L[18] = fmaf(3.0f * m012, t313, L[18]);
     L[17] = fmaf(m003, t313, L[17]);
+    L[13] = fmaf(9.0f * m202, t313, L[13]);
+    L[11] = fmaf(9.0f * m112, t313, L[11]);
+    L[10] = fmaf(3.0f * m103, t313, L[10]);
+    L[9] = fmaf(m013, t313, L[9]);
     const float t304 = -fmaf(c1_7, rx * t204 + rz * t303, c2_7 * (t104 + t302));
     L[18] = fmaf(4.0f * m003, t304, L[18]);
+    L[11] = fmaf(12.0f * m103, t304, L[11]);
+    L[9] = fmaf(m004, t304, L[9]);
     const float t250 = -fmaf(c1_7, rx * t150 + ry * t240, c2_7 * (t050 + t230));
     L[45] = fmaf(2.0f * mdx, t250, L[45]);
     L[43] = fmaf(5.0f * mdy, t250, L[43]);
@@ -2965,6 +3163,9 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[23] = fmaf(5.0f * m210, t250, L[23]);
     L[21] = fmaf(20.0f * m120, t250, L[21]);
     L[19] = fmaf(10.0f * m030, t250, L[19]);
+    L[14] = fmaf(10.0f * m220, t250, L[14]);
+    L[12] = fmaf(20.0f * m130, t250, L[12]);
+    L[10] = fmaf(5.0f * m040, t250, L[10]);
     const float t241 = -fmaf(c1_7, rx * t141 + ry * t231 + rz * t240, c2_7 * (t041 + t221));
     L[46] = fmaf(2.0f * mdx, t241, L[46]);
     L[44] = fmaf(4.0f * mdy, t241, L[44]);
@@ -2980,6 +3181,12 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[21] = fmaf(8.0f * m111, t241, L[21]);
     L[20] = fmaf(4.0f * m030, t241, L[20]);
     L[19] = fmaf(6.0f * m021, t241, L[19]);
+    L[15] = fmaf(6.0f * m220, t241, L[15]);
+    L[14] = fmaf(4.0f * m211, t241, L[14]);
+    L[13] = fmaf(8.0f * m130, t241, L[13]);
+    L[12] = fmaf(12.0f * m121, t241, L[12]);
+    L[11] = fmaf(m040, t241, L[11]);
+    L[10] = fmaf(4.0f * m031, t241, L[10]);
     const float t232 = -fmaf(c1_7, rx * t132 + ry * t222 + rz * t231, c2_7 * (t032 + t212 + t230));
     L[44] = fmaf(2.0f * mdz, t232, L[44]);
     L[30] = fmaf(mzz, t232, L[30]);
@@ -2990,13 +3197,29 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[21] = fmaf(2.0f * m102, t232, L[21]);
     L[20] = fmaf(6.0f * m021, t232, L[20]);
     L[19] = fmaf(3.0f * m012, t232, L[19]);
+    L[15] = fmaf(6.0f * m211, t232, L[15]);
+    L[14] = fmaf(m202, t232, L[14]);
+    L[13] = fmaf(12.0f * m121, t232, L[13]);
+    L[12] = fmaf(6.0f * m112, t232, L[12]);
+    L[11] = fmaf(2.0f * m031, t232, L[11]);
+    L[10] = fmaf(3.0f * m022, t232, L[10]);
     const float t223 = -fmaf(c1_7, rx * t123 + ry * t213 + rz * t222, c2_7 * (t023 + t203 + t221));
     L[31] = fmaf(3.0f * mzz, t223, L[31]);
     L[22] = fmaf(6.0f * m102, t223, L[22]);
     L[20] = fmaf(6.0f * m012, t223, L[20]);
     L[19] = fmaf(m003, t223, L[19]);
+    L[15] = fmaf(3.0f * m202, t223, L[15]);
+    L[13] = fmaf(12.0f * m112, t223, L[13]);
+    L[12] = fmaf(2.0f * m103, t223, L[12]);
+    L[11] = fmaf(3.0f * m022, t223, L[11]);
+    L[10] = fmaf(2.0f * m013, t223, L[10]);
     const float t214 = -fmaf(c1_7, rx * t114 + ry * t204 + rz * t213, c2_7 * (t014 + t212));
     L[20] = fmaf(4.0f * m003, t214, L[20]);
+    L[13] = fmaf(8.0f * m103, t214, L[13]);
+    L[11] = fmaf(4.0f * m013, t214, L[11]);
+    L[10] = fmaf(m004, t214, L[10]);
+    const float t205 = -fmaf(c1_7, rx * t105 + rz * t204, c2_7 * (t005 + t203));
+    L[11] = fmaf(5.0f * m004, t205, L[11]);
     const float t160 = -fmaf(c1_7, rx * t060 + ry * t150, c2_7 * (t140));
     L[47] = fmaf(mdx, t160, L[47]);
     L[45] = fmaf(6.0f * mdy, t160, L[45]);
@@ -3004,6 +3227,8 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[34] = fmaf(6.0f * mxy, t160, L[34]);
     L[23] = fmaf(15.0f * m120, t160, L[23]);
     L[21] = fmaf(20.0f * m030, t160, L[21]);
+    L[14] = fmaf(20.0f * m130, t160, L[14]);
+    L[12] = fmaf(15.0f * m040, t160, L[12]);
     const float t151 = -fmaf(c1_7, rx * t051 + ry * t141 + rz * t150, c2_7 * (t131));
     L[48] = fmaf(mdx, t151, L[48]);
     L[46] = fmaf(5.0f * mdy, t151, L[46]);
@@ -3016,6 +3241,10 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[23] = fmaf(5.0f * m111, t151, L[23]);
     L[22] = fmaf(10.0f * m030, t151, L[22]);
     L[21] = fmaf(10.0f * m021, t151, L[21]);
+    L[15] = fmaf(10.0f * m130, t151, L[15]);
+    L[14] = fmaf(10.0f * m121, t151, L[14]);
+    L[13] = fmaf(5.0f * m040, t151, L[13]);
+    L[12] = fmaf(10.0f * m031, t151, L[12]);
     const float t142 = -fmaf(c1_7, rx * t042 + ry * t132 + rz * t141, c2_7 * (t122 + t140));
     L[46] = fmaf(2.0f * mdz, t142, L[46]);
     L[32] = fmaf(mzz, t142, L[32]);
@@ -3025,17 +3254,31 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[23] = fmaf(m102, t142, L[23]);
     L[22] = fmaf(12.0f * m021, t142, L[22]);
     L[21] = fmaf(4.0f * m012, t142, L[21]);
+    L[15] = fmaf(12.0f * m121, t142, L[15]);
+    L[14] = fmaf(4.0f * m112, t142, L[14]);
+    L[13] = fmaf(8.0f * m031, t142, L[13]);
+    L[12] = fmaf(6.0f * m022, t142, L[12]);
     const float t133 = -fmaf(c1_7, rx * t033 + ry * t123 + rz * t132, c2_7 * (t113 + t131));
     L[33] = fmaf(3.0f * mzz, t133, L[33]);
     L[24] = fmaf(3.0f * m102, t133, L[24]);
     L[22] = fmaf(9.0f * m012, t133, L[22]);
     L[21] = fmaf(m003, t133, L[21]);
+    L[15] = fmaf(9.0f * m112, t133, L[15]);
+    L[14] = fmaf(m103, t133, L[14]);
+    L[13] = fmaf(9.0f * m022, t133, L[13]);
+    L[12] = fmaf(3.0f * m013, t133, L[12]);
     const float t124 = -fmaf(c1_7, rx * t024 + ry * t114 + rz * t123, c2_7 * (t104 + t122));
     L[22] = fmaf(4.0f * m003, t124, L[22]);
+    L[15] = fmaf(4.0f * m103, t124, L[15]);
+    L[13] = fmaf(8.0f * m013, t124, L[13]);
+    L[12] = fmaf(m004, t124, L[12]);
+    const float t115 = -fmaf(c1_7, rx * t015 + ry * t105 + rz * t114, c2_7 * (t113));
+    L[13] = fmaf(5.0f * m004, t115, L[13]);
     const float t070 = -fmaf(c1_7, ry * t060, c2_7 * (t050));
     L[47] = fmaf(7.0f * mdy, t070, L[47]);
     L[34] = fmaf(21.0f * myy, t070, L[34]);
     L[23] = fmaf(35.0f * m030, t070, L[23]);
+    L[14] = fmaf(35.0f * m040, t070, L[14]);
     const float t061 = -fmaf(c1_7, ry * t051 + rz * t060, c2_7 * (t041));
     L[48] = fmaf(6.0f * mdy, t061, L[48]);
     L[47] = fmaf(mdz, t061, L[47]);
@@ -3043,18 +3286,28 @@ __device__ inline void m2l_acc_p6hot7(float rx, float ry, float rz, float q, flo
     L[34] = fmaf(6.0f * myz, t061, L[34]);
     L[24] = fmaf(20.0f * m030, t061, L[24]);
     L[23] = fmaf(15.0f * m021, t061, L[23]);
+    L[15] = fmaf(15.0f * m040, t061, L[15]);
+    L[14] = fmaf(20.0f * m031, t061, L[14]);
     const float t052 = -fmaf(c1_7, ry * t042 + rz * t051, c2_7 * (t032 + t050));
     L[48] = fmaf(2.0f * mdz, t052, L[48]);
     L[34] = fmaf(mzz, t052, L[34]);
     L[35] = fmaf(10.0f * myz, t052, L[35]);
     L[24] = fmaf(20.0f * m021, t052, L[24]);
     L[23] = fmaf(5.0f * m012, t052, L[23]);
+    L[15] = fmaf(20.0f * m031, t052, L[15]);
+    L[14] = fmaf(10.0f * m022, t052, L[14]);
     const float t043 = -fmaf(c1_7, ry * t033 + rz * t042, c2_7 * (t023 + t041));
     L[35] = fmaf(3.0f * mzz, t043, L[35]);
     L[24] = fmaf(12.0f * m012, t043, L[24]);
     L[23] = fmaf(m003, t043, L[23]);
+    L[15] = fmaf(18.0f * m022, t043, L[15]);
+    L[14] = fmaf(4.0f * m013, t043, L[14]);
     const float t034 = -fmaf(c1_7, ry * t024 + rz * t033, c2_7 * (t014 + t032));
     L[24] = fmaf(4.0f * m003, t034, L[24]);
+    L[15] = fmaf(12.0f * m013, t034, L[15]);
+    L[14] = fmaf(m004, t034, L[14]);
+    const float t025 = -fmaf(c1_7, ry * t015 + rz * t024, c2_7 * (t005 + t023));
+    L[15] = fmaf(5.0f * m004, t025, L[15]);
 }
 
 // potential and field at offset y from the expansion centre (order P = 6, from the

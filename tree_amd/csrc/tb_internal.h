@@ -67,7 +67,7 @@ struct SpeciesDev {
 // about each cell's geometric centre (a leaf: its particle).
 // joint-tree moments per cell: net charge, dipole, second moment (xx yy zz
 // xy yz zx) and third moment (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
-constexpr int TB_JQ = 20;
+constexpr int TB_JQ = 35;
 
 struct JointDev {
     int n = 0;                  // ne + ni
@@ -135,11 +135,13 @@ struct CellRecs {
     double4 *bmac = nullptr;    // centre (a leaf: its particle), s / theta (a leaf: 0)
     uint32_t *binfo = nullptr;  // first child << 3 | children - 1, or leaf | joint particle
     float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr, *bq3 = nullptr, *bq4 = nullptr, *bf = nullptr;
+    float4 *bq5 = nullptr, *bq6 = nullptr;
     uint8_t *bflag = nullptr;   // bit 0: first child of its parent; bit 1: leaf
     uint64_t *bkey = nullptr;   // a cell's first particle's key (its centre replays from it)
     int32_t *bcnt = nullptr;    // a cell's particle count
     double *bnq = nullptr;      // [nn][TB_JQ] a cell's FP64 q, D(3), M(6) (xx yy zz xy yz zx),
-                                // O(10) about its centre
+                                // O(10), H(15) (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz
+                                // yyyy yyyz yyzz yzzz zzzz) about its centre
     double theta = 0.0;
     int32_t *flag = nullptr;    // 16: malformed node table
 };
@@ -176,7 +178,7 @@ void tb_walk64_list(hipStream_t s, int nt, const int32_t *tsp, const int32_t *id
 int tb_walk64_blocks(int n);
 // local.hip: the far-field local expansion of cells about one centre, evaluated at offsets
 // (TB_LOCAL_CELL floats per cell)
-constexpr int TB_LOCAL_CELL = 19;
+constexpr int TB_LOCAL_CELL = 28;
 int tb_local_run(hipStream_t s, int order, int ncell, const float *cells, int ny, const float *y, float *out);
 
 void tb_direct(hipStream_t s, int nt, const int32_t *tsp, const int32_t *tidx_sorted, const SpeciesDev &E,

@@ -10,7 +10,9 @@
 //   bf    float4   the learned criterion's per-cell features (log2 / 8 of
 //                  |q|/count, |D|/(count s), |Q'|/(count s^2), count)
 //   bq3   float4   O'xxx, O'xxy, O'xxz, O'xyy   traceless third moment
-//   bq4   float4   O'xyz, O'yyy, O'yyz, 0       (the others follow from tr = 0)
+//   bq4   float4   O'xyz, O'yyy, O'yyz, H'xxxx  (the other O' follow from tr = 0)
+//   bq5   float4   H'xxxy, H'xxxz, H'xxyy, H'xxyz   traceless fourth moment
+//   bq6   float4   H'xyyy, H'xyyz, H'yyyy, H'yyyz   (the others follow from tr = 0)
 #pragma once
 #include "tb_internal.h"
 #include "tb_mlp.h"
@@ -20,7 +22,8 @@ struct MixedTree {
     double4 *bmac = nullptr;
     uint32_t *binfo = nullptr;
     float4 *bq0 = nullptr, *bq1 = nullptr, *bq2 = nullptr, *bf = nullptr;
-    float4 *bq3 = nullptr, *bq4 = nullptr;   // traceless third moment (far-field local expansion)
+    float4 *bq3 = nullptr, *bq4 = nullptr;   // traceless third moment (and H'xxxx)
+    float4 *bq5 = nullptr, *bq6 = nullptr;   // traceless fourth moment
     uint32_t *inv = nullptr;
     // the upward pass's breadth-first companions (CellRecs)
     uint8_t *bflag = nullptr;
@@ -43,7 +46,7 @@ struct Domain {
     int n[2], ng[2];
     const double4 *bmac;
     const uint32_t *binfo;
-    const float4 *bq0, *bq1, *bq2, *bf, *bq3, *bq4;
+    const float4 *bq0, *bq1, *bq2, *bf, *bq3, *bq4, *bq5, *bq6;
     float inv_rs, rs;
     float close2;     // (22 rs)^2: unlike pairs closer than this take the Kelbg term
     int gbase;        // first global group index of this domain

@@ -39,7 +39,7 @@ namespace {
 
 constexpr int BT = 256;
 constexpr int WPB = BT / 64;
-constexpr int STACK = 1024;
+constexpr int STACK = 512;
 // Interaction lists are flushed once FLUSH_M (cells) or FLUSH_P (leaves)
 // entries are pending.  Each pass of the walk loop (a traversal step, or a
 // learned band decision) adds at most 64 entries to a list before the flush
@@ -51,11 +51,11 @@ constexpr int STACK = 1024;
 // walk flushes at the lengths of one of TB_LIST_VARIANTS, chosen by the
 // autotuner with the group size (shorter lists: more flushes of fewer
 // entries, each flush's compensated sums and loop set-up paid more often).
-constexpr int FLUSH_M = 15;
+constexpr int FLUSH_M = 11;
 constexpr int LCAP_M = FLUSH_M - 1 + 64;
 constexpr int FLUSH_P = 24;
 constexpr int LCAP_P = FLUSH_P - 1 + 64;
-constexpr int LIST_M[TB_LIST_VARIANTS] = {15, 11, 7};
+constexpr int LIST_M[TB_LIST_VARIANTS] = {11, 9, 7};
 constexpr int LIST_P[TB_LIST_VARIANTS] = {24, 16, 12};
 constexpr int QCAP = 128;
 // Unlike leaves within 22 rs of some target of the group: up to NEXACT of
@@ -69,12 +69,13 @@ constexpr int CTR_FAR = 4;    // far-chunk allocation counter
 constexpr int FAR_SLAB = 256; // far chunks a wave reserves at a time
 constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decided by the network
 // A cell of at most SMALL_OPEN particles is always opened: its particles'
-// P2P is exact and, for two particles, cheaper than one M2P with the third
-// moment (~80 VALU instructions per target).  Measured at N = 4M, 1e-4 rms,
-// theta re-tuned: walk 70.0 ms without the rule, 67.7 ms with it (3 and 4
-// particles: no further gain).  Far cells are opened too, so that the far
-// path changes how accepted cells are evaluated, never which are accepted.
-constexpr int SMALL_OPEN = 2;
+// P2P (15 VALU instructions per pair) is exact and cheaper than one M2P to
+// the fourth moment (~120 per target).  Measured at N = 4M, 1e-4 rms, far
+// threshold 4, theta re-tuned (0.395-0.397 for each): walk 40.0 ms at 2
+// particles, 37.6 at 4, 37.5 at 6, 38.1 at 8.  Far cells are opened too, so
+// that the far path changes how accepted cells are evaluated, never which
+// are accepted.
+constexpr int SMALL_OPEN = 5;
 
 const char *g_err = "";
 
@@ -88,14 +89,18 @@ constexpr int NLOC = FAR_ORDER == 3   ? TB_LOCAL_N3
 // truncation error of the order-P expansion falls as (1/K)^P at the group's
 // edge, and a higher order costs more per far cell.  Measured at N = 4M,
 // groups of 128, theta re-tuned to 1e-4 rms for each (order, K), walk + far
-// evaluation: order 4 at K = 6, 64.5 ms; order 5 at K = 4, 58.4 ms; order 6
-// at K = 3, 55.5 ms (K = 3.5: 57.1; K = 2.5 needs theta below 0.22); order 7
-// at K = 2.5 or 3, 65 ms (its far evaluation costs more than it saves).
-// Order 6 keeping only the terms of total order |b| + |c| <= 7 (2 of the
-// 3 orders of moment-times-coefficient products beyond it dropped): 53.0 ms
-// at K = 3 against 56.2 for the full order 6 on the same GPU (<= 8: 53.9;
-// <= 6: theta falls to 0.256 and the gain is lost).
-constexpr double FAR_RATIO_DEFAULT = 3.0;
+// evaluation, cells to the third moment: order 4 at K = 6, 64.5 ms; order 5
+// at K = 4, 58.4 ms; order 6 at K = 3, 55.5 ms (K = 3.5: 57.1; K = 2.5 needs
+// theta below 0.22); order 7 at K = 2.5 or 3, 65 ms (its far evaluation
+// costs more than it saves).  Order 6 keeping only the terms of total order
+// |b| + |c| <= 7 (2 of the 3 orders of moment-times-coefficient products
+// beyond it dropped): 53.0 ms at K = 3 against 56.2 for the full order 6 on
+// the same GPU (<= 8: 53.9; <= 6: theta falls to 0.256 and the gain is lost).
+// With the cells' fourth moment (in the M2P and in the far cells' M2L) theta
+// rises to ~0.39 and the balance moves: K = 3, 38.8 ms; K = 4, 37.5 ms; K = 5,
+// 38.6 ms (cells of up to 6 particles opened); K = 2.5 reaches 1e-4 at no theta
+// below 0.5.  Without the fourth moment in the M2L, theta stayed at 0.28.
+constexpr double FAR_RATIO_DEFAULT = 4.0;
 
 struct WaveLDS {
     uint32_t stack[STACK];
@@ -103,8 +108,9 @@ struct WaveLDS {
     uint32_t bandq[QCAP];                 // learned: band cells waiting for a decision
     uint32_t farq[QCAP];                  // far accepted cells waiting for the local expansion
     // accepted cells: x y z q | Dx Dy Dz Q'xx | Q'yy Q'xy Q'yz Q'zx |
-    // O'xxx O'xxy O'xxz O'xyy | O'xyz O'yyy O'yyz -
-    float4 m0[LCAP_M], m1[LCAP_M], m2[LCAP_M], m3[LCAP_M], m4[LCAP_M];
+    // O'xxx O'xxy O'xxz O'xyy | O'xyz O'yyy O'yyz H'xxxx |
+    // H'xxxy H'xxxz H'xxyy H'xxyz | H'xyyy H'xyyz H'yyyy H'yyyz
+    float4 m0[LCAP_M], m1[LCAP_M], m2[LCAP_M], m3[LCAP_M], m4[LCAP_M], m5[LCAP_M], m6[LCAP_M];
     float4 p[LCAP_P];                     // leaves x y z, record index: like charges from 0 up, unlike from LCAP_P - 1 down
     uint32_t exact[NEXACT];               // leaves within 22 rs of some target: FP64 separations after the walk
 };
@@ -277,9 +283,11 @@ struct Acc {
     float ps[TPL], pc[TPL];      // compensated potential
 };
 
-// monopole + dipole + traceless quadrupole + traceless third moment about
-// the cell centre: phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5) + 5/2 O'rrr/r^7,
-// E = -grad phi (the O' part: -15/2 v/r^7 + 35/2 (r.v) r/r^9, v_i = O'_ijk r_j r_k)
+// monopole + dipole + traceless quadrupole + traceless third and fourth
+// moments about the cell centre: phi = q/r + D.r/r^3 + r.Q'.r/(2 r^5) +
+// 5/2 O'rrr/r^7 + 35/8 H'rrrr/r^9, E = -grad phi (the O' part: -15/2 v/r^7 +
+// 35/2 (r.v) r/r^9, v_i = O'_ijk r_j r_k; the H' part: -35/2 w/r^9 + 315/8
+// (r.w) r/r^11, w_i = H'_ijkl r_j r_k r_l)
 template <int TPL>
 __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
                                  const float (&rz)[TPL], Acc<TPL> &A)
@@ -288,13 +296,19 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
 #pragma unroll
     for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
     for (int j = 0; j < cnt; j++) {
-        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j], o1 = L.m3[j], o2 = L.m4[j];
+        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j], o1 = L.m3[j], o2 = L.m4[j], h1 = L.m5[j],
+                     h2 = L.m6[j];
         const float qxx = b.w, qyy = c.x, qzz = -b.w - c.x, qxy = c.y, qyz = c.z, qzx = c.w;
         // O' from its 7 independent components (tr = 0: O'_xzz = -(O'_xxx +
         // O'_xyy), O'_yzz = -(O'_xxy + O'_yyy), O'_zzz = -(O'_xxz + O'_yyz)),
         // so v = O'rr needs xx - zz, yy - zz and the doubled mixed products
         const float oxxx = o1.x, oxxy = o1.y, oxxz = o1.z, oxyy = o1.w, oxyz = o2.x, oyyy = o2.y, oyyz = o2.z;
         const float sxy = oxxx + oxyy, syz = oxxy + oyyy;
+        // H' from its 9 independent components; w = H'rrr in the basis of
+        // the cubic harmonics A1..A7 below (tr = 0 folded in)
+        const float hxxxx = o2.w, hxxxy = h1.x, hxxxz = h1.y, hxxyy = h1.z, hxxyz = h1.w, hxyyy = h2.x,
+                    hxyyz = h2.y, hyyyy = h2.z, hyyyz = h2.w;
+        const float s1 = hxxxx + hxxyy, s2 = hxxxy + hxyyy, s3 = hxxyy + hyyyy;
 #pragma unroll
         for (int t = 0; t < TPL; t++) {
             const float dx = rx[t] - a.x, dy = ry[t] - a.y, dz = rz[t] - a.z;
@@ -314,15 +328,23 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
             const float vy = oxxy * xmz + oyyy * ymz + oxyy * xy2 + oxyz * xz2 + oyyz * yz2;
             const float vz = oxxz * xmz + oyyz * ymz + oxyz * xy2 - sxy * xz2 - syz * yz2;
             const float t3 = u * (dx * vx + dy * vy + dz * vz);
-            // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3))) - ri3 (D + u (Q.dx + 7.5 u v))
-            // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3)))
-            const float a5 = u * fmaf(17.5f, t3, 2.5f * xqx);
+            const float zz2 = zz + zz;
+            const float A1 = dx * fmaf(-2.0f, zz, xmz), A2 = 3.0f * (dy * xmz), A3 = dz * fmaf(3.0f, xmz, zz2);
+            const float A4 = 3.0f * (dx * ymz), A5 = 3.0f * (xy2 * dz), A6 = dy * fmaf(-2.0f, zz, ymz);
+            const float A7 = dz * fmaf(3.0f, ymz, zz2);
+            const float wx = hxxxx * A1 + hxxxy * A2 + hxxxz * A3 + hxxyy * A4 + hxxyz * A5 + hxyyy * A6 + hxyyz * A7;
+            const float wy = hxxxy * A1 + hxxyy * A2 + hxxyz * A3 + hxyyy * A4 + hxyyz * A5 + hyyyy * A6 + hyyyz * A7;
+            const float wz = hxxxz * A1 + hxxyz * A2 + hxyyz * A4 + hyyyz * A6 - s1 * A3 - s2 * A5 - s3 * A7;
+            const float t4 = u * u * (dx * wx + dy * wy + dz * wz);
+            // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3 + 39.375 t4))) - ri3 (D + u (Q.dx + u (7.5 v + 17.5 u w)))
+            // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3 + 4.375 t4)))
+            const float a5 = u * fmaf(39.375f, t4, fmaf(17.5f, t3, 2.5f * xqx));
             const float w = ri3 * fmaf(u, fmaf(3.0f, dp, a5), a.w);
-            const float u75 = 7.5f * u;
-            ax[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vx, qx), b.x), fmaf(dx, w, ax[t]));
-            ay[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vy, qy), b.y), fmaf(dy, w, ay[t]));
-            az[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vz, qz), b.z), fmaf(dz, w, az[t]));
-            ap[t] = fmaf(ri, fmaf(u, dp + u * fmaf(2.5f, t3, 0.5f * xqx), a.w), ap[t]);
+            const float u75 = 7.5f * u, u175 = 17.5f * u * u;
+            ax[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wx, fmaf(u75, vx, qx)), b.x), fmaf(dx, w, ax[t]));
+            ay[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wy, fmaf(u75, vy, qy)), b.y), fmaf(dy, w, ay[t]));
+            az[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wz, fmaf(u75, vz, qz)), b.z), fmaf(dz, w, az[t]));
+            ap[t] = fmaf(ri, fmaf(u, dp + u * fmaf(4.375f, t4, fmaf(2.5f, t3, 0.5f * xqx)), a.w), ap[t]);
         }
     }
 #pragma unroll
@@ -488,10 +510,11 @@ __device__ inline void exact_pairs(const WaveLDS &L, int nx, const float (&rx)[T
     }
 }
 
-// q0..q2: q, D, traceless quadrupole; q3, q4: traceless third moment (orders 4 to 6)
+// q0..q2: q, D, traceless quadrupole; q3, q4: traceless third moment (orders 4 to 6);
+// q4.w, q5, q6: traceless fourth moment (order 6)
 template <int N>
 __device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4,
-                               float (&L)[N])
+                               float4 q5, float4 q6, float (&L)[N])
 {
     if constexpr (N == TB_LOCAL_N3)
         m2l_acc_p3(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, L);
@@ -502,8 +525,8 @@ __device__ inline void m2l_acc(float rx, float ry, float rz, float4 q0, float4 q
         m2l_acc_p5ho(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
                      q4.y, q4.z, L);
     else
-        m2l_acc_p6hot7(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
-                     q4.y, q4.z, L);
+        m2l_acc_p6hxt7(rx, ry, rz, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q3.x, q3.y, q3.z, q3.w, q4.x,
+                       q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w, q6.x, q6.y, q6.z, q6.w, L);
 }
 
 template <int N>
@@ -814,6 +837,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
                     L.m3[k] = D.bq3[node];
                     L.m4[k] = D.bq4[node];
+                    L.m5[k] = D.bq5[node];
+                    L.m6[k] = D.bq6[node];
                 }
                 nm += __popcll(mk);
                 // leaves -> P2P list: like charges from the bottom, unlike from the top
@@ -921,6 +946,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
                     L.m3[k] = D.bq3[nd];
                     L.m4[k] = D.bq4[nd];
+                    L.m5[k] = D.bq5[nd];
+                    L.m6[k] = D.bq6[nd];
                 }
                 nm += __popcll(bk);
                 const uint32_t binf = bopen ? D.binfo[nd] : 0u;
@@ -1241,7 +1268,7 @@ __global__ __launch_bounds__(BT) void k_sfar_eval(SegFarArgs A)
     for (int c = uni(A.sfar[sg]); c >= 0; c = uni(A.far_link[c])) {
         const uint32_t nd = A.far_cells[(size_t)c * 64 + lane];
         float rx = 1.0f, ry = 0.0f, rz = 0.0f;
-        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0;
+        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0, q5 = q0, q6 = q0;
         if (nd != 0xffffffffu) {
             const double4 m = D.bmac[nd];
             rx = (float)(sc.x - m.x);
@@ -1252,8 +1279,10 @@ __global__ __launch_bounds__(BT) void k_sfar_eval(SegFarArgs A)
             q2 = D.bq2[nd];
             q3 = D.bq3[nd];
             q4 = D.bq4[nd];
+            q5 = D.bq5[nd];
+            q6 = D.bq6[nd];
         }
-        m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, Lp);
+        m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, q5, q6, Lp);
     }
 #pragma unroll
     for (int q = 0; q < NLOC; q++) {
@@ -1309,7 +1338,7 @@ __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
     while (c >= 0) {
         const uint32_t nd = A.far_cells[(size_t)c * 64 + lane];
         float rx = 1.0f, ry = 0.0f, rz = 0.0f;
-        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0;
+        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0, q5 = q0, q6 = q0;
         if (nd != 0xffffffffu) {
             const double4 m = D.bmac[nd];
             rx = (float)(gx - m.x);
@@ -1320,8 +1349,10 @@ __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
             q2 = D.bq2[nd];
             q3 = D.bq3[nd];
             q4 = D.bq4[nd];
+            q5 = D.bq5[nd];
+            q6 = D.bq6[nd];
         }
-        m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, Lp);
+        m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, q5, q6, Lp);
         c = uni(A.far_link[c]);
     }
 #pragma unroll
@@ -1434,7 +1465,7 @@ struct LabelArgs {
 
 // label of a band pair: log10 of the rms, over the group's targets, of
 // |F_M2P - F_exact| / |F_target| with F_M2P the cell's monopole + dipole +
-// quadrupole + third-moment field and F_exact the FP64 sum over the cell's particles with the
+// quadrupole + third- and fourth-moment field and F_exact the FP64 sum over the cell's particles with the
 // reference's pair rules (like: Coulomb beyond 0.1 um; unlike: Kelbg).  The
 // rms (not the largest) is the cell's share of the rms force error that the
 // walk is tuned to.
@@ -1459,6 +1490,16 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
     const double oxxx = O[0] - 0.6 * tx, oxxy = O[1] - 0.2 * ty, oxxz = O[2] - 0.2 * tz, oxyy = O[3] - 0.2 * tx;
     const double oxyz = O[4], oxzz = O[5] - 0.2 * tx, oyyy = O[6] - 0.6 * ty, oyyz = O[7] - 0.2 * tz;
     const double oyzz = O[8] - 0.2 * ty, ozzz = O[9] - 0.6 * tz;
+    // traceless fourth moment (raw H: xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz
+    // xyzz xzzz yyyy yyyz yyzz yzzz zzzz), its 9 independent components
+    const double *H = Q + 20;
+    const double txx = H[0] + H[3] + H[5], tyy = H[3] + H[10] + H[12], tzz = H[5] + H[12] + H[14];
+    const double txy = H[1] + H[6] + H[8], tyz = H[4] + H[11] + H[13], tzx = H[2] + H[7] + H[9];
+    const double t0 = txx + tyy + tzz;
+    const double hxxxx = H[0] - 6.0 * txx / 7.0 + 3.0 * t0 / 35.0, hxxxy = H[1] - 3.0 * txy / 7.0;
+    const double hxxxz = H[2] - 3.0 * tzx / 7.0, hxxyy = H[3] - (txx + tyy) / 7.0 + t0 / 35.0;
+    const double hxxyz = H[4] - tyz / 7.0, hxyyy = H[6] - 3.0 * txy / 7.0, hxyyz = H[7] - tzx / 7.0;
+    const double hyyyy = H[10] - 6.0 * tyy / 7.0 + 3.0 * t0 / 35.0, hyyyz = H[11] - 3.0 * tyz / 7.0;
     const int s0 = A.nstart[p], cnt = A.ncount[p];
     double sq = 0.0;  // sum over the group's targets of the squared relative error
     for (int t = threadIdx.x; t < gr.y; t += 64) {
@@ -1475,11 +1516,22 @@ __global__ __launch_bounds__(64) void k_labels(LabelArgs A)
         const double vx = oxxx * dx * dx + oxyy * dy * dy + oxzz * dz * dz + 2.0 * (oxxy * dx * dy + oxxz * dx * dz + oxyz * dy * dz);
         const double vy = oxxy * dx * dx + oyyy * dy * dy + oyzz * dz * dz + 2.0 * (oxyy * dx * dy + oxyz * dx * dz + oyyz * dy * dz);
         const double vz = oxxz * dx * dx + oyyz * dy * dy + ozzz * dz * dz + 2.0 * (oxyz * dx * dy + oxzz * dx * dz + oyzz * dy * dz);
-        const double s3 = dx * vx + dy * vy + dz * vz, ri9 = ri7 * ri2;
-        const double k = q * ri3 + 3.0 * dp * ri5 + 2.5 * xqx * ri7 + 17.5 * s3 * ri9;
-        double ex = dx * k - Dx * ri3 - qx * ri5 - 7.5 * vx * ri7;
-        double ey = dy * k - Dy * ri3 - qy * ri5 - 7.5 * vy * ri7;
-        double ez = dz * k - Dz * ri3 - qz * ri5 - 7.5 * vz * ri7;
+        const double s3 = dx * vx + dy * vy + dz * vz, ri9 = ri7 * ri2, ri11 = ri9 * ri2;
+        // w = H'rrr in the cubic harmonics A1..A7 (the walk's form, in FP64)
+        const double zz = dz * dz, xmz = dx * dx - zz, ymz = dy * dy - zz;
+        const double A1 = dx * (xmz - 2.0 * zz), A2 = 3.0 * dy * xmz, A3 = dz * (3.0 * xmz + 2.0 * zz);
+        const double A4 = 3.0 * dx * ymz, A5 = 6.0 * dx * dy * dz, A6 = dy * (ymz - 2.0 * zz);
+        const double A7 = dz * (3.0 * ymz + 2.0 * zz);
+        double wv[3];
+        wv[0] = hxxxx * A1 + hxxxy * A2 + hxxxz * A3 + hxxyy * A4 + hxxyz * A5 + hxyyy * A6 + hxyyz * A7;
+        wv[1] = hxxxy * A1 + hxxyy * A2 + hxxyz * A3 + hxyyy * A4 + hxyyz * A5 + hyyyy * A6 + hyyyz * A7;
+        wv[2] = hxxxz * A1 + hxxyz * A2 + hxyyz * A4 + hyyyz * A6 - (hxxxx + hxxyy) * A3 - (hxxxy + hxyyy) * A5 -
+                (hxxyy + hyyyy) * A7;
+        const double s4 = dx * wv[0] + dy * wv[1] + dz * wv[2];
+        const double k = q * ri3 + 3.0 * dp * ri5 + 2.5 * xqx * ri7 + 17.5 * s3 * ri9 + 39.375 * s4 * ri11;
+        double ex = dx * k - Dx * ri3 - qx * ri5 - 7.5 * vx * ri7 - 17.5 * wv[0] * ri9;
+        double ey = dy * k - Dy * ri3 - qy * ri5 - 7.5 * vy * ri7 - 17.5 * wv[1] * ri9;
+        double ez = dz * k - Dz * ri3 - qz * ri5 - 7.5 * vz * ri7 - 17.5 * wv[2] * ri9;
         double gx = 0.0, gy = 0.0, gz = 0.0;
         for (int j = s0; j < s0 + cnt; j++) {
             const double4 sj = A.pos[j];
@@ -1565,7 +1617,7 @@ Domain make_domain(SpeciesDev *const sp[2], const MixedTree *t, double rs, int g
         d.grp[k] = sp[k]->grp;
     }
     d.bmac = t->bmac; d.binfo = t->binfo; d.bq0 = t->bq0; d.bq1 = t->bq1; d.bq2 = t->bq2; d.bf = t->bf;
-    d.bq3 = t->bq3; d.bq4 = t->bq4;
+    d.bq3 = t->bq3; d.bq4 = t->bq4; d.bq5 = t->bq5; d.bq6 = t->bq6;
     d.inv_rs = (float)(1.0 / rs);
     d.close2 = (float)(484.0 * rs * rs);  // (22 rs)^2: closer unlike pairs take the Kelbg term
     d.rs = (float)rs;
@@ -1850,6 +1902,7 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, const JointDev *jt, size_t *bytes)
     int cap = T.cap, rc;
     int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap, c8 = cap, c9 = cap;
     int c10 = cap, c11 = cap, c12 = cap, c13 = cap * TB_JQ;  // bnq: TB_JQ doubles per node
+    int c14 = cap, c15 = cap;
     if ((rc = grow(&T.bmac, &c1, J.nn, bytes))) return rc;
     if ((rc = grow(&T.binfo, &c2, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq0, &c3, J.nn, bytes))) return rc;
@@ -1859,12 +1912,15 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, const JointDev *jt, size_t *bytes)
     if ((rc = grow(&T.bf, &c7, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq3, &c8, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bq4, &c9, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bq5, &c14, J.nn, bytes))) return rc;
+    if ((rc = grow(&T.bq6, &c15, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bflag, &c10, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bkey, &c11, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bcnt, &c12, J.nn, bytes))) return rc;
     if ((rc = grow(&T.bnq, &c13, J.nn * TB_JQ, bytes))) return rc;
     T.cap = std::min(std::min(std::min(std::min(c1, c2), std::min(c3, c4)), std::min(std::min(c5, c6), c7)),
-                     std::min(std::min(c8, c9), std::min(std::min(c10, c11), std::min(c12, c13 / TB_JQ))));
+                     std::min(std::min(std::min(c8, c9), std::min(c14, c15)),
+                              std::min(std::min(c10, c11), std::min(c12, c13 / TB_JQ))));
     T.nn = J.nn;
     if (hipGetLastError() != hipSuccess) {
         g_err = "mixed walk: prep launch failed";
@@ -1885,6 +1941,8 @@ CellRecs tb_mixed_recs(MixedDev &M, const tb_config &cfg)
     R.bq2 = T.bq2;
     R.bq3 = T.bq3;
     R.bq4 = T.bq4;
+    R.bq5 = T.bq5;
+    R.bq6 = T.bq6;
     R.bf = T.bf;
     R.bflag = T.bflag;
     R.bkey = T.bkey;
@@ -2005,7 +2063,7 @@ void tb_mixed_free(MixedDev &M, size_t *bytes)
 {
     MixedTree &T = M.t[0];
     hipFree(T.bmac); hipFree(T.binfo); hipFree(T.bq0); hipFree(T.bq1); hipFree(T.bq2); hipFree(T.inv);
-    hipFree(T.bf); hipFree(T.bq3); hipFree(T.bq4);
+    hipFree(T.bf); hipFree(T.bq3); hipFree(T.bq4); hipFree(T.bq5); hipFree(T.bq6);
     hipFree(T.bflag); hipFree(T.bkey); hipFree(T.bcnt); hipFree(T.bnq);
     T = MixedTree();
     if (M.far_cells) hipFree(M.far_cells);

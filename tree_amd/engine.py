@@ -287,9 +287,11 @@ class Engine:
         return f
 
     def local_eval(self, order, cells, y):
-        """The far-field local expansion check: cells [n][19], offsets y [m][3] ->
+        """The far-field local expansion check: cells [n][28], offsets y [m][3] ->
         [m][4] potential and field (FP32)."""
         cells = np.ascontiguousarray(cells, np.float32)
+        if cells.ndim != 2 or cells.shape[1] != 28:
+            raise ValueError("local_eval: cells must be [n][28]")
         y = np.ascontiguousarray(y, np.float32)
         out = np.zeros((len(y), 4), np.float32)
         _check(self.lib.tb_local_eval(self.h, order, len(cells), _ptr(cells), len(y), _ptr(y), _ptr(out)))
