@@ -299,7 +299,7 @@ def train_mlp(a, th_lo, th_hi):
 def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
     """The learned criterion in the band theta_lo .. theta_hi, its error threshold bisected to the target."""
     eng = ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=th_hi, theta_lo=th_lo, mac="learned",
-                    err_target=1e-9, group=a.group)
+                    err_target=1e-9, group=a.group, cost_weight=a.cost_weight)
     eng.set_particles(qel, qion)
     eng.mlp_set(params)
     eng.build()
@@ -331,7 +331,8 @@ def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
     eng.close()
     log("mixed+learned: err_target=%.2e rms=%.2e %.3f steps/s walk %.2f ms"
         % (et, rms_l, 1 / r["s"], r["ms_walk"]))
-    return {"variant": "mixed+learned", "theta": th_hi, "theta_lo": th_lo, "err_target": et, "rms": rms_l,
+    return {"variant": "mixed+learned", "theta": th_hi, "theta_lo": th_lo, "err_target": et,
+            "cost_weight": a.cost_weight, "rms": rms_l,
             "max": mx_l, "st": st_l, "prec": "fp32", "group": g, "lists": lists, **r}
 
 
@@ -607,13 +608,17 @@ def main():
     ap.add_argument("--group", type=int, default=0, help="mixed-walk targets per group (0 = autotuned)")
     ap.add_argument("--err-target", type=float, default=0.0,
                     help="fixed error threshold for the learned criterion (skip tuning)")
-    # measured at N = 4M, 1e-4 rms, groups of 128 (far path at order 4,
-    # K = 6): band 0.8 .. 1.25 -> 10.6 steps/s, 0.9 .. 1.12 -> 11.5,
-    # 0.95 .. 1.06 -> 12.4, 0.97 .. 1.03 -> 12.7 (the fixed-theta walk:
-    # 12.7); with the third-moment M2P the learned decisions away from theta
-    # cost more work than they save
+    # the band and the cost weight were swept together (below): with the
+    # cost term the band's width matters little; without it wider bands cost
+    # more (0.8 .. 1.25: walk 44.0 ms against 42.6 at 0.97 .. 1.03)
     ap.add_argument("--band", type=float, nargs=2, default=(0.97, 1.03), metavar=("LO", "HI"),
                     help="learned band: theta_lo = LO * theta, theta_hi = HI * theta (theta: the tuned fixed walk's)")
+    # measured at N = 4M, 1e-4 rms (walk ms; fixed theta 36.5): band 0.97 ..
+    # 1.03, w 0 / 0.25 / 0.5 / 0.75: 42.6 / 38.5 / 38.1 / 38.3; band 0.95 ..
+    # 1.05: 38.6 / 38.4 / 37.9 (w 0.25 / 0.5 / 0.75); 0.9 .. 1.12: 38.9 / 38.2
+    # / 38.2
+    ap.add_argument("--cost-weight", type=float, default=0.5,
+                    help="learned criterion: weight of the work a band cell's acceptance saves in its threshold")
     ap.add_argument("--config", choices=("c2", "c3", "c4", "c5", "build", "nonneutral"), default=None,
                     help="measure another configuration instead of the headline (one JSON line)")
     a = ap.parse_args()
@@ -660,7 +665,7 @@ def main():
                "ms_force": v["ms_force"], "ms_build": v["ms_build"]}
         if "ms_walk" in v:
             out["ms_walk_kernel"] = v["ms_walk"]
-        for k in ("drift", "stride", "err_target", "theta_lo", "group", "lists"):
+        for k in ("drift", "stride", "err_target", "cost_weight", "theta_lo", "group", "lists"):
             if k in v:
                 out[k] = v[k]
         if "st" in v:

@@ -82,7 +82,13 @@ typedef struct tb_config {
                        /* cells far from all of it share the segment's local      */
                        /* expansion (same acceptance, fewer visits and M2L pairs; */
                        /* no faster at 4M, sigma 600 um, theta_mixed 0.27)        */
-    int32_t pad_;
+    float cost_weight; /* TB_MAC_LEARNED: weight w of the work a band cell's       */
+                       /* acceptance saves.  The cell is accepted when the        */
+                       /* predicted log10 error is below log10(err_target) +      */
+                       /* w log10(S), S = (estimated cost of opening it - one      */
+                       /* M2P) / one M2P, from its children's records (leaves and */
+                       /* small cells: P2P per particle; larger cells: one M2P    */
+                       /* each); S <= 0 opens it.  0: error only                  */
 } tb_config;
 
 typedef struct tb_handle tb_handle;

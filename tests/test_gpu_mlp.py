@@ -197,3 +197,64 @@ def test_learned_mac_gains_in_non_neutral_plasma(theta):
     print("non-neutral theta %.2f: fixed rms %.3e work %.3g %.2f ms | learned rms %.3e work x%.3f time x%.3f"
           % (theta, ef, wf, tf, el, wl / wf, tl / tf))
     assert wl <= 0.9 * wf, (ef, wf, el, wl)
+
+
+# VALU instructions per interaction in the compiled walk kernels (gfx950):
+# M2P per (cell, target) pair and P2P per (leaf, target) pair from the flush
+# loops of k_walk32, a far cell's contribution to a group's local expansion
+# per (cell, group) pair from k_far_eval
+C_M2P, C_P2P, C_FAR = 126.5, 15.1, 1321.0
+
+
+def _cost(st):
+    return C_M2P * sum(st["m2p"]) + C_P2P * sum(st["p2p"]) + C_FAR * sum(st["far_pairs"])
+
+
+def test_cost_weight_spends_the_error_where_it_saves_work():
+    """The cost-aware threshold (cost_weight w): a band cell is accepted when
+    the predicted log10 error is below log10(err_target) + w log10 S, S the
+    work its acceptance saves in M2Ps (its children's P2P or M2P against its
+    own M2P).  Neutral plasma at the C2 configuration's size and width (2^19 +
+    2^19, sigma 400 um), theta 0.4 (near the benchmark's tuned 0.396), band
+    0.97 .. 1.03 theta, err_target bisected to the fixed walk's rms error
+    (within 1%) for w = 0 and w = 0.5; work in VALU instructions per force
+    (M2P 126.5, P2P 15.1 per pair, 1321 per far (cell, group) pair).  Without
+    the cost term the network accepts cells whose children are leaves (one
+    M2P for a few P2Ps) and does more work than the fixed walk; with it, less.
+    Measured: 1.053x and 0.959x the fixed walk's instructions (at N = 4M,
+    theta 0.396: 1.07x and 0.96x; at 2^17 + 2^17, sigma 300 um: 0.977x and
+    0.961x)."""
+    announce("cost_weight theta=0.4")
+    n, m, theta = 1 << 19, 2048, 0.4
+    qel, qion = ta.gen_plasma(33, n, n, 400.0)
+    tsp = np.r_[np.zeros(m, np.int32), np.ones(m, np.int32)]
+    tix = np.r_[np.arange(m), np.arange(m)].astype(np.int32)
+    with _eng(theta=theta, theta_mixed=1.03 * theta, theta_lo=0.97 * theta, walk="mixed", err_target=1e-7,
+              mac="learned", group=128) as eng:
+        eng.set_particles(qel, qion)
+        fd = eng.direct_sample(tsp, tix)
+        eng.build()
+        X, y = eng.mlp_collect(cap=1 << 21)
+        eng.mlp_init(0)
+        eng.mlp_train(X, y, steps=3000, batch=8192, lr=2e-3)
+        eng.configure(mac="fixed", theta_mixed=theta)
+        err_fix = _direct_err_fd(eng, fd, m)
+        cost_fix = _cost(eng.count_interactions())
+        cost = {}
+        for w in (0.0, 0.5):
+            eng.configure(mac="learned", theta_mixed=1.03 * theta, cost_weight=w)
+            lo, hi, best = -10.0, -1.0, None
+            for _ in range(12):
+                mid = 0.5 * (lo + hi)
+                eng.configure(err_target=10.0 ** mid)
+                e = _direct_err_fd(eng, fd, m)
+                if e <= 1.01 * err_fix:
+                    best, lo = (e, _cost(eng.count_interactions())), mid
+                else:
+                    hi = mid
+            assert best is not None, "w %g: the learned walk never reaches the fixed walk's error %g" % (w, err_fix)
+            cost[w] = best[1]
+            print("w %.1f: rms %.3e (fixed %.3e), VALU work x%.3f of the fixed walk's" % (w, best[0], err_fix,
+                                                                                      best[1] / cost_fix))
+    assert cost[0.5] <= 0.97 * cost[0.0], cost
+    assert cost[0.5] <= cost_fix, (cost, cost_fix)

@@ -342,6 +342,8 @@ int check_cfg(const tb_config *c)
         return fail(TB_ERR_ARG, "far_ratio must be 0 (default), negative (off) or >= 2 (got %g)", c->far_ratio);
     if (c->far_levels < 0 || c->far_levels > 2)
         return fail(TB_ERR_ARG, "far_levels must be 0 or 1 (default, group level) or 2 (got %d)", c->far_levels);
+    if (!(c->cost_weight >= 0.0f && c->cost_weight <= 4.0f))
+        return fail(TB_ERR_ARG, "cost_weight must be in [0, 4] (got %g)", (double)c->cost_weight);
     return TB_OK;
 }
 

@@ -218,11 +218,18 @@ __device__ inline void kelbg_gh(float u, float &g, float &h)
     }
 }
 
+// The domain descriptors are read-only while a walk runs: read through the
+// constant address space, so their (wave-uniform) fields come by scalar loads
+// and are not re-read from global memory after each of the walk's stores
+typedef const __attribute__((address_space(4))) Domain CDomain;
+__device__ inline CDomain *cdom(const Domain *p) { return (CDomain *)p; }
+
 struct WalkArgs {
     const Domain *dom;    // ndom descriptors, ordered by gbase
     int ndom, ngt;        // domains, groups in all domains
     float lo_scale;       // learned / collect: theta / theta_lo; cells beyond lo_scale * s/theta are always accepted
     float log_err;        // learned: accept a band cell when the predicted log10 error is below this
+    float cost_w;         // learned: ... plus cost_w * log10 of the work its acceptance saves (in M2Ps)
     float far_k2;         // square of the local-expansion threshold (distance / group radius); 0: off
     uint32_t *far_cells;  // far cells, 64 per chunk (0xffffffff: none)
     int32_t *far_link;    // per chunk: the group's previous chunk, or -1
@@ -275,6 +282,33 @@ __device__ inline void band_features(float4 bf, float kls, float d, float h, flo
     f[5] = k * __log2f(h / d + 9.5367431640625e-07f);
     f[6] = klemin + 2.0f * kld - bf.w;
     f[7] = like_net ? knact : -knact;
+}
+
+// P2P per pair against one M2P to the fourth moment, per target, in VALU
+// instructions (15 against ~120, from the compiled flush loops)
+constexpr float P2P_PER_M2P = 0.125f;
+
+// log10 of the work that accepting band cell nd saves, in units of one M2P
+// per target: opening it costs its children's evaluation (a leaf or a cell
+// of at most SMALL_OPEN particles: P2P per particle; a larger cell: one
+// M2P, as it is then half the size at the same distance) against the one
+// M2P of accepting it.  No saving (opening is no dearer) gives -30, so the
+// cell is opened.  A child range that is not strictly after the cell is
+// left to the caller's check (no children read).
+template <class Dom>
+__device__ inline float band_saving_log10(const Dom &D, uint32_t nd)
+{
+    const uint32_t inf = D.binfo[nd];
+    const uint32_t st = inf >> 3, nch = (inf & 7u) + 1u;
+    if (st <= nd || st + nch > (uint32_t)D.nn) return 0.0f;
+    float open = 0.0f;
+    for (uint32_t k = 0; k < nch; k++) {
+        const uint32_t ci = D.binfo[st + k];
+        const float cnt = (ci & TB_LEAF) ? 1.0f : D.bq2[st + k].z;
+        open += cnt <= (float)SMALL_OPEN ? cnt * P2P_PER_M2P : 1.0f;
+    }
+    const float save = open - 1.0f;
+    return save > 0.0f ? __log10f(save) : -30.0f;
 }
 
 template <int TPL>
@@ -634,10 +668,10 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         int lo = 0, hi = W.ndom - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (W.dom[mid].gbase <= g) lo = mid;
+            if (cdom(W.dom)[mid].gbase <= g) lo = mid;
             else hi = mid - 1;
         }
-        const Domain &D = W.dom[lo];
+        CDomain &D = cdom(W.dom)[lo];
         const int gl = g - D.gbase;
         const int tsp = gl < D.ng[0] ? 0 : 1;
         const float qt = tsp ? 1.0f : -1.0f;
@@ -722,13 +756,37 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         }
         L.stack[0] = D.root << 3;  // the domain's root: a range of one cell
         wave_sync();
-        while (sp > 0 || fc >= 0 || (MODE == MODE_LEARNED && nb > 0)) {
-            // learned: a pass either decides the queued band cells (once 64
-            // wait, or nothing else is left) or takes a traversal step; the
-            // flushes after it serve both, so a pass adds at most 64 entries
-            // to each list
-            const bool decide = MODE == MODE_LEARNED && (nb >= 64 || (sp == 0 && fc < 0 && nb > 0));
-            if (!decide) {
+        // the lists are evaluated once they reach their flush lengths (and
+        // the far queue handed on once 64 wait), after every pass
+        auto flushes = [&]() {
+            if (nm >= W.flush_m) {
+                if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
+                flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+                nm = 0;
+                wave_sync();
+            }
+            if (npl + npu >= W.flush_p) {
+                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, D.close2, A, nx);
+                npl = npu = 0;
+                wave_sync();
+            }
+            if (nf >= 64) {
+                push_far(L, 64, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
+                // the rest of the queue (at most 63 entries) moves down
+                const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
+                wave_sync();
+                if (lane + 64 < nf) L.farq[lane] = rest;
+                nf -= 64;
+                wave_sync();
+            }
+        };
+        // learned: band cells wait in a queue.  Once 64 wait, or nothing
+        // else is left, one pass decides up to 64 of them between the
+        // traversal steps; the traversal loop itself is the fixed walk's.
+        // Each pass (a traversal step or a decision) adds at most 64 entries
+        // to each list before the flushes.
+        for (;;) {
+            while (sp > 0 || fc >= 0) {
                 bool have;
                 uint32_t node;
                 if (sp > 0) {
@@ -775,24 +833,28 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         d2 = box_d2(cx, cy, cz, hx, hy, hz);
                         const float rc = (float)m.w;  // s / theta_mixed
                         const bool far_hi = d2 > rc * rc * 1.000004f;
-                        if (D.bq2[node].z <= (float)SMALL_OPEN) {
-                            open = true;  // its particles' P2P is exact and cheaper than its M2P
-                        } else if (MODE == MODE_FIXED) {
-                            acc = far_hi;
-                            open = !far_hi;
+                        const float cnt = D.bq2[node].z;
+                        // a cell of at most SMALL_OPEN particles is opened: its
+                        // particles' P2P is exact and cheaper than its M2P
+                        const bool small = cnt <= (float)SMALL_OPEN;
+                        if (MODE == MODE_FIXED) {
+                            acc = far_hi && !small;
+                            open = !acc;
                         } else {
-                            const float rlo = rc * W.lo_scale;  // s / theta_lo
-                            if (d2 > rlo * rlo * 1.000004f) acc = true;
-                            else if (!far_hi) open = true;
-                            else if (MODE == MODE_COLLECT ||
-                                     (D.bq2[node].z <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2))
-                                band = true;
-                            // larger cells are never labelled (the label sums the
+                            // as selects, not branches: accepted beyond s/theta_lo,
+                            // opened within s/theta_mixed, the band between.
+                            // Larger cells are never labelled (the label sums the
                             // cell's particles), and far cells are cheap to accept
                             // (the group's local expansion): in the learned walk
                             // the band's geometric-mean radius decides both
-                            else if (d2 > rlo * rc * 1.000004f) acc = true;
-                            else open = true;
+                            const float rlo = rc * W.lo_scale;  // s / theta_lo
+                            const bool far_lo = d2 > rlo * rlo * 1.000004f;
+                            const bool far_mid = d2 > rlo * rc * 1.000004f;
+                            const bool net = MODE == MODE_COLLECT ||
+                                             (cnt <= (float)BAND_MAXCNT && cx * cx + cy * cy + cz * cz <= far_r2);
+                            band = !small && !far_lo && far_hi && net;
+                            acc = !small && (far_lo || (far_hi && !net && far_mid));
+                            open = !acc && !band;
                         }
                     }
                 }
@@ -870,7 +932,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 if (sp + npush > STACK) {
                     atomicOr(&W.ctr[1], 1);
                     sp = 0;
-                    nm = npl = npu = 0;
+                    fc = -1;
+                    nm = npl = npu = nb = 0;
                     break;
                 }
                 if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
@@ -884,7 +947,11 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         wave_sync();
                     }
                 }
-            } else {
+                flushes();
+                if (MODE == MODE_LEARNED && nb >= 64) break;
+            }
+            if (MODE != MODE_LEARNED || nb == 0) break;
+            {
                 // Decide up to 64 queued cells with one batched MLP pass on
                 // the matrix cores (tb_mlp_eval32_mfma, two tiles of 32).
                 // Entry e's features are recomputed from its records.
@@ -918,7 +985,12 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 // previous force (emin = 0) band cells are opened
                 const bool mine = lane < ne;
                 const uint32_t nd = mine ? L.bandq[lane] : 0u;
-                bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < W.log_err;
+                // cost-aware threshold: the error budget goes where accepting
+                // saves the most work (log_err + cost_w log10 S, S the
+                // saving in M2Ps, see band_saving)
+                float thr = W.log_err;
+                if (W.cost_w > 0.0f && mine) thr += W.cost_w * band_saving_log10(D, nd);
+                bool bacc = mine && emin > 0.0f && (lane < 32 ? pr[0] : pr[1]) < thr;
                 bool bopen = mine && !bacc;
                 // accepted band cells: far ones to the far queue, the rest to M2P
                 float bx = 0.0f, by = 0.0f, bz = 0.0f;
@@ -975,26 +1047,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 nb -= ne;
                 wave_sync();
             }
-            if (nm >= W.flush_m) {
-                if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
-                flush_m2p<TPL>(L, nm, rx, ry, rz, A);
-                nm = 0;
-                wave_sync();
-            }
-            if (npl + npu >= W.flush_p) {
-                flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, D.close2, A, nx);
-                npl = npu = 0;
-                wave_sync();
-            }
-            if (nf >= 64) {
-                push_far(L, 64, W.far_cells, W.far_link, W.far_cap, W.ctr + CTR_FAR, F);
-                // the rest of the queue (at most 63 entries) moves down
-                const uint32_t rest = lane + 64 < nf ? L.farq[lane + 64] : 0u;
-                wave_sync();
-                if (lane + 64 < nf) L.farq[lane] = rest;
-                nf -= 64;
-                wave_sync();
-            }
+            flushes();
         }
         if (nm > 0) {
             if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
@@ -1722,6 +1775,7 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     const double th_lo = cfg.theta_lo > 0.0 ? cfg.theta_lo : th_hi;
     W.lo_scale = (float)(th_hi / th_lo);
     W.log_err = (float)log10(cfg.err_target > 0.0 ? cfg.err_target : 1e-5);
+    W.cost_w = cfg.cost_weight;
     const double kfar = cfg.far_ratio == 0.0 ? FAR_RATIO_DEFAULT : cfg.far_ratio;
     W.far_k2 = kfar > 0.0 ? (float)(kfar * kfar) : 0.0f;
     W.mlpk = M.mlp.Ppk;

@@ -60,7 +60,7 @@ class Config(ctypes.Structure):
         ("mem_cap", ctypes.c_int64),
         ("far_ratio", ctypes.c_double),
         ("far_levels", ctypes.c_int32),
-        ("pad_", ctypes.c_int32),
+        ("cost_weight", ctypes.c_float),
     ]
 
 
@@ -176,7 +176,8 @@ class Engine:
     """One resident domain on one GPU (a tb_handle)."""
 
     def __init__(self, rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed",
-                 theta_lo=0.0, err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0, far_levels=0):
+                 theta_lo=0.0, err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0, far_levels=0,
+                 cost_weight=0.0):
         self.lib = load_library()
         self.cfg = Config()
         self.cfg.xm[0], self.cfg.xm[1] = xm
@@ -192,6 +193,7 @@ class Engine:
         self.cfg.mem_cap = mem_cap
         self.cfg.far_ratio = far_ratio
         self.cfg.far_levels = far_levels
+        self.cfg.cost_weight = cost_weight
         h = ctypes.c_void_p()
         _check(self.lib.tb_create(ctypes.byref(h), ctypes.byref(self.cfg)))
         self.h = h
@@ -390,13 +392,14 @@ class Engine:
 
 
 def make_config(rs, theta=0.5, xm=(XM_E, XM_I), walk="ref64", theta_mixed=0.0, mac="fixed", theta_lo=0.0,
-                err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0, far_levels=0):
+                err_target=0.0, group=0, device=0, mem_cap=0, far_ratio=0.0, far_levels=0, cost_weight=0.0):
     c = Config()
     c.xm[0], c.xm[1] = xm
     c.rs, c.theta, c.theta_mixed, c.theta_lo, c.err_target = rs, theta, theta_mixed, theta_lo, err_target
     c.walk, c.mac, c.group, c.device, c.mem_cap = WALK[walk], MAC[mac], group, device, mem_cap
     c.far_ratio = far_ratio
     c.far_levels = far_levels
+    c.cost_weight = cost_weight
     return c
 
 

@@ -12,7 +12,8 @@ MODULE TREEMI_CAPI
      INTEGER(C_INT32_T) :: walk, mac, group, device
      INTEGER(C_INT64_T) :: mem_cap
      REAL(C_DOUBLE) :: far_ratio
-     INTEGER(C_INT32_T) :: far_levels, pad
+     INTEGER(C_INT32_T) :: far_levels
+     REAL(C_FLOAT) :: cost_weight
   END TYPE TB_CONFIG
 
   INTERFACE

@@ -39,7 +39,7 @@ CONTAINS
     cfg%mem_cap = 0
     cfg%far_ratio = 0.0_C_DOUBLE
     cfg%far_levels = 0
-    cfg%pad = 0
+    cfg%cost_weight = 0.0_C_FLOAT
     CALL tb_check(tb_create(hnd, cfg), "Engine_Init")
   END SUBROUTINE Engine_Init
 
