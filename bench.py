@@ -249,23 +249,40 @@ def run_plain(qel, qion, rs, tsp, tix, fdir, a):
 
 
 def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
-    """The fixed-theta mixed walk, theta bisected to the error target."""
+    """The fixed-theta mixed walk, theta bisected to the error target.  With
+    the group size left to the autotuner, theta is bisected at each group
+    size (the theta that meets the target depends on the groups' extent) and
+    the size whose walk is fastest at the target is kept, with the list
+    lengths the autotuner picks there."""
     eng = ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=0.5, group=a.group)
     eng.set_particles(qel, qion)
-    if a.theta32 > 0:
-        eng.configure(theta_mixed=a.theta32)
-        th = a.theta32
-        rms, mx = err_full(eng, tsp, tix, fdir)
+
+    def tune():
+        if a.theta32 > 0:
+            eng.configure(theta_mixed=a.theta32)
+            return (a.theta32,) + tuple(err_full(eng, tsp, tix, fdir))
+        return bisect(lambda v: eng.configure(theta_mixed=v), lambda: err_full(eng, tsp, tix, fdir), 0.05, 1.0, 10)
+
+    if a.group == 0:
+        log("tuning the mixed walk at each group size")
+        best = None
+        for k, gs in enumerate((64, 128, 256)):
+            eng.configure(group=gs)
+            th, rms, mx = tune()
+            eng.build()
+            _, ms = eng.autotune()  # per group size, at this theta and the best list lengths
+            log("  groups of %d: theta %.4f rms %.2e, walk %.2f ms" % (gs, th, rms, ms[k]))
+            if best is None or ms[k] < best[0]:
+                best = (ms[k], gs, th, rms, mx)
+        _, g, th, rms, mx = best
+        eng.configure(group=g, theta_mixed=th)
+        eng.build()
+        eng.autotune()  # the list lengths for this group size and theta
+        log("  chosen: groups of %d, lists %s" % (g, eng.walk_lists()))
     else:
         log("tuning the mixed walk")
-        th, rms, mx = bisect(lambda v: eng.configure(theta_mixed=v), lambda: err_full(eng, tsp, tix, fdir),
-                             0.05, 1.0, 10)
-    eng.build()
-    g = a.group
-    if a.group == 0:
-        g, ms = eng.autotune()
-        log("  autotuned group size %d, lists %s (ms per walk for 64/128/256: %s)"
-            % (g, eng.walk_lists(), np.round(ms, 2)))
+        g = a.group
+        th, rms, mx = tune()
     lists = eng.walk_lists()
     eng.build()
     eng.force()
@@ -278,13 +295,15 @@ def run_mixed(qel, qion, rs, dt, tsp, tix, fdir, a):
             "lists": lists, **r}
 
 
-def train_mlp(a, th_lo, th_hi):
+def train_mlp(a, th_lo, th_hi, group=0):
     """Labels from the method itself on a 2^20-particle plasma (the band the
-    learned walk will decide), then GPU training."""
+    learned walk will decide), then GPU training.  The labels' target groups
+    have the size the learned walk will use (`group`, 0: autotuned): two of
+    the features (group extent, targets in the group) depend on it."""
     n = 1 << 19
     qel, qion = ta.gen_plasma(a.seed + 1, n, n, 400.0, te=a.te, ti=1e-3)
     rs = ta.kelbg_length(a.te)
-    with ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=th_hi, theta_lo=th_lo) as eng:
+    with ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=th_hi, theta_lo=th_lo, group=group) as eng:
         eng.set_particles(qel, qion)
         eng.build()
         X, y = eng.mlp_collect(cap=1 << 20)
@@ -297,7 +316,11 @@ def train_mlp(a, th_lo, th_hi):
 
 
 def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
-    """The learned criterion in the band theta_lo .. theta_hi, its error threshold bisected to the target."""
+    """The learned criterion in the band theta_lo .. theta_hi, its error
+    threshold bisected to the target.  The walk runs with the target groups
+    the network was trained on (a.group: two features depend on the group's
+    size); list lengths as autotuned for this size."""
+    assert a.group in (64, 128, 256), "the learned walk runs at the group size its labels were collected with"
     eng = ta.Engine(rs=rs, theta=0.5, walk="mixed", theta_mixed=th_hi, theta_lo=th_lo, mac="learned",
                     err_target=1e-9, group=a.group, cost_weight=a.cost_weight)
     eng.set_particles(qel, qion)
@@ -305,15 +328,6 @@ def run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi):
     eng.build()
     eng.force()  # the fixed-theta fallback: gives the first previous forces
     g = a.group
-    if a.group == 0:
-        # the learned walk's own group size (its band decisions change the
-        # register budget), chosen before the error threshold is tuned
-        eng.configure(err_target=a.err_target if a.err_target > 0 else 1e-6)
-        eng.build()
-        eng.force()
-        g, ms = eng.autotune()
-        log("  learned walk: autotuned group size %d, lists %s (ms per walk for 64/128/256: %s)"
-            % (g, eng.walk_lists(), np.round(ms, 2)))
     lists = eng.walk_lists()
     if a.err_target > 0:
         et = a.err_target
@@ -482,16 +496,18 @@ def run_c5(a):
     with ta.Engine(rs=rs, theta=0.5) as eng:
         eng.set_particles(qel, qion)
         fdir = eng.direct_sample(tsp, tix)
-    with ta.Engine(rs=rs, theta=0.5, theta_mixed=0.5, walk="mixed") as eng:
+    g = a.group if a.group else 128  # both walks on groups of one size (the network's features depend on it)
+    with ta.Engine(rs=rs, theta=0.5, theta_mixed=0.5, walk="mixed", group=g) as eng:
         eng.set_particles(qel, qion)
         rms_f, mx_f = err_full(eng, tsp, tix, fdir)
         st_f = eng.count_interactions()
         eng.set_particles(qel, qion)
         sps_f, stt_f = _steps_per_s(eng, dt, a.steps)
     th_lo, th_hi = a.band[0] * 0.5, min(1.0, a.band[1] * 0.5)
-    params, info = train_mlp(a, th_lo, th_hi)
+    params, info = train_mlp(a, th_lo, th_hi, group=g)
     TARGET = rms_f
-    r = run_learned(qel, qion, rs, dt, tsp, tix, fdir, argparse.Namespace(**{**vars(a), "err_target": 0.0}),
+    r = run_learned(qel, qion, rs, dt, tsp, tix, fdir, argparse.Namespace(**{**vars(a), "err_target": 0.0,
+                                                                            "group": g}),
                     params, th_lo, th_hi)
     out = {"config": "C5: 2^23 e- + 2^23 ions, sigma 1 mm, fixed theta 0.5 vs learned", "steps": a.steps,
            "fixed": {"steps_per_s": sps_f, "rms": rms_f, "max": mx_f, "ms_walk": stt_f["ms_walk"],
@@ -520,7 +536,8 @@ def run_nonneutral(a):
         fdir = eng.direct_sample(tsp, tix)
     q2e, q2i = ta.gen_plasma(7, 1 << 19, 1 << 19, 400.0)
     q2e["xx"] *= 0.7
-    with ta.Engine(rs=rs, theta=theta, walk="mixed", theta_mixed=bhi * theta, theta_lo=blo * theta) as eng:
+    with ta.Engine(rs=rs, theta=theta, walk="mixed", theta_mixed=bhi * theta, theta_lo=blo * theta,
+                   group=128) as eng:
         eng.set_particles(q2e, q2i)
         eng.build()
         X, y = eng.mlp_collect(cap=1 << 21)
@@ -645,9 +662,11 @@ def main():
     learned, mlp_info = None, None
     if not a.no_learned:
         th_lo, th_hi = a.band[0] * fixed["theta"], min(1.0, a.band[1] * fixed["theta"])
-        log("collecting labels and training the opening-criterion MLP")
-        params, mlp_info = train_mlp(a, th_lo, th_hi)
-        learned = run_learned(qel, qion, rs, dt, tsp, tix, fdir, a, params, th_lo, th_hi)
+        # the network is trained on, and decides for, the fixed walk's autotuned groups
+        al = argparse.Namespace(**{**vars(a), "group": fixed["group"]})
+        log("collecting labels and training the opening-criterion MLP (groups of %d)" % al.group)
+        params, mlp_info = train_mlp(al, th_lo, th_hi, group=al.group)
+        learned = run_learned(qel, qion, rs, dt, tsp, tix, fdir, al, params, th_lo, th_hi)
     head = learned if learned is not None else fixed
 
     flops = walk_flops32(head["st"])

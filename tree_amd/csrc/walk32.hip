@@ -1357,7 +1357,7 @@ struct FarArgs {
     const float *sloc;
 };
 
-__global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
+__global__ __launch_bounds__(BT, 2) void k_far_eval(FarArgs A)
 {
     const int lane = lane_id();
     const int g = uni((int)(blockIdx.x * WPB + (threadIdx.x >> 6)));
@@ -1388,8 +1388,14 @@ __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
     float Lp[NLOC];
 #pragma unroll
     for (int k = 0; k < NLOC; k++) Lp[k] = 0.0f;
+    // the chunk list is walked one chunk ahead: the next chunk's link and
+    // cell ids load while this chunk's cells are expanded, so only the
+    // records' loads wait on a list entry
+    uint32_t ndn = c >= 0 ? A.far_cells[(size_t)c * 64 + lane] : 0xffffffffu;
     while (c >= 0) {
-        const uint32_t nd = A.far_cells[(size_t)c * 64 + lane];
+        const uint32_t nd = ndn;
+        c = uni(A.far_link[c]);
+        if (c >= 0) ndn = A.far_cells[(size_t)c * 64 + lane];
         float rx = 1.0f, ry = 0.0f, rz = 0.0f;
         float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0, q4 = q0, q5 = q0, q6 = q0;
         if (nd != 0xffffffffu) {
@@ -1406,7 +1412,6 @@ __global__ __launch_bounds__(BT) void k_far_eval(FarArgs A)
             q6 = D.bq6[nd];
         }
         m2l_acc(rx, ry, rz, q0, q1, q2, q3, q4, q5, q6, Lp);
-        c = uni(A.far_link[c]);
     }
 #pragma unroll
     for (int k = 0; k < NLOC; k++)
