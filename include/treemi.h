@@ -74,7 +74,7 @@ typedef struct tb_config {
     double far_ratio;  /* TB_WALK_MIXED: an accepted cell farther than far_ratio    */
                        /* times the target group's radius from the group's centre */
                        /* goes through the group's local expansion instead of     */
-                       /* being evaluated at every target (0: default 4, < 0:   */
+                       /* being evaluated at every target (0: default 3.5, < 0: */
                        /* off, else >= 2)                                        */
     int32_t far_levels;/* TB_WALK_MIXED with the far path: 0 or 1 (default) the   */
                        /* group level only; 2: two levels, a segment of up to 8   */

@@ -172,11 +172,11 @@ def test_c2_sampled_walk_timing_matches_full_walk():
 @pytest.mark.parametrize("te", [1.0, 10.0, 100.0])
 def test_c3_mixed_walk_theta_sweep(te):
     """C3 at full size (2^21 + 2^21, sigma 600 um): the mixed walk at theta
-    0.3 / 0.5 / 0.7 against the FP64 direct sum, rms below 3e-5 / 3.8e-4 /
-    3.2e-3 (measured 2.38e-5 / 2.94e-4 / 2.49e-3, alike for every T_e: the
-    Kelbg length only changes close unlike pairs), max below 0.1 (measured
-    0.067 at theta 0.7: one target of weak net force); the error rises with
-    theta.  The target groups follow the species' key order over the joint
+    0.3 / 0.5 / 0.7 against the FP64 direct sum, rms below 3.8e-5 / 3.9e-4 /
+    3.2e-3 (measured 3.05e-5 / 3.13e-4 / 2.49e-3 with far cells beyond 3.5
+    group radii through the local expansion, alike for every T_e: the Kelbg
+    length only changes close unlike pairs), max below 0.1 (measured 0.067 at
+    theta 0.7: one target of weak net force); the error rises with theta.  The target groups follow the species' key order over the joint
     box."""
     announce("c3_sweep te=%g" % te)
     n = 1 << 21
@@ -187,15 +187,18 @@ def test_c3_mixed_walk_theta_sweep(te):
     errs = []
     with ta.Engine(rs=rs, theta=0.5, theta_mixed=0.3, walk="mixed") as eng:
         eng.set_particles(qel, qion)
-        for th, bound in ((0.3, 3e-5), (0.5, 3.8e-4), (0.7, 3.2e-3)):
+        rows = []
+        for th, bound in ((0.3, 3.8e-5), (0.5, 3.9e-4), (0.7, 3.2e-3)):
             eng.configure(theta_mixed=th)
             eng.build()
             eng.force()
             e, i = eng.get_particles()
             rms, mx = rel_err(pick(e, i, tsp, tix), fd)
             print("C3 T_e %g theta %.1f: rms %.3e max %.3e" % (te, th, rms, mx))
-            assert np.isfinite(rms) and rms < bound and mx < 0.1, (th, rms, mx)
+            rows.append((th, bound, rms, mx))
             errs.append(rms)
+    for th, bound, rms, mx in rows:
+        assert np.isfinite(rms) and rms < bound and mx < 0.1, rows
     assert errs[0] < errs[1] < errs[2], errs
 
 

@@ -99,8 +99,12 @@ constexpr int NLOC = FAR_ORDER == 3   ? TB_LOCAL_N3
 // With the cells' fourth moment (in the M2P and in the far cells' M2L) theta
 // rises to ~0.39 and the balance moves: K = 3, 38.8 ms; K = 4, 37.5 ms; K = 5,
 // 38.6 ms (cells of up to 6 particles opened); K = 2.5 reaches 1e-4 at no theta
-// below 0.5.  Without the fourth moment in the M2L, theta stayed at 0.28.
-constexpr double FAR_RATIO_DEFAULT = 4.0;
+// below 0.5.  Without the fourth moment in the M2L, theta stayed at 0.28.  With
+// k_far_eval at two waves per SIMD (was one) the far path costs less and the
+// balance moves again (groups of 128): K = 3, 34.6 ms; 3.25, 33.5; 3.5, 32.7-33.2;
+// 3.75, 34.5; 4, 34.1; 5, 35.5 (groups of 64: 39.6 / 34.8 / 35.0 / 36.6 at K = 3 /
+// 3.5 / 4 / 5).
+constexpr double FAR_RATIO_DEFAULT = 3.5;
 
 struct WaveLDS {
     uint32_t stack[STACK];
