@@ -48,6 +48,9 @@ W_VISIT, W_M2P, W_P2P_C, W_P2P_K = 11, 144, 20, 28
 # for a lane's 2 targets; Coulomb P2P 18, Kelbg P2P 22); MAC tests are per
 # cell and group and are not counted
 W32_M2P, W32_P2P_C, W32_P2P_K = 202, 18, 22
+# cells far from the group for their size take the M2P to the third moment:
+# 122 flops in 77 VALU instructions per (cell, target)
+W32_M2P3 = 122
 # far cells: flops per (cell, target group) pair of the order-6 local
 # expansion with the cell's third and fourth moments, terms to total order 7
 # (FMA = 2), counted from the compiled loop of k_far_eval (m2l_acc_p6hxt7;
@@ -124,6 +127,8 @@ def walk_flops32(st):
     for p in range(4):
         wp = W32_P2P_C if p in (0, 3) else W32_P2P_K
         fl += W32_M2P * st["m2p"][p] + wp * st["p2p"][p]
+    # of the M2P pairs, those to the third moment only
+    fl -= (W32_M2P - W32_M2P3) * sum(st.get("m2p3", [0, 0]))
     # far cells: (cell, group) pairs, and (cell, segment) pairs of the two-level far field
     return fl + W32_M2L_PAIR * (sum(st.get("far_pairs", [0, 0])) + sum(st.get("seg_pairs", [0, 0])))
 
@@ -690,7 +695,8 @@ def main():
         if "st" in v:
             out["interactions_per_s"] = interactions(v["st"]) / v["s"]
             st = v["st"]
-            out["counts_per_force"] = {"m2p": int(sum(st["m2p"])), "p2p": int(sum(st["p2p"])),
+            out["counts_per_force"] = {"m2p": int(sum(st["m2p"])), "m2p_third_order": int(sum(st.get("m2p3", [0]))),
+                                       "p2p": int(sum(st["p2p"])),
                                        "far_m2l": int(sum(st.get("m2l", [0]))),
                                        "far_pairs": int(sum(st.get("far_pairs", [0])))}
         return out

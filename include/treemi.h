@@ -189,6 +189,10 @@ typedef struct tb_stats {
     int64_t seg_pairs[2];    /* two-level far field: (far cell, segment) pairs,  */
                              /* by target species (their (cell, target) pairs   */
                              /* are in m2l)                                     */
+    int64_t m2p3[2];         /* mixed walk: of the m2p, the (cell, target) pairs */
+                             /* evaluated to the third moment only (cells far   */
+                             /* from the group for their size), by target       */
+                             /* species                                         */
 } tb_stats;
 /* counting pass: re-runs the last force with interaction counters */
 int tb_count_interactions(tb_handle *h, tb_stats *s);

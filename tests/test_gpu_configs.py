@@ -173,8 +173,9 @@ def test_c2_sampled_walk_timing_matches_full_walk():
 def test_c3_mixed_walk_theta_sweep(te):
     """C3 at full size (2^21 + 2^21, sigma 600 um): the mixed walk at theta
     0.3 / 0.5 / 0.7 against the FP64 direct sum, rms below 3.8e-5 / 3.9e-4 /
-    3.2e-3 (measured 3.05e-5 / 3.13e-4 / 2.49e-3 with far cells beyond 3.5
-    group radii through the local expansion, alike for every T_e: the Kelbg
+    3.2e-3 (measured 3.22e-5 / 3.25e-4 / 2.51e-3 with far cells beyond 3.5
+    group radii through the local expansion and cells beyond 1.5 opening radii
+    to the third moment, alike for every T_e: the Kelbg
     length only changes close unlike pairs), max below 0.1 (measured 0.067 at
     theta 0.7: one target of weak net force); the error rises with theta.  The target groups follow the species' key order over the joint
     box."""

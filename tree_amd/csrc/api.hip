@@ -1211,6 +1211,9 @@ int tb_count_interactions(tb_handle *h, tb_stats *st)
     // two-level far field: (far cell, segment) pairs by target species
     h->st.seg_pairs[0] = c[16];
     h->st.seg_pairs[1] = c[17];
+    // M2P pairs to the third moment only, by target species
+    h->st.m2p3[0] = c[18];
+    h->st.m2p3[1] = c[19];
     *st = h->st;
     return TB_OK;
 }

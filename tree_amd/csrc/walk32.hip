@@ -76,6 +76,21 @@ constexpr int BAND_MAXCNT = 4096;  // largest band cell that is labelled / decid
 // that the far path changes how accepted cells are evaluated, never which
 // are accepted.
 constexpr int SMALL_OPEN = 5;
+// An accepted cell farther than OCT_RADIUS of its opening radii (s/theta)
+// from the group's box takes the M2P to the third moment: there its fourth
+// moment's term, falling as (s/d)^5, is below the truncation error of the
+// cells at the opening radius.  Measured at N = 4M, groups of 128, theta
+// bisected to 1e-4 for each: with the fourth moment dropped beyond 1.26 /
+// 1.5 / 1.75 / 2.0 radii, theta 0.369 / 0.393 / 0.395 / 0.395 (0.395
+// without), 84 / 46 / 26 / 9% of the M2P pairs to the third moment.
+constexpr float OCT_RADIUS = 1.5f;
+// In the learned walk the children of a band cell the network opened take the
+// full M2P whatever their distance, so that opening a band cell never costs
+// accuracy (with the third-moment M2P for its children it could: one
+// fourth-moment M2P of the cell against eight third-moment ones).  The flag
+// rides in bit 31 of the cell's child range on the stack; child indices are
+// below 2^28 (bit 31 of a twig's record is clear: it marks leaves).
+constexpr uint32_t FULL4 = 0x80000000u;
 
 const char *g_err = "";
 
@@ -326,16 +341,21 @@ struct Acc {
 // 5/2 O'rrr/r^7 + 35/8 H'rrrr/r^9, E = -grad phi (the O' part: -15/2 v/r^7 +
 // 35/2 (r.v) r/r^9, v_i = O'_ijk r_j r_k; the H' part: -35/2 w/r^9 + 315/8
 // (r.w) r/r^11, w_i = H'_ijkl r_j r_k r_l)
-template <int TPL>
-__device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TPL], const float (&ry)[TPL],
-                                 const float (&rz)[TPL], Acc<TPL> &A)
+// One M2P loop over list entries [j0, j1): to the fourth moment (HEX) or to
+// the third (cells far enough from the group that their H' term is below the
+// walk's error, see OCT_RADIUS; entries without m5 / m6)
+template <int TPL, bool HEX>
+__device__ inline void m2p_loop(const WaveLDS &L, int j0, int j1, const float (&rx)[TPL], const float (&ry)[TPL],
+                                const float (&rz)[TPL], float (&ax)[TPL], float (&ay)[TPL], float (&az)[TPL],
+                                float (&ap)[TPL])
 {
-    float ax[TPL], ay[TPL], az[TPL], ap[TPL];
-#pragma unroll
-    for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
-    for (int j = 0; j < cnt; j++) {
-        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j], o1 = L.m3[j], o2 = L.m4[j], h1 = L.m5[j],
-                     h2 = L.m6[j];
+    for (int j = j0; j < j1; j++) {
+        const float4 a = L.m0[j], b = L.m1[j], c = L.m2[j], o1 = L.m3[j], o2 = L.m4[j];
+        float4 h1 = make_float4(0.f, 0.f, 0.f, 0.f), h2 = h1;
+        if constexpr (HEX) {
+            h1 = L.m5[j];
+            h2 = L.m6[j];
+        }
         const float qxx = b.w, qyy = c.x, qzz = -b.w - c.x, qxy = c.y, qyz = c.z, qzx = c.w;
         // O' from its 7 independent components (tr = 0: O'_xzz = -(O'_xxx +
         // O'_xyy), O'_yzz = -(O'_xxy + O'_yyy), O'_zzz = -(O'_xxz + O'_yyz)),
@@ -354,7 +374,7 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
             // raw v_rsq_f32: an accepted cell is at least s/theta away, so r2
             // is a normal number and the denormal-safe rsqrtf is not needed
             const float ri = __builtin_amdgcn_rsqf(r2);
-            const float u = ri * ri, ri3 = ri * u, ri5 = ri3 * u;
+            const float u = ri * ri, ri3 = ri * u;
             const float dp = b.x * dx + b.y * dy + b.z * dz;
             const float qx = qxx * dx + qxy * dy + qzx * dz;
             const float qy = qxy * dx + qyy * dy + qyz * dz;
@@ -366,25 +386,51 @@ __device__ inline void flush_m2p(const WaveLDS &L, int cnt, const float (&rx)[TP
             const float vy = oxxy * xmz + oyyy * ymz + oxyy * xy2 + oxyz * xz2 + oyyz * yz2;
             const float vz = oxxz * xmz + oyyz * ymz + oxyz * xy2 - sxy * xz2 - syz * yz2;
             const float t3 = u * (dx * vx + dy * vy + dz * vz);
-            const float zz2 = zz + zz;
-            const float A1 = dx * fmaf(-2.0f, zz, xmz), A2 = 3.0f * (dy * xmz), A3 = dz * fmaf(3.0f, xmz, zz2);
-            const float A4 = 3.0f * (dx * ymz), A5 = 3.0f * (xy2 * dz), A6 = dy * fmaf(-2.0f, zz, ymz);
-            const float A7 = dz * fmaf(3.0f, ymz, zz2);
-            const float wx = hxxxx * A1 + hxxxy * A2 + hxxxz * A3 + hxxyy * A4 + hxxyz * A5 + hxyyy * A6 + hxyyz * A7;
-            const float wy = hxxxy * A1 + hxxyy * A2 + hxxyz * A3 + hxyyy * A4 + hxyyz * A5 + hyyyy * A6 + hyyyz * A7;
-            const float wz = hxxxz * A1 + hxxyz * A2 + hxyyz * A4 + hyyyz * A6 - s1 * A3 - s2 * A5 - s3 * A7;
-            const float t4 = u * u * (dx * wx + dy * wy + dz * wz);
-            // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3 + 39.375 t4))) - ri3 (D + u (Q.dx + u (7.5 v + 17.5 u w)))
-            // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3 + 4.375 t4)))
-            const float a5 = u * fmaf(39.375f, t4, fmaf(17.5f, t3, 2.5f * xqx));
-            const float w = ri3 * fmaf(u, fmaf(3.0f, dp, a5), a.w);
-            const float u75 = 7.5f * u, u175 = 17.5f * u * u;
-            ax[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wx, fmaf(u75, vx, qx)), b.x), fmaf(dx, w, ax[t]));
-            ay[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wy, fmaf(u75, vy, qy)), b.y), fmaf(dy, w, ay[t]));
-            az[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wz, fmaf(u75, vz, qz)), b.z), fmaf(dz, w, az[t]));
-            ap[t] = fmaf(ri, fmaf(u, dp + u * fmaf(4.375f, t4, fmaf(2.5f, t3, 0.5f * xqx)), a.w), ap[t]);
+            const float u75 = 7.5f * u;
+            if constexpr (HEX) {
+                const float zz2 = zz + zz;
+                const float A1 = dx * fmaf(-2.0f, zz, xmz), A2 = 3.0f * (dy * xmz), A3 = dz * fmaf(3.0f, xmz, zz2);
+                const float A4 = 3.0f * (dx * ymz), A5 = 3.0f * (xy2 * dz), A6 = dy * fmaf(-2.0f, zz, ymz);
+                const float A7 = dz * fmaf(3.0f, ymz, zz2);
+                const float wx = hxxxx * A1 + hxxxy * A2 + hxxxz * A3 + hxxyy * A4 + hxxyz * A5 + hxyyy * A6 + hxyyz * A7;
+                const float wy = hxxxy * A1 + hxxyy * A2 + hxxyz * A3 + hxyyy * A4 + hxyyz * A5 + hyyyy * A6 + hyyyz * A7;
+                const float wz = hxxxz * A1 + hxxyz * A2 + hxyyz * A4 + hyyyz * A6 - s1 * A3 - s2 * A5 - s3 * A7;
+                const float t4 = u * u * (dx * wx + dy * wy + dz * wz);
+                // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3 + 39.375 t4))) - ri3 (D + u (Q.dx + u (7.5 v + 17.5 u w)))
+                // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3 + 4.375 t4)))
+                const float a5 = u * fmaf(39.375f, t4, fmaf(17.5f, t3, 2.5f * xqx));
+                const float w = ri3 * fmaf(u, fmaf(3.0f, dp, a5), a.w);
+                const float u175 = 17.5f * u * u;
+                ax[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wx, fmaf(u75, vx, qx)), b.x), fmaf(dx, w, ax[t]));
+                ay[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wy, fmaf(u75, vy, qy)), b.y), fmaf(dy, w, ay[t]));
+                az[t] = fmaf(-ri3, fmaf(u, fmaf(u175, wz, fmaf(u75, vz, qz)), b.z), fmaf(dz, w, az[t]));
+                ap[t] = fmaf(ri, fmaf(u, dp + u * fmaf(4.375f, t4, fmaf(2.5f, t3, 0.5f * xqx)), a.w), ap[t]);
+            } else {
+                // E = dx ri3 (q + u (3 dp + u (2.5 xqx + 17.5 t3))) - ri3 (D + u (Q.dx + 7.5 u v))
+                // phi = ri (q + u (dp + u (0.5 xqx + 2.5 t3)))
+                const float a5 = u * fmaf(17.5f, t3, 2.5f * xqx);
+                const float w = ri3 * fmaf(u, fmaf(3.0f, dp, a5), a.w);
+                ax[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vx, qx), b.x), fmaf(dx, w, ax[t]));
+                ay[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vy, qy), b.y), fmaf(dy, w, ay[t]));
+                az[t] = fmaf(-ri3, fmaf(u, fmaf(u75, vz, qz), b.z), fmaf(dz, w, az[t]));
+                ap[t] = fmaf(ri, fmaf(u, dp + u * fmaf(2.5f, t3, 0.5f * xqx), a.w), ap[t]);
+            }
         }
     }
+}
+
+// The M2P list: nh cells to the fourth moment from the bottom of the list,
+// nl cells to the third from the top ([LCAP_M - nl, LCAP_M)); one
+// compensated add per list.
+template <int TPL>
+__device__ inline void flush_m2p(const WaveLDS &L, int nh, int nl, const float (&rx)[TPL], const float (&ry)[TPL],
+                                 const float (&rz)[TPL], Acc<TPL> &A)
+{
+    float ax[TPL], ay[TPL], az[TPL], ap[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; t++) ax[t] = ay[t] = az[t] = ap[t] = 0.0f;
+    m2p_loop<TPL, true>(L, 0, nh, rx, ry, rz, ax, ay, az, ap);
+    m2p_loop<TPL, false>(L, LCAP_M - nl, LCAP_M, rx, ry, rz, ax, ay, az, ap);
 #pragma unroll
     for (int t = 0; t < TPL; t++) {
         nadd(A.s[t][0], A.c[t][0], ax[t]);
@@ -641,7 +687,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
     const int G = 64 * TPL;
     // per target species: cells tested, M2P and like / unlike P2P interactions
     unsigned long long c_visit[2] = {0, 0}, c_m2p[2] = {0, 0}, c_like[2] = {0, 0}, c_unlike[2] = {0, 0};
-    unsigned long long c_m2l[2] = {0, 0}, c_far[2] = {0, 0};
+    unsigned long long c_m2l[2] = {0, 0}, c_far[2] = {0, 0}, c_m2p3[2] = {0, 0};
 
     // Work queue, XCD-aware: workgroups go to the 8 XCDs round-robin, so
     // block b's waves first take groups from chunk b % 8 of the group range
@@ -749,7 +795,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         F.head = -1;
 
         // stack, M2P, P2P (like, unlike), band-queue, far-queue lengths
-        int sp = 1, nm = 0, npl = 0, npu = 0, nb = 0, nf = 0, nx = 0;
+        int sp = 1, nm = 0, nm3 = 0, npl = 0, npu = 0, nb = 0, nf = 0, nx = 0;
         // two-level: start from the segment's frontier (chunks of cells its
         // segment walk left to the groups) instead of the root
         int fc = -1;
@@ -763,10 +809,13 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         // the lists are evaluated once they reach their flush lengths (and
         // the far queue handed on once 64 wait), after every pass
         auto flushes = [&]() {
-            if (nm >= W.flush_m) {
-                if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
-                flush_m2p<TPL>(L, nm, rx, ry, rz, A);
-                nm = 0;
+            if (nm + nm3 >= W.flush_m) {
+                if (COUNT) {
+                    c_m2p[tsp] += (unsigned long long)(nm + nm3) * nact;
+                    c_m2p3[tsp] += (unsigned long long)nm3 * nact;
+                }
+                flush_m2p<TPL>(L, nm, nm3, rx, ry, rz, A);
+                nm = nm3 = 0;
                 wave_sync();
             }
             if (npl + npu >= W.flush_p) {
@@ -791,7 +840,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         // to each list before the flushes.
         for (;;) {
             while (sp > 0 || fc >= 0) {
-                bool have;
+                bool have, full4 = false;
                 uint32_t node;
                 if (sp > 0) {
                     // a batch of <= 64 cells from the top ranges of the stack
@@ -806,15 +855,21 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     const int K = __popcll(__ballot(take));
                     const int total = K > 0 ? __shfl(incl, K - 1, 64) : 0;
                     if (take) {
-                        const uint32_t st = R >> 3;
+                        // learned: FULL4 on a range marks the children of a band
+                        // cell the network opened (carried to its batch entries)
+                        const uint32_t st = (R & ~FULL4) >> 3, fb = MODE == MODE_LEARNED ? R & FULL4 : 0u;
                         const int ex = incl - len;
-                        for (int o = 0; o < len; o++) L.batch[ex + o] = st + (uint32_t)o;
+                        for (int o = 0; o < len; o++) L.batch[ex + o] = (st + (uint32_t)o) | fb;
                     }
                     sp -= K;
                     wave_sync();
                     have = lane < total;
                     node = have ? L.batch[lane] : 0u;
                     wave_sync();
+                    if (MODE == MODE_LEARNED) {
+                        full4 = (node & FULL4) != 0u;
+                        node &= ~FULL4;
+                    }
                 } else {
                     // the next frontier chunk: up to 64 cells
                     const uint32_t v = W.far_cells[(size_t)fc * 64 + lane];
@@ -822,7 +877,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     node = have ? v : 0u;
                     fc = uni(W.far_link[fc]);
                 }
-                bool leaf = false, acc = false, open = false, band = false;
+                bool leaf = false, acc = false, open = false, band = false, oct = false;
                 float cx = 0.f, cy = 0.f, cz = 0.f, d2 = 0.f;
                 uint32_t info = 0;
                 if (have) {
@@ -837,6 +892,12 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                         d2 = box_d2(cx, cy, cz, hx, hy, hz);
                         const float rc = (float)m.w;  // s / theta_mixed
                         const bool far_hi = d2 > rc * rc * 1.000004f;
+                        // accepted beyond OCT_RADIUS opening radii: M2P to the third
+                        // moment (the learned walk: radii of its band's outer edge
+                        // s/theta_lo, so that it is never less accurate than the
+                        // fixed walk at any theta in its band)
+                        const float roct = (MODE == MODE_FIXED ? rc : rc * W.lo_scale) * OCT_RADIUS;
+                        oct = !full4 && d2 > roct * roct;
                         const float cnt = D.bq2[node].z;
                         // a cell of at most SMALL_OPEN particles is opened: its
                         // particles' P2P is exact and cheaper than its M2P
@@ -894,19 +955,26 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     c_far[tsp] += (unsigned long long)__popcll(fk);
                 }
                 acc = acc && !far;
-                uint64_t mk = __ballot(acc);
+                // to the fourth moment from the bottom of the M2P list, to the
+                // third (no m5, m6) from the top
+                const bool acc4 = acc && !oct, acc3 = acc && oct;
+                uint64_t mk = __ballot(acc4);
+                const uint64_t mk3 = __ballot(acc3);
                 if (acc) {
-                    const int k = nm + __popcll(mk & lanemask_lt());
+                    const int k = acc4 ? nm + __popcll(mk & lanemask_lt()) : LCAP_M - 1 - nm3 - __popcll(mk3 & lanemask_lt());
                     const float4 q0 = D.bq0[node], q1 = D.bq1[node], q2 = D.bq2[node];
                     L.m0[k] = make_float4(cx, cy, cz, q0.x);
                     L.m1[k] = make_float4(q0.y, q0.z, q0.w, q1.x);
                     L.m2[k] = make_float4(q1.y, q1.z, q1.w, q2.x);
                     L.m3[k] = D.bq3[node];
                     L.m4[k] = D.bq4[node];
-                    L.m5[k] = D.bq5[node];
-                    L.m6[k] = D.bq6[node];
+                    if (acc4) {
+                        L.m5[k] = D.bq5[node];
+                        L.m6[k] = D.bq6[node];
+                    }
                 }
                 nm += __popcll(mk);
+                nm3 += __popcll(mk3);
                 // leaves -> P2P list: like charges from the bottom, unlike from the top
                 const float qleaf = leaf ? D.bq0[node].x : 0.0f;
                 const bool lk = leaf && qleaf == qt;
@@ -937,7 +1005,7 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                     atomicOr(&W.ctr[1], 1);
                     sp = 0;
                     fc = -1;
-                    nm = npl = npu = nb = 0;
+                    nm = nm3 = npl = npu = nb = 0;
                     break;
                 }
                 if (open) L.stack[sp + __popcll(mk & lanemask_lt())] = info;
@@ -1039,10 +1107,10 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
                 if (sp + bpush > STACK) {
                     atomicOr(&W.ctr[1], 1);
                     sp = 0;
-                    nm = npl = npu = nb = 0;
+                    nm = nm3 = npl = npu = nb = 0;
                     break;
                 }
-                if (bopen) L.stack[sp + __popcll(bk & lanemask_lt())] = binf;
+                if (bopen) L.stack[sp + __popcll(bk & lanemask_lt())] = binf | FULL4;
                 sp += bpush;
                 // the rest of the queue (at most 63 entries) moves down
                 const uint32_t rest = lane + 64 < nb ? L.bandq[lane + 64] : 0u;
@@ -1053,9 +1121,12 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
             }
             flushes();
         }
-        if (nm > 0) {
-            if (COUNT) c_m2p[tsp] += (unsigned long long)nm * nact;
-            flush_m2p<TPL>(L, nm, rx, ry, rz, A);
+        if (nm + nm3 > 0) {
+            if (COUNT) {
+                c_m2p[tsp] += (unsigned long long)(nm + nm3) * nact;
+                c_m2p3[tsp] += (unsigned long long)nm3 * nact;
+            }
+            flush_m2p<TPL>(L, nm, nm3, rx, ry, rz, A);
         }
         if (npl + npu > 0) {
             flush_p2p<TPL>(L, npl, npu, rx, ry, rz, qt, D.inv_rs, D.close2, A, nx);
@@ -1099,6 +1170,8 @@ __global__ __launch_bounds__(BT, 3) void k_walk32(WalkArgs W)
         atomicAdd(&W.cnt[13], c_m2l[1]);
         atomicAdd(&W.cnt[14], c_far[0]);
         atomicAdd(&W.cnt[15], c_far[1]);
+        atomicAdd(&W.cnt[18], c_m2p3[0]);
+        atomicAdd(&W.cnt[19], c_m2p3[1]);
     }
 }
 
@@ -1961,6 +2034,11 @@ int tb_mixed_prep(hipStream_t s, MixedDev &M, const JointDev *jt, size_t *bytes)
     }
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, M.ctr, 2, 3);
     const JointDev &J = *jt;
+    if (J.nn >= (1 << 28)) {
+        // a cell's record holds its first child's index in bits 3..30
+        g_err = "mixed walk: joint tree of 2^28 nodes or more";
+        return TB_ERR_NOMEM;
+    }
     MixedTree &T = M.t[0];
     int cap = T.cap, rc;
     int c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap, c8 = cap, c9 = cap;

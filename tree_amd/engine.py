@@ -79,6 +79,7 @@ class Stats(ctypes.Structure):
         ("m2l", ctypes.c_int64 * 4),
         ("far_pairs", ctypes.c_int64 * 2),
         ("seg_pairs", ctypes.c_int64 * 2),
+        ("m2p3", ctypes.c_int64 * 2),
     ]
 
 
@@ -472,7 +473,7 @@ def _stats_dict(s):
         "nodes": list(s.nodes), "max_depth": list(s.max_depth),
         "visits": list(s.visits), "m2p": list(s.m2p), "p2p": list(s.p2p),
         "ms_build": s.ms_build, "ms_force": s.ms_force, "ms_vv": s.ms_vv, "ms_walk": s.ms_walk,
-        "m2l": list(s.m2l), "far_pairs": list(s.far_pairs), "seg_pairs": list(s.seg_pairs),
+        "m2l": list(s.m2l), "far_pairs": list(s.far_pairs), "seg_pairs": list(s.seg_pairs), "m2p3": list(s.m2p3),
     }
 
 
