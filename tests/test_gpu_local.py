@@ -149,6 +149,47 @@ def test_far_buffer_regrows(monkeypatch):
     assert out[0][1] == out[1][1]
 
 
+def test_far_buffer_regrows_learned(monkeypatch):
+    """The learned walk's re-run after a far-buffer regrow decides its band
+    cells as the first attempt would: the first attempt has overwritten the
+    forces its E_min feature reads, so the re-run takes E_min saved before
+    it.  A first force at theta 0.6 sizes the buffer; the second, at 0.35,
+    needs more far chunks (smaller accepted cells) and is re-run, with the
+    first force's field as its previous force.  Forces and energies are
+    bit-identical to the same two forces with a buffer large from the start."""
+    announce("far_buffer_regrow learned")
+    n = 1 << 14
+    qel, qion = ta.gen_plasma(34, n, n, 200.0)
+    with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.37, theta_lo=0.33, walk="mixed") as eng:
+        eng.set_particles(qel, qion)
+        eng.build()
+        X, y = eng.mlp_collect(cap=1 << 18)
+        assert len(X) > 1000
+        eng.mlp_init(0)
+        eng.mlp_train(X, y, steps=300, batch=4096, lr=2e-3)
+        params = eng.mlp_get()
+    out = []
+    for chunks in (None, "1"):
+        if chunks:
+            monkeypatch.setenv("TREEMI_FAR_CHUNKS", chunks)
+        else:
+            monkeypatch.delenv("TREEMI_FAR_CHUNKS", raising=False)
+        with ta.Engine(rs=RS10, theta=0.5, theta_mixed=0.6, walk="mixed") as eng:
+            eng.set_particles(qel, qion)
+            eng.mlp_set(params)
+            eng.build()
+            eng.force()
+            eng.configure(theta_mixed=0.37, theta_lo=0.33, mac="learned", err_target=1e-5)
+            eng.build()
+            ep = eng.force()
+            e, i = eng.get_particles()
+            st = eng.count_interactions()
+            out.append((np.r_[e["ff"], i["ff"]], ep, st))
+    assert sum(out[0][2]["m2l"]) > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 @pytest.mark.parametrize("mac", ["fixed", "learned"])
 def test_two_level_far_field(mac):
     """far_levels 2 (segments of up to 8 groups walk the top of the tree once;

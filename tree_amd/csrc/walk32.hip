@@ -1912,13 +1912,16 @@ int launch_domains(hipStream_t s, MixedDev &M, const std::vector<Domain> &doms, 
     W.scen = M.scen;
     // every wave of the segment walk and of the group walk may hold a part-used slab
     const long slack = (long)(blocks + (two ? nblk(std::min(W.nsegt, resident_waves()), WPB) : 0)) * WPB * FAR_SLAB;
+    // TREEMI_FAR_CHUNKS caps the buffer the first attempt of every walk may
+    // use (tests use it to take the re-run path on any force)
+    const char *env = getenv("TREEMI_FAR_CHUNKS");
+    const long env_cap = env ? atol(env) : 0;
     if (far_on && M.far_cap == 0) {
-        // TREEMI_FAR_CHUNKS overrides the first guess (tests use it to take the re-run path)
-        const char *env = getenv("TREEMI_FAR_CHUNKS");
-        const long first = env && atol(env) > 0 ? atol(env) : (long)ngt * 512 + slack;
+        const long first = env_cap > 0 ? env_cap : (long)ngt * 512 + slack;
         const int rc = grow_far(M, first);
         if (rc) return rc;
     }
+    if (far_on && env_cap > 0 && M.far_cap > env_cap) M.far_cap = (int)env_cap;  // the allocation stays larger
     for (int attempt = 0;; attempt++) {
         W.far_cells = M.far_cells;
         W.far_link = M.far_link;
