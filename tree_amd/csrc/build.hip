@@ -429,23 +429,26 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
     lm = min(lm, mx);
     int leafd = 1 + mx;
     int base = offs[i] + 1;
-    for (int d = lm + 1; d <= leafd; d++) {
+    // deepest first: a cell holds the one a level deeper that starts at the
+    // same particle, so its end is at or past that one's and the search for
+    // it starts there (one gallop over all the levels, not one per level)
+    int end = i + 1;
+    for (int d = leafd; d >= lm + 1; d--) {
         int id = base + (d - lm - 1);
         ndepth[id] = d;
         nstart[id] = i;
         dkey[id] = (uint64_t)d;
-        int end;
-        if (d == leafd) {
-            end = i + 1;
-        } else {
-            // first particle past the cell: gallop then bisect on the key prefix
+        if (d < leafd) {
+            // first particle past the cell: gallop from the deeper cell's last
+            // particle, then bisect on the key prefix
             int sh = 3 * (TB_LEVELS - d);
             uint64_t p = k >> sh;
-            int lo = i, hi = i + 1, step = 1;
+            const int b = end - 1;
+            int lo = b, hi = end, step = 1;
             while (hi < n && (key[hi] >> sh) == p) {
                 lo = hi;
                 step <<= 1;
-                hi = i + step;
+                hi = b + step;
             }
             if (hi > n) hi = n;
             while (hi - lo > 1) {
@@ -752,22 +755,22 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
     lm = min(lm, mx);
     int leafd = 1 + mx;
     int base = offs[i] + 1;
-    for (int d = lm + 1; d <= leafd; d++) {
+    // deepest first, each search from the deeper cell's end (as k_emit)
+    int end = i + 1;
+    for (int d = leafd; d >= lm + 1; d--) {
         int id = base + (d - lm - 1);
         ndepth[id] = d;
         nstart[id] = i;
         dkey[id] = (uint64_t)d;
-        int end;
-        if (d == leafd) {
-            end = i + 1;
-        } else {
+        if (d < leafd) {
             int sh = 3 * (TB_LEVELS - d);
             uint64_t p = k >> sh;
-            int lo = i, hi = i + 1, step = 1;
+            const int b = end - 1;
+            int lo = b, hi = end, step = 1;
             while (hi < n && (key[hi] >> sh) == p) {
                 lo = hi;
                 step <<= 1;
-                hi = i + step;
+                hi = b + step;
             }
             if (hi > n) hi = n;
             while (hi - lo > 1) {
