@@ -85,3 +85,20 @@ def test_no_gpu_fails_loudly():
         assert e.code in (7, 2)
     else:
         raise AssertionError("engine started without a GPU")
+
+
+def test_config_is_checked_before_any_device():
+    """tb_create rejects a bad configuration with TB_ERR_ARG and a message
+    before it looks for a device (the same on a GPU box as without one):
+    theta, group, far_ratio, far_levels and the learned criterion's cost
+    weight outside their ranges."""
+    bad = [dict(theta=0.0), dict(group=100), dict(far_ratio=1.5), dict(far_levels=3),
+           dict(cost_weight=-0.1), dict(cost_weight=5.0), dict(cost_weight=float("nan"))]
+    for kw in bad:
+        try:
+            ta.Engine(**{"rs": 6.6e-3, "walk": "mixed", **kw})
+        except ta.TreeError as e:
+            assert e.code == 1, (kw, e)
+            assert str(e), kw
+        else:
+            raise AssertionError("accepted %r" % kw)
