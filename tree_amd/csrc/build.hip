@@ -155,9 +155,9 @@ __global__ __launch_bounds__(BT) void k_rs_up(int n, const uint64_t *__restrict_
     for (int r = 0; r < RS_ITEMS; r++) {
         long i = base + r * 64 + lane;
         uint64_t k = i < n ? key[i] : ~0ull;  // pad: digit 255, sorts after all real keys
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
-        uint64_t m = match8(d);
-        if ((int)__builtin_ctzll(m) == lane) atomicAdd(&h[d], (uint32_t)__popcll(m));
+        // a plain LDS add per key: cheaper than grouping equal digits with
+        // ballots first (k_rs_down needs those for its ranks, the counts do not)
+        atomicAdd(&h[(uint32_t)(k >> shift) & 255u], 1u);
     }
     __syncthreads();
     hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
