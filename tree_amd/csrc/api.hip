@@ -176,7 +176,7 @@ void free_species(tb_handle *h, SpeciesDev &S)
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
     dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
     dfree(h, S.bkey, c * 8);
-    for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
+    for (int b = 0; b < 2; b++) dfree(h, S.dval[b], c * 4);
     S = SpeciesDev();
 }
 
@@ -214,7 +214,7 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     dfree(h, S.ndx, c * 8); dfree(h, S.nstart, c * 4); dfree(h, S.ncount, c * 4); dfree(h, S.ndepth, c * 4);
     dfree(h, S.binv, c * 4); dfree(h, S.bflag, c); dfree(h, S.bkids, c * 4); dfree(h, S.bxq, c * 32);
     dfree(h, S.bkey, c * 8);
-    for (int b = 0; b < 2; b++) { dfree(h, S.dkey[b], c * 8); dfree(h, S.dval[b], c * 4); }
+    for (int b = 0; b < 2; b++) dfree(h, S.dval[b], c * 4);
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
     S.cap = 0;
     int rc;
@@ -231,7 +231,6 @@ int ensure_nodes(tb_handle *h, SpeciesDev &S, int nn)
     if ((rc = dalloc_t(h, &S.bkey, cap))) return rc;
     if ((rc = dalloc_t(h, &S.bxq, cap))) return rc;
     for (int b = 0; b < 2; b++) {
-        if ((rc = dalloc_t(h, &S.dkey[b], cap))) return rc;
         if ((rc = dalloc_t(h, &S.dval[b], cap))) return rc;
     }
     S.cap = cap;
@@ -244,7 +243,6 @@ void free_joint(tb_handle *h, JointDev &J)
     for (int b = 0; b < 2; b++) {
         dfree(h, J.key[b], n * 8);
         dfree(h, J.val[b], n * 4);
-        dfree(h, J.dkey[b], c * 8);
         dfree(h, J.dval[b], c * 4);
     }
     dfree(h, J.pos, n * 32);
@@ -279,7 +277,7 @@ int ensure_joint_nodes(tb_handle *h, int nn)
     JointDev &J = h->jt;
     if (nn <= J.cap) return TB_OK;
     size_t c = (size_t)J.cap;
-    for (int b = 0; b < 2; b++) { dfree(h, J.dkey[b], c * 8); dfree(h, J.dval[b], c * 4); }
+    for (int b = 0; b < 2; b++) dfree(h, J.dval[b], c * 4);
     dfree(h, J.nc, c * 32); dfree(h, J.nnext, c * 4);
     dfree(h, J.nstart, c * 4); dfree(h, J.ncount, c * 4); dfree(h, J.ndepth, c * 4);
     int cap = (int)std::min<long>((long)nn + nn / 8 + 64, (long)INT32_MAX / 2);
@@ -291,7 +289,6 @@ int ensure_joint_nodes(tb_handle *h, int nn)
     if ((rc = dalloc_t(h, &J.ncount, cap))) return rc;
     if ((rc = dalloc_t(h, &J.ndepth, cap))) return rc;
     for (int b = 0; b < 2; b++) {
-        if ((rc = dalloc_t(h, &J.dkey[b], cap))) return rc;
         if ((rc = dalloc_t(h, &J.dval[b], cap))) return rc;
     }
     J.cap = cap;
@@ -502,9 +499,8 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         }
         const uint64_t *key = J.key[which[2]];
         tb_jemit(s, J, key);
-        int w2;
-        tb_radix_sort(s, J.nn, J.dkey, J.dval, 0, 8, h->sort_tmp, &w2, true);
-        J.bfs = J.dval[w2];
+        tb_depth_order(s, J.nn, J.ndepth, J.dval[0], h->sort_tmp);
+        J.bfs = J.dval[0];
         if (eb && J.lvl_off[eb->T + 1] - J.lvl_off[eb->T] != eb->nreal)
             // realisation r's root cell is the r-th cell at depth T (the tag digits)
             return fail(TB_ERR_HIP, "ensemble tree: %d cells at depth %d for %d realisations",
@@ -555,14 +551,10 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         h->st.nodes[sp] = S.nn;
         h->st.max_depth[sp] = S.maxdepth;
     }
-    {
-        const int n2[2] = {h->sp[0].nn, h->sp[1].nn};
-        uint64_t **k2[2] = {h->sp[0].dkey, h->sp[1].dkey};
-        uint32_t **v2[2] = {h->sp[0].dval, h->sp[1].dval};
-        void *const t2[2] = {h->sort_tmp, h->sort_tmp1};
-        int w2[2];
-        tb_radix_sort2(st2, n2, k2, v2, 0, 8, t2, w2, true);
-        for (int sp = 0; sp < 2; sp++) h->sp[sp].bfs = h->sp[sp].dval[w2[sp]];
+    for (int sp = 0; sp < 2; sp++) {
+        SpeciesDev &S = h->sp[sp];
+        tb_depth_order(st2[sp], S.nn, S.ndepth, S.dval[0], sp ? h->sort_tmp1 : h->sort_tmp);
+        S.bfs = S.dval[0];
     }
     for (int sp = 0; sp < 2; sp++)
         tb_upward_prep(st2[sp], h->sp[sp], h->sp[sp].key[which[sp]], sp == 0 ? -1 : 1, h->d_misc + sp);

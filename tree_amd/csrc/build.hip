@@ -144,7 +144,8 @@ __device__ inline uint64_t match8(uint32_t d)
     return m;
 }
 
-__global__ __launch_bounds__(BT) void k_rs_up(int n, const uint64_t *__restrict__ key, int shift,
+template<typename K>
+__global__ __launch_bounds__(BT) void k_rs_up(int n, const K *__restrict__ key, int shift,
                                               uint32_t *__restrict__ hist, int nb)
 {
     __shared__ uint32_t h[256];
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(BT) void k_rs_up(int n, const uint64_t *__restrict_
     const long base = (long)blockIdx.x * RS_TILE + wave * RS_WSEG;
     for (int r = 0; r < RS_ITEMS; r++) {
         long i = base + r * 64 + lane;
-        uint64_t k = i < n ? key[i] : ~0ull;  // pad: digit 255, sorts after all real keys
+        K k = i < n ? key[i] : (K)~(K)0;  // pad: digit 255, sorts after all real keys
         // a plain LDS add per key: cheaper than grouping equal digits with
         // ballots first (k_rs_down needs those for its ranks, the counts do not)
         atomicAdd(&h[(uint32_t)(k >> shift) & 255u], 1u);
@@ -198,13 +199,15 @@ __global__ __launch_bounds__(BT) void k_rs_scan(int nb, uint32_t *__restrict__ h
     if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(BT) void k_rs_down(int n, const uint64_t *__restrict__ kin,
-                                                const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+// kout may be null: the permutation alone (vout) is wanted
+template<typename K>
+__global__ __launch_bounds__(BT) void k_rs_down(int n, const K *__restrict__ kin,
+                                                const uint32_t *__restrict__ vin, K *__restrict__ kout,
                                                 uint32_t *__restrict__ vout, int shift,
                                                 const uint32_t *__restrict__ hist, const uint32_t *__restrict__ totals,
                                                 int nb, int iota)
 {
-    __shared__ uint64_t skey[RS_TILE];
+    __shared__ K skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
     __shared__ uint32_t wcnt[RS_WAVES][256];
     __shared__ uint32_t bstart[256];
@@ -216,13 +219,13 @@ __global__ __launch_bounds__(BT) void k_rs_down(int n, const uint64_t *__restric
 
     const long tile = (long)blockIdx.x * RS_TILE;
     const long base = tile + wave * RS_WSEG;
-    uint64_t k[RS_ITEMS];
+    K k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
     uint32_t pos[RS_ITEMS];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
         long i = base + r * 64 + lane;
-        k[r] = i < n ? kin[i] : ~0ull;
+        k[r] = i < n ? kin[i] : (K)~(K)0;
         v[r] = i < n ? (iota ? (uint32_t)i : vin[i]) : 0u;
     }
 #pragma unroll
@@ -261,11 +264,11 @@ __global__ __launch_bounds__(BT) void k_rs_down(int n, const uint64_t *__restric
     }
     __syncthreads();
     for (int j = threadIdx.x; j < RS_TILE; j += BT) {
-        uint64_t kk = skey[j];
+        K kk = skey[j];
         uint32_t d = (uint32_t)(kk >> shift) & 255u;
         long g = (long)gstart[d] + (j - (long)bstart[d]);
         if (g < n) {
-            kout[g] = kk;
+            if (kout) kout[g] = kk;
             vout[g] = sval[j];
         }
     }
@@ -415,12 +418,12 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
                                              const double *__restrict__ pz, double sign, double4 *__restrict__ nmac,
                                              uint32_t *__restrict__ nnext, int32_t *__restrict__ nstart,
                                              int32_t *__restrict__ ncount,
-                                             int32_t *__restrict__ ndepth, uint64_t *__restrict__ dkey)
+                                             int32_t *__restrict__ ndepth)
 {
     int i = blockIdx.x * BT + threadIdx.x;
     if (i >= n) return;
     if (i == 0) {
-        ndepth[0] = 0; nstart[0] = 0; ncount[0] = n; nnext[0] = (uint32_t)total; dkey[0] = 0;
+        ndepth[0] = 0; nstart[0] = 0; ncount[0] = n; nnext[0] = (uint32_t)total;
     }
     uint64_t k = key[i];
     int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
@@ -437,7 +440,6 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
         int id = base + (d - lm - 1);
         ndepth[id] = d;
         nstart[id] = i;
-        dkey[id] = (uint64_t)d;
         if (d < leafd) {
             // first particle past the cell: gallop from the deeper cell's last
             // particle, then bisect on the key prefix
@@ -739,13 +741,12 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
                                               int total, const double4 *__restrict__ pos,
                                               double4 *__restrict__ nc, uint32_t *__restrict__ nnext,
                                               int32_t *__restrict__ nstart,
-                                              int32_t *__restrict__ ncount, int32_t *__restrict__ ndepth,
-                                              uint64_t *__restrict__ dkey)
+                                              int32_t *__restrict__ ncount, int32_t *__restrict__ ndepth)
 {
     int i = blockIdx.x * BT + threadIdx.x;
     if (i >= n) return;
     if (i == 0) {
-        ndepth[0] = 0; nstart[0] = 0; ncount[0] = n; nnext[0] = (uint32_t)total; dkey[0] = 0;
+        ndepth[0] = 0; nstart[0] = 0; ncount[0] = n; nnext[0] = (uint32_t)total;
     }
     uint64_t k = key[i];
     int lm = i > 0 ? lcp_digits(key[i - 1], k) : 0;
@@ -761,7 +762,6 @@ __global__ __launch_bounds__(BT) void k_jemit(int n, const uint64_t *__restrict_
         int id = base + (d - lm - 1);
         ndepth[id] = d;
         nstart[id] = i;
-        dkey[id] = (uint64_t)d;
         if (d < leafd) {
             int sh = 3 * (TB_LEVELS - d);
             uint64_t p = k >> sh;
@@ -1335,9 +1335,9 @@ void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int
     int src = 0;
     bool iota = init_iota;
     for (int sh = lo_bit; sh < hi_bit; sh += 8) {
-        hipLaunchKernelGGL(k_rs_up, dim3(nb), dim3(BT), 0, s, n, key[src], sh, hist, nb);
+        hipLaunchKernelGGL(k_rs_up<uint64_t>, dim3(nb), dim3(BT), 0, s, n, key[src], sh, hist, nb);
         hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s, nb, hist, totals);
-        hipLaunchKernelGGL(k_rs_down, dim3(nb), dim3(BT), 0, s, n, key[src], val[src], key[1 - src], val[1 - src],
+        hipLaunchKernelGGL(k_rs_down<uint64_t>, dim3(nb), dim3(BT), 0, s, n, key[src], val[src], key[1 - src], val[1 - src],
                            sh, hist, totals, nb, iota ? 1 : 0);
         iota = false;
         src = 1 - src;
@@ -1357,15 +1357,28 @@ void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], u
             uint32_t *totals = hist + (size_t)256 * nb[k];
             uint64_t **kk = key[k];
             uint32_t **vv = val[k];
-            hipLaunchKernelGGL(k_rs_up, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], sh, hist, nb[k]);
+            hipLaunchKernelGGL(k_rs_up<uint64_t>, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], sh, hist, nb[k]);
             hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s[k], nb[k], hist, totals);
-            hipLaunchKernelGGL(k_rs_down, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], vv[src[k]],
+            hipLaunchKernelGGL(k_rs_down<uint64_t>, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], vv[src[k]],
                                kk[1 - src[k]], vv[1 - src[k]], sh, hist, totals, nb[k], iota ? 1 : 0);
             src[k] = 1 - src[k];
         }
         iota = false;
     }
     for (int k = 0; k < 2; k++) which[k] = src[k];
+}
+
+void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order, void *tmp)
+{
+    // one stable 8-bit pass over the nodes' depths (< 2^8), node ids as values
+    const int nb = nblk(n, RS_TILE);
+    uint32_t *hist = (uint32_t *)tmp;
+    uint32_t *totals = hist + (size_t)256 * nb;
+    const uint32_t *key = (const uint32_t *)depth;
+    hipLaunchKernelGGL(k_rs_up<uint32_t>, dim3(nb), dim3(BT), 0, s, n, key, 0, hist, nb);
+    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s, nb, hist, totals);
+    hipLaunchKernelGGL(k_rs_down<uint32_t>, dim3(nb), dim3(BT), 0, s, n, key, (const uint32_t *)nullptr,
+                       (uint32_t *)nullptr, order, 0, hist, totals, nb, 1);
 }
 
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err)
@@ -1400,8 +1413,7 @@ void tb_emit(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign)
 {
     const int c = S.cur;
     hipLaunchKernelGGL(k_emit, dim3(nblk(S.n, BT)), dim3(BT), 0, s, S.n, key, S.offs, S.nn, S.x[c][0], S.x[c][1],
-                       S.x[c][2], (double)sign, S.nmac, S.nnext, S.nstart, S.ncount, S.ndepth,
-                       S.dkey[0]);
+                       S.x[c][2], (double)sign, S.nmac, S.nnext, S.nstart, S.ncount, S.ndepth);
 }
 
 void tb_upward_prep(hipStream_t s, SpeciesDev &S, const uint64_t *key, int sign, int32_t *err)
@@ -1501,7 +1513,7 @@ void tb_ekeys(hipStream_t s, int n, const double *x, const double *y, const doub
 void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)
 {
     hipLaunchKernelGGL(k_jemit, dim3(nblk(J.n, BT)), dim3(BT), 0, s, J.n, key, J.offs, J.nn, J.pos, J.nc, J.nnext,
-                       J.nstart, J.ncount, J.ndepth, J.dkey[0]);
+                       J.nstart, J.ncount, J.ndepth);
 }
 
 void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, const CellRecs &R, int T, const double *roots)

@@ -34,8 +34,7 @@ struct SpeciesDev {
     double *nmom = nullptr;     // [nn][10] qeff, D(3), Q(3), QQ(3)  (child-point, tree.f90:537-551)
     double *ndx = nullptr;      // centroid offset from the geometric centre
     int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
-    uint64_t *dkey[2] = {};     // depth-sort keys / node ids -> breadth-first order
-    uint32_t *dval[2] = {};
+    uint32_t *dval[2] = {};     // node ids in breadth-first order (dval[0])
     uint32_t *bfs = nullptr;    // preorder ids grouped by depth, preorder within a depth
     int lvl_off[TB_LEVELS + 3] = {};
     int twig_hist[TB_LEVELS + 2] = {};
@@ -80,7 +79,6 @@ struct JointDev {
     double4 *nc = nullptr;      // leaves: the particle's position and charge (cells: unused)
     uint32_t *nnext = nullptr;
     int32_t *nstart = nullptr, *ncount = nullptr, *ndepth = nullptr;
-    uint64_t *dkey[2] = {};
     uint32_t *dval[2] = {};
     uint32_t *bfs = nullptr;
     int lvl_off[TB_LEVELS + 3] = {};
@@ -105,6 +103,8 @@ void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int
 // passes enqueued alternately on two streams so both run from the start
 void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], uint32_t **val[2], int lo_bit,
                     int hi_bit, void *const tmp[2], int which[2], bool init_iota);
+// breadth-first order: node ids stably sorted by depth (one pass over ndepth)
+void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order, void *tmp);
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err);
 // joint: the mixed walk's joint tree (up to 8 particles may share a full key)
 void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist /*2*(TB_LEVELS+2)*/,
