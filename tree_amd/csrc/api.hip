@@ -499,8 +499,6 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         }
         const uint64_t *key = J.key[which[2]];
         tb_jemit(s, J, key);
-        tb_depth_order(s, J.nn, J.ndepth, J.dval[0], h->sort_tmp);
-        J.bfs = J.dval[0];
         if (eb && J.lvl_off[eb->T + 1] - J.lvl_off[eb->T] != eb->nreal)
             // realisation r's root cell is the r-th cell at depth T (the tag digits)
             return fail(TB_ERR_HIP, "ensemble tree: %d cells at depth %d for %d realisations",
@@ -508,6 +506,8 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
         // the walk's breadth-first records, filled by the upward pass
         if ((rc = tb_mixed_prep(h->s, h->mx, &h->jt, &h->bytes))) return fail(rc, "%s", tb_mixed_error());
         const CellRecs R = tb_mixed_recs(h->mx, h->cfg);
+        tb_depth_order(s, J.nn, J.ndepth, J.dval[0], R.inv, h->sort_tmp);
+        J.bfs = J.dval[0];
         if (eb) tb_jupward(s, J, key, R, eb->T, eb->d_roots + (size_t)2 * eb->nreal * 6);
         else tb_jupward(s, J, key, R);
         h->st.nodes[0] = J.nn;
@@ -553,7 +553,7 @@ int build_phase_b(tb_handle *h, const int *which, const EnsBuild *eb = nullptr)
     }
     for (int sp = 0; sp < 2; sp++) {
         SpeciesDev &S = h->sp[sp];
-        tb_depth_order(st2[sp], S.nn, S.ndepth, S.dval[0], sp ? h->sort_tmp1 : h->sort_tmp);
+        tb_depth_order(st2[sp], S.nn, S.ndepth, S.dval[0], nullptr, sp ? h->sort_tmp1 : h->sort_tmp);
         S.bfs = S.dval[0];
     }
     for (int sp = 0; sp < 2; sp++)

@@ -199,13 +199,14 @@ __global__ __launch_bounds__(BT) void k_rs_scan(int nb, uint32_t *__restrict__ h
     if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
-// kout may be null: the permutation alone (vout) is wanted
+// kout may be null: the permutation alone (vout) is wanted; vinv, if given,
+// receives its inverse (vinv[vout[g]] = g)
 template<typename K>
 __global__ __launch_bounds__(BT) void k_rs_down(int n, const K *__restrict__ kin,
                                                 const uint32_t *__restrict__ vin, K *__restrict__ kout,
                                                 uint32_t *__restrict__ vout, int shift,
                                                 const uint32_t *__restrict__ hist, const uint32_t *__restrict__ totals,
-                                                int nb, int iota)
+                                                int nb, int iota, uint32_t *__restrict__ vinv)
 {
     __shared__ K skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
@@ -270,6 +271,7 @@ __global__ __launch_bounds__(BT) void k_rs_down(int n, const K *__restrict__ kin
         if (g < n) {
             if (kout) kout[g] = kk;
             vout[g] = sval[j];
+            if (vinv) vinv[sval[j]] = (uint32_t)g;
         }
     }
 }
@@ -476,7 +478,10 @@ __global__ __launch_bounds__(BT) void k_emit(int n, const uint64_t *__restrict__
 
 // ---------------------------------------------------------------- upward
 
-// The inverse of the breadth-first order (preorder node -> breadth-first index).
+// The inverse of the breadth-first order (preorder node -> breadth-first
+// index) for the species trees.  The joint tree's comes from its depth-order
+// scatter instead (faster there; here, beside the other species' stream, the
+// scatter inside the sort was slower: 1.61 against 1.59 ms per build).
 __global__ __launch_bounds__(BT) void k_bfs_inv(int nn, const uint32_t *__restrict__ bfs, uint32_t *__restrict__ binv)
 {
     const int j = blockIdx.x * BT + threadIdx.x;
@@ -1338,7 +1343,7 @@ void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int
         hipLaunchKernelGGL(k_rs_up<uint64_t>, dim3(nb), dim3(BT), 0, s, n, key[src], sh, hist, nb);
         hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s, nb, hist, totals);
         hipLaunchKernelGGL(k_rs_down<uint64_t>, dim3(nb), dim3(BT), 0, s, n, key[src], val[src], key[1 - src], val[1 - src],
-                           sh, hist, totals, nb, iota ? 1 : 0);
+                           sh, hist, totals, nb, iota ? 1 : 0, (uint32_t *)nullptr);
         iota = false;
         src = 1 - src;
     }
@@ -1360,7 +1365,8 @@ void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], u
             hipLaunchKernelGGL(k_rs_up<uint64_t>, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], sh, hist, nb[k]);
             hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s[k], nb[k], hist, totals);
             hipLaunchKernelGGL(k_rs_down<uint64_t>, dim3(nb[k]), dim3(BT), 0, s[k], n[k], kk[src[k]], vv[src[k]],
-                               kk[1 - src[k]], vv[1 - src[k]], sh, hist, totals, nb[k], iota ? 1 : 0);
+                               kk[1 - src[k]], vv[1 - src[k]], sh, hist, totals, nb[k], iota ? 1 : 0,
+                               (uint32_t *)nullptr);
             src[k] = 1 - src[k];
         }
         iota = false;
@@ -1368,9 +1374,10 @@ void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], u
     for (int k = 0; k < 2; k++) which[k] = src[k];
 }
 
-void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order, void *tmp)
+void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order, uint32_t *inv, void *tmp)
 {
-    // one stable 8-bit pass over the nodes' depths (< 2^8), node ids as values
+    // one stable 8-bit pass over the nodes' depths (< 2^8), node ids as
+    // values; the scatter also writes the inverse order
     const int nb = nblk(n, RS_TILE);
     uint32_t *hist = (uint32_t *)tmp;
     uint32_t *totals = hist + (size_t)256 * nb;
@@ -1378,7 +1385,7 @@ void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order,
     hipLaunchKernelGGL(k_rs_up<uint32_t>, dim3(nb), dim3(BT), 0, s, n, key, 0, hist, nb);
     hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(BT), 0, s, nb, hist, totals);
     hipLaunchKernelGGL(k_rs_down<uint32_t>, dim3(nb), dim3(BT), 0, s, n, key, (const uint32_t *)nullptr,
-                       (uint32_t *)nullptr, order, 0, hist, totals, nb, 1);
+                       (uint32_t *)nullptr, order, 0, hist, totals, nb, 1, inv);
 }
 
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err)
@@ -1518,7 +1525,6 @@ void tb_jemit(hipStream_t s, JointDev &J, const uint64_t *key)
 
 void tb_jupward(hipStream_t s, JointDev &J, const uint64_t *key, const CellRecs &R, int T, const double *roots)
 {
-    hipLaunchKernelGGL(k_bfs_inv, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, R.inv);
     hipLaunchKernelGGL(k_jprep, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.nnext, J.ndepth, J.nstart, J.ncount,
                        key, J.nc, R);
     hipLaunchKernelGGL(k_jkids, dim3(nblk(J.nn, BT)), dim3(BT), 0, s, J.nn, J.bfs, R);

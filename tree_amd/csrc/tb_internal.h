@@ -103,8 +103,9 @@ void tb_radix_sort(hipStream_t s, int n, uint64_t *key[2], uint32_t *val[2], int
 // passes enqueued alternately on two streams so both run from the start
 void tb_radix_sort2(const hipStream_t s[2], const int n[2], uint64_t **key[2], uint32_t **val[2], int lo_bit,
                     int hi_bit, void *const tmp[2], int which[2], bool init_iota);
-// breadth-first order: node ids stably sorted by depth (one pass over ndepth)
-void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order, void *tmp);
+// breadth-first order: node ids stably sorted by depth (one pass over ndepth),
+// and, if inv is given, its inverse
+void tb_depth_order(hipStream_t s, int n, const int32_t *depth, uint32_t *order, uint32_t *inv, void *tmp);
 void tb_permute(hipStream_t s, SpeciesDev &S, const uint32_t *perm, int32_t *err);
 // joint: the mixed walk's joint tree (up to 8 particles may share a full key)
 void tb_lcp(hipStream_t s, int n, const uint64_t *key, int32_t *cnt, int32_t *hist /*2*(TB_LEVELS+2)*/,
