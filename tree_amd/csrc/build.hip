@@ -153,13 +153,17 @@ __global__ __launch_bounds__(BT) void k_rs_up(int n, const K *__restrict__ key, 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long base = (long)blockIdx.x * RS_TILE + wave * RS_WSEG;
+    // the tile's keys loaded together (all in flight at once), then a plain
+    // LDS add per key: cheaper than grouping equal digits with ballots first
+    // (k_rs_down needs those for its ranks, the counts do not)
+    K k[RS_ITEMS];
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        long i = base + r * 64 + lane;
-        K k = i < n ? key[i] : (K)~(K)0;  // pad: digit 255, sorts after all real keys
-        // a plain LDS add per key: cheaper than grouping equal digits with
-        // ballots first (k_rs_down needs those for its ranks, the counts do not)
-        atomicAdd(&h[(uint32_t)(k >> shift) & 255u], 1u);
+        const long i = base + r * 64 + lane;
+        k[r] = i < n ? key[i] : (K)~(K)0;  // pad: digit 255, sorts after all real keys
     }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) atomicAdd(&h[(uint32_t)(k[r] >> shift) & 255u], 1u);
     __syncthreads();
     hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
